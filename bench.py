@@ -1,0 +1,191 @@
+"""FP64 RK4 point-updates/s of the fused generalized-harmonic step on one MI355X.
+
+Workload (the plain method at real shape): a static spinning Kerr-Schild hole,
+M = 1, a = 0.7, in the Dirichlet box [-8M, 8M]^3 at h = M/16 -- a uniform
+forest of 16^3 blocks of 16^3 cells (4096 blocks, 16.8 M points by the
+reference's N^3-per-block convention), q = 4, G = 3, the :damped interior
+(r_0 = 0.71, r_1 = 1.21, rho_max = 4/M, ramps 1/2), Gaussian constraint
+damping (near 1/M, far 1/(10M), width 3M), gamma2 = 0, eps_KO = 1/2, the
+closed-form Kerr-Schild gauge source, analytic initial data, cfl = 1/4.
+
+A step is one classical RK4 step (4 stages, each a ghost fill and the fused
+right-hand side with the stage update in its epilogue) captured in a hipGraph.
+The state is resident in HBM before the timed region.
+
+Prints one JSON line (rank 0) with the metric, a roofline object for the
+dominant kernel (the fused right-hand side) and a CPU baseline (the oracle,
+OpenMP, on a bounded sample).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+# Algorithmic work per point-update (one point, N^3 per block, advanced by one
+# full RK4 step), frozen from the oracle's counting scalar: F_rhs FP64 ops per
+# point per right-hand side (the :none branch, q = 4, dissipation on, no gauge
+# source), and the stage combination's ops per point per step.
+F_RHS = None  # filled from the oracle count at import time of the roofline
+B_POINT_UPDATE = 2560.0  # bytes: 4 stages x 640 B (a fused stepper's reads and writes)
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--nblocks", type=int, default=16, help="blocks per axis (16 = the full workload)")
+    return ap.parse_args()
+
+
+def kerr_schild_desc(tgh, nblocks):
+    d = tgh.TghDesc()
+    d.float_bits, d.q, d.N = 64, 4, 16
+    d.nblocks[:] = [nblocks] * 3
+    d.periodic[:] = [0, 0, 0]
+    L = 8.0 * nblocks / 16
+    d.lo[:] = [-L] * 3
+    d.hi[:] = [L] * 3
+    d.background, d.gauge, d.damping, d.interior = 2, 2, 1, 1
+    d.M, d.a = 1.0, 0.7
+    d.gamma0_near, d.gamma0_far, d.gamma0_width, d.gamma2 = 1.0, 0.1, 3.0, 0.0
+    d.eps_ko = 0.5
+    d.r0, d.r1, d.rho_max, d.w_ramp, d.rho_ramp = 0.71, 1.21, 4.0, 0.5, 0.5
+    return d
+
+
+def flop_count():
+    """F_rhs and F_combine from the oracle's counting instantiation."""
+    import oracle_lib
+    import tgh_desc as TD
+    d = TD.make_desc(q=4, N=8, nblocks=(1, 1, 1), background=TD.TGH_BG_GAUGE_WAVE, gamma0=1.0,
+                     gamma2=0.0, eps_ko=0.5, wave_amp=0.05, wave_len=1.0)
+    p = oracle_lib.OracleProblem(d)
+    u = p.fill_exact(0.0)
+    p.prepare(u, 0.0)
+    f_rhs = p.count_flops(0, 3, 4, 5)
+    # The stage combination: per variable, stage 1 acc = u + c k, Y = u + c k
+    # (4 ops); stages 2, 3 acc += c k, Y = u + c k (4); stage 4 u = acc + c k (2).
+    f_combine = 20 * (4 + 4 + 4 + 2)
+    return f_rhs, f_combine
+
+
+def cpu_baseline(seconds_target=15.0):
+    """The oracle's RK4 step on a bounded sample of the same case: the hole's
+    own 4x4x4 blocks ([-2, 2]^3 at h = 1/16: core, layer and exterior),
+    OpenMP over blocks on this host's cores."""
+    import oracle_lib
+    import tgh_desc as TD
+    threads = min(16, os.cpu_count() or 1)
+    d = TD.kerr_schild_hole_case(N=16, nblocks=4, halfwidth=2.0, a=0.7)
+    p = oracle_lib.OracleProblem(d)
+    u = p.fill_exact(0.0)
+    dt = 0.25 * (1 / 16) / p.max_speed(u)
+    p.rk4_step(u, 0.0, dt, threads)  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        p.rk4_step(u, (n + 1) * dt, dt, threads)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds_target or n >= 50:
+            break
+    points = 64 * 16 ** 3
+    return {"value": points * n / el, "unit": "point-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle RK4 steps of the same case on the hole's 64 blocks ([-2,2]^3, h=1/16, "
+                      f"{points} points), {n} steps in {el:.1f} s, {threads} OpenMP threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one MI355X and this build has no "
+              f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import __graft_entry__
+    tgh = __graft_entry__.load_package()
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    L = tgh._load()
+
+    desc = kerr_schild_desc(tgh, args.nblocks)
+    ptr = C.c_void_p()
+    tgh._check(L.tgh_problem_create(C.byref(desc), C.byref(ptr)))
+    n_state = int(L.tgh_state_length(ptr))
+    u = torch.empty(n_state, dtype=torch.float64, device="cuda")
+    uptr = C.c_void_p(u.data_ptr())
+    tgh._check(L.tgh_initial_state(ptr, uptr, 0.0))
+    lam = C.c_double()
+    tgh._check(L.tgh_max_speed(ptr, uptr, 0.0, C.byref(lam)))
+    h = 1.0 / 16
+    dt = 0.25 * h / lam.value
+    nb = args.nblocks ** 3
+    points = nb * 16 ** 3
+
+    # warm-up (also captures the graph)
+    tgh._check(L.tgh_rk4(ptr, uptr, 0.0, dt, args.warmup, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tgh._check(L.tgh_rk4(ptr, uptr, args.warmup * dt, dt, args.steps, 1))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = points * args.steps / el
+    assert bool(torch.isfinite(u).all()), "the state is not finite after the timed steps"
+
+    # Per-kernel durations from hipEvents on the problem's stream, over a
+    # second pass of the same steps launched without the graph.
+    tgh._check(L.tgh_profile(ptr, 1))
+    tgh._check(L.tgh_rk4(ptr, uptr, (args.warmup + args.steps) * dt, dt, args.steps, 0))
+    prof = (C.c_double * 4)()
+    tgh._check(L.tgh_profile_read(ptr, prof))
+    tgh._check(L.tgh_profile(ptr, 0))
+    rhs_ms = prof[0] / max(prof[1], 1)
+    fill_ms = prof[2] / max(prof[3], 1)
+
+    f_rhs, f_combine = flop_count()
+    # per launch of the fused RHS: F_rhs per point over the N^3 points of each
+    # block, plus a quarter of the stage combination
+    flops_launch = (f_rhs + f_combine / 4) * points
+    achieved = flops_launch / (rhs_ms * 1e-3) / 1e12
+    step_ms = el / args.steps * 1e3
+    t_roof = max((4 * f_rhs + f_combine) / (FP64_PEAK_TFLOPS * 1e12), B_POINT_UPDATE / (HBM_PEAK_GBS * 1e9))
+    result = {
+        "metric": "FP64 RK4 point-updates/s, 1 MI355X (batched vs plain); adjoint/forward cost",
+        "value": value,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic Kerr-Schild initial data)",
+        "config": {"workload": f"static Kerr-Schild hole M=1 a=0.7, [-8M,8M]^3 at h=M/16, "
+                               f"{nb} blocks of 16^3, q=4, damped interior, KS closed-form gauge "
+                               f"source, RK4 hipGraph step",
+                   "blocks": nb, "points": points, "q": 4, "dt": dt},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_rhs<4, KS, hole, RK4 epilogue>", "kernel_ms": rhs_ms,
+                     "fill_ms": fill_ms, "flops_per_point_rhs": f_rhs,
+                     "flops_per_point_update": 4 * f_rhs + f_combine,
+                     "step_roofline_frac": t_roof / (step_ms * 1e-3 / points)},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline()
+    L.tgh_problem_destroy(ptr)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

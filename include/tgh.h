@@ -1,0 +1,166 @@
+/*
+ * tgh.h — the C ABI of the MI355X generalized-harmonic right-hand side, its
+ * RK4 step and its Kreiss–Oliger term (libtgh_mi355x.so).
+ *
+ * The library replaces the device paths of eschnett/TreeGeneralizedHarmonic.jl
+ * behind the package's own verbs. Every entry point below names the reference
+ * interface it stands in for (file:line in that repository). Arrays passed in
+ * are DEVICE pointers (hipMalloc'd or a torch CUDA tensor's data_ptr()), FP64,
+ * in the reference's state layout: column-major (i, j, k, v, b) with
+ * (N+1)^3 owned points per block (closed vertex blocks), 20 variables
+ * (h_ab then Pi_ab in the packed order tt,tx,ty,tz,xx,xy,xz,yy,yz,zz) and the
+ * blocks in the order tgh_block_coords reports (Morton order of the brick).
+ *
+ * Every function returns TGH_OK or an error code; tgh_last_error() gives the
+ * message of the last failure on the calling thread. All work is ordered on
+ * the problem's HIP stream and every verb synchronises before returning.
+ * Nothing here allocates on the device except tgh_problem_create.
+ */
+#ifndef TGH_H
+#define TGH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TGH_OK = 0,
+  TGH_EINVAL = 1,      /* an argument the reference refuses (ArgumentError) */
+  TGH_EDEVICE = 2,     /* a HIP runtime failure */
+  TGH_ENOTFINITE = 3,  /* max_speed: non-finite evolved state or speed */
+  TGH_EUNSUPPORTED = 4 /* a reference feature this library does not build */
+};
+
+enum { /* SpacetimeMetrics backgrounds, restated in closed form */
+  TGH_BG_MINKOWSKI = 0,
+  TGH_BG_GAUGE_WAVE = 1,        /* AwA gauge wave, H = 1 - A sin(2 pi (x - t)/d) */
+  TGH_BG_KERR_SCHILD = 2,       /* KerrSchild(M, a), spin along +z, at center */
+  TGH_BG_SHIFTED_MINKOWSKI = 3  /* t^ = t + A w tanh(x / w) */
+};
+enum { /* gauge source kinds, src/gauge.jl:342-359 */
+  TGH_GAUGE_NONE = 0,        /* harmonic background: H_a = 0 */
+  TGH_GAUGE_SAMPLED = 1,     /* Hsrc sampled once at problem creation */
+  TGH_GAUGE_KERR_SCHILD = 2  /* closed-form KerrSchildSource, in-kernel */
+};
+enum { TGH_DAMPING_CONSTANT = 0, TGH_DAMPING_GAUSSIAN = 1 }; /* src/gauge.jl:1094-1156 */
+enum { /* interior variants, src/interior.jl:178-238 */
+  TGH_INTERIOR_NONE = 0,
+  TGH_INTERIOR_DAMPED = 1,
+  TGH_INTERIOR_FROZEN = 2,
+  TGH_INTERIOR_PASTED = 3
+};
+enum { /* tgh_rk4 flags */
+  TGH_RK4_GRAPH = 1 /* capture the step in a hipGraph and replay it */
+};
+
+/* The case and its uniform forest: GHCase (src/initialdata.jl:158-311) plus
+ * gh_forest (src/initialdata.jl:1370-1400) and GHProblem's q
+ * (src/evolution.jl:1034-1149). */
+typedef struct tgh_desc {
+  int32_t float_bits;   /* 64: FP64 only; anything else is refused */
+  int32_t q;            /* finite-difference order: 2, 4, 6 or 8; G = q/2 + 1 */
+  int32_t N;            /* cells per block and axis */
+  int32_t nblocks[3];   /* the brick of root blocks */
+  int32_t periodic[3];  /* 1 = periodic, 0 = Dirichlet from the background */
+  int32_t background;   /* TGH_BG_* */
+  int32_t gauge;        /* TGH_GAUGE_* */
+  int32_t damping;      /* TGH_DAMPING_* */
+  int32_t interior;     /* TGH_INTERIOR_* */
+  int32_t reserved;
+  double lo[3], hi[3];  /* the box; cells must be cubic */
+  double M, a;          /* hole mass and spin parameter */
+  double center[3];     /* the hole's (static) center */
+  double wave_amp, wave_len; /* gauge wave A, d; shifted Minkowski A, w */
+  double gamma0;        /* constant damping rate */
+  double gamma0_near, gamma0_far, gamma0_width; /* Gaussian profile */
+  double gamma2;
+  double eps_ko;        /* Kreiss–Oliger amplitude; 0 compiles the term out */
+  double r0, r1, rho_max, w_ramp, rho_ramp; /* the interior's layer */
+} tgh_desc;
+
+/* Matter's 3+1 projections at every owned point (src/evolution.jl:1295-1311,
+ * src/pointwise.jl:667-698): rho (N+1)^3 x nb, S (N+1)^3 x 3 x nb,
+ * Sij (N+1)^3 x 6 x nb in (xx, xy, xz, yy, yz, zz) order; device pointers. */
+typedef struct tgh_matter {
+  const double* rho;
+  const double* S;
+  const double* Sij;
+} tgh_matter;
+
+typedef struct tgh_problem tgh_problem;
+
+/* GHProblem(U, schedule, case; q) — src/evolution.jl:1034-1149. Checks what
+ * the reference's constructor checks (order, ghost width, interior radii
+ * against the mesh) and allocates the work arrays on the current device. */
+int tgh_problem_create(const tgh_desc* desc, tgh_problem** out);
+/* No reference counterpart: releases the problem's device memory. */
+int tgh_problem_destroy(tgh_problem* p);
+
+/* Doubles in one state vector: (N+1)^3 * 20 * nblocks (statevector(fs)). */
+int64_t tgh_state_length(const tgh_problem* p);
+/* Block b's integer coordinates in the brick, 3 per block (block_corners,
+ * src/evolution.jl:120-130). */
+int tgh_block_coords(const tgh_problem* p, int32_t* out);
+/* Order the problem's work on a caller's hipStream_t (NULL = its own). */
+int tgh_set_stream(tgh_problem* p, void* stream);
+
+/* prepare!(p, u, t) — src/evolution.jl:1349-1357: scatter u into the work
+ * array and fill the ghosts (periodic copies, closed-block face copies and the
+ * Dirichlet hook of src/boundaries.jl:49-63 at this t). u is not written. */
+int tgh_prepare(tgh_problem* p, const double* u, double t);
+/* rhs!(du, p, t; matter) — src/evolution.jl:1386-1390: the fused kernel
+ * gh_rhs_kernel! (src/evolution.jl:846-863) at every owned point of the work
+ * array prepare left. matter may be NULL (vacuum). */
+int tgh_rhs(tgh_problem* p, double* du, double t, const tgh_matter* matter);
+/* gh_rhs!(du, u, p, t; matter) — src/evolution.jl:1327-1331. */
+int tgh_gh_rhs(tgh_problem* p, double* du, const double* u, double t,
+               const tgh_matter* matter);
+/* settle!(p, u, t) — src/evolution.jl:1370-1374: prepare, then gather back. */
+int tgh_settle(tgh_problem* p, double* u, double t);
+/* stage_limit!(u, p, t) / step_limit!(u, p, t) — src/stepping.jl:91-97: the
+ * :pasted overwrite (src/evolution.jl:888-905); a no-op for the other
+ * variants. Cases with range bounds are refused at creation. */
+int tgh_stage_limit(tgh_problem* p, double* u, double t);
+int tgh_step_limit(tgh_problem* p, double* u, double t);
+/* max_speed(p, u, t) — src/evolution.jl:1539-1560: the largest
+ * alpha sqrt(tr gamma^ij) + |beta| over evolved points, checked:
+ * TGH_ENOTFINITE if an evolved value or the speed is not finite and positive. */
+int tgh_max_speed(tgh_problem* p, const double* u, double t, double* out);
+/* initial_state!(u, p, t) — src/stepping.jl:110-115: the case's exact state
+ * with the core rule, then the limiter. */
+int tgh_initial_state(tgh_problem* p, double* u, double t);
+/* add_noise!(U, rng; amplitude) — src/initialdata.jl:1961-2013 with the
+ * SplitMix64 point draw; base is the draw's seed word. */
+int tgh_add_noise(tgh_problem* p, double* u, uint64_t base, double amplitude);
+/* gh_solve / IRK.step! x nsteps — src/stepping.jl:208-239: classical RK4 with
+ * the limiter on every stage value and on the result, fixed dt, in place.
+ * The stage update is fused into the right-hand side's epilogue. On return
+ * u is settled (face copies equal their primaries). */
+int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t flags);
+/* gh_error! + error_norms — src/constraints.jl:311-396,629-671: masked
+ * volume-weighted L2 and Linf of u - u_exact(t) over evolved primaries. */
+int tgh_error_norms(tgh_problem* p, const double* u, double t, double* l2, double* linf);
+/* Doubles in the problem's work array: (N+1+2G)^3 * 20 * nblocks. */
+int64_t tgh_work_length(const tgh_problem* p);
+/* Copy the work array prepare filled (ghosts included) into a caller's
+ * device buffer of tgh_work_length doubles, for inspection; the layout is the
+ * reference's ghosted FieldSet work array. */
+int tgh_copy_work(tgh_problem* p, double* dst);
+/* No reference counterpart: per-kernel timing for the benchmark. While
+ * enabled, tgh_rk4 without TGH_RK4_GRAPH brackets every launch of the fused
+ * right-hand side and of the ghost fill with hipEvents on the problem's stream;
+ * tgh_profile_read gives out[0..3] = summed milliseconds of the right-hand
+ * side launches, their count, summed milliseconds of the fills, their count,
+ * since the last tgh_profile(p, 1). */
+int tgh_profile(tgh_problem* p, int32_t enable);
+int tgh_profile_read(tgh_problem* p, double* out);
+/* Message of the last failure on this thread ("" if none). */
+const char* tgh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TGH_H */

@@ -1,0 +1,578 @@
+// CPU oracle, mesh half: a uniform forest of closed vertex blocks, the ghost
+// fill, the fused right-hand side at every owned point, RK4 and the analysis
+// reductions.
+//
+// TEST INFRASTRUCTURE ONLY (see gh_oracle.hpp). Restates
+// src/evolution.jl:100-435,448-695,932-949,1327-1560, src/stepping.jl:53-125,
+// src/constraints.jl:311-396 and TreeAMR 0.3's uniform-forest ghost fill.
+#pragma once
+#include "gh_oracle.hpp"
+
+namespace orc {
+
+// Morton key of a block's integer coordinates, x lowest.
+inline uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  uint64_t k = 0;
+  for (int bit = 0; bit < 21; ++bit) {
+    k |= (uint64_t)((x >> bit) & 1) << (3 * bit);
+    k |= (uint64_t)((y >> bit) & 1) << (3 * bit + 1);
+    k |= (uint64_t)((z >> bit) & 1) << (3 * bit + 2);
+  }
+  return k;
+}
+
+struct Mesh {
+  int N = 16, G = 3, q = 4;
+  int nb[3] = {1, 1, 1};
+  double lo[3] = {0, 0, 0};
+  double h = 1.0 / 16;
+  bool periodic[3] = {true, true, true};
+  std::vector<std::array<int, 3>> blocks;  // block coordinates, Morton order
+  long S = 0, P = 0;                       // work and state extents per axis
+
+  void init() {
+    G = q / 2 + 1;
+    S = N + 1 + 2 * G;
+    P = N + 1;
+    blocks.clear();
+    for (int k = 0; k < nb[2]; ++k)
+      for (int j = 0; j < nb[1]; ++j)
+        for (int i = 0; i < nb[0]; ++i) blocks.push_back({i, j, k});
+    std::stable_sort(blocks.begin(), blocks.end(), [](const auto& a, const auto& b) {
+      return morton3(a[0], a[1], a[2]) < morton3(b[0], b[1], b[2]);
+    });
+  }
+  long nblocks() const { return (long)blocks.size(); }
+  long state_size() const { return P * P * P * 20 * nblocks(); }
+  long work_size() const { return S * S * S * 20 * nblocks(); }
+  long sidx(long b, int v, long i, long j, long k) const {
+    return i + P * (j + P * (k + P * (v + 20 * b)));
+  }
+  // work index of owned-relative l (may be negative, down to -G)
+  long widx(long b, int v, long i, long j, long k) const {
+    return (i + G) + S * ((j + G) + S * ((k + G) + S * (v + 20 * b)));
+  }
+  void position(long b, long i, long j, long k, double x[3]) const {
+    long g[3] = {blocks[b][0] * (long)N + i, blocks[b][1] * (long)N + j, blocks[b][2] * (long)N + k};
+    for (int d = 0; d < 3; ++d) x[d] = lo[d] + (double)g[d] * h;
+  }
+  // Source of a work point: -2 Dirichlet, -1 own primary, else owner block and
+  // local index.
+  int classify(long b, const long l[3], long& owner, long loc[3]) const {
+    long ob[3];
+    for (int d = 0; d < 3; ++d) {
+      long g = blocks[b][d] * (long)N + l[d];
+      long tot = (long)nb[d] * N;
+      if (periodic[d]) {
+        long gw = ((g % tot) + tot) % tot;
+        ob[d] = gw / N;
+        loc[d] = gw - ob[d] * N;
+      } else {
+        if (g < 0 || g > tot) return -2;
+        ob[d] = std::min(g / N, (long)nb[d] - 1);
+        loc[d] = g - ob[d] * N;
+      }
+    }
+    // owner block index (linear search over the Morton order is fine here)
+    owner = block_index(ob[0], ob[1], ob[2]);
+    if (owner == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return -1;
+    return 0;
+  }
+  std::vector<long> lookup;
+  long block_index(long i, long j, long k) const {
+    return lookup[i + nb[0] * (j + nb[1] * k)];
+  }
+  void build_lookup() {
+    lookup.assign((size_t)nb[0] * nb[1] * nb[2], -1);
+    for (long b = 0; b < nblocks(); ++b)
+      lookup[blocks[b][0] + nb[0] * (blocks[b][1] + nb[1] * blocks[b][2])] = b;
+  }
+  // Is owned point l of block b a primary (counted once in sums)?
+  bool is_primary(long b, long i, long j, long k) const {
+    long l[3] = {i, j, k}, owner, loc[3];
+    return classify(b, l, owner, loc) == -1;
+  }
+};
+
+struct Problem {
+  Mesh mesh;
+  Case cs;
+  std::vector<double> work;
+  std::vector<double> Hsrc;  // sampled gauge source at owned points, 20 per point
+  std::vector<double> w1, w2, wD;
+
+  void setup() {
+    mesh.init();
+    mesh.build_lookup();
+    work.assign(mesh.work_size(), 0.0);
+    w1 = to_double(rational_derivative_weights(mesh.q, 1));
+    w2 = to_double(rational_derivative_weights(mesh.q, 2));
+    wD = to_double(rational_dissipation_weights(dissipation_rank(mesh.q)));
+    if (cs.gauge == GAUGE_SAMPLED) sample_gauge_source(0.0);
+  }
+  // sample_gauge_source! (gauge.jl:156-184) with the core rule.
+  void sample_gauge_source(double t) {
+    const Mesh& m = mesh;
+    Hsrc.assign(m.state_size(), 0.0);
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3], xc[3], Hl[4], dHl[4][4];
+            m.position(b, i, j, k, x);
+            core_position(cs, x, xc);
+            gauge_source_grad(cs.bg, t, xc[0], xc[1], xc[2], Hl, dHl);
+            for (int a = 0; a < 4; ++a) Hsrc[m.sidx(b, a, i, j, k)] = Hl[a];
+            // column-major dHl[c][a] (gauge.jl: entries 5:20 in SMatrix order)
+            for (int n = 0; n < 16; ++n) Hsrc[m.sidx(b, 4 + n, i, j, k)] = dHl[n % 4][n / 4];
+          }
+  }
+
+  // fill_exact! (initialdata.jl:1913-1914): every owned point of the state.
+  void fill_exact(double* u, double t) const {
+    const Mesh& m = mesh;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3], s[20];
+            m.position(b, i, j, k, x);
+            case_state(cs, t, x, s);
+            for (int v = 0; v < 20; ++v) u[m.sidx(b, v, i, j, k)] = s[v];
+          }
+  }
+  // add_noise! (initialdata.jl:1961-2000), with the SplitMix64 draw.
+  void add_noise(double* u, uint64_t base, double amp) const {
+    const Mesh& m = mesh;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (int v = 0; v < 20; ++v)
+        for (long k = 0; k <= m.N; ++k)
+          for (long j = 0; j <= m.N; ++j)
+            for (long i = 0; i <= m.N; ++i) {
+              double x[3];
+              m.position(b, i, j, k, x);
+              u[m.sidx(b, v, i, j, k)] += amp * (2.0 * point_draw(base, x, v + 1) - 1.0);
+            }
+  }
+
+  // prepare! (evolution.jl:1349-1357): scatter and fill.
+  void prepare(const double* u, double t) {
+    const Mesh& m = mesh;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (int v = 0; v < 20; ++v)
+        for (long k = 0; k <= m.N; ++k)
+          for (long j = 0; j <= m.N; ++j)
+            for (long i = 0; i <= m.N; ++i) work[m.widx(b, v, i, j, k)] = u[m.sidx(b, v, i, j, k)];
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = -m.G; k <= m.N + m.G; ++k)
+        for (long j = -m.G; j <= m.N + m.G; ++j)
+          for (long i = -m.G; i <= m.N + m.G; ++i) {
+            long l[3] = {i, j, k}, owner, loc[3];
+            int c = m.classify(b, l, owner, loc);
+            if (c == -1) continue;
+            if (c == -2) {
+              long g[3] = {m.blocks[b][0] * (long)m.N + i, m.blocks[b][1] * (long)m.N + j,
+                           m.blocks[b][2] * (long)m.N + k};
+              double x[3], s[20];
+              for (int d = 0; d < 3; ++d) x[d] = m.lo[d] + (double)g[d] * m.h;
+              case_state(cs, t, x, s);
+              for (int v = 0; v < 20; ++v) work[m.widx(b, v, i, j, k)] = s[v];
+            } else {
+              for (int v = 0; v < 20; ++v)
+                work[m.widx(b, v, i, j, k)] = work[m.widx(owner, v, loc[0], loc[1], loc[2])];
+            }
+          }
+  }
+
+  // ---- the fused kernel at one owned point (evolution.jl:258-435,448-695) ----
+  template <class T>
+  T axis_stencil(const std::vector<double>& w, long base, long stride) const {
+    int n = (int)w.size(), r = (n - 1) / 2;
+    T s = T(0.0);
+    bool first = true;
+    for (int k = 0; k < n; ++k) {
+      if (w[k] == 0.0) continue;  // zero weights are bitwise-neutral (CODE.md:6647-6653)
+      T t = w[k] * T(work[base + (long)(k - r) * stride]);
+      s = first ? t : s + t;
+      first = false;
+    }
+    return s;
+  }
+  template <class T>
+  T mixed_stencil(const std::vector<double>& w, long base, long s1, long s2) const {
+    int n = (int)w.size(), r = (n - 1) / 2;
+    T outer = T(0.0);
+    bool fo = true;
+    for (int a = 0; a < n; ++a) {
+      if (w[a] == 0.0) continue;
+      T inner = T(0.0);
+      bool fi = true;
+      for (int e = 0; e < n; ++e) {
+        if (w[e] == 0.0) continue;
+        T t = w[e] * T(work[base + (long)(a - r) * s1 + (long)(e - r) * s2]);
+        inner = fi ? t : inner + t;
+        fi = false;
+      }
+      T t = w[a] * inner;
+      outer = fo ? t : outer + t;
+      fo = false;
+    }
+    return outer;
+  }
+
+  // F(u) at one owned point into dth, dtP (gh_rhs_at_point, evolution.jl:423-435).
+  template <class T>
+  void rhs_point_F(long b, long i, long j, long k, double tt, T g0, T g2, T epsh, bool diss,
+                   V10<T>& dth, V10<T>& dtP) const {
+    const Mesh& m = mesh;
+    long st[3] = {1, m.S, m.S * m.S};
+    long sv = m.S * m.S * m.S;
+    long var = m.widx(b, 0, i, j, k);
+    T inv_h = T(1.0) / T(m.h);
+    // (1) the state, the 30 first derivatives, the coefficient set
+    V10<T> hv, Pv, dh[3];
+    for (int v = 0; v < NC; ++v) {
+      hv[v] = T(work[var + v * sv]);
+      Pv[v] = T(work[var + (NC + v) * sv]);
+    }
+    for (int d = 0; d < 3; ++d)
+      for (int v = 0; v < NC; ++v) dh[d][v] = inv_h * axis_stencil<T>(w1, var + v * sv, st[d]);
+    MQ<T> mq = metric_quantities(hv);
+    T a_div = mq.alpha / mq.sqrtg;
+    // (2) d_t h, complete (evolution.jl:301-307)
+    for (int v = 0; v < NC; ++v) {
+      T s = mq.beta[0] * dh[0][v] + mq.beta[1] * dh[1][v] + mq.beta[2] * dh[2][v] + a_div * Pv[v];
+      if (diss) {
+        long bh = var + v * sv;
+        s = s + epsh * (axis_stencil<T>(wD, bh, st[0]) + axis_stencil<T>(wD, bh, st[1]) +
+                        axis_stencil<T>(wD, bh, st[2]));
+      }
+      dth[v] = s;
+    }
+    // (3) the gauge source and the source
+    T Hl[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
+    T dHl[4][4];
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 4; ++c) dHl[a][c] = T(0.0);
+    if (cs.gauge == GAUGE_SAMPLED) {
+      for (int a = 0; a < 4; ++a) Hl[a] = T(Hsrc[m.sidx(b, a, i, j, k)]);
+      for (int n = 0; n < 16; ++n) dHl[n % 4][n / 4] = T(Hsrc[m.sidx(b, 4 + n, i, j, k)]);
+    } else if (cs.gauge == GAUGE_KS) {
+      ks_algebraic_source<T>(cs.bg.M, cs.ks_S, cs.ks_u, hv, dth, dh, Hl, dHl);
+    }
+    V10<T> msrc = gh_node_source_lean(mq, dth, dh, Hl, dHl, g0, g2);
+    // (4) the coefficients of the Pi components
+    T A[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) A[a][c] = (mq.alpha * mq.sqrtg) * mq.gamu[a][c];
+    T divb, divA[3];
+    metric_divergences(mq, dh, divb, divA);
+    // gh_rhs_pi per component (evolution.jl:324-354)
+    T inv_h2 = inv_h * inv_h;
+    for (int v = 0; v < NC; ++v) {
+      long bh = var + v * sv, bP = bh + NC * sv;
+      T Pi = T(work[bP]);
+      T dh1 = inv_h * axis_stencil<T>(w1, bh, st[0]);
+      T dh2 = inv_h * axis_stencil<T>(w1, bh, st[1]);
+      T dh3 = inv_h * axis_stencil<T>(w1, bh, st[2]);
+      T dP1 = inv_h * axis_stencil<T>(w1, bP, st[0]);
+      T dP2 = inv_h * axis_stencil<T>(w1, bP, st[1]);
+      T dP3 = inv_h * axis_stencil<T>(w1, bP, st[2]);
+      T s = mq.beta[0] * dP1 + mq.beta[1] * dP2 + mq.beta[2] * dP3 + divb * Pi + divA[0] * dh1 +
+            divA[1] * dh2 + divA[2] * dh3;
+      s = s + (A[0][0] * (inv_h2 * axis_stencil<T>(w2, bh, st[0])) +
+               A[1][1] * (inv_h2 * axis_stencil<T>(w2, bh, st[1])) +
+               A[2][2] * (inv_h2 * axis_stencil<T>(w2, bh, st[2])));
+      T dxy = inv_h2 * mixed_stencil<T>(w1, bh, st[0], st[1]);
+      T dxz = inv_h2 * mixed_stencil<T>(w1, bh, st[0], st[2]);
+      T dyz = inv_h2 * mixed_stencil<T>(w1, bh, st[1], st[2]);
+      s = s + 2.0 * (A[0][1] * dxy + A[0][2] * dxz + A[1][2] * dyz);
+      if (diss)
+        s = s + epsh * (axis_stencil<T>(wD, bP, st[0]) + axis_stencil<T>(wD, bP, st[1]) +
+                        axis_stencil<T>(wD, bP, st[2]));
+      dtP[v] = s + msrc[v];
+    }
+  }
+
+  // gh_rhs_point! (evolution.jl:448-695) for the static interior variants.
+  void rhs_point(long b, long i, long j, long k, double t, double* du) const {
+    const Mesh& m = mesh;
+    double x[3];
+    m.position(b, i, j, k, x);
+    double g0 = damping_rate(cs, x);
+    double epsh = cs.eps_ko / m.h;
+    bool diss = cs.eps_ko != 0.0;
+    V10<double> dth, dtP;
+    if (cs.interior != INT_NONE) {
+      double r = interior_radius(cs, x);
+      bool frozen = cs.interior == INT_PASTED ? r < cs.r1 : r < cs.r0;
+      if (frozen) {
+        for (int v = 0; v < 20; ++v) du[m.sidx(b, v, i, j, k)] = 0.0;
+        return;
+      }
+      rhs_point_F<double>(b, i, j, k, t, g0, cs.g2, epsh, diss, dth, dtP);
+      if (r >= cs.r1) {
+        for (int v = 0; v < NC; ++v) {
+          du[m.sidx(b, v, i, j, k)] = dth[v];
+          du[m.sidx(b, NC + v, i, j, k)] = dtP[v];
+        }
+        return;
+      }
+      double w, rho;
+      layer_profiles(r, cs.r0, cs.r1, cs.rho_max, cs.w_ramp, cs.rho_ramp, w, rho);
+      if (cs.interior == INT_FROZEN) rho = 0.0;
+      V10<double> he, Pe, dhe[3];
+      background_state(cs.bg, t, x, he, Pe, dhe);
+      long var = m.widx(b, 0, i, j, k), sv = m.S * m.S * m.S;
+      for (int v = 0; v < NC; ++v) {
+        du[m.sidx(b, v, i, j, k)] = w * dth[v] - rho * (work[var + v * sv] - he[v]);
+        du[m.sidx(b, NC + v, i, j, k)] = w * dtP[v] - rho * (work[var + (NC + v) * sv] - Pe[v]);
+      }
+      return;
+    }
+    rhs_point_F<double>(b, i, j, k, t, g0, cs.g2, epsh, diss, dth, dtP);
+    for (int v = 0; v < NC; ++v) {
+      du[m.sidx(b, v, i, j, k)] = dth[v];
+      du[m.sidx(b, NC + v, i, j, k)] = dtP[v];
+    }
+  }
+
+  // rhs! (evolution.jl:1386-1390): every owned point of every block.
+  void rhs(double* du, double t, int nthreads = 1) const {
+    const Mesh& m = mesh;
+    long nb = m.nblocks();
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (long b = 0; b < nb; ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) rhs_point(b, i, j, k, t, du);
+  }
+  void gh_rhs(double* du, const double* u, double t, int nthreads = 1) {
+    prepare(u, t);
+    rhs(du, t, nthreads);
+  }
+
+  // The one-expression assembly the reference's kernel test measures the
+  // streaming kernel against (test/evolution_tests.jl:37-89): stencils from
+  // apply_stencil and gh_node_rhs_expanded, plus the dissipation and the
+  // source's correction for the dissipated d_t h.
+  void host_rhs_at(long b, long i, long j, long k, double t, double out[20]) const {
+    const Mesh& m = mesh;
+    double x[3];
+    m.position(b, i, j, k, x);
+    double g0 = damping_rate(cs, x);
+    double hsp = m.h;
+    auto at = [&](int v, long ii, long jj, long kk) { return work[m.widx(b, v, ii, jj, kk)]; };
+    auto apply = [&](const std::vector<double>& w, int v, int d) {
+      int n = (int)w.size(), r = (n - 1) / 2;
+      double s = 0;
+      for (int kk = 0; kk < n; ++kk) {
+        long o = kk - r;
+        double val_ = at(v, i + (d == 0) * o, j + (d == 1) * o, k + (d == 2) * o);
+        s = (kk == 0) ? w[kk] * val_ : s + w[kk] * val_;
+      }
+      return s;
+    };
+    auto apply_mixed = [&](int v, int d1, int d2) {
+      int n = (int)w1.size(), r = (n - 1) / 2;
+      double outer = 0;
+      for (int a = 0; a < n; ++a) {
+        double inner = 0;
+        for (int e = 0; e < n; ++e) {
+          long oa = a - r, oe = e - r;
+          long ii = i + (d1 == 0) * oa + (d2 == 0) * oe;
+          long jj = j + (d1 == 1) * oa + (d2 == 1) * oe;
+          long kk = k + (d1 == 2) * oa + (d2 == 2) * oe;
+          double t_ = w1[e] * at(v, ii, jj, kk);
+          inner = (e == 0) ? t_ : inner + t_;
+        }
+        double t_ = w1[a] * inner;
+        outer = (a == 0) ? t_ : outer + t_;
+      }
+      return outer;
+    };
+    V10<double> hv, Pv, dh[3], dP[3], ddh[6];
+    for (int v = 0; v < NC; ++v) {
+      hv[v] = at(v, i, j, k);
+      Pv[v] = at(NC + v, i, j, k);
+      for (int d = 0; d < 3; ++d) {
+        dh[d][v] = apply(w1, v, d) / hsp;
+        dP[d][v] = apply(w1, NC + v, d) / hsp;
+      }
+    }
+    const int pairs[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int n = 0; n < 6; ++n)
+      for (int v = 0; v < NC; ++v) {
+        int d1 = pairs[n][0], d2 = pairs[n][1];
+        ddh[n][v] = d1 == d2 ? apply(w2, v, d1) / (hsp * hsp) : apply_mixed(v, d1, d2) / (hsp * hsp);
+      }
+    double Hl[4] = {0, 0, 0, 0}, dHl[4][4] = {{0}};
+    if (cs.gauge == GAUGE_SAMPLED) {
+      for (int a = 0; a < 4; ++a) Hl[a] = Hsrc[m.sidx(b, a, i, j, k)];
+      for (int n = 0; n < 16; ++n) dHl[n % 4][n / 4] = Hsrc[m.sidx(b, 4 + n, i, j, k)];
+    }
+    V10<double> dth, dtP;
+    if (cs.gauge == GAUGE_KS) {
+      // the algebraic source reads d_t h, which the expanded form forms inside;
+      // evaluate it from the undissipated d_t h first, then correct below
+      MQ<double> mq0 = metric_quantities(hv);
+      V10<double> dth0;
+      for (int n = 0; n < NC; ++n)
+        dth0[n] = mq0.beta[0] * dh[0][n] + mq0.beta[1] * dh[1][n] + mq0.beta[2] * dh[2][n] +
+                  (mq0.alpha / mq0.sqrtg) * Pv[n];
+      ks_algebraic_source<double>(cs.bg.M, cs.ks_S, cs.ks_u, hv, dth0, dh, Hl, dHl);
+    }
+    gh_node_rhs_expanded(hv, Pv, dh, dP, ddh, Hl, dHl, g0, cs.g2, dth, dtP);
+    if (cs.eps_ko != 0.0 || cs.gauge == GAUGE_KS) {
+      V10<double> Qh, QP;
+      for (int v = 0; v < NC; ++v) {
+        Qh[v] = cs.eps_ko == 0.0 ? 0.0
+                                 : (cs.eps_ko / hsp) * ((apply(wD, v, 0) + apply(wD, v, 1)) + apply(wD, v, 2));
+        QP[v] = cs.eps_ko == 0.0
+                    ? 0.0
+                    : (cs.eps_ko / hsp) *
+                          ((apply(wD, NC + v, 0) + apply(wD, NC + v, 1)) + apply(wD, NC + v, 2));
+      }
+      MQ<double> mq = metric_quantities(hv);
+      V10<double> dthQ;
+      for (int v = 0; v < NC; ++v) dthQ[v] = dth[v] + Qh[v];
+      double Hl2[4], dHl2[4][4];
+      for (int a = 0; a < 4; ++a) {
+        Hl2[a] = Hl[a];
+        for (int c = 0; c < 4; ++c) dHl2[a][c] = dHl[a][c];
+      }
+      if (cs.gauge == GAUGE_KS) ks_algebraic_source<double>(cs.bg.M, cs.ks_S, cs.ks_u, hv, dthQ, dh, Hl2, dHl2);
+      double dgA[4][4][4], dgB[4][4][4];
+      dg4(dthQ, dh, dgA);
+      dg4(dth, dh, dgB);
+      V10<double> sA = gh_node_source(mq, dgA, Hl2, dHl2, g0, cs.g2);
+      V10<double> sB = gh_node_source(mq, dgB, Hl, dHl, g0, cs.g2);
+      for (int v = 0; v < NC; ++v) {
+        dth[v] = dthQ[v];
+        dtP[v] = dtP[v] + QP[v] + (sA[v] - sB[v]);
+      }
+    }
+    for (int v = 0; v < NC; ++v) {
+      out[v] = dth[v];
+      out[NC + v] = dtP[v];
+    }
+  }
+  // rhs_scale (test/evolution_tests.jl:97-108).
+  double rhs_scale(long b, long i, long j, long k) const {
+    const Mesh& m = mesh;
+    double s = 0;
+    int n = (int)w2.size(), r = (n - 1) / 2;
+    for (int v = 0; v < 20; ++v)
+      for (int d = 0; d < 3; ++d) {
+        double acc = 0;
+        for (int kk = 0; kk < n; ++kk) {
+          long o = kk - r;
+          double t_ = w2[kk] * work[m.widx(b, v, i + (d == 0) * o, j + (d == 1) * o, k + (d == 2) * o)];
+          acc = kk == 0 ? t_ : acc + t_;
+        }
+        s = std::max(s, std::abs(acc) / (m.h * m.h));
+        s = std::max(s, std::abs(work[m.widx(b, v, i, j, k)]));
+      }
+    return std::max(s, 1.0);
+  }
+
+  // ---- the time step and the analysis reductions ----
+  bool is_evolved(const double x[3]) const {
+    if (cs.interior == INT_NONE) return true;
+    return interior_radius(cs, x) >= cs.r1;
+  }
+  // max_speed (evolution.jl:932-949,1539-1560) over evolved owned points.
+  double max_speed(const double* u) const {
+    const Mesh& m = mesh;
+    double lam = 0;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3];
+            m.position(b, i, j, k, x);
+            if (!is_evolved(x)) continue;
+            V10<double> hv;
+            for (int v = 0; v < NC; ++v) hv[v] = u[m.sidx(b, v, i, j, k)];
+            MQ<double> mq = metric_quantities(hv);
+            double l = mq.alpha * std::sqrt(mq.gamu[0][0] + mq.gamu[1][1] + mq.gamu[2][2]) +
+                       std::sqrt(mq.beta[0] * mq.beta[0] + mq.beta[1] * mq.beta[1] + mq.beta[2] * mq.beta[2]);
+            if (!(l <= lam)) lam = l;  // NaN propagates
+          }
+    return lam;
+  }
+  // Masked volume-weighted L2 and Linf of u - u_exact(t) over evolved
+  // primaries (constraints.jl:311-396, 655-671; CODE.md: the L2 divides by the
+  // evolved volume).
+  void error_norms(const double* u, double t, double& l2, double& linf) const {
+    const Mesh& m = mesh;
+    double num = 0, den = 0;
+    linf = 0;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3], s[20];
+            m.position(b, i, j, k, x);
+            if (!is_evolved(x)) continue;
+            case_state(cs, t, x, s);
+            double e2 = 0;
+            for (int v = 0; v < 20; ++v) {
+              double e = u[m.sidx(b, v, i, j, k)] - s[v];
+              e2 += e * e;
+              linf = std::max(linf, std::abs(e));
+            }
+            if (!m.is_primary(b, i, j, k)) continue;
+            num += e2;
+            den += 1.0;
+          }
+    l2 = den > 0 ? std::sqrt(num / den) : 0.0;
+  }
+
+  // The :pasted overwrite (evolution.jl:888-905); a no-op for other variants.
+  void stage_limit(double* u, double t) const {
+    if (cs.interior != INT_PASTED) return;
+    const Mesh& m = mesh;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3], s[20];
+            m.position(b, i, j, k, x);
+            if (interior_radius(cs, x) >= cs.r1) continue;
+            case_state(cs, t, x, s);
+            for (int v = 0; v < 20; ++v) u[m.sidx(b, v, i, j, k)] = s[v];
+          }
+  }
+
+  // One classical RK4 step with the limiter on each stage value and on the
+  // result (stepping.jl:53-73, IMEXRungeKutta RK4).
+  void rk4_step(double* u, double t, double dt, int nthreads = 1) {
+    long n = mesh.state_size();
+    std::vector<double> k1(n), k2(n), k3(n), k4(n), y(n);
+    gh_rhs(k1.data(), u, t, nthreads);
+    for (long i = 0; i < n; ++i) y[i] = u[i] + (dt / 2) * k1[i];
+    stage_limit(y.data(), t + dt / 2);
+    gh_rhs(k2.data(), y.data(), t + dt / 2, nthreads);
+    for (long i = 0; i < n; ++i) y[i] = u[i] + (dt / 2) * k2[i];
+    stage_limit(y.data(), t + dt / 2);
+    gh_rhs(k3.data(), y.data(), t + dt / 2, nthreads);
+    for (long i = 0; i < n; ++i) y[i] = u[i] + dt * k3[i];
+    stage_limit(y.data(), t + dt);
+    gh_rhs(k4.data(), y.data(), t + dt, nthreads);
+    for (long i = 0; i < n; ++i) u[i] = u[i] + (dt / 6) * (k1[i] + 2 * k2[i] + 2 * k3[i] + k4[i]);
+    stage_limit(u, t + dt);
+  }
+  // settle! (evolution.jl:1370-1374): fill, then gather back.
+  void settle(double* u, double t) {
+    prepare(u, t);
+    const Mesh& m = mesh;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (int v = 0; v < 20; ++v)
+        for (long k = 0; k <= m.N; ++k)
+          for (long j = 0; j <= m.N; ++j)
+            for (long i = 0; i <= m.N; ++i) u[m.sidx(b, v, i, j, k)] = work[m.widx(b, v, i, j, k)];
+  }
+};
+
+}  // namespace orc

@@ -1,0 +1,297 @@
+// C entry points of the CPU oracle, for the tests (ctypes) and bench.py's
+// cpu_baseline leg. TEST INFRASTRUCTURE ONLY: the product library never links
+// this. It reads the case descriptor of include/tgh.h so that a test hands
+// the same case to the oracle and to the library.
+#include <omp.h>
+
+#include "../include/tgh.h"
+#include "gh_oracle_mesh.hpp"
+
+using namespace orc;
+
+namespace {
+BgParams bg_from(const tgh_desc* d) {
+  BgParams p;
+  p.kind = d->background;
+  p.M = d->M;
+  p.a = d->a;
+  for (int i = 0; i < 3; ++i) p.c[i] = d->center[i];
+  if (d->background == TGH_BG_GAUGE_WAVE) {
+    p.A = d->wave_amp;
+    p.d = d->wave_len;
+  } else if (d->background == TGH_BG_SHIFTED_MINKOWSKI) {
+    p.A = d->wave_amp;
+    p.w = d->wave_len;
+  }
+  return p;
+}
+Case case_from(const tgh_desc* d) {
+  Case c;
+  c.bg = bg_from(d);
+  c.gauge = d->gauge;
+  c.ks_S[0] = 0; c.ks_S[1] = 0; c.ks_S[2] = 0; c.ks_S[3] = d->a;  // static hole, spin along z
+  c.ks_u[0] = 1; c.ks_u[1] = 0; c.ks_u[2] = 0; c.ks_u[3] = 0;
+  c.damping = d->damping;
+  c.g0 = d->gamma0;
+  c.g0_near = d->gamma0_near;
+  c.g0_far = d->gamma0_far;
+  c.g0_width = d->gamma0_width;
+  c.g2 = d->gamma2;
+  c.eps_ko = d->eps_ko;
+  c.interior = d->interior;
+  for (int i = 0; i < 3; ++i) c.center[i] = d->center[i];
+  c.r0 = d->r0;
+  c.r1 = d->r1;
+  c.rho_max = d->rho_max;
+  c.w_ramp = d->w_ramp;
+  c.rho_ramp = d->rho_ramp;
+  return c;
+}
+V10<double> v10(const double* p) {
+  V10<double> v;
+  for (int i = 0; i < NC; ++i) v[i] = p[i];
+  return v;
+}
+void mq_out(const MQ<double>& m, double* out) {
+  // alpha, beta[3], sqrtg, gamu[9] (row-major), gu4[16], guo[16], g4[16]
+  out[0] = m.alpha;
+  for (int i = 0; i < 3; ++i) out[1 + i] = m.beta[i];
+  out[4] = m.sqrtg;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[5 + 3 * i + j] = m.gamu[i][j];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      out[14 + 4 * a + b] = m.gu4[a][b];
+      out[30 + 4 * a + b] = m.guo[a][b];
+      out[46 + 4 * a + b] = m.g4[a][b];
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// ---- stencils ----
+int orc_derivative_weights(int q, int m, long long* num, long long* den, double* w) {
+  auto ws = rational_derivative_weights(q, m);
+  for (size_t i = 0; i < ws.size(); ++i) {
+    num[i] = (long long)ws[i].n;
+    den[i] = (long long)ws[i].d;
+    w[i] = (double)num[i] / (double)den[i];
+  }
+  return (int)ws.size();
+}
+int orc_dissipation_weights(int r, long long* num, long long* den, double* w) {
+  auto ws = rational_dissipation_weights(r);
+  for (size_t i = 0; i < ws.size(); ++i) {
+    num[i] = (long long)ws[i].n;
+    den[i] = (long long)ws[i].d;
+    w[i] = (double)num[i] / (double)den[i];
+  }
+  return (int)ws.size();
+}
+
+// ---- pointwise algebra ----
+void orc_metric_quantities(const double* h, double* out) { mq_out(metric_quantities(v10(h)), out); }
+
+// Forward-mode derivative of metric_quantities along direction dh (for the
+// chain-rule checks): out = d(alpha), d(beta)[3], d(sqrtg), d(gamu)[9].
+void orc_metric_quantities_jvp(const double* h, const double* dh, double* out) {
+  using D = Dual<double, 1>;
+  V10<D> hv;
+  for (int i = 0; i < NC; ++i) {
+    hv[i] = D(h[i]);
+    hv[i].d[0] = dh[i];
+  }
+  MQ<D> m = metric_quantities(hv);
+  out[0] = m.alpha.d[0];
+  for (int i = 0; i < 3; ++i) out[1 + i] = m.beta[i].d[0];
+  out[4] = m.sqrtg.d[0];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[5 + 3 * i + j] = m.gamu[i][j].d[0];
+}
+
+void orc_gh_node_source(const double* h, const double* dg, const double* Hl, const double* dHl,
+                        double g0, double g2, double* out) {
+  MQ<double> m = metric_quantities(v10(h));
+  double d[4][4][4], H2[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      H2[a][b] = dHl[4 * a + b];
+      for (int c = 0; c < 4; ++c) d[a][b][c] = dg[16 * a + 4 * b + c];
+    }
+  auto s = gh_node_source(m, d, Hl, H2, g0, g2);
+  for (int n = 0; n < NC; ++n) out[n] = s[n];
+}
+void orc_gh_node_source_lean(const double* h, const double* dth, const double* dh, const double* Hl,
+                             const double* dHl, double g0, double g2, double* out) {
+  MQ<double> m = metric_quantities(v10(h));
+  V10<double> d[3] = {v10(dh), v10(dh + 10), v10(dh + 20)};
+  double H2[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) H2[a][b] = dHl[4 * a + b];
+  auto s = gh_node_source_lean(m, v10(dth), d, Hl, H2, g0, g2);
+  for (int n = 0; n < NC; ++n) out[n] = s[n];
+}
+void orc_metric_divergences(const double* h, const double* dh, double* out) {
+  MQ<double> m = metric_quantities(v10(h));
+  V10<double> d[3] = {v10(dh), v10(dh + 10), v10(dh + 20)};
+  double divb, divA[3];
+  metric_divergences(m, d, divb, divA);
+  out[0] = divb;
+  for (int i = 0; i < 3; ++i) out[1 + i] = divA[i];
+}
+void orc_metric_derivatives(const double* h, const double* dh, double* out) {
+  MQ<double> m = metric_quantities(v10(h));
+  V10<double> d[3] = {v10(dh), v10(dh + 10), v10(dh + 20)};
+  MDer<double> md = metric_derivatives(m, d);
+  for (int i = 0; i < 3; ++i) out[i] = md.dalpha[i];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[3 + 3 * i + j] = md.dbeta[i][j];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) out[12 + 9 * i + 3 * j + k] = md.dA[i][j][k];
+}
+void orc_gh_node_rhs_expanded(const double* h, const double* P, const double* dh, const double* dP,
+                              const double* ddh, const double* Hl, const double* dHl, double g0,
+                              double g2, double* dth, double* dtP) {
+  V10<double> d[3] = {v10(dh), v10(dh + 10), v10(dh + 20)};
+  V10<double> e[3] = {v10(dP), v10(dP + 10), v10(dP + 20)};
+  V10<double> dd[6];
+  for (int n = 0; n < 6; ++n) dd[n] = v10(ddh + 10 * n);
+  double H2[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) H2[a][b] = dHl[4 * a + b];
+  V10<double> o1, o2;
+  gh_node_rhs_expanded(v10(h), v10(P), d, e, dd, Hl, H2, g0, g2, o1, o2);
+  for (int n = 0; n < NC; ++n) {
+    dth[n] = o1[n];
+    dtP[n] = o2[n];
+  }
+}
+void orc_gh_node_rhs_flux(const double* h, const double* P, const double* dh, const double* Hl,
+                          const double* dHl, double g0, double g2, double* dtg, double* F,
+                          double* msrc) {
+  V10<double> d[3] = {v10(dh), v10(dh + 10), v10(dh + 20)};
+  double H2[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) H2[a][b] = dHl[4 * a + b];
+  V10<double> o, f[3], m;
+  gh_node_rhs_flux(v10(h), v10(P), d, Hl, H2, g0, g2, o, f, m);
+  for (int n = 0; n < NC; ++n) {
+    dtg[n] = o[n];
+    msrc[n] = m[n];
+    for (int i = 0; i < 3; ++i) F[10 * i + n] = f[i][n];
+  }
+}
+void orc_matter_source(const double* h, double rho, const double* S, const double* Sij6, double* out) {
+  MQ<double> m = metric_quantities(v10(h));
+  double Sij[3][3] = {{Sij6[0], Sij6[1], Sij6[2]}, {Sij6[1], Sij6[3], Sij6[4]}, {Sij6[2], Sij6[4], Sij6[5]}};
+  auto o = matter_source(m, rho, S, Sij);
+  for (int n = 0; n < NC; ++n) out[n] = o[n];
+}
+void orc_ks_algebraic_source(double M, const double* S, const double* u, const double* h,
+                             const double* dth, const double* dh, double* Hl, double* dHl) {
+  V10<double> d[3] = {v10(dh), v10(dh + 10), v10(dh + 20)};
+  double H2[4][4];
+  ks_algebraic_source<double>(M, S, u, v10(h), v10(dth), d, Hl, H2);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) dHl[4 * a + b] = H2[a][b];
+}
+
+// ---- backgrounds ----
+void orc_dmetric(const tgh_desc* desc, double t, const double* x, double* g, double* dg) {
+  double G[4][4], DG[4][4][4];
+  dmetric(bg_from(desc), t, x[0], x[1], x[2], G, DG);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      g[4 * a + b] = G[a][b];
+      for (int c = 0; c < 4; ++c) dg[16 * a + 4 * b + c] = DG[a][b][c];
+    }
+}
+void orc_background_state(const tgh_desc* desc, double t, const double* x, double* h, double* P,
+                          double* dh) {
+  V10<double> hh, pp, d[3];
+  background_state(bg_from(desc), t, x, hh, pp, d);
+  for (int n = 0; n < NC; ++n) {
+    h[n] = hh[n];
+    P[n] = pp[n];
+    for (int i = 0; i < 3; ++i) dh[10 * i + n] = d[i][n];
+  }
+}
+void orc_gauge_source_grad(const tgh_desc* desc, double t, const double* x, double* Hl, double* dHl) {
+  double H2[4][4];
+  gauge_source_grad(bg_from(desc), t, x[0], x[1], x[2], Hl, H2);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) dHl[4 * a + b] = H2[a][b];
+}
+void orc_case_state(const tgh_desc* desc, double t, const double* x, double* out) {
+  Case c = case_from(desc);
+  case_state(c, t, x, out);
+}
+double orc_point_draw(uint64_t base, const double* x, int v) { return point_draw(base, x, v); }
+double orc_smoothstep(double s) { return smoothstep(s); }
+
+// ---- the mesh problem ----
+void* orc_problem_create(const tgh_desc* d) {
+  auto* p = new Problem();
+  p->mesh.q = d->q;
+  p->mesh.N = d->N;
+  for (int i = 0; i < 3; ++i) {
+    p->mesh.nb[i] = d->nblocks[i];
+    p->mesh.lo[i] = d->lo[i];
+    p->mesh.periodic[i] = d->periodic[i] != 0;
+  }
+  p->mesh.h = (d->hi[0] - d->lo[0]) / ((double)d->nblocks[0] * d->N);
+  p->cs = case_from(d);
+  p->setup();
+  return p;
+}
+void orc_problem_destroy(void* p) { delete (Problem*)p; }
+long orc_state_length(void* p) { return ((Problem*)p)->mesh.state_size(); }
+long orc_work_length(void* p) { return ((Problem*)p)->mesh.work_size(); }
+const double* orc_work(void* p) { return ((Problem*)p)->work.data(); }
+void orc_block_coords(void* p, int* out) {
+  auto& bl = ((Problem*)p)->mesh.blocks;
+  for (size_t b = 0; b < bl.size(); ++b)
+    for (int d = 0; d < 3; ++d) out[3 * b + d] = bl[b][d];
+}
+void orc_fill_exact(void* p, double* u, double t) { ((Problem*)p)->fill_exact(u, t); }
+void orc_add_noise(void* p, double* u, uint64_t base, double amp) { ((Problem*)p)->add_noise(u, base, amp); }
+void orc_prepare(void* p, const double* u, double t) { ((Problem*)p)->prepare(u, t); }
+void orc_rhs(void* p, double* du, double t, int nthreads) { ((Problem*)p)->rhs(du, t, nthreads); }
+void orc_gh_rhs(void* p, double* du, const double* u, double t, int nthreads) {
+  ((Problem*)p)->gh_rhs(du, u, t, nthreads);
+}
+void orc_host_rhs_at(void* p, long b, long i, long j, long k, double t, double* out) {
+  ((Problem*)p)->host_rhs_at(b, i, j, k, t, out);
+}
+double orc_rhs_scale(void* p, long b, long i, long j, long k) { return ((Problem*)p)->rhs_scale(b, i, j, k); }
+void orc_rk4_step(void* p, double* u, double t, double dt, int nthreads) {
+  ((Problem*)p)->rk4_step(u, t, dt, nthreads);
+}
+void orc_settle(void* p, double* u, double t) { ((Problem*)p)->settle(u, t); }
+void orc_stage_limit(void* p, double* u, double t) { ((Problem*)p)->stage_limit(u, t); }
+double orc_max_speed(void* p, const double* u) { return ((Problem*)p)->max_speed(u); }
+void orc_error_norms(void* p, const double* u, double t, double* l2, double* linf) {
+  ((Problem*)p)->error_norms(u, t, *l2, *linf);
+}
+int orc_is_primary(void* p, long b, long i, long j, long k) {
+  return ((Problem*)p)->mesh.is_primary(b, i, j, k) ? 1 : 0;
+}
+
+// The FP64 operation count of F at one interior point of the :none branch
+// (q, dissipation on iff eps_ko != 0, no gauge source): the streaming kernel
+// instantiated with the counting type on a prepared work array.
+double orc_count_rhs_flops(void* pp, long b, long i, long j, long k) {
+  Problem* p = (Problem*)pp;
+  double x[3];
+  p->mesh.position(b, i, j, k, x);
+  Cnt::ops = 0;
+  V10<Cnt> dth, dtP;
+  Cnt g0(damping_rate(p->cs, x)), g2(p->cs.g2), epsh(p->cs.eps_ko / p->mesh.h);
+  p->rhs_point_F<Cnt>(b, i, j, k, 0.0, g0, g2, epsh, p->cs.eps_ko != 0.0, dth, dtP);
+  return (double)Cnt::ops;
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+}

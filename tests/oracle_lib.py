@@ -1,0 +1,177 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is test infrastructure: it is the checker the product library is
+compared against, never the thing measured or shipped.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+
+from tgh_desc import TghDesc  # noqa: E402  (tests/ is on sys.path under pytest)
+
+_lib = None
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        D = C.c_double
+        Lg = C.c_long
+        desc = C.POINTER(TghDesc)
+        sig = {
+            "orc_derivative_weights": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]),
+            "orc_dissipation_weights": (C.c_int, [C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]),
+            "orc_metric_quantities": (None, [dp, dp]),
+            "orc_metric_quantities_jvp": (None, [dp, dp, dp]),
+            "orc_gh_node_source": (None, [dp, dp, dp, dp, D, D, dp]),
+            "orc_gh_node_source_lean": (None, [dp, dp, dp, dp, dp, D, D, dp]),
+            "orc_metric_divergences": (None, [dp, dp, dp]),
+            "orc_metric_derivatives": (None, [dp, dp, dp]),
+            "orc_gh_node_rhs_expanded": (None, [dp, dp, dp, dp, dp, dp, dp, D, D, dp, dp]),
+            "orc_gh_node_rhs_flux": (None, [dp, dp, dp, dp, dp, D, D, dp, dp, dp]),
+            "orc_matter_source": (None, [dp, D, dp, dp, dp]),
+            "orc_ks_algebraic_source": (None, [D, dp, dp, dp, dp, dp, dp, dp]),
+            "orc_dmetric": (None, [desc, D, dp, dp, dp]),
+            "orc_background_state": (None, [desc, D, dp, dp, dp, dp]),
+            "orc_gauge_source_grad": (None, [desc, D, dp, dp, dp]),
+            "orc_case_state": (None, [desc, D, dp, dp]),
+            "orc_point_draw": (D, [C.c_uint64, dp, C.c_int]),
+            "orc_smoothstep": (D, [D]),
+            "orc_problem_create": (P, [desc]),
+            "orc_problem_destroy": (None, [P]),
+            "orc_state_length": (Lg, [P]),
+            "orc_work_length": (Lg, [P]),
+            "orc_work": (C.POINTER(D), [P]),
+            "orc_block_coords": (None, [P, C.POINTER(C.c_int)]),
+            "orc_fill_exact": (None, [P, dp, D]),
+            "orc_add_noise": (None, [P, dp, C.c_uint64, D]),
+            "orc_prepare": (None, [P, dp, D]),
+            "orc_rhs": (None, [P, dp, D, C.c_int]),
+            "orc_gh_rhs": (None, [P, dp, dp, D, C.c_int]),
+            "orc_host_rhs_at": (None, [P, Lg, Lg, Lg, Lg, D, dp]),
+            "orc_rhs_scale": (D, [P, Lg, Lg, Lg, Lg]),
+            "orc_rk4_step": (None, [P, dp, D, D, C.c_int]),
+            "orc_settle": (None, [P, dp, D]),
+            "orc_stage_limit": (None, [P, dp, D]),
+            "orc_max_speed": (D, [P, dp]),
+            "orc_error_norms": (None, [P, dp, D, C.POINTER(D), C.POINTER(D)]),
+            "orc_is_primary": (C.c_int, [P, Lg, Lg, Lg, Lg]),
+            "orc_count_rhs_flops": (D, [P, Lg, Lg, Lg, Lg]),
+            "orc_max_threads": (C.c_int, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def weights(q, m):
+    n = q + 1
+    num = (C.c_longlong * n)()
+    den = (C.c_longlong * n)()
+    w = np.zeros(n)
+    lib().orc_derivative_weights(q, m, num, den, w)
+    return list(num), list(den), w
+
+
+def diss_weights(r):
+    n = 2 * r + 1
+    num = (C.c_longlong * n)()
+    den = (C.c_longlong * n)()
+    w = np.zeros(n)
+    lib().orc_dissipation_weights(r, num, den, w)
+    return list(num), list(den), w
+
+
+def metric_quantities(h):
+    out = np.zeros(62)
+    lib().orc_metric_quantities(np.ascontiguousarray(h, dtype=np.float64), out)
+    return dict(alpha=out[0], beta=out[1:4], sqrtg=out[4], gamu=out[5:14].reshape(3, 3),
+                gu4=out[14:30].reshape(4, 4), guo=out[30:46].reshape(4, 4), g4=out[46:62].reshape(4, 4))
+
+
+class OracleProblem:
+    """The oracle's uniform-forest problem for one case descriptor."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        self.p = lib().orc_problem_create(C.byref(desc))
+        self.n = lib().orc_state_length(self.p)
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            lib().orc_problem_destroy(self.p)
+            self.p = None
+
+    def zeros(self):
+        return np.zeros(self.n)
+
+    def block_coords(self):
+        nb = self.n // ((self.desc.N + 1) ** 3 * 20)
+        out = (C.c_int * (3 * nb))()
+        lib().orc_block_coords(self.p, out)
+        return np.array(list(out)).reshape(nb, 3)
+
+    def fill_exact(self, t):
+        u = self.zeros()
+        lib().orc_fill_exact(self.p, u, t)
+        return u
+
+    def add_noise(self, u, base, amp):
+        lib().orc_add_noise(self.p, u, base, amp)
+
+    def prepare(self, u, t):
+        lib().orc_prepare(self.p, u, t)
+
+    def work(self):
+        n = lib().orc_work_length(self.p)
+        ptr = lib().orc_work(self.p)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+    def gh_rhs(self, u, t, nthreads=1):
+        du = self.zeros()
+        lib().orc_gh_rhs(self.p, du, u, t, nthreads)
+        return du
+
+    def rk4_step(self, u, t, dt, nthreads=1):
+        lib().orc_rk4_step(self.p, u, t, dt, nthreads)
+
+    def settle(self, u, t):
+        lib().orc_settle(self.p, u, t)
+
+    def max_speed(self, u):
+        return lib().orc_max_speed(self.p, u)
+
+    def error_norms(self, u, t):
+        l2 = C.c_double()
+        li = C.c_double()
+        lib().orc_error_norms(self.p, u, t, C.byref(l2), C.byref(li))
+        return l2.value, li.value
+
+    def host_rhs_at(self, b, i, j, k, t):
+        out = np.zeros(20)
+        lib().orc_host_rhs_at(self.p, b, i, j, k, t, out)
+        return out
+
+    def rhs_scale(self, b, i, j, k):
+        return lib().orc_rhs_scale(self.p, b, i, j, k)
+
+    def count_flops(self, b, i, j, k):
+        return lib().orc_count_rhs_flops(self.p, b, i, j, k)

@@ -1,0 +1,329 @@
+"""Parity of the HIP path against the CPU oracle, through the C ABI.
+
+Every comparison is per component against the reference's rhs_scale
+(test/evolution_tests.jl:97-108): |GPU − oracle| / scale ≤ 1e-12, the bar of
+the reference's own kernel test (test/evolution_tests.jl:141-179). Sizes are
+ones the oracle finishes in seconds.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib
+import tgh_desc as TD
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+TOL = 1e-12
+
+
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a visible GPU")
+    return torch
+
+
+def rhs_scale_all(work, desc):
+    """rhs_scale at every owned point, from a work array (numpy, vectorised)."""
+    N, q = desc.N, desc.q
+    G = q // 2 + 1
+    S = N + 1 + 2 * G
+    nb = len(work) // (S ** 3 * 20)
+    W = work.reshape(nb, 20, S, S, S)  # (b, v, k, j, i)
+    _, _, w2 = oracle_lib.weights(q, 2)
+    r = q // 2
+    h = (desc.hi[0] - desc.lo[0]) / (desc.nblocks[0] * N)
+    own = slice(G, G + N + 1)
+    scale = np.abs(W[:, :, own, own, own]).max(axis=1)
+    for ax in (2, 3, 4):
+        acc = 0
+        for kk, wk in enumerate(w2):
+            o = kk - r
+            sl = [slice(None), slice(None), own, own, own]
+            sl[ax] = slice(G + o, G + o + N + 1)
+            acc = acc + wk * W[tuple(sl)]
+        scale = np.maximum(scale, (np.abs(acc) / (h * h)).max(axis=1))
+    scale = np.maximum(scale, 1.0)  # (b, k, j, i)
+    return scale
+
+
+def state_view(u, desc):
+    N = desc.N
+    P = N + 1
+    nb = len(u) // (P ** 3 * 20)
+    return u.reshape(nb, 20, P, P, P)
+
+
+def make_pair(tgh, desc):
+    torch = torch_cuda()
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    assert p.state_length == o.n
+    return torch, p, o
+
+
+def tgh_problem_from_desc(tgh, desc):
+    """A GHProblem-like handle over a raw descriptor (the C ABI directly)."""
+    import ctypes as C
+
+    class Handle:
+        pass
+
+    h = Handle()
+    h.desc = desc
+    ptr = C.c_void_p()
+    L = tgh._load()
+    rc = L.tgh_problem_create(C.byref(tgh.TghDesc.from_buffer_copy(desc)), C.byref(ptr))
+    tgh._check(rc)
+    h._ptr = ptr
+    h.ptr = ptr
+    h.state_length = int(L.tgh_state_length(ptr))
+    h.work_length = int(L.tgh_work_length(ptr))
+    h.nblocks = desc.nblocks[0] * desc.nblocks[1] * desc.nblocks[2]
+    h.case = None
+
+    def work():
+        import torch
+        w = torch.empty(h.work_length, dtype=torch.float64, device="cuda")
+        tgh._check(L.tgh_copy_work(ptr, C.c_void_p(w.data_ptr())))
+        return w
+
+    def coords():
+        out = (C.c_int32 * (3 * h.nblocks))()
+        tgh._check(L.tgh_block_coords(ptr, out))
+        return np.array(list(out)).reshape(h.nblocks, 3)
+
+    h.work = work
+    h.block_coords = coords
+    h.__del__ = lambda: L.tgh_problem_destroy(ptr)
+    return h
+
+
+def rel_err(a, b, scale_state, desc):
+    d = np.abs(state_view(a, desc) - state_view(b, desc))  # (b, v, k, j, i)
+    return float((d / scale_state[:, None]).max())
+
+
+def check_rhs(tgh, desc, t, noise=None, tol=TOL):
+    torch, p, o = make_pair(tgh, desc)
+    np.testing.assert_array_equal(p.block_coords(), o.block_coords())
+    u = o.fill_exact(t)
+    if noise is not None:
+        o.add_noise(u, noise[0], noise[1])
+    ud = torch.from_numpy(u.copy()).cuda()
+    du = torch.full_like(ud, float("nan"))
+    tgh.gh_rhs(du, ud, p, t)
+    ref = o.gh_rhs(u, t, NTHREADS)
+    work_o = o.work()
+    work_g = p.work().cpu().numpy()
+    scale = rhs_scale_all(work_o, desc)
+    assert np.isfinite(du.cpu().numpy()).all()
+    err = rel_err(du.cpu().numpy(), ref, scale, desc)
+    return err, work_o, work_g, p, o, u, ud
+
+
+GW_ROWS = [(2, 0.5, 8), (4, 0.0, 8), (4, 0.5, 8), (6, 0.5, 10), (8, 0.5, 10)]
+
+
+@pytest.mark.parametrize("q,eps,N", GW_ROWS)
+def test_gauge_wave_rhs_matches_oracle(tgh, q, eps, N):
+    desc = TD.gauge_wave_case(N=N, roots=2, q=q, eps_ko=eps, gamma0=1.0, gamma2=-0.5)
+    err, wo, wg, *_ = check_rhs(tgh, desc, 0.25)
+    print(f"gauge wave q={q} eps={eps}: max rel err {err:.3e}")
+    assert err <= TOL
+    # the periodic fill copies values: the work arrays agree bit for bit
+    np.testing.assert_array_equal(wg, wo)
+
+
+def test_minkowski_noise_rhs_matches_oracle(tgh):
+    desc = TD.minkowski_noise_case(N=16, q=4)
+    err, wo, wg, *_ = check_rhs(tgh, desc, 0.0, noise=(0x5EED20260917, 1e-6))
+    print(f"Minkowski + noise: max rel err {err:.3e}")
+    assert err <= TOL
+    np.testing.assert_array_equal(wg, wo)
+
+
+def test_minkowski_is_exactly_stationary(tgh):
+    torch = torch_cuda()
+    for q in (2, 4, 6):
+        N = 10 if q == 6 else 8
+        desc = TD.make_desc(q=q, N=N, nblocks=(2, 2, 2), gamma0=1.0, gamma2=-0.5, eps_ko=0.5)
+        p = tgh_problem_from_desc(tgh, desc)
+        u = torch.zeros(p.state_length, dtype=torch.float64, device="cuda")
+        du = torch.full_like(u, float("nan"))
+        tgh.gh_rhs(du, u, p, 3 / 7)
+        assert bool((du == 0).all())
+        tgh.gh_solve(p, u, (0.0, 0.5), dt=0.05)
+        assert bool((u == 0).all())
+
+
+def ks_small(gauge=TD.TGH_GAUGE_KERR_SCHILD, interior=TD.TGH_INTERIOR_DAMPED):
+    d = TD.kerr_schild_hole_case(N=16, nblocks=4, halfwidth=2.0, a=0.7, gauge=gauge)
+    d.interior = interior
+    return d
+
+
+@pytest.mark.parametrize("interior", [TD.TGH_INTERIOR_DAMPED, TD.TGH_INTERIOR_FROZEN])
+def test_kerr_schild_hole_rhs_matches_oracle(tgh, interior):
+    desc = ks_small(interior=interior)
+    err, wo, wg, p, o, u, ud = check_rhs(tgh, desc, 0.0)
+    print(f"KS hole a=0.7 interior={interior}: max rel err {err:.3e}")
+    assert err <= TOL
+    # Dirichlet ghosts: the analytic state at this t, to roundoff of a dual pass
+    denom = np.maximum(np.abs(wo), 1.0)
+    assert float((np.abs(wg - wo) / denom).max()) <= 1e-13
+
+
+def test_kerr_schild_noise_rhs_matches_oracle(tgh):
+    desc = ks_small()
+    err, *_ = check_rhs(tgh, desc, 0.0, noise=(12345, 1e-6))
+    assert err <= TOL
+
+
+def test_dirichlet_hook_is_the_exact_state_at_t(tgh):
+    """test/evolution_tests.jl:239-288: a moving background, a non-zero t."""
+    torch = torch_cuda()
+    desc = TD.make_desc(q=4, N=8, nblocks=(1, 1, 1), periodic=(0, 0, 0), lo=(0, 0, 0), hi=(2, 2, 2),
+                        background=TD.TGH_BG_GAUGE_WAVE, wave_amp=1 / 20, wave_len=2.0)
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    t = 1 / 3
+    u = o.fill_exact(t)
+    ud = torch.from_numpy(u.copy()).cuda()
+    tgh.prepare(p, ud, t)
+    o.prepare(u, t)
+    wg, wo = p.work().cpu().numpy(), o.work()
+    assert float(np.abs(wg - wo).max()) <= 1e-15
+    # and a hook at another time would not have been zero
+    o.prepare(o.fill_exact(0.0), 0.0)
+    assert float(np.abs(o.work() - wo).max()) > 1e-2
+
+
+def test_rk4_config1_ten_steps_match_oracle(tgh):
+    """The CPU-runnable workload: Minkowski + 1e-6 noise, one periodic 16^3
+    block, 10 RK4 steps at cfl 1/4."""
+    torch = torch_cuda()
+    desc = TD.minkowski_noise_case(N=16, q=4)
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = o.fill_exact(0.0)
+    o.add_noise(u, 0x5EED20260917, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    lam_o = o.max_speed(u)
+    lam_g = tgh.max_speed(p, ud, 0.0)
+    assert abs(lam_g - lam_o) <= 1e-14 * lam_o
+    dt = 0.25 * (1 / 16) / lam_o
+    for n in range(10):
+        o.rk4_step(u, n * dt, dt, NTHREADS)
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 10, 0))
+    diff = float(np.abs(ud.cpu().numpy() - u).max())
+    print(f"config 1, 10 RK4 steps: max |GPU - oracle| = {diff:.3e}")
+    assert diff <= 1e-12 * max(1.0, float(np.abs(u).max()))
+
+
+def test_rk4_kerr_schild_two_steps_match_oracle(tgh):
+    torch = torch_cuda()
+    desc = ks_small()
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = o.fill_exact(0.0)
+    ud = torch.from_numpy(u.copy()).cuda()
+    lam = o.max_speed(u)
+    assert abs(tgh.max_speed(p, ud, 0.0) - lam) <= 1e-13 * lam
+    dt = 0.25 * (1 / 16) / lam
+    for n in range(2):
+        o.rk4_step(u, n * dt, dt, NTHREADS)
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, 0))
+    g = ud.cpu().numpy()
+    scale = np.maximum(np.abs(u), 1.0)
+    diff = float((np.abs(g - u) / scale).max())
+    print(f"KS hole, 2 RK4 steps: max rel diff = {diff:.3e}")
+    assert diff <= 1e-12
+    l2g, lig = tgh.error_norms(p, ud, 2 * dt)
+    l2o, lio = o.error_norms(u, 2 * dt)
+    assert abs(l2g - l2o) <= 1e-9 * l2o and abs(lig - lio) <= 1e-9 * lio
+
+
+def test_fused_stepper_equals_verbs_and_graph_replay(tgh):
+    """The fused RK4 (stage update in the RHS epilogue) against RK4 written
+    with the verbs, and the hipGraph replay against the launch sequence."""
+    torch = torch_cuda()
+    desc = TD.gauge_wave_case(N=8, roots=2, q=4, eps_ko=0.5, gamma0=1.0, gamma2=0.0)
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u0 = torch.from_numpy(o.fill_exact(0.0)).cuda()
+    dt = 0.01
+    a = u0.clone()
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(a.data_ptr()), 0.0, dt, 3, 0))
+    b = u0.clone()
+    for rep in range(2):  # replays, with eager work in between
+        b.copy_(u0)
+        tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(b.data_ptr()), 0.0, dt, 3, 1))
+        torch.cuda.synchronize()
+        assert torch.equal(a, b), f"graph replay {rep} differs"
+    c = u0.clone()
+    k = [torch.empty_like(c) for _ in range(4)]
+    for n in range(3):
+        t = n * dt
+        tgh.gh_rhs(k[0], c, p, t)
+        y = c + (dt / 2) * k[0]
+        tgh.gh_rhs(k[1], y, p, t + dt / 2)
+        y = c + (dt / 2) * k[1]
+        tgh.gh_rhs(k[2], y, p, t + dt / 2)
+        y = c + dt * k[2]
+        tgh.gh_rhs(k[3], y, p, t + dt)
+        c = c + (dt / 6) * (k[0] + 2 * k[1] + 2 * k[2] + k[3])
+    assert float((a - c).abs().max()) <= 1e-14
+
+
+def test_rhs_is_pure_and_repeatable(tgh):
+    torch = torch_cuda()
+    desc = ks_small()
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = torch.from_numpy(o.fill_exact(0.0)).cuda()
+    u0 = u.clone()
+    d1, d2 = torch.empty_like(u), torch.empty_like(u)
+    tgh.gh_rhs(d1, u, p, 0.125)
+    tgh.gh_rhs(d2, u, p, 0.125)
+    assert torch.equal(u, u0)
+    assert torch.equal(d1, d2)
+    assert bool(torch.isfinite(d1).all())
+
+
+def test_initial_state_and_noise_match_oracle(tgh):
+    """initial_state! (the exact state with the core rule) and add_noise! (the
+    SplitMix64 draw of the position bits) on the device."""
+    torch = torch_cuda()
+    desc = ks_small()
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = torch.full((p.state_length,), float("nan"), dtype=torch.float64, device="cuda")
+    tgh.initial_state(u, p, 0.0)
+    ref = o.fill_exact(0.0)
+    g = u.cpu().numpy()
+    assert float((np.abs(g - ref) / np.maximum(np.abs(ref), 1.0)).max()) <= 1e-13
+    tgh.add_noise(u, p, 0x5EED20260917, 1e-6)
+    o.add_noise(ref, 0x5EED20260917, 1e-6)
+    assert float((np.abs(u.cpu().numpy() - ref) / np.maximum(np.abs(ref), 1.0)).max()) <= 1e-13
+
+
+def test_max_speed_finds_a_planted_maximum(tgh):
+    """The speed reduction visits every owned point: a shift planted at each
+    of several points is the maximum wherever it sits."""
+    torch = torch_cuda()
+    desc = TD.make_desc(q=4, N=8, nblocks=(2, 1, 1), gamma0=1.0)
+    p = tgh_problem_from_desc(tgh, desc)
+    P = 9
+    for (b, i, j, k) in ((0, 0, 0, 0), (1, 8, 8, 8), (0, 2, 3, 7), (1, 5, 0, 4)):
+        u = torch.zeros(p.state_length, dtype=torch.float64, device="cuda")
+        idx = ((b * 20 + 1) * P + k) * P * P + j * P + i  # h_tx
+        u[idx] = 0.3
+        lam = tgh.max_speed(p, u, 0.0)
+        o = oracle_lib.OracleProblem(desc)
+        assert abs(lam - o.max_speed(u.cpu().numpy())) <= 1e-14 * lam
+        assert lam > 1.9

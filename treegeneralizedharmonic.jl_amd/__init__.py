@@ -1,0 +1,488 @@
+"""MI355X drop-in for the right-hand side verbs of TreeGeneralizedHarmonic.jl.
+
+The host side above the C ABI of ``libtgh_mi355x.so`` (``include/tgh.h``), in
+Python because the reference's own host language (Julia) is not installed
+here. It mirrors the reference's interface for this path, with the same names,
+argument meaning and errors (``!`` dropped from the verbs):
+
+* cases:   ``minkowski_case``, ``gauge_wave_case``, ``kerr_schild_case``
+           (src/initialdata.jl:869-1326), each float-type-first;
+* forest:  ``gh_forest`` (src/initialdata.jl:1370-1400), a uniform brick;
+* problem: ``GHProblem(forest, case; q, backend=MI355X())``
+           (src/evolution.jl:1034-1149);
+* verbs:   ``gh_rhs``, ``prepare``, ``rhs``, ``settle``, ``stage_limit``,
+           ``step_limit``, ``max_speed``, ``initial_state``, ``evolved_sets``,
+           ``gh_dt``, ``gh_solve`` (src/evolution.jl:1327-1560,
+           src/stepping.jl:91-239).
+
+State vectors are ``torch`` float64 CUDA tensors of ``tgh_state_length``
+doubles in the reference's state layout; the library reads and writes them in
+place through their device pointers. There is no CPU fallback: without the
+built library or without a GPU every call raises.
+"""
+import ctypes as C
+import math
+import os
+
+__all__ = [
+    "ArgumentError", "TGHError", "MI355X", "Minkowski", "GaugeWave", "KerrSchild",
+    "ConstantDamping", "GaussianDamping", "Interior", "GHCase", "minkowski_case",
+    "gauge_wave_case", "kerr_schild_case", "gh_forest", "GHProblem", "gh_rhs", "prepare",
+    "rhs", "settle", "stage_limit", "step_limit", "max_speed", "initial_state",
+    "evolved_sets", "gh_dt", "gh_solve", "add_noise", "error_norms", "library_path",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libtgh_mi355x.so"
+
+
+def library_path():
+    return os.path.join(_HERE, _LIB_NAME)
+
+
+class ArgumentError(ValueError):
+    """The reference's ArgumentError: an argument it refuses, with its reason."""
+
+
+class TGHError(RuntimeError):
+    """A device-side failure reported by the library."""
+
+
+# --- the C ABI ---------------------------------------------------------------
+
+TGH_OK, TGH_EINVAL, TGH_EDEVICE, TGH_ENOTFINITE, TGH_EUNSUPPORTED = 0, 1, 2, 3, 4
+TGH_RK4_GRAPH = 1
+
+
+class TghDesc(C.Structure):
+    """``tgh_desc`` of include/tgh.h."""
+    _fields_ = [
+        ("float_bits", C.c_int32), ("q", C.c_int32), ("N", C.c_int32),
+        ("nblocks", C.c_int32 * 3), ("periodic", C.c_int32 * 3),
+        ("background", C.c_int32), ("gauge", C.c_int32), ("damping", C.c_int32),
+        ("interior", C.c_int32), ("reserved", C.c_int32),
+        ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+        ("M", C.c_double), ("a", C.c_double), ("center", C.c_double * 3),
+        ("wave_amp", C.c_double), ("wave_len", C.c_double),
+        ("gamma0", C.c_double), ("gamma0_near", C.c_double), ("gamma0_far", C.c_double),
+        ("gamma0_width", C.c_double), ("gamma2", C.c_double), ("eps_ko", C.c_double),
+        ("r0", C.c_double), ("r1", C.c_double), ("rho_max", C.c_double),
+        ("w_ramp", C.c_double), ("rho_ramp", C.c_double),
+    ]
+
+
+class TghMatter(C.Structure):
+    _fields_ = [("rho", C.c_void_p), ("S", C.c_void_p), ("Sij", C.c_void_p)]
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise TGHError(f"{path} is not built: run __graft_entry__.build() (hipcc for gfx950) first; "
+                       "this package has no CPU fallback")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    D = C.c_double
+    sig = {
+        "tgh_problem_create": (C.c_int, [C.POINTER(TghDesc), C.POINTER(P)]),
+        "tgh_problem_destroy": (C.c_int, [P]),
+        "tgh_state_length": (C.c_int64, [P]),
+        "tgh_work_length": (C.c_int64, [P]),
+        "tgh_block_coords": (C.c_int, [P, C.POINTER(C.c_int32)]),
+        "tgh_set_stream": (C.c_int, [P, P]),
+        "tgh_prepare": (C.c_int, [P, P, D]),
+        "tgh_rhs": (C.c_int, [P, P, D, C.POINTER(TghMatter)]),
+        "tgh_gh_rhs": (C.c_int, [P, P, P, D, C.POINTER(TghMatter)]),
+        "tgh_settle": (C.c_int, [P, P, D]),
+        "tgh_stage_limit": (C.c_int, [P, P, D]),
+        "tgh_step_limit": (C.c_int, [P, P, D]),
+        "tgh_max_speed": (C.c_int, [P, P, D, C.POINTER(D)]),
+        "tgh_initial_state": (C.c_int, [P, P, D]),
+        "tgh_add_noise": (C.c_int, [P, P, C.c_uint64, D]),
+        "tgh_rk4": (C.c_int, [P, P, D, D, C.c_int32, C.c_int32]),
+        "tgh_error_norms": (C.c_int, [P, P, D, C.POINTER(D), C.POINTER(D)]),
+        "tgh_copy_work": (C.c_int, [P, P]),
+        "tgh_profile": (C.c_int, [P, C.c_int32]),
+        "tgh_profile_read": (C.c_int, [P, C.POINTER(D)]),
+        "tgh_last_error": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc == TGH_OK:
+        return
+    msg = _load().tgh_last_error().decode()
+    if rc in (TGH_EINVAL, TGH_ENOTFINITE):
+        raise ArgumentError(msg)
+    if rc == TGH_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise TGHError(msg)
+
+
+# --- float types and backends ------------------------------------------------
+
+def _check_float(T):
+    import torch
+    if T in (None, "Float64", float, torch.float64):
+        return
+    raise ArgumentError(f"this backend computes in Float64 only (the horizon interior and the "
+                        f"constraint damping need it), got {T}")
+
+
+class MI355X:
+    """The backend object passed as ``backend=`` (the reference's
+    KernelAbstractions backend keyword, src/driver.jl:467-479)."""
+
+    def __init__(self, device=0):
+        import torch
+        if not torch.cuda.is_available():
+            raise TGHError("MI355X backend: no GPU is visible to this process")
+        self.device = torch.device("cuda", device)
+
+    def __repr__(self):
+        return f"MI355X({self.device.index})"
+
+
+# --- backgrounds, profiles, interior, case -----------------------------------
+
+class Minkowski:
+    kind = 0
+
+
+class GaugeWave:
+    """The AwA gauge wave H = 1 - A sin(2 pi (x - t)/d) (src/initialdata.jl:897-901)."""
+    kind = 1
+
+    def __init__(self, A, d):
+        self.A, self.d = float(A), float(d)
+
+
+class KerrSchild:
+    """KerrSchild(M, a): spin a along +z (SpacetimeMetrics, restated)."""
+    kind = 2
+
+    def __init__(self, M, a):
+        self.M, self.a = float(M), float(a)
+
+
+class ConstantDamping:
+    def __init__(self, g0):
+        self.g0 = float(g0)
+
+
+class GaussianDamping:
+    """far + (near - far) exp(-r^2 / 2 width^2) (src/gauge.jl:1122-1147)."""
+
+    def __init__(self, *, near, far, width, center=(0.0, 0.0, 0.0)):
+        if not float(width) > 0:
+            raise ArgumentError(f"the damping profile's width must be positive, got {width}")
+        if float(near) < 0 or float(far) < 0:
+            raise ArgumentError(f"both constraint-damping rates must satisfy γ0 ≥ 0, got near = "
+                                f"{near} and far = {far}")
+        self.near, self.far, self.width = float(near), float(far), float(width)
+        self.center = tuple(float(c) for c in center)
+
+
+class Interior:
+    """Interior(T; center, r_0, r_1, ρ_max, variant, w_ramp, ρ_ramp)
+    (src/interior.jl:178-238)."""
+    VARIANTS = {"damped": 1, "frozen": 2, "pasted": 3}
+
+    def __init__(self, T=None, *, center=(0.0, 0.0, 0.0), r_0, r_1, rho_max=0.0,
+                 variant="damped", w_ramp=0.5, rho_ramp=0.5):
+        _check_float(T)
+        if variant not in self.VARIANTS:
+            raise ArgumentError(f"the interior variant must be one of {tuple(self.VARIANTS)}, "
+                                f"got :{variant}")
+        if not r_0 > 0:
+            raise ArgumentError(f"the frozen core's radius must be positive, got r_0 = {r_0}")
+        if not r_1 > r_0:
+            raise ArgumentError(f"the layer needs r_1 > r_0, got r_0 = {r_0} and r_1 = {r_1}")
+        if not (0 < w_ramp <= 1 and 0 < rho_ramp <= 1):
+            raise ArgumentError(f"the ramp fractions must lie in (0, 1], but w_ramp = {w_ramp} "
+                                f"and ρ_ramp = {rho_ramp}")
+        if rho_max < 0:
+            raise ArgumentError(f"the relaxation rate must satisfy ρ_max ≥ 0, got {rho_max}")
+        self.center = tuple(float(c) for c in center)
+        self.r_0, self.r_1, self.rho_max = float(r_0), float(r_1), float(rho_max)
+        self.variant, self.w_ramp, self.rho_ramp = variant, float(w_ramp), float(rho_ramp)
+
+    def with_rho_max(self, rho_max):
+        return Interior(center=self.center, r_0=self.r_0, r_1=self.r_1, rho_max=rho_max,
+                        variant=self.variant, w_ramp=self.w_ramp, rho_ramp=self.rho_ramp)
+
+
+class GHCase:
+    """GHCase(T, background; box, periodic, ε_KO, γ0, γ2, interior, gauge_source)
+    (src/initialdata.jl:158-311). ``gauge_source`` is None (harmonic
+    backgrounds) or "algebraic" (the closed-form Kerr-Schild source)."""
+
+    def __init__(self, T, background, *, box, periodic, eps_ko, gamma0, gamma2,
+                 interior=None, gauge_source=None):
+        _check_float(T)
+        if not gamma2 > -1:
+            raise ArgumentError(f"γ2 must satisfy γ2 > −1, got {gamma2}")
+        if eps_ko < 0:
+            raise ArgumentError(f"the Kreiss–Oliger amplitude must be ≥ 0, got {eps_ko}")
+        self.background = background
+        self.box = tuple((float(a), float(b)) for a, b in box)
+        self.periodic = tuple(bool(p) for p in periodic)
+        self.eps_ko = float(eps_ko)
+        self.gamma0 = gamma0 if isinstance(gamma0, (ConstantDamping, GaussianDamping)) \
+            else ConstantDamping(gamma0)
+        self.gamma2 = float(gamma2)
+        self.interior = interior
+        harmonic = isinstance(background, (Minkowski, GaugeWave))
+        if gauge_source is None and not harmonic:
+            raise NotImplementedError(
+                "this background is not harmonic and would take the sampled gauge source Hsrc, "
+                "which this backend does not build yet: pass gauge_source=\"algebraic\"")
+        if gauge_source == "algebraic" and not isinstance(background, KerrSchild):
+            raise ArgumentError("the algebraic gauge source is the Kerr-Schild hole's own")
+        self.gauge_source = gauge_source
+
+
+def minkowski_case(T=None, *, L, eps_ko, gamma0, gamma2):
+    """Flat space on a periodic box of side L (src/initialdata.jl:869-871)."""
+    return GHCase(T, Minkowski(), box=[(0.0, L)] * 3, periodic=(True,) * 3, eps_ko=eps_ko,
+                  gamma0=gamma0, gamma2=gamma2)
+
+
+def gauge_wave_case(T=None, *, A=1 / 20, d=1.0, eps_ko, gamma0, gamma2):
+    """The AwA gauge wave on one periodic wavelength cubed (src/initialdata.jl:908-911)."""
+    return GHCase(T, GaugeWave(A, d), box=[(0.0, d)] * 3, periodic=(True,) * 3, eps_ko=eps_ko,
+                  gamma0=gamma0, gamma2=gamma2)
+
+
+def kerr_schild_case(T=None, *, M=1.0, a=0.0, halfwidth, r_0, r_1, rho_max=None, eps_ko=0.5,
+                     gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
+                     gauge_source="algebraic"):
+    """kerr_schild_case / hole_case (src/initialdata.jl:1044-1174,1274-1281) for a
+    hole at rest at the origin in a Dirichlet box. ρ_max defaults to the
+    physical rate 4/M the driver sets per chunk."""
+    damping = gamma0 if gamma0 is not None else GaussianDamping(near=1 / M, far=1 / (10 * M),
+                                                                width=3 * M)
+    inter = Interior(T, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
+                     variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp)
+    L = float(halfwidth)
+    return GHCase(T, KerrSchild(M, a), box=[(-L, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
+                  gamma0=damping, gamma2=gamma2, interior=inter, gauge_source=gauge_source)
+
+
+class Forest:
+    """A uniform brick of root blocks of N cells (gh_forest's roots)."""
+
+    def __init__(self, case, N, roots):
+        roots = (roots,) * 3 if isinstance(roots, int) else tuple(roots)
+        if len(roots) != 3 or any(r < 1 for r in roots):
+            raise ArgumentError(f"roots is one count of root blocks for every dimension or a "
+                                f"brick of three counts, got {roots!r}")
+        self.N, self.roots, self.case = int(N), roots, case
+
+
+def gh_forest(T=None, case=None, *, N, roots):
+    _check_float(T)
+    return Forest(case, N, roots)
+
+
+def _desc(forest, case, q):
+    d = TghDesc()
+    d.float_bits = 64
+    d.q, d.N = int(q), forest.N
+    d.nblocks[:] = list(forest.roots)
+    d.periodic[:] = [1 if p else 0 for p in case.periodic]
+    bg = case.background
+    d.background = bg.kind
+    d.lo[:] = [b[0] for b in case.box]
+    d.hi[:] = [b[1] for b in case.box]
+    if isinstance(bg, GaugeWave):
+        d.wave_amp, d.wave_len = bg.A, bg.d
+    if isinstance(bg, KerrSchild):
+        d.M, d.a = bg.M, bg.a
+    d.gauge = 2 if case.gauge_source == "algebraic" else 0
+    g0 = case.gamma0
+    if isinstance(g0, GaussianDamping):
+        d.damping = 1
+        d.gamma0_near, d.gamma0_far, d.gamma0_width = g0.near, g0.far, g0.width
+    else:
+        d.damping, d.gamma0 = 0, g0.g0
+    d.gamma2, d.eps_ko = case.gamma2, case.eps_ko
+    it = case.interior
+    if it is not None:
+        d.interior = Interior.VARIANTS[it.variant]
+        d.center[:] = list(it.center)
+        d.r0, d.r1, d.rho_max, d.w_ramp, d.rho_ramp = it.r_0, it.r_1, it.rho_max, it.w_ramp, it.rho_ramp
+    return d
+
+
+class GHProblem:
+    """GHProblem(forest, case; q, backend=MI355X()) (src/evolution.jl:1034-1149):
+    the device work arrays, the block table and the case, built once."""
+
+    def __init__(self, forest, case, *, q, backend=None):
+        if not (q >= 2 and q % 2 == 0):
+            raise ArgumentError(f"the finite-difference order must be even and at least 2, so "
+                                f"that the centered stencils have an integer half-width q/2, but q={q}")
+        self.backend = backend if backend is not None else MI355X()
+        self.forest, self.case, self.q = forest, case, int(q)
+        self.desc = _desc(forest, case, q)
+        self._ptr = C.c_void_p()
+        import torch
+        with torch.cuda.device(self.backend.device):
+            _check(_load().tgh_problem_create(C.byref(self.desc), C.byref(self._ptr)))
+        self.state_length = int(_load().tgh_state_length(self._ptr))
+        self.work_length = int(_load().tgh_work_length(self._ptr))
+        self.nblocks = forest.roots[0] * forest.roots[1] * forest.roots[2]
+
+    def __del__(self):
+        if getattr(self, "_ptr", None) and self._ptr.value:
+            _load().tgh_problem_destroy(self._ptr)
+            self._ptr = C.c_void_p()
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def statevector(self):
+        import torch
+        return torch.zeros(self.state_length, dtype=torch.float64, device=self.backend.device)
+
+    def block_coords(self):
+        out = (C.c_int32 * (3 * self.nblocks))()
+        _check(_load().tgh_block_coords(self._ptr, out))
+        return [tuple(out[3 * b:3 * b + 3]) for b in range(self.nblocks)]
+
+    def work(self):
+        import torch
+        w = torch.empty(self.work_length, dtype=torch.float64, device=self.backend.device)
+        _check(_load().tgh_copy_work(self._ptr, C.c_void_p(w.data_ptr())))
+        return w
+
+
+def _dev(x, p, name):
+    import torch
+    if not isinstance(x, torch.Tensor) or x.dtype != torch.float64 or not x.is_cuda:
+        raise ArgumentError(f"{name} must be a float64 CUDA tensor, got {type(x).__name__}")
+    if not x.is_contiguous() or x.numel() < p.state_length:
+        raise ArgumentError(f"{name} must be contiguous with at least {p.state_length} entries")
+    return C.c_void_p(x.data_ptr())
+
+
+def _matter(matter):
+    if matter is None:
+        return None
+    rho, S, Sij = matter
+    m = TghMatter(C.c_void_p(rho.data_ptr()), C.c_void_p(S.data_ptr()), C.c_void_p(Sij.data_ptr()))
+    return C.byref(m)
+
+
+def prepare(p, u, t):
+    """prepare!(p, u, t) -> p (src/evolution.jl:1349-1357)."""
+    _check(_load().tgh_prepare(p.ptr, _dev(u, p, "u"), float(t)))
+    return p
+
+
+def rhs(du, p, t, matter=None):
+    """rhs!(du, p, t; matter) (src/evolution.jl:1386-1390)."""
+    _check(_load().tgh_rhs(p.ptr, _dev(du, p, "du"), float(t), _matter(matter)))
+
+
+def gh_rhs(du, u, p, t, matter=None):
+    """gh_rhs!(du, u, p, t; matter) (src/evolution.jl:1327-1331)."""
+    _check(_load().tgh_gh_rhs(p.ptr, _dev(du, p, "du"), _dev(u, p, "u"), float(t), _matter(matter)))
+
+
+def settle(p, u, t):
+    """settle!(p, u, t) -> u (src/evolution.jl:1370-1374)."""
+    _check(_load().tgh_settle(p.ptr, _dev(u, p, "u"), float(t)))
+    return u
+
+
+def stage_limit(u, p, t):
+    """stage_limit!(u, p, t) (src/stepping.jl:91-95)."""
+    _check(_load().tgh_stage_limit(p.ptr, _dev(u, p, "u"), float(t)))
+
+
+def step_limit(u, p, t):
+    """step_limit!(u, p, t) (src/stepping.jl:97)."""
+    _check(_load().tgh_step_limit(p.ptr, _dev(u, p, "u"), float(t)))
+
+
+def max_speed(p, u, t):
+    """max_speed(p, u, t) -> λ, checked (src/evolution.jl:1539-1560)."""
+    out = C.c_double()
+    _check(_load().tgh_max_speed(p.ptr, _dev(u, p, "u"), float(t), C.byref(out)))
+    return out.value
+
+
+def initial_state(u, p, t):
+    """initial_state!(u, p, t) -> u (src/stepping.jl:110-115)."""
+    _check(_load().tgh_initial_state(p.ptr, _dev(u, p, "u"), float(t)))
+    return u
+
+
+def add_noise(u, p, base, amplitude):
+    """add_noise! with the SplitMix64 point draw (src/initialdata.jl:1961-2013)."""
+    _check(_load().tgh_add_noise(p.ptr, _dev(u, p, "u"), C.c_uint64(base), float(amplitude)))
+    return u
+
+
+def evolved_sets(p):
+    """evolved_sets(p) (src/stepping.jl:125): the one evolved field set."""
+    return (p,)
+
+
+def error_norms(p, u, t):
+    """(L2, L∞) of the masked error u − u_exact(t) over evolved points
+    (src/constraints.jl:311-396,655-671)."""
+    l2, li = C.c_double(), C.c_double()
+    _check(_load().tgh_error_norms(p.ptr, _dev(u, p, "u"), float(t), C.byref(l2), C.byref(li)))
+    return l2.value, li.value
+
+
+def gh_dt(p, u, *, cfl, t=0.0):
+    """gh_dt(p, u; cfl) = cfl · h_min / λ_max (src/evolution.jl:1511-1521)."""
+    lam = max_speed(p, u, t)
+    lo, hi = p.case.box[0]
+    h = (hi - lo) / (p.forest.roots[0] * p.forest.N)
+    return float(cfl) * h / lam
+
+
+def gh_solve(p, u, tspan, *, dt, graph=False):
+    """gh_solve(p, u, (t0, t1); dt) (src/stepping.jl:235-239): fixed-step RK4 with
+    nsteps = ⌈(t1 − t0)/dt⌉ and Δt = (t1 − t0)/nsteps, the limiters on every
+    stage, in place. Returns u (settled)."""
+    t0, t1 = float(tspan[0]), float(tspan[1])
+    nsteps = max(1, math.ceil((t1 - t0) / dt * (1 - 4 * 2.0 ** -52)))
+    step = (t1 - t0) / nsteps
+    case = getattr(p, "case", None)
+    if case is not None and case.interior is not None and case.interior.variant == "pasted":
+        # The fused stepper does not run the paste; use the verbs.
+        import torch
+        k = [torch.empty_like(u) for _ in range(4)]
+        for n in range(nsteps):
+            t = t0 + n * step
+            y = u.clone()
+            gh_rhs(k[0], u, p, t)
+            torch.add(u, k[0], alpha=step / 2, out=y); stage_limit(y, p, t + step / 2)
+            gh_rhs(k[1], y, p, t + step / 2)
+            torch.add(u, k[1], alpha=step / 2, out=y); stage_limit(y, p, t + step / 2)
+            gh_rhs(k[2], y, p, t + step / 2)
+            torch.add(u, k[2], alpha=step, out=y); stage_limit(y, p, t + step)
+            gh_rhs(k[3], y, p, t + step)
+            u.add_((k[0] + 2 * k[1] + 2 * k[2] + k[3]) * (step / 6)); step_limit(u, p, t + step)
+        return u
+    _check(_load().tgh_rk4(p.ptr, _dev(u, p, "u"), t0, step, nsteps, TGH_RK4_GRAPH if graph else 0))
+    return u

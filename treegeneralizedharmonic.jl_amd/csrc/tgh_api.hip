@@ -1,0 +1,583 @@
+// The C ABI (include/tgh.h): problem set-up, the verbs and the fused RK4
+// stepper, over the kernels of tgh_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tgh.h"
+#include "tgh_kernels.hpp"
+
+using namespace tgh;
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(TGH_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  uint64_t k = 0;
+  for (int bit = 0; bit < 21; ++bit) {
+    k |= (uint64_t)((x >> bit) & 1) << (3 * bit);
+    k |= (uint64_t)((y >> bit) & 1) << (3 * bit + 1);
+    k |= (uint64_t)((z >> bit) & 1) << (3 * bit + 2);
+  }
+  return k;
+}
+}  // namespace
+
+struct tgh_problem {
+  tgh_desc desc;
+  Geom geom;
+  CaseP cs;
+  int q = 4;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<int> bcoord_host;
+  int* d_bcoord = nullptr;
+  int* d_lookup = nullptr;
+  double* W[2] = {nullptr, nullptr};  // work arrays (prepare uses W[0])
+  double* acc = nullptr;              // RK4 accumulator, state layout
+  double* Hsrc = nullptr;             // sampled gauge source, state layout x 20
+  double* red = nullptr;              // per-block reduction partials (3 per block)
+  std::vector<double> red_host;
+  long state_len = 0, work_len = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  double graph_key[3] = {0, 0, 0};    // (dt, u pointer, t) the graph was captured for
+  // Benchmark timing: event pairs around each launch while profiling.
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_rhs, ev_fill;  // indices into ev_pool
+  size_t ev_used = 0;
+  double prof[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+dim3 grid_points(const tgh_problem* p) {
+  const int P3 = p->geom.P * p->geom.P * p->geom.P;
+  return dim3((P3 + 255) / 256, p->geom.nblocks);
+}
+dim3 grid_work(const tgh_problem* p) {
+  const int S3 = p->geom.S * p->geom.S * p->geom.S;
+  return dim3((S3 + 255) / 256, p->geom.nblocks);
+}
+
+int check_desc(const tgh_desc* d) {
+  if (!d) return fail(TGH_EINVAL, "desc is NULL");
+  if (d->float_bits != 64)
+    return fail(TGH_EINVAL, "this library computes in Float64 only (the horizon interior and the "
+                            "constraint damping need it); got a float type of " +
+                                std::to_string(d->float_bits) + " bits");
+  if (d->q < 2 || d->q > 8 || d->q % 2)
+    return fail(TGH_EINVAL, "the finite-difference order must be even and between 2 and 8, so that the "
+                            "centered stencils have an integer half-width q/2 and the ghost width "
+                            "G = q/2 + 1 covers them, but q=" + std::to_string(d->q));
+  const int G = d->q / 2 + 1;
+  if (d->N < 2 * G)
+    return fail(TGH_EINVAL, "a closed vertex block needs N >= 2G = " + std::to_string(2 * G) +
+                                " at q=" + std::to_string(d->q) + ", got N=" + std::to_string(d->N));
+  for (int i = 0; i < 3; ++i)
+    if (d->nblocks[i] < 1) return fail(TGH_EINVAL, "every dimension needs at least one block");
+  double h0 = (d->hi[0] - d->lo[0]) / ((double)d->nblocks[0] * d->N);
+  for (int i = 0; i < 3; ++i) {
+    double hi_ = (d->hi[i] - d->lo[i]) / ((double)d->nblocks[i] * d->N);
+    if (!(hi_ > 0) || std::fabs(hi_ - h0) > 1e-12 * h0)
+      return fail(TGH_EINVAL, "the blocks' cells must be cubic: the box widths over the brick of blocks "
+                              "give different spacings per axis");
+  }
+  if (d->background < 0 || d->background > 3) return fail(TGH_EINVAL, "unknown background");
+  if (d->gauge < 0 || d->gauge > 2) return fail(TGH_EINVAL, "unknown gauge source kind");
+  if (d->gauge == TGH_GAUGE_SAMPLED)
+    return fail(TGH_EUNSUPPORTED, "the sampled gauge source (Hsrc) is not built in this library yet; "
+                                  "use TGH_GAUGE_KERR_SCHILD, the closed-form source of a hole at rest");
+  if (d->gauge == TGH_GAUGE_KERR_SCHILD && d->background != TGH_BG_KERR_SCHILD)
+    return fail(TGH_EINVAL, "the closed-form Kerr-Schild gauge source reproduces H_a = -Gamma_a of a "
+                            "Kerr-Schild background only");
+  if (d->gauge == TGH_GAUGE_NONE &&
+      (d->background == TGH_BG_KERR_SCHILD || d->background == TGH_BG_SHIFTED_MINKOWSKI))
+    return fail(TGH_EINVAL, "this background is not harmonic, so it needs a gauge source: H_a = 0 "
+                            "would evolve it with a wrong gauge (src/gauge.jl isharmonic)");
+  if (d->interior < 0 || d->interior > 3) return fail(TGH_EINVAL, "unknown interior variant");
+  if (d->interior != TGH_INTERIOR_NONE) {
+    if (!(d->r0 > 0))
+      return fail(TGH_EINVAL, "the frozen core's radius must be positive, got r_0 = " + std::to_string(d->r0));
+    if (!(d->r1 > d->r0)) return fail(TGH_EINVAL, "the layer needs r_1 > r_0");
+    if (!(d->w_ramp > 0 && d->w_ramp <= 1 && d->rho_ramp > 0 && d->rho_ramp <= 1))
+      return fail(TGH_EINVAL, "the ramp fractions must lie in (0, 1]");
+    if (!(d->rho_max >= 0)) return fail(TGH_EINVAL, "the relaxation rate must satisfy rho_max >= 0");
+    // check_interior_radii (src/interior.jl:1104-1150): no stencil of an
+    // evolved point reaches the core.
+    if (d->r1 - d->r0 < 2.0 * (G + 1) * h0)
+      return fail(TGH_EINVAL, "r_1 - r_0 must be at least 2(G+1)h so that no stencil of an evolved point "
+                              "reaches the frozen core (check_interior_radii)");
+    if (d->background == TGH_BG_KERR_SCHILD && !(d->r0 > std::fabs(d->a)))
+      return fail(TGH_EINVAL, "r_0 must lie outside the Kerr-Schild ring singularity, r_0 > |a|");
+  }
+  if (d->damping == TGH_DAMPING_GAUSSIAN && !(d->gamma0_width > 0))
+    return fail(TGH_EINVAL, "the damping profile's width must be positive");
+  return TGH_OK;
+}
+
+void launch_fill(tgh_problem* p, double* W, double t) {
+  FillArgs a{p->geom, p->cs, t, W};
+  hipLaunchKernelGGL(k_fill, grid_work(p), dim3(256), 0, p->stream, a);
+}
+void launch_scatter(tgh_problem* p, const double* u, double* W, int to_work) {
+  CopyArgs a{p->geom, to_work ? u : W, to_work ? W : (double*)u, to_work};
+  hipLaunchKernelGGL(k_scatter, grid_points(p), dim3(256), 0, p->stream, a);
+}
+
+template <int Q, int GAUGE, bool HOLE, int EPI>
+void launch_rhs_q(tgh_problem* p, const RhsArgs& a) {
+  hipLaunchKernelGGL((k_rhs<Q, GAUGE, HOLE, EPI>), grid_points(p), dim3(256), 0, p->stream, a);
+}
+template <int Q, int EPI>
+void launch_rhs_g(tgh_problem* p, const RhsArgs& a) {
+  const bool hole = p->cs.interior != INT_NONE;
+  switch (p->cs.gauge) {
+    case GAUGE_KS:
+      if (hole) launch_rhs_q<Q, GAUGE_KS, true, EPI>(p, a);
+      else launch_rhs_q<Q, GAUGE_KS, false, EPI>(p, a);
+      break;
+    default:
+      if (hole) launch_rhs_q<Q, GAUGE_NONE, true, EPI>(p, a);
+      else launch_rhs_q<Q, GAUGE_NONE, false, EPI>(p, a);
+      break;
+  }
+}
+template <int EPI>
+void launch_rhs(tgh_problem* p, const RhsArgs& a) {
+  switch (p->q) {
+    case 2: launch_rhs_g<2, EPI>(p, a); break;
+    case 4: launch_rhs_g<4, EPI>(p, a); break;
+    case 6: launch_rhs_g<6, EPI>(p, a); break;
+    default: launch_rhs_g<8, EPI>(p, a); break;
+  }
+}
+
+RhsArgs rhs_args(tgh_problem* p, const double* W, double t, const tgh_matter* m) {
+  RhsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = p->geom;
+  a.c = p->cs;
+  a.t = t;
+  a.W = W;
+  a.Hsrc = p->Hsrc;
+  if (m && m->rho) {
+    a.m_rho = m->rho;
+    a.m_S = m->S;
+    a.m_Sij = m->Sij;
+  }
+  return a;
+}
+
+int sync(tgh_problem* p) {
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return TGH_OK;
+}
+
+int reduce(tgh_problem* p, const double* u, double t, int mode) {
+  ReduceArgs a{p->geom, p->cs, t, u, p->red, mode};
+  hipLaunchKernelGGL(k_reduce, grid_points(p), dim3(256), 0, p->stream, a);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(p->red_host.data(), p->red, sizeof(double) * p->red_host.size(),
+                         hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return TGH_OK;
+}
+
+int next_event(tgh_problem* p) {
+  if (p->ev_used == p->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    p->ev_pool.push_back(e);
+  }
+  return (int)p->ev_used++;
+}
+int mark(tgh_problem* p) {
+  int i = next_event(p);
+  if (i >= 0) (void)hipEventRecord(p->ev_pool[i], p->stream);
+  return i;
+}
+// Fold the recorded event pairs into the running totals (after a sync).
+void collect_profile(tgh_problem* p) {
+  for (auto& pr : p->ev_rhs) {
+    float ms = 0;
+    if (pr.first >= 0 && pr.second >= 0 &&
+        hipEventElapsedTime(&ms, p->ev_pool[pr.first], p->ev_pool[pr.second]) == hipSuccess) {
+      p->prof[0] += ms;
+      p->prof[1] += 1;
+    }
+  }
+  for (auto& pr : p->ev_fill) {
+    float ms = 0;
+    if (pr.first >= 0 && pr.second >= 0 &&
+        hipEventElapsedTime(&ms, p->ev_pool[pr.first], p->ev_pool[pr.second]) == hipSuccess) {
+      p->prof[2] += ms;
+      p->prof[3] += 1;
+    }
+  }
+  p->ev_rhs.clear();
+  p->ev_fill.clear();
+  p->ev_used = 0;
+}
+
+// One RK4 step on device arrays: u (state, in place), stage work arrays
+// W[0] (holding u_n with ghosts unfilled) and W[1].
+void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
+  const double c_acc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+  const double c_next[4] = {dt / 2, dt / 2, dt, 0.0};
+  const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
+  for (int s = 0; s < 4; ++s) {
+    double* Wc = p->W[s % 2];
+    double* Wn = p->W[(s + 1) % 2];
+    const bool prof = p->profiling;
+    int e0 = prof ? mark(p) : -1;
+    launch_fill(p, Wc, t_stage[s]);
+    int e1 = prof ? mark(p) : -1;
+    if (prof) p->ev_fill.push_back({e0, e1});
+    RhsArgs a = rhs_args(p, Wc, t_stage[s], nullptr);
+    a.u0 = u;
+    a.acc = p->acc;
+    a.Wn = Wn;
+    a.uout = u;
+    a.cacc = c_acc[s];
+    a.cnext = c_next[s];
+    a.stage = s + 1;
+    launch_rhs<EPI_RK4>(p, a);
+    if (prof) p->ev_rhs.push_back({e1, mark(p)});
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tgh_last_error(void) { return g_last_error.c_str(); }
+
+int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
+  if (!out) return fail(TGH_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = check_desc(d);
+  if (rc) return rc;
+  auto* p = new tgh_problem();
+  p->desc = *d;
+  p->q = d->q;
+  Geom& g = p->geom;
+  g.N = d->N;
+  g.G = d->q / 2 + 1;
+  g.S = d->N + 1 + 2 * g.G;
+  g.P = d->N + 1;
+  for (int i = 0; i < 3; ++i) {
+    g.nb[i] = d->nblocks[i];
+    g.periodic[i] = d->periodic[i] ? 1 : 0;
+    g.lo[i] = d->lo[i];
+  }
+  g.nblocks = d->nblocks[0] * d->nblocks[1] * d->nblocks[2];
+  g.h = (d->hi[0] - d->lo[0]) / ((double)d->nblocks[0] * d->N);
+  g.inv_h = 1.0 / g.h;
+  // Blocks in Morton order of the brick.
+  std::vector<std::array<int, 3>> bl;
+  for (int k = 0; k < g.nb[2]; ++k)
+    for (int j = 0; j < g.nb[1]; ++j)
+      for (int i = 0; i < g.nb[0]; ++i) bl.push_back({i, j, k});
+  std::stable_sort(bl.begin(), bl.end(), [](const auto& a, const auto& b) {
+    return morton3(a[0], a[1], a[2]) < morton3(b[0], b[1], b[2]);
+  });
+  p->bcoord_host.resize(3 * bl.size());
+  std::vector<int> lookup((size_t)g.nblocks);
+  for (size_t b = 0; b < bl.size(); ++b) {
+    for (int d2 = 0; d2 < 3; ++d2) p->bcoord_host[3 * b + d2] = bl[b][d2];
+    lookup[bl[b][0] + g.nb[0] * (bl[b][1] + g.nb[1] * bl[b][2])] = (int)b;
+  }
+  CaseP& c = p->cs;
+  std::memset(&c, 0, sizeof(c));
+  c.bg.kind = d->background;
+  c.bg.M = d->M;
+  c.bg.a = d->a;
+  for (int i = 0; i < 3; ++i) c.bg.c[i] = d->center[i];
+  c.bg.A = d->wave_amp;
+  c.bg.len = d->wave_len;
+  c.gauge = d->gauge;
+  c.damping = d->damping;
+  c.interior = d->interior;
+  c.has_diss = d->eps_ko != 0.0;
+  c.g0 = d->gamma0;
+  c.g0_near = d->gamma0_near;
+  c.g0_far = d->gamma0_far;
+  c.g0_width = d->gamma0_width;
+  c.g2 = d->gamma2;
+  c.eps_ko = d->eps_ko;
+  for (int i = 0; i < 3; ++i) c.center[i] = d->center[i];
+  c.r0 = d->r0;
+  c.r1 = d->r1;
+  c.rho_max = d->rho_max;
+  c.w_ramp = d->w_ramp;
+  c.rho_ramp = d->rho_ramp;
+
+  const long P3 = (long)g.P * g.P * g.P, S3 = (long)g.S * g.S * g.S;
+  p->state_len = P3 * 20 * g.nblocks;
+  p->work_len = S3 * 20 * g.nblocks;
+  const long wg_per_block = (P3 + 255) / 256;
+  p->red_host.assign(3 * (size_t)g.nblocks * wg_per_block, 0.0);
+  auto cleanup = [&](int code, const std::string& m) {
+    tgh_problem_destroy(p);
+    return fail(code, m);
+  };
+  hipError_t e;
+  if ((e = hipStreamCreate(&p->stream)) != hipSuccess) return cleanup(TGH_EDEVICE, hipGetErrorString(e));
+  p->own_stream = true;
+  if ((e = hipMalloc(&p->d_bcoord, sizeof(int) * p->bcoord_host.size())) != hipSuccess ||
+      (e = hipMalloc(&p->d_lookup, sizeof(int) * lookup.size())) != hipSuccess ||
+      (e = hipMalloc(&p->W[0], sizeof(double) * p->work_len)) != hipSuccess ||
+      (e = hipMalloc(&p->W[1], sizeof(double) * p->work_len)) != hipSuccess ||
+      (e = hipMalloc(&p->acc, sizeof(double) * p->state_len)) != hipSuccess ||
+      (e = hipMalloc(&p->red, sizeof(double) * p->red_host.size())) != hipSuccess)
+    return cleanup(TGH_EDEVICE, std::string("allocating the problem's device arrays: ") + hipGetErrorString(e));
+  if ((e = hipMemcpy(p->d_bcoord, p->bcoord_host.data(), sizeof(int) * p->bcoord_host.size(),
+                     hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->d_lookup, lookup.data(), sizeof(int) * lookup.size(), hipMemcpyHostToDevice)) !=
+          hipSuccess)
+    return cleanup(TGH_EDEVICE, hipGetErrorString(e));
+  // The ghosts of both work arrays start defined (a fill writes every one of them
+  // before any kernel reads it; this only keeps inspection deterministic).
+  if ((e = hipMemset(p->W[0], 0, sizeof(double) * p->work_len)) != hipSuccess ||
+      (e = hipMemset(p->W[1], 0, sizeof(double) * p->work_len)) != hipSuccess)
+    return cleanup(TGH_EDEVICE, hipGetErrorString(e));
+  g.bcoord = p->d_bcoord;
+  g.lookup = p->d_lookup;
+  *out = p;
+  return TGH_OK;
+}
+
+int tgh_problem_destroy(tgh_problem* p) {
+  if (!p) return TGH_OK;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
+  (void)hipFree(p->d_bcoord);
+  (void)hipFree(p->d_lookup);
+  (void)hipFree(p->W[0]);
+  (void)hipFree(p->W[1]);
+  (void)hipFree(p->acc);
+  (void)hipFree(p->Hsrc);
+  (void)hipFree(p->red);
+  if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return TGH_OK;
+}
+
+int64_t tgh_state_length(const tgh_problem* p) { return p ? p->state_len : -1; }
+
+int tgh_block_coords(const tgh_problem* p, int32_t* out) {
+  if (!p || !out) return fail(TGH_EINVAL, "NULL argument");
+  std::memcpy(out, p->bcoord_host.data(), sizeof(int) * p->bcoord_host.size());
+  return TGH_OK;
+}
+
+int tgh_set_stream(tgh_problem* p, void* stream) {
+  if (!p) return fail(TGH_EINVAL, "NULL problem");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  if (p->own_stream) HIP_TRY(hipStreamDestroy(p->stream));
+  if (stream) {
+    p->stream = (hipStream_t)stream;
+    p->own_stream = false;
+  } else {
+    HIP_TRY(hipStreamCreate(&p->stream));
+    p->own_stream = true;
+  }
+  return TGH_OK;
+}
+
+int tgh_prepare(tgh_problem* p, const double* u, double t) {
+  if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
+  launch_scatter(p, u, p->W[0], 1);
+  launch_fill(p, p->W[0], t);
+  return sync(p);
+}
+
+int tgh_rhs(tgh_problem* p, double* du, double t, const tgh_matter* matter) {
+  if (!p || !du) return fail(TGH_EINVAL, "NULL argument");
+  if (matter && matter->rho && (!matter->S || !matter->Sij))
+    return fail(TGH_EINVAL, "a matter adapter needs rho, S and Sij");
+  RhsArgs a = rhs_args(p, p->W[0], t, matter);
+  a.du = du;
+  launch_rhs<EPI_STORE>(p, a);
+  return sync(p);
+}
+
+int tgh_gh_rhs(tgh_problem* p, double* du, const double* u, double t, const tgh_matter* matter) {
+  int rc = tgh_prepare(p, u, t);
+  if (rc) return rc;
+  return tgh_rhs(p, du, t, matter);
+}
+
+int tgh_settle(tgh_problem* p, double* u, double t) {
+  int rc = tgh_prepare(p, u, t);
+  if (rc) return rc;
+  launch_scatter(p, u, p->W[0], 0);
+  return sync(p);
+}
+
+int tgh_stage_limit(tgh_problem* p, double* u, double t) {
+  if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
+  if (p->cs.interior != INT_PASTED) return TGH_OK;
+  PointArgs a{p->geom, p->cs, t, u, 0, 0.0, 2};
+  hipLaunchKernelGGL(k_pointwise, grid_points(p), dim3(256), 0, p->stream, a);
+  return sync(p);
+}
+int tgh_step_limit(tgh_problem* p, double* u, double t) { return tgh_stage_limit(p, u, t); }
+
+int tgh_initial_state(tgh_problem* p, double* u, double t) {
+  if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
+  PointArgs a{p->geom, p->cs, t, u, 0, 0.0, 0};
+  hipLaunchKernelGGL(k_pointwise, grid_points(p), dim3(256), 0, p->stream, a);
+  int rc = sync(p);
+  if (rc) return rc;
+  return tgh_stage_limit(p, u, t);
+}
+
+int tgh_add_noise(tgh_problem* p, double* u, uint64_t base, double amplitude) {
+  if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
+  if (!(amplitude >= 0)) return fail(TGH_EINVAL, "the noise amplitude must be >= 0");
+  PointArgs a{p->geom, p->cs, 0.0, u, base, amplitude, 1};
+  hipLaunchKernelGGL(k_pointwise, grid_points(p), dim3(256), 0, p->stream, a);
+  return sync(p);
+}
+
+int tgh_max_speed(tgh_problem* p, const double* u, double t, double* out) {
+  if (!p || !u || !out) return fail(TGH_EINVAL, "NULL argument");
+  int rc = reduce(p, u, t, 0);
+  if (rc) return rc;
+  double lam = 0.0, nbad = 0.0;
+  for (size_t w = 0; w < p->red_host.size() / 3; ++w) {
+    const double l = p->red_host[3 * w];
+    lam = (l > lam || std::isnan(l)) ? l : lam;
+    nbad += p->red_host[3 * w + 1];
+  }
+  if (nbad > 0)
+    return fail(TGH_ENOTFINITE, "the state at t = " + std::to_string(t) + " is not finite where it is "
+                "evolved: " + std::to_string((long)nbad) + " points outside the interior hold NaN or Inf");
+  if (!(std::isfinite(lam) && lam > 0))
+    return fail(TGH_ENOTFINITE, "the maximum characteristic speed alpha sqrt(tr gamma^ij) + |beta| over "
+                "the evolved points came out as " + std::to_string(lam) + ", so there is no CFL-limited "
+                "time step");
+  *out = lam;
+  return TGH_OK;
+}
+
+int tgh_error_norms(tgh_problem* p, const double* u, double t, double* l2, double* linf) {
+  if (!p || !u || !l2 || !linf) return fail(TGH_EINVAL, "NULL argument");
+  int rc = reduce(p, u, t, 1);
+  if (rc) return rc;
+  double num = 0, den = 0, li = 0;
+  for (size_t w = 0; w < p->red_host.size() / 3; ++w) {
+    num += p->red_host[3 * w];
+    den += p->red_host[3 * w + 1];
+    li = std::max(li, p->red_host[3 * w + 2]);
+  }
+  *l2 = den > 0 ? std::sqrt(num / den) : 0.0;
+  *linf = li;
+  return TGH_OK;
+}
+
+int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t flags) {
+  if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
+  if (nsteps < 0 || !(dt > 0)) return fail(TGH_EINVAL, "tgh_rk4 needs dt > 0 and nsteps >= 0");
+  if (p->cs.interior == INT_PASTED)
+    return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
+  if (nsteps == 0) return TGH_OK;
+  // The stage input of the first stage: u_n scattered into W[0].
+  launch_scatter(p, u, p->W[0], 1);
+  const bool graph = (flags & TGH_RK4_GRAPH) != 0;
+  if (!graph) {
+    for (int n = 0; n < nsteps; ++n) {
+      rk4_step_launches(p, u, t0 + n * dt, dt);
+      if (p->profiling && p->ev_used > 4096) {
+        int rc = sync(p);
+        if (rc) return rc;
+        collect_profile(p);
+      }
+    }
+    int rc = sync(p);
+    if (rc) return rc;
+    if (p->profiling) collect_profile(p);
+    return TGH_OK;
+  }
+  // A hipGraph of one step (4 x (fill + fused RHS)), replayed nsteps times.
+  // The stage times enter the kernels as arguments, so a graph is valid for
+  // one (u, dt, t) triple; with a static background the time only matters to
+  // the Dirichlet hook of a moving one, so the graph is re-captured whenever
+  // the step's time changes unless the background is static.
+  const bool static_bg = p->cs.bg.kind == BG_MINKOWSKI || p->cs.bg.kind == BG_KERR_SCHILD ||
+                         p->cs.bg.kind == BG_SHIFTED;
+  for (int n = 0; n < nsteps; ++n) {
+    const double t = t0 + n * dt;
+    const bool reuse = p->graph_exec && p->graph_key[0] == dt && p->graph_key[1] == (double)(uintptr_t)u &&
+                       (static_bg || p->graph_key[2] == t);
+    if (!reuse) {
+      if (p->graph_exec) {
+        HIP_TRY(hipGraphExecDestroy(p->graph_exec));
+        p->graph_exec = nullptr;
+      }
+      if (p->graph) {
+        HIP_TRY(hipGraphDestroy(p->graph));
+        p->graph = nullptr;
+      }
+      HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+      rk4_step_launches(p, u, t, dt);
+      HIP_TRY(hipStreamEndCapture(p->stream, &p->graph));
+      HIP_TRY(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
+      p->graph_key[0] = dt;
+      p->graph_key[1] = (double)(uintptr_t)u;
+      p->graph_key[2] = t;
+    }
+    HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
+  }
+  return sync(p);
+}
+
+int tgh_profile(tgh_problem* p, int32_t enable) {
+  if (!p) return fail(TGH_EINVAL, "NULL problem");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->profiling = enable != 0;
+  p->ev_rhs.clear();
+  p->ev_fill.clear();
+  p->ev_used = 0;
+  for (double& x : p->prof) x = 0;
+  return TGH_OK;
+}
+
+int tgh_profile_read(tgh_problem* p, double* out) {
+  if (!p || !out) return fail(TGH_EINVAL, "NULL argument");
+  for (int i = 0; i < 4; ++i) out[i] = p->prof[i];
+  return TGH_OK;
+}
+
+int64_t tgh_work_length(const tgh_problem* p) { return p ? p->work_len : -1; }
+
+int tgh_copy_work(tgh_problem* p, double* dst) {
+  if (!p || !dst) return fail(TGH_EINVAL, "NULL argument");
+  HIP_TRY(hipMemcpyAsync(dst, p->W[0], sizeof(double) * p->work_len, hipMemcpyDeviceToDevice, p->stream));
+  return sync(p);
+}
+
+}  // extern "C"

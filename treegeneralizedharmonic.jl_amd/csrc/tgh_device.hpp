@@ -1,0 +1,499 @@
+// Device-side pointwise algebra of the generalized-harmonic system, written for
+// gfx950: straight-line FP64 code over packed symmetric tensors, every loop
+// unrolled at compile time so that every index is a constant and every
+// intermediate a register.
+//
+// Reference (eschnett/TreeGeneralizedHarmonic.jl): src/pointwise.jl (packing
+// :83-112, metric_quantities :145-184, gh_node_source_lean :612-741,
+// metric_divergences :754-778, matter_source :667-698), src/gauge.jl
+// (KerrSchildSource :293-326, damping profiles :1094-1156), src/interior.jl
+// (smoothstep :117-120, profiles :567-590, core_position :615-625),
+// src/initialdata.jl (background_state :1674-1694, point_draw :2004-2013),
+// and SpacetimeMetrics' closed-form backgrounds.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TGH_HD __host__ __device__ __forceinline__
+
+namespace tgh {
+
+constexpr int NC = 10;
+
+// Packed slot of the symmetric pair (a, b): order tt,tx,ty,tz,xx,xy,xz,yy,yz,zz.
+TGH_HD constexpr int pidx(int a, int b) {
+  return a >= b ? b * 4 - (b * (b - 1)) / 2 + (a - b) : a * 4 - (a * (a - 1)) / 2 + (b - a);
+}
+// Row and column of packed slot n (row >= column).
+TGH_HD constexpr int prow(int n) { return n <= 3 ? n : n <= 6 ? n - 3 : n <= 8 ? n - 5 : 3; }
+TGH_HD constexpr int pcol(int n) { return n <= 3 ? 0 : n <= 6 ? 1 : n <= 8 ? 2 : 3; }
+// Packed slot of a spatial pair (i, j), i, j in 0..2: xx,xy,xz,yy,yz,zz.
+TGH_HD constexpr int sidx3(int i, int j) {
+  return i >= j ? j * 3 - (j * (j - 1)) / 2 + (i - j) : i * 3 - (i * (i - 1)) / 2 + (j - i);
+}
+TGH_HD constexpr double eta(int a, int b) { return a != b ? 0.0 : (a == 0 ? -1.0 : 1.0); }
+
+// ---------------------------------------------------------------------------
+// The coefficient set of a point, from the offset h = g - eta, by the 3+1
+// split: gamma_ij = delta_ij + h_ij, beta_i = h_ti, beta^i = gamma^ij beta_j,
+// alpha^2 = (1 - h_tt) + beta^i beta_i, and
+//   g^tt = -1/alpha^2, g^ti = beta^i/alpha^2, g^ij = gamma^ij - beta^i beta^j/alpha^2.
+// Every quantity is formed from the offsets (det(gamma) - 1 from the
+// elementary symmetric polynomials of h_ij), so alpha, beta and sqrt(gamma)
+// keep full relative precision for small h, as metric_quantities
+// (src/pointwise.jl:145-184) does by its 4x4 offset identities.
+struct Coef {
+  double gu[NC];   // packed g^{ab}
+  double alpha, sqrtg;
+  double beta[3];  // beta^i
+  double gamu[6];  // gamma^{ij}, packed xx,xy,xz,yy,yz,zz
+};
+
+TGH_HD void metric_quantities(const double h[NC], Coef& c) {
+  const double hxx = h[4], hxy = h[5], hxz = h[6], hyy = h[7], hyz = h[8], hzz = h[9];
+  // det(I + H) - 1 = e1 + e2 + e3 of the spatial offset H.
+  const double e1 = hxx + hyy + hzz;
+  const double e2 = (hxx * hyy - hxy * hxy) + (hxx * hzz - hxz * hxz) + (hyy * hzz - hyz * hyz);
+  const double e3 = hxx * (hyy * hzz - hyz * hyz) - hxy * (hxy * hzz - hyz * hxz) +
+                    hxz * (hxy * hyz - hyy * hxz);
+  const double detg = 1.0 + (e1 + (e2 + e3));
+  c.sqrtg = sqrt(detg);
+  const double gxx = 1.0 + hxx, gyy = 1.0 + hyy, gzz = 1.0 + hzz;
+  const double idet = 1.0 / detg;
+  double gi[6];
+  gi[0] = (gyy * gzz - hyz * hyz) * idet;
+  gi[1] = (hxz * hyz - hxy * gzz) * idet;
+  gi[2] = (hxy * hyz - hxz * gyy) * idet;
+  gi[3] = (gxx * gzz - hxz * hxz) * idet;
+  gi[4] = (hxy * hxz - gxx * hyz) * idet;
+  gi[5] = (gxx * gyy - hxy * hxy) * idet;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) c.gamu[n] = gi[n];
+  const double bl0 = h[1], bl1 = h[2], bl2 = h[3];
+  const double b0 = gi[0] * bl0 + gi[1] * bl1 + gi[2] * bl2;
+  const double b1 = gi[1] * bl0 + gi[3] * bl1 + gi[4] * bl2;
+  const double b2 = gi[2] * bl0 + gi[4] * bl1 + gi[5] * bl2;
+  c.beta[0] = b0;
+  c.beta[1] = b1;
+  c.beta[2] = b2;
+  const double a2 = (1.0 - h[0]) + (b0 * bl0 + b1 * bl1 + b2 * bl2);
+  c.alpha = sqrt(a2);
+  const double ia2 = 1.0 / a2;
+  c.gu[0] = -ia2;
+  c.gu[1] = b0 * ia2;
+  c.gu[2] = b1 * ia2;
+  c.gu[3] = b2 * ia2;
+  c.gu[4] = gi[0] - b0 * b0 * ia2;
+  c.gu[5] = gi[1] - b1 * b0 * ia2;
+  c.gu[6] = gi[2] - b2 * b0 * ia2;
+  c.gu[7] = gi[3] - b1 * b1 * ia2;
+  c.gu[8] = gi[4] - b2 * b1 * ia2;
+  c.gu[9] = gi[5] - b2 * b2 * ia2;
+}
+
+// ---------------------------------------------------------------------------
+// The source -alpha sqrt(gamma) (S0 + Z) (gh_node_source_lean,
+// src/pointwise.jl:612-741): D[a] is d_a g packed (a = t, x, y, z), Hl the
+// lowered gauge source and dH[c][a] = d_c H_a.
+template <bool HAS_H>
+TGH_HD void gh_source(const Coef& c, const double h[NC], const double D[4][NC], const double Hl[4],
+                      const double dH[4][4], double g0, double g2, double out[NC]) {
+  const double* Gp = c.gu;
+  double S[NC];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) S[n] = 0.0;
+  // (1) C2 + C2^T, one first index a at a time: E = G D_a, Cuu = E G,
+  //     C2[a][b] = sum_{mu,nu} Cuu[mu nu] D[mu](nu, b).
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double E[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int col = 0; col < 4; ++col) {
+        double s = Gp[pidx(r, 0)] * D[a][pidx(0, col)];
+#pragma unroll
+        for (int x = 1; x < 4; ++x) s += Gp[pidx(r, x)] * D[a][pidx(x, col)];
+        E[r][col] = s;
+      }
+    double Cuu[NC];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+      const int i = prow(n), j = pcol(n);
+      double s = E[i][0] * Gp[pidx(0, j)];
+#pragma unroll
+      for (int y = 1; y < 4; ++y) s += E[i][y] * Gp[pidx(y, j)];
+      Cuu[n] = s;
+    }
+    double C2[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) s += Cuu[pidx(m % 4, m / 4)] * D[m % 4][pidx(m / 4, b)];
+      C2[b] = s;
+    }
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+      const int i = prow(n), j = pcol(n);
+      if (i == a) S[n] += C2[j];
+      if (j == a) S[n] += C2[i];
+    }
+  }
+  // (2) the forty Christoffels Gamma^a_bc = G^{ax} (D_b[x,c] + D_c[x,b] - D_x[b,c]) / 2.
+  double Gam[4][NC];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
+    const int i = prow(n), j = pcol(n);
+    double low[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) low[x] = 0.5 * (D[i][pidx(x, j)] + D[j][pidx(x, i)] - D[x][n]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      double s = Gp[pidx(a, 0)] * low[0];
+#pragma unroll
+      for (int x = 1; x < 4; ++x) s += Gp[pidx(a, x)] * low[x];
+      Gam[a][n] = s;
+    }
+  }
+  double Gup[4];
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc) {
+    double s = 0.0;
+#pragma unroll
+    for (int n = 0; n < NC; ++n) s += (prow(n) == pcol(n) ? 1.0 : 2.0) * Gp[n] * Gam[cc][n];
+    Gup[cc] = s;
+  }
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
+    const int a = prow(n), b = pcol(n);
+    double t2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) t2 += Gam[m % 4][pidx(m / 4, a)] * Gam[m / 4][pidx(m % 4, b)];
+    const double t5 = Gup[0] * D[0][n] + Gup[1] * D[1][n] + Gup[2] * D[2][n] + Gup[3] * D[3][n];
+    double s = -2.0 * t2 - t5;
+    if (HAS_H) {
+      const double t4 = Gam[0][n] * Hl[0] + Gam[1][n] * Hl[1] + Gam[2][n] * Hl[2] + Gam[3][n] * Hl[3];
+      s += 2.0 * t4 - (dH[a][b] + dH[b][a]);
+    }
+    S[n] += s;
+  }
+  // (3) Gundlach–Pretorius damping, t_a = -alpha delta_a^t.
+  if (g0 != 0.0) {
+    double Cup[4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+      double gh = 0.0;
+      if (HAS_H) {
+        gh = Gp[pidx(cc, 0)] * Hl[0] + Gp[pidx(cc, 1)] * Hl[1] + Gp[pidx(cc, 2)] * Hl[2] +
+             Gp[pidx(cc, 3)] * Hl[3];
+      }
+      Cup[cc] = Gup[cc] + gh;
+    }
+    double Cl[4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+      double s = 0.0;
+#pragma unroll
+      for (int x = 0; x < 4; ++x) s += (eta(cc, x) + h[pidx(cc, x)]) * Cup[x];
+      Cl[cc] = s;
+    }
+    const double tC = -c.alpha * (Gp[0] * Cl[0] + Gp[1] * Cl[1] + Gp[2] * Cl[2] + Gp[3] * Cl[3]);
+    const double tl = -c.alpha;
+    const double k1 = 1.0 + g2;
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+      const int a = prow(n), b = pcol(n);
+      double d = 0.0;
+      if (a == 0) d += tl * Cl[b];
+      if (b == 0) d += tl * Cl[a];
+      S[n] += g0 * (d - k1 * (eta(a, b) + h[n]) * tC);
+    }
+  }
+  const double s = -(c.alpha * c.sqrtg);
+#pragma unroll
+  for (int n = 0; n < NC; ++n) out[n] = s * S[n];
+}
+
+// d_i beta^i and d_i (alpha sqrt(gamma) gamma^{ij}) (metric_divergences,
+// src/pointwise.jl:754-778), from rows t and i of d_i g^{ab} = -g^{ac}(d_i g_cd)g^{db}.
+TGH_HD void metric_divergences(const Coef& c, const double dh[3][NC], double& divb, double divA[3]) {
+  const double* Gp = c.gu;
+  double r1[3][4], ri[3][4];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double v0[4], vi[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      double s0 = 0.0, si = 0.0;
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        s0 += Gp[pidx(0, cc)] * dh[i][pidx(cc, d)];
+        si += Gp[pidx(i + 1, cc)] * dh[i][pidx(cc, d)];
+      }
+      v0[d] = s0;
+      vi[d] = si;
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      double s0 = 0.0, si = 0.0;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        s0 += v0[d] * Gp[pidx(d, b)];
+        si += vi[d] * Gp[pidx(d, b)];
+      }
+      r1[i][b] = -s0;
+      ri[i][b] = -si;
+    }
+  }
+  const double gutt = Gp[0];
+  divb = -((r1[0][1] + c.beta[0] * r1[0][0]) + (r1[1][2] + c.beta[1] * r1[1][0]) +
+           (r1[2][3] + c.beta[2] * r1[2][0])) / gutt;
+  const double a3 = c.alpha * c.alpha * c.alpha;
+  double da[3], ds[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    da[i] = 0.5 * a3 * r1[i][0];
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) s += c.gamu[sidx3(m % 3, m / 3)] * dh[i][pidx(m % 3 + 1, m / 3 + 1)];
+    ds[i] = 0.5 * c.sqrtg * s;
+  }
+  const double as = c.alpha * c.sqrtg;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double gik = c.gamu[sidx3(i, k)];
+      s += (da[i] * c.sqrtg + c.alpha * ds[i]) * gik +
+           as * (ri[i][k + 1] + c.beta[k] * r1[i][i + 1] + c.beta[i] * r1[i][k + 1] +
+                 c.beta[i] * c.beta[k] * r1[i][0]);
+    }
+    divA[k] = s;
+  }
+}
+
+// The algebraic Kerr-Schild gauge source of a hole at rest with spin a along z
+// (src/gauge.jl:293-326 with u = (1,0,0,0), S = (0,0,0,a)): H_a = -K w_a / D,
+// w_a = h_at, K = h_tt, s = a h_tz, D = M + sqrt(M^2 - s^2); dH[c][a] = d_c H_a.
+TGH_HD void ks_source(double M, double a, const double h[NC], const double D[4][NC], double Hl[4],
+                      double dH[4][4]) {
+  const double w[4] = {h[0], h[1], h[2], h[3]};
+  const double K = w[0];
+  const double sw = a * w[3];
+  const double rad = M * M - sw * sw;
+  const double root = rad > 0.0 ? sqrt(rad) : 0.0;
+  const double Dn = M + root;
+  const double iD = 1.0 / Dn;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) Hl[q] = (-K * iD) * w[q];
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc) {
+    const double dw0 = D[cc][0], dw3 = D[cc][3];
+    const double dK = dw0;
+    const double dD = root > 0.0 ? -sw * (a * dw3) / root : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double dwq = D[cc][q];
+      dH[cc][q] = -(dK * w[q] + K * dwq) * iD + K * w[q] * dD * (iD * iD);
+    }
+  }
+}
+
+// matter_source (src/pointwise.jl:667-698): 16 pi alpha sqrt(gamma) (T_ab - g_ab T/2).
+TGH_HD void matter_source(const Coef& c, const double h[NC], double rho, const double S[3],
+                          const double Sij[6], double out[NC]) {
+  const double pi = 3.14159265358979323846;
+  const double c8 = 8.0 * pi, c16 = 16.0 * pi;
+  const double* b = c.beta;
+  const double Sb0 = Sij[0] * b[0] + Sij[1] * b[1] + Sij[2] * b[2];
+  const double Sb1 = Sij[1] * b[0] + Sij[3] * b[1] + Sij[4] * b[2];
+  const double Sb2 = Sij[2] * b[0] + Sij[4] * b[1] + Sij[5] * b[2];
+  const double bS = b[0] * S[0] + b[1] * S[1] + b[2] * S[2];
+  const double Ttt = c.alpha * c.alpha * rho - 2.0 * (c.alpha * bS) + (b[0] * Sb0 + b[1] * Sb1 + b[2] * Sb2);
+  const double Tt[3] = {Sb0 - c.alpha * S[0], Sb1 - c.alpha * S[1], Sb2 - c.alpha * S[2]};
+  const double* gi = c.gamu;
+  const double trS = gi[0] * Sij[0] + gi[3] * Sij[3] + gi[5] * Sij[5] +
+                     2.0 * (gi[1] * Sij[1] + gi[2] * Sij[2] + gi[4] * Sij[4]);
+  const double tr8 = c8 * (trS - rho);
+  const double am = c.alpha * c.sqrtg;
+  out[0] = am * (c16 * Ttt - (h[0] - 1.0) * tr8);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out[1 + i] = am * (c16 * Tt[i] - h[1 + i] * tr8);
+#pragma unroll
+  for (int n = 4; n < NC; ++n) {
+    const int i = prow(n) - 1, j = pcol(n) - 1;
+    out[n] = am * (c16 * Sij[sidx3(i, j)] - (eta(i + 1, j + 1) + h[n]) * tr8);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Profiles.
+TGH_HD double smoothstep(double s) {  // src/interior.jl:117-120
+  double u = fmin(fmax(s, 0.0), 1.0);
+  double v = u * u * u * (10.0 + u * (6.0 * u - 15.0));
+  return fmin(fmax(v, 0.0), 1.0);
+}
+
+// ---------------------------------------------------------------------------
+// Forward-mode dual numbers for the analytic backgrounds' first derivatives.
+template <int K>
+struct Dual {
+  double v;
+  double d[K];
+};
+template <int K>
+TGH_HD Dual<K> dconst(double x) {
+  Dual<K> r;
+  r.v = x;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = 0.0;
+  return r;
+}
+template <int K> TGH_HD Dual<K> operator+(const Dual<K>& a, const Dual<K>& b) {
+  Dual<K> r; r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+template <int K> TGH_HD Dual<K> operator-(const Dual<K>& a, const Dual<K>& b) {
+  Dual<K> r; r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+template <int K> TGH_HD Dual<K> operator-(const Dual<K>& a) {
+  Dual<K> r; r.v = -a.v;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = -a.d[k];
+  return r;
+}
+template <int K> TGH_HD Dual<K> operator*(const Dual<K>& a, const Dual<K>& b) {
+  Dual<K> r; r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+template <int K> TGH_HD Dual<K> operator*(double a, const Dual<K>& b) {
+  Dual<K> r; r.v = a * b.v;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = a * b.d[k];
+  return r;
+}
+template <int K> TGH_HD Dual<K> operator+(double a, const Dual<K>& b) {
+  Dual<K> r = b; r.v = a + b.v; return r;
+}
+template <int K> TGH_HD Dual<K> operator/(const Dual<K>& a, const Dual<K>& b) {
+  Dual<K> r; const double ib = 1.0 / b.v; r.v = a.v * ib;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+  return r;
+}
+template <int K> TGH_HD Dual<K> dsqrt(const Dual<K>& a) {
+  Dual<K> r; r.v = sqrt(a.v); const double f = 0.5 / r.v;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * f;
+  return r;
+}
+template <int K> TGH_HD Dual<K> dsin(const Dual<K>& a) {
+  Dual<K> r; r.v = sin(a.v); const double cc = cos(a.v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * cc;
+  return r;
+}
+template <int K> TGH_HD Dual<K> dtanh(const Dual<K>& a) {
+  Dual<K> r; r.v = tanh(a.v); const double s = 1.0 - r.v * r.v;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * s;
+  return r;
+}
+
+enum { BG_MINKOWSKI = 0, BG_GAUGE_WAVE = 1, BG_KERR_SCHILD = 2, BG_SHIFTED = 3 };
+
+struct Bg {
+  int kind;
+  double M, a;
+  double c[3];
+  double A, len;  // gauge wave A, d; shifted Minkowski A, w
+};
+
+// The packed offset h = g - eta of a background at (t, x) with its four
+// derivatives (t, x, y, z) carried as dual partials.
+TGH_HD void bg_offset(const Bg& bg, double t, const double x[3], Dual<4> hd[NC]) {
+  typedef Dual<4> D;
+#pragma unroll
+  for (int n = 0; n < NC; ++n) hd[n] = dconst<4>(0.0);
+  if (bg.kind == BG_MINKOWSKI) return;
+  D X[4];
+  const double p[4] = {t, x[0], x[1], x[2]};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    X[k] = dconst<4>(p[k]);
+    X[k].d[k] = 1.0;
+  }
+  if (bg.kind == BG_GAUGE_WAVE) {
+    const double pi = 3.14159265358979323846;
+    D s = dsin((2.0 * pi / bg.len) * (X[1] - X[0]));
+    D Hm1 = -(bg.A * s);  // H - 1
+    hd[pidx(0, 0)] = -Hm1;
+    hd[pidx(1, 1)] = Hm1;
+    return;
+  }
+  if (bg.kind == BG_SHIFTED) {
+    D th = dtanh((1.0 / bg.len) * X[1]);
+    D pp = bg.A * (1.0 + (-(th * th)));
+    hd[pidx(0, 1)] = -pp;
+    hd[pidx(1, 1)] = -(pp * pp);
+    return;
+  }
+  // Kerr-Schild with spin a along +z about c:
+  //   r^4 - r^2 (rho^2 - a^2) - a^2 z^2 = 0, f = 2 M r^3 / (r^4 + a^2 z^2),
+  //   l = (1, (r x + a y)/(r^2 + a^2), (r y - a x)/(r^2 + a^2), z / r), h = f l l.
+  const double a = bg.a, M = bg.M;
+  D Xr = X[1] + dconst<4>(-bg.c[0]), Yr = X[2] + dconst<4>(-bg.c[1]), Zr = X[3] + dconst<4>(-bg.c[2]);
+  D rho2 = Xr * Xr + Yr * Yr + Zr * Zr;
+  D hb = 0.5 * (rho2 + dconst<4>(-a * a));
+  D r2 = hb + dsqrt(hb * hb + (a * a) * (Zr * Zr));
+  D r = dsqrt(r2);
+  D f = (2.0 * M) * (r * r2) / (r2 * r2 + (a * a) * (Zr * Zr));
+  D den = r2 + dconst<4>(a * a);
+  D l[4];
+  l[0] = dconst<4>(1.0);
+  l[1] = (r * Xr + a * Yr) / den;
+  l[2] = (r * Yr + (-a) * Xr) / den;
+  l[3] = Zr / r;
+#pragma unroll
+  for (int n = 0; n < NC; ++n) hd[n] = f * (l[prow(n)] * l[pcol(n)]);
+}
+
+// background_state (src/initialdata.jl:1674-1694): h and
+// Pi = (sqrt(gamma)/alpha)(d_t g - beta^i d_i g) at (t, x).
+TGH_HD void background_state(const Bg& bg, double t, const double x[3], double h[NC], double P[NC]) {
+  Dual<4> hd[NC];
+  bg_offset(bg, t, x, hd);
+#pragma unroll
+  for (int n = 0; n < NC; ++n) h[n] = hd[n].v;
+  Coef c;
+  metric_quantities(h, c);
+  const double s = c.sqrtg / c.alpha;
+#pragma unroll
+  for (int n = 0; n < NC; ++n)
+    P[n] = s * (hd[n].d[0] - c.beta[0] * hd[n].d[1] - c.beta[1] * hd[n].d[2] - c.beta[2] * hd[n].d[3]);
+}
+
+// SplitMix64 point draw in [0, 1) (src/initialdata.jl:2004-2013).
+TGH_HD double point_draw(uint64_t base, const double x[3], int v) {
+  uint64_t z = base;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint64_t cc = i < 3 ? (uint64_t)__double_as_longlong(x[i]) : (uint64_t)v;
+    z = (z ^ cc) + 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+  }
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+}  // namespace tgh
