@@ -353,3 +353,52 @@ def test_max_speed_finds_a_planted_maximum(tgh):
         o = oracle_lib.OracleProblem(desc)
         assert abs(lam - o.max_speed(u.cpu().numpy())) <= 1e-14 * lam
         assert lam > 1.9
+
+
+@pytest.mark.parametrize("interior", [TD.TGH_INTERIOR_DAMPED])
+def test_kerr_schild_sampled_gauge_source_matches_oracle(tgh, interior):
+    """The reference's default for a hole: H_a = −Γ_a sampled once
+    (src/gauge.jl:156-184) through the core rule; second derivatives of the
+    metric by jets on the device, by nested duals in the oracle."""
+    desc = ks_small(gauge=TD.TGH_GAUGE_SAMPLED, interior=interior)
+    err, *_ = check_rhs(tgh, desc, 0.0, noise=(777, 1e-6))
+    print(f"KS hole, sampled Hsrc: max rel err {err:.3e}")
+    assert err <= TOL
+
+
+def test_shifted_minkowski_sampled_source_through_a_dirichlet_face(tgh):
+    desc = TD.make_desc(q=4, N=8, nblocks=(2, 2, 2), periodic=(0, 1, 1), lo=(-2, -2, -2), hi=(2, 2, 2),
+                        background=TD.TGH_BG_SHIFTED_MINKOWSKI, gauge=TD.TGH_GAUGE_SAMPLED,
+                        wave_amp=0.5, wave_len=2.0, gamma0=1.0, gamma2=-0.5, eps_ko=0.5)
+    err, *_ = check_rhs(tgh, desc, 0.3)
+    assert err <= TOL
+
+
+def _noise_growth(tgh, eps):
+    """gh_noise_growth (test/evolution_cases.jl:167-184): 1e-8 noise on flat
+    space, q = 4, one 8^3 block, 1000 RK4 steps at cfl 1/4."""
+    torch = torch_cuda()
+    desc = TD.make_desc(q=4, N=8, nblocks=(1, 1, 1), gamma0=1.0, gamma2=0.0, eps_ko=eps)
+    p = tgh_problem_from_desc(tgh, desc)
+    u = torch.zeros(p.state_length, dtype=torch.float64, device="cuda")
+    tgh.add_noise(u, p, 20260917, 1e-8)
+    lam = tgh.max_speed(p, u, 0.0)
+    dt = 0.25 * (1 / 8) / lam
+    l2_0, li_0 = tgh.error_norms(p, u, 0.0)
+    tgh.gh_solve(p, u, (0.0, 1000 * dt), dt=dt, graph=True)
+    l2_1, li_1 = tgh.error_norms(p, u, 1000 * dt)
+    return 1000 * dt, l2_1 / l2_0, li_1 / li_0, li_1, bool(torch.isfinite(u).all())
+
+
+def test_white_noise_stays_bounded_with_dissipation(tgh):
+    """test/noise_tests.jl:23-48: with eps_KO = 1/2 the L2 ratio < 2, Linf < 5."""
+    t_end, r2, ri, _, fin = _noise_growth(tgh, 0.5)
+    print(f"eps=0.5: t_end={t_end:.2f} l2 x{r2:.3f} linf x{ri:.3f}")
+    assert fin and t_end > 15 and r2 < 2 and ri < 5
+
+
+def test_without_dissipation_the_noise_grows(tgh):
+    """test/noise_tests.jl:50-62: the negative control, L2 grows by more than 2."""
+    t_end, r2, ri, li, fin = _noise_growth(tgh, 0.0)
+    print(f"eps=0: t_end={t_end:.2f} l2 x{r2:.3f} linf x{ri:.3f}")
+    assert fin and r2 > 2 and li < 1e-5

@@ -244,11 +244,9 @@ class GHCase:
             else ConstantDamping(gamma0)
         self.gamma2 = float(gamma2)
         self.interior = interior
-        harmonic = isinstance(background, (Minkowski, GaugeWave))
-        if gauge_source is None and not harmonic:
-            raise NotImplementedError(
-                "this background is not harmonic and would take the sampled gauge source Hsrc, "
-                "which this backend does not build yet: pass gauge_source=\"algebraic\"")
+        if gauge_source not in (None, "algebraic"):
+            raise ArgumentError(f"gauge_source is nothing (sampled where the background is not "
+                                f"harmonic) or :algebraic, got {gauge_source!r}")
         if gauge_source == "algebraic" and not isinstance(background, KerrSchild):
             raise ArgumentError("the algebraic gauge source is the Kerr-Schild hole's own")
         self.gauge_source = gauge_source
@@ -268,10 +266,12 @@ def gauge_wave_case(T=None, *, A=1 / 20, d=1.0, eps_ko, gamma0, gamma2):
 
 def kerr_schild_case(T=None, *, M=1.0, a=0.0, halfwidth, r_0, r_1, rho_max=None, eps_ko=0.5,
                      gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
-                     gauge_source="algebraic"):
+                     gauge_source=None):
     """kerr_schild_case / hole_case (src/initialdata.jl:1044-1174,1274-1281) for a
     hole at rest at the origin in a Dirichlet box. ρ_max defaults to the
-    physical rate 4/M the driver sets per chunk."""
+    physical rate 4/M the driver sets per chunk. gauge_source=None samples
+    H_a = −Γ_a of the background once (the reference's default); "algebraic" is
+    the closed-form KerrSchildSource evaluated in the kernel."""
     damping = gamma0 if gamma0 is not None else GaussianDamping(near=1 / M, far=1 / (10 * M),
                                                                 width=3 * M)
     inter = Interior(T, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
@@ -311,7 +311,8 @@ def _desc(forest, case, q):
         d.wave_amp, d.wave_len = bg.A, bg.d
     if isinstance(bg, KerrSchild):
         d.M, d.a = bg.M, bg.a
-    d.gauge = 2 if case.gauge_source == "algebraic" else 0
+    harmonic = isinstance(bg, (Minkowski, GaugeWave))
+    d.gauge = 2 if case.gauge_source == "algebraic" else (0 if harmonic else 1)
     g0 = case.gamma0
     if isinstance(g0, GaussianDamping):
         d.damping = 1

@@ -103,9 +103,9 @@ int check_desc(const tgh_desc* d) {
   }
   if (d->background < 0 || d->background > 3) return fail(TGH_EINVAL, "unknown background");
   if (d->gauge < 0 || d->gauge > 2) return fail(TGH_EINVAL, "unknown gauge source kind");
-  if (d->gauge == TGH_GAUGE_SAMPLED)
-    return fail(TGH_EUNSUPPORTED, "the sampled gauge source (Hsrc) is not built in this library yet; "
-                                  "use TGH_GAUGE_KERR_SCHILD, the closed-form source of a hole at rest");
+  if (d->gauge == TGH_GAUGE_SAMPLED && d->background == TGH_BG_GAUGE_WAVE)
+    return fail(TGH_EINVAL, "a time-dependent gauge source is not supported: the gauge wave is harmonic "
+                            "(H = 0) and takes no sampled source (src/gauge.jl isstatic, GHCase)");
   if (d->gauge == TGH_GAUGE_KERR_SCHILD && d->background != TGH_BG_KERR_SCHILD)
     return fail(TGH_EINVAL, "the closed-form Kerr-Schild gauge source reproduces H_a = -Gamma_a of a "
                             "Kerr-Schild background only");
@@ -154,6 +154,10 @@ void launch_rhs_g(tgh_problem* p, const RhsArgs& a) {
     case GAUGE_KS:
       if (hole) launch_rhs_q<Q, GAUGE_KS, true, EPI>(p, a);
       else launch_rhs_q<Q, GAUGE_KS, false, EPI>(p, a);
+      break;
+    case GAUGE_SAMPLED:
+      if (hole) launch_rhs_q<Q, GAUGE_SAMPLED, true, EPI>(p, a);
+      else launch_rhs_q<Q, GAUGE_SAMPLED, false, EPI>(p, a);
       break;
     default:
       if (hole) launch_rhs_q<Q, GAUGE_NONE, true, EPI>(p, a);
@@ -363,6 +367,16 @@ int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
     return cleanup(TGH_EDEVICE, hipGetErrorString(e));
   g.bcoord = p->d_bcoord;
   g.lookup = p->d_lookup;
+  if (c.gauge == GAUGE_SAMPLED) {
+    // The sample is taken once, here, at t = 0: the background is static (a
+    // moving non-harmonic background is refused, src/initialdata.jl GHCase).
+    if ((e = hipMalloc(&p->Hsrc, sizeof(double) * p->state_len)) != hipSuccess)
+      return cleanup(TGH_EDEVICE, hipGetErrorString(e));
+    SampleArgs sa{g, c, 0.0, p->Hsrc};
+    hipLaunchKernelGGL(k_sample_gauge, grid_points(p), dim3(256), 0, p->stream, sa);
+    if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(p->stream)) != hipSuccess)
+      return cleanup(TGH_EDEVICE, std::string("sampling the gauge source: ") + hipGetErrorString(e));
+  }
   *out = p;
   return TGH_OK;
 }

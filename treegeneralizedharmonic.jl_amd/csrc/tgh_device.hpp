@@ -418,31 +418,126 @@ struct Bg {
   double A, len;  // gauge wave A, d; shifted Minkowski A, w
 };
 
-// The packed offset h = g - eta of a background at (t, x) with its four
-// derivatives (t, x, y, z) carried as dual partials.
-TGH_HD void bg_offset(const Bg& bg, double t, const double x[3], Dual<4> hd[NC]) {
-  typedef Dual<4> D;
+// Second-order jets (value, gradient and packed Hessian in the four
+// coordinates t, x, y, z) for the sampled gauge source, which needs the
+// metric's second derivatives.
+struct Jet2 {
+  double v;
+  double d[4];
+  double dd[NC];  // d_a d_b, packed as the metric's components
+};
+TGH_HD Jet2 jconst(double x) {
+  Jet2 r;
+  r.v = x;
 #pragma unroll
-  for (int n = 0; n < NC; ++n) hd[n] = dconst<4>(0.0);
+  for (int k = 0; k < 4; ++k) r.d[k] = 0.0;
+#pragma unroll
+  for (int n = 0; n < NC; ++n) r.dd[n] = 0.0;
+  return r;
+}
+TGH_HD Jet2 operator+(const Jet2& a, const Jet2& b) {
+  Jet2 r; r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] + b.d[k];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) r.dd[n] = a.dd[n] + b.dd[n];
+  return r;
+}
+TGH_HD Jet2 operator-(const Jet2& a, const Jet2& b) {
+  Jet2 r; r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] - b.d[k];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) r.dd[n] = a.dd[n] - b.dd[n];
+  return r;
+}
+TGH_HD Jet2 operator-(const Jet2& a) {
+  Jet2 r; r.v = -a.v;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.d[k] = -a.d[k];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) r.dd[n] = -a.dd[n];
+  return r;
+}
+TGH_HD Jet2 operator*(const Jet2& a, const Jet2& b) {
+  Jet2 r; r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
+    const int i = prow(n), j = pcol(n);
+    r.dd[n] = a.dd[n] * b.v + a.v * b.dd[n] + a.d[i] * b.d[j] + a.d[j] * b.d[i];
+  }
+  return r;
+}
+TGH_HD Jet2 operator*(double a, const Jet2& b) {
+  Jet2 r; r.v = a * b.v;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.d[k] = a * b.d[k];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) r.dd[n] = a * b.dd[n];
+  return r;
+}
+TGH_HD Jet2 operator+(double a, const Jet2& b) { Jet2 r = b; r.v = a + b.v; return r; }
+// A unary function f applied to a jet: f(a), f'(a), f''(a) given.
+TGH_HD Jet2 jchain(const Jet2& a, double f0, double f1, double f2) {
+  Jet2 r; r.v = f0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.d[k] = f1 * a.d[k];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
+    const int i = prow(n), j = pcol(n);
+    r.dd[n] = f1 * a.dd[n] + f2 * a.d[i] * a.d[j];
+  }
+  return r;
+}
+TGH_HD Jet2 operator/(const Jet2& a, const Jet2& b) {
+  const double ib = 1.0 / b.v;
+  return a * jchain(b, ib, -ib * ib, 2.0 * ib * ib * ib);
+}
+TGH_HD Jet2 dsqrt(const Jet2& a) {
+  const double s = sqrt(a.v);
+  return jchain(a, s, 0.5 / s, -0.25 / (s * a.v));
+}
+TGH_HD Jet2 dsin(const Jet2& a) {
+  const double sn = sin(a.v), cs = cos(a.v);
+  return jchain(a, sn, cs, -sn);
+}
+TGH_HD Jet2 dtanh(const Jet2& a) {
+  const double th = tanh(a.v), s = 1.0 - th * th;
+  return jchain(a, th, s, -2.0 * th * s);
+}
+template <class J> TGH_HD J jet_const(double x);
+template <> TGH_HD Dual<4> jet_const<Dual<4>>(double x) { return dconst<4>(x); }
+template <> TGH_HD Jet2 jet_const<Jet2>(double x) { return jconst(x); }
+TGH_HD void jet_seed(Dual<4>& x, int k) { x.d[k] = 1.0; }
+TGH_HD void jet_seed(Jet2& x, int k) { x.d[k] = 1.0; }
+
+// The packed offset h = g - eta of a background at (t, x), carried as jets in
+// (t, x, y, z): Dual<4> for first derivatives, Jet2 for second.
+template <class J>
+TGH_HD void bg_offset(const Bg& bg, double t, const double x[3], J hd[NC]) {
+#pragma unroll
+  for (int n = 0; n < NC; ++n) hd[n] = jet_const<J>(0.0);
   if (bg.kind == BG_MINKOWSKI) return;
-  D X[4];
+  J X[4];
   const double p[4] = {t, x[0], x[1], x[2]};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    X[k] = dconst<4>(p[k]);
-    X[k].d[k] = 1.0;
+    X[k] = jet_const<J>(p[k]);
+    jet_seed(X[k], k);
   }
   if (bg.kind == BG_GAUGE_WAVE) {
     const double pi = 3.14159265358979323846;
-    D s = dsin((2.0 * pi / bg.len) * (X[1] - X[0]));
-    D Hm1 = -(bg.A * s);  // H - 1
+    J s = dsin((2.0 * pi / bg.len) * (X[1] - X[0]));
+    J Hm1 = -(bg.A * s);  // H - 1
     hd[pidx(0, 0)] = -Hm1;
     hd[pidx(1, 1)] = Hm1;
     return;
   }
   if (bg.kind == BG_SHIFTED) {
-    D th = dtanh((1.0 / bg.len) * X[1]);
-    D pp = bg.A * (1.0 + (-(th * th)));
+    J th = dtanh((1.0 / bg.len) * X[1]);
+    J pp = bg.A * (1.0 + (-(th * th)));
     hd[pidx(0, 1)] = -pp;
     hd[pidx(1, 1)] = -(pp * pp);
     return;
@@ -451,15 +546,16 @@ TGH_HD void bg_offset(const Bg& bg, double t, const double x[3], Dual<4> hd[NC])
   //   r^4 - r^2 (rho^2 - a^2) - a^2 z^2 = 0, f = 2 M r^3 / (r^4 + a^2 z^2),
   //   l = (1, (r x + a y)/(r^2 + a^2), (r y - a x)/(r^2 + a^2), z / r), h = f l l.
   const double a = bg.a, M = bg.M;
-  D Xr = X[1] + dconst<4>(-bg.c[0]), Yr = X[2] + dconst<4>(-bg.c[1]), Zr = X[3] + dconst<4>(-bg.c[2]);
-  D rho2 = Xr * Xr + Yr * Yr + Zr * Zr;
-  D hb = 0.5 * (rho2 + dconst<4>(-a * a));
-  D r2 = hb + dsqrt(hb * hb + (a * a) * (Zr * Zr));
-  D r = dsqrt(r2);
-  D f = (2.0 * M) * (r * r2) / (r2 * r2 + (a * a) * (Zr * Zr));
-  D den = r2 + dconst<4>(a * a);
-  D l[4];
-  l[0] = dconst<4>(1.0);
+  J Xr = X[1] + jet_const<J>(-bg.c[0]), Yr = X[2] + jet_const<J>(-bg.c[1]),
+    Zr = X[3] + jet_const<J>(-bg.c[2]);
+  J rho2 = Xr * Xr + Yr * Yr + Zr * Zr;
+  J hb = 0.5 * (rho2 + jet_const<J>(-a * a));
+  J r2 = hb + dsqrt(hb * hb + (a * a) * (Zr * Zr));
+  J r = dsqrt(r2);
+  J f = (2.0 * M) * (r * r2) / (r2 * r2 + (a * a) * (Zr * Zr));
+  J den = r2 + jet_const<J>(a * a);
+  J l[4];
+  l[0] = jet_const<J>(1.0);
   l[1] = (r * Xr + a * Yr) / den;
   l[2] = (r * Yr + (-a) * Xr) / den;
   l[3] = Zr / r;
@@ -467,11 +563,123 @@ TGH_HD void bg_offset(const Bg& bg, double t, const double x[3], Dual<4> hd[NC])
   for (int n = 0; n < NC; ++n) hd[n] = f * (l[prow(n)] * l[pcol(n)]);
 }
 
+// The sampled gauge source of a background (gauge.jl:156-212): the lowered
+// H_b = -Gamma_b = -g^{de} Gamma_{b,de} and dH[a][b] = d_a H_b, with
+// Gamma_{b,de} = (d_d g_be + d_e g_bd - d_b g_de)/2 and
+// d_a g^{de} = -g^{dp} (d_a g_pq) g^{qe}.
+TGH_HD void gauge_source_grad(const Bg& bg, double t, const double x[3], double Hl[4],
+                              double dH[4][4]) {
+  Jet2 hj[NC];
+  bg_offset<Jet2>(bg, t, x, hj);
+  // g, dg[c][n] = d_c g_n, ddg[c][e][n] = d_c d_e g_n
+  double g[4][4], gi[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) g[a][b] = eta(a, b) + hj[pidx(a, b)].v;
+  // general 4x4 inverse by cofactors (values only, once per point)
+  {
+    double m[16], inv[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = g[i / 4][i % 4];
+    inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+             m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+             m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+             m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+              m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+             m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+             m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+             m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+              m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+             m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+             m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+              m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+              m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+             m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+             m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+              m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+              m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    const double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+    const double idet = 1.0 / det;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) gi[i / 4][i % 4] = inv[i] * idet;
+  }
+  // Gamma_{b,de} and its derivative d_a Gamma_{b,de}
+  double Gam[4][4][4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        Gam[b][d][e] = 0.5 * (hj[pidx(b, e)].d[d] + hj[pidx(b, d)].d[e] - hj[pidx(d, e)].d[b]);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) s += gi[d][e] * Gam[b][d][e];
+    Hl[b] = -s;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    // d_a g^{de} = -g^{dp} (d_a g_pq) g^{qe}
+    double t1[4][4], dgi[4][4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double s = 0.0;
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) s += gi[d][pp] * hj[pidx(pp, q)].d[a];
+        t1[d][q] = s;
+      }
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s += t1[d][q] * gi[q][e];
+        dgi[d][e] = -s;
+      }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      double s = 0.0;
+#pragma unroll
+      for (int d = 0; d < 4; ++d)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double dG = 0.5 * (hj[pidx(b, e)].dd[pidx(a, d)] + hj[pidx(b, d)].dd[pidx(a, e)] -
+                                   hj[pidx(d, e)].dd[pidx(a, b)]);
+          s += dgi[d][e] * Gam[b][d][e] + gi[d][e] * dG;
+        }
+      dH[a][b] = -s;
+    }
+  }
+}
+
 // background_state (src/initialdata.jl:1674-1694): h and
 // Pi = (sqrt(gamma)/alpha)(d_t g - beta^i d_i g) at (t, x).
 TGH_HD void background_state(const Bg& bg, double t, const double x[3], double h[NC], double P[NC]) {
   Dual<4> hd[NC];
-  bg_offset(bg, t, x, hd);
+  bg_offset<Dual<4>>(bg, t, x, hd);
 #pragma unroll
   for (int n = 0; n < NC; ++n) h[n] = hd[n].v;
   Coef c;

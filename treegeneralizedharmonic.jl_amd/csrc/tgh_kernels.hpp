@@ -58,8 +58,10 @@ TGH_HD double hole_radius(const CaseP& c, const double x[3]) {
 
 // case_state_tuple with the core rule (src/initialdata.jl:1795-1796,
 // src/interior.jl:615-625).
-TGH_HD void case_state(const CaseP& c, double t, const double x[3], double out[20]) {
-  double xc[3] = {x[0], x[1], x[2]};
+TGH_HD void core_point(const CaseP& c, const double x[3], double xc[3]) {
+  xc[0] = x[0];
+  xc[1] = x[1];
+  xc[2] = x[2];
   if (c.interior != INT_NONE) {
     const double d[3] = {x[0] - c.center[0], x[1] - c.center[1], x[2] - c.center[2]};
     const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -73,6 +75,10 @@ TGH_HD void case_state(const CaseP& c, double t, const double x[3], double out[2
       for (int q = 0; q < 3; ++q) xc[q] = c.center[q] + c.r0 * n[q];
     }
   }
+}
+TGH_HD void case_state(const CaseP& c, double t, const double x[3], double out[20]) {
+  double xc[3];
+  core_point(c, x, xc);
   background_state(c.bg, t, xc, out, out + NC);
 }
 
@@ -413,6 +419,33 @@ __global__ __launch_bounds__(256) void k_scatter(CopyArgs a) {
     else
       a.dst[si + v * psz] = a.src[wi + v * sv];
   }
+}
+
+// sample_gauge_source! (src/gauge.jl:156-184): H_b and d_a H_b of the
+// background at every owned point, through the core rule, into Hsrc (state
+// layout, 20 variables: H_b then d_a H_b in column-major (a, b) order).
+struct SampleArgs {
+  Geom g;
+  CaseP c;
+  double t;
+  double* __restrict__ Hsrc;
+};
+__global__ __launch_bounds__(256) void k_sample_gauge(SampleArgs a) {
+  const Geom& g = a.g;
+  const int P = g.P;
+  const int b = blockIdx.y;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P * P * P) return;
+  const int i = p % P, j = (p / P) % P, k = p / (P * P);
+  const long psz = (long)P * P * P;
+  double x[3], xc[3];
+  position(g, b, i, j, k, x);
+  core_point(a.c, x, xc);
+  double Hl[4], dH[4][4];
+  gauge_source_grad(a.c.bg, a.t, xc, Hl, dH);
+  double* out = a.Hsrc + (long)b * 20 * psz + p;
+  for (int q = 0; q < 4; ++q) out[q * psz] = Hl[q];
+  for (int n = 0; n < 16; ++n) out[(4 + n) * psz] = dH[n % 4][n / 4];
 }
 
 // Pointwise kernels over owned points of a state vector.
