@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--nblocks", type=int, default=16, help="blocks per axis (16 = the full workload)")
+    ap.add_argument("--form", choices=("split", "fused"), default="split",
+                    help="the right-hand side as head + Pi launches, or one fused kernel")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain (verbs) form")
     return ap.parse_args()
 
 
@@ -73,10 +76,11 @@ def flop_count():
     u = p.fill_exact(0.0)
     p.prepare(u, 0.0)
     f_rhs = p.count_flops(0, 3, 4, 5)
+    f_head = p.count_flops(0, 3, 4, 5, head_only=True)
     # The stage combination: per variable, stage 1 acc = u + c k, Y = u + c k
     # (4 ops); stages 2, 3 acc += c k, Y = u + c k (4); stage 4 u = acc + c k (2).
     f_combine = 20 * (4 + 4 + 4 + 2)
-    return f_rhs, f_combine
+    return f_rhs, f_head, f_combine
 
 
 def cpu_baseline(seconds_target=15.0):
@@ -119,6 +123,9 @@ def main():
     desc = kerr_schild_desc(tgh, args.nblocks)
     ptr = C.c_void_p()
     tgh._check(L.tgh_problem_create(C.byref(desc), C.byref(ptr)))
+    # the library's work and torch's share one stream: the plain form below
+    # interleaves them
+    tgh._check(L.tgh_set_stream(ptr, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
     n_state = int(L.tgh_state_length(ptr))
     u = torch.empty(n_state, dtype=torch.float64, device="cuda")
     uptr = C.c_void_p(u.data_ptr())
@@ -130,11 +137,12 @@ def main():
     nb = args.nblocks ** 3
     points = nb * 16 ** 3
 
+    form = 2 if args.form == "fused" else 0
     # warm-up (also captures the graph)
-    tgh._check(L.tgh_rk4(ptr, uptr, 0.0, dt, args.warmup, 1))
+    tgh._check(L.tgh_rk4(ptr, uptr, 0.0, dt, args.warmup, 1 | form))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    tgh._check(L.tgh_rk4(ptr, uptr, args.warmup * dt, dt, args.steps, 1))
+    tgh._check(L.tgh_rk4(ptr, uptr, args.warmup * dt, dt, args.steps, 1 | form))
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     value = points * args.steps / el
@@ -143,18 +151,54 @@ def main():
     # Per-kernel durations from hipEvents on the problem's stream, over a
     # second pass of the same steps launched without the graph.
     tgh._check(L.tgh_profile(ptr, 1))
-    tgh._check(L.tgh_rk4(ptr, uptr, (args.warmup + args.steps) * dt, dt, args.steps, 0))
-    prof = (C.c_double * 4)()
+    tgh._check(L.tgh_rk4(ptr, uptr, (args.warmup + args.steps) * dt, dt, args.steps, form))
+    prof = (C.c_double * 6)()
     tgh._check(L.tgh_profile_read(ptr, prof))
     tgh._check(L.tgh_profile(ptr, 0))
-    rhs_ms = prof[0] / max(prof[1], 1)
+    head_ms = prof[0] / max(prof[1], 1)
     fill_ms = prof[2] / max(prof[3], 1)
+    pi_ms = prof[4] / max(prof[5], 1)
 
-    f_rhs, f_combine = flop_count()
-    # per launch of the fused RHS: F_rhs per point over the N^3 points of each
-    # block, plus a quarter of the stage combination
-    flops_launch = (f_rhs + f_combine / 4) * points
-    achieved = flops_launch / (rhs_ms * 1e-3) / 1e12
+    # The plain form: RK4 written with the verbs (gh_rhs! = prepare! + rhs!,
+    # then the stage arithmetic as separate passes), as the reference's
+    # integrator drives the kernel -- a few steps of it on the same state.
+    plain_steps = 0 if args.no_plain else max(1, min(3, args.steps))
+    k = [torch.empty_like(u) for _ in range(4)]
+    y = torch.empty_like(u)
+
+    def rhs_(du, uu, t):
+        tgh._check(L.tgh_gh_rhs(ptr, C.c_void_p(du.data_ptr()), C.c_void_p(uu.data_ptr()), t, None))
+
+    t = (args.warmup + 2 * args.steps) * dt
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for n in range(plain_steps):
+        rhs_(k[0], u, t)
+        torch.add(u, k[0], alpha=dt / 2, out=y)
+        rhs_(k[1], y, t + dt / 2)
+        torch.add(u, k[1], alpha=dt / 2, out=y)
+        rhs_(k[2], y, t + dt / 2)
+        torch.add(u, k[2], alpha=dt, out=y)
+        rhs_(k[3], y, t + dt)
+        u.add_(k[0] + 2 * k[1] + 2 * k[2] + k[3], alpha=dt / 6)
+        t += dt
+    torch.cuda.synchronize()
+    plain_value = points * plain_steps / (time.perf_counter() - t1) if plain_steps else None
+
+    f_rhs, f_head, f_combine = flop_count()
+    f_pi = f_rhs - f_head
+    # Per launch: F per point of that kernel's half over the N^3 points of each
+    # block, plus its half of a quarter of the stage combination.
+    if args.form == "fused":
+        head_tf = (f_rhs + f_combine / 4) * points / (head_ms * 1e-3) / 1e12
+        pi_tf = None
+        dom = "fused"
+        achieved = head_tf
+    else:
+        head_tf = (f_head + f_combine / 8) * points / (head_ms * 1e-3) / 1e12
+        pi_tf = (f_pi + f_combine / 8) * points / (pi_ms * 1e-3) / 1e12
+        dom = "head" if head_ms >= pi_ms else "pi"
+        achieved = head_tf if dom == "head" else pi_tf
     step_ms = el / args.steps * 1e3
     t_roof = max((4 * f_rhs + f_combine) / (FP64_PEAK_TFLOPS * 1e12), B_POINT_UPDATE / (HBM_PEAK_GBS * 1e9))
     result = {
@@ -176,10 +220,14 @@ def main():
                    "blocks": nb, "points": points, "q": 4, "dt": dt},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_rhs<4, KS, hole, RK4 epilogue>", "kernel_ms": rhs_ms,
-                     "fill_ms": fill_ms, "flops_per_point_rhs": f_rhs,
+                     "kernel": {"head": "k_head<4, KS, hole, RK4 epilogue>", "pi": "k_pi<4, RK4 epilogue>",
+                                "fused": "k_rhs<4, KS, hole, RK4 epilogue>"}[dom],
+                     "kernel_ms": pi_ms if dom == "pi" else head_ms,
+                     "head_ms": head_ms, "head_tflops": head_tf, "pi_ms": pi_ms, "pi_tflops": pi_tf,
+                     "fill_ms": fill_ms, "flops_per_point_rhs": f_rhs, "flops_per_point_head": f_head,
                      "flops_per_point_update": 4 * f_rhs + f_combine,
                      "step_roofline_frac": t_roof / (step_ms * 1e-3 / points)},
+        "forms": {f"{args.form}_graph": value, "plain_verbs": plain_value},
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()

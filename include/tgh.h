@@ -53,7 +53,9 @@ enum { /* interior variants, src/interior.jl:178-238 */
   TGH_INTERIOR_PASTED = 3
 };
 enum { /* tgh_rk4 flags */
-  TGH_RK4_GRAPH = 1 /* capture the step in a hipGraph and replay it */
+  TGH_RK4_GRAPH = 1, /* capture the step in a hipGraph and replay it */
+  TGH_RK4_FUSED = 2  /* one kernel per right-hand side (default: the head and
+                        Pi-component halves as two launches) */
 };
 
 /* The case and its uniform forest: GHCase (src/initialdata.jl:158-311) plus
@@ -104,7 +106,9 @@ int64_t tgh_state_length(const tgh_problem* p);
 /* Block b's integer coordinates in the brick, 3 per block (block_corners,
  * src/evolution.jl:120-130). */
 int tgh_block_coords(const tgh_problem* p, int32_t* out);
-/* Order the problem's work on a caller's hipStream_t (NULL = its own). */
+/* Order the problem's work on a caller's hipStream_t from now on (NULL = the
+ * device's null stream), e.g. the stream a framework's tensors are produced
+ * on; until then the problem uses a stream of its own. */
 int tgh_set_stream(tgh_problem* p, void* stream);
 
 /* prepare!(p, u, t) — src/evolution.jl:1349-1357: scatter u into the work
@@ -150,11 +154,11 @@ int64_t tgh_work_length(const tgh_problem* p);
  * reference's ghosted FieldSet work array. */
 int tgh_copy_work(tgh_problem* p, double* dst);
 /* No reference counterpart: per-kernel timing for the benchmark. While
- * enabled, tgh_rk4 without TGH_RK4_GRAPH brackets every launch of the fused
- * right-hand side and of the ghost fill with hipEvents on the problem's stream;
- * tgh_profile_read gives out[0..3] = summed milliseconds of the right-hand
- * side launches, their count, summed milliseconds of the fills, their count,
- * since the last tgh_profile(p, 1). */
+ * enabled, tgh_rk4 without TGH_RK4_GRAPH brackets every kernel launch with
+ * hipEvents on the problem's stream; tgh_profile_read gives out[0..5] =
+ * summed milliseconds and count of the right-hand side's first launch (the
+ * head, or the whole fused kernel), of the ghost fills, and of the
+ * Pi-component launches, since the last tgh_profile(p, 1). */
 int tgh_profile(tgh_problem* p, int32_t enable);
 int tgh_profile_read(tgh_problem* p, double* out);
 /* Message of the last failure on this thread ("" if none). */

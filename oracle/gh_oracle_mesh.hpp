@@ -223,7 +223,7 @@ struct Problem {
   // F(u) at one owned point into dth, dtP (gh_rhs_at_point, evolution.jl:423-435).
   template <class T>
   void rhs_point_F(long b, long i, long j, long k, double tt, T g0, T g2, T epsh, bool diss,
-                   V10<T>& dth, V10<T>& dtP) const {
+                   V10<T>& dth, V10<T>& dtP, bool head_only = false) const {
     const Mesh& m = mesh;
     long st[3] = {1, m.S, m.S * m.S};
     long sv = m.S * m.S * m.S;
@@ -267,6 +267,7 @@ struct Problem {
       for (int c = 0; c < 3; ++c) A[a][c] = (mq.alpha * mq.sqrtg) * mq.gamu[a][c];
     T divb, divA[3];
     metric_divergences(mq, dh, divb, divA);
+    if (head_only) return;
     // gh_rhs_pi per component (evolution.jl:324-354)
     T inv_h2 = inv_h * inv_h;
     for (int v = 0; v < NC; ++v) {

@@ -282,14 +282,14 @@ int orc_is_primary(void* p, long b, long i, long j, long k) {
 // The FP64 operation count of F at one interior point of the :none branch
 // (q, dissipation on iff eps_ko != 0, no gauge source): the streaming kernel
 // instantiated with the counting type on a prepared work array.
-double orc_count_rhs_flops(void* pp, long b, long i, long j, long k) {
+double orc_count_rhs_flops(void* pp, long b, long i, long j, long k, int head_only) {
   Problem* p = (Problem*)pp;
   double x[3];
   p->mesh.position(b, i, j, k, x);
   Cnt::ops = 0;
   V10<Cnt> dth, dtP;
   Cnt g0(damping_rate(p->cs, x)), g2(p->cs.g2), epsh(p->cs.eps_ko / p->mesh.h);
-  p->rhs_point_F<Cnt>(b, i, j, k, 0.0, g0, g2, epsh, p->cs.eps_ko != 0.0, dth, dtP);
+  p->rhs_point_F<Cnt>(b, i, j, k, 0.0, g0, g2, epsh, p->cs.eps_ko != 0.0, dth, dtP, head_only != 0);
   return (double)Cnt::ops;
 }
 

@@ -71,7 +71,7 @@ def lib():
             "orc_max_speed": (D, [P, dp]),
             "orc_error_norms": (None, [P, dp, D, C.POINTER(D), C.POINTER(D)]),
             "orc_is_primary": (C.c_int, [P, Lg, Lg, Lg, Lg]),
-            "orc_count_rhs_flops": (D, [P, Lg, Lg, Lg, Lg]),
+            "orc_count_rhs_flops": (D, [P, Lg, Lg, Lg, Lg, C.c_int]),
             "orc_max_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
@@ -173,5 +173,5 @@ class OracleProblem:
     def rhs_scale(self, b, i, j, k):
         return lib().orc_rhs_scale(self.p, b, i, j, k)
 
-    def count_flops(self, b, i, j, k):
-        return lib().orc_count_rhs_flops(self.p, b, i, j, k)
+    def count_flops(self, b, i, j, k, head_only=False):
+        return lib().orc_count_rhs_flops(self.p, b, i, j, k, 1 if head_only else 0)

@@ -79,6 +79,8 @@ def tgh_problem_from_desc(tgh, desc):
     L = tgh._load()
     rc = L.tgh_problem_create(C.byref(tgh.TghDesc.from_buffer_copy(desc)), C.byref(ptr))
     tgh._check(rc)
+    import torch
+    tgh._check(L.tgh_set_stream(ptr, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
     h._ptr = ptr
     h.ptr = ptr
     h.state_length = int(L.tgh_state_length(ptr))

@@ -343,6 +343,8 @@ class GHProblem:
         import torch
         with torch.cuda.device(self.backend.device):
             _check(_load().tgh_problem_create(C.byref(self.desc), C.byref(self._ptr)))
+            # order the library's work after torch's on the same stream
+            _check(_load().tgh_set_stream(self._ptr, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
         self.state_length = int(_load().tgh_state_length(self._ptr))
         self.work_length = int(_load().tgh_work_length(self._ptr))
         self.nblocks = forest.roots[0] * forest.roots[1] * forest.roots[2]

@@ -52,6 +52,7 @@ struct tgh_problem {
   int* d_lookup = nullptr;
   double* W[2] = {nullptr, nullptr};  // work arrays (prepare uses W[0])
   double* acc = nullptr;              // RK4 accumulator, state layout
+  double* coef = nullptr;             // the split form's per-point coefficients (CF_N x points)
   double* Hsrc = nullptr;             // sampled gauge source, state layout x 20
   double* red = nullptr;              // per-block reduction partials (3 per block)
   std::vector<double> red_host;
@@ -61,21 +62,30 @@ struct tgh_problem {
   double graph_key[3] = {0, 0, 0};    // (dt, u pointer, t) the graph was captured for
   // Benchmark timing: event pairs around each launch while profiling.
   bool profiling = false;
+  bool fused = false;                 // one kernel per RHS instead of head + Pi
+  bool graph_fused = false;           // the form the captured graph launches
+  hipStream_t cap_stream = nullptr;   // capture stream when the problem runs on the null stream
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, int>> ev_rhs, ev_fill;  // indices into ev_pool
+  std::vector<std::pair<int, int>> ev_rhs, ev_fill, ev_pi;  // indices into ev_pool
   size_t ev_used = 0;
-  double prof[4] = {0, 0, 0, 0};
+  double prof[6] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace {
 
+// 1-D grids of 8 * ceil(total / 8) workgroups of 256 threads, which the
+// kernels map to (block, workgroup-in-block) XCD by XCD (wg_map).
+dim3 xcd_grid(long total) {
+  const long per = (total + NUM_XCD - 1) / NUM_XCD;
+  return dim3((unsigned)(NUM_XCD * per));
+}
 dim3 grid_points(const tgh_problem* p) {
-  const int P3 = p->geom.P * p->geom.P * p->geom.P;
-  return dim3((P3 + 255) / 256, p->geom.nblocks);
+  const long P3 = (long)p->geom.P * p->geom.P * p->geom.P;
+  return xcd_grid((P3 + 255) / 256 * p->geom.nblocks);
 }
 dim3 grid_work(const tgh_problem* p) {
-  const int S3 = p->geom.S * p->geom.S * p->geom.S;
-  return dim3((S3 + 255) / 256, p->geom.nblocks);
+  const long S3 = (long)p->geom.S * p->geom.S * p->geom.S;
+  return xcd_grid((S3 + 255) / 256 * p->geom.nblocks);
 }
 
 int check_desc(const tgh_desc* d) {
@@ -143,9 +153,31 @@ void launch_scatter(tgh_problem* p, const double* u, double* W, int to_work) {
   hipLaunchKernelGGL(k_scatter, grid_points(p), dim3(256), 0, p->stream, a);
 }
 
+int mark(tgh_problem* p);
+
+// One right-hand side: the fused kernel, or the head and the Pi components as
+// two launches. While profiling, the head is timed as the "rhs" pair and the
+// Pi kernel as its own pair.
 template <int Q, int GAUGE, bool HOLE, int EPI>
 void launch_rhs_q(tgh_problem* p, const RhsArgs& a) {
-  hipLaunchKernelGGL((k_rhs<Q, GAUGE, HOLE, EPI>), grid_points(p), dim3(256), 0, p->stream, a);
+  const bool prof = p->profiling && EPI == EPI_RK4;
+  if (p->fused) {
+    int e0 = prof ? mark(p) : -1;
+    hipLaunchKernelGGL((k_rhs<Q, GAUGE, HOLE, EPI>), grid_points(p), dim3(256), 0, p->stream, a, a.W,
+                       a.du, a.us, a.acc, a.Wn);
+    if (prof) p->ev_rhs.push_back({e0, mark(p)});
+  } else {
+    int e0 = prof ? mark(p) : -1;
+    hipLaunchKernelGGL((k_head<Q, GAUGE, HOLE, EPI>), grid_points(p), dim3(256), 0, p->stream, a, a.W,
+                       a.coef, a.du, a.us, a.acc, a.Wn);
+    int e1 = prof ? mark(p) : -1;
+    hipLaunchKernelGGL((k_pi<Q, EPI>), grid_points(p), dim3(256), 0, p->stream, a, a.W,
+                       (const double*)a.coef, a.du, a.us, a.acc, a.Wn);
+    if (prof) {
+      p->ev_rhs.push_back({e0, e1});
+      p->ev_pi.push_back({e1, mark(p)});
+    }
+  }
 }
 template <int Q, int EPI>
 void launch_rhs_g(tgh_problem* p, const RhsArgs& a) {
@@ -183,6 +215,7 @@ RhsArgs rhs_args(tgh_problem* p, const double* W, double t, const tgh_matter* m)
   a.t = t;
   a.W = W;
   a.Hsrc = p->Hsrc;
+  a.coef = p->coef;
   if (m && m->rho) {
     a.m_rho = m->rho;
     a.m_S = m->S;
@@ -238,8 +271,17 @@ void collect_profile(tgh_problem* p) {
       p->prof[3] += 1;
     }
   }
+  for (auto& pr : p->ev_pi) {
+    float ms = 0;
+    if (pr.first >= 0 && pr.second >= 0 &&
+        hipEventElapsedTime(&ms, p->ev_pool[pr.first], p->ev_pool[pr.second]) == hipSuccess) {
+      p->prof[4] += ms;
+      p->prof[5] += 1;
+    }
+  }
   p->ev_rhs.clear();
   p->ev_fill.clear();
+  p->ev_pi.clear();
   p->ev_used = 0;
 }
 
@@ -255,18 +297,15 @@ void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
     const bool prof = p->profiling;
     int e0 = prof ? mark(p) : -1;
     launch_fill(p, Wc, t_stage[s]);
-    int e1 = prof ? mark(p) : -1;
-    if (prof) p->ev_fill.push_back({e0, e1});
+    if (prof) p->ev_fill.push_back({e0, mark(p)});
     RhsArgs a = rhs_args(p, Wc, t_stage[s], nullptr);
-    a.u0 = u;
+    a.us = u;
     a.acc = p->acc;
     a.Wn = Wn;
-    a.uout = u;
     a.cacc = c_acc[s];
     a.cnext = c_next[s];
     a.stage = s + 1;
     launch_rhs<EPI_RK4>(p, a);
-    if (prof) p->ev_rhs.push_back({e1, mark(p)});
   }
 }
 
@@ -353,6 +392,7 @@ int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
       (e = hipMalloc(&p->W[0], sizeof(double) * p->work_len)) != hipSuccess ||
       (e = hipMalloc(&p->W[1], sizeof(double) * p->work_len)) != hipSuccess ||
       (e = hipMalloc(&p->acc, sizeof(double) * p->state_len)) != hipSuccess ||
+      (e = hipMalloc(&p->coef, sizeof(double) * (p->state_len / 20) * CF_N)) != hipSuccess ||
       (e = hipMalloc(&p->red, sizeof(double) * p->red_host.size())) != hipSuccess)
     return cleanup(TGH_EDEVICE, std::string("allocating the problem's device arrays: ") + hipGetErrorString(e));
   if ((e = hipMemcpy(p->d_bcoord, p->bcoord_host.data(), sizeof(int) * p->bcoord_host.size(),
@@ -392,9 +432,11 @@ int tgh_problem_destroy(tgh_problem* p) {
   (void)hipFree(p->W[0]);
   (void)hipFree(p->W[1]);
   (void)hipFree(p->acc);
+  (void)hipFree(p->coef);
   (void)hipFree(p->Hsrc);
   (void)hipFree(p->red);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   delete p;
   return TGH_OK;
 }
@@ -411,12 +453,11 @@ int tgh_set_stream(tgh_problem* p, void* stream) {
   if (!p) return fail(TGH_EINVAL, "NULL problem");
   HIP_TRY(hipStreamSynchronize(p->stream));
   if (p->own_stream) HIP_TRY(hipStreamDestroy(p->stream));
-  if (stream) {
-    p->stream = (hipStream_t)stream;
-    p->own_stream = false;
-  } else {
-    HIP_TRY(hipStreamCreate(&p->stream));
-    p->own_stream = true;
+  p->stream = (hipStream_t)stream;  // NULL: the device's null stream
+  p->own_stream = false;
+  if (p->graph_exec) {  // a captured graph belongs to the old stream's work order
+    HIP_TRY(hipGraphExecDestroy(p->graph_exec));
+    p->graph_exec = nullptr;
   }
   return TGH_OK;
 }
@@ -522,6 +563,12 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
   // The stage input of the first stage: u_n scattered into W[0].
   launch_scatter(p, u, p->W[0], 1);
   const bool graph = (flags & TGH_RK4_GRAPH) != 0;
+  const bool fused = (flags & TGH_RK4_FUSED) != 0;
+  if (fused != p->graph_fused && p->graph_exec) {  // the captured graph holds the other form
+    HIP_TRY(hipGraphExecDestroy(p->graph_exec));
+    p->graph_exec = nullptr;
+  }
+  p->fused = fused;
   if (!graph) {
     for (int n = 0; n < nsteps; ++n) {
       rk4_step_launches(p, u, t0 + n * dt, dt);
@@ -532,6 +579,7 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
       }
     }
     int rc = sync(p);
+    p->fused = false;
     if (rc) return rc;
     if (p->profiling) collect_profile(p);
     return TGH_OK;
@@ -556,17 +604,31 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
         HIP_TRY(hipGraphDestroy(p->graph));
         p->graph = nullptr;
       }
-      HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-      rk4_step_launches(p, u, t, dt);
-      HIP_TRY(hipStreamEndCapture(p->stream, &p->graph));
+      // The null stream cannot be captured: record on a private stream then,
+      // and launch the graph on the problem's stream below.
+      hipStream_t saved = p->stream;
+      if (saved == nullptr) {
+        if (!p->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        p->stream = p->cap_stream;
+      }
+      hipError_t ec = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
+      if (ec == hipSuccess) {
+        rk4_step_launches(p, u, t, dt);
+        ec = hipStreamEndCapture(p->stream, &p->graph);
+      }
+      p->stream = saved;
+      if (ec != hipSuccess) return fail(TGH_EDEVICE, std::string("capturing the RK4 step: ") + hipGetErrorString(ec));
       HIP_TRY(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
+      p->graph_fused = fused;
       p->graph_key[0] = dt;
       p->graph_key[1] = (double)(uintptr_t)u;
       p->graph_key[2] = t;
     }
     HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
   }
-  return sync(p);
+  int rc = sync(p);
+  p->fused = false;
+  return rc;
 }
 
 int tgh_profile(tgh_problem* p, int32_t enable) {
@@ -575,6 +637,7 @@ int tgh_profile(tgh_problem* p, int32_t enable) {
   p->profiling = enable != 0;
   p->ev_rhs.clear();
   p->ev_fill.clear();
+  p->ev_pi.clear();
   p->ev_used = 0;
   for (double& x : p->prof) x = 0;
   return TGH_OK;
@@ -582,7 +645,7 @@ int tgh_profile(tgh_problem* p, int32_t enable) {
 
 int tgh_profile_read(tgh_problem* p, double* out) {
   if (!p || !out) return fail(TGH_EINVAL, "NULL argument");
-  for (int i = 0; i < 4; ++i) out[i] = p->prof[i];
+  for (int i = 0; i < 6; ++i) out[i] = p->prof[i];
   return TGH_OK;
 }
 

@@ -110,6 +110,24 @@ TGH_HD int classify(const Geom& g, int b, const int l[3], int& owner, int loc[3]
   return 0;
 }
 
+// XCD-aware workgroup order. MI355X dispatches consecutive workgroups of a
+// launch round-robin over its 8 XCDs, each with its own L2. A launch here is
+// 1-D with 8 * ceil(total / 8) workgroups, and hardware workgroup w takes
+// logical workgroup (w mod 8) * per + w / 8, so each XCD runs one contiguous
+// range of logical workgroups: a run of whole blocks, consecutive in Morton
+// order, whose work arrays and neighbours stay in that XCD's L2.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ bool wg_map(int wgpb, int nblocks, int& b, int& wx) {
+  const int total = wgpb * nblocks;
+  const int per = (total + NUM_XCD - 1) / NUM_XCD;
+  const int flat = blockIdx.x;
+  const int L = (flat % NUM_XCD) * per + flat / NUM_XCD;
+  if (L >= total) return false;
+  b = L / wgpb;
+  wx = L - b * wgpb;
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // Stencils on the work array. Weights are exact rationals rounded once
 // (src/stencils.jl:124-157); the zero center weight of the first derivative is
@@ -198,33 +216,53 @@ struct RhsArgs {
   const double* __restrict__ m_S;
   const double* __restrict__ m_Sij;
   // EPI_STORE
-  double* __restrict__ du;
-  // EPI_RK4: stage s = 1..4 of acc/Y recurrences
-  const double* __restrict__ u0;
-  double* __restrict__ acc;
-  double* __restrict__ Wn;
-  double* __restrict__ uout;
+  double* du;
+  // EPI_RK4: stage s = 1..4 of the acc/Y recurrences; us is u_n, read in
+  // stages 1-3 and overwritten by u_{n+1} in stage 4
+  double* us;
+  double* acc;
+  double* Wn;
   double cacc, cnext;
   int stage;
+  // the split form: per-point coefficients the head leaves for the Pi kernel
+  double* __restrict__ coef;
 };
 
+// Coefficient slots of the split form, each an array over the (N+1)^3 x nb
+// owned points: beta^i, d_i beta^i, d_i A^{ij}, A^{ij} (xx,xy,xz,yy,yz,zz),
+// the layer weight w (-1 marks the frozen core, where F is not evaluated),
+// then per Pi component c1_v = w msrc_v - rho (Pi_v - Pi_exact,v), so that
+// dPi_v = w (principal part) + c1_v.
+enum { CF_B0 = 0, CF_DIVB = 3, CF_DIVA = 4, CF_A = 7, CF_W = 13, CF_C1 = 14, CF_N = 24 };
+
+// The epilogue: store F (the rhs! verb) or apply RK4 stage s to it: with k = F,
+//   s = 1: acc = u + c_acc k,  Y = u + c_next k
+//   s = 2, 3: acc += c_acc k,  Y = u + c_next k
+//   s = 4: u = acc + c_acc k (and Y = u, the next step's first stage input)
+// where Y is the next stage's input, written into the other work array's
+// owned point. The pointers are kernel arguments marked __restrict__ so that
+// the stores do not order the stencil loads around them.
+#define TGH_OUT_PARAMS                                                                          \
+  double *__restrict__ du, double *__restrict__ us, double *__restrict__ acc,                   \
+      double *__restrict__ Wn
+#define TGH_OUT_ARGS du, us, acc, Wn
 template <int EPI>
-__device__ __forceinline__ void emit(const RhsArgs& a, long si, long wi, double k) {
+__device__ __forceinline__ void emit(const RhsArgs& a, TGH_OUT_PARAMS, long si, long wi, double k) {
   if (EPI == EPI_STORE) {
-    a.du[si] = k;
+    du[si] = k;
   } else {
     if (a.stage == 1) {
-      const double u = a.u0[si];
-      a.acc[si] = u + a.cacc * k;
-      a.Wn[wi] = u + a.cnext * k;
+      const double u = us[si];
+      acc[si] = u + a.cacc * k;
+      Wn[wi] = u + a.cnext * k;
     } else if (a.stage < 4) {
-      const double u = a.u0[si];
-      a.acc[si] = a.acc[si] + a.cacc * k;
-      a.Wn[wi] = u + a.cnext * k;
+      const double u = us[si];
+      acc[si] = acc[si] + a.cacc * k;
+      Wn[wi] = u + a.cnext * k;
     } else {
-      const double un = a.acc[si] + a.cacc * k;
-      a.uout[si] = un;
-      a.Wn[wi] = un;
+      const double un = acc[si] + a.cacc * k;
+      us[si] = un;
+      Wn[wi] = un;
     }
   }
 }
@@ -235,16 +273,17 @@ __device__ __forceinline__ void emit(const RhsArgs& a, long si, long wi, double 
 // derivatives and the coefficients; d_t h complete; the source; then the ten
 // Pi components one at a time.
 template <int Q, int GAUGE, bool HOLE, int EPI>
-__global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
+__global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict__ W, TGH_OUT_PARAMS) {
   const Geom& g = a.g;
   const int P = g.P, S = g.S, G = g.G;
-  const int b = blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int b, wx;
+  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
   if (p >= P * P * P) return;
   const int i = p % P, j = (p / P) % P, k = p / (P * P);
   const long sv = (long)S * S * S;
   const int s1 = 1, s2 = S, s3 = S * S;
-  const double* __restrict__ Wb = a.W + (long)b * 20 * sv;
+  const double* __restrict__ Wb = W + (long)b * 20 * sv;
   const long off = (long)(i + G) + (long)S * ((j + G) + (long)S * (k + G));
   const double* __restrict__ w0 = Wb + off;
   const long psz = (long)P * P * P;
@@ -260,7 +299,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
     const bool frozen = c.interior == INT_PASTED ? r < c.r1 : r < c.r0;
     if (frozen) {  // F is not evaluated in the core (a hole at rest: du = 0)
 #pragma unroll 1
-      for (int v = 0; v < 20; ++v) emit<EPI>(a, si0 + v * psz, wi0 + v * sv, 0.0);
+      for (int v = 0; v < 20; ++v) emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, 0.0);
       return;
     }
   }
@@ -340,7 +379,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
   for (int v = 0; v < NC; ++v) {
     double f = D[0][v];
     if (layer) f = wgt * f - rho * (hv[v] - ue[v]);
-    emit<EPI>(a, si0 + v * psz, wi0 + v * sv, f);
+    emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, f);
   }
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
@@ -358,7 +397,209 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
     if (c.has_diss) s += epsh * (st_ko<Q>(pP, s1) + st_ko<Q>(pP, s2) + st_ko<Q>(pP, s3));
     double f = s + msrc[v];
     if (layer) f = wgt * f - rho * (Pi - ue[NC + v]);
-    emit<EPI>(a, si0 + (NC + v) * psz, wi0 + (NC + v) * sv, f);
+    emit<EPI>(a, TGH_OUT_ARGS, si0 + (NC + v) * psz, wi0 + (NC + v) * sv, f);
+  }
+}
+
+// The split form, first half: gh_rhs_head (src/evolution.jl:258-297) at one
+// owned point per thread -- the state, the 30 first derivatives, the
+// coefficient set, d_t h complete (stored through the epilogue), the source --
+// leaving the Pi components' coefficients in a.coef for k_pi. The frozen core
+// is decided here, before any stencil is read.
+template <int Q, int GAUGE, bool HOLE, int EPI>
+__global__ __launch_bounds__(256) void k_head(RhsArgs a, const double* __restrict__ W,
+                                              double* __restrict__ coef, TGH_OUT_PARAMS) {
+  const Geom& g = a.g;
+  const int P = g.P, S = g.S, G = g.G;
+  int b, wx;
+  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
+  if (p >= P * P * P) return;
+  const int i = p % P, j = (p / P) % P, k = p / (P * P);
+  const long sv = (long)S * S * S;
+  const int s1 = 1, s2 = S, s3 = S * S;
+  const double* __restrict__ Wb = W + (long)b * 20 * sv;
+  const long off = (long)(i + G) + (long)S * ((j + G) + (long)S * (k + G));
+  const double* __restrict__ w0 = Wb + off;
+  const long psz = (long)P * P * P;
+  const long si0 = (long)b * 20 * psz + p;
+  const long wi0 = (long)b * 20 * sv + off;
+
+  double x[3];
+  position(g, b, i, j, k, x);
+  const CaseP& c = a.c;
+  double wgt = 1.0, rho = 0.0, r = 0.0;
+  if (HOLE) {
+    r = hole_radius(c, x);
+    const bool frozen = c.interior == INT_PASTED ? r < c.r1 : r < c.r0;
+    if (frozen) {  // F is not evaluated in the core (a hole at rest: du = 0)
+#pragma unroll 1
+      for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, 0.0);
+      coef[(long)CF_W * g.nblocks * psz + (long)b * psz + p] = -1.0;
+      return;
+    }
+  }
+  const double g0 = damping_rate(c, x);
+  const double epsh = c.eps_ko * g.inv_h;
+  const double ih = g.inv_h, ih2 = g.inv_h * g.inv_h;
+
+  double hv[NC], D[4][NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    const double* pv = w0 + v * sv;
+    hv[v] = pv[0];
+    D[1][v] = ih * st_d1<Q>(pv, s1);
+    D[2][v] = ih * st_d1<Q>(pv, s2);
+    D[3][v] = ih * st_d1<Q>(pv, s3);
+  }
+  Coef cf;
+  metric_quantities(hv, cf);
+  const double a_div = cf.alpha / cf.sqrtg;
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    const double* pv = w0 + v * sv;
+    double s = cf.beta[0] * D[1][v] + cf.beta[1] * D[2][v] + cf.beta[2] * D[3][v] +
+               a_div * pv[NC * sv];
+    if (c.has_diss) s += epsh * (st_ko<Q>(pv, s1) + st_ko<Q>(pv, s2) + st_ko<Q>(pv, s3));
+    D[0][v] = s;
+  }
+  double msrc[NC];
+  {
+    double Hl[4] = {0.0, 0.0, 0.0, 0.0}, dH[4][4];
+    if (GAUGE == GAUGE_SAMPLED) {
+      const double* Hp = a.Hsrc + si0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Hl[q] = Hp[q * psz];
+#pragma unroll
+      for (int n = 0; n < 16; ++n) dH[n % 4][n / 4] = Hp[(4 + n) * psz];
+    } else if (GAUGE == GAUGE_KS) {
+      ks_source(c.bg.M, c.bg.a, hv, D, Hl, dH);
+    }
+    gh_source<GAUGE != GAUGE_NONE>(cf, hv, D, Hl, dH, g0, c.g2, msrc);
+  }
+  if (a.m_rho != nullptr) {
+    const long mi = (long)b * psz + p;
+    const double rr = a.m_rho[mi];
+    const double Sm[3] = {a.m_S[(long)b * 3 * psz + p], a.m_S[(long)b * 3 * psz + psz + p],
+                          a.m_S[(long)b * 3 * psz + 2 * psz + p]};
+    double Sij[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) Sij[n] = a.m_Sij[(long)b * 6 * psz + n * psz + p];
+    double ms[NC];
+    matter_source(cf, hv, rr, Sm, Sij, ms);
+#pragma unroll
+    for (int n = 0; n < NC; ++n) msrc[n] += ms[n];
+  }
+  double divb, divA[3];
+  metric_divergences(cf, &D[1], divb, divA);
+  const double as = cf.alpha * cf.sqrtg;
+  const double A00 = as * cf.gamu[0], A01 = as * cf.gamu[1], A02 = as * cf.gamu[2];
+  const double A11 = as * cf.gamu[3], A12 = as * cf.gamu[4], A22 = as * cf.gamu[5];
+  const double b0 = cf.beta[0], b1 = cf.beta[1], b2 = cf.beta[2];
+
+  if (HOLE) {
+    if (r < c.r1) {
+      double w_, rho_;
+      const double sl = (r - c.r0) / (c.r1 - c.r0);
+      w_ = smoothstep(sl / c.w_ramp);
+      rho_ = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
+      wgt = w_;
+      rho = rho_;
+    }
+  }
+  double ue[20];
+  const bool layer = HOLE && r < c.r1;
+  if (layer) case_state(c, a.t, x, ue);  // r >= r0 here: the core rule is the identity
+
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    double f = D[0][v];
+    if (layer) f = wgt * f - rho * (hv[v] - ue[v]);
+    emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, f);
+  }
+  {
+    const long np = (long)g.nblocks * psz;
+    double* cf0 = coef + (long)b * psz + p;
+    cf0[CF_B0 * np] = b0;
+    cf0[(CF_B0 + 1) * np] = b1;
+    cf0[(CF_B0 + 2) * np] = b2;
+    cf0[CF_DIVB * np] = divb;
+    cf0[CF_DIVA * np] = divA[0];
+    cf0[(CF_DIVA + 1) * np] = divA[1];
+    cf0[(CF_DIVA + 2) * np] = divA[2];
+    cf0[CF_A * np] = A00;
+    cf0[(CF_A + 1) * np] = A01;
+    cf0[(CF_A + 2) * np] = A02;
+    cf0[(CF_A + 3) * np] = A11;
+    cf0[(CF_A + 4) * np] = A12;
+    cf0[(CF_A + 5) * np] = A22;
+    cf0[CF_W * np] = layer ? wgt : 1.0;
+#pragma unroll
+    for (int v = 0; v < NC; ++v) {
+      double c1 = msrc[v];
+      if (layer) c1 = wgt * c1 - rho * (w0[(NC + v) * sv] - ue[NC + v]);
+      cf0[(CF_C1 + v) * np] = c1;
+    }
+  }
+}
+
+// The split form, second half: the ten Pi components (gh_rhs_pi,
+// src/evolution.jl:324-354) at one owned point per thread, from the work
+// array and the coefficients k_head left: nine stencils of h_v and seven of
+// Pi_v per component, contracted and stored through the epilogue. Light on
+// registers, so many waves hide the loads.
+#ifndef TGH_PI_WAVES_PER_EU
+#define TGH_PI_WAVES_PER_EU 2
+#endif
+template <int Q, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_PI_WAVES_PER_EU))) void k_pi(RhsArgs a, const double* __restrict__ W,
+                                            const double* __restrict__ coef, TGH_OUT_PARAMS) {
+  const Geom& g = a.g;
+  const int P = g.P, S = g.S, G = g.G;
+  int b, wx;
+  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
+  if (p >= P * P * P) return;
+  const int i = p % P, j = (p / P) % P, k = p / (P * P);
+  const long sv = (long)S * S * S;
+  const int s1 = 1, s2 = S, s3 = S * S;
+  const long off = (long)(i + G) + (long)S * ((j + G) + (long)S * (k + G));
+  const double* __restrict__ w0 = W + (long)b * 20 * sv + off;
+  const long psz = (long)P * P * P;
+  const long si0 = (long)b * 20 * psz + p;
+  const long wi0 = (long)b * 20 * sv + off;
+  const long np = (long)g.nblocks * psz;
+  const double* __restrict__ cf0 = coef + (long)b * psz + p;
+  const double wl = cf0[CF_W * np];
+  if (wl < 0.0) {
+#pragma unroll 1
+    for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, si0 + (NC + v) * psz, wi0 + (NC + v) * sv, 0.0);
+    return;
+  }
+  const double b0 = cf0[CF_B0 * np], b1 = cf0[(CF_B0 + 1) * np], b2 = cf0[(CF_B0 + 2) * np];
+  const double divb = cf0[CF_DIVB * np];
+  const double dA0 = cf0[CF_DIVA * np], dA1 = cf0[(CF_DIVA + 1) * np], dA2 = cf0[(CF_DIVA + 2) * np];
+  const double A00 = cf0[CF_A * np], A01 = cf0[(CF_A + 1) * np], A02 = cf0[(CF_A + 2) * np];
+  const double A11 = cf0[(CF_A + 3) * np], A12 = cf0[(CF_A + 4) * np], A22 = cf0[(CF_A + 5) * np];
+  const double epsh = a.c.eps_ko * g.inv_h;
+  const double ih = g.inv_h, ih2 = g.inv_h * g.inv_h;
+  const bool diss = a.c.has_diss;
+#pragma unroll 1
+  for (int v = 0; v < NC; ++v) {
+    const double* ph = w0 + v * sv;
+    const double* pP = ph + NC * sv;
+    const double Pi = pP[0];
+    const double dh1 = ih * st_d1<Q>(ph, s1), dh2 = ih * st_d1<Q>(ph, s2), dh3 = ih * st_d1<Q>(ph, s3);
+    const double dP1 = ih * st_d1<Q>(pP, s1), dP2 = ih * st_d1<Q>(pP, s2), dP3 = ih * st_d1<Q>(pP, s3);
+    double s = b0 * dP1 + b1 * dP2 + b2 * dP3 + divb * Pi + dA0 * dh1 + dA1 * dh2 + dA2 * dh3;
+    s += A00 * (ih2 * st_d2<Q>(ph, s1)) + A11 * (ih2 * st_d2<Q>(ph, s2)) + A22 * (ih2 * st_d2<Q>(ph, s3));
+    const double dxy = ih2 * st_mixed<Q>(ph, s1, s2);
+    const double dxz = ih2 * st_mixed<Q>(ph, s1, s3);
+    const double dyz = ih2 * st_mixed<Q>(ph, s2, s3);
+    s += 2.0 * (A01 * dxy + A02 * dxz + A12 * dyz);
+    if (diss) s += epsh * (st_ko<Q>(pP, s1) + st_ko<Q>(pP, s2) + st_ko<Q>(pP, s3));
+    const double f = wl * s + cf0[(CF_C1 + v) * np];
+    emit<EPI>(a, TGH_OUT_ARGS, si0 + (NC + v) * psz, wi0 + (NC + v) * sv, f);
   }
 }
 
@@ -373,8 +614,9 @@ struct FillArgs {
 __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
   const Geom& g = a.g;
   const int S = g.S, G = g.G;
-  const int b = blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int b, wx;
+  if (!wg_map((g.S * g.S * g.S + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
   if (p >= S * S * S) return;
   const int l[3] = {p % S - G, (p / S) % S - G, p / (S * S) - G};
   int owner, loc[3];
@@ -405,8 +647,9 @@ struct CopyArgs {
 __global__ __launch_bounds__(256) void k_scatter(CopyArgs a) {
   const Geom& g = a.g;
   const int P = g.P, S = g.S, G = g.G;
-  const int b = blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int b, wx;
+  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
   if (p >= P * P * P) return;
   const int i = p % P, j = (p / P) % P, k = p / (P * P);
   const long sv = (long)S * S * S, psz = (long)P * P * P;
@@ -433,8 +676,9 @@ struct SampleArgs {
 __global__ __launch_bounds__(256) void k_sample_gauge(SampleArgs a) {
   const Geom& g = a.g;
   const int P = g.P;
-  const int b = blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int b, wx;
+  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
   if (p >= P * P * P) return;
   const int i = p % P, j = (p / P) % P, k = p / (P * P);
   const long psz = (long)P * P * P;
@@ -461,8 +705,9 @@ struct PointArgs {
 __global__ __launch_bounds__(256) void k_pointwise(PointArgs a) {
   const Geom& g = a.g;
   const int P = g.P;
-  const int b = blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int b, wx;
+  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
   if (p >= P * P * P) return;
   const int i = p % P, j = (p / P) % P, k = p / (P * P);
   const long psz = (long)P * P * P;
@@ -497,8 +742,9 @@ __device__ __forceinline__ bool is_evolved(const CaseP& c, const double x[3]) {
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
   const Geom& g = a.g;
   const int P = g.P;
-  const int b = blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int b, wx;
+  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
   const long psz = (long)P * P * P;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0;
   if (p < P * P * P) {
@@ -558,7 +804,7 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
     __syncthreads();
   }
   if (tid == 0) {
-    double* o = a.out + 3 * ((long)b * gridDim.x + blockIdx.x);
+    double* o = a.out + 3 * ((long)b * ((g.P * g.P * g.P + 255) / 256) + wx);
     o[0] = sh[0][0];
     o[1] = sh[1][0];
     o[2] = sh[2][0];
