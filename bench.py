@@ -37,6 +37,13 @@ FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 
+# tgh_rk4 flags per form (include/tgh.h)
+FORMS = {"brick-split": 8, "brick-fused": 2, "blocks-fused": 4 | 2, "blocks-split": 4 | 8}
+KERNEL = {"brick-fused": "k_brick_rhs<4, KS, hole>", "blocks-fused": "k_rhs<4, KS, hole, RK4 epilogue>",
+          "brick-split:head": "k_brick_head<4, KS, hole>", "brick-split:pi": "k_brick_pi<4>",
+          "blocks-split:head": "k_head<4, KS, hole, RK4 epilogue>", "blocks-split:pi": "k_pi<4, RK4 epilogue>"}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,8 +51,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--nblocks", type=int, default=16, help="blocks per axis (16 = the full workload)")
-    ap.add_argument("--form", choices=("split", "fused"), default="split",
-                    help="the right-hand side as head + Pi launches, or one fused kernel")
+    ap.add_argument("--form", choices=tuple(FORMS), default="brick-split",
+                    help="the stepper's layout and kernels: the brick layout (default), or the "
+                         "per-block work arrays with one fused kernel or head + Pi launches")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain (verbs) form")
     return ap.parse_args()
 
@@ -137,7 +145,7 @@ def main():
     nb = args.nblocks ** 3
     points = nb * 16 ** 3
 
-    form = 2 if args.form == "fused" else 0
+    form = FORMS[args.form]
     # warm-up (also captures the graph)
     tgh._check(L.tgh_rk4(ptr, uptr, 0.0, dt, args.warmup, 1 | form))
     torch.cuda.synchronize()
@@ -189,16 +197,16 @@ def main():
     f_pi = f_rhs - f_head
     # Per launch: F per point of that kernel's half over the N^3 points of each
     # block, plus its half of a quarter of the stage combination.
-    if args.form == "fused":
+    if not args.form.endswith("split"):
         head_tf = (f_rhs + f_combine / 4) * points / (head_ms * 1e-3) / 1e12
         pi_tf = None
-        dom = "fused"
+        dom = args.form
         achieved = head_tf
     else:
         head_tf = (f_head + f_combine / 8) * points / (head_ms * 1e-3) / 1e12
         pi_tf = (f_pi + f_combine / 8) * points / (pi_ms * 1e-3) / 1e12
-        dom = "head" if head_ms >= pi_ms else "pi"
-        achieved = head_tf if dom == "head" else pi_tf
+        dom = args.form + (":head" if head_ms >= pi_ms else ":pi")
+        achieved = head_tf if dom.endswith("head") else pi_tf
     step_ms = el / args.steps * 1e3
     t_roof = max((4 * f_rhs + f_combine) / (FP64_PEAK_TFLOPS * 1e12), B_POINT_UPDATE / (HBM_PEAK_GBS * 1e9))
     result = {
@@ -220,9 +228,8 @@ def main():
                    "blocks": nb, "points": points, "q": 4, "dt": dt},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": {"head": "k_head<4, KS, hole, RK4 epilogue>", "pi": "k_pi<4, RK4 epilogue>",
-                                "fused": "k_rhs<4, KS, hole, RK4 epilogue>"}[dom],
-                     "kernel_ms": pi_ms if dom == "pi" else head_ms,
+                     "kernel": KERNEL[dom],
+                     "kernel_ms": pi_ms if dom.endswith("pi") else head_ms,
                      "head_ms": head_ms, "head_tflops": head_tf, "pi_ms": pi_ms, "pi_tflops": pi_tf,
                      "fill_ms": fill_ms, "flops_per_point_rhs": f_rhs, "flops_per_point_head": f_head,
                      "flops_per_point_update": 4 * f_rhs + f_combine,

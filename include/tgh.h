@@ -53,9 +53,15 @@ enum { /* interior variants, src/interior.jl:178-238 */
   TGH_INTERIOR_PASTED = 3
 };
 enum { /* tgh_rk4 flags */
-  TGH_RK4_GRAPH = 1, /* capture the step in a hipGraph and replay it */
-  TGH_RK4_FUSED = 2  /* one kernel per right-hand side (default: the head and
-                        Pi-component halves as two launches) */
+  TGH_RK4_GRAPH = 1,  /* capture the step in a hipGraph and replay it */
+  TGH_RK4_FUSED = 2,  /* one kernel per right-hand side */
+  TGH_RK4_BLOCKS = 4, /* run the stages on the per-block ghosted work arrays
+                         (a ghost fill per stage) instead of the default brick
+                         layout (every point once, a frame of ghosts around
+                         the whole brick) */
+  TGH_RK4_SPLIT = 8   /* the right-hand side as two launches: the head (with
+                         d_t h) and the ten Pi components; without FUSED or
+                         SPLIT the layout's faster form is used */
 };
 
 /* The case and its uniform forest: GHCase (src/initialdata.jl:158-311) plus
@@ -141,8 +147,13 @@ int tgh_initial_state(tgh_problem* p, double* u, double t);
 int tgh_add_noise(tgh_problem* p, double* u, uint64_t base, double amplitude);
 /* gh_solve / IRK.step! x nsteps — src/stepping.jl:208-239: classical RK4 with
  * the limiter on every stage value and on the result, fixed dt, in place.
- * The stage update is fused into the right-hand side's epilogue. On return
- * u is settled (face copies equal their primaries). */
+ * The stage update is fused into the right-hand side's epilogue. The stages
+ * run on a copy of u in the brick layout (each mesh point stored once) unless
+ * TGH_RK4_BLOCKS is set; either way every owned point of u, face copies
+ * included, advances as the reference steps its state vector (a copy by its
+ * primary's increment). The first call in the brick layout allocates its
+ * arrays (4 x 20 doubles per point of the ghosted brick, and 24 more with
+ * TGH_RK4_SPLIT). */
 int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t flags);
 /* gh_error! + error_norms — src/constraints.jl:311-396,629-671: masked
  * volume-weighted L2 and Linf of u - u_exact(t) over evolved primaries. */

@@ -222,7 +222,13 @@ def test_dirichlet_hook_is_the_exact_state_at_t(tgh):
     assert float(np.abs(o.work() - wo).max()) > 1e-2
 
 
-def test_rk4_config1_ten_steps_match_oracle(tgh):
+# tgh_rk4's forms (include/tgh.h flags): the brick layout and the block
+# layout, each with the right-hand side as one fused kernel or as head + Pi.
+RK4_FORMS = {"brick-fused": 2, "brick-split": 8, "blocks-split": 4 | 8, "blocks-fused": 4 | 2}
+
+
+@pytest.mark.parametrize("form", list(RK4_FORMS))
+def test_rk4_config1_ten_steps_match_oracle(tgh, form):
     """The CPU-runnable workload: Minkowski + 1e-6 noise, one periodic 16^3
     block, 10 RK4 steps at cfl 1/4."""
     torch = torch_cuda()
@@ -238,13 +244,14 @@ def test_rk4_config1_ten_steps_match_oracle(tgh):
     dt = 0.25 * (1 / 16) / lam_o
     for n in range(10):
         o.rk4_step(u, n * dt, dt, NTHREADS)
-    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 10, 0))
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 10, RK4_FORMS[form]))
     diff = float(np.abs(ud.cpu().numpy() - u).max())
-    print(f"config 1, 10 RK4 steps: max |GPU - oracle| = {diff:.3e}")
+    print(f"config 1, 10 RK4 steps ({form}): max |GPU - oracle| = {diff:.3e}")
     assert diff <= 1e-12 * max(1.0, float(np.abs(u).max()))
 
 
-def test_rk4_kerr_schild_two_steps_match_oracle(tgh):
+@pytest.mark.parametrize("form", list(RK4_FORMS))
+def test_rk4_kerr_schild_two_steps_match_oracle(tgh, form):
     torch = torch_cuda()
     desc = ks_small()
     p = tgh_problem_from_desc(tgh, desc)
@@ -256,18 +263,19 @@ def test_rk4_kerr_schild_two_steps_match_oracle(tgh):
     dt = 0.25 * (1 / 16) / lam
     for n in range(2):
         o.rk4_step(u, n * dt, dt, NTHREADS)
-    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, 0))
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, RK4_FORMS[form]))
     g = ud.cpu().numpy()
     scale = np.maximum(np.abs(u), 1.0)
     diff = float((np.abs(g - u) / scale).max())
-    print(f"KS hole, 2 RK4 steps: max rel diff = {diff:.3e}")
+    print(f"KS hole, 2 RK4 steps ({form}): max rel diff = {diff:.3e}")
     assert diff <= 1e-12
     l2g, lig = tgh.error_norms(p, ud, 2 * dt)
     l2o, lio = o.error_norms(u, 2 * dt)
     assert abs(l2g - l2o) <= 1e-9 * l2o and abs(lig - lio) <= 1e-9 * lio
 
 
-def test_fused_stepper_equals_verbs_and_graph_replay(tgh):
+@pytest.mark.parametrize("form", list(RK4_FORMS))
+def test_fused_stepper_equals_verbs_and_graph_replay(tgh, form):
     """The fused RK4 (stage update in the RHS epilogue) against RK4 written
     with the verbs, and the hipGraph replay against the launch sequence."""
     torch = torch_cuda()
@@ -276,12 +284,13 @@ def test_fused_stepper_equals_verbs_and_graph_replay(tgh):
     o = oracle_lib.OracleProblem(desc)
     u0 = torch.from_numpy(o.fill_exact(0.0)).cuda()
     dt = 0.01
+    flags = RK4_FORMS[form]
     a = u0.clone()
-    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(a.data_ptr()), 0.0, dt, 3, 0))
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(a.data_ptr()), 0.0, dt, 3, flags))
     b = u0.clone()
     for rep in range(2):  # replays, with eager work in between
         b.copy_(u0)
-        tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(b.data_ptr()), 0.0, dt, 3, 1))
+        tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(b.data_ptr()), 0.0, dt, 3, flags | 1))
         torch.cuda.synchronize()
         assert torch.equal(a, b), f"graph replay {rep} differs"
     c = u0.clone()
@@ -404,3 +413,54 @@ def test_without_dissipation_the_noise_grows(tgh):
     t_end, r2, ri, li, fin = _noise_growth(tgh, 0.0)
     print(f"eps=0: t_end={t_end:.2f} l2 x{r2:.3f} linf x{ri:.3f}")
     assert fin and r2 > 2 and li < 1e-5
+
+
+@pytest.mark.parametrize("brick_form", ["brick-fused", "brick-split"])
+@pytest.mark.parametrize("case", ["shifted-mixed-frame", "gauge-wave-dirichlet", "ks-hole"])
+def test_brick_stepper_equals_block_stepper(tgh, case, brick_form):
+    """The brick layout (every point once, a ghost frame around the brick)
+    against the block layout (per-block ghosts, a fill per stage), 3 steps:
+    a Dirichlet face beside periodic ones with a sampled gauge source, a
+    moving Dirichlet hook (the frame refilled at every stage time), and the
+    hole with its static frame, replayed from a hipGraph."""
+    torch = torch_cuda()
+    if case == "shifted-mixed-frame":
+        desc = TD.make_desc(q=4, N=8, nblocks=(2, 2, 2), periodic=(0, 1, 1), lo=(-2, -2, -2), hi=(2, 2, 2),
+                            background=TD.TGH_BG_SHIFTED_MINKOWSKI, gauge=TD.TGH_GAUGE_SAMPLED,
+                            wave_amp=0.5, wave_len=2.0, gamma0=1.0, gamma2=-0.5, eps_ko=0.5)
+        flags = 0
+    elif case == "gauge-wave-dirichlet":
+        desc = TD.make_desc(q=4, N=8, nblocks=(2, 1, 1), periodic=(0, 0, 0), lo=(0, 0, 0), hi=(2, 1, 1),
+                            background=TD.TGH_BG_GAUGE_WAVE, wave_amp=1 / 20, wave_len=2.0, eps_ko=0.5)
+        flags = 0
+    else:
+        desc = ks_small()
+        flags = 1
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = o.fill_exact(0.0)
+    o.add_noise(u, 77, 1e-7)
+    u0 = torch.from_numpy(u).cuda()
+    lam = tgh.max_speed(p, u0, 0.0)
+    dt = 0.25 * (desc.hi[0] - desc.lo[0]) / (desc.nblocks[0] * desc.N) / lam
+    a, b = u0.clone(), u0.clone()
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(a.data_ptr()), 0.1, dt, 3,
+                                   RK4_FORMS[brick_form] | flags))
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(b.data_ptr()), 0.1, dt, 3,
+                                   RK4_FORMS["blocks-fused"] | flags))
+    scale = torch.clamp(b.abs(), min=1.0)
+    diff = float(((a - b).abs() / scale).max())
+    print(f"{case}: {brick_form} vs blocks, 3 steps: max rel diff = {diff:.3e}")
+    assert diff <= 1e-13
+
+
+def test_rk4_flags_are_checked(tgh):
+    torch = torch_cuda()
+    desc = TD.minkowski_noise_case(N=8, q=4)
+    p = tgh_problem_from_desc(tgh, desc)
+    u = torch.zeros(p.state_length, dtype=torch.float64, device="cuda")
+    L = tgh._load()
+    assert L.tgh_rk4(p.ptr, tgh.C.c_void_p(u.data_ptr()), 0.0, 0.01, 1, 2 | 8) == 1  # FUSED with SPLIT
+    assert "exclude" in L.tgh_last_error().decode()
+    assert L.tgh_rk4(p.ptr, tgh.C.c_void_p(u.data_ptr()), 0.0, 0.01, 1, 64) == 1
+    assert "unknown" in L.tgh_last_error().decode()

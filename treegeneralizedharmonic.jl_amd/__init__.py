@@ -52,6 +52,8 @@ class TGHError(RuntimeError):
 
 TGH_OK, TGH_EINVAL, TGH_EDEVICE, TGH_ENOTFINITE, TGH_EUNSUPPORTED = 0, 1, 2, 3, 4
 TGH_RK4_GRAPH = 1
+TGH_RK4_FUSED = 2
+TGH_RK4_BLOCKS = 4
 
 
 class TghDesc(C.Structure):

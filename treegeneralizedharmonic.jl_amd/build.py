@@ -10,12 +10,16 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libtgh_mi355x.so")
-SOURCES = [os.path.join(HERE, "csrc", "tgh_api.hip")]
-DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("tgh_device.hpp", "tgh_kernels.hpp")] + \
+# The API and pointwise kernels, and the right-hand side at each order in a
+# translation unit of its own: the five compile in parallel.
+SOURCES = [os.path.join(HERE, "csrc", f) for f in
+           ("tgh_api.hip", "tgh_q2.hip", "tgh_q4.hip", "tgh_q6.hip", "tgh_q8.hip")]
+DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in
+                  ("tgh_device.hpp", "tgh_kernels.hpp", "tgh_problem.hpp", "tgh_launch.hpp")] + \
     [os.path.join(ROOT, "include", "tgh.h")]
 
 HIPCC_FLAGS = [
-    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
 ]
@@ -29,14 +33,33 @@ def hipcc():
 
 
 def build(verbose=False):
+    """Compile every translation unit (in parallel) and link the library."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        cmd = [hipcc()] + HIPCC_FLAGS + ["-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    failed = []
+    for src, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(os.path.basename(src))
+    if failed:
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)} building {LIB}")
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + SOURCES
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode}) building {LIB}")
+        raise RuntimeError(f"linking {LIB} failed ({res.returncode})")
     os.replace(tmp, LIB)
     return LIB
 

@@ -228,12 +228,6 @@ struct RhsArgs {
   double* __restrict__ coef;
 };
 
-// Coefficient slots of the split form, each an array over the (N+1)^3 x nb
-// owned points: beta^i, d_i beta^i, d_i A^{ij}, A^{ij} (xx,xy,xz,yy,yz,zz),
-// the layer weight w (-1 marks the frozen core, where F is not evaluated),
-// then per Pi component c1_v = w msrc_v - rho (Pi_v - Pi_exact,v), so that
-// dPi_v = w (principal part) + c1_v.
-enum { CF_B0 = 0, CF_DIVB = 3, CF_DIVA = 4, CF_A = 7, CF_W = 13, CF_C1 = 14, CF_N = 24 };
 
 // The epilogue: store F (the rhs! verb) or apply RK4 stage s to it: with k = F,
 //   s = 1: acc = u + c_acc k,  Y = u + c_next k
@@ -267,45 +261,60 @@ __device__ __forceinline__ void emit(const RhsArgs& a, TGH_OUT_PARAMS, long si, 
   }
 }
 
-// The fused right-hand side at one owned point per thread
-// (gh_rhs_kernel!/gh_rhs_point!, src/evolution.jl:448-695,846-863) in the
-// streaming order of src/evolution.jl:258-388: the state, the 30 first
-// derivatives and the coefficients; d_t h complete; the source; then the ten
-// Pi components one at a time.
-template <int Q, int GAUGE, bool HOLE, int EPI>
-__global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict__ W, TGH_OUT_PARAMS) {
-  const Geom& g = a.g;
-  const int P = g.P, S = g.S, G = g.G;
-  int b, wx;
-  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
-  const int p = wx * blockDim.x + threadIdx.x;
-  if (p >= P * P * P) return;
-  const int i = p % P, j = (p / P) % P, k = p / (P * P);
-  const long sv = (long)S * S * S;
-  const int s1 = 1, s2 = S, s3 = S * S;
-  const double* __restrict__ Wb = W + (long)b * 20 * sv;
-  const long off = (long)(i + G) + (long)S * ((j + G) + (long)S * (k + G));
-  const double* __restrict__ w0 = Wb + off;
-  const long psz = (long)P * P * P;
-  const long si0 = (long)b * 20 * psz + p;
-  const long wi0 = (long)b * 20 * sv + off;
-
+// Where one point's inputs and outputs live: the stage input's center w0 with
+// its field stride sv and row and plane strides s2, s3 (the x stride is 1);
+// the epilogue's state index si0 with field stride ssv (us, acc, du) and work
+// index wi0 with field stride sv (Wn); the sampled gauge source Hp with field
+// stride hs; the matter fields (rho, S_i, S_ij) with field stride ms.
+struct PointRef {
+  const double* __restrict__ w0;
+  long sv;
+  int s2, s3;
   double x[3];
-  position(g, b, i, j, k, x);
+  long si0, ssv, wi0;
+  const double* __restrict__ Hp;
+  long hs;
+  const double* __restrict__ m_rho;
+  const double* __restrict__ m_S;
+  const double* __restrict__ m_Sij;
+  long ms;
+};
+
+// What the Pi components need from the head at a point: beta^i, d_i beta^i,
+// d_i A^{ij}, A^{ij} = alpha sqrt(gamma) gamma^{ij} (xx,xy,xz,yy,yz,zz), the
+// layer weight w and c1_v = w msrc_v - rho (Pi_v - Pi_exact,v), so that
+//   dPi_v = w (principal part of component v) + c1_v.
+struct PiCoef {
+  double b[3], divb, divA[3], A[6], w;
+  double c1[NC];
+};
+
+// The head at one point (gh_rhs_head, src/evolution.jl:258-297, inside
+// gh_rhs_point!, src/evolution.jl:448-695,846-863): the state, the 30 first
+// derivatives and the coefficients, d_t h complete (through the epilogue), the
+// source, the interior layer's weights. Returns false in the frozen core,
+// where F is not evaluated (a hole at rest: d_t h = 0 is emitted).
+template <int Q, int GAUGE, bool HOLE, int EPI>
+__device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr, double inv_h, PiCoef& pc,
+                                           TGH_OUT_PARAMS) {
+  const double* __restrict__ w0 = pr.w0;
+  const long sv = pr.sv, psz = pr.ssv, si0 = pr.si0, wi0 = pr.wi0;
+  const int s1 = 1, s2 = pr.s2, s3 = pr.s3;
+  const double* x = pr.x;
   const CaseP& c = a.c;
   double wgt = 1.0, rho = 0.0, r = 0.0;
   if (HOLE) {
     r = hole_radius(c, x);
     const bool frozen = c.interior == INT_PASTED ? r < c.r1 : r < c.r0;
-    if (frozen) {  // F is not evaluated in the core (a hole at rest: du = 0)
+    if (frozen) {
 #pragma unroll 1
-      for (int v = 0; v < 20; ++v) emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, 0.0);
-      return;
+      for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, 0.0);
+      return false;
     }
   }
   const double g0 = damping_rate(c, x);
-  const double epsh = c.eps_ko * g.inv_h;
-  const double ih = g.inv_h, ih2 = g.inv_h * g.inv_h;
+  const double epsh = c.eps_ko * inv_h;
+  const double ih = inv_h;
 
   double hv[NC], D[4][NC];
 #pragma unroll
@@ -331,44 +340,40 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict
   {
     double Hl[4] = {0.0, 0.0, 0.0, 0.0}, dH[4][4];
     if (GAUGE == GAUGE_SAMPLED) {
-      const double* Hp = a.Hsrc + si0;
+      const double* Hp = pr.Hp;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Hl[q] = Hp[q * psz];
+      for (int q = 0; q < 4; ++q) Hl[q] = Hp[q * pr.hs];
 #pragma unroll
-      for (int n = 0; n < 16; ++n) dH[n % 4][n / 4] = Hp[(4 + n) * psz];
+      for (int n = 0; n < 16; ++n) dH[n % 4][n / 4] = Hp[(4 + n) * pr.hs];
     } else if (GAUGE == GAUGE_KS) {
       ks_source(c.bg.M, c.bg.a, hv, D, Hl, dH);
     }
     gh_source<GAUGE != GAUGE_NONE>(cf, hv, D, Hl, dH, g0, c.g2, msrc);
   }
-  if (a.m_rho != nullptr) {
-    const long mi = (long)b * psz + p;
-    const double rr = a.m_rho[mi];
-    const double Sm[3] = {a.m_S[(long)b * 3 * psz + p], a.m_S[(long)b * 3 * psz + psz + p],
-                          a.m_S[(long)b * 3 * psz + 2 * psz + p]};
+  if (pr.m_rho != nullptr) {
+    const double rr = pr.m_rho[0];
+    const double Sm[3] = {pr.m_S[0], pr.m_S[pr.ms], pr.m_S[2 * pr.ms]};
     double Sij[6];
 #pragma unroll
-    for (int n = 0; n < 6; ++n) Sij[n] = a.m_Sij[(long)b * 6 * psz + n * psz + p];
+    for (int n = 0; n < 6; ++n) Sij[n] = pr.m_Sij[n * pr.ms];
     double ms[NC];
     matter_source(cf, hv, rr, Sm, Sij, ms);
 #pragma unroll
     for (int n = 0; n < NC; ++n) msrc[n] += ms[n];
   }
-  double divb, divA[3];
-  metric_divergences(cf, &D[1], divb, divA);
+  metric_divergences(cf, &D[1], pc.divb, pc.divA);
   const double as = cf.alpha * cf.sqrtg;
-  const double A00 = as * cf.gamu[0], A01 = as * cf.gamu[1], A02 = as * cf.gamu[2];
-  const double A11 = as * cf.gamu[3], A12 = as * cf.gamu[4], A22 = as * cf.gamu[5];
-  const double b0 = cf.beta[0], b1 = cf.beta[1], b2 = cf.beta[2];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) pc.A[n] = as * cf.gamu[n];
+  pc.b[0] = cf.beta[0];
+  pc.b[1] = cf.beta[1];
+  pc.b[2] = cf.beta[2];
 
   if (HOLE) {
     if (r < c.r1) {
-      double w_, rho_;
       const double sl = (r - c.r0) / (c.r1 - c.r0);
-      w_ = smoothstep(sl / c.w_ramp);
-      rho_ = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
-      wgt = w_;
-      rho = rho_;
+      wgt = smoothstep(sl / c.w_ramp);
+      rho = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
     }
   }
   double ue[20];
@@ -381,228 +386,173 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict
     if (layer) f = wgt * f - rho * (hv[v] - ue[v]);
     emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, f);
   }
+  pc.w = layer ? wgt : 1.0;
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    double c1 = msrc[v];
+    if (layer) c1 = wgt * c1 - rho * (w0[(NC + v) * sv] - ue[NC + v]);
+    pc.c1[v] = c1;
+  }
+  return true;
+}
+
+// The principal part of Pi component v (gh_rhs_pi, src/evolution.jl:324-354)
+// from the stencils of h_v (ph) and Pi_v (pP), with strides 1, s2, s3 --
+// of a work array or of a staged copy in LDS:
+//   beta^i d_i Pi + d_i beta^i Pi + d_i A^{ij} d_j h + A^{ij} d_i d_j h + KO(Pi).
+template <int Q>
+__device__ __forceinline__ double pi_principal(const double* ph, const double* pP, int s2, int s3,
+                                               const double b[3], double divb, const double dA[3],
+                                               const double A[6], double ih, double ih2, double epsh,
+                                               bool diss) {
+  const int s1 = 1;
+  const double Pi = pP[0];
+  const double dh1 = ih * st_d1<Q>(ph, s1), dh2 = ih * st_d1<Q>(ph, s2), dh3 = ih * st_d1<Q>(ph, s3);
+  const double dP1 = ih * st_d1<Q>(pP, s1), dP2 = ih * st_d1<Q>(pP, s2), dP3 = ih * st_d1<Q>(pP, s3);
+  double s = b[0] * dP1 + b[1] * dP2 + b[2] * dP3 + divb * Pi + dA[0] * dh1 + dA[1] * dh2 + dA[2] * dh3;
+  s += A[0] * (ih2 * st_d2<Q>(ph, s1)) + A[3] * (ih2 * st_d2<Q>(ph, s2)) + A[5] * (ih2 * st_d2<Q>(ph, s3));
+  const double dxy = ih2 * st_mixed<Q>(ph, s1, s2);
+  const double dxz = ih2 * st_mixed<Q>(ph, s1, s3);
+  const double dyz = ih2 * st_mixed<Q>(ph, s2, s3);
+  s += 2.0 * (A[1] * dxy + A[2] * dxz + A[4] * dyz);
+  if (diss) s += epsh * (st_ko<Q>(pP, s1) + st_ko<Q>(pP, s2) + st_ko<Q>(pP, s3));
+  return s;
+}
+
+// The fused right-hand side at one point in the streaming order of
+// src/evolution.jl:258-388: the head, then the ten Pi components one at a time.
+template <int Q, int GAUGE, bool HOLE, int EPI>
+__device__ __forceinline__ void rhs_point(const RhsArgs& a, const PointRef& pr, double inv_h, TGH_OUT_PARAMS) {
+  PiCoef pc;
+  const long sv = pr.sv, psz = pr.ssv;
+  if (!head_point<Q, GAUGE, HOLE, EPI>(a, pr, inv_h, pc, TGH_OUT_ARGS)) {
+#pragma unroll 1
+    for (int v = NC; v < 20; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * psz, pr.wi0 + v * sv, 0.0);
+    return;
+  }
+  const double ih = inv_h, ih2 = inv_h * inv_h, epsh = a.c.eps_ko * inv_h;
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
-    const double* ph = w0 + v * sv;
-    const double* pP = ph + NC * sv;
-    const double Pi = pP[0];
-    const double dh1 = ih * st_d1<Q>(ph, s1), dh2 = ih * st_d1<Q>(ph, s2), dh3 = ih * st_d1<Q>(ph, s3);
-    const double dP1 = ih * st_d1<Q>(pP, s1), dP2 = ih * st_d1<Q>(pP, s2), dP3 = ih * st_d1<Q>(pP, s3);
-    double s = b0 * dP1 + b1 * dP2 + b2 * dP3 + divb * Pi + divA[0] * dh1 + divA[1] * dh2 + divA[2] * dh3;
-    s += A00 * (ih2 * st_d2<Q>(ph, s1)) + A11 * (ih2 * st_d2<Q>(ph, s2)) + A22 * (ih2 * st_d2<Q>(ph, s3));
-    const double dxy = ih2 * st_mixed<Q>(ph, s1, s2);
-    const double dxz = ih2 * st_mixed<Q>(ph, s1, s3);
-    const double dyz = ih2 * st_mixed<Q>(ph, s2, s3);
-    s += 2.0 * (A01 * dxy + A02 * dxz + A12 * dyz);
-    if (c.has_diss) s += epsh * (st_ko<Q>(pP, s1) + st_ko<Q>(pP, s2) + st_ko<Q>(pP, s3));
-    double f = s + msrc[v];
-    if (layer) f = wgt * f - rho * (Pi - ue[NC + v]);
-    emit<EPI>(a, TGH_OUT_ARGS, si0 + (NC + v) * psz, wi0 + (NC + v) * sv, f);
+    const double* ph = pr.w0 + v * sv;
+    const double s = pi_principal<Q>(ph, ph + NC * sv, pr.s2, pr.s3, pc.b, pc.divb, pc.divA, pc.A, ih, ih2,
+                                     epsh, a.c.has_diss);
+    emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + (NC + v) * psz, pr.wi0 + (NC + v) * sv, pc.w * s + pc.c1[v]);
   }
 }
 
-// The split form, first half: gh_rhs_head (src/evolution.jl:258-297) at one
-// owned point per thread -- the state, the 30 first derivatives, the
-// coefficient set, d_t h complete (stored through the epilogue), the source --
-// leaving the Pi components' coefficients in a.coef for k_pi. The frozen core
-// is decided here, before any stencil is read.
+// Coefficient slots of the split form, each an array of cs doubles indexed
+// like the points: the PiCoef fields in order (w = -1 marks the frozen core).
+enum { CF_B0 = 0, CF_DIVB = 3, CF_DIVA = 4, CF_A = 7, CF_W = 13, CF_C1 = 14, CF_N = 24 };
+__device__ __forceinline__ void store_coef(double* __restrict__ cf0, long cs, const PiCoef& pc) {
+#pragma unroll
+  for (int n = 0; n < 3; ++n) cf0[(CF_B0 + n) * cs] = pc.b[n];
+  cf0[CF_DIVB * cs] = pc.divb;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) cf0[(CF_DIVA + n) * cs] = pc.divA[n];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) cf0[(CF_A + n) * cs] = pc.A[n];
+  cf0[CF_W * cs] = pc.w;
+#pragma unroll
+  for (int n = 0; n < NC; ++n) cf0[(CF_C1 + n) * cs] = pc.c1[n];
+}
+__device__ __forceinline__ void load_coef(const double* __restrict__ cf0, long cs, PiCoef& pc) {
+#pragma unroll
+  for (int n = 0; n < 3; ++n) pc.b[n] = cf0[(CF_B0 + n) * cs];
+  pc.divb = cf0[CF_DIVB * cs];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) pc.divA[n] = cf0[(CF_DIVA + n) * cs];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) pc.A[n] = cf0[(CF_A + n) * cs];
+  pc.w = cf0[CF_W * cs];
+}
+
+// The block layout: one owned point of one block per thread.
+__device__ __forceinline__ bool block_point(const RhsArgs& a, const double* __restrict__ W, PointRef& pr,
+                                            long& pidx) {
+  const Geom& g = a.g;
+  const int P = g.P, S = g.S, G = g.G;
+  int b, wx;
+  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return false;
+  const int p = wx * blockDim.x + threadIdx.x;
+  if (p >= P * P * P) return false;
+  const int i = p % P, j = (p / P) % P, k = p / (P * P);
+  const long sv = (long)S * S * S, psz = (long)P * P * P;
+  const long off = (long)(i + G) + (long)S * ((j + G) + (long)S * (k + G));
+  pr.w0 = W + (long)b * 20 * sv + off;
+  pr.sv = sv;
+  pr.s2 = S;
+  pr.s3 = S * S;
+  position(g, b, i, j, k, pr.x);
+  pr.si0 = (long)b * 20 * psz + p;
+  pr.ssv = psz;
+  pr.wi0 = (long)b * 20 * sv + off;
+  pr.Hp = a.Hsrc ? a.Hsrc + pr.si0 : nullptr;
+  pr.hs = psz;
+  pr.m_rho = a.m_rho ? a.m_rho + (long)b * psz + p : nullptr;
+  pr.m_S = a.m_S ? a.m_S + (long)b * 3 * psz + p : nullptr;
+  pr.m_Sij = a.m_Sij ? a.m_Sij + (long)b * 6 * psz + p : nullptr;
+  pr.ms = psz;
+  pidx = (long)b * psz + p;
+  return true;
+}
+
+template <int Q, int GAUGE, bool HOLE, int EPI>
+__global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict__ W, TGH_OUT_PARAMS) {
+  PointRef pr;
+  long pi;
+  if (!block_point(a, W, pr, pi)) return;
+  rhs_point<Q, GAUGE, HOLE, EPI>(a, pr, a.g.inv_h, TGH_OUT_ARGS);
+}
+
+// The split form, first half: the head at one owned point per thread, leaving
+// the Pi components' coefficients in coef (CF_N arrays of nblocks (N+1)^3).
 template <int Q, int GAUGE, bool HOLE, int EPI>
 __global__ __launch_bounds__(256) void k_head(RhsArgs a, const double* __restrict__ W,
                                               double* __restrict__ coef, TGH_OUT_PARAMS) {
-  const Geom& g = a.g;
-  const int P = g.P, S = g.S, G = g.G;
-  int b, wx;
-  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
-  const int p = wx * blockDim.x + threadIdx.x;
-  if (p >= P * P * P) return;
-  const int i = p % P, j = (p / P) % P, k = p / (P * P);
-  const long sv = (long)S * S * S;
-  const int s1 = 1, s2 = S, s3 = S * S;
-  const double* __restrict__ Wb = W + (long)b * 20 * sv;
-  const long off = (long)(i + G) + (long)S * ((j + G) + (long)S * (k + G));
-  const double* __restrict__ w0 = Wb + off;
-  const long psz = (long)P * P * P;
-  const long si0 = (long)b * 20 * psz + p;
-  const long wi0 = (long)b * 20 * sv + off;
-
-  double x[3];
-  position(g, b, i, j, k, x);
-  const CaseP& c = a.c;
-  double wgt = 1.0, rho = 0.0, r = 0.0;
-  if (HOLE) {
-    r = hole_radius(c, x);
-    const bool frozen = c.interior == INT_PASTED ? r < c.r1 : r < c.r0;
-    if (frozen) {  // F is not evaluated in the core (a hole at rest: du = 0)
-#pragma unroll 1
-      for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, 0.0);
-      coef[(long)CF_W * g.nblocks * psz + (long)b * psz + p] = -1.0;
-      return;
-    }
-  }
-  const double g0 = damping_rate(c, x);
-  const double epsh = c.eps_ko * g.inv_h;
-  const double ih = g.inv_h, ih2 = g.inv_h * g.inv_h;
-
-  double hv[NC], D[4][NC];
-#pragma unroll
-  for (int v = 0; v < NC; ++v) {
-    const double* pv = w0 + v * sv;
-    hv[v] = pv[0];
-    D[1][v] = ih * st_d1<Q>(pv, s1);
-    D[2][v] = ih * st_d1<Q>(pv, s2);
-    D[3][v] = ih * st_d1<Q>(pv, s3);
-  }
-  Coef cf;
-  metric_quantities(hv, cf);
-  const double a_div = cf.alpha / cf.sqrtg;
-#pragma unroll
-  for (int v = 0; v < NC; ++v) {
-    const double* pv = w0 + v * sv;
-    double s = cf.beta[0] * D[1][v] + cf.beta[1] * D[2][v] + cf.beta[2] * D[3][v] +
-               a_div * pv[NC * sv];
-    if (c.has_diss) s += epsh * (st_ko<Q>(pv, s1) + st_ko<Q>(pv, s2) + st_ko<Q>(pv, s3));
-    D[0][v] = s;
-  }
-  double msrc[NC];
-  {
-    double Hl[4] = {0.0, 0.0, 0.0, 0.0}, dH[4][4];
-    if (GAUGE == GAUGE_SAMPLED) {
-      const double* Hp = a.Hsrc + si0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) Hl[q] = Hp[q * psz];
-#pragma unroll
-      for (int n = 0; n < 16; ++n) dH[n % 4][n / 4] = Hp[(4 + n) * psz];
-    } else if (GAUGE == GAUGE_KS) {
-      ks_source(c.bg.M, c.bg.a, hv, D, Hl, dH);
-    }
-    gh_source<GAUGE != GAUGE_NONE>(cf, hv, D, Hl, dH, g0, c.g2, msrc);
-  }
-  if (a.m_rho != nullptr) {
-    const long mi = (long)b * psz + p;
-    const double rr = a.m_rho[mi];
-    const double Sm[3] = {a.m_S[(long)b * 3 * psz + p], a.m_S[(long)b * 3 * psz + psz + p],
-                          a.m_S[(long)b * 3 * psz + 2 * psz + p]};
-    double Sij[6];
-#pragma unroll
-    for (int n = 0; n < 6; ++n) Sij[n] = a.m_Sij[(long)b * 6 * psz + n * psz + p];
-    double ms[NC];
-    matter_source(cf, hv, rr, Sm, Sij, ms);
-#pragma unroll
-    for (int n = 0; n < NC; ++n) msrc[n] += ms[n];
-  }
-  double divb, divA[3];
-  metric_divergences(cf, &D[1], divb, divA);
-  const double as = cf.alpha * cf.sqrtg;
-  const double A00 = as * cf.gamu[0], A01 = as * cf.gamu[1], A02 = as * cf.gamu[2];
-  const double A11 = as * cf.gamu[3], A12 = as * cf.gamu[4], A22 = as * cf.gamu[5];
-  const double b0 = cf.beta[0], b1 = cf.beta[1], b2 = cf.beta[2];
-
-  if (HOLE) {
-    if (r < c.r1) {
-      double w_, rho_;
-      const double sl = (r - c.r0) / (c.r1 - c.r0);
-      w_ = smoothstep(sl / c.w_ramp);
-      rho_ = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
-      wgt = w_;
-      rho = rho_;
-    }
-  }
-  double ue[20];
-  const bool layer = HOLE && r < c.r1;
-  if (layer) case_state(c, a.t, x, ue);  // r >= r0 here: the core rule is the identity
-
-#pragma unroll
-  for (int v = 0; v < NC; ++v) {
-    double f = D[0][v];
-    if (layer) f = wgt * f - rho * (hv[v] - ue[v]);
-    emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, f);
-  }
-  {
-    const long np = (long)g.nblocks * psz;
-    double* cf0 = coef + (long)b * psz + p;
-    cf0[CF_B0 * np] = b0;
-    cf0[(CF_B0 + 1) * np] = b1;
-    cf0[(CF_B0 + 2) * np] = b2;
-    cf0[CF_DIVB * np] = divb;
-    cf0[CF_DIVA * np] = divA[0];
-    cf0[(CF_DIVA + 1) * np] = divA[1];
-    cf0[(CF_DIVA + 2) * np] = divA[2];
-    cf0[CF_A * np] = A00;
-    cf0[(CF_A + 1) * np] = A01;
-    cf0[(CF_A + 2) * np] = A02;
-    cf0[(CF_A + 3) * np] = A11;
-    cf0[(CF_A + 4) * np] = A12;
-    cf0[(CF_A + 5) * np] = A22;
-    cf0[CF_W * np] = layer ? wgt : 1.0;
-#pragma unroll
-    for (int v = 0; v < NC; ++v) {
-      double c1 = msrc[v];
-      if (layer) c1 = wgt * c1 - rho * (w0[(NC + v) * sv] - ue[NC + v]);
-      cf0[(CF_C1 + v) * np] = c1;
-    }
-  }
+  PointRef pr;
+  long pi;
+  if (!block_point(a, W, pr, pi)) return;
+  const long cs = (long)a.g.nblocks * a.g.P * a.g.P * a.g.P;
+  PiCoef pc;
+  if (head_point<Q, GAUGE, HOLE, EPI>(a, pr, a.g.inv_h, pc, TGH_OUT_ARGS))
+    store_coef(coef + pi, cs, pc);
+  else
+    coef[CF_W * cs + pi] = -1.0;
 }
 
-// The split form, second half: the ten Pi components (gh_rhs_pi,
-// src/evolution.jl:324-354) at one owned point per thread, from the work
-// array and the coefficients k_head left: nine stencils of h_v and seven of
-// Pi_v per component, contracted and stored through the epilogue. Light on
-// registers, so many waves hide the loads.
+// The split form, second half: the ten Pi components at one owned point per
+// thread, from the work array and the coefficients k_head left.
 #ifndef TGH_PI_WAVES_PER_EU
 #define TGH_PI_WAVES_PER_EU 2
 #endif
 template <int Q, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_PI_WAVES_PER_EU))) void k_pi(RhsArgs a, const double* __restrict__ W,
-                                            const double* __restrict__ coef, TGH_OUT_PARAMS) {
-  const Geom& g = a.g;
-  const int P = g.P, S = g.S, G = g.G;
-  int b, wx;
-  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
-  const int p = wx * blockDim.x + threadIdx.x;
-  if (p >= P * P * P) return;
-  const int i = p % P, j = (p / P) % P, k = p / (P * P);
-  const long sv = (long)S * S * S;
-  const int s1 = 1, s2 = S, s3 = S * S;
-  const long off = (long)(i + G) + (long)S * ((j + G) + (long)S * (k + G));
-  const double* __restrict__ w0 = W + (long)b * 20 * sv + off;
-  const long psz = (long)P * P * P;
-  const long si0 = (long)b * 20 * psz + p;
-  const long wi0 = (long)b * 20 * sv + off;
-  const long np = (long)g.nblocks * psz;
-  const double* __restrict__ cf0 = coef + (long)b * psz + p;
-  const double wl = cf0[CF_W * np];
-  if (wl < 0.0) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_PI_WAVES_PER_EU))) void k_pi(
+    RhsArgs a, const double* __restrict__ W, const double* __restrict__ coef, TGH_OUT_PARAMS) {
+  PointRef pr;
+  long pi;
+  if (!block_point(a, W, pr, pi)) return;
+  const long cs = (long)a.g.nblocks * a.g.P * a.g.P * a.g.P;
+  const long sv = pr.sv, psz = pr.ssv;
+  const double* __restrict__ cf0 = coef + pi;
+  PiCoef pc;
+  load_coef(cf0, cs, pc);
+  if (pc.w < 0.0) {
 #pragma unroll 1
-    for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, si0 + (NC + v) * psz, wi0 + (NC + v) * sv, 0.0);
+    for (int v = NC; v < 20; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * psz, pr.wi0 + v * sv, 0.0);
     return;
   }
-  const double b0 = cf0[CF_B0 * np], b1 = cf0[(CF_B0 + 1) * np], b2 = cf0[(CF_B0 + 2) * np];
-  const double divb = cf0[CF_DIVB * np];
-  const double dA0 = cf0[CF_DIVA * np], dA1 = cf0[(CF_DIVA + 1) * np], dA2 = cf0[(CF_DIVA + 2) * np];
-  const double A00 = cf0[CF_A * np], A01 = cf0[(CF_A + 1) * np], A02 = cf0[(CF_A + 2) * np];
-  const double A11 = cf0[(CF_A + 3) * np], A12 = cf0[(CF_A + 4) * np], A22 = cf0[(CF_A + 5) * np];
-  const double epsh = a.c.eps_ko * g.inv_h;
-  const double ih = g.inv_h, ih2 = g.inv_h * g.inv_h;
-  const bool diss = a.c.has_diss;
+  const double ih = a.g.inv_h, ih2 = ih * ih, epsh = a.c.eps_ko * ih;
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
-    const double* ph = w0 + v * sv;
-    const double* pP = ph + NC * sv;
-    const double Pi = pP[0];
-    const double dh1 = ih * st_d1<Q>(ph, s1), dh2 = ih * st_d1<Q>(ph, s2), dh3 = ih * st_d1<Q>(ph, s3);
-    const double dP1 = ih * st_d1<Q>(pP, s1), dP2 = ih * st_d1<Q>(pP, s2), dP3 = ih * st_d1<Q>(pP, s3);
-    double s = b0 * dP1 + b1 * dP2 + b2 * dP3 + divb * Pi + dA0 * dh1 + dA1 * dh2 + dA2 * dh3;
-    s += A00 * (ih2 * st_d2<Q>(ph, s1)) + A11 * (ih2 * st_d2<Q>(ph, s2)) + A22 * (ih2 * st_d2<Q>(ph, s3));
-    const double dxy = ih2 * st_mixed<Q>(ph, s1, s2);
-    const double dxz = ih2 * st_mixed<Q>(ph, s1, s3);
-    const double dyz = ih2 * st_mixed<Q>(ph, s2, s3);
-    s += 2.0 * (A01 * dxy + A02 * dxz + A12 * dyz);
-    if (diss) s += epsh * (st_ko<Q>(pP, s1) + st_ko<Q>(pP, s2) + st_ko<Q>(pP, s3));
-    const double f = wl * s + cf0[(CF_C1 + v) * np];
-    emit<EPI>(a, TGH_OUT_ARGS, si0 + (NC + v) * psz, wi0 + (NC + v) * sv, f);
+    const double* ph = pr.w0 + v * sv;
+    const double s = pi_principal<Q>(ph, ph + NC * sv, pr.s2, pr.s3, pc.b, pc.divb, pc.divA, pc.A, ih, ih2,
+                                     epsh, a.c.has_diss);
+    emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + (NC + v) * psz, pr.wi0 + (NC + v) * sv,
+              pc.w * s + cf0[(CF_C1 + v) * cs]);
   }
 }
 
+#ifndef TGH_RHS_ONLY  // the pointwise kernels: built once, in the API's translation unit
 // ---------------------------------------------------------------------------
 // The ghost fill of a work array (prepare!, src/evolution.jl:1349-1357).
 struct FillArgs {
@@ -810,5 +760,320 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
     o[2] = sh[2][0];
   }
 }
+
+#endif  // TGH_RHS_ONLY
+
+// ---------------------------------------------------------------------------
+// The brick layout of the RK4 stepper. On a uniform forest of closed vertex
+// blocks every mesh point is the primary of exactly one block, and the
+// stepper holds the primaries of all blocks in one array per variable over
+// the whole brick of points, with a frame of G ghost points on every side:
+// point (x, y, z) of variable v, x in [-G, n_0 + G) and so on, sits at
+//   (x + XO) + py (y + G) + pz (z + G) + fs v,
+// with n_d = nb_d N + 1 points on a non-periodic axis (the boundary planes
+// are evolved) and nb_d N on a periodic one. A stencil of an interior point
+// then reads its neighbours' values where they are stored, no per-block ghost
+// copies exist, and only the frame is filled: from the Dirichlet hook or by
+// periodic wrap, with the same rule as the block fill (a point outside a
+// non-periodic axis takes the exact state at its own unwrapped position).
+struct Brick {
+  int n[3];      // evolved points per axis
+  int tot[3];    // nb_d N
+  int periodic[3];
+  int G, XO;     // ghost width; x offset of x = 0 in a row (XO >= G)
+  long py, pz, fs;
+  double lo[3], h, inv_h;
+};
+TGH_HD long brick_index(const Brick& k, int x, int y, int z) {
+  return (long)(x + k.XO) + k.py * (long)(y + k.G) + k.pz * (long)(z + k.G);
+}
+// Interior points, one per thread: each z-plane is cut into runs of 256
+// consecutive points in x-fastest order, so a wave reads 64 consecutive x
+// values of a row (or the end of one row and the start of the next), and the
+// runs are dealt to the XCDs in contiguous ranges (wg_map's order): each XCD
+// sweeps one slab of planes, whose stencil neighbours it shares in its L2.
+__device__ __forceinline__ bool brick_map(const Brick& k, int& x, int& y, int& z) {
+  const long plane = (long)k.n[0] * k.n[1];
+  const int wpp = (int)((plane + 255) / 256);
+  int r;
+  if (!wg_map(wpp, k.n[2], z, r)) return false;
+  const long lin = (long)r * 256 + threadIdx.x;
+  if (lin >= plane) return false;
+  y = (int)(lin / k.n[0]);
+  x = (int)(lin - (long)y * k.n[0]);
+  return true;
+}
+
+// The fused right-hand side with the RK4 epilogue on the brick layout: the
+// stage input W, its successor Wn, u_n and the accumulator all in the brick
+// layout, so one index addresses a point in every one of them.
+template <int Q, int GAUGE, bool HOLE>
+__global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const double* __restrict__ W,
+                                                   TGH_OUT_PARAMS) {
+  int x, y, z;
+  if (!brick_map(k, x, y, z)) return;
+  const long off = brick_index(k, x, y, z);
+  PointRef pr;
+  pr.w0 = W + off;
+  pr.sv = k.fs;
+  pr.s2 = (int)k.py;
+  pr.s3 = (int)k.pz;
+  pr.x[0] = k.lo[0] + (double)x * k.h;
+  pr.x[1] = k.lo[1] + (double)y * k.h;
+  pr.x[2] = k.lo[2] + (double)z * k.h;
+  pr.si0 = off;
+  pr.ssv = k.fs;
+  pr.wi0 = off;
+  pr.Hp = a.Hsrc ? a.Hsrc + off : nullptr;
+  pr.hs = k.fs;
+  pr.m_rho = nullptr;
+  pr.m_S = nullptr;
+  pr.m_Sij = nullptr;
+  pr.ms = 0;
+  rhs_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, k.inv_h, TGH_OUT_ARGS);
+}
+
+// The split form on the brick layout, first half: the head at one point per
+// thread (brick_map's order), leaving the Pi coefficients in coef (CF_N
+// arrays of the brick's field size, indexed like the brick).
+template <int Q, int GAUGE, bool HOLE>
+__global__ __launch_bounds__(256) void k_brick_head(RhsArgs a, Brick k, const double* __restrict__ W,
+                                                    double* __restrict__ coef, TGH_OUT_PARAMS) {
+  int x, y, z;
+  if (!brick_map(k, x, y, z)) return;
+  const long off = brick_index(k, x, y, z);
+  PointRef pr;
+  pr.w0 = W + off;
+  pr.sv = k.fs;
+  pr.s2 = (int)k.py;
+  pr.s3 = (int)k.pz;
+  pr.x[0] = k.lo[0] + (double)x * k.h;
+  pr.x[1] = k.lo[1] + (double)y * k.h;
+  pr.x[2] = k.lo[2] + (double)z * k.h;
+  pr.si0 = off;
+  pr.ssv = k.fs;
+  pr.wi0 = off;
+  pr.Hp = a.Hsrc ? a.Hsrc + off : nullptr;
+  pr.hs = k.fs;
+  pr.m_rho = nullptr;
+  pr.m_S = nullptr;
+  pr.m_Sij = nullptr;
+  pr.ms = 0;
+  PiCoef pc;
+  if (head_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, k.inv_h, pc, TGH_OUT_ARGS))
+    store_coef(coef + off, k.fs, pc);
+  else
+    coef[CF_W * k.fs + off] = -1.0;
+}
+
+// The split form on the brick layout, second half: a workgroup takes a tile
+// of TX x TY x TZ points (one per thread) and, one component at a time,
+// stages the tile's box of h_v and Pi_v (the tile and a halo of G = q/2 + 1
+// points, the reach of the dissipation stencil) in LDS with coalesced loads;
+// every stencil of the component is then read from LDS. Tiles go to the XCDs
+// in contiguous ranges of x-fastest tile order.
+template <int Q> struct PiTile {  // 8 x 8 x 4 (8 x 8 x 2 at q = 8, to fit 64 KiB of LDS)
+  static constexpr int X = 8, Y = 8, Z = Q == 8 ? 2 : 4;
+};
+template <int Q>
+__global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) void k_brick_pi(
+    RhsArgs a, Brick k, const double* __restrict__ W, const double* __restrict__ coef, TGH_OUT_PARAMS) {
+  constexpr int TX = PiTile<Q>::X, TY = PiTile<Q>::Y, TZ = PiTile<Q>::Z, NT = TX * TY * TZ;
+  constexpr int H = ::tgh::W<Q>::R + 1;
+  constexpr int BX = TX + 2 * H, BY = TY + 2 * H, BZ = TZ + 2 * H, BV = BX * BY * BZ;
+  __shared__ double sh[2][BV];
+  const int ntx = (k.n[0] + TX - 1) / TX, nty = (k.n[1] + TY - 1) / TY, ntz = (k.n[2] + TZ - 1) / TZ;
+  int tile, unused;
+  if (!wg_map(1, ntx * nty * ntz, tile, unused)) return;  // uniform over the workgroup
+  const int X0 = (tile % ntx) * TX, Y0 = ((tile / ntx) % nty) * TY, Z0 = (tile / (ntx * nty)) * TZ;
+  const int t = threadIdx.x;
+  const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
+  const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+  const bool valid = x < k.n[0] && y < k.n[1] && z < k.n[2];
+  const long off = valid ? brick_index(k, x, y, z) : 0;
+  PiCoef pc;
+  pc.w = -1.0;
+  if (valid) load_coef(coef + off, k.fs, pc);
+  const double ih = k.inv_h, ih2 = ih * ih, epsh = a.c.eps_ko * ih;
+  const int c0 = (lx + H) + BX * ((ly + H) + BY * (lz + H));
+  // box point (bx, by, bz) is brick point (X0 - H + bx, ...): never below -G,
+  // and skipped at or beyond n + G, where only points outside the brick
+  // (no thread computes them) would read it
+  const int ex = k.n[0] + k.G - (X0 - H), ey = k.n[1] + k.G - (Y0 - H), ez = k.n[2] + k.G - (Z0 - H);
+  const long box0 = brick_index(k, X0 - H, Y0 - H, Z0 - H);
+#pragma unroll 1
+  for (int v = 0; v < NC; ++v) {
+    __syncthreads();  // the previous component's reads are done
+    const double* __restrict__ Wh = W + box0 + v * k.fs;
+    for (int e = t; e < BV; e += NT) {
+      const int bx = e % BX, by = (e / BX) % BY, bz = e / (BX * BY);
+      double hv = 0.0, pv = 0.0;
+      if (bx < ex && by < ey && bz < ez) {
+        const long gi = bx + k.py * by + k.pz * bz;
+        hv = Wh[gi];
+        pv = Wh[gi + NC * k.fs];
+      }
+      sh[0][e] = hv;
+      sh[1][e] = pv;
+    }
+    __syncthreads();
+    if (valid) {
+      double f = 0.0;
+      if (pc.w >= 0.0) {
+        const double s = pi_principal<Q>(&sh[0][c0], &sh[1][c0], BX, BX * BY, pc.b, pc.divb, pc.divA, pc.A,
+                                         ih, ih2, epsh, a.c.has_diss);
+        f = pc.w * s + coef[(CF_C1 + v) * k.fs + off];
+      }
+      emit<EPI_RK4>(a, TGH_OUT_ARGS, off + (NC + v) * k.fs, off + (NC + v) * k.fs, f);
+    }
+  }
+}
+
+#ifndef TGH_RHS_ONLY
+// The frame of a brick array at time t, one point of the whole extent per
+// thread (interior points return at once).
+struct FrameArgs {
+  Brick k;
+  CaseP c;
+  double t;
+  double* __restrict__ W;
+};
+__global__ __launch_bounds__(256) void k_brick_frame(FrameArgs a) {
+  const Brick& k = a.k;
+  const long E0 = k.n[0] + 2 * k.G, E1 = k.n[1] + 2 * k.G, E2 = k.n[2] + 2 * k.G;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= E0 * E1 * E2) return;
+  const int gp[3] = {(int)(t % E0) - k.G, (int)((t / E0) % E1) - k.G, (int)(t / (E0 * E1)) - k.G};
+  bool inside = true, dirichlet = false;
+  int w[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (gp[d] >= 0 && gp[d] < k.n[d]) {
+      w[d] = gp[d];
+    } else {
+      inside = false;
+      if (k.periodic[d])
+        w[d] = ((gp[d] % k.tot[d]) + k.tot[d]) % k.tot[d];
+      else
+        dirichlet = true;
+    }
+  }
+  if (inside) return;
+  double* dst = a.W + brick_index(k, gp[0], gp[1], gp[2]);
+  if (dirichlet) {
+    double x[3], s[20];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[d] = k.lo[d] + (double)gp[d] * k.h;
+    case_state(a.c, a.t, x, s);
+#pragma unroll
+    for (int v = 0; v < 20; ++v) dst[v * k.fs] = s[v];
+  } else {
+    const double* src = a.W + brick_index(k, w[0], w[1], w[2]);
+#pragma unroll
+    for (int v = 0; v < 20; ++v) dst[v * k.fs] = src[v * k.fs];
+  }
+}
+
+// Between the state (block layout) and the brick. In: every brick point reads
+// the owned point of the block whose primary it is (into two brick arrays:
+// the stage input and u_n). Out, after the steps, in two passes so that no
+// value is read after it is overwritten: first every copy (an owned point
+// that is not its brick point's primary) advances by its primary's increment,
+// u_c += u_brick - u_primary, which is the reference's stepping of the whole
+// state vector, where a copy keeps its own value plus the primary's stage
+// sums (periodic copies sit at the unwrapped position and can differ); then
+// every primary takes its brick point's value.
+struct BrickCopyArgs {
+  Geom g;
+  Brick k;
+  const double* __restrict__ src;
+  double* __restrict__ dst0;
+  double* __restrict__ dst1;
+  int pass;  // out: 0 copies, 1 primaries
+};
+// The primary (block, local index) of brick point gp.
+__device__ __forceinline__ int brick_primary(const Geom& g, const int gp[3], int loc[3]) {
+  int ob[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int o = gp[d] / g.N;
+    ob[d] = o < g.nb[d] - 1 ? o : g.nb[d] - 1;
+    loc[d] = gp[d] - ob[d] * g.N;
+  }
+  return g.lookup[ob[0] + g.nb[0] * (ob[1] + g.nb[1] * ob[2])];
+}
+__global__ __launch_bounds__(256) void k_brick_in(BrickCopyArgs a) {
+  const Brick& k = a.k;
+  const Geom& g = a.g;
+  int x, y, z;
+  if (!brick_map(k, x, y, z)) return;
+  const int gp[3] = {x, y, z};
+  int loc[3];
+  const int b = brick_primary(g, gp, loc);
+  const long psz = (long)g.P * g.P * g.P;
+  const double* s = a.src + (long)b * 20 * psz + loc[0] + (long)g.P * (loc[1] + (long)g.P * loc[2]);
+  const long off = brick_index(k, x, y, z);
+#pragma unroll 4
+  for (int v = 0; v < 20; ++v) {
+    const double val = s[v * psz];
+    a.dst0[off + v * k.fs] = val;
+    a.dst1[off + v * k.fs] = val;
+  }
+}
+__global__ __launch_bounds__(256) void k_brick_out(BrickCopyArgs a) {
+  const Geom& g = a.g;
+  const Brick& k = a.k;
+  const int P = g.P;
+  int b, wx;
+  if (!wg_map((P * P * P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
+  if (p >= P * P * P) return;
+  const int l[3] = {p % P, (p / P) % P, p / (P * P)};
+  const int* B = g.bcoord + 3 * b;
+  int gp[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    gp[d] = B[d] * g.N + l[d];
+    if (k.periodic[d] && gp[d] == k.tot[d]) gp[d] = 0;
+  }
+  int loc[3];
+  const int ob = brick_primary(g, gp, loc);
+  const bool primary = ob == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2];
+  if (primary != (a.pass == 1)) return;
+  const long off = brick_index(k, gp[0], gp[1], gp[2]);
+  const long psz = (long)P * P * P;
+  double* d0 = a.dst0 + (long)b * 20 * psz + p;
+  if (primary) {
+#pragma unroll 4
+    for (int v = 0; v < 20; ++v) d0[v * psz] = a.src[off + v * k.fs];
+  } else {
+    const double* up = a.dst0 + (long)ob * 20 * psz + loc[0] + (long)P * (loc[1] + (long)P * loc[2]);
+#pragma unroll 4
+    for (int v = 0; v < 20; ++v) d0[v * psz] = d0[v * psz] + (a.src[off + v * k.fs] - up[v * psz]);
+  }
+}
+
+// The sampled gauge source on the brick layout (as k_sample_gauge).
+struct BrickSampleArgs {
+  Brick k;
+  CaseP c;
+  double t;
+  double* __restrict__ Hsrc;
+};
+__global__ __launch_bounds__(256) void k_brick_sample(BrickSampleArgs a) {
+  const Brick& k = a.k;
+  int x, y, z;
+  if (!brick_map(k, x, y, z)) return;
+  const double xp[3] = {k.lo[0] + (double)x * k.h, k.lo[1] + (double)y * k.h, k.lo[2] + (double)z * k.h};
+  double xc[3];
+  core_point(a.c, xp, xc);
+  double Hl[4], dH[4][4];
+  gauge_source_grad(a.c.bg, a.t, xc, Hl, dH);
+  double* out = a.Hsrc + brick_index(k, x, y, z);
+  for (int q = 0; q < 4; ++q) out[q * k.fs] = Hl[q];
+  for (int n = 0; n < 16; ++n) out[(4 + n) * k.fs] = dH[n % 4][n / 4];
+}
+
+#endif  // TGH_RHS_ONLY
 
 }  // namespace tgh

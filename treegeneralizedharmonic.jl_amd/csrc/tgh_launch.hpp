@@ -1,0 +1,108 @@
+// The right-hand side launches, templated on the order: one translation unit
+// per order (tgh_q<Q>.hip) includes this and instantiates them.
+#pragma once
+#define TGH_RHS_ONLY
+#include "tgh_problem.hpp"
+
+namespace tgh {
+
+// One right-hand side: the fused kernel, or the head and the Pi components as
+// two launches. While profiling, the head is timed as the "rhs" pair and the
+// Pi kernel as its own pair.
+template <int Q, int GAUGE, bool HOLE, int EPI>
+void launch_rhs_q(tgh_problem* p, const RhsArgs& a) {
+  const bool prof = p->profiling && EPI == EPI_RK4;
+  if (p->fused) {
+    int e0 = prof ? mark(p) : -1;
+    hipLaunchKernelGGL((k_rhs<Q, GAUGE, HOLE, EPI>), grid_points(p), dim3(256), 0, p->stream, a, a.W,
+                       a.du, a.us, a.acc, a.Wn);
+    if (prof) p->ev_rhs.push_back({e0, mark(p)});
+  } else {
+    int e0 = prof ? mark(p) : -1;
+    hipLaunchKernelGGL((k_head<Q, GAUGE, HOLE, EPI>), grid_points(p), dim3(256), 0, p->stream, a, a.W,
+                       a.coef, a.du, a.us, a.acc, a.Wn);
+    int e1 = prof ? mark(p) : -1;
+    hipLaunchKernelGGL((k_pi<Q, EPI>), grid_points(p), dim3(256), 0, p->stream, a, a.W,
+                       (const double*)a.coef, a.du, a.us, a.acc, a.Wn);
+    if (prof) {
+      p->ev_rhs.push_back({e0, e1});
+      p->ev_pi.push_back({e1, mark(p)});
+    }
+  }
+}
+template <int Q, int EPI>
+void launch_rhs_g(tgh_problem* p, const RhsArgs& a) {
+  const bool hole = p->cs.interior != INT_NONE;
+  switch (p->cs.gauge) {
+    case GAUGE_KS:
+      if (hole) launch_rhs_q<Q, GAUGE_KS, true, EPI>(p, a);
+      else launch_rhs_q<Q, GAUGE_KS, false, EPI>(p, a);
+      break;
+    case GAUGE_SAMPLED:
+      if (hole) launch_rhs_q<Q, GAUGE_SAMPLED, true, EPI>(p, a);
+      else launch_rhs_q<Q, GAUGE_SAMPLED, false, EPI>(p, a);
+      break;
+    default:
+      if (hole) launch_rhs_q<Q, GAUGE_NONE, true, EPI>(p, a);
+      else launch_rhs_q<Q, GAUGE_NONE, false, EPI>(p, a);
+      break;
+  }
+}
+template <int Q>
+void launch_rhs_blocks(tgh_problem* p, const RhsArgs& a, int epi) {
+  if (epi == EPI_RK4)
+    launch_rhs_g<Q, EPI_RK4>(p, a);
+  else
+    launch_rhs_g<Q, EPI_STORE>(p, a);
+}
+
+template <int Q, int GAUGE, bool HOLE>
+void launch_brick_rhs_q(tgh_problem* p, const RhsArgs& a) {
+  const bool prof = p->profiling;
+  int e0 = prof ? mark(p) : -1;
+  if (p->fused) {
+    hipLaunchKernelGGL((k_brick_rhs<Q, GAUGE, HOLE>), grid_brick(p), dim3(256), 0, p->stream, a, p->brick,
+                       a.W, a.du, a.us, a.acc, a.Wn);
+    if (prof) p->ev_rhs.push_back({e0, mark(p)});
+  } else {
+    hipLaunchKernelGGL((k_brick_head<Q, GAUGE, HOLE>), grid_brick(p), dim3(256), 0, p->stream, a, p->brick,
+                       a.W, p->Bcoef, a.du, a.us, a.acc, a.Wn);
+    int e1 = prof ? mark(p) : -1;
+    using T = PiTile<Q>;
+    hipLaunchKernelGGL((k_brick_pi<Q>), grid_brick_tiles(p, T::X, T::Y, T::Z), dim3(T::X * T::Y * T::Z), 0,
+                       p->stream, a, p->brick, a.W, (const double*)p->Bcoef, a.du, a.us, a.acc, a.Wn);
+    if (prof) {
+      p->ev_rhs.push_back({e0, e1});
+      p->ev_pi.push_back({e1, mark(p)});
+    }
+  }
+}
+template <int Q>
+void launch_brick_rhs_g(tgh_problem* p, const RhsArgs& a) {
+  const bool hole = p->cs.interior != INT_NONE;
+  switch (p->cs.gauge) {
+    case GAUGE_KS:
+      if (hole) launch_brick_rhs_q<Q, GAUGE_KS, true>(p, a);
+      else launch_brick_rhs_q<Q, GAUGE_KS, false>(p, a);
+      break;
+    case GAUGE_SAMPLED:
+      if (hole) launch_brick_rhs_q<Q, GAUGE_SAMPLED, true>(p, a);
+      else launch_brick_rhs_q<Q, GAUGE_SAMPLED, false>(p, a);
+      break;
+    default:
+      if (hole) launch_brick_rhs_q<Q, GAUGE_NONE, true>(p, a);
+      else launch_brick_rhs_q<Q, GAUGE_NONE, false>(p, a);
+      break;
+  }
+}
+
+template <int Q>
+void launch_rhs_brick(tgh_problem* p, const RhsArgs& a) {
+  launch_brick_rhs_g<Q>(p, a);
+}
+
+}  // namespace tgh
+
+#define TGH_INSTANTIATE_ORDER(Q)                                                            \
+  template void tgh::launch_rhs_blocks<Q>(tgh_problem*, const tgh::RhsArgs&, int);          \
+  template void tgh::launch_rhs_brick<Q>(tgh_problem*, const tgh::RhsArgs&);

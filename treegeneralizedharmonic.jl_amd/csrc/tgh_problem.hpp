@@ -1,0 +1,76 @@
+// The library's problem object and the pieces of host code shared between
+// the API's translation unit (tgh_api.hip) and the right-hand side's, one per
+// finite-difference order (tgh_q2.hip ... tgh_q8.hip), which compile in
+// parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/tgh.h"
+#include "tgh_kernels.hpp"
+
+struct tgh_problem {
+  tgh_desc desc;
+  tgh::Geom geom;
+  tgh::CaseP cs;
+  int q = 4;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<int> bcoord_host;
+  int* d_bcoord = nullptr;
+  int* d_lookup = nullptr;
+  double* W[2] = {nullptr, nullptr};  // work arrays (prepare uses W[0])
+  double* acc = nullptr;              // RK4 accumulator, state layout
+  double* coef = nullptr;             // the split form's per-point coefficients (CF_N x points)
+  double* Hsrc = nullptr;             // sampled gauge source, state layout x 20
+  double* red = nullptr;              // per-block reduction partials (3 per block)
+  std::vector<double> red_host;
+  long state_len = 0, work_len = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  double graph_key[3] = {0, 0, 0};    // (dt, u pointer, t) the graph was captured for
+  // Benchmark timing: event pairs around each launch while profiling.
+  bool profiling = false;
+  bool fused = false;                 // one kernel per RHS instead of head + Pi
+  int graph_form = -1;                // the form the captured graph launches
+  // The stepper's brick layout (allocated at the first tgh_rk4 that uses it):
+  // two stage arrays, u_n and the accumulator, and the sampled gauge source.
+  tgh::Brick brick;
+  double* BW[2] = {nullptr, nullptr};
+  double* Bus = nullptr;
+  double* Bacc = nullptr;
+  double* BH = nullptr;
+  double* Bcoef = nullptr;            // the split form's coefficients (CF_N brick fields)
+  bool brick_ready = false;
+  bool frame_static = false;          // the frame never changes: filled once
+  hipStream_t cap_stream = nullptr;   // capture stream when the problem runs on the null stream
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_rhs, ev_fill, ev_pi;  // indices into ev_pool
+  size_t ev_used = 0;
+  double prof[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace tgh {
+dim3 xcd_grid(long total);
+dim3 grid_points(const tgh_problem* p);
+dim3 grid_brick(const tgh_problem* p);
+dim3 grid_brick_tiles(const tgh_problem* p, int tx, int ty, int tz);
+int mark(tgh_problem* p);  // a profiling event on the problem's stream
+
+// One right-hand side on the block layout (EPI_STORE or EPI_RK4) and one RK4
+// stage's right-hand side on the brick layout, at order Q; defined in
+// tgh_launch.hpp, instantiated in tgh_q<Q>.hip.
+template <int Q> void launch_rhs_blocks(tgh_problem* p, const RhsArgs& a, int epi);
+template <int Q> void launch_rhs_brick(tgh_problem* p, const RhsArgs& a);
+extern template void launch_rhs_blocks<2>(tgh_problem*, const RhsArgs&, int);
+extern template void launch_rhs_blocks<4>(tgh_problem*, const RhsArgs&, int);
+extern template void launch_rhs_blocks<6>(tgh_problem*, const RhsArgs&, int);
+extern template void launch_rhs_blocks<8>(tgh_problem*, const RhsArgs&, int);
+extern template void launch_rhs_brick<2>(tgh_problem*, const RhsArgs&);
+extern template void launch_rhs_brick<4>(tgh_problem*, const RhsArgs&);
+extern template void launch_rhs_brick<6>(tgh_problem*, const RhsArgs&);
+extern template void launch_rhs_brick<8>(tgh_problem*, const RhsArgs&);
+}  // namespace tgh

@@ -1,0 +1,4 @@
+// The right-hand side kernels and their launches at finite-difference order q = 8.
+#include "tgh_launch.hpp"
+
+TGH_INSTANTIATE_ORDER(8)
