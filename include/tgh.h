@@ -52,6 +52,10 @@ enum { /* interior variants, src/interior.jl:178-238 */
   TGH_INTERIOR_FROZEN = 2,
   TGH_INTERIOR_PASTED = 3
 };
+enum { /* the Kreiss–Oliger amplitude (src/gauge.jl:1172-1264) */
+  TGH_DISSIPATION_CONSTANT = 0,
+  TGH_DISSIPATION_HORIZON = 1
+};
 enum { /* tgh_rk4 flags */
   TGH_RK4_GRAPH = 1,  /* capture the step in a hipGraph and replay it */
   TGH_RK4_FUSED = 2,  /* one kernel per right-hand side */
@@ -77,7 +81,7 @@ typedef struct tgh_desc {
   int32_t gauge;        /* TGH_GAUGE_* */
   int32_t damping;      /* TGH_DAMPING_* */
   int32_t interior;     /* TGH_INTERIOR_* */
-  int32_t reserved;
+  int32_t dissipation;  /* TGH_DISSIPATION_*: eps_ko everywhere, or the horizon profile */
   double lo[3], hi[3];  /* the box; cells must be cubic */
   double M, a;          /* hole mass and spin parameter */
   double center[3];     /* the hole's (static) center */
@@ -85,8 +89,12 @@ typedef struct tgh_desc {
   double gamma0;        /* constant damping rate */
   double gamma0_near, gamma0_far, gamma0_width; /* Gaussian profile */
   double gamma2;
-  double eps_ko;        /* Kreiss–Oliger amplitude; 0 compiles the term out */
+  double eps_ko;        /* Kreiss–Oliger amplitude (eps_out of the horizon profile);
+                           0 (and eps_in 0) compiles the term out */
   double r0, r1, rho_max, w_ramp, rho_ramp; /* the interior's layer */
+  double eps_in, eps_r1, eps_rh; /* HorizonDissipation (src/gauge.jl:1192-1264): eps_in at
+                                    and inside eps_r1 < eps_rh, eps_ko at and outside
+                                    eps_rh, about center */
 } tgh_desc;
 
 /* Matter's 3+1 projections at every owned point (src/evolution.jl:1295-1311,

@@ -1016,6 +1016,7 @@ inline void layer_profiles(double r, double r0, double r1, double rho_max, doubl
 enum Interior { INT_NONE = 0, INT_DAMPED = 1, INT_FROZEN = 2, INT_PASTED = 3 };
 enum Gauge { GAUGE_NONE = 0, GAUGE_SAMPLED = 1, GAUGE_KS = 2 };
 enum Damping { DAMP_CONSTANT = 0, DAMP_GAUSSIAN = 1 };
+enum Dissipation { DISS_CONSTANT = 0, DISS_HORIZON = 1 };
 
 struct Case {
   BgParams bg;
@@ -1023,7 +1024,9 @@ struct Case {
   double ks_S[4] = {0, 0, 0, 0}, ks_u[4] = {1, 0, 0, 0};
   int damping = DAMP_CONSTANT;
   double g0 = 0, g0_near = 1, g0_far = 0.1, g0_width = 3, g2 = 0;
-  double eps_ko = 0;
+  double eps_ko = 0;  // the number, or HorizonDissipation's eps_out
+  int diss = DISS_CONSTANT;
+  double eps_in = 0, eps_r1 = 0, eps_rh = 0;  // HorizonDissipation (gauge.jl:1221-1227)
   int interior = INT_NONE;
   double center[3] = {0, 0, 0};
   double r0 = 0, r1 = 0, rho_max = 0, w_ramp = 0.5, rho_ramp = 0.5;
@@ -1034,6 +1037,23 @@ inline double damping_rate(const Case& c, const double x[3]) {
   double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
   double r2 = d1 * d1 + d2 * d2 + d3 * d3;
   return c.g0_far + (c.g0_near - c.g0_far) * std::exp(-r2 / (2.0 * c.g0_width * c.g0_width));
+}
+// dissipation_rate (gauge.jl:1180, 1254-1264): the number itself, or the
+// HorizonDissipation profile about the (static) center: eps_out at and
+// outside r_h, eps_in at and inside r_1, each through a branch, and the
+// quintic smoothstep of (r_h - r)/(r_h - r_1) between.
+inline double dissipation_rate(const Case& c, const double x[3]) {
+  if (c.diss == DISS_CONSTANT) return c.eps_ko;
+  double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
+  double r = std::sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+  if (r >= c.eps_rh) return c.eps_ko;
+  if (r <= c.eps_r1) return c.eps_in;
+  double s = (c.eps_rh - r) / (c.eps_rh - c.eps_r1);
+  return c.eps_ko + (c.eps_in - c.eps_ko) * smoothstep(s);
+}
+// has_dissipation (gauge.jl:1190, 1266).
+inline bool has_dissipation(const Case& c) {
+  return c.diss == DISS_CONSTANT ? c.eps_ko != 0.0 : !(c.eps_ko == 0.0 && c.eps_in == 0.0);
 }
 inline double interior_radius(const Case& c, const double x[3]) {
   double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];

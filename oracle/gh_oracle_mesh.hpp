@@ -301,8 +301,8 @@ struct Problem {
     double x[3];
     m.position(b, i, j, k, x);
     double g0 = damping_rate(cs, x);
-    double epsh = cs.eps_ko / m.h;
-    bool diss = cs.eps_ko != 0.0;
+    double epsh = dissipation_rate(cs, x) * (1.0 / m.h);  // evolution.jl:478
+    bool diss = has_dissipation(cs);
     V10<double> dth, dtP;
     if (cs.interior != INT_NONE) {
       double r = interior_radius(cs, x);
@@ -424,15 +424,14 @@ struct Problem {
       ks_algebraic_source<double>(cs.bg.M, cs.ks_S, cs.ks_u, hv, dth0, dh, Hl, dHl);
     }
     gh_node_rhs_expanded(hv, Pv, dh, dP, ddh, Hl, dHl, g0, cs.g2, dth, dtP);
-    if (cs.eps_ko != 0.0 || cs.gauge == GAUGE_KS) {
+    const double eps = dissipation_rate(cs, x);
+    const bool diss = has_dissipation(cs);
+    if (diss || cs.gauge == GAUGE_KS) {
       V10<double> Qh, QP;
       for (int v = 0; v < NC; ++v) {
-        Qh[v] = cs.eps_ko == 0.0 ? 0.0
-                                 : (cs.eps_ko / hsp) * ((apply(wD, v, 0) + apply(wD, v, 1)) + apply(wD, v, 2));
-        QP[v] = cs.eps_ko == 0.0
-                    ? 0.0
-                    : (cs.eps_ko / hsp) *
-                          ((apply(wD, NC + v, 0) + apply(wD, NC + v, 1)) + apply(wD, NC + v, 2));
+        Qh[v] = !diss ? 0.0 : (eps / hsp) * ((apply(wD, v, 0) + apply(wD, v, 1)) + apply(wD, v, 2));
+        QP[v] = !diss ? 0.0
+                      : (eps / hsp) * ((apply(wD, NC + v, 0) + apply(wD, NC + v, 1)) + apply(wD, NC + v, 2));
       }
       MQ<double> mq = metric_quantities(hv);
       V10<double> dthQ;

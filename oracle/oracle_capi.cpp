@@ -38,6 +38,10 @@ Case case_from(const tgh_desc* d) {
   c.g0_width = d->gamma0_width;
   c.g2 = d->gamma2;
   c.eps_ko = d->eps_ko;
+  c.diss = d->dissipation;
+  c.eps_in = d->eps_in;
+  c.eps_r1 = d->eps_r1;
+  c.eps_rh = d->eps_rh;
   c.interior = d->interior;
   for (int i = 0; i < 3; ++i) c.center[i] = d->center[i];
   c.r0 = d->r0;

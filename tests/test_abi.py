@@ -32,12 +32,52 @@ def test_library_exports_every_declared_symbol(tgh):
     assert not missing, missing
 
 
-def test_descriptor_layout_matches_the_header(tgh):
+def test_descriptor_layout_matches_the_header(tgh, tmp_path):
+    """Every field of the ctypes tgh_desc sits at the offset the C compiler gives
+    it in include/tgh.h, and the sizes agree."""
+    import subprocess
     import tgh_desc
-    assert ctypes.sizeof(tgh.TghDesc) == ctypes.sizeof(tgh_desc.TghDesc)
     assert [f[0] for f in tgh.TghDesc._fields_] == [f[0] for f in tgh_desc.TghDesc._fields_]
-    # 13 int32 (incl. the reserved pad) then 29 doubles, no hidden padding
-    assert ctypes.sizeof(tgh.TghDesc) == 13 * 4 + 4 + 29 * 8 - 4 or ctypes.sizeof(tgh.TghDesc) % 8 == 0
+    names = [f[0] for f in tgh.TghDesc._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "tgh.h"\nint main(void) {\n'
+                   '  printf("%zu\\n", sizeof(tgh_desc));\n' +
+                   "".join(f'  printf("%zu\\n", offsetof(tgh_desc, {n}));\n' for n in names) + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(tgh.TghDesc)
+    assert out[1:] == [getattr(tgh.TghDesc, n).offset for n in names]
+
+
+def test_descriptor_checks_refuse_bad_dissipation_profiles(tgh):
+    """HorizonDissipation's refusals (src/gauge.jl:1229-1252) at problem creation:
+    the descriptor is checked before any device call."""
+    import tgh_desc as TD
+    L = tgh._load()
+    for kw in (dict(eps_in=-1.0), dict(eps_in=5.0), dict(eps_r1=2.0), dict(eps_r1=0.0), dict(dissipation=7)):
+        d = TD.make_desc(q=4, N=16, nblocks=(3, 3, 3), periodic=(0, 0, 0), lo=(-3, -3, -3), hi=(3, 3, 3),
+                         background=TD.TGH_BG_KERR_SCHILD, gauge=TD.TGH_GAUGE_KERR_SCHILD, M=1.0, a=0.5,
+                         eps_ko=0.5, dissipation=TD.TGH_DISSIPATION_HORIZON, eps_in=3.0, eps_r1=1.71,
+                         eps_rh=1.866)
+        for k, v in kw.items():
+            setattr(d, k, v)
+        ptr = ctypes.c_void_p()
+        assert L.tgh_problem_create(ctypes.byref(tgh.TghDesc.from_buffer_copy(d)), ctypes.byref(ptr)) == 1, kw
+        assert L.tgh_last_error()
+
+
+def test_python_case_builds_the_horizon_profile(tgh):
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=3.0, r_0=0.71, r_1=1.71)
+    hd = tgh.horizon_dissipation(case, eps_in=3.0)
+    assert hd.r_1 == 1.71 and abs(hd.r_h - (1 + 0.75 ** 0.5)) < 1e-15 and hd.eps_out == 0.5
+    case2 = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=3.0, r_0=0.71, r_1=1.71, eps_ko=hd)
+    d = tgh._desc(tgh.gh_forest(None, case2, N=16, roots=3), case2, 4)
+    assert (d.dissipation, d.eps_ko, d.eps_in, d.eps_r1) == (1, 0.5, 3.0, 1.71)
+    with pytest.raises(tgh.ArgumentError):
+        tgh.HorizonDissipation(eps_out=0.5, eps_in=5.0, r_1=1.0, r_h=2.0)
+    with pytest.raises(tgh.ArgumentError):
+        tgh.HorizonDissipation(eps_out=0.5, eps_in=1.0, r_1=2.0, r_h=1.0)
 
 
 def test_float_type_first_refuses_other_types(tgh):

@@ -464,3 +464,42 @@ def test_rk4_flags_are_checked(tgh):
     assert "exclude" in L.tgh_last_error().decode()
     assert L.tgh_rk4(p.ptr, tgh.C.c_void_p(u.data_ptr()), 0.0, 0.01, 1, 64) == 1
     assert "unknown" in L.tgh_last_error().decode()
+
+
+def horizon_profile_case():
+    """A spinning hole with the HorizonDissipation profile (eps_out 1/2 outside the
+    horizon's axis radius, eps_in 3 inside r_1)."""
+    a = 0.5
+    rh = 1.0 + (1.0 - a * a) ** 0.5
+    return TD.make_desc(q=4, N=16, nblocks=(3, 3, 3), periodic=(0, 0, 0), lo=(-3, -3, -3), hi=(3, 3, 3),
+                        background=TD.TGH_BG_KERR_SCHILD, gauge=TD.TGH_GAUGE_KERR_SCHILD,
+                        damping=TD.TGH_DAMPING_GAUSSIAN, gamma0_near=1.0, gamma0_far=0.1, gamma0_width=3.0,
+                        interior=TD.TGH_INTERIOR_DAMPED, M=1.0, a=a, r0=0.71, r1=1.71, rho_max=4.0,
+                        eps_ko=0.5, dissipation=TD.TGH_DISSIPATION_HORIZON, eps_in=3.0, eps_r1=1.71, eps_rh=rh)
+
+
+def test_horizon_dissipation_rhs_matches_oracle(tgh):
+    err, *_ = check_rhs(tgh, horizon_profile_case(), 0.0, noise=(4242, 1e-6))
+    print(f"horizon dissipation profile: max rel err {err:.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("form", ["brick-split", "blocks-fused"])
+def test_horizon_dissipation_rk4_matches_oracle(tgh, form):
+    torch = torch_cuda()
+    desc = horizon_profile_case()
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = o.fill_exact(0.0)
+    o.add_noise(u, 4242, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    lam = o.max_speed(u)
+    dt = 0.25 * 0.125 / lam
+    for n in range(2):
+        o.rk4_step(u, n * dt, dt, NTHREADS)
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, RK4_FORMS[form]))
+    g = ud.cpu().numpy()
+    diff = float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max())
+    print(f"horizon dissipation, 2 RK4 steps ({form}): max rel diff = {diff:.3e}")
+    assert diff <= 1e-12
+

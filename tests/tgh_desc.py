@@ -7,6 +7,7 @@ TGH_BG_MINKOWSKI, TGH_BG_GAUGE_WAVE, TGH_BG_KERR_SCHILD, TGH_BG_SHIFTED_MINKOWSK
 TGH_GAUGE_NONE, TGH_GAUGE_SAMPLED, TGH_GAUGE_KERR_SCHILD = 0, 1, 2
 TGH_DAMPING_CONSTANT, TGH_DAMPING_GAUSSIAN = 0, 1
 TGH_INTERIOR_NONE, TGH_INTERIOR_DAMPED, TGH_INTERIOR_FROZEN, TGH_INTERIOR_PASTED = 0, 1, 2, 3
+TGH_DISSIPATION_CONSTANT, TGH_DISSIPATION_HORIZON = 0, 1
 
 
 class TghDesc(C.Structure):
@@ -14,7 +15,7 @@ class TghDesc(C.Structure):
         ("float_bits", C.c_int32), ("q", C.c_int32), ("N", C.c_int32),
         ("nblocks", C.c_int32 * 3), ("periodic", C.c_int32 * 3),
         ("background", C.c_int32), ("gauge", C.c_int32), ("damping", C.c_int32),
-        ("interior", C.c_int32), ("reserved", C.c_int32),
+        ("interior", C.c_int32), ("dissipation", C.c_int32),
         ("lo", C.c_double * 3), ("hi", C.c_double * 3),
         ("M", C.c_double), ("a", C.c_double), ("center", C.c_double * 3),
         ("wave_amp", C.c_double), ("wave_len", C.c_double),
@@ -22,6 +23,7 @@ class TghDesc(C.Structure):
         ("gamma0_width", C.c_double), ("gamma2", C.c_double), ("eps_ko", C.c_double),
         ("r0", C.c_double), ("r1", C.c_double), ("rho_max", C.c_double),
         ("w_ramp", C.c_double), ("rho_ramp", C.c_double),
+        ("eps_in", C.c_double), ("eps_r1", C.c_double), ("eps_rh", C.c_double),
     ]
 
 
@@ -30,7 +32,8 @@ def make_desc(*, q=4, N=16, nblocks=(1, 1, 1), periodic=(1, 1, 1), lo=(0.0, 0.0,
               damping=TGH_DAMPING_CONSTANT, gamma0=0.0, gamma0_near=1.0, gamma0_far=0.1,
               gamma0_width=3.0, gamma2=0.0, eps_ko=0.0, interior=TGH_INTERIOR_NONE,
               M=1.0, a=0.0, center=(0.0, 0.0, 0.0), wave_amp=0.05, wave_len=1.0,
-              r0=0.0, r1=0.0, rho_max=0.0, w_ramp=0.5, rho_ramp=0.5):
+              r0=0.0, r1=0.0, rho_max=0.0, w_ramp=0.5, rho_ramp=0.5, dissipation=0, eps_in=0.0,
+              eps_r1=0.0, eps_rh=0.0):
     d = TghDesc()
     d.float_bits = 64
     d.q, d.N = q, N
@@ -45,6 +48,7 @@ def make_desc(*, q=4, N=16, nblocks=(1, 1, 1), periodic=(1, 1, 1), lo=(0.0, 0.0,
     d.gamma0, d.gamma0_near, d.gamma0_far, d.gamma0_width = gamma0, gamma0_near, gamma0_far, gamma0_width
     d.gamma2, d.eps_ko = gamma2, eps_ko
     d.r0, d.r1, d.rho_max, d.w_ramp, d.rho_ramp = r0, r1, rho_max, w_ramp, rho_ramp
+    d.dissipation, d.eps_in, d.eps_r1, d.eps_rh = dissipation, eps_in, eps_r1, eps_rh
     return d
 
 

@@ -62,7 +62,7 @@ class TghDesc(C.Structure):
         ("float_bits", C.c_int32), ("q", C.c_int32), ("N", C.c_int32),
         ("nblocks", C.c_int32 * 3), ("periodic", C.c_int32 * 3),
         ("background", C.c_int32), ("gauge", C.c_int32), ("damping", C.c_int32),
-        ("interior", C.c_int32), ("reserved", C.c_int32),
+        ("interior", C.c_int32), ("dissipation", C.c_int32),
         ("lo", C.c_double * 3), ("hi", C.c_double * 3),
         ("M", C.c_double), ("a", C.c_double), ("center", C.c_double * 3),
         ("wave_amp", C.c_double), ("wave_len", C.c_double),
@@ -70,6 +70,7 @@ class TghDesc(C.Structure):
         ("gamma0_width", C.c_double), ("gamma2", C.c_double), ("eps_ko", C.c_double),
         ("r0", C.c_double), ("r1", C.c_double), ("rho_max", C.c_double),
         ("w_ramp", C.c_double), ("rho_ramp", C.c_double),
+        ("eps_in", C.c_double), ("eps_r1", C.c_double), ("eps_rh", C.c_double),
     ]
 
 
@@ -197,6 +198,51 @@ class GaussianDamping:
         self.center = tuple(float(c) for c in center)
 
 
+class HorizonDissipation:
+    """HorizonDissipation(T; ε_out, ε_in, r_1, r_h, center) (src/gauge.jl:1192-1252): the
+    Kreiss–Oliger amplitude ε_out at and outside r_h, ε_in at and inside r_1, a quintic
+    smoothstep of r = |x − c| between."""
+
+    def __init__(self, T=None, *, eps_out, eps_in, r_1, r_h, center=(0.0, 0.0, 0.0)):
+        _check_float(T)
+        eo, ei, r1, rh = float(eps_out), float(eps_in), float(r_1), float(r_h)
+        if not (eo >= 0 and ei >= 0):
+            raise ArgumentError(f"both Kreiss–Oliger amplitudes must be non-negative, got ε_out = "
+                                f"{eo} and ε_in = {ei}: the operator carries the damping sign in "
+                                f"its weights, so a negative amplitude amplifies the grid-scale "
+                                f"modes it is there to remove.")
+        if not (eo <= 4 and ei <= 4):
+            raise ArgumentError(f"the Kreiss–Oliger amplitude is capped at 4, got ε_out = {eo} and "
+                                f"ε_in = {ei}: the operator damps the lattice's corner mode at "
+                                f"3ε/h, and RK4 is stable on the negative real axis only to 2.79.")
+        if not 0 < r1 < rh:
+            raise ArgumentError(f"the profile rises from the horizon inward, so it needs "
+                                f"0 < r_1 < r_h, got r_1 = {r1} and r_h = {rh}")
+        self.eps_out, self.eps_in, self.r_1, self.r_h = eo, ei, r1, rh
+        self.center = tuple(float(c) for c in center)
+
+
+def horizon_min_radius(background):
+    """The horizon's smallest coordinate radius (src/interior.jl:760): r₊ = M + √(M² − a²)
+    on the Kerr–Schild axis."""
+    if isinstance(background, KerrSchild):
+        return background.M + (background.M ** 2 - background.a ** 2) ** 0.5
+    raise ArgumentError(f"{type(background).__name__} has no horizon")
+
+
+def horizon_dissipation(case, *, eps_in, eps_out=None):
+    """horizon_dissipation(case; ε_in, ε_out = case.ε_KO) (src/initialdata.jl:838-860)."""
+    if case.interior is None:
+        raise ArgumentError("the dissipation profile rises from the horizon to the layer's outer "
+                            "radius r_1, and this case has no interior, so no r_1.")
+    eo = case.eps_ko if eps_out is None else eps_out
+    if isinstance(eo, HorizonDissipation):
+        raise ArgumentError("the profile's exterior amplitude is a number: pass eps_out "
+                            "explicitly for a case whose ε_KO is already a profile.")
+    return HorizonDissipation(eps_out=eo, eps_in=eps_in, r_1=case.interior.r_1,
+                              r_h=horizon_min_radius(case.background), center=case.interior.center)
+
+
 class Interior:
     """Interior(T; center, r_0, r_1, ρ_max, variant, w_ramp, ρ_ramp)
     (src/interior.jl:178-238)."""
@@ -236,12 +282,16 @@ class GHCase:
         _check_float(T)
         if not gamma2 > -1:
             raise ArgumentError(f"γ2 must satisfy γ2 > −1, got {gamma2}")
-        if eps_ko < 0:
+        if isinstance(eps_ko, HorizonDissipation):
+            c0 = interior.center if interior is not None else (0.0, 0.0, 0.0)
+            if tuple(eps_ko.center) != tuple(c0):
+                raise ArgumentError("the dissipation profile is centred on the case's hole")
+        elif eps_ko < 0:
             raise ArgumentError(f"the Kreiss–Oliger amplitude must be ≥ 0, got {eps_ko}")
         self.background = background
         self.box = tuple((float(a), float(b)) for a, b in box)
         self.periodic = tuple(bool(p) for p in periodic)
-        self.eps_ko = float(eps_ko)
+        self.eps_ko = eps_ko if isinstance(eps_ko, HorizonDissipation) else float(eps_ko)
         self.gamma0 = gamma0 if isinstance(gamma0, (ConstantDamping, GaussianDamping)) \
             else ConstantDamping(gamma0)
         self.gamma2 = float(gamma2)
@@ -321,7 +371,13 @@ def _desc(forest, case, q):
         d.gamma0_near, d.gamma0_far, d.gamma0_width = g0.near, g0.far, g0.width
     else:
         d.damping, d.gamma0 = 0, g0.g0
-    d.gamma2, d.eps_ko = case.gamma2, case.eps_ko
+    if isinstance(case.eps_ko, HorizonDissipation):
+        hd = case.eps_ko
+        d.dissipation = 1
+        d.eps_ko, d.eps_in, d.eps_r1, d.eps_rh = hd.eps_out, hd.eps_in, hd.r_1, hd.r_h
+    else:
+        d.dissipation, d.eps_ko = 0, case.eps_ko
+    d.gamma2 = case.gamma2
     it = case.interior
     if it is not None:
         d.interior = Interior.VARIANTS[it.variant]

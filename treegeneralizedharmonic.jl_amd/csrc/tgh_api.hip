@@ -112,6 +112,23 @@ int check_desc(const tgh_desc* d) {
     if (d->background == TGH_BG_KERR_SCHILD && !(d->r0 > std::fabs(d->a)))
       return fail(TGH_EINVAL, "r_0 must lie outside the Kerr-Schild ring singularity, r_0 > |a|");
   }
+  if (d->dissipation != TGH_DISSIPATION_CONSTANT && d->dissipation != TGH_DISSIPATION_HORIZON)
+    return fail(TGH_EINVAL, "unknown dissipation kind");
+  if (!(d->eps_ko >= 0))
+    return fail(TGH_EINVAL, "the Kreiss-Oliger amplitude must be non-negative: the operator carries the "
+                            "damping sign in its weights");
+  if (d->dissipation == TGH_DISSIPATION_HORIZON) {
+    // HorizonDissipation's checks (src/gauge.jl:1229-1252)
+    if (!(d->eps_in >= 0))
+      return fail(TGH_EINVAL, "both Kreiss-Oliger amplitudes must be non-negative, got eps_in = " +
+                                  std::to_string(d->eps_in));
+    if (!(d->eps_ko <= 4 && d->eps_in <= 4))
+      return fail(TGH_EINVAL, "the Kreiss-Oliger amplitude is capped at 4: the operator damps the lattice's "
+                              "corner mode at 3 eps/h and RK4 is stable on the negative real axis only to 2.79");
+    if (!(0 < d->eps_r1 && d->eps_r1 < d->eps_rh))
+      return fail(TGH_EINVAL, "the profile rises from the horizon inward, so it needs 0 < r_1 < r_h, got r_1 = " +
+                                  std::to_string(d->eps_r1) + " and r_h = " + std::to_string(d->eps_rh));
+  }
   if (d->damping == TGH_DAMPING_GAUSSIAN && !(d->gamma0_width > 0))
     return fail(TGH_EINVAL, "the damping profile's width must be positive");
   return TGH_OK;
@@ -397,7 +414,12 @@ int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
   c.gauge = d->gauge;
   c.damping = d->damping;
   c.interior = d->interior;
-  c.has_diss = d->eps_ko != 0.0;
+  c.diss = d->dissipation;
+  c.eps_in = d->eps_in;
+  c.eps_r1 = d->eps_r1;
+  c.eps_rh = d->eps_rh;
+  c.has_diss = d->dissipation == TGH_DISSIPATION_CONSTANT ? d->eps_ko != 0.0
+                                                           : !(d->eps_ko == 0.0 && d->eps_in == 0.0);
   c.g0 = d->gamma0;
   c.g0_near = d->gamma0_near;
   c.g0_far = d->gamma0_far;

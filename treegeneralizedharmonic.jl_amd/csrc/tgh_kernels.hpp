@@ -32,7 +32,9 @@ struct CaseP {
   Bg bg;
   int gauge, damping, interior, has_diss;
   double g0, g0_near, g0_far, g0_width, g2;
-  double eps_ko;
+  double eps_ko;  // the number, or the horizon profile's eps_out
+  int diss;       // 0 the number, 1 the horizon profile
+  double eps_in, eps_r1, eps_rh;
   double center[3];
   double r0, r1, rho_max, w_ramp, rho_ramp;
 };
@@ -49,6 +51,20 @@ TGH_HD double damping_rate(const CaseP& c, const double x[3]) {  // src/gauge.jl
   const double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
   const double r2 = d1 * d1 + d2 * d2 + d3 * d3;
   return c.g0_far + (c.g0_near - c.g0_far) * exp(-r2 / (2.0 * c.g0_width * c.g0_width));
+}
+
+// dissipation_rate (src/gauge.jl:1180,1254-1264): eps_KO itself, or the
+// HorizonDissipation profile about the hole's center -- eps_out at and
+// outside r_h, eps_in at and inside r_1, each through a branch, and the
+// quintic smoothstep of (r_h - r)/(r_h - r_1) between.
+TGH_HD double dissipation_rate(const CaseP& c, const double x[3]) {
+  if (c.diss == 0) return c.eps_ko;
+  const double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
+  const double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+  if (r >= c.eps_rh) return c.eps_ko;
+  if (r <= c.eps_r1) return c.eps_in;
+  const double s = (c.eps_rh - r) / (c.eps_rh - c.eps_r1);
+  return c.eps_ko + (c.eps_in - c.eps_ko) * smoothstep(s);
 }
 
 TGH_HD double hole_radius(const CaseP& c, const double x[3]) {
@@ -313,7 +329,7 @@ __device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr,
     }
   }
   const double g0 = damping_rate(c, x);
-  const double epsh = c.eps_ko * inv_h;
+  const double epsh = dissipation_rate(c, x) * inv_h;
   const double ih = inv_h;
 
   double hv[NC], D[4][NC];
@@ -430,7 +446,7 @@ __device__ __forceinline__ void rhs_point(const RhsArgs& a, const PointRef& pr, 
     for (int v = NC; v < 20; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * psz, pr.wi0 + v * sv, 0.0);
     return;
   }
-  const double ih = inv_h, ih2 = inv_h * inv_h, epsh = a.c.eps_ko * inv_h;
+  const double ih = inv_h, ih2 = inv_h * inv_h, epsh = dissipation_rate(a.c, pr.x) * inv_h;
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
     const double* ph = pr.w0 + v * sv;
@@ -541,7 +557,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_PI_WAVE
     for (int v = NC; v < 20; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * psz, pr.wi0 + v * sv, 0.0);
     return;
   }
-  const double ih = a.g.inv_h, ih2 = ih * ih, epsh = a.c.eps_ko * ih;
+  const double ih = a.g.inv_h, ih2 = ih * ih, epsh = dissipation_rate(a.c, pr.x) * ih;
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
     const double* ph = pr.w0 + v * sv;
@@ -894,7 +910,8 @@ __global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) void k_
   PiCoef pc;
   pc.w = -1.0;
   if (valid) load_coef(coef + off, k.fs, pc);
-  const double ih = k.inv_h, ih2 = ih * ih, epsh = a.c.eps_ko * ih;
+  const double xp[3] = {k.lo[0] + (double)x * k.h, k.lo[1] + (double)y * k.h, k.lo[2] + (double)z * k.h};
+  const double ih = k.inv_h, ih2 = ih * ih, epsh = dissipation_rate(a.c, xp) * ih;
   const int c0 = (lx + H) + BX * ((ly + H) + BY * (lz + H));
   // box point (bx, by, bz) is brick point (X0 - H + bx, ...): never below -G,
   // and skipped at or beyond n + G, where only points outside the brick
