@@ -95,6 +95,9 @@ typedef struct tgh_desc {
   double eps_in, eps_r1, eps_rh; /* HorizonDissipation (src/gauge.jl:1192-1264): eps_in at
                                     and inside eps_r1 < eps_rh, eps_ko at and outside
                                     eps_rh, about center */
+  /* StateBounds (src/bounds.jl:51-136): the range projection's ranges, applied at
+   * r < r_gate from center; r_gate = 0 is a case without bounds (`nothing`). */
+  double alpha_min, alpha_max, lambda_min, lambda_max, beta_max, K_max, r_gate;
 } tgh_desc;
 
 /* Matter's 3+1 projections at every owned point (src/evolution.jl:1295-1311,
@@ -139,10 +142,19 @@ int tgh_gh_rhs(tgh_problem* p, double* du, const double* u, double t,
 /* settle!(p, u, t) — src/evolution.jl:1370-1374: prepare, then gather back. */
 int tgh_settle(tgh_problem* p, double* u, double t);
 /* stage_limit!(u, p, t) / step_limit!(u, p, t) — src/stepping.jl:91-97: the
- * :pasted overwrite (src/evolution.jl:888-905); a no-op for the other
- * variants. Cases with range bounds are refused at creation. */
+ * range projection where the case has bounds (src/bounds.jl:780-800), then
+ * the :pasted overwrite (src/evolution.jl:888-905); a no-op for a case with
+ * neither. */
 int tgh_stage_limit(tgh_problem* p, double* u, double t);
 int tgh_step_limit(tgh_problem* p, double* u, double t);
+/* apply_bounds!(p, u, t) — src/bounds.jl:733-770: the range projection
+ * (bounds_project, :510-589) at every owned point with r < r_gate, written back
+ * only where it fired; the number of points moved, of those with a non-finite
+ * component, and the outermost radius it fired at (-1: it did not). A no-op
+ * returning 0 for a case without bounds. tgh_stage_limit and tgh_step_limit
+ * apply it first, then the :pasted overwrite; tgh_rk4 applies it to every
+ * stage value and to the result. */
+int tgh_apply_bounds(tgh_problem* p, double* u, double t, int64_t* moved, int64_t* nonfinite, double* r_max);
 /* max_speed(p, u, t) — src/evolution.jl:1539-1560: the largest
  * alpha sqrt(tr gamma^ij) + |beta| over evolved points, checked:
  * TGH_ENOTFINITE if an evolved value or the speed is not finite and positive. */

@@ -7,6 +7,7 @@
 // src/constraints.jl:311-396 and TreeAMR 0.3's uniform-forest ghost fill.
 #pragma once
 #include "gh_oracle.hpp"
+#include "gh_oracle_bounds.hpp"
 
 namespace orc {
 
@@ -97,6 +98,7 @@ struct Mesh {
 struct Problem {
   Mesh mesh;
   Case cs;
+  StateBounds bounds;
   std::vector<double> work;
   std::vector<double> Hsrc;  // sampled gauge source at owned points, 20 per point
   std::vector<double> w1, w2, wD;
@@ -529,8 +531,44 @@ struct Problem {
     l2 = den > 0 ? std::sqrt(num / den) : 0.0;
   }
 
-  // The :pasted overwrite (evolution.jl:888-905); a no-op for other variants.
+  // apply_bounds! (bounds.jl:613-660, 733-770): the range projection at every
+  // owned point with r < r_gate, written back only where it fired; returns
+  // the number of points moved (a no-op without bounds).
+  long apply_bounds(double* u) const {
+    if (!bounds.on) return 0;
+    const Mesh& m = mesh;
+    long hits = 0;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3];
+            m.position(b, i, j, k, x);
+            if (!(interior_radius(cs, x) < bounds.r_gate)) continue;
+            double h[10], P[10], ho[10], Po[10];
+            for (int v = 0; v < 10; ++v) {
+              h[v] = u[m.sidx(b, v, i, j, k)];
+              P[v] = u[m.sidx(b, 10 + v, i, j, k)];
+            }
+            bool nf;
+            if (bounds_project(h, P, bounds, ho, Po, nf)) {
+              ++hits;
+              for (int v = 0; v < 10; ++v) {
+                u[m.sidx(b, v, i, j, k)] = ho[v];
+                u[m.sidx(b, 10 + v, i, j, k)] = Po[v];
+              }
+            }
+          }
+    return hits;
+  }
+
+  // gh_limiter! (stepping.jl, bounds.jl:780-800, evolution.jl:888-905): the
+  // range projection, then the :pasted overwrite, which has the last word.
   void stage_limit(double* u, double t) const {
+    apply_bounds(u);
+    paste(u, t);
+  }
+  void paste(double* u, double t) const {
     if (cs.interior != INT_PASTED) return;
     const Mesh& m = mesh;
     for (long b = 0; b < m.nblocks(); ++b)

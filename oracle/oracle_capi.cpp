@@ -247,6 +247,16 @@ void* orc_problem_create(const tgh_desc* d) {
   }
   p->mesh.h = (d->hi[0] - d->lo[0]) / ((double)d->nblocks[0] * d->N);
   p->cs = case_from(d);
+  if (d->r_gate > 0) {
+    p->bounds.on = true;
+    p->bounds.a_min = d->alpha_min;
+    p->bounds.a_max = d->alpha_max;
+    p->bounds.l_min = d->lambda_min;
+    p->bounds.l_max = d->lambda_max;
+    p->bounds.b_max = d->beta_max;
+    p->bounds.K_max = d->K_max;
+    p->bounds.r_gate = d->r_gate;
+  }
   p->setup();
   return p;
 }
@@ -275,6 +285,33 @@ void orc_rk4_step(void* p, double* u, double t, double dt, int nthreads) {
 }
 void orc_settle(void* p, double* u, double t) { ((Problem*)p)->settle(u, t); }
 void orc_stage_limit(void* p, double* u, double t) { ((Problem*)p)->stage_limit(u, t); }
+long orc_apply_bounds(void* p, double* u) { return ((Problem*)p)->apply_bounds(u); }
+
+// bounds_project at one point (bounds.jl:510-527), for the tests: h, P in,
+// ho, Po out; returns hit | nonfinite << 1.
+int orc_bounds_project(const double* h, const double* P, const double* bnd, double* ho, double* Po) {
+  StateBounds bd;
+  bd.on = true;
+  bd.a_min = bnd[0];
+  bd.a_max = bnd[1];
+  bd.l_min = bnd[2];
+  bd.l_max = bnd[3];
+  bd.b_max = bnd[4];
+  bd.K_max = bnd[5];
+  bd.r_gate = bnd[6];
+  bool nf;
+  const bool hit = bounds_project(h, P, bd, ho, Po, nf);
+  return (hit ? 1 : 0) | (nf ? 2 : 0);
+}
+// sym_eigen3 (bounds.jl:382-396) of a symmetric 3x3 (row-major in A), for the tests.
+void orc_sym_eigen3(const double* A, double* lam, double* V) {
+  double a[3][3], v[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = A[3 * i + j];
+  sym_eigen3(a, lam, v);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) V[3 * i + j] = v[i][j];
+}
 double orc_max_speed(void* p, const double* u) { return ((Problem*)p)->max_speed(u); }
 void orc_error_norms(void* p, const double* u, double t, double* l2, double* linf) {
   ((Problem*)p)->error_norms(u, t, *l2, *linf);

@@ -68,6 +68,9 @@ def lib():
             "orc_rk4_step": (None, [P, dp, D, D, C.c_int]),
             "orc_settle": (None, [P, dp, D]),
             "orc_stage_limit": (None, [P, dp, D]),
+            "orc_apply_bounds": (C.c_long, [P, dp]),
+            "orc_bounds_project": (C.c_int, [dp, dp, dp, dp, dp]),
+            "orc_sym_eigen3": (None, [dp, dp, dp]),
             "orc_max_speed": (D, [P, dp]),
             "orc_error_norms": (None, [P, dp, D, C.POINTER(D), C.POINTER(D)]),
             "orc_is_primary": (C.c_int, [P, Lg, Lg, Lg, Lg]),
@@ -156,6 +159,12 @@ class OracleProblem:
     def settle(self, u, t):
         lib().orc_settle(self.p, u, t)
 
+    def stage_limit(self, u, t):
+        lib().orc_stage_limit(self.p, u, t)
+
+    def apply_bounds(self, u):
+        return lib().orc_apply_bounds(self.p, u)
+
     def max_speed(self, u):
         return lib().orc_max_speed(self.p, u)
 
@@ -175,3 +184,18 @@ class OracleProblem:
 
     def count_flops(self, b, i, j, k, head_only=False):
         return lib().orc_count_rhs_flops(self.p, b, i, j, k, 1 if head_only else 0)
+
+
+def bounds_project(h, P, bnd):
+    """(h', P', hit, nonfinite) of the oracle's range projection at one point;
+    bnd = (alpha_min, alpha_max, lambda_min, lambda_max, beta_max, K_max, r_gate)."""
+    ho, Po = np.zeros(10), np.zeros(10)
+    r = lib().orc_bounds_project(np.ascontiguousarray(h, dtype=np.float64), np.ascontiguousarray(P, dtype=np.float64),
+                                 np.ascontiguousarray(bnd, dtype=np.float64), ho, Po)
+    return ho, Po, bool(r & 1), bool(r & 2)
+
+
+def sym_eigen3(A):
+    lam, V = np.zeros(3), np.zeros(9)
+    lib().orc_sym_eigen3(np.ascontiguousarray(A, dtype=np.float64).ravel(), lam, V)
+    return lam, V.reshape(3, 3)

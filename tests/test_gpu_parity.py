@@ -503,3 +503,59 @@ def test_horizon_dissipation_rk4_matches_oracle(tgh, form):
     print(f"horizon dissipation, 2 RK4 steps ({form}): max rel diff = {diff:.3e}")
     assert diff <= 1e-12
 
+
+
+def bounded_hole_case():
+    """ks_small with default_bounds(M = 1) and default_gate r_1 - 2 G h (src/bounds.jl:138-184)."""
+    d = ks_small()
+    h = (d.hi[0] - d.lo[0]) / (d.nblocks[0] * d.N)
+    d.alpha_min, d.alpha_max, d.lambda_min, d.lambda_max, d.beta_max, d.K_max = 1 / 50, 50.0, 1 / 100, 1000.0, 10.0, 100.0
+    d.r_gate = d.r1 - 2 * (d.q // 2 + 1) * h
+    return d
+
+
+def test_range_projection_matches_oracle_bitwise(tgh):
+    """apply_bounds! on the device against the oracle on a state with wild
+    values (and non-finite ones) scattered through the gate: the same points
+    move, to the same bits (both evaluate the map without contraction)."""
+    torch = torch_cuda()
+    desc = bounded_hole_case()
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = o.fill_exact(0.0)
+    rng = np.random.default_rng(7)
+    pick = rng.uniform(size=u.size) < 0.03
+    u[pick] += rng.uniform(-6, 6, pick.sum()) * 10 ** rng.uniform(-2, 1, pick.sum())
+    u[rng.choice(u.size, 5, replace=False)] = np.nan
+    ud = torch.from_numpy(u.copy()).cuda()
+    n_o = o.apply_bounds(u)
+    import ctypes as C
+    n, nf, rm = C.c_int64(), C.c_int64(), C.c_double()
+    tgh._check(tgh._load().tgh_apply_bounds(p.ptr, C.c_void_p(ud.data_ptr()), 0.0, C.byref(n), C.byref(nf),
+                                            C.byref(rm)))
+    g = ud.cpu().numpy()
+    print(f"range projection: {n.value} points moved (oracle {n_o}), {nf.value} non-finite, r_max {rm.value:.3f}")
+    assert n.value == n_o and n_o > 100
+    same = (g == u) | (np.isnan(g) & np.isnan(u))
+    assert same.all(), f"{(~same).sum()} entries differ"
+    assert 0 < rm.value < desc.r_gate
+
+
+@pytest.mark.parametrize("form", ["brick-split", "blocks-fused"])
+def test_rk4_with_range_projection_matches_oracle(tgh, form):
+    torch = torch_cuda()
+    desc = bounded_hole_case()
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = o.fill_exact(0.0)
+    o.add_noise(u, 99, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    lam = o.max_speed(u)
+    dt = 0.25 * (1 / 16) / lam
+    for n in range(2):
+        o.rk4_step(u, n * dt, dt, NTHREADS)
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, RK4_FORMS[form]))
+    g = ud.cpu().numpy()
+    diff = float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max())
+    print(f"range projection, 2 RK4 steps ({form}): max rel diff = {diff:.3e}")
+    assert diff <= 1e-12

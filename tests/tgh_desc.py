@@ -24,6 +24,8 @@ class TghDesc(C.Structure):
         ("r0", C.c_double), ("r1", C.c_double), ("rho_max", C.c_double),
         ("w_ramp", C.c_double), ("rho_ramp", C.c_double),
         ("eps_in", C.c_double), ("eps_r1", C.c_double), ("eps_rh", C.c_double),
+        ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("lambda_min", C.c_double),
+        ("lambda_max", C.c_double), ("beta_max", C.c_double), ("K_max", C.c_double), ("r_gate", C.c_double),
     ]
 
 

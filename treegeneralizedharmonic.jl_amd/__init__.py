@@ -71,6 +71,8 @@ class TghDesc(C.Structure):
         ("r0", C.c_double), ("r1", C.c_double), ("rho_max", C.c_double),
         ("w_ramp", C.c_double), ("rho_ramp", C.c_double),
         ("eps_in", C.c_double), ("eps_r1", C.c_double), ("eps_rh", C.c_double),
+        ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("lambda_min", C.c_double),
+        ("lambda_max", C.c_double), ("beta_max", C.c_double), ("K_max", C.c_double), ("r_gate", C.c_double),
     ]
 
 
@@ -114,6 +116,7 @@ def _load():
         "tgh_profile": (C.c_int, [P, C.c_int32]),
         "tgh_profile_read": (C.c_int, [P, C.POINTER(D)]),
         "tgh_last_error": (C.c_char_p, []),
+        "tgh_apply_bounds": (C.c_int, [P, P, D, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(D)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -272,14 +275,63 @@ class Interior:
                         variant=self.variant, w_ramp=self.w_ramp, rho_ramp=self.rho_ramp)
 
 
+class StateBounds:
+    """StateBounds(T; α_min, α_max, λ_min, λ_max, β_max, K_max, r_gate)
+    (src/bounds.jl:51-136): the range projection's ranges and its gate radius."""
+
+    def __init__(self, T=None, *, alpha_min, alpha_max, lambda_min, lambda_max, beta_max, K_max, r_gate):
+        _check_float(T)
+        vals = [float(x) for x in (alpha_min, alpha_max, lambda_min, lambda_max, beta_max, K_max, r_gate)]
+        if not all(math.isfinite(x) for x in vals):
+            raise ArgumentError(f"every bound must be finite, got {vals}: a bound at infinity is no "
+                                f"bound, and it is spelled by leaving the case's bounds at None instead.")
+        a0, a1, l0, l1, b, K, rg = vals
+        if not 0 < a0 < 1 < a1:
+            raise ArgumentError(f"the lapse range needs 0 < α_min < 1 < α_max, got α_min = {a0} and "
+                                f"α_max = {a1}")
+        if not 0 < l0 < 1 < l1:
+            raise ArgumentError(f"the spatial metric's eigenvalue range needs 0 < λ_min < 1 < λ_max, "
+                                f"got λ_min = {l0} and λ_max = {l1}")
+        if not b > 0:
+            raise ArgumentError(f"the shift's norm bound must be positive, got β_max = {b}")
+        if not K > 0:
+            raise ArgumentError(f"the momentum's scale bound must be positive, got K_max = {K}")
+        if not rg > 0:
+            raise ArgumentError(f"the gate radius must be positive, got r_gate = {rg}")
+        (self.alpha_min, self.alpha_max, self.lambda_min, self.lambda_max, self.beta_max, self.K_max,
+         self.r_gate) = vals
+
+
+def default_bounds(T=None, *, M, r_gate):
+    """default_bounds(T; M, r_gate) (src/bounds.jl:138-160)."""
+    return StateBounds(T, alpha_min=1 / 50, alpha_max=50.0, lambda_min=1 / 100, lambda_max=1000.0,
+                       beta_max=10.0, K_max=100.0 / M, r_gate=r_gate)
+
+
+def default_gate(interior, forest, q):
+    """default_gate(interior, forest, q) = r_1 − 2 G h (src/bounds.jl:162-184), h the
+    uniform forest's spacing."""
+    box = forest.case.box if forest.case is not None else None
+    if box is None:
+        raise ArgumentError("default_gate needs the forest's case for its spacing")
+    h = (box[0][1] - box[0][0]) / (forest.roots[0] * forest.N)
+    return interior.r_1 - 2 * (q // 2 + 1) * h
+
+
 class GHCase:
     """GHCase(T, background; box, periodic, ε_KO, γ0, γ2, interior, gauge_source)
     (src/initialdata.jl:158-311). ``gauge_source`` is None (harmonic
     backgrounds) or "algebraic" (the closed-form Kerr-Schild source)."""
 
     def __init__(self, T, background, *, box, periodic, eps_ko, gamma0, gamma2,
-                 interior=None, gauge_source=None):
+                 interior=None, gauge_source=None, bounds=None):
         _check_float(T)
+        if bounds is not None and not isinstance(bounds, StateBounds):
+            raise ArgumentError("bounds is None or a StateBounds")
+        if bounds is not None and interior is None:
+            raise ArgumentError("this case carries range bounds but no interior: the projection is "
+                                "gated on r < r_gate from the hole's center")
+        self.bounds = bounds
         if not gamma2 > -1:
             raise ArgumentError(f"γ2 must satisfy γ2 > −1, got {gamma2}")
         if isinstance(eps_ko, HorizonDissipation):
@@ -318,7 +370,7 @@ def gauge_wave_case(T=None, *, A=1 / 20, d=1.0, eps_ko, gamma0, gamma2):
 
 def kerr_schild_case(T=None, *, M=1.0, a=0.0, halfwidth, r_0, r_1, rho_max=None, eps_ko=0.5,
                      gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
-                     gauge_source=None):
+                     gauge_source=None, bounds=None):
     """kerr_schild_case / hole_case (src/initialdata.jl:1044-1174,1274-1281) for a
     hole at rest at the origin in a Dirichlet box. ρ_max defaults to the
     physical rate 4/M the driver sets per chunk. gauge_source=None samples
@@ -330,7 +382,8 @@ def kerr_schild_case(T=None, *, M=1.0, a=0.0, halfwidth, r_0, r_1, rho_max=None,
                      variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp)
     L = float(halfwidth)
     return GHCase(T, KerrSchild(M, a), box=[(-L, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
-                  gamma0=damping, gamma2=gamma2, interior=inter, gauge_source=gauge_source)
+                  gamma0=damping, gamma2=gamma2, interior=inter, gauge_source=gauge_source,
+                  bounds=bounds)
 
 
 class Forest:
@@ -378,6 +431,11 @@ def _desc(forest, case, q):
     else:
         d.dissipation, d.eps_ko = 0, case.eps_ko
     d.gamma2 = case.gamma2
+    bd = getattr(case, "bounds", None)
+    if bd is not None:
+        (d.alpha_min, d.alpha_max, d.lambda_min, d.lambda_max, d.beta_max, d.K_max,
+         d.r_gate) = (bd.alpha_min, bd.alpha_max, bd.lambda_min, bd.lambda_max, bd.beta_max, bd.K_max,
+                      bd.r_gate)
     it = case.interior
     if it is not None:
         d.interior = Interior.VARIANTS[it.variant]
@@ -479,6 +537,15 @@ def stage_limit(u, p, t):
 def step_limit(u, p, t):
     """step_limit!(u, p, t) (src/stepping.jl:97)."""
     _check(_load().tgh_step_limit(p.ptr, _dev(u, p, "u"), float(t)))
+
+
+def apply_bounds(p, u, t):
+    """apply_bounds!(p, u, t) -> points moved (src/bounds.jl:733-770): the range
+    projection inside the gate, written back only where it fired."""
+    n, nf, rm = C.c_int64(), C.c_int64(), C.c_double()
+    _check(_load().tgh_apply_bounds(p.ptr, _dev(u, p, "u"), float(t), C.byref(n), C.byref(nf), C.byref(rm)))
+    p.bounds_last = (n.value, nf.value, rm.value)
+    return n.value
 
 
 def max_speed(p, u, t):

@@ -129,6 +129,31 @@ int check_desc(const tgh_desc* d) {
       return fail(TGH_EINVAL, "the profile rises from the horizon inward, so it needs 0 < r_1 < r_h, got r_1 = " +
                                   std::to_string(d->eps_r1) + " and r_h = " + std::to_string(d->eps_rh));
   }
+  if (d->r_gate != 0) {
+    // StateBounds (src/bounds.jl:94-135) and check_bounds_gate (:214-236)
+    const double v[7] = {d->alpha_min, d->alpha_max, d->lambda_min, d->lambda_max, d->beta_max, d->K_max, d->r_gate};
+    for (double x : v)
+      if (!std::isfinite(x))
+        return fail(TGH_EINVAL, "every bound must be finite: a bound at infinity is no bound, and it is spelled "
+                                "r_gate = 0 (no range projection)");
+    if (!(0 < d->alpha_min && d->alpha_min < 1 && 1 < d->alpha_max))
+      return fail(TGH_EINVAL, "the lapse range needs 0 < alpha_min < 1 < alpha_max: flat space's alpha = 1 has to "
+                              "be inside the range for the non-finite repair to be idempotent");
+    if (!(0 < d->lambda_min && d->lambda_min < 1 && 1 < d->lambda_max))
+      return fail(TGH_EINVAL, "the spatial metric's eigenvalue range needs 0 < lambda_min < 1 < lambda_max");
+    if (!(d->beta_max > 0)) return fail(TGH_EINVAL, "the shift's norm bound must be positive");
+    if (!(d->K_max > 0)) return fail(TGH_EINVAL, "the momentum's scale bound must be positive");
+    if (!(d->r_gate > 0))
+      return fail(TGH_EINVAL, "the gate radius must be positive (r_gate = 0 is a case without bounds)");
+    if (d->interior == TGH_INTERIOR_NONE)
+      return fail(TGH_EINVAL, "this case carries range bounds but no interior: the projection is gated on "
+                              "r < r_gate from the hole's center");
+    const double reach = std::max((double)G, std::sqrt(2.0) * (d->q / 2));
+    if (!(d->r_gate <= d->r1 - reach * h0))
+      return fail(TGH_EINVAL, "the range projection's gate is too shallow: r_gate = " + std::to_string(d->r_gate) +
+                                  " but an evolved point (r >= r_1) reads points down to r_1 - R h = " +
+                                  std::to_string(d->r1 - reach * h0) + " (check_bounds_gate)");
+  }
   if (d->damping == TGH_DAMPING_GAUSSIAN && !(d->gamma0_width > 0))
     return fail(TGH_EINVAL, "the damping profile's width must be positive");
   return TGH_OK;
@@ -307,6 +332,8 @@ int ensure_brick(tgh_problem* p) {
   return TGH_OK;
 }
 
+void launch_bounds_brick(tgh_problem* p, double* A, double* B);
+
 // One RK4 step on the brick arrays (u_n in Bus, the stage input in BW[0]).
 void rk4_brick_launches(tgh_problem* p, double t, double dt) {
   const double c_acc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
@@ -335,7 +362,53 @@ void rk4_brick_launches(tgh_problem* p, double t, double dt) {
       case 6: launch_rhs_brick<6>(p, a); break;
       default: launch_rhs_brick<8>(p, a); break;
     }
+    // the stage limiter on the stage value (and on the step's result)
+    if (p->has_bounds) launch_bounds_brick(p, s == 3 ? p->Bus : Wn, s == 3 ? Wn : nullptr);
   }
+}
+
+// The range projection on the state (mode 0, with accounting partials), on
+// a block work array's owned points (+ the state), or on the brick points
+// inside the gate's bounding box (+ a second brick array).
+void launch_bounds_state(tgh_problem* p, double* u) {
+  BoundsArgs a{};
+  a.g = p->geom;
+  a.c = p->cs;
+  a.bd = p->bd;
+  a.dst0 = u;
+  a.out = p->red;
+  a.mode = 0;
+  hipLaunchKernelGGL(k_bounds, grid_points(p), dim3(256), 0, p->stream, a);
+}
+void launch_bounds_work(tgh_problem* p, double* W, double* us) {
+  BoundsArgs a{};
+  a.g = p->geom;
+  a.c = p->cs;
+  a.bd = p->bd;
+  a.dst0 = W;
+  a.dst1 = us;
+  a.mode = 1;
+  hipLaunchKernelGGL(k_bounds, grid_points(p), dim3(256), 0, p->stream, a);
+}
+void launch_bounds_brick(tgh_problem* p, double* A, double* B) {
+  const Brick& k = p->brick;
+  BoundsArgs a{};
+  a.g = p->geom;
+  a.k = k;
+  a.c = p->cs;
+  a.bd = p->bd;
+  a.dst0 = A;
+  a.dst1 = B;
+  a.mode = 2;
+  long n = 1;
+  for (int d = 0; d < 3; ++d) {  // the gate's bounding box, one point of margin each side
+    const int i0 = (int)std::floor((p->cs.center[d] - p->bd.r_gate - k.lo[d]) * k.inv_h) - 1;
+    const int i1 = (int)std::ceil((p->cs.center[d] + p->bd.r_gate - k.lo[d]) * k.inv_h) + 1;
+    a.lo[d] = std::max(0, i0);
+    a.ext[d] = std::max(0, std::min(k.n[d] - 1, i1) - a.lo[d] + 1);
+    n *= a.ext[d];
+  }
+  if (n > 0) hipLaunchKernelGGL(k_bounds, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, a);
 }
 
 // One RK4 step on device arrays: u (state, in place), stage work arrays
@@ -359,6 +432,8 @@ void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
     a.cnext = c_next[s];
     a.stage = s + 1;
     launch_rhs(p, a, EPI_RK4);
+    // the stage limiter on the stage value (and on the step's result)
+    if (p->has_bounds) launch_bounds_work(p, Wn, s == 3 ? u : nullptr);
   }
 }
 
@@ -418,6 +493,10 @@ int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
   c.eps_in = d->eps_in;
   c.eps_r1 = d->eps_r1;
   c.eps_rh = d->eps_rh;
+  if (d->r_gate > 0) {
+    p->has_bounds = true;
+    p->bd = Bounds{d->alpha_min, d->alpha_max, d->lambda_min, d->lambda_max, d->beta_max, d->K_max, d->r_gate};
+  }
   c.has_diss = d->dissipation == TGH_DISSIPATION_CONSTANT ? d->eps_ko != 0.0
                                                            : !(d->eps_ko == 0.0 && d->eps_in == 0.0);
   c.g0 = d->gamma0;
@@ -556,8 +635,40 @@ int tgh_settle(tgh_problem* p, double* u, double t) {
   return sync(p);
 }
 
+int tgh_apply_bounds(tgh_problem* p, double* u, double t, int64_t* moved, int64_t* nonfinite, double* r_max) {
+  if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
+  (void)t;  // the gate's center is static
+  int64_t n = 0, nf = 0;
+  double rm = -1.0;
+  if (p->has_bounds) {
+    launch_bounds_state(p, u);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(p->red_host.data(), p->red, sizeof(double) * p->red_host.size(),
+                           hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    double hits = 0, nfs = 0, r = 0;
+    for (size_t w = 0; w < p->red_host.size() / 3; ++w) {
+      hits += p->red_host[3 * w];
+      nfs += p->red_host[3 * w + 1];
+      r = std::max(r, p->red_host[3 * w + 2]);
+    }
+    n = (int64_t)hits;
+    nf = (int64_t)nfs;
+    if (n > 0) rm = r;
+  }
+  if (moved) *moved = n;
+  if (nonfinite) *nonfinite = nf;
+  if (r_max) *r_max = rm;
+  return TGH_OK;
+}
+
 int tgh_stage_limit(tgh_problem* p, double* u, double t) {
   if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
+  if (p->has_bounds) {
+    launch_bounds_state(p, u);
+    int rc = sync(p);
+    if (rc) return rc;
+  }
   if (p->cs.interior != INT_PASTED) return TGH_OK;
   PointArgs a{p->geom, p->cs, t, u, 0, 0.0, 2};
   hipLaunchKernelGGL(k_pointwise, grid_points(p), dim3(256), 0, p->stream, a);

@@ -7,6 +7,7 @@
 //   work   W[i, j, k, v, b], (N+1+2G)^3 per block, owned point (i,j,k) at
 //          (i+G, j+G, k+G); G = q/2 + 1 (src/evolution.jl:1047-1053).
 #pragma once
+#include "tgh_bounds.hpp"
 #include "tgh_device.hpp"
 
 namespace tgh {
@@ -1089,6 +1090,118 @@ __global__ __launch_bounds__(256) void k_brick_sample(BrickSampleArgs a) {
   double* out = a.Hsrc + brick_index(k, x, y, z);
   for (int q = 0; q < 4; ++q) out[q * k.fs] = Hl[q];
   for (int n = 0; n < 16; ++n) out[(4 + n) * k.fs] = dH[n % 4][n / 4];
+}
+
+
+// The range projection over points within the gate r < r_gate (apply_bounds!,
+// src/bounds.jl:613-660,733-770), written back only where it fired.
+//   mode 0: the state u (block layout), one partial (hits, non-finite, r_max)
+//           per workgroup into out, folded on the host in order;
+//   mode 1: a block work array's owned points (src = dst0 = W), and the
+//           state at the same point too when dst1 (the RK4 result) is given;
+//   mode 2: brick points of the box [lo, lo + ext) (src = dst0), and dst1 at
+//           the same index when given.
+struct BoundsArgs {
+  Geom g;
+  Brick k;
+  CaseP c;
+  Bounds bd;
+  double* __restrict__ dst0;
+  double* __restrict__ dst1;
+  double* __restrict__ out;
+  int mode;
+  int lo[3], ext[3];
+};
+__global__ __launch_bounds__(256) void k_bounds(BoundsArgs a) {
+  const Geom& g = a.g;
+  double x[3];
+  long i0 = 0, s0 = 0, i1 = 0, s1 = 0;  // index and field stride in dst0 / dst1
+  bool valid = false;
+  int b = 0, wx = 0;
+  if (a.mode == 2) {
+    const long n = (long)a.ext[0] * a.ext[1] * a.ext[2];
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) {
+      const int gp[3] = {a.lo[0] + (int)(t % a.ext[0]), a.lo[1] + (int)((t / a.ext[0]) % a.ext[1]),
+                         a.lo[2] + (int)(t / ((long)a.ext[0] * a.ext[1]))};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) x[d] = a.k.lo[d] + (double)gp[d] * a.k.h;
+      i0 = i1 = brick_index(a.k, gp[0], gp[1], gp[2]);
+      s0 = s1 = a.k.fs;
+      valid = true;
+    }
+  } else {
+    const int P = g.P, S = g.S, G = g.G;
+    if (!wg_map((P * P * P + 255) / 256, g.nblocks, b, wx)) return;
+    const int p = wx * blockDim.x + threadIdx.x;
+    if (p < P * P * P) {
+      const int i = p % P, j = (p / P) % P, kk = p / (P * P);
+      position(g, b, i, j, kk, x);
+      const long psz = (long)P * P * P, sv = (long)S * S * S;
+      const long si = (long)b * 20 * psz + p;
+      if (a.mode == 0) {
+        i0 = si;
+        s0 = psz;
+      } else {
+        i0 = (long)b * 20 * sv + (i + G) + (long)S * ((j + G) + (long)S * (kk + G));
+        s0 = sv;
+        i1 = si;
+        s1 = psz;
+      }
+      valid = true;
+    }
+  }
+  double hit = 0.0, nf = 0.0, rm = 0.0;
+  if (valid) {
+    const double r = hole_radius(a.c, x);
+    if (r < a.bd.r_gate) {
+      double h[NC], Pi[NC];
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        h[v] = a.dst0[i0 + v * s0];
+        Pi[v] = a.dst0[i0 + (NC + v) * s0];
+      }
+      bool nonfin;
+      if (bounds_project(h, Pi, a.bd, nonfin)) {
+        hit = 1.0;
+        nf = nonfin ? 1.0 : 0.0;
+        rm = r;
+#pragma unroll
+        for (int v = 0; v < NC; ++v) {
+          a.dst0[i0 + v * s0] = h[v];
+          a.dst0[i0 + (NC + v) * s0] = Pi[v];
+        }
+        if (a.dst1 != nullptr) {
+#pragma unroll
+          for (int v = 0; v < NC; ++v) {
+            a.dst1[i1 + v * s1] = h[v];
+            a.dst1[i1 + (NC + v) * s1] = Pi[v];
+          }
+        }
+      }
+    }
+  }
+  if (a.mode != 0) return;
+  __shared__ double sh[3][256];
+  const int tid = threadIdx.x;
+  sh[0][tid] = hit;
+  sh[1][tid] = nf;
+  sh[2][tid] = rm;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      sh[0][tid] += sh[0][tid + w];
+      sh[1][tid] += sh[1][tid + w];
+      sh[2][tid] = fmax(sh[2][tid], sh[2][tid + w]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double* o = a.out + 3 * ((long)b * ((g.P * g.P * g.P + 255) / 256) + wx);
+    o[0] = sh[0][0];
+    o[1] = sh[1][0];
+    o[2] = sh[2][0];
+  }
 }
 
 #endif  // TGH_RHS_ONLY

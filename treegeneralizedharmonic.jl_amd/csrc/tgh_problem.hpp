@@ -16,6 +16,8 @@ struct tgh_problem {
   tgh_desc desc;
   tgh::Geom geom;
   tgh::CaseP cs;
+  tgh::Bounds bd{};                   // the range projection's ranges
+  bool has_bounds = false;
   int q = 4;
   hipStream_t stream = nullptr;
   bool own_stream = false;
