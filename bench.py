@@ -55,6 +55,9 @@ def parse():
                     help="the stepper's layout and kernels: the brick layout (default), or the "
                          "per-block work arrays with one fused kernel or head + Pi launches")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain (verbs) form")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with a 1 GiB device-to-device copy, the known-size reference for "
+                         "rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -72,6 +75,21 @@ def kerr_schild_desc(tgh, nblocks):
     d.eps_ko = 0.5
     d.r0, d.r1, d.rho_max, d.w_ramp, d.rho_ramp = 0.71, 1.21, 4.0, 0.5, 0.5
     return d
+
+
+def measured_traffic(kernel, nblocks):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes (profiles/traffic.json), each
+    scaled by what the same counter reads for a 1 GiB device copy in the same
+    run; None where that kernel and workload were not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh).get(kernel)
+    if not rec or rec.get("blocks") != nblocks:
+        return None
+    return rec["fetch_bytes"] + rec["write_bytes"]
 
 
 def flop_count():
@@ -227,7 +245,8 @@ def main():
                                f"source, RK4 hipGraph step",
                    "blocks": nb, "points": points, "q": 4, "dt": dt},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic(KERNEL[dom], nb),
                      "kernel": KERNEL[dom],
                      "kernel_ms": pi_ms if dom.endswith("pi") else head_ms,
                      "head_ms": head_ms, "head_tflops": head_tf, "pi_ms": pi_ms, "pi_tflops": pi_tf,
@@ -239,6 +258,13 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
     L.tgh_problem_destroy(ptr)
+    if args.calibrate_copy:
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     print(json.dumps(result), flush=True)
 
 

@@ -32,14 +32,16 @@ def hipcc():
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
-def build(verbose=False):
-    """Compile every translation unit (in parallel) and link the library."""
-    objdir = os.path.join(HERE, "_build")
+def build(verbose=False, defines=(), out=None):
+    """Compile every translation unit (in parallel) and link the library.
+    defines: extra -D tuning macros; out: another library path (a variant)."""
+    lib = out or LIB
+    objdir = os.path.join(os.path.dirname(lib), "_build", os.path.basename(lib))
     os.makedirs(objdir, exist_ok=True)
     procs, objs = [], []
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
-        cmd = [hipcc()] + HIPCC_FLAGS + ["-c", "-o", obj, src]
+        cmd = [hipcc()] + HIPCC_FLAGS + [f"-D{d}" for d in defines] + ["-c", "-o", obj, src]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
@@ -51,17 +53,17 @@ def build(verbose=False):
             sys.stderr.write(out)
             failed.append(os.path.basename(src))
     if failed:
-        raise RuntimeError(f"hipcc failed on {', '.join(failed)} building {LIB}")
-    tmp = LIB + ".tmp"
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)} building {lib}")
+    tmp = lib + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"linking {LIB} failed ({res.returncode})")
-    os.replace(tmp, LIB)
-    return LIB
+        raise RuntimeError(f"linking {lib} failed ({res.returncode})")
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

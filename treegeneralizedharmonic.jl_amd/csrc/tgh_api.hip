@@ -578,6 +578,7 @@ int tgh_problem_destroy(tgh_problem* p) {
   (void)hipFree(p->Bacc);
   (void)hipFree(p->BH);
   (void)hipFree(p->Bcoef);
+  (void)hipFree(p->Bd);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   delete p;
@@ -752,6 +753,8 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     if (form == 3 && !p->Bcoef) {
       HIP_TRY(hipMalloc(&p->Bcoef, sizeof(double) * CF_N * (size_t)p->brick.fs));
       HIP_TRY(hipMemsetAsync(p->Bcoef, 0, sizeof(double) * CF_N * (size_t)p->brick.fs, p->stream));
+      HIP_TRY(hipMalloc(&p->Bd, sizeof(double) * 4 * NC * (size_t)p->brick.fs));
+      HIP_TRY(hipMemsetAsync(p->Bd, 0, sizeof(double) * 4 * NC * (size_t)p->brick.fs, p->stream));
     }
     // u_n into the stepper's u_n and first stage input
     BrickCopyArgs ca{p->geom, p->brick, u, p->BW[0], p->Bus, 0};

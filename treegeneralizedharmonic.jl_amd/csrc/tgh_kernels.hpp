@@ -278,6 +278,30 @@ __device__ __forceinline__ void emit(const RhsArgs& a, TGH_OUT_PARAMS, long si, 
   }
 }
 
+// The RK4 epilogue in two halves, so that a kernel can issue the loads of u_n
+// and of the accumulator early: epi_load reads what stage s needs (u_n in
+// stages 1-3, the accumulator in stages 2-4), epi_store applies the stage.
+__device__ __forceinline__ void epi_load(const RhsArgs& a, const double* __restrict__ us,
+                                         const double* __restrict__ acc, long si, double& u, double& ac) {
+  u = a.stage < 4 ? us[si] : 0.0;
+  ac = a.stage > 1 ? acc[si] : 0.0;
+}
+__device__ __forceinline__ void epi_store(const RhsArgs& a, double* __restrict__ us, double* __restrict__ acc,
+                                          double* __restrict__ Wn, long si, long wi, double k, double u,
+                                          double ac) {
+  if (a.stage == 1) {
+    acc[si] = u + a.cacc * k;
+    Wn[wi] = u + a.cnext * k;
+  } else if (a.stage < 4) {
+    acc[si] = ac + a.cacc * k;
+    Wn[wi] = u + a.cnext * k;
+  } else {
+    const double un = ac + a.cacc * k;
+    us[si] = un;
+    Wn[wi] = un;
+  }
+}
+
 // Where one point's inputs and outputs live: the stage input's center w0 with
 // its field stride sv and row and plane strides s2, s3 (the x stride is 1);
 // the epilogue's state index si0 with field stride ssv (us, acc, du) and work
@@ -311,48 +335,78 @@ struct PiCoef {
 // derivatives and the coefficients, d_t h complete (through the epilogue), the
 // source, the interior layer's weights. Returns false in the frozen core,
 // where F is not evaluated (a hole at rest: d_t h = 0 is emitted).
-template <int Q, int GAUGE, bool HOLE, int EPI>
-__device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr, double inv_h, PiCoef& pc,
-                                           TGH_OUT_PARAMS) {
-  const double* __restrict__ w0 = pr.w0;
-  const long sv = pr.sv, psz = pr.ssv, si0 = pr.si0, wi0 = pr.wi0;
-  const int s1 = 1, s2 = pr.s2, s3 = pr.s3;
-  const double* x = pr.x;
-  const CaseP& c = a.c;
-  double wgt = 1.0, rho = 0.0, r = 0.0;
-  if (HOLE) {
-    r = hole_radius(c, x);
-    const bool frozen = c.interior == INT_PASTED ? r < c.r1 : r < c.r0;
-    if (frozen) {
-#pragma unroll 1
-      for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, 0.0);
-      return false;
-    }
-  }
-  const double g0 = damping_rate(c, x);
-  const double epsh = dissipation_rate(c, x) * inv_h;
-  const double ih = inv_h;
+// Where the head reads a point's stencils: along x from X(v) (stride 1), along
+// y from Y(v) (stride sy), along z from Z(v) (stride sz), the centre h_v at
+// X(v)[0] and Pi_v from P(v). GlobalSrc reads the stage input in place (the
+// three are the same pointer); the LDS head stages them per tile.
+struct GlobalSrc {
+  const double* __restrict__ w0;
+  long sv;
+  int sy, sz;
+  __device__ __forceinline__ const double* X(int v) const { return w0 + v * sv; }
+  __device__ __forceinline__ const double* Y(int v) const { return w0 + v * sv; }
+  __device__ __forceinline__ const double* Z(int v) const { return w0 + v * sv; }
+  __device__ __forceinline__ double P(int v) const { return w0[(NC + v) * sv]; }
+};
 
-  double hv[NC], D[4][NC];
+// The frozen core (F is not evaluated there: a hole at rest, du = 0) and the
+// point's radius from the hole's centre.
+template <bool HOLE>
+__device__ __forceinline__ bool frozen_core(const CaseP& c, const double x[3], double& r) {
+  r = 0.0;
+  if (!HOLE) return false;
+  r = hole_radius(c, x);
+  return c.interior == INT_PASTED ? r < c.r1 : r < c.r0;
+}
+
+// The head's first half (gh_rhs_head, src/evolution.jl:258-307): the state,
+// the 30 first derivatives, the coefficient set and d_t h with its
+// dissipation (D[0]); D[1..3] = d_i h.
+template <int Q, class SRC>
+__device__ __forceinline__ void head_derivs(const CaseP& c, const SRC& src, double inv_h, double epsh,
+                                            double hv[NC], double D[4][NC], Coef& cf) {
+  const double ih = inv_h;
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
-    const double* pv = w0 + v * sv;
-    hv[v] = pv[0];
-    D[1][v] = ih * st_d1<Q>(pv, s1);
-    D[2][v] = ih * st_d1<Q>(pv, s2);
-    D[3][v] = ih * st_d1<Q>(pv, s3);
+    hv[v] = src.X(v)[0];
+    D[1][v] = ih * st_d1<Q>(src.X(v), 1);
+    D[2][v] = ih * st_d1<Q>(src.Y(v), src.sy);
+    D[3][v] = ih * st_d1<Q>(src.Z(v), src.sz);
   }
-  Coef cf;
   metric_quantities(hv, cf);
   const double a_div = cf.alpha / cf.sqrtg;
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
-    const double* pv = w0 + v * sv;
-    double s = cf.beta[0] * D[1][v] + cf.beta[1] * D[2][v] + cf.beta[2] * D[3][v] +
-               a_div * pv[NC * sv];
-    if (c.has_diss) s += epsh * (st_ko<Q>(pv, s1) + st_ko<Q>(pv, s2) + st_ko<Q>(pv, s3));
+    double s = cf.beta[0] * D[1][v] + cf.beta[1] * D[2][v] + cf.beta[2] * D[3][v] + a_div * src.P(v);
+    if (c.has_diss)
+      s += epsh * (st_ko<Q>(src.X(v), 1) + st_ko<Q>(src.Y(v), src.sy) + st_ko<Q>(src.Z(v), src.sz));
     D[0][v] = s;
   }
+}
+
+// The Pi components' geometric coefficients: beta^i, d_i beta^i, d_i A^{ij} and
+// A^{ij} = alpha sqrt(gamma) gamma^{ij} (metric_divergences, src/pointwise.jl:754-778).
+__device__ __forceinline__ void head_geometry(const Coef& cf, const double D[4][NC], PiCoef& pc) {
+  metric_divergences(cf, &D[1], pc.divb, pc.divA);
+  const double as = cf.alpha * cf.sqrtg;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) pc.A[n] = as * cf.gamu[n];
+  pc.b[0] = cf.beta[0];
+  pc.b[1] = cf.beta[1];
+  pc.b[2] = cf.beta[2];
+}
+
+// The head's second half: the gauge source, the source msrc (gh_node_source_lean),
+// matter, and the interior layer -- d_t h through the epilogue, and the
+// weight w and c1_v = w msrc_v - rho (Pi_v - Pi_exact,v) into pc. Pi_v is
+// read through src.P.
+template <int GAUGE, bool HOLE, int EPI, class SRC>
+__device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr, const SRC& src, double r,
+                                            const Coef& cf, const double hv[NC], const double D[4][NC],
+                                            PiCoef& pc, TGH_OUT_PARAMS) {
+  const long sv = pr.sv, psz = pr.ssv, si0 = pr.si0, wi0 = pr.wi0;
+  const CaseP& c = a.c;
+  const double g0 = damping_rate(c, pr.x);
   double msrc[NC];
   {
     double Hl[4] = {0.0, 0.0, 0.0, 0.0}, dH[4][4];
@@ -378,25 +432,15 @@ __device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr,
 #pragma unroll
     for (int n = 0; n < NC; ++n) msrc[n] += ms[n];
   }
-  metric_divergences(cf, &D[1], pc.divb, pc.divA);
-  const double as = cf.alpha * cf.sqrtg;
-#pragma unroll
-  for (int n = 0; n < 6; ++n) pc.A[n] = as * cf.gamu[n];
-  pc.b[0] = cf.beta[0];
-  pc.b[1] = cf.beta[1];
-  pc.b[2] = cf.beta[2];
-
-  if (HOLE) {
-    if (r < c.r1) {
-      const double sl = (r - c.r0) / (c.r1 - c.r0);
-      wgt = smoothstep(sl / c.w_ramp);
-      rho = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
-    }
+  double wgt = 1.0, rho = 0.0;
+  const bool layer = HOLE && r < c.r1;
+  if (layer) {
+    const double sl = (r - c.r0) / (c.r1 - c.r0);
+    wgt = smoothstep(sl / c.w_ramp);
+    rho = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
   }
   double ue[20];
-  const bool layer = HOLE && r < c.r1;
-  if (layer) case_state(c, a.t, x, ue);  // r >= r0 here: the core rule is the identity
-
+  if (layer) case_state(c, a.t, pr.x, ue);  // r >= r0 here: the core rule is the identity
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
     double f = D[0][v];
@@ -407,9 +451,28 @@ __device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr,
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
     double c1 = msrc[v];
-    if (layer) c1 = wgt * c1 - rho * (w0[(NC + v) * sv] - ue[NC + v]);
+    if (layer) c1 = wgt * c1 - rho * (src.P(v) - ue[NC + v]);
     pc.c1[v] = c1;
   }
+}
+
+// The whole head at one point (gh_rhs_head, src/evolution.jl:258-297, inside
+// gh_rhs_point!, :448-695,846-863). Returns false in the frozen core, where
+// d_t h = 0 is emitted.
+template <int Q, int GAUGE, bool HOLE, int EPI, class SRC>
+__device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr, const SRC& src, double inv_h,
+                                           PiCoef& pc, TGH_OUT_PARAMS) {
+  double r;
+  if (frozen_core<HOLE>(a.c, pr.x, r)) {
+#pragma unroll 1
+    for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * pr.ssv, pr.wi0 + v * pr.sv, 0.0);
+    return false;
+  }
+  double hv[NC], D[4][NC];
+  Coef cf;
+  head_derivs<Q>(a.c, src, inv_h, dissipation_rate(a.c, pr.x) * inv_h, hv, D, cf);
+  head_source<GAUGE, HOLE, EPI>(a, pr, src, r, cf, hv, D, pc, TGH_OUT_ARGS);
+  head_geometry(cf, D, pc);
   return true;
 }
 
@@ -442,7 +505,8 @@ template <int Q, int GAUGE, bool HOLE, int EPI>
 __device__ __forceinline__ void rhs_point(const RhsArgs& a, const PointRef& pr, double inv_h, TGH_OUT_PARAMS) {
   PiCoef pc;
   const long sv = pr.sv, psz = pr.ssv;
-  if (!head_point<Q, GAUGE, HOLE, EPI>(a, pr, inv_h, pc, TGH_OUT_ARGS)) {
+  const GlobalSrc src{pr.w0, pr.sv, pr.s2, pr.s3};
+  if (!head_point<Q, GAUGE, HOLE, EPI>(a, pr, src, inv_h, pc, TGH_OUT_ARGS)) {
 #pragma unroll 1
     for (int v = NC; v < 20; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * psz, pr.wi0 + v * sv, 0.0);
     return;
@@ -468,6 +532,22 @@ __device__ __forceinline__ void store_coef(double* __restrict__ cf0, long cs, co
   for (int n = 0; n < 3; ++n) cf0[(CF_DIVA + n) * cs] = pc.divA[n];
 #pragma unroll
   for (int n = 0; n < 6; ++n) cf0[(CF_A + n) * cs] = pc.A[n];
+  cf0[CF_W * cs] = pc.w;
+#pragma unroll
+  for (int n = 0; n < NC; ++n) cf0[(CF_C1 + n) * cs] = pc.c1[n];
+}
+// The geometric and the source halves of the coefficients, for the head in
+// two launches.
+__device__ __forceinline__ void store_coef_geometry(double* __restrict__ cf0, long cs, const PiCoef& pc) {
+#pragma unroll
+  for (int n = 0; n < 3; ++n) cf0[(CF_B0 + n) * cs] = pc.b[n];
+  cf0[CF_DIVB * cs] = pc.divb;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) cf0[(CF_DIVA + n) * cs] = pc.divA[n];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) cf0[(CF_A + n) * cs] = pc.A[n];
+}
+__device__ __forceinline__ void store_coef_source(double* __restrict__ cf0, long cs, const PiCoef& pc) {
   cf0[CF_W * cs] = pc.w;
 #pragma unroll
   for (int n = 0; n < NC; ++n) cf0[(CF_C1 + n) * cs] = pc.c1[n];
@@ -531,7 +611,8 @@ __global__ __launch_bounds__(256) void k_head(RhsArgs a, const double* __restric
   if (!block_point(a, W, pr, pi)) return;
   const long cs = (long)a.g.nblocks * a.g.P * a.g.P * a.g.P;
   PiCoef pc;
-  if (head_point<Q, GAUGE, HOLE, EPI>(a, pr, a.g.inv_h, pc, TGH_OUT_ARGS))
+  const GlobalSrc src{pr.w0, pr.sv, pr.s2, pr.s3};
+  if (head_point<Q, GAUGE, HOLE, EPI>(a, pr, src, a.g.inv_h, pc, TGH_OUT_ARGS))
     store_coef(coef + pi, cs, pc);
   else
     coef[CF_W * cs + pi] = -1.0;
@@ -813,11 +894,38 @@ __device__ __forceinline__ bool brick_map(const Brick& k, int& x, int& y, int& z
   const long plane = (long)k.n[0] * k.n[1];
   const int wpp = (int)((plane + 255) / 256);
   int r;
+#ifdef TGH_BRICK_ZFAST
+  // runs ordered z-fastest: consecutive workgroups of an XCD take the same
+  // run of consecutive planes, which share their z-stencil planes
+  if (!wg_map(k.n[2], wpp, r, z)) return false;
+#else
   if (!wg_map(wpp, k.n[2], z, r)) return false;
+#endif
   const long lin = (long)r * 256 + threadIdx.x;
   if (lin >= plane) return false;
   y = (int)(lin / k.n[0]);
   x = (int)(lin - (long)y * k.n[0]);
+  return true;
+}
+
+// Tiles of TX x TY x TZ points over the brick, dealt to the XCDs in
+// contiguous ranges of a z-fastest (TGH_TILES_ZFAST) or x-fastest order.
+// z-fastest: the tiles an XCD holds at once are a column of consecutive
+// z-tiles, which share their z-halo planes in its L2.
+__device__ __forceinline__ bool tile_map(const Brick& k, int TX, int TY, int TZ, int& X0, int& Y0, int& Z0) {
+  const int ntx = (k.n[0] + TX - 1) / TX, nty = (k.n[1] + TY - 1) / TY, ntz = (k.n[2] + TZ - 1) / TZ;
+  int tile, unused;
+  if (!wg_map(1, ntx * nty * ntz, tile, unused)) return false;
+#ifdef TGH_TILES_ZFAST
+  const int tz = tile % ntz, txy = tile / ntz;
+  X0 = (txy % ntx) * TX;
+  Y0 = (txy / ntx) * TY;
+  Z0 = tz * TZ;
+#else
+  X0 = (tile % ntx) * TX;
+  Y0 = ((tile / ntx) % nty) * TY;
+  Z0 = (tile / (ntx * nty)) * TZ;
+#endif
   return true;
 }
 
@@ -853,11 +961,29 @@ __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const dou
 // The split form on the brick layout, first half: the head at one point per
 // thread (brick_map's order), leaving the Pi coefficients in coef (CF_N
 // arrays of the brick's field size, indexed like the brick).
+#ifdef TGH_HEAD_WAVES_PER_EU
+#define TGH_HEAD_ATTR __attribute__((amdgpu_waves_per_eu(TGH_HEAD_WAVES_PER_EU)))
+#else
+#define TGH_HEAD_ATTR
+#endif
 template <int Q, int GAUGE, bool HOLE>
-__global__ __launch_bounds__(256) void k_brick_head(RhsArgs a, Brick k, const double* __restrict__ W,
+__global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head(RhsArgs a, Brick k, const double* __restrict__ W,
                                                     double* __restrict__ coef, TGH_OUT_PARAMS) {
   int x, y, z;
+#ifdef TGH_HEAD_TILE_X
+  {  // tiles of TGH_HEAD_TILE_X x (256 / (TGH_HEAD_TILE_X * 4)) x 4 points
+    constexpr int TX = TGH_HEAD_TILE_X, TZ = 4, TY = 256 / (TX * TZ);
+    int X0, Y0, Z0;
+    if (!tile_map(k, TX, TY, TZ, X0, Y0, Z0)) return;
+    const int t = threadIdx.x;
+    x = X0 + t % TX;
+    y = Y0 + (t / TX) % TY;
+    z = Z0 + t / (TX * TY);
+    if (!(x < k.n[0] && y < k.n[1] && z < k.n[2])) return;
+  }
+#else
   if (!brick_map(k, x, y, z)) return;
+#endif
   const long off = brick_index(k, x, y, z);
   PointRef pr;
   pr.w0 = W + off;
@@ -877,72 +1003,354 @@ __global__ __launch_bounds__(256) void k_brick_head(RhsArgs a, Brick k, const do
   pr.m_Sij = nullptr;
   pr.ms = 0;
   PiCoef pc;
-  if (head_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, k.inv_h, pc, TGH_OUT_ARGS))
+  const GlobalSrc src{pr.w0, pr.sv, pr.s2, pr.s3};
+  if (head_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, src, k.inv_h, pc, TGH_OUT_ARGS))
     store_coef(coef + off, k.fs, pc);
   else
     coef[CF_W * k.fs + off] = -1.0;
 }
 
-// The split form on the brick layout, second half: a workgroup takes a tile
-// of TX x TY x TZ points (one per thread) and, one component at a time,
-// stages the tile's box of h_v and Pi_v (the tile and a halo of G = q/2 + 1
-// points, the reach of the dissipation stencil) in LDS with coalesced loads;
-// every stencil of the component is then read from LDS. Tiles go to the XCDs
-// in contiguous ranges of x-fastest tile order.
-template <int Q> struct PiTile {  // 8 x 8 x 4 (8 x 8 x 2 at q = 8, to fit 64 KiB of LDS)
-  static constexpr int X = 8, Y = 8, Z = Q == 8 ? 2 : 4;
+// The head in two launches on the brick (the default). The stencils and the
+// algebra of the head do not fit one kernel at more than one wave per SIMD:
+// held together, the 190 stencil values the compiler issues early and the
+// source's live set take ~400 registers. So k_brick_derivs evaluates the
+// stencil half -- h, the 30 first derivatives, d_t h with its dissipation, the
+// geometric coefficients -- and leaves D = (d_t h, d_x h, d_y h, d_z h), 40
+// values per point, in a brick-layout scratch array; k_brick_source reads
+// them back with 20 contiguous loads of the state and evaluates the source,
+// the interior layer and the RK4 epilogue of d_t h. Every value is computed by
+// the same code in the same order as the one-kernel head, so the two forms
+// agree bit for bit.
+#ifndef TGH_DERIVS_WAVES_PER_EU
+#define TGH_DERIVS_WAVES_PER_EU 2
+#endif
+__device__ __forceinline__ void brick_point_ref(const RhsArgs& a, const Brick& k, const double* W, int x, int y,
+                                                int z, PointRef& pr) {
+  const long off = brick_index(k, x, y, z);
+  pr.w0 = W + off;
+  pr.sv = k.fs;
+  pr.s2 = (int)k.py;
+  pr.s3 = (int)k.pz;
+  pr.x[0] = k.lo[0] + (double)x * k.h;
+  pr.x[1] = k.lo[1] + (double)y * k.h;
+  pr.x[2] = k.lo[2] + (double)z * k.h;
+  pr.si0 = off;
+  pr.ssv = k.fs;
+  pr.wi0 = off;
+  pr.Hp = a.Hsrc ? a.Hsrc + off : nullptr;
+  pr.hs = k.fs;
+  pr.m_rho = nullptr;
+  pr.m_S = nullptr;
+  pr.m_Sij = nullptr;
+  pr.ms = 0;
+}
+template <int Q, bool HOLE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_DERIVS_WAVES_PER_EU))) void k_brick_derivs(
+    RhsArgs a, Brick k, const double* __restrict__ W, double* __restrict__ coef, double* __restrict__ Dbuf) {
+  int x, y, z;
+  if (!brick_map(k, x, y, z)) return;
+  const long off = brick_index(k, x, y, z);
+  const double xp[3] = {k.lo[0] + (double)x * k.h, k.lo[1] + (double)y * k.h, k.lo[2] + (double)z * k.h};
+  double r;
+  if (frozen_core<HOLE>(a.c, xp, r)) return;
+  const GlobalSrc src{W + off, k.fs, (int)k.py, (int)k.pz};
+  double hv[NC], D[4][NC];
+  Coef cf;
+  head_derivs<Q>(a.c, src, k.inv_h, dissipation_rate(a.c, xp) * k.inv_h, hv, D, cf);
+  PiCoef pc;
+  head_geometry(cf, D, pc);
+  store_coef_geometry(coef + off, k.fs, pc);
+  double* __restrict__ d0 = Dbuf + off;
+#pragma unroll
+  for (int n = 0; n < 4 * NC; ++n) d0[n * k.fs] = D[n / NC][n % NC];
+}
+template <int GAUGE, bool HOLE>
+__global__ __launch_bounds__(256) void k_brick_source(RhsArgs a, Brick k, const double* __restrict__ W,
+                                                      const double* __restrict__ Dbuf, double* __restrict__ coef,
+                                                      TGH_OUT_PARAMS) {
+  int x, y, z;
+  if (!brick_map(k, x, y, z)) return;
+  PointRef pr;
+  brick_point_ref(a, k, W, x, y, z, pr);
+  const long off = pr.si0;
+  double r;
+  if (frozen_core<HOLE>(a.c, pr.x, r)) {
+#pragma unroll 1
+    for (int v = 0; v < NC; ++v) emit<EPI_RK4>(a, TGH_OUT_ARGS, off + v * k.fs, off + v * k.fs, 0.0);
+    coef[CF_W * k.fs + off] = -1.0;
+    return;
+  }
+  double hv[NC], D[4][NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) hv[v] = W[off + v * k.fs];
+#pragma unroll
+  for (int n = 0; n < 4 * NC; ++n) D[n / NC][n % NC] = Dbuf[off + n * k.fs];
+  Coef cf;
+  metric_quantities(hv, cf);
+  const GlobalSrc src{W + off, k.fs, (int)k.py, (int)k.pz};
+  PiCoef pc;
+  head_source<GAUGE, HOLE, EPI_RK4>(a, pr, src, r, cf, hv, D, pc, TGH_OUT_ARGS);
+  store_coef_source(coef + off, k.fs, pc);
+}
+
+// The head with its inputs staged in LDS (q <= 4): a workgroup takes a tile
+// of 8 x 8 x 4 points, one per thread, and first loads, for the ten h_v, the
+// three lines of points its stencils read -- the tile widened by H = G along
+// x, along y and along z (three arrays, the centre in each) -- and the tile's
+// Pi_v, 70 coalesced loads per thread issued together; then each thread
+// evaluates the head from LDS. 143 KB of LDS at q = 4 (gfx950 allows 160 KB
+// per workgroup), so one workgroup per CU, the head's register count already
+// keeping it at one wave per SIMD.
+template <int Q> struct HeadTile {
+  static constexpr int X = 8, Y = 8, Z = 4, H = ::tgh::W<Q>::R + 1;
+  static constexpr int XA = (X + 2 * H) * Y * Z, YA = X * (Y + 2 * H) * Z, ZA = X * Y * (Z + 2 * H);
+  static constexpr int PA = X * Y * Z;
+  static constexpr int DOUBLES = NC * (XA + YA + ZA + PA);
 };
-template <int Q>
-__global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) void k_brick_pi(
-    RhsArgs a, Brick k, const double* __restrict__ W, const double* __restrict__ coef, TGH_OUT_PARAMS) {
-  constexpr int TX = PiTile<Q>::X, TY = PiTile<Q>::Y, TZ = PiTile<Q>::Z, NT = TX * TY * TZ;
-  constexpr int H = ::tgh::W<Q>::R + 1;
-  constexpr int BX = TX + 2 * H, BY = TY + 2 * H, BZ = TZ + 2 * H, BV = BX * BY * BZ;
-  __shared__ double sh[2][BV];
+struct LdsSrc {
+  const double* xa;
+  const double* ya;
+  const double* za;
+  const double* pa;
+  int fx, fy, fz, fp;
+  int sy, sz;
+  __device__ __forceinline__ const double* X(int v) const { return xa + v * fx; }
+  __device__ __forceinline__ const double* Y(int v) const { return ya + v * fy; }
+  __device__ __forceinline__ const double* Z(int v) const { return za + v * fz; }
+  __device__ __forceinline__ double P(int v) const { return pa[v * fp]; }
+};
+template <int Q, int GAUGE, bool HOLE>
+__global__ __launch_bounds__(256) void k_brick_head_lds(RhsArgs a, Brick k, const double* __restrict__ W,
+                                                        double* __restrict__ coef, TGH_OUT_PARAMS) {
+  using T = HeadTile<Q>;
+  constexpr int TX = T::X, TY = T::Y, TZ = T::Z, H = T::H, NT = TX * TY * TZ;
+  constexpr int WX = TX + 2 * H, WY = TY + 2 * H, WZ = TZ + 2 * H;
+  __shared__ double lds[T::DOUBLES];
+  double* const xa = lds;
+  double* const ya = xa + NC * T::XA;
+  double* const za = ya + NC * T::YA;
+  double* const pa = za + NC * T::ZA;
   const int ntx = (k.n[0] + TX - 1) / TX, nty = (k.n[1] + TY - 1) / TY, ntz = (k.n[2] + TZ - 1) / TZ;
   int tile, unused;
   if (!wg_map(1, ntx * nty * ntz, tile, unused)) return;  // uniform over the workgroup
   const int X0 = (tile % ntx) * TX, Y0 = ((tile / ntx) % nty) * TY, Z0 = (tile / (ntx * nty)) * TZ;
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
+  // brick points at or beyond n + G on any axis are outside the stored extent
+  // (they are read by no evolved point); below -G none is asked for
+  const int mx = k.n[0] + k.G, my = k.n[1] + k.G, mz = k.n[2] + k.G;
+  for (int e = t; e < T::XA; e += NT) {
+    const int gx = X0 - H + e % WX, gy = Y0 + (e / WX) % TY, gz = Z0 + e / (WX * TY);
+    const bool in = gx < mx && gy < my && gz < mz;
+    const long g = in ? brick_index(k, gx, gy, gz) : 0;
+#pragma unroll
+    for (int v = 0; v < NC; ++v) xa[v * T::XA + e] = in ? W[g + v * k.fs] : 0.0;
+  }
+  for (int e = t; e < T::YA; e += NT) {
+    const int gx = X0 + e % TX, gy = Y0 - H + (e / TX) % WY, gz = Z0 + e / (TX * WY);
+    const bool in = gx < mx && gy < my && gz < mz;
+    const long g = in ? brick_index(k, gx, gy, gz) : 0;
+#pragma unroll
+    for (int v = 0; v < NC; ++v) ya[v * T::YA + e] = in ? W[g + v * k.fs] : 0.0;
+  }
+  for (int e = t; e < T::ZA; e += NT) {
+    const int gx = X0 + e % TX, gy = Y0 + (e / TX) % TY, gz = Z0 - H + e / (TX * TY);
+    const bool in = gx < mx && gy < my && gz < mz;
+    const long g = in ? brick_index(k, gx, gy, gz) : 0;
+#pragma unroll
+    for (int v = 0; v < NC; ++v) za[v * T::ZA + e] = in ? W[g + v * k.fs] : 0.0;
+  }
+  {
+    const int gx = X0 + lx, gy = Y0 + ly, gz = Z0 + lz;
+    const bool in = gx < mx && gy < my && gz < mz;
+    const long g = in ? brick_index(k, gx, gy, gz) : 0;
+#pragma unroll
+    for (int v = 0; v < NC; ++v) pa[v * T::PA + t] = in ? W[g + (NC + v) * k.fs] : 0.0;
+  }
+  __syncthreads();
   const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-  const bool valid = x < k.n[0] && y < k.n[1] && z < k.n[2];
-  const long off = valid ? brick_index(k, x, y, z) : 0;
+  if (!(x < k.n[0] && y < k.n[1] && z < k.n[2])) return;  // no barrier follows
+  const long off = brick_index(k, x, y, z);
+  PointRef pr;
+  pr.w0 = W + off;
+  pr.sv = k.fs;
+  pr.s2 = (int)k.py;
+  pr.s3 = (int)k.pz;
+  pr.x[0] = k.lo[0] + (double)x * k.h;
+  pr.x[1] = k.lo[1] + (double)y * k.h;
+  pr.x[2] = k.lo[2] + (double)z * k.h;
+  pr.si0 = off;
+  pr.ssv = k.fs;
+  pr.wi0 = off;
+  pr.Hp = a.Hsrc ? a.Hsrc + off : nullptr;
+  pr.hs = k.fs;
+  pr.m_rho = nullptr;
+  pr.m_S = nullptr;
+  pr.m_Sij = nullptr;
+  pr.ms = 0;
+  const LdsSrc src{xa + (lx + H) + WX * (ly + TY * lz), ya + lx + TX * ((ly + H) + WY * lz),
+                   za + lx + TX * (ly + TY * (lz + H)), pa + t, T::XA, T::YA, T::ZA, T::PA, TX, TX * TY};
   PiCoef pc;
-  pc.w = -1.0;
-  if (valid) load_coef(coef + off, k.fs, pc);
-  const double xp[3] = {k.lo[0] + (double)x * k.h, k.lo[1] + (double)y * k.h, k.lo[2] + (double)z * k.h};
-  const double ih = k.inv_h, ih2 = ih * ih, epsh = dissipation_rate(a.c, xp) * ih;
-  const int c0 = (lx + H) + BX * ((ly + H) + BY * (lz + H));
+  if (head_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, src, k.inv_h, pc, TGH_OUT_ARGS))
+    store_coef(coef + off, k.fs, pc);
+  else
+    coef[CF_W * k.fs + off] = -1.0;
+}
+
+// The split form on the brick layout, second half: a workgroup takes a tile
+// of points (PiTile) and, one component at a time,
+// stages the tile's box of h_v and Pi_v (the tile and a halo of G = q/2 + 1
+// points, the reach of the dissipation stencil) in LDS with coalesced loads;
+// every stencil of the component is then read from LDS. Tiles go to the XCDs
+// in contiguous ranges of x-fastest tile order.
+#ifndef TGH_PI_TILE_X
+#define TGH_PI_TILE_X 8
+#endif
+#ifndef TGH_PI_TILE_Y
+#define TGH_PI_TILE_Y 8
+#endif
+#ifndef TGH_PI_TILE_Z
+#define TGH_PI_TILE_Z 4
+#endif
+#ifndef TGH_PI_POINTS_Z
+#define TGH_PI_POINTS_Z 1
+#endif
+// The tile: X x Y x Z threads, each taking PZ points along z (at z, z + Z, ...);
+// at q = 8 the halo of 5 leaves room for Z = 2, one point each, in 64 KiB.
+#ifndef TGH_PI_PITCH_PAD
+#define TGH_PI_PITCH_PAD 0
+#endif
+template <int Q> struct PiTile {
+  static constexpr int X = TGH_PI_TILE_X, Y = TGH_PI_TILE_Y;
+  static constexpr int Z = Q == 8 ? 2 : TGH_PI_TILE_Z;
+  static constexpr int PZ = Q == 8 ? 1 : TGH_PI_POINTS_Z;
+  static constexpr int pitch(int bx) { return Q == 8 ? bx : bx + TGH_PI_PITCH_PAD; }
+};
+#ifdef TGH_BRICK_PI_WAVES_PER_EU
+#define TGH_BRICK_PI_ATTR __attribute__((amdgpu_waves_per_eu(TGH_BRICK_PI_WAVES_PER_EU)))
+#else
+#define TGH_BRICK_PI_ATTR
+#endif
+template <int Q>
+__global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRICK_PI_ATTR void k_brick_pi(
+    RhsArgs a, Brick k, const double* __restrict__ W, const double* __restrict__ coef, TGH_OUT_PARAMS) {
+  constexpr int TX = PiTile<Q>::X, TY = PiTile<Q>::Y, TZ = PiTile<Q>::Z, PZ = PiTile<Q>::PZ;
+  constexpr int NT = TX * TY * TZ, DZ = TZ * PZ;  // threads; the tile's depth in points
+  constexpr int H = ::tgh::W<Q>::R + 1;
+  // the box: BX x BY x BZ points, stored with row pitch RX >= BX (a pitch
+  // that spreads a wave's rows over the LDS banks) and plane pitch RX * BY
+  constexpr int BX = TX + 2 * H, BY = TY + 2 * H, BZ = DZ + 2 * H, BV = BX * BY * BZ;
+  constexpr int RX = PiTile<Q>::pitch(BX), RV = RX * BY * BZ;
+  __shared__ double sh[2][RV];
+  int X0, Y0, Z0;
+  if (!tile_map(k, TX, TY, DZ, X0, Y0, Z0)) return;  // uniform over the workgroup
+  const int t = threadIdx.x;
+  const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
+  const int x = X0 + lx, y = Y0 + ly;
+  const double ih = k.inv_h, ih2 = ih * ih;
+  bool valid[PZ];
+  long off[PZ];
+  int c0[PZ];
+  double epsh[PZ];
+  PiCoef pc[PZ];
+#pragma unroll
+  for (int q = 0; q < PZ; ++q) {
+    const int z = Z0 + lz + q * TZ;
+    valid[q] = x < k.n[0] && y < k.n[1] && z < k.n[2];
+    off[q] = valid[q] ? brick_index(k, x, y, z) : 0;
+    pc[q].w = -1.0;
+    if (valid[q]) load_coef(coef + off[q], k.fs, pc[q]);
+    const double xp[3] = {k.lo[0] + (double)x * k.h, k.lo[1] + (double)y * k.h, k.lo[2] + (double)z * k.h};
+    epsh[q] = dissipation_rate(a.c, xp) * ih;
+    c0[q] = (lx + H) + RX * ((ly + H) + BY * (lz + q * TZ + H));
+  }
   // box point (bx, by, bz) is brick point (X0 - H + bx, ...): never below -G,
   // and skipped at or beyond n + G, where only points outside the brick
   // (no thread computes them) would read it
   const int ex = k.n[0] + k.G - (X0 - H), ey = k.n[1] + k.G - (Y0 - H), ez = k.n[2] + k.G - (Z0 - H);
   const long box0 = brick_index(k, X0 - H, Y0 - H, Z0 - H);
+  // Staging: each thread owns at most one box column (bx, by) (BX * BY <= NT
+  // for q <= 6; q = 8 takes the loop below) and copies it plane by plane. The
+  // next component's column is loaded into registers while the current one
+  // is computed, so the loads' latency hides behind the arithmetic.
+  // Only what the stencils read is staged: h_v in the box of halo R = H - 1
+  // (first, second and mixed derivatives), Pi_v on the three axis lines
+  // through the tile (first derivatives and the dissipation, reach H): a
+  // column inside the tile's footprint takes every plane, a column beside it
+  // along x or y the tile's own planes, a corner column none -- 2176 values
+  // of the 3920 in the two boxes at q = 4.
+  const int nbz = ez < BZ ? ez : BZ;  // planes inside the stored extent
+  constexpr bool ONE_COL = BX * BY <= NT;
+  const int sbx = t % BX, sby = t / BX;
+  const bool stager = ONE_COL && t < BX * BY;
+  const bool col_in = stager && sbx < ex && sby < ey;
+  const bool hcol = col_in && sbx >= 1 && sbx < BX - 1 && sby >= 1 && sby < BY - 1;
+  const bool ix = sbx >= H && sbx < BX - H, iy = sby >= H && sby < BY - H;
+  const int pz0 = (ix && iy) ? 0 : ((ix || iy) ? H : BZ);
+  const int pz1 = (ix && iy) ? BZ : ((ix || iy) ? H + DZ : BZ);
+  const double* __restrict__ Wc = W + box0 + (sbx + k.py * sby);
+  const int lcol = sbx + RX * sby;
+  double ph[ONE_COL ? BZ : 1], pp[ONE_COL ? BZ : 1];
+  auto fetch = [&](int v) {
+    if constexpr (ONE_COL) {
+#pragma unroll
+      for (int bz = 0; bz < BZ; ++bz) {
+        const bool inh = hcol && bz >= 1 && bz < BZ - 1 && bz < nbz;
+        const bool inp = col_in && bz >= pz0 && bz < pz1 && bz < nbz;
+        ph[bz] = inh ? Wc[v * k.fs + (long)bz * k.pz] : 0.0;
+        pp[bz] = inp ? Wc[(NC + v) * k.fs + (long)bz * k.pz] : 0.0;
+      }
+    }
+  };
+  fetch(0);
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
     __syncthreads();  // the previous component's reads are done
-    const double* __restrict__ Wh = W + box0 + v * k.fs;
-    for (int e = t; e < BV; e += NT) {
-      const int bx = e % BX, by = (e / BX) % BY, bz = e / (BX * BY);
-      double hv = 0.0, pv = 0.0;
-      if (bx < ex && by < ey && bz < ez) {
-        const long gi = bx + k.py * by + k.pz * bz;
-        hv = Wh[gi];
-        pv = Wh[gi + NC * k.fs];
+    if constexpr (ONE_COL) {
+      if (stager) {
+#pragma unroll
+        for (int bz = 0; bz < BZ; ++bz) {
+          if (bz >= 1 && bz < BZ - 1) sh[0][lcol + RX * BY * bz] = ph[bz];
+          if (bz >= pz0 && bz < pz1) sh[1][lcol + RX * BY * bz] = pp[bz];
+        }
       }
-      sh[0][e] = hv;
-      sh[1][e] = pv;
+    } else {
+      for (int col = t; col < BX * BY; col += NT) {
+        const int cx = col % BX, cy = col / BX;
+        const bool cin = cx < ex && cy < ey;
+        const double* __restrict__ Wh = W + box0 + v * k.fs + (cx + k.py * cy);
+        const int lc = cx + RX * cy;
+        for (int bz = 0; bz < BZ; ++bz) {
+          const bool in = cin && bz < nbz;
+          sh[0][lc + RX * BY * bz] = in ? Wh[(long)bz * k.pz] : 0.0;
+          sh[1][lc + RX * BY * bz] = in ? Wh[NC * k.fs + (long)bz * k.pz] : 0.0;
+        }
+      }
     }
     __syncthreads();
-    if (valid) {
-      double f = 0.0;
-      if (pc.w >= 0.0) {
-        const double s = pi_principal<Q>(&sh[0][c0], &sh[1][c0], BX, BX * BY, pc.b, pc.divb, pc.divA, pc.A,
-                                         ih, ih2, epsh, a.c.has_diss);
-        f = pc.w * s + coef[(CF_C1 + v) * k.fs + off];
+    // this component's epilogue inputs first, then the next component's column
+    double u0[PZ], a0[PZ], c1[PZ];
+#pragma unroll
+    for (int q = 0; q < PZ; ++q) {
+      const long si = off[q] + (NC + v) * k.fs;
+      if (valid[q]) {
+        epi_load(a, us, acc, si, u0[q], a0[q]);
+        c1[q] = coef[(CF_C1 + v) * k.fs + off[q]];
       }
-      emit<EPI_RK4>(a, TGH_OUT_ARGS, off + (NC + v) * k.fs, off + (NC + v) * k.fs, f);
+    }
+    if (v + 1 < NC) fetch(v + 1);
+#pragma unroll
+    for (int q = 0; q < PZ; ++q) {
+      if (valid[q]) {
+        double f = 0.0;
+        if (pc[q].w >= 0.0) {
+          const double s = pi_principal<Q>(&sh[0][c0[q]], &sh[1][c0[q]], RX, RX * BY, pc[q].b, pc[q].divb,
+                                           pc[q].divA, pc[q].A, ih, ih2, epsh[q], a.c.has_diss);
+          f = pc[q].w * s + c1[q];
+        }
+        const long si = off[q] + (NC + v) * k.fs;
+        epi_store(a, us, acc, Wn, si, si, f, u0[q], a0[q]);
+      }
     }
   }
 }

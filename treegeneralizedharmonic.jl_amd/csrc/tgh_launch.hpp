@@ -6,6 +6,15 @@
 
 namespace tgh {
 
+#ifndef TGH_HEAD_LDS
+#define TGH_HEAD_LDS 0
+#endif
+constexpr bool kHeadLds = TGH_HEAD_LDS != 0;
+#ifndef TGH_HEAD_TWO
+#define TGH_HEAD_TWO 0
+#endif
+constexpr bool kHeadTwo = TGH_HEAD_TWO != 0;
+
 // One right-hand side: the fused kernel, or the head and the Pi components as
 // two launches. While profiling, the head is timed as the "rhs" pair and the
 // Pi kernel as its own pair.
@@ -65,11 +74,29 @@ void launch_brick_rhs_q(tgh_problem* p, const RhsArgs& a) {
                        a.W, a.du, a.us, a.acc, a.Wn);
     if (prof) p->ev_rhs.push_back({e0, mark(p)});
   } else {
-    hipLaunchKernelGGL((k_brick_head<Q, GAUGE, HOLE>), grid_brick(p), dim3(256), 0, p->stream, a, p->brick,
-                       a.W, p->Bcoef, a.du, a.us, a.acc, a.Wn);
+    if (kHeadTwo) {  // the head as derivatives + source launches
+      hipLaunchKernelGGL((k_brick_derivs<Q, HOLE>), grid_brick(p), dim3(256), 0, p->stream, a, p->brick, a.W,
+                         p->Bcoef, p->Bd);
+      hipLaunchKernelGGL((k_brick_source<GAUGE, HOLE>), grid_brick(p), dim3(256), 0, p->stream, a, p->brick, a.W,
+                         (const double*)p->Bd, p->Bcoef, a.du, a.us, a.acc, a.Wn);
+    } else if constexpr (Q <= 4 && kHeadLds) {  // the head from LDS-staged lines (fits LDS up to q = 4)
+      using HT = HeadTile<Q>;
+      hipLaunchKernelGGL((k_brick_head_lds<Q, GAUGE, HOLE>), grid_brick_tiles(p, HT::X, HT::Y, HT::Z),
+                         dim3(HT::X * HT::Y * HT::Z), 0, p->stream, a, p->brick, a.W, p->Bcoef, a.du, a.us, a.acc,
+                         a.Wn);
+    } else {
+#ifdef TGH_HEAD_TILE_X
+      constexpr int HX = TGH_HEAD_TILE_X, HZ = 4, HY = 256 / (HX * HZ);
+      const dim3 hgrid = grid_brick_tiles(p, HX, HY, HZ);
+#else
+      const dim3 hgrid = grid_brick(p);
+#endif
+      hipLaunchKernelGGL((k_brick_head<Q, GAUGE, HOLE>), hgrid, dim3(256), 0, p->stream, a, p->brick,
+                         a.W, p->Bcoef, a.du, a.us, a.acc, a.Wn);
+    }
     int e1 = prof ? mark(p) : -1;
     using T = PiTile<Q>;
-    hipLaunchKernelGGL((k_brick_pi<Q>), grid_brick_tiles(p, T::X, T::Y, T::Z), dim3(T::X * T::Y * T::Z), 0,
+    hipLaunchKernelGGL((k_brick_pi<Q>), grid_brick_tiles(p, T::X, T::Y, T::Z * T::PZ), dim3(T::X * T::Y * T::Z), 0,
                        p->stream, a, p->brick, a.W, (const double*)p->Bcoef, a.du, a.us, a.acc, a.Wn);
     if (prof) {
       p->ev_rhs.push_back({e0, e1});
