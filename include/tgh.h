@@ -38,7 +38,8 @@ enum { /* SpacetimeMetrics backgrounds, restated in closed form */
   TGH_BG_MINKOWSKI = 0,
   TGH_BG_GAUGE_WAVE = 1,        /* AwA gauge wave, H = 1 - A sin(2 pi (x - t)/d) */
   TGH_BG_KERR_SCHILD = 2,       /* KerrSchild(M, a), spin along +z, at center */
-  TGH_BG_SHIFTED_MINKOWSKI = 3  /* t^ = t + A w tanh(x / w) */
+  TGH_BG_SHIFTED_MINKOWSKI = 3, /* t^ = t + A w tanh(x / w) */
+  TGH_BG_HARMONIC = 4           /* Harmonic(M, a): Kerr in harmonic coordinates, spin along +z */
 };
 enum { /* gauge source kinds, src/gauge.jl:342-359 */
   TGH_GAUGE_NONE = 0,        /* harmonic background: H_a = 0 */

@@ -841,7 +841,7 @@ inline void ks_algebraic_source(double M, const double S[4], const double u[4], 
 
 // ---------------------------------------------------------------------------
 // Analytic backgrounds (SpacetimeMetrics, restated from their closed forms).
-enum Background { BG_MINKOWSKI = 0, BG_GAUGE_WAVE = 1, BG_KERR_SCHILD = 2, BG_SHIFTED = 3 };
+enum Background { BG_MINKOWSKI = 0, BG_GAUGE_WAVE = 1, BG_KERR_SCHILD = 2, BG_SHIFTED = 3, BG_HARMONIC = 4 };
 
 struct BgParams {
   int kind = BG_MINKOWSKI;
@@ -850,6 +850,63 @@ struct BgParams {
   double A = 0.05, d = 1.0;       // gauge wave amplitude and wavelength / shifted A
   double w = 2.0;                 // shifted Minkowski width
 };
+
+// The Kerr metric in harmonic coordinates (SpacetimeMetrics' Harmonic(M, a),
+// an unvendored dependency; restated from Cook & Scheel, Phys. Rev. D 56,
+// 4775 (1997)), spin along +z, at the hole-relative harmonic position
+// (X, Y, Z): the harmonic coordinates are those of ingoing Kerr-Schild with the
+// radius lowered by M,
+//   X + i Y = (R + i a) e^{i phi} sin(theta),  Z = R cos(theta),  R = r - M,
+// phi the ingoing Kerr-Schild angle, and the harmonic time
+//   t_H = t_BL + int 2 M r_+ / Delta dr = t_KS - 2 M ln(r - r_-) + const.
+// So the Kerr-Schild coordinates of a harmonic point are
+//   x + i y = (X + i Y)(r + i a)/(R + i a),  z = r Z / R,  t = t_H + 2 M ln(r - r_-),
+// R solving the Kerr-Schild quartic in the harmonic (X, Y, Z), and
+// g_H = J^T g_KS J with J the Jacobian of that map (a nested dual pass).
+template <class S>
+inline void harmonic_kerr_metric(const BgParams& p, S X, S Y, S Z, S g[4][4]) {
+  using D3 = Dual<S, 3>;
+  const double a = p.a, M = p.M;
+  const double rm = M - std::sqrt(M * M - a * a);
+  D3 Xd(X, 0), Yd(Y, 0), Zd(Z, 0);
+  Xd.d[0] = S(1.0);
+  Yd.d[1] = S(1.0);
+  Zd.d[2] = S(1.0);
+  D3 rho2 = Xd * Xd + Yd * Yd + Zd * Zd;
+  D3 hb = (rho2 - a * a) / 2.0;
+  D3 R2 = hb + sqrt(hb * hb + (a * a) * Zd * Zd);
+  D3 R = sqrt(R2);
+  D3 r = R + M;
+  D3 den = R2 + a * a;
+  D3 c1 = (r * R + a * a) / den, c2 = (-a * M) / den;
+  D3 xk = c1 * Xd - c2 * Yd, yk = c2 * Xd + c1 * Yd, zk = r * Zd / R;
+  // J[c][b] = d x_KS^c / d x_H^b
+  S J[4][4];
+  for (int c = 0; c < 4; ++c)
+    for (int b = 0; b < 4; ++b) J[c][b] = S(c == 0 && b == 0 ? 1.0 : 0.0);
+  const S tfac = (2.0 * M) / (r.v - rm);
+  for (int i = 0; i < 3; ++i) {
+    J[0][i + 1] = tfac * R.d[i];
+    J[1][i + 1] = xk.d[i];
+    J[2][i + 1] = yk.d[i];
+    J[3][i + 1] = zk.d[i];
+  }
+  // g_KS at (xk, yk, zk), with the Kerr-Schild radius r = R + M
+  S rv = r.v, r2v = rv * rv, zv = zk.v;
+  S f = (2.0 * M) * rv * r2v / (r2v * r2v + (a * a) * zv * zv);
+  S dl = r2v + a * a;
+  S l[4] = {S(1.0), (rv * xk.v + a * yk.v) / dl, (rv * yk.v - a * xk.v) / dl, zv / rv};
+  S gk[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) gk[i][j] = S(eta(i, j)) + f * l[i] * l[j];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      S sum(0.0);
+      for (int c = 0; c < 4; ++c)
+        for (int d = 0; d < 4; ++d) sum = sum + J[c][i] * J[d][j] * gk[c][d];
+      g[i][j] = sum;
+    }
+}
 
 // The metric g_ab at (t, x, y, z), generic in the scalar (dual numbers give
 // dmetric).
@@ -875,6 +932,10 @@ inline void bg_metric(const BgParams& p, S t, S x, S y, S z, S g[4][4]) {
     g[0][0] = S(-1.0);
     g[0][1] = g[1][0] = -pp;
     g[1][1] = S(1.0) - pp * pp;
+    return;
+  }
+  if (p.kind == BG_HARMONIC) {
+    harmonic_kerr_metric(p, x - p.c[0], y - p.c[1], z - p.c[2], g);
     return;
   }
   // Kerr-Schild, g = eta + f l l with

@@ -559,3 +559,29 @@ def test_rk4_with_range_projection_matches_oracle(tgh, form):
     diff = float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max())
     print(f"range projection, 2 RK4 steps ({form}): max rel diff = {diff:.3e}")
     assert diff <= 1e-12
+
+
+@pytest.mark.parametrize("q", [2, 6, 8])
+def test_brick_stepper_other_orders_equal_block_stepper(tgh, q):
+    """The brick stepper's LDS tiles at q = 2, 6, 8 (halo G = q/2 + 1) against the
+    block stepper, 3 steps of the gauge wave on a Dirichlet box with noise."""
+    torch = torch_cuda()
+    N = 10 if q >= 6 else 8
+    desc = TD.make_desc(q=q, N=N, nblocks=(2, 2, 1), periodic=(0, 0, 1), lo=(0, 0, 0), hi=(2, 2, 1),
+                        background=TD.TGH_BG_GAUGE_WAVE, wave_amp=1 / 20, wave_len=2.0, eps_ko=0.5,
+                        gamma0=1.0, gamma2=-0.5)
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = o.fill_exact(0.0)
+    o.add_noise(u, 31, 1e-7)
+    u0 = torch.from_numpy(u).cuda()
+    dt = 0.25 * (desc.hi[0] - desc.lo[0]) / (desc.nblocks[0] * desc.N) / tgh.max_speed(p, u0, 0.0)
+    res = {}
+    for form in ("brick-split", "brick-fused", "blocks-fused"):
+        w = u0.clone()
+        tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(w.data_ptr()), 0.2, dt, 3, RK4_FORMS[form]))
+        res[form] = w
+    for form in ("brick-split", "brick-fused"):
+        diff = float(((res[form] - res["blocks-fused"]).abs() / torch.clamp(res["blocks-fused"].abs(), min=1.0)).max())
+        print(f"q={q} {form} vs blocks-fused: {diff:.3e}")
+        assert diff <= 1e-13

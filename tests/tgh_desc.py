@@ -3,7 +3,7 @@ the tests and the benchmark build (restating the reference's case builders,
 src/initialdata.jl:869-1326)."""
 import ctypes as C
 
-TGH_BG_MINKOWSKI, TGH_BG_GAUGE_WAVE, TGH_BG_KERR_SCHILD, TGH_BG_SHIFTED_MINKOWSKI = 0, 1, 2, 3
+TGH_BG_MINKOWSKI, TGH_BG_GAUGE_WAVE, TGH_BG_KERR_SCHILD, TGH_BG_SHIFTED_MINKOWSKI, TGH_BG_HARMONIC = 0, 1, 2, 3, 4
 TGH_GAUGE_NONE, TGH_GAUGE_SAMPLED, TGH_GAUGE_KERR_SCHILD = 0, 1, 2
 TGH_DAMPING_CONSTANT, TGH_DAMPING_GAUSSIAN = 0, 1
 TGH_INTERIOR_NONE, TGH_INTERIOR_DAMPED, TGH_INTERIOR_FROZEN, TGH_INTERIOR_PASTED = 0, 1, 2, 3

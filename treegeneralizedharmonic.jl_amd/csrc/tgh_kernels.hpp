@@ -1002,12 +1002,24 @@ __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head(RhsArgs a, Bri
   pr.m_S = nullptr;
   pr.m_Sij = nullptr;
   pr.ms = 0;
-  PiCoef pc;
   const GlobalSrc src{pr.w0, pr.sv, pr.s2, pr.s3};
-  if (head_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, src, k.inv_h, pc, TGH_OUT_ARGS))
-    store_coef(coef + off, k.fs, pc);
-  else
+  double r;
+  if (frozen_core<HOLE>(a.c, pr.x, r)) {
+#pragma unroll 1
+    for (int v = 0; v < NC; ++v) emit<EPI_RK4>(a, TGH_OUT_ARGS, off + v * k.fs, off + v * k.fs, 0.0);
     coef[CF_W * k.fs + off] = -1.0;
+    return;
+  }
+  // the geometric coefficients are stored as soon as they are known, so that
+  // they are not held through the source
+  double hv[NC], D[4][NC];
+  Coef cf;
+  PiCoef pc;
+  head_derivs<Q>(a.c, src, k.inv_h, dissipation_rate(a.c, pr.x) * k.inv_h, hv, D, cf);
+  head_geometry(cf, D, pc);
+  store_coef_geometry(coef + off, k.fs, pc);
+  head_source<GAUGE, HOLE, EPI_RK4>(a, pr, src, r, cf, hv, D, pc, TGH_OUT_ARGS);
+  store_coef_source(coef + off, k.fs, pc);
 }
 
 // The head in two launches on the brick (the default). The stencils and the
@@ -1216,15 +1228,15 @@ __global__ __launch_bounds__(256) void k_brick_head_lds(RhsArgs a, Brick k, cons
 #define TGH_PI_POINTS_Z 1
 #endif
 // The tile: X x Y x Z threads, each taking PZ points along z (at z, z + Z, ...);
-// at q = 8 the halo of 5 leaves room for Z = 2, one point each, in 64 KiB.
-#ifndef TGH_PI_PITCH_PAD
-#define TGH_PI_PITCH_PAD 0
-#endif
+// at q = 8 (halo 5) two planes of threads, one point each.
+// The box's LDS row pitch: 24 doubles (at least the row's BX), so that the
+// eight rows a wave reads fall on distinct banks of the 64 four-byte LDS
+// banks (24 = -8 mod 32: row r starts at bank 48 r mod 64).
 template <int Q> struct PiTile {
   static constexpr int X = TGH_PI_TILE_X, Y = TGH_PI_TILE_Y;
   static constexpr int Z = Q == 8 ? 2 : TGH_PI_TILE_Z;
   static constexpr int PZ = Q == 8 ? 1 : TGH_PI_POINTS_Z;
-  static constexpr int pitch(int bx) { return Q == 8 ? bx : bx + TGH_PI_PITCH_PAD; }
+  static constexpr int pitch(int bx) { return bx <= 24 ? 24 : bx; }
 };
 #ifdef TGH_BRICK_PI_WAVES_PER_EU
 #define TGH_BRICK_PI_ATTR __attribute__((amdgpu_waves_per_eu(TGH_BRICK_PI_WAVES_PER_EU)))
