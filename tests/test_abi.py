@@ -67,6 +67,39 @@ def test_descriptor_checks_refuse_bad_dissipation_profiles(tgh):
         assert L.tgh_last_error()
 
 
+def test_descriptor_checks_refuse_bad_harmonic_cases(tgh):
+    """The harmonic chart takes no gauge source (src/gauge.jl:64) and its frozen core
+    contains the singular disk |x| <= |a| (src/interior.jl:1000, 1120-1126)."""
+    import tgh_desc as TD
+    L = tgh._load()
+    good = TD.harmonic_hole_case()
+    ptr = ctypes.c_void_p()
+    rc = L.tgh_problem_create(ctypes.byref(tgh.TghDesc.from_buffer_copy(good)), ctypes.byref(ptr))
+    assert rc != 1, L.tgh_last_error()  # passes the checks (no device here: rc 2)
+    if rc == 0:
+        L.tgh_problem_destroy(ptr)
+    for kw in (dict(gauge=TD.TGH_GAUGE_SAMPLED), dict(gauge=TD.TGH_GAUGE_KERR_SCHILD), dict(r0=0.25),
+               dict(background=5)):
+        d = tgh.TghDesc.from_buffer_copy(good)
+        for k, v in kw.items():
+            setattr(d, k, v)
+        ptr = ctypes.c_void_p()
+        assert L.tgh_problem_create(ctypes.byref(d), ctypes.byref(ptr)) == 1, kw
+        assert L.tgh_last_error()
+
+
+def test_python_harmonic_case_builds_the_descriptor(tgh):
+    import tgh_desc as TD
+    case = tgh.harmonic_kerr_case(M=1.0, a=0.3, halfwidth=2.0, r_0=0.35, r_1=0.9)
+    assert abs(tgh.horizon_min_radius(case.background) - 0.91 ** 0.5) < 1e-15
+    d = tgh._desc(tgh.gh_forest(None, case, N=16, roots=4), case, 4)
+    ref = TD.harmonic_hole_case()
+    ref.wave_amp, ref.wave_len = d.wave_amp, d.wave_len  # the gauge wave's, unused here
+    assert bytes(d) == bytes(ref)
+    with pytest.raises(tgh.ArgumentError):
+        tgh.Harmonic(1.0, 1.0)
+
+
 def test_python_case_builds_the_horizon_profile(tgh):
     case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=3.0, r_0=0.71, r_1=1.71)
     hd = tgh.horizon_dissipation(case, eps_in=3.0)

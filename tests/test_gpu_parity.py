@@ -585,3 +585,53 @@ def test_brick_stepper_other_orders_equal_block_stepper(tgh, q):
         diff = float(((res[form] - res["blocks-fused"]).abs() / torch.clamp(res["blocks-fused"].abs(), min=1.0)).max())
         print(f"q={q} {form} vs blocks-fused: {diff:.3e}")
         assert diff <= 1e-13
+
+
+# --- the Kerr hole in harmonic coordinates (harmonic_kerr_case) -------------
+
+@pytest.mark.parametrize("a", [0.0, 0.3])
+def test_harmonic_hole_rhs_matches_oracle(tgh, a):
+    """harmonic_kerr_case (src/initialdata.jl:1300-1326): no gauge source; the
+    Dirichlet ghosts and the layer's target are the harmonic chart's state."""
+    desc = TD.harmonic_hole_case(a=a)
+    err, wo, wg, *_ = check_rhs(tgh, desc, 0.0)
+    print(f"harmonic hole a={a}: max rel err {err:.3e}")
+    assert err <= TOL
+    denom = np.maximum(np.abs(wo), 1.0)
+    assert float((np.abs(wg - wo) / denom).max()) <= 1e-13
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.5])
+def test_harmonic_kerr_off_centre_rhs_matches_oracle(tgh, eps):
+    """The reference's own harmonic row (test/evolution_tests.jl:110-179):
+    Harmonic(1, 9/10) on the box (2, 4)^3, N = 8, q = 4, gamma0 = 1,
+    gamma2 = -1/2, t = 1/4, no interior, Dirichlet faces."""
+    desc = TD.make_desc(q=4, N=8, nblocks=(1, 1, 1), periodic=(0, 0, 0), lo=(2, 2, 2), hi=(4, 4, 4),
+                        background=TD.TGH_BG_HARMONIC, gauge=TD.TGH_GAUGE_NONE, M=1.0, a=0.9,
+                        gamma0=1.0, gamma2=-0.5, eps_ko=eps)
+    err, wo, wg, *_ = check_rhs(tgh, desc, 0.25)
+    print(f"harmonic Kerr a=0.9 off centre eps={eps}: max rel err {err:.3e}")
+    assert err <= TOL
+    assert float(np.abs(wg - wo).max()) <= 1e-15
+
+
+@pytest.mark.parametrize("form", ["brick-split", "brick-fused", "blocks-split", "blocks-fused"])
+def test_rk4_harmonic_hole_two_steps_match_oracle(tgh, form):
+    torch = torch_cuda()
+    desc = TD.harmonic_hole_case(a=0.3)
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = o.fill_exact(0.0)
+    o.add_noise(u, 777, 1e-7)
+    ud = torch.from_numpy(u.copy()).cuda()
+    lam = o.max_speed(u)
+    assert abs(tgh.max_speed(p, ud, 0.0) - lam) <= 1e-13 * lam
+    dt = 0.25 * (1 / 16) / lam
+    for n in range(2):
+        o.rk4_step(u, n * dt, dt, NTHREADS)
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, RK4_FORMS[form]))
+    g = ud.cpu().numpy()
+    scale = np.maximum(np.abs(u), 1.0)
+    diff = float((np.abs(g - u) / scale).max())
+    print(f"harmonic hole, 2 RK4 steps ({form}): max rel diff = {diff:.3e}")
+    assert diff <= 1e-12

@@ -77,3 +77,14 @@ def kerr_schild_hole_case(*, N=16, nblocks=16, halfwidth=8.0, a=0.7, M=1.0, q=4,
                      background=TGH_BG_KERR_SCHILD, gauge=gauge, damping=TGH_DAMPING_GAUSSIAN,
                      gamma0_near=1.0 / M, gamma0_far=1.0 / (10 * M), gamma0_width=3.0 * M, gamma2=0.0,
                      eps_ko=0.5, interior=TGH_INTERIOR_DAMPED, M=M, a=a, r0=r0, r1=r1, rho_max=rho_max)
+
+
+def harmonic_hole_case(*, N=16, nblocks=4, halfwidth=2.0, a=0.3, M=1.0, q=4, r0=0.35, r1=0.9,
+                       rho_max=4.0):
+    """harmonic_kerr_case (src/initialdata.jl:1300-1326): hole_case(Harmonic(M, a)) with
+    no gauge source, the rest as kerr_schild_hole_case."""
+    L = halfwidth
+    return make_desc(q=q, N=N, nblocks=(nblocks,) * 3, periodic=(0, 0, 0), lo=(-L, -L, -L), hi=(L, L, L),
+                     background=TGH_BG_HARMONIC, gauge=TGH_GAUGE_NONE, damping=TGH_DAMPING_GAUSSIAN,
+                     gamma0_near=1.0 / M, gamma0_far=1.0 / (10 * M), gamma0_width=3.0 * M, gamma2=0.0,
+                     eps_ko=0.5, interior=TGH_INTERIOR_DAMPED, M=M, a=a, r0=r0, r1=r1, rho_max=rho_max)

@@ -25,9 +25,9 @@ import math
 import os
 
 __all__ = [
-    "ArgumentError", "TGHError", "MI355X", "Minkowski", "GaugeWave", "KerrSchild",
+    "ArgumentError", "TGHError", "MI355X", "Minkowski", "GaugeWave", "KerrSchild", "Harmonic",
     "ConstantDamping", "GaussianDamping", "Interior", "GHCase", "minkowski_case",
-    "gauge_wave_case", "kerr_schild_case", "gh_forest", "GHProblem", "gh_rhs", "prepare",
+    "gauge_wave_case", "kerr_schild_case", "harmonic_kerr_case", "gh_forest", "GHProblem", "gh_rhs", "prepare",
     "rhs", "settle", "stage_limit", "step_limit", "max_speed", "initial_state",
     "evolved_sets", "gh_dt", "gh_solve", "add_noise", "error_norms", "library_path",
 ]
@@ -183,6 +183,19 @@ class KerrSchild:
         self.M, self.a = float(M), float(a)
 
 
+class Harmonic:
+    """Harmonic(M, a): the Kerr hole in harmonic coordinates, spin a along +z
+    (SpacetimeMetrics, restated from Cook & Scheel 1997). H_a = 0 exactly, so a
+    case on it takes no gauge source (src/gauge.jl:64)."""
+    kind = 4
+
+    def __init__(self, M, a):
+        self.M, self.a = float(M), float(a)
+        if not (self.M > 0 and abs(self.a) < self.M):
+            raise ArgumentError(f"the harmonic Kerr chart needs M > 0 and |a| < M, got M = {M}, "
+                                f"a = {a}")
+
+
 class ConstantDamping:
     def __init__(self, g0):
         self.g0 = float(g0)
@@ -226,10 +239,12 @@ class HorizonDissipation:
 
 
 def horizon_min_radius(background):
-    """The horizon's smallest coordinate radius (src/interior.jl:760): r₊ = M + √(M² − a²)
-    on the Kerr–Schild axis."""
+    """The horizon's smallest coordinate radius (src/interior.jl:760-764): r₊ = M + √(M² − a²)
+    on the Kerr–Schild axis, R₊ = √(M² − a²) in harmonic coordinates."""
     if isinstance(background, KerrSchild):
         return background.M + (background.M ** 2 - background.a ** 2) ** 0.5
+    if isinstance(background, Harmonic):
+        return (background.M ** 2 - background.a ** 2) ** 0.5
     raise ArgumentError(f"{type(background).__name__} has no horizon")
 
 
@@ -386,6 +401,21 @@ def kerr_schild_case(T=None, *, M=1.0, a=0.0, halfwidth, r_0, r_1, rho_max=None,
                   bounds=bounds)
 
 
+def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, halfwidth, r_0, r_1, rho_max=None, eps_ko=0.5,
+                       gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
+                       bounds=None):
+    """harmonic_kerr_case / hole_case (src/initialdata.jl:1300-1326, 1044-1174) for a
+    hole at rest at the origin: Harmonic(M, a) in a Dirichlet box, with no gauge
+    source (the chart is harmonic). Defaults as kerr_schild_case's."""
+    damping = gamma0 if gamma0 is not None else GaussianDamping(near=1 / M, far=1 / (10 * M),
+                                                                width=3 * M)
+    inter = Interior(T, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
+                     variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp)
+    L = float(halfwidth)
+    return GHCase(T, Harmonic(M, a), box=[(-L, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
+                  gamma0=damping, gamma2=gamma2, interior=inter, bounds=bounds)
+
+
 class Forest:
     """A uniform brick of root blocks of N cells (gh_forest's roots)."""
 
@@ -414,9 +444,9 @@ def _desc(forest, case, q):
     d.hi[:] = [b[1] for b in case.box]
     if isinstance(bg, GaugeWave):
         d.wave_amp, d.wave_len = bg.A, bg.d
-    if isinstance(bg, KerrSchild):
+    if isinstance(bg, (KerrSchild, Harmonic)):
         d.M, d.a = bg.M, bg.a
-    harmonic = isinstance(bg, (Minkowski, GaugeWave))
+    harmonic = isinstance(bg, (Minkowski, GaugeWave, Harmonic))
     d.gauge = 2 if case.gauge_source == "algebraic" else (0 if harmonic else 1)
     g0 = case.gamma0
     if isinstance(g0, GaussianDamping):

@@ -84,7 +84,7 @@ int check_desc(const tgh_desc* d) {
       return fail(TGH_EINVAL, "the blocks' cells must be cubic: the box widths over the brick of blocks "
                               "give different spacings per axis");
   }
-  if (d->background < 0 || d->background > 3) return fail(TGH_EINVAL, "unknown background");
+  if (d->background < 0 || d->background > 4) return fail(TGH_EINVAL, "unknown background");
   if (d->gauge < 0 || d->gauge > 2) return fail(TGH_EINVAL, "unknown gauge source kind");
   if (d->gauge == TGH_GAUGE_SAMPLED && d->background == TGH_BG_GAUGE_WAVE)
     return fail(TGH_EINVAL, "a time-dependent gauge source is not supported: the gauge wave is harmonic "
@@ -92,6 +92,9 @@ int check_desc(const tgh_desc* d) {
   if (d->gauge == TGH_GAUGE_KERR_SCHILD && d->background != TGH_BG_KERR_SCHILD)
     return fail(TGH_EINVAL, "the closed-form Kerr-Schild gauge source reproduces H_a = -Gamma_a of a "
                             "Kerr-Schild background only");
+  if (d->gauge == TGH_GAUGE_SAMPLED && d->background == TGH_BG_HARMONIC)
+    return fail(TGH_EINVAL, "the harmonic Kerr chart has H_a = 0 exactly, so it takes no gauge source "
+                            "(src/gauge.jl isharmonic, src/initialdata.jl harmonic_kerr_case)");
   if (d->gauge == TGH_GAUGE_NONE &&
       (d->background == TGH_BG_KERR_SCHILD || d->background == TGH_BG_SHIFTED_MINKOWSKI))
     return fail(TGH_EINVAL, "this background is not harmonic, so it needs a gauge source: H_a = 0 "
@@ -109,8 +112,12 @@ int check_desc(const tgh_desc* d) {
     if (d->r1 - d->r0 < 2.0 * (G + 1) * h0)
       return fail(TGH_EINVAL, "r_1 - r_0 must be at least 2(G+1)h so that no stencil of an evolved point "
                               "reaches the frozen core (check_interior_radii)");
-    if (d->background == TGH_BG_KERR_SCHILD && !(d->r0 > std::fabs(d->a)))
-      return fail(TGH_EINVAL, "r_0 must lie outside the Kerr-Schild ring singularity, r_0 > |a|");
+    // singular_radius (src/interior.jl:997-1010, checked at :1120-1126): both
+    // Kerr charts are singular on the disk |x| <= |a|, z = 0.
+    if ((d->background == TGH_BG_KERR_SCHILD || d->background == TGH_BG_HARMONIC) &&
+        !(d->r0 > std::fabs(d->a)))
+      return fail(TGH_EINVAL, "the frozen core must contain the chart's singular disk |x| <= |a|, z = 0: "
+                              "r_0 > |a|");
   }
   if (d->dissipation != TGH_DISSIPATION_CONSTANT && d->dissipation != TGH_DISSIPATION_HORIZON)
     return fail(TGH_EINVAL, "unknown dissipation kind");

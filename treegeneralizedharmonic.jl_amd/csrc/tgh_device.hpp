@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #define TGH_HD __host__ __device__ __forceinline__
 
@@ -409,7 +410,7 @@ template <int K> TGH_HD Dual<K> dtanh(const Dual<K>& a) {
   return r;
 }
 
-enum { BG_MINKOWSKI = 0, BG_GAUGE_WAVE = 1, BG_KERR_SCHILD = 2, BG_SHIFTED = 3 };
+enum { BG_MINKOWSKI = 0, BG_GAUGE_WAVE = 1, BG_KERR_SCHILD = 2, BG_SHIFTED = 3, BG_HARMONIC = 4 };
 
 struct Bg {
   int kind;
@@ -513,13 +514,233 @@ template <> TGH_HD Jet2 jet_const<Jet2>(double x) { return jconst(x); }
 TGH_HD void jet_seed(Dual<4>& x, int k) { x.d[k] = 1.0; }
 TGH_HD void jet_seed(Jet2& x, int k) { x.d[k] = 1.0; }
 
+// Nested forward-mode duals for the harmonic chart, whose Jacobian needs its
+// own derivatives. Every operation rounds in a fixed order with contraction
+// off, the order of the oracle's restatement (a dual times a constant is the
+// product with the constant's zero-derivative dual, a quotient's derivative
+// (a' b - a b') / b^2, a root's a' / (2 sqrt a)), so that the chart's values
+// -- the Dirichlet data of a box around the hole, where the right-hand side's
+// stencils multiply them by 1/h^2 -- agree with it to the last bits.
+template <class V, int K>
+struct NDual {
+  V v;
+  V d[K];
+};
+TGH_HD double nd_cst(double x, const double*) { return x; }
+template <class V, int K>
+TGH_HD NDual<V, K> nd_cst(double x, const NDual<V, K>*) {
+  NDual<V, K> r;
+  r.v = nd_cst(x, (const V*)nullptr);
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = nd_cst(0.0, (const V*)nullptr);
+  return r;
+}
+TGH_HD double nd_add(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+TGH_HD double nd_sub(double a, double b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
+TGH_HD double nd_mul(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+TGH_HD double nd_div(double a, double b) {
+#pragma clang fp contract(off)
+  return a / b;
+}
+TGH_HD double nd_neg(double a) { return -a; }
+TGH_HD double nd_sqrt(double a) { return sqrt(a); }
+template <class V, int K>
+TGH_HD NDual<V, K> nd_add(const NDual<V, K>& a, const NDual<V, K>& b) {
+  NDual<V, K> r;
+  r.v = nd_add(a.v, b.v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = nd_add(a.d[k], b.d[k]);
+  return r;
+}
+template <class V, int K>
+TGH_HD NDual<V, K> nd_sub(const NDual<V, K>& a, const NDual<V, K>& b) {
+  NDual<V, K> r;
+  r.v = nd_sub(a.v, b.v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = nd_sub(a.d[k], b.d[k]);
+  return r;
+}
+template <class V, int K>
+TGH_HD NDual<V, K> nd_neg(const NDual<V, K>& a) {
+  NDual<V, K> r;
+  r.v = nd_neg(a.v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = nd_neg(a.d[k]);
+  return r;
+}
+template <class V, int K>
+TGH_HD NDual<V, K> nd_mul(const NDual<V, K>& a, const NDual<V, K>& b) {
+  NDual<V, K> r;
+  r.v = nd_mul(a.v, b.v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = nd_add(nd_mul(a.d[k], b.v), nd_mul(a.v, b.d[k]));
+  return r;
+}
+template <class V, int K>
+TGH_HD NDual<V, K> nd_div(const NDual<V, K>& a, const NDual<V, K>& b) {
+  NDual<V, K> r;
+  r.v = nd_div(a.v, b.v);
+  const V bb = nd_mul(b.v, b.v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = nd_div(nd_sub(nd_mul(a.d[k], b.v), nd_mul(a.v, b.d[k])), bb);
+  return r;
+}
+template <class V, int K>
+TGH_HD NDual<V, K> nd_sqrt(const NDual<V, K>& a) {
+  NDual<V, K> r;
+  r.v = nd_sqrt(a.v);
+  const V two_r = nd_mul(nd_cst(2.0, (const V*)nullptr), r.v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = nd_div(a.d[k], two_r);
+  return r;
+}
+template <class V, int K>
+TGH_HD NDual<V, K> operator+(const NDual<V, K>& a, const NDual<V, K>& b) { return nd_add(a, b); }
+template <class V, int K>
+TGH_HD NDual<V, K> operator-(const NDual<V, K>& a, const NDual<V, K>& b) { return nd_sub(a, b); }
+template <class V, int K>
+TGH_HD NDual<V, K> operator-(const NDual<V, K>& a) { return nd_neg(a); }
+template <class V, int K>
+TGH_HD NDual<V, K> operator*(const NDual<V, K>& a, const NDual<V, K>& b) { return nd_mul(a, b); }
+template <class V, int K>
+TGH_HD NDual<V, K> operator/(const NDual<V, K>& a, const NDual<V, K>& b) { return nd_div(a, b); }
+template <class V, int K>
+TGH_HD NDual<V, K> operator+(const NDual<V, K>& a, double b) { return nd_add(a, nd_cst(b, &a)); }
+template <class V, int K>
+TGH_HD NDual<V, K> operator-(const NDual<V, K>& a, double b) { return nd_sub(a, nd_cst(b, &a)); }
+template <class V, int K>
+TGH_HD NDual<V, K> operator*(double a, const NDual<V, K>& b) { return nd_mul(nd_cst(a, &b), b); }
+template <class V, int K>
+TGH_HD NDual<V, K> operator/(const NDual<V, K>& a, double b) { return nd_div(a, nd_cst(b, &a)); }
+template <class V, int K>
+TGH_HD NDual<V, K> operator/(double a, const NDual<V, K>& b) { return nd_div(nd_cst(a, &b), b); }
+
+// The Kerr metric in harmonic coordinates (SpacetimeMetrics' Harmonic(M, a),
+// restated from Cook & Scheel, Phys. Rev. D 56, 4775 (1997)) as the packed
+// offset h = g - eta with first derivatives in (t, x, y, z), spin along +z
+// about c. The harmonic chart is ingoing Kerr-Schild with the radius lowered
+// by M:
+//   X + i Y = (R + i a) e^{i phi} sin(theta), Z = R cos(theta), R = r - M,
+//   t_H = t_KS - 2 M ln(r - r_-),
+// so a harmonic point maps to the Kerr-Schild point
+//   x + i y = (X + i Y)(r + i a)/(R + i a), z = r Z / R, t = t_H + 2 M ln(r - r_-),
+// R solving the Kerr-Schild quartic in (X, Y, Z), and g_H = J^T g_KS J with
+// J the map's Jacobian, J^t_i = 2 M / (r - r_-) d_i R. The map is evaluated
+// in duals over (X, Y, Z) whose components are duals over (t, x, y, z), so
+// each entry of J carries its own first derivatives.
+TGH_HD void bg_harmonic_offset(const Bg& bg, const double x[3], Dual<4> hd[NC]) {
+  using S = NDual<double, 4>;
+  using D3 = NDual<S, 3>;
+  const double a = bg.a, M = bg.M;
+  const double rm = M - sqrt(M * M - a * a);
+  const double aa = a * a;
+  D3 P[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    S xi = nd_cst(x[i], (const S*)nullptr);
+    xi.d[i + 1] = 1.0;
+    xi = xi - bg.c[i];
+    P[i] = nd_cst(0.0, (const D3*)nullptr);
+    P[i].v = xi;
+    P[i].d[i] = nd_cst(1.0, (const S*)nullptr);
+  }
+  const D3 &X = P[0], &Y = P[1], &Z = P[2];
+  D3 rho2 = X * X + Y * Y + Z * Z;
+  D3 hb = (rho2 - aa) / 2.0;
+  D3 R2 = hb + nd_sqrt(hb * hb + aa * Z * Z);
+  D3 R = nd_sqrt(R2);
+  D3 r = R + M;
+  D3 den = R2 + aa;
+  D3 c1 = (r * R + aa) / den, c2 = (-a * M) / den;
+  D3 m[3];
+  m[0] = c1 * X - c2 * Y;
+  m[1] = c2 * X + c1 * Y;
+  m[2] = r * Z / R;
+  // J[c][b] = d x_KS^c / d x_H^b; the time column is the identity's.
+  S J[4][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) J[c][b] = nd_cst(c == 0 && b == 0 ? 1.0 : 0.0, (const S*)nullptr);
+  const S tfac = (2.0 * M) / (r.v - rm);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    J[0][i + 1] = tfac * R.d[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) J[c + 1][i + 1] = m[c].d[i];
+  }
+  // g_KS at the mapped point, whose Kerr-Schild radius is r = R + M.
+  const S rv = r.v, r2v = rv * rv, xv = m[0].v, yv = m[1].v, zv = m[2].v;
+  const S f = (2.0 * M) * rv * r2v / (r2v * r2v + aa * zv * zv);
+  const S dl = r2v + aa;
+  S l[4];
+  l[0] = nd_cst(1.0, (const S*)nullptr);
+  l[1] = (rv * xv + a * yv) / dl;
+  l[2] = (rv * yv - a * xv) / dl;
+  l[3] = zv / rv;
+  S gk[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) gk[i][j] = nd_cst(eta(i, j), (const S*)nullptr) + f * l[i] * l[j];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
+    const int i = prow(n), j = pcol(n);
+    S sum = nd_cst(0.0, (const S*)nullptr);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) sum = sum + J[c][i] * J[e][j] * gk[c][e];
+    hd[n].v = nd_sub(sum.v, eta(i, j));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) hd[n].d[k] = sum.d[k];
+  }
+}
+
+// Sets of backgrounds, as bit masks over the kinds: a kernel compiled for one
+// set keeps only those backgrounds' code (and registers). The gauge-source
+// kind fixes the set: H_a = 0 admits only harmonic charts, and every chart
+// that is not harmonic needs a source.
+constexpr int bg_bit(int kind) { return 1 << kind; }
+constexpr int BGS_ALL = 0x1f;
+constexpr int BGS_HARMONIC = bg_bit(BG_MINKOWSKI) | bg_bit(BG_GAUGE_WAVE) | bg_bit(BG_HARMONIC);
+constexpr int BGS_SOURCED = bg_bit(BG_MINKOWSKI) | bg_bit(BG_KERR_SCHILD) | bg_bit(BG_SHIFTED);
+
 // The packed offset h = g - eta of a background at (t, x), carried as jets in
-// (t, x, y, z): Dual<4> for first derivatives, Jet2 for second.
-template <class J>
+// (t, x, y, z): Dual<4> for first derivatives, Jet2 for second. A kind
+// outside SET gives NaN.
+template <class J, int SET = BGS_ALL>
 TGH_HD void bg_offset(const Bg& bg, double t, const double x[3], J hd[NC]) {
 #pragma unroll
   for (int n = 0; n < NC; ++n) hd[n] = jet_const<J>(0.0);
   if (bg.kind == BG_MINKOWSKI) return;
+  if (bg.kind < 0 || bg.kind > BG_HARMONIC || !(SET & bg_bit(bg.kind))) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n) hd[n] = jet_const<J>(__builtin_nan(""));
+    return;
+  }
+  if constexpr ((SET & bg_bit(BG_HARMONIC)) != 0) {
+    if (bg.kind == BG_HARMONIC) {
+      // First derivatives only: the harmonic chart has H_a = 0, so no gauge
+      // source (which needs second derivatives) is ever taken from it.
+      if constexpr (std::is_same<J, Dual<4>>::value) {
+        bg_harmonic_offset(bg, x, hd);
+      } else {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) hd[n] = jet_const<J>(__builtin_nan(""));
+      }
+      return;
+    }
+  }
   J X[4];
   const double p[4] = {t, x[0], x[1], x[2]};
 #pragma unroll
@@ -527,40 +748,46 @@ TGH_HD void bg_offset(const Bg& bg, double t, const double x[3], J hd[NC]) {
     X[k] = jet_const<J>(p[k]);
     jet_seed(X[k], k);
   }
-  if (bg.kind == BG_GAUGE_WAVE) {
-    const double pi = 3.14159265358979323846;
-    J s = dsin((2.0 * pi / bg.len) * (X[1] - X[0]));
-    J Hm1 = -(bg.A * s);  // H - 1
-    hd[pidx(0, 0)] = -Hm1;
-    hd[pidx(1, 1)] = Hm1;
-    return;
+  if constexpr ((SET & bg_bit(BG_GAUGE_WAVE)) != 0) {
+    if (bg.kind == BG_GAUGE_WAVE) {
+      const double pi = 3.14159265358979323846;
+      J s = dsin((2.0 * pi / bg.len) * (X[1] - X[0]));
+      J Hm1 = -(bg.A * s);  // H - 1
+      hd[pidx(0, 0)] = -Hm1;
+      hd[pidx(1, 1)] = Hm1;
+      return;
+    }
   }
-  if (bg.kind == BG_SHIFTED) {
-    J th = dtanh((1.0 / bg.len) * X[1]);
-    J pp = bg.A * (1.0 + (-(th * th)));
-    hd[pidx(0, 1)] = -pp;
-    hd[pidx(1, 1)] = -(pp * pp);
-    return;
+  if constexpr ((SET & bg_bit(BG_SHIFTED)) != 0) {
+    if (bg.kind == BG_SHIFTED) {
+      J th = dtanh((1.0 / bg.len) * X[1]);
+      J pp = bg.A * (1.0 + (-(th * th)));
+      hd[pidx(0, 1)] = -pp;
+      hd[pidx(1, 1)] = -(pp * pp);
+      return;
+    }
   }
-  // Kerr-Schild with spin a along +z about c:
-  //   r^4 - r^2 (rho^2 - a^2) - a^2 z^2 = 0, f = 2 M r^3 / (r^4 + a^2 z^2),
-  //   l = (1, (r x + a y)/(r^2 + a^2), (r y - a x)/(r^2 + a^2), z / r), h = f l l.
-  const double a = bg.a, M = bg.M;
-  J Xr = X[1] + jet_const<J>(-bg.c[0]), Yr = X[2] + jet_const<J>(-bg.c[1]),
-    Zr = X[3] + jet_const<J>(-bg.c[2]);
-  J rho2 = Xr * Xr + Yr * Yr + Zr * Zr;
-  J hb = 0.5 * (rho2 + jet_const<J>(-a * a));
-  J r2 = hb + dsqrt(hb * hb + (a * a) * (Zr * Zr));
-  J r = dsqrt(r2);
-  J f = (2.0 * M) * (r * r2) / (r2 * r2 + (a * a) * (Zr * Zr));
-  J den = r2 + jet_const<J>(a * a);
-  J l[4];
-  l[0] = jet_const<J>(1.0);
-  l[1] = (r * Xr + a * Yr) / den;
-  l[2] = (r * Yr + (-a) * Xr) / den;
-  l[3] = Zr / r;
+  if constexpr ((SET & bg_bit(BG_KERR_SCHILD)) != 0) {
+    // Kerr-Schild with spin a along +z about c:
+    //   r^4 - r^2 (rho^2 - a^2) - a^2 z^2 = 0, f = 2 M r^3 / (r^4 + a^2 z^2),
+    //   l = (1, (r x + a y)/(r^2 + a^2), (r y - a x)/(r^2 + a^2), z / r), h = f l l.
+    const double a = bg.a, M = bg.M;
+    J Xr = X[1] + jet_const<J>(-bg.c[0]), Yr = X[2] + jet_const<J>(-bg.c[1]),
+      Zr = X[3] + jet_const<J>(-bg.c[2]);
+    J rho2 = Xr * Xr + Yr * Yr + Zr * Zr;
+    J hb = 0.5 * (rho2 + jet_const<J>(-a * a));
+    J r2 = hb + dsqrt(hb * hb + (a * a) * (Zr * Zr));
+    J r = dsqrt(r2);
+    J f = (2.0 * M) * (r * r2) / (r2 * r2 + (a * a) * (Zr * Zr));
+    J den = r2 + jet_const<J>(a * a);
+    J l[4];
+    l[0] = jet_const<J>(1.0);
+    l[1] = (r * Xr + a * Yr) / den;
+    l[2] = (r * Yr + (-a) * Xr) / den;
+    l[3] = Zr / r;
 #pragma unroll
-  for (int n = 0; n < NC; ++n) hd[n] = f * (l[prow(n)] * l[pcol(n)]);
+    for (int n = 0; n < NC; ++n) hd[n] = f * (l[prow(n)] * l[pcol(n)]);
+  }
 }
 
 // The sampled gauge source of a background (gauge.jl:156-212): the lowered
@@ -570,7 +797,7 @@ TGH_HD void bg_offset(const Bg& bg, double t, const double x[3], J hd[NC]) {
 TGH_HD void gauge_source_grad(const Bg& bg, double t, const double x[3], double Hl[4],
                               double dH[4][4]) {
   Jet2 hj[NC];
-  bg_offset<Jet2>(bg, t, x, hj);
+  bg_offset<Jet2, BGS_SOURCED>(bg, t, x, hj);
   // g, dg[c][n] = d_c g_n, ddg[c][e][n] = d_c d_e g_n
   double g[4][4], gi[4][4];
 #pragma unroll
@@ -677,9 +904,10 @@ TGH_HD void gauge_source_grad(const Bg& bg, double t, const double x[3], double 
 
 // background_state (src/initialdata.jl:1674-1694): h and
 // Pi = (sqrt(gamma)/alpha)(d_t g - beta^i d_i g) at (t, x).
+template <int SET = BGS_ALL>
 TGH_HD void background_state(const Bg& bg, double t, const double x[3], double h[NC], double P[NC]) {
   Dual<4> hd[NC];
-  bg_offset<Dual<4>>(bg, t, x, hd);
+  bg_offset<Dual<4>, SET>(bg, t, x, hd);
 #pragma unroll
   for (int n = 0; n < NC; ++n) h[n] = hd[n].v;
   Coef c;

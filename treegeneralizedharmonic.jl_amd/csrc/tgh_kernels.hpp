@@ -93,11 +93,14 @@ TGH_HD void core_point(const CaseP& c, const double x[3], double xc[3]) {
     }
   }
 }
+template <int SET = BGS_ALL>
 TGH_HD void case_state(const CaseP& c, double t, const double x[3], double out[20]) {
   double xc[3];
   core_point(c, x, xc);
-  background_state(c.bg, t, xc, out, out + NC);
+  background_state<SET>(c.bg, t, xc, out, out + NC);
 }
+// The backgrounds a right-hand side compiled for a gauge-source kind can meet.
+constexpr int bg_set_for(int gauge) { return gauge == GAUGE_NONE ? BGS_HARMONIC : BGS_SOURCED; }
 
 // Where a point of block b's work array takes its value from (TreeAMR 0.3's
 // closed-block fill on a uniform forest): -2 the Dirichlet hook, -1 an own
@@ -440,7 +443,7 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
     rho = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
   }
   double ue[20];
-  if (layer) case_state(c, a.t, pr.x, ue);  // r >= r0 here: the core rule is the identity
+  if (layer) case_state<bg_set_for(GAUGE)>(c, a.t, pr.x, ue);  // r >= r0 here: the core rule is the identity
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
     double f = D[0][v];
