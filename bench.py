@@ -13,8 +13,11 @@ right-hand side with the stage update in its epilogue) captured in a hipGraph.
 The state is resident in HBM before the timed region.
 
 Prints one JSON line (rank 0) with the metric, a roofline object for the
-dominant kernel (the fused right-hand side) and a CPU baseline (the oracle,
-OpenMP, on a bounded sample).
+dominant kernel, the other forms of the same step (the plain verbs, and the
+plain per-block form: the fused block kernel launched once per block and
+stage), a batch of four holes of uneven block counts stepped by one launch per
+kernel against the same holes stepped alone and per block, and a CPU baseline
+(the oracle, OpenMP, on a bounded sample).
 """
 import argparse
 import ctypes as C
@@ -55,6 +58,8 @@ def parse():
                     help="the stepper's layout and kernels: the brick layout (default), or the "
                          "per-block work arrays with one fused kernel or head + Pi launches")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain (verbs) form")
+    ap.add_argument("--no-per-block", action="store_true", help="skip the plain per-block form")
+    ap.add_argument("--no-batch", action="store_true", help="skip the four-hole batch")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a 1 GiB device-to-device copy, the known-size reference for "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -134,6 +139,58 @@ def cpu_baseline(seconds_target=15.0):
                       f"{points} points), {n} steps in {el:.1f} s, {threads} OpenMP threads"}
 
 
+# The batch workload: four Kerr-Schild holes a = 0, 0.3, 0.5, 0.7 (closed-form
+# gauge source, damped interior r_0 = 0.71, r_1 = 1.21) on uneven forests of
+# 7^3, 9^3, 11^3 and 13^3 blocks of 16^3 at h = M/16 (4600 blocks, 18.8 M points).
+BATCH_HOLES = ((0.0, 7), (0.3, 9), (0.5, 11), (0.7, 13))
+
+
+def batch_bench(tgh, torch, steps):
+    """One batched hipGraph step over the four holes, against the same holes
+    stepped one after another (each its own graph) and by the plain per-block
+    form (one launch per block and stage)."""
+    L = tgh._load()
+    probs, us = [], []
+    for a, roots in BATCH_HOLES:
+        case = tgh.kerr_schild_case(M=1.0, a=a, halfwidth=roots / 2, r_0=0.71, r_1=1.21,
+                                    gauge_source="algebraic")
+        p = tgh.GHProblem(tgh.gh_forest(None, case, N=16, roots=roots), case, q=4)
+        u = p.statevector()
+        tgh.initial_state(u, p, 0.0)
+        probs.append(p)
+        us.append(u)
+    dt = min(tgh.gh_dt(p, u, cfl=0.25) for p, u in zip(probs, us))
+    points = sum(r ** 3 for _, r in BATCH_HOLES) * 16 ** 3
+    batch = tgh.GHBatch(probs)
+    uarr = (C.c_void_p * len(us))(*[u.data_ptr() for u in us])
+    tgh._check(L.tgh_batch_rk4(batch.ptr, uarr, 0.0, dt, 2, 1))  # warm-up and capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tgh._check(L.tgh_batch_rk4(batch.ptr, uarr, 2 * dt, dt, steps, 1))
+    torch.cuda.synchronize()
+    batched = points * steps / (time.perf_counter() - t0)
+    for p, u in zip(probs, us):  # warm-up and capture of each problem's own graph
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, 1, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p, u in zip(probs, us):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, steps, 1))
+    torch.cuda.synchronize()
+    sequential = points * steps / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for p, u in zip(probs, us):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, 1, tgh.TGH_RK4_PER_BLOCK))
+    torch.cuda.synchronize()
+    per_block = points / (time.perf_counter() - t0)
+    assert all(bool(torch.isfinite(u).all()) for u in us), "a batch state is not finite"
+    del batch
+    return {"workload": "four static Kerr-Schild holes a = 0, 0.3, 0.5, 0.7 (closed-form gauge "
+                        "source, damped interior) on 7^3, 9^3, 11^3, 13^3 blocks of 16^3 at h = M/16",
+            "blocks": sum(r ** 3 for _, r in BATCH_HOLES), "points": points, "steps": steps,
+            "batched_graph": batched, "sequential_graphs": sequential, "per_block": per_block,
+            "batched_over_per_block": batched / per_block}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -211,6 +268,17 @@ def main():
     torch.cuda.synchronize()
     plain_value = points * plain_steps / (time.perf_counter() - t1) if plain_steps else None
 
+    # The plain per-block form: the fused block kernel launched once per block
+    # and stage (4096 launches a stage), one step from the current state.
+    per_block_value = None
+    if not args.no_per_block:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        tgh._check(L.tgh_rk4(ptr, uptr, t, dt, 1, tgh.TGH_RK4_PER_BLOCK))
+        torch.cuda.synchronize()
+        per_block_value = points / (time.perf_counter() - t1)
+        t += dt
+
     f_rhs, f_head, f_combine = flop_count()
     f_pi = f_rhs - f_head
     # Per launch: F per point of that kernel's half over the N^3 points of each
@@ -253,11 +321,16 @@ def main():
                      "fill_ms": fill_ms, "flops_per_point_rhs": f_rhs, "flops_per_point_head": f_head,
                      "flops_per_point_update": 4 * f_rhs + f_combine,
                      "step_roofline_frac": t_roof / (step_ms * 1e-3 / points)},
-        "forms": {f"{args.form}_graph": value, "plain_verbs": plain_value},
+        "forms": {f"{args.form}_graph": value, "plain_verbs": plain_value, "per_block": per_block_value,
+                  "graph_over_per_block": value / per_block_value if per_block_value else None},
     }
+    L.tgh_problem_destroy(ptr)
+    del k, y
+    if not args.no_batch:
+        torch.cuda.empty_cache()
+        result["batch"] = batch_bench(tgh, torch, max(1, min(5, args.steps)))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
-    L.tgh_problem_destroy(ptr)
     if args.calibrate_copy:
         x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)

@@ -64,9 +64,13 @@ enum { /* tgh_rk4 flags */
                          (a ghost fill per stage) instead of the default brick
                          layout (every point once, a frame of ghosts around
                          the whole brick) */
-  TGH_RK4_SPLIT = 8   /* the right-hand side as two launches: the head (with
+  TGH_RK4_SPLIT = 8,  /* the right-hand side as two launches: the head (with
                          d_t h) and the ten Pi components; without FUSED or
                          SPLIT the layout's faster form is used */
+  TGH_RK4_PER_BLOCK = 16 /* the plain per-block form: the per-block work arrays
+                         and the fused kernel launched once per block and
+                         stage (implies BLOCKS and FUSED; the baseline the
+                         batched forms are measured against) */
 };
 
 /* The case and its uniform forest: GHCase (src/initialdata.jl:158-311) plus
@@ -111,6 +115,7 @@ typedef struct tgh_matter {
 } tgh_matter;
 
 typedef struct tgh_problem tgh_problem;
+typedef struct tgh_batch tgh_batch;
 
 /* GHProblem(U, schedule, case; q) — src/evolution.jl:1034-1149. Checks what
  * the reference's constructor checks (order, ghost width, interior radii
@@ -176,6 +181,20 @@ int tgh_add_noise(tgh_problem* p, double* u, uint64_t base, double amplitude);
  * arrays (4 x 20 doubles per point of the ghosted brick, and 24 more with
  * TGH_RK4_SPLIT). */
 int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t flags);
+/* Several independent problems stepped together (the batched form: holes of
+ * uneven block counts in one launch per kernel, driven by a device-side table
+ * of each problem's ranges and arguments). The reference steps each problem
+ * by its own gh_solve (src/stepping.jl:208-239); a problem in a batch computes
+ * the bits tgh_rk4 computes for it alone in the brick layout's split form.
+ * The problems share the order q, the gauge-source kind and whether they have
+ * an interior (one kernel specialisation), and none is :pasted; each keeps
+ * its own brick arrays (allocated here if missing). The batch holds the
+ * problems, which must outlive it. TGH_EINVAL on a mismatch. */
+int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out);
+/* gh_solve for every problem of the batch over the same (t0, dt, nsteps):
+ * u[i] is problem i's state (in place). flags: 0 or TGH_RK4_GRAPH. */
+int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t nsteps, int32_t flags);
+int tgh_batch_destroy(tgh_batch* b);
 /* gh_error! + error_norms — src/constraints.jl:311-396,629-671: masked
  * volume-weighted L2 and Linf of u - u_exact(t) over evolved primaries. */
 int tgh_error_norms(tgh_problem* p, const double* u, double t, double* l2, double* linf);

@@ -224,7 +224,8 @@ def test_dirichlet_hook_is_the_exact_state_at_t(tgh):
 
 # tgh_rk4's forms (include/tgh.h flags): the brick layout and the block
 # layout, each with the right-hand side as one fused kernel or as head + Pi.
-RK4_FORMS = {"brick-fused": 2, "brick-split": 8, "blocks-split": 4 | 8, "blocks-fused": 4 | 2}
+RK4_FORMS = {"brick-fused": 2, "brick-split": 8, "blocks-split": 4 | 8, "blocks-fused": 4 | 2,
+             "per-block": 16}
 
 
 @pytest.mark.parametrize("form", list(RK4_FORMS))

@@ -29,7 +29,8 @@ __all__ = [
     "ConstantDamping", "GaussianDamping", "Interior", "GHCase", "minkowski_case",
     "gauge_wave_case", "kerr_schild_case", "harmonic_kerr_case", "gh_forest", "GHProblem", "gh_rhs", "prepare",
     "rhs", "settle", "stage_limit", "step_limit", "max_speed", "initial_state",
-    "evolved_sets", "gh_dt", "gh_solve", "add_noise", "error_norms", "library_path",
+    "evolved_sets", "gh_dt", "gh_solve", "add_noise", "error_norms", "library_path", "GHBatch",
+    "gh_solve_batch",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -54,6 +55,8 @@ TGH_OK, TGH_EINVAL, TGH_EDEVICE, TGH_ENOTFINITE, TGH_EUNSUPPORTED = 0, 1, 2, 3, 
 TGH_RK4_GRAPH = 1
 TGH_RK4_FUSED = 2
 TGH_RK4_BLOCKS = 4
+TGH_RK4_SPLIT = 8
+TGH_RK4_PER_BLOCK = 16
 
 
 class TghDesc(C.Structure):
@@ -117,6 +120,9 @@ def _load():
         "tgh_profile_read": (C.c_int, [P, C.POINTER(D)]),
         "tgh_last_error": (C.c_char_p, []),
         "tgh_apply_bounds": (C.c_int, [P, P, D, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(D)]),
+        "tgh_batch_create": (C.c_int, [C.POINTER(P), C.c_int32, C.POINTER(P)]),
+        "tgh_batch_rk4": (C.c_int, [P, C.POINTER(P), D, D, C.c_int32, C.c_int32]),
+        "tgh_batch_destroy": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -644,3 +650,45 @@ def gh_solve(p, u, tspan, *, dt, graph=False):
         return u
     _check(_load().tgh_rk4(p.ptr, _dev(u, p, "u"), t0, step, nsteps, TGH_RK4_GRAPH if graph else 0))
     return u
+
+
+class GHBatch:
+    """Several independent problems (holes of uneven block counts) stepped
+    together: each RK4 stage is one head and one Pi launch over all of them,
+    driven by a device-side table of the problems' ranges and arguments. The
+    problems share q, the gauge-source kind and whether they have an interior;
+    each computes the bits gh_solve computes for it alone. No reference
+    counterpart: the reference steps each problem by its own gh_solve
+    (src/stepping.jl:208-239)."""
+
+    def __init__(self, problems):
+        self.problems = list(problems)
+        n = len(self.problems)
+        arr = (C.c_void_p * n)(*[p.ptr.value for p in self.problems])
+        self._ptr = C.c_void_p()
+        _check(_load().tgh_batch_create(arr, n, C.byref(self._ptr)))
+
+    def __del__(self):
+        if getattr(self, "_ptr", None) and self._ptr.value:
+            _load().tgh_batch_destroy(self._ptr)
+            self._ptr = C.c_void_p()
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+
+def gh_solve_batch(batch, us, tspan, *, dt, graph=False):
+    """gh_solve for every problem of the batch over the same span (nsteps and
+    Δt as gh_solve's); us[i] is problem i's state, stepped in place."""
+    if len(us) != len(batch.problems):
+        raise ArgumentError(f"the batch holds {len(batch.problems)} problems, got {len(us)} states")
+    t0, t1 = float(tspan[0]), float(tspan[1])
+    nsteps = max(1, math.ceil((t1 - t0) / dt * (1 - 4 * 2.0 ** -52)))
+    step = (t1 - t0) / nsteps
+    ptrs = (C.c_void_p * len(us))(*[_dev(u, p, f"us[{i}]").value
+                                    for i, (u, p) in enumerate(zip(us, batch.problems))])
+    import torch
+    torch.cuda.current_stream().synchronize()
+    _check(_load().tgh_batch_rk4(batch.ptr, ptrs, t0, step, nsteps, TGH_RK4_GRAPH if graph else 0))
+    return us

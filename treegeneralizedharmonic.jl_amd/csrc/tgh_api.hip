@@ -293,6 +293,11 @@ void launch_frame(tgh_problem* p, double* W, double t) {
   hipLaunchKernelGGL(k_brick_frame, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, p->stream, a);
 }
 
+// A background that does not depend on t (the gauge wave is the one that does).
+bool static_background(int kind) {
+  return kind == BG_MINKOWSKI || kind == BG_KERR_SCHILD || kind == BG_SHIFTED || kind == BG_HARMONIC;
+}
+
 int ensure_brick(tgh_problem* p) {
   if (p->brick_ready) return TGH_OK;
   const Geom& g = p->geom;
@@ -320,8 +325,7 @@ int ensure_brick(tgh_problem* p) {
                                    " MiB each): " + hipGetErrorString(e));
     HIP_TRY(hipMemsetAsync(*a, 0, bytes, p->stream));
   }
-  const bool static_bg = p->cs.bg.kind == BG_MINKOWSKI || p->cs.bg.kind == BG_KERR_SCHILD ||
-                         p->cs.bg.kind == BG_SHIFTED;
+  const bool static_bg = static_background(p->cs.bg.kind);
   p->frame_static = static_bg && !g.periodic[0] && !g.periodic[1] && !g.periodic[2];
   if (p->frame_static) {
     launch_frame(p, p->BW[0], 0.0);
@@ -745,16 +749,20 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
   if (nsteps == 0) return TGH_OK;
   const bool graph = (flags & TGH_RK4_GRAPH) != 0;
   const bool blocks = (flags & TGH_RK4_BLOCKS) != 0;
+  if (flags & ~(TGH_RK4_GRAPH | TGH_RK4_FUSED | TGH_RK4_BLOCKS | TGH_RK4_SPLIT | TGH_RK4_PER_BLOCK))
+    return fail(TGH_EINVAL, "unknown tgh_rk4 flags " + std::to_string(flags));
+  if ((flags & TGH_RK4_PER_BLOCK) && (flags & TGH_RK4_SPLIT))
+    return fail(TGH_EINVAL, "TGH_RK4_PER_BLOCK launches the fused kernel: it excludes TGH_RK4_SPLIT");
+  if (flags & TGH_RK4_PER_BLOCK) flags |= TGH_RK4_BLOCKS | TGH_RK4_FUSED;
   if ((flags & TGH_RK4_FUSED) && (flags & TGH_RK4_SPLIT))
     return fail(TGH_EINVAL, "TGH_RK4_FUSED and TGH_RK4_SPLIT exclude each other");
-  if (flags & ~(TGH_RK4_GRAPH | TGH_RK4_FUSED | TGH_RK4_BLOCKS | TGH_RK4_SPLIT))
-    return fail(TGH_EINVAL, "unknown tgh_rk4 flags " + std::to_string(flags));
   // forms: 0 block layout, head + Pi; 1 block layout, fused; 2 brick layout,
-  // fused; 3 brick layout, head + LDS-tiled Pi. Without FUSED or SPLIT the
-  // layout's faster form on MI355X.
+  // fused; 3 brick layout, head + LDS-tiled Pi; 4 block layout, fused, one
+  // launch per block. Without FUSED or SPLIT the layout's faster form on MI355X.
   const bool split = (flags & TGH_RK4_SPLIT) != 0 || (!(flags & TGH_RK4_FUSED) && !blocks && kBrickSplitDefault);
-  const int form = blocks ? (split ? 0 : 1) : (split ? 3 : 2);
-  if (form >= 2) {
+  const int form = (flags & TGH_RK4_PER_BLOCK) ? 4 : blocks ? (split ? 0 : 1) : (split ? 3 : 2);
+  const bool in_brick = form == 2 || form == 3;
+  if (in_brick) {
     int rc = ensure_brick(p);
     if (rc) return rc;
     if (form == 3 && !p->Bcoef) {
@@ -774,9 +782,10 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     HIP_TRY(hipGraphExecDestroy(p->graph_exec));
     p->graph_exec = nullptr;
   }
-  p->fused = form == 1 || form == 2;
+  p->fused = form == 1 || form == 2 || form == 4;
+  p->per_block = form == 4;
   auto step = [&](double t) {
-    if (form >= 2)
+    if (in_brick)
       rk4_brick_launches(p, t, dt);
     else
       rk4_step_launches(p, u, t, dt);
@@ -796,8 +805,7 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     // one (u, dt, t) triple; the time only matters to the Dirichlet hook of a
     // moving background, so with a static one the graph is replayed across
     // steps and re-captured only when dt, u or the form change.
-    const bool static_bg = p->cs.bg.kind == BG_MINKOWSKI || p->cs.bg.kind == BG_KERR_SCHILD ||
-                           p->cs.bg.kind == BG_SHIFTED;
+    const bool static_bg = static_background(p->cs.bg.kind);
     for (int n = 0; n < nsteps; ++n) {
       const double t = t0 + n * dt;
       const bool reuse = p->graph_exec && p->graph_key[0] == dt &&
@@ -837,7 +845,7 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
       HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
     }
   }
-  if (rc == TGH_OK && form >= 2) {
+  if (rc == TGH_OK && in_brick) {
     // u_{n+1} back into the state: the copies, then the primaries
     for (int pass = 0; pass < 2; ++pass) {
       BrickCopyArgs ca{p->geom, p->brick, p->Bus, u, nullptr, pass};
@@ -846,6 +854,7 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
   }
   if (rc == TGH_OK) rc = sync(p);
   p->fused = false;
+  p->per_block = false;
   if (rc == TGH_OK && p->profiling && !graph) collect_profile(p);
   return rc;
 }
@@ -874,6 +883,243 @@ int tgh_copy_work(tgh_problem* p, double* dst) {
   if (!p || !dst) return fail(TGH_EINVAL, "NULL argument");
   HIP_TRY(hipMemcpyAsync(dst, p->W[0], sizeof(double) * p->work_len, hipMemcpyDeviceToDevice, p->stream));
   return sync(p);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: several problems stepped by one head and one Pi launch per stage.
+struct tgh_batch {
+  std::vector<tgh_problem*> probs;
+  int q = 4, gauge = 0;
+  bool hole = false, all_static = true;
+  BatchEntry* d_tab = nullptr;  // 4 stage tables of n entries
+  BatchEntry* h_tab = nullptr;  // pinned staging of the tables
+  int head_total = 0, pi_total = 0;
+  double tab_key[2] = {0, 0};   // (dt, t) the uploaded tables hold
+  bool tab_valid = false;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double graph_key[2] = {0, 0};  // (dt, t) of the captured graph
+};
+
+namespace {
+void pi_tile_dims(int q, int& tx, int& ty, int& tz) {
+  switch (q) {
+    case 2: tx = PiTile<2>::X; ty = PiTile<2>::Y; tz = PiTile<2>::Z * PiTile<2>::PZ; break;
+    case 4: tx = PiTile<4>::X; ty = PiTile<4>::Y; tz = PiTile<4>::Z * PiTile<4>::PZ; break;
+    case 6: tx = PiTile<6>::X; ty = PiTile<6>::Y; tz = PiTile<6>::Z * PiTile<6>::PZ; break;
+    default: tx = PiTile<8>::X; ty = PiTile<8>::Y; tz = PiTile<8>::Z * PiTile<8>::PZ; break;
+  }
+}
+// The stage tables at step time t: problem i's stage-s arguments, as
+// rk4_brick_launches sets them for the problem alone.
+int batch_tables(tgh_batch* b, double t, double dt) {
+  const int n = (int)b->probs.size();
+  const double c_acc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+  const double c_next[4] = {dt / 2, dt / 2, dt, 0.0};
+  const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
+  int h0 = 0, p0 = 0, tx, ty, tz;
+  pi_tile_dims(b->q, tx, ty, tz);
+  for (int i = 0; i < n; ++i) {
+    tgh_problem* p = b->probs[i];
+    const Brick& k = p->brick;
+    const long plane = (long)k.n[0] * k.n[1];
+    const int runs = (int)((plane + 255) / 256) * k.n[2];
+    const int tiles = ((k.n[0] + tx - 1) / tx) * ((k.n[1] + ty - 1) / ty) * ((k.n[2] + tz - 1) / tz);
+    for (int s = 0; s < 4; ++s) {
+      BatchEntry& e = b->h_tab[s * n + i];
+      e.a = rhs_args(p, p->BW[s % 2], t_stage[s], nullptr);
+      e.a.Hsrc = p->BH;
+      e.a.us = p->Bus;
+      e.a.acc = p->Bacc;
+      e.a.Wn = p->BW[(s + 1) % 2];
+      e.a.du = nullptr;
+      e.a.coef = p->Bcoef;
+      e.a.cacc = c_acc[s];
+      e.a.cnext = c_next[s];
+      e.a.stage = s + 1;
+      e.k = k;
+      e.head0 = h0;
+      e.pi0 = p0;
+    }
+    h0 += runs;
+    p0 += tiles;
+  }
+  b->head_total = h0;
+  b->pi_total = p0;
+  HIP_TRY(hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(BatchEntry) * 4 * n, hipMemcpyHostToDevice, b->stream));
+  // the pinned tables are rewritten only after this copy has run
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->tab_key[0] = dt;
+  b->tab_key[1] = t;
+  b->tab_valid = true;
+  return TGH_OK;
+}
+void batch_step_launches(tgh_batch* b, double t, double dt) {
+  const int n = (int)b->probs.size();
+  const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
+  for (int s = 0; s < 4; ++s) {
+    for (tgh_problem* p : b->probs)
+      if (!p->frame_static) launch_frame(p, p->BW[s % 2], t_stage[s]);
+    const BatchEntry* tab = b->d_tab + s * n;
+    switch (b->q) {
+      case 2: launch_batch<2>(b->stream, tab, n, b->head_total, b->pi_total, b->gauge, b->hole); break;
+      case 4: launch_batch<4>(b->stream, tab, n, b->head_total, b->pi_total, b->gauge, b->hole); break;
+      case 6: launch_batch<6>(b->stream, tab, n, b->head_total, b->pi_total, b->gauge, b->hole); break;
+      default: launch_batch<8>(b->stream, tab, n, b->head_total, b->pi_total, b->gauge, b->hole); break;
+    }
+    for (tgh_problem* p : b->probs)
+      if (p->has_bounds) launch_bounds_brick(p, s == 3 ? p->Bus : p->BW[(s + 1) % 2], s == 3 ? p->BW[(s + 1) % 2] : nullptr);
+  }
+}
+void batch_drop_graph(tgh_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+}  // namespace
+
+extern "C" {
+
+int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
+  if (!probs || !out) return fail(TGH_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (n < 1 || n > 1024) return fail(TGH_EINVAL, "a batch holds 1 to 1024 problems, got " + std::to_string(n));
+  for (int i = 0; i < n; ++i) {
+    if (!probs[i]) return fail(TGH_EINVAL, "problem " + std::to_string(i) + " of the batch is NULL");
+    for (int j = 0; j < i; ++j)
+      if (probs[j] == probs[i])
+        return fail(TGH_EINVAL, "problem " + std::to_string(i) + " appears twice in the batch: its brick arrays "
+                                "would be stepped twice per stage");
+  }
+  const tgh_problem* p0 = probs[0];
+  for (int i = 0; i < n; ++i) {
+    const tgh_problem* p = probs[i];
+    if (p->q != p0->q || p->cs.gauge != p0->cs.gauge ||
+        (p->cs.interior != INT_NONE) != (p0->cs.interior != INT_NONE))
+      return fail(TGH_EINVAL, "the problems of a batch share the order, the gauge-source kind and whether they "
+                              "have an interior (one kernel specialisation runs them all); problem " +
+                                  std::to_string(i) + " differs from problem 0");
+    if (p->cs.interior == INT_PASTED)
+      return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
+  }
+  tgh_batch* b = new tgh_batch;
+  b->probs.assign(probs, probs + n);
+  b->q = p0->q;
+  b->gauge = p0->cs.gauge;
+  b->hole = p0->cs.interior != INT_NONE;
+  auto bail = [&](int rc) {
+    tgh_batch_destroy(b);
+    return rc;
+  };
+  for (tgh_problem* p : b->probs) {
+    int rc = ensure_brick(p);
+    if (rc) return bail(rc);
+    if (!p->Bcoef) {
+      const size_t bytes = sizeof(double) * CF_N * (size_t)p->brick.fs;
+      if (hipMalloc(&p->Bcoef, bytes) != hipSuccess) return bail(fail(TGH_EDEVICE, "allocating a batch's coefficients"));
+      if (hipMemsetAsync(p->Bcoef, 0, bytes, p->stream) != hipSuccess) return bail(fail(TGH_EDEVICE, "clearing them"));
+    }
+    rc = sync(p);
+    if (rc) return bail(rc);
+    b->all_static = b->all_static && p->frame_static;
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(TGH_EDEVICE, "creating the batch's stream"));
+  if (hipMalloc(&b->d_tab, sizeof(BatchEntry) * 4 * n) != hipSuccess ||
+      hipHostMalloc(&b->h_tab, sizeof(BatchEntry) * 4 * n) != hipSuccess)
+    return bail(fail(TGH_EDEVICE, "allocating the batch's tables"));
+  *out = b;
+  return TGH_OK;
+}
+
+int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t nsteps, int32_t flags) {
+  if (!b || !u) return fail(TGH_EINVAL, "NULL argument");
+  if (nsteps < 0 || !(dt > 0)) return fail(TGH_EINVAL, "tgh_batch_rk4 needs dt > 0 and nsteps >= 0");
+  if (flags & ~TGH_RK4_GRAPH) return fail(TGH_EINVAL, "tgh_batch_rk4 takes 0 or TGH_RK4_GRAPH, got " + std::to_string(flags));
+  const int n = (int)b->probs.size();
+  for (int i = 0; i < n; ++i)
+    if (!u[i]) return fail(TGH_EINVAL, "state " + std::to_string(i) + " of the batch is NULL");
+  if (nsteps == 0) return TGH_OK;
+  const bool graph = (flags & TGH_RK4_GRAPH) != 0;
+  // every launch of the batch (the problems' own frame, bounds and copies
+  // included) goes on the batch's stream
+  std::vector<hipStream_t> saved(n);
+  for (int i = 0; i < n; ++i) {
+    saved[i] = b->probs[i]->stream;
+    HIP_TRY(hipStreamSynchronize(saved[i]));
+    b->probs[i]->stream = b->stream;
+  }
+  int rc = TGH_OK;
+  for (int i = 0; i < n; ++i) {
+    tgh_problem* p = b->probs[i];
+    BrickCopyArgs ca{p->geom, p->brick, u[i], p->BW[0], p->Bus, 0};
+    hipLaunchKernelGGL(k_brick_in, grid_brick(p), dim3(256), 0, b->stream, ca);
+  }
+  for (int s = 0; s < nsteps && rc == TGH_OK; ++s) {
+    const double t = t0 + s * dt;
+    // static backgrounds ignore t: the tables (and the graph) are kept across steps
+    const bool stale = !b->tab_valid || b->tab_key[0] != dt || (!b->all_static && b->tab_key[1] != t);
+    if (stale) {
+      rc = batch_tables(b, t, dt);
+      if (rc) break;
+    }
+    if (!graph) {
+      batch_step_launches(b, t, dt);
+      continue;
+    }
+    const bool reuse = b->exec && b->graph_key[0] == dt && (b->all_static || b->graph_key[1] == t);
+    if (!reuse) {
+      batch_drop_graph(b);
+      hipError_t ec = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+      if (ec == hipSuccess) {
+        batch_step_launches(b, t, dt);
+        ec = hipStreamEndCapture(b->stream, &b->graph);
+      }
+      if (ec == hipSuccess) ec = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+      if (ec != hipSuccess) {
+        batch_drop_graph(b);
+        rc = fail(TGH_EDEVICE, std::string("capturing the batch's RK4 step: ") + hipGetErrorString(ec));
+        break;
+      }
+      b->graph_key[0] = dt;
+      b->graph_key[1] = t;
+    }
+    if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) {
+      rc = fail(TGH_EDEVICE, "launching the batch's RK4 graph");
+      break;
+    }
+  }
+  if (rc == TGH_OK) {
+    for (int i = 0; i < n; ++i) {
+      tgh_problem* p = b->probs[i];
+      for (int pass = 0; pass < 2; ++pass) {
+        BrickCopyArgs ca{p->geom, p->brick, p->Bus, u[i], nullptr, pass};
+        hipLaunchKernelGGL(k_brick_out, grid_points(p), dim3(256), 0, b->stream, ca);
+      }
+    }
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(TGH_EDEVICE, std::string("the batch's RK4 steps: ") + hipGetErrorString(e));
+  } else {
+    hipStreamSynchronize(b->stream);
+  }
+  for (int i = 0; i < n; ++i) b->probs[i]->stream = saved[i];
+  return rc;
+}
+
+int tgh_batch_destroy(tgh_batch* b) {
+  if (!b) return TGH_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  if (b->d_tab) hipFree(b->d_tab);
+  if (b->h_tab) hipHostFree(b->h_tab);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return TGH_OK;
 }
 
 }  // extern "C"

@@ -137,11 +137,14 @@ TGH_HD int classify(const Geom& g, int b, const int l[3], int& owner, int loc[3]
 // range of logical workgroups: a run of whole blocks, consecutive in Morton
 // order, whose work arrays and neighbours stay in that XCD's L2.
 constexpr int NUM_XCD = 8;
-__device__ __forceinline__ bool wg_map(int wgpb, int nblocks, int& b, int& wx) {
-  const int total = wgpb * nblocks;
+__device__ __forceinline__ int wg_logical(int total) {
   const int per = (total + NUM_XCD - 1) / NUM_XCD;
   const int flat = blockIdx.x;
-  const int L = (flat % NUM_XCD) * per + flat / NUM_XCD;
+  return (flat % NUM_XCD) * per + flat / NUM_XCD;
+}
+__device__ __forceinline__ bool wg_map(int wgpb, int nblocks, int& b, int& wx) {
+  const int total = wgpb * nblocks;
+  const int L = wg_logical(total);
   if (L >= total) return false;
   b = L / wgpb;
   wx = L - b * wgpb;
@@ -893,16 +896,24 @@ TGH_HD long brick_index(const Brick& k, int x, int y, int z) {
 // values of a row (or the end of one row and the start of the next), and the
 // runs are dealt to the XCDs in contiguous ranges (wg_map's order): each XCD
 // sweeps one slab of planes, whose stencil neighbours it shares in its L2.
-__device__ __forceinline__ bool brick_map(const Brick& k, int& x, int& y, int& z) {
+// The runs of the brick: wpp per plane, brick_runs in all.
+__device__ __forceinline__ int brick_runs(const Brick& k) {
+  const long plane = (long)k.n[0] * k.n[1];
+  return (int)((plane + 255) / 256) * k.n[2];
+}
+// This thread's point of logical run L (0 <= L < brick_runs(k)).
+__device__ __forceinline__ bool brick_run(const Brick& k, int L, int& x, int& y, int& z) {
   const long plane = (long)k.n[0] * k.n[1];
   const int wpp = (int)((plane + 255) / 256);
   int r;
 #ifdef TGH_BRICK_ZFAST
   // runs ordered z-fastest: consecutive workgroups of an XCD take the same
   // run of consecutive planes, which share their z-stencil planes
-  if (!wg_map(k.n[2], wpp, r, z)) return false;
+  r = L / k.n[2];
+  z = L - r * k.n[2];
 #else
-  if (!wg_map(wpp, k.n[2], z, r)) return false;
+  z = L / wpp;
+  r = L - z * wpp;
 #endif
   const long lin = (long)r * 256 + threadIdx.x;
   if (lin >= plane) return false;
@@ -910,16 +921,26 @@ __device__ __forceinline__ bool brick_map(const Brick& k, int& x, int& y, int& z
   x = (int)(lin - (long)y * k.n[0]);
   return true;
 }
+__device__ __forceinline__ bool brick_map(const Brick& k, int& x, int& y, int& z) {
+  const int total = brick_runs(k);
+  const int L = wg_logical(total);
+  if (L >= total) return false;
+  return brick_run(k, L, x, y, z);
+}
 
 // Tiles of TX x TY x TZ points over the brick, dealt to the XCDs in
 // contiguous ranges of a z-fastest (TGH_TILES_ZFAST) or x-fastest order.
 // z-fastest: the tiles an XCD holds at once are a column of consecutive
 // z-tiles, which share their z-halo planes in its L2.
-__device__ __forceinline__ bool tile_map(const Brick& k, int TX, int TY, int TZ, int& X0, int& Y0, int& Z0) {
-  const int ntx = (k.n[0] + TX - 1) / TX, nty = (k.n[1] + TY - 1) / TY, ntz = (k.n[2] + TZ - 1) / TZ;
-  int tile, unused;
-  if (!wg_map(1, ntx * nty * ntz, tile, unused)) return false;
+__device__ __forceinline__ int brick_tiles(const Brick& k, int TX, int TY, int TZ) {
+  return ((k.n[0] + TX - 1) / TX) * ((k.n[1] + TY - 1) / TY) * ((k.n[2] + TZ - 1) / TZ);
+}
+// The corner of logical tile L (0 <= L < brick_tiles).
+__device__ __forceinline__ void tile_at(const Brick& k, int TX, int TY, int TZ, int tile, int& X0, int& Y0,
+                                        int& Z0) {
+  const int ntx = (k.n[0] + TX - 1) / TX, nty = (k.n[1] + TY - 1) / TY;
 #ifdef TGH_TILES_ZFAST
+  const int ntz = (k.n[2] + TZ - 1) / TZ;
   const int tz = tile % ntz, txy = tile / ntz;
   X0 = (txy % ntx) * TX;
   Y0 = (txy / ntx) * TY;
@@ -929,6 +950,12 @@ __device__ __forceinline__ bool tile_map(const Brick& k, int TX, int TY, int TZ,
   Y0 = ((tile / ntx) % nty) * TY;
   Z0 = (tile / (ntx * nty)) * TZ;
 #endif
+}
+__device__ __forceinline__ bool tile_map(const Brick& k, int TX, int TY, int TZ, int& X0, int& Y0, int& Z0) {
+  const int total = brick_tiles(k, TX, TY, TZ);
+  const int L = wg_logical(total);
+  if (L >= total) return false;
+  tile_at(k, TX, TY, TZ, L, X0, Y0, Z0);
   return true;
 }
 
@@ -970,6 +997,9 @@ __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const dou
 #define TGH_HEAD_ATTR
 #endif
 template <int Q, int GAUGE, bool HOLE>
+__device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
+                                              double* __restrict__ coef, TGH_OUT_PARAMS, int x, int y, int z);
+template <int Q, int GAUGE, bool HOLE>
 __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head(RhsArgs a, Brick k, const double* __restrict__ W,
                                                     double* __restrict__ coef, TGH_OUT_PARAMS) {
   int x, y, z;
@@ -987,6 +1017,11 @@ __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head(RhsArgs a, Bri
 #else
   if (!brick_map(k, x, y, z)) return;
 #endif
+  brick_head_at<Q, GAUGE, HOLE>(a, k, W, coef, TGH_OUT_ARGS, x, y, z);
+}
+template <int Q, int GAUGE, bool HOLE>
+__device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
+                                              double* __restrict__ coef, TGH_OUT_PARAMS, int x, int y, int z) {
   const long off = brick_index(k, x, y, z);
   PointRef pr;
   pr.w0 = W + off;
@@ -1247,8 +1282,21 @@ template <int Q> struct PiTile {
 #define TGH_BRICK_PI_ATTR
 #endif
 template <int Q>
+__device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
+                                              const double* __restrict__ coef, TGH_OUT_PARAMS, int X0, int Y0,
+                                              int Z0);
+template <int Q>
 __global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRICK_PI_ATTR void k_brick_pi(
     RhsArgs a, Brick k, const double* __restrict__ W, const double* __restrict__ coef, TGH_OUT_PARAMS) {
+  int X0, Y0, Z0;
+  if (!tile_map(k, PiTile<Q>::X, PiTile<Q>::Y, PiTile<Q>::Z * PiTile<Q>::PZ, X0, Y0, Z0)) return;  // uniform
+  brick_pi_tile<Q>(a, k, W, coef, TGH_OUT_ARGS, X0, Y0, Z0);
+}
+// One tile of the Pi kernel, corner (X0, Y0, Z0), one workgroup.
+template <int Q>
+__device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
+                                              const double* __restrict__ coef, TGH_OUT_PARAMS, int X0, int Y0,
+                                              int Z0) {
   constexpr int TX = PiTile<Q>::X, TY = PiTile<Q>::Y, TZ = PiTile<Q>::Z, PZ = PiTile<Q>::PZ;
   constexpr int NT = TX * TY * TZ, DZ = TZ * PZ;  // threads; the tile's depth in points
   constexpr int H = ::tgh::W<Q>::R + 1;
@@ -1257,8 +1305,6 @@ __global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRI
   constexpr int BX = TX + 2 * H, BY = TY + 2 * H, BZ = DZ + 2 * H, BV = BX * BY * BZ;
   constexpr int RX = PiTile<Q>::pitch(BX), RV = RX * BY * BZ;
   __shared__ double sh[2][RV];
-  int X0, Y0, Z0;
-  if (!tile_map(k, TX, TY, DZ, X0, Y0, Z0)) return;  // uniform over the workgroup
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
   const int x = X0 + lx, y = Y0 + ly;
@@ -1368,6 +1414,45 @@ __global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRI
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Several problems in one launch (independent holes of uneven block counts):
+// a device-side table holds each problem's stage arguments, its brick and the
+// first logical workgroup of its range in the head and the Pi launches. A
+// workgroup takes its logical index in wg_logical's XCD-contiguous order over
+// the whole launch, finds the problem whose range holds it, and runs that
+// problem's run (head) or tile (Pi) -- the same code as the one-problem
+// kernels, so a problem in a batch computes the bits it computes alone.
+struct BatchEntry {
+  RhsArgs a;  // W, Wn, us, acc, coef, t, stage and the stage coefficients set
+  Brick k;
+  int head0, pi0;  // first logical workgroup in the head and the Pi launch
+};
+__device__ __forceinline__ int batch_find(const BatchEntry* __restrict__ tab, int n, int L, bool pi) {
+  int p = 0;
+  while (p + 1 < n && L >= (pi ? tab[p + 1].pi0 : tab[p + 1].head0)) ++p;
+  return p;
+}
+template <int Q, int GAUGE, bool HOLE>
+__global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head_batch(const BatchEntry* __restrict__ tab, int n,
+                                                                        int total) {
+  const int L = wg_logical(total);
+  if (L >= total) return;
+  const BatchEntry& e = tab[batch_find(tab, n, L, false)];
+  int x, y, z;
+  if (!brick_run(e.k, L - e.head0, x, y, z)) return;
+  brick_head_at<Q, GAUGE, HOLE>(e.a, e.k, e.a.W, e.a.coef, e.a.du, e.a.us, e.a.acc, e.a.Wn, x, y, z);
+}
+template <int Q>
+__global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRICK_PI_ATTR void k_brick_pi_batch(
+    const BatchEntry* __restrict__ tab, int n, int total) {
+  const int L = wg_logical(total);
+  if (L >= total) return;  // uniform over the workgroup
+  const BatchEntry& e = tab[batch_find(tab, n, L, true)];
+  int X0, Y0, Z0;
+  tile_at(e.k, PiTile<Q>::X, PiTile<Q>::Y, PiTile<Q>::Z * PiTile<Q>::PZ, L - e.pi0, X0, Y0, Z0);
+  brick_pi_tile<Q>(e.a, e.k, e.a.W, e.a.coef, e.a.du, e.a.us, e.a.acc, e.a.Wn, X0, Y0, Z0);
 }
 
 #ifndef TGH_RHS_ONLY

@@ -21,6 +21,28 @@ constexpr bool kHeadTwo = TGH_HEAD_TWO != 0;
 template <int Q, int GAUGE, bool HOLE, int EPI>
 void launch_rhs_q(tgh_problem* p, const RhsArgs& a) {
   const bool prof = p->profiling && EPI == EPI_RK4;
+  if (p->per_block) {
+    // One launch per block (the fused kernel over that block alone): the
+    // plain per-block form, kept as the baseline of the batched forms. Block
+    // b sees a one-block mesh whose coordinates and arrays start at its own.
+    const long P3 = (long)a.g.P * a.g.P * a.g.P, S3 = (long)a.g.S * a.g.S * a.g.S;
+    const dim3 grid = xcd_grid((P3 + 255) / 256);
+    for (int b = 0; b < a.g.nblocks; ++b) {
+      RhsArgs ab = a;
+      ab.g.nblocks = 1;
+      ab.g.bcoord = a.g.bcoord + 3 * b;
+      const long ws = (long)b * 20 * S3, ss = (long)b * 20 * P3;
+      ab.W = a.W + ws;
+      if (a.Wn) ab.Wn = a.Wn + ws;
+      if (a.us) ab.us = a.us + ss;
+      if (a.acc) ab.acc = a.acc + ss;
+      if (a.du) ab.du = a.du + ss;
+      if (a.Hsrc) ab.Hsrc = a.Hsrc + ss;
+      hipLaunchKernelGGL((k_rhs<Q, GAUGE, HOLE, EPI>), grid, dim3(256), 0, p->stream, ab, ab.W, ab.du, ab.us,
+                         ab.acc, ab.Wn);
+    }
+    return;
+  }
   if (p->fused) {
     int e0 = prof ? mark(p) : -1;
     hipLaunchKernelGGL((k_rhs<Q, GAUGE, HOLE, EPI>), grid_points(p), dim3(256), 0, p->stream, a, a.W,
@@ -128,8 +150,36 @@ void launch_rhs_brick(tgh_problem* p, const RhsArgs& a) {
   launch_brick_rhs_g<Q>(p, a);
 }
 
+// One RK4 stage of a batch of problems: the head and the Pi kernel, each one
+// launch over every problem's runs or tiles (the table's ranges).
+template <int Q, int GAUGE, bool HOLE>
+void launch_batch_q(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total) {
+  using T = PiTile<Q>;
+  hipLaunchKernelGGL((k_brick_head_batch<Q, GAUGE, HOLE>), xcd_grid(head_total), dim3(256), 0, st, tab, n,
+                     head_total);
+  hipLaunchKernelGGL((k_brick_pi_batch<Q>), xcd_grid(pi_total), dim3(T::X * T::Y * T::Z), 0, st, tab, n, pi_total);
+}
+template <int Q>
+void launch_batch(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total, int gauge, bool hole) {
+  switch (gauge) {
+    case GAUGE_KS:
+      if (hole) launch_batch_q<Q, GAUGE_KS, true>(st, tab, n, head_total, pi_total);
+      else launch_batch_q<Q, GAUGE_KS, false>(st, tab, n, head_total, pi_total);
+      break;
+    case GAUGE_SAMPLED:
+      if (hole) launch_batch_q<Q, GAUGE_SAMPLED, true>(st, tab, n, head_total, pi_total);
+      else launch_batch_q<Q, GAUGE_SAMPLED, false>(st, tab, n, head_total, pi_total);
+      break;
+    default:
+      if (hole) launch_batch_q<Q, GAUGE_NONE, true>(st, tab, n, head_total, pi_total);
+      else launch_batch_q<Q, GAUGE_NONE, false>(st, tab, n, head_total, pi_total);
+      break;
+  }
+}
+
 }  // namespace tgh
 
 #define TGH_INSTANTIATE_ORDER(Q)                                                            \
   template void tgh::launch_rhs_blocks<Q>(tgh_problem*, const tgh::RhsArgs&, int);          \
-  template void tgh::launch_rhs_brick<Q>(tgh_problem*, const tgh::RhsArgs&);
+  template void tgh::launch_rhs_brick<Q>(tgh_problem*, const tgh::RhsArgs&);                \
+  template void tgh::launch_batch<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int, bool);

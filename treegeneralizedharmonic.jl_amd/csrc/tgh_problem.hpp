@@ -37,6 +37,7 @@ struct tgh_problem {
   // Benchmark timing: event pairs around each launch while profiling.
   bool profiling = false;
   bool fused = false;                 // one kernel per RHS instead of head + Pi
+  bool per_block = false;             // the fused block kernel launched once per block
   int graph_form = -1;                // the form the captured graph launches
   // The stepper's brick layout (allocated at the first tgh_rk4 that uses it):
   // two stage arrays, u_n and the accumulator, and the sampled gauge source.
@@ -76,4 +77,12 @@ extern template void launch_rhs_brick<2>(tgh_problem*, const RhsArgs&);
 extern template void launch_rhs_brick<4>(tgh_problem*, const RhsArgs&);
 extern template void launch_rhs_brick<6>(tgh_problem*, const RhsArgs&);
 extern template void launch_rhs_brick<8>(tgh_problem*, const RhsArgs&);
+// One RK4 stage of a batch of problems on the brick layout (the head and the
+// Pi launch over the table's ranges), at order Q.
+template <int Q>
+void launch_batch(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total, int gauge, bool hole);
+extern template void launch_batch<2>(hipStream_t, const BatchEntry*, int, int, int, int, bool);
+extern template void launch_batch<4>(hipStream_t, const BatchEntry*, int, int, int, int, bool);
+extern template void launch_batch<6>(hipStream_t, const BatchEntry*, int, int, int, int, bool);
+extern template void launch_batch<8>(hipStream_t, const BatchEntry*, int, int, int, int, bool);
 }  // namespace tgh
