@@ -941,6 +941,12 @@ int batch_tables(tgh_batch* b, double t, double dt) {
       e.a.cnext = c_next[s];
       e.a.stage = s + 1;
       e.k = k;
+      e.W = (decltype(e.W))e.a.W;
+      e.Hsrc = (decltype(e.Hsrc))e.a.Hsrc;
+      e.coef = (decltype(e.coef))e.a.coef;
+      e.us = (decltype(e.us))e.a.us;
+      e.acc = (decltype(e.acc))e.a.acc;
+      e.Wn = (decltype(e.Wn))e.a.Wn;
       e.head0 = h0;
       e.pi0 = p0;
     }

@@ -998,7 +998,8 @@ __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const dou
 #endif
 template <int Q, int GAUGE, bool HOLE>
 __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
-                                              double* __restrict__ coef, TGH_OUT_PARAMS, int x, int y, int z);
+                                              const double* __restrict__ Hsrc, double* __restrict__ coef,
+                                              TGH_OUT_PARAMS, int x, int y, int z);
 template <int Q, int GAUGE, bool HOLE>
 __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head(RhsArgs a, Brick k, const double* __restrict__ W,
                                                     double* __restrict__ coef, TGH_OUT_PARAMS) {
@@ -1017,11 +1018,12 @@ __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head(RhsArgs a, Bri
 #else
   if (!brick_map(k, x, y, z)) return;
 #endif
-  brick_head_at<Q, GAUGE, HOLE>(a, k, W, coef, TGH_OUT_ARGS, x, y, z);
+  brick_head_at<Q, GAUGE, HOLE>(a, k, W, a.Hsrc, coef, TGH_OUT_ARGS, x, y, z);
 }
 template <int Q, int GAUGE, bool HOLE>
 __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
-                                              double* __restrict__ coef, TGH_OUT_PARAMS, int x, int y, int z) {
+                                              const double* __restrict__ Hsrc, double* __restrict__ coef,
+                                              TGH_OUT_PARAMS, int x, int y, int z) {
   const long off = brick_index(k, x, y, z);
   PointRef pr;
   pr.w0 = W + off;
@@ -1034,7 +1036,7 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
   pr.si0 = off;
   pr.ssv = k.fs;
   pr.wi0 = off;
-  pr.Hp = a.Hsrc ? a.Hsrc + off : nullptr;
+  pr.Hp = Hsrc ? Hsrc + off : nullptr;
   pr.hs = k.fs;
   pr.m_rho = nullptr;
   pr.m_S = nullptr;
@@ -1424,9 +1426,22 @@ __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, 
 // the whole launch, finds the problem whose range holds it, and runs that
 // problem's run (head) or tile (Pi) -- the same code as the one-problem
 // kernels, so a problem in a batch computes the bits it computes alone.
+// The table's arrays, typed as global memory on the device: pointers loaded
+// from memory are otherwise generic, and their loads and stores flat.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define TGH_GLOBAL __attribute__((address_space(1)))
+#else
+#define TGH_GLOBAL
+#endif
 struct BatchEntry {
-  RhsArgs a;  // W, Wn, us, acc, coef, t, stage and the stage coefficients set
+  RhsArgs a;  // the stage's t, stage and stage coefficients (and the case)
   Brick k;
+  TGH_GLOBAL const double* W;  // the stage input
+  TGH_GLOBAL const double* Hsrc;
+  TGH_GLOBAL double* coef;
+  TGH_GLOBAL double* us;
+  TGH_GLOBAL double* acc;
+  TGH_GLOBAL double* Wn;  // the next stage's input
   int head0, pi0;  // first logical workgroup in the head and the Pi launch
 };
 __device__ __forceinline__ int batch_find(const BatchEntry* __restrict__ tab, int n, int L, bool pi) {
@@ -1442,7 +1457,8 @@ __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head_batch(const Ba
   const BatchEntry& e = tab[batch_find(tab, n, L, false)];
   int x, y, z;
   if (!brick_run(e.k, L - e.head0, x, y, z)) return;
-  brick_head_at<Q, GAUGE, HOLE>(e.a, e.k, e.a.W, e.a.coef, e.a.du, e.a.us, e.a.acc, e.a.Wn, x, y, z);
+  brick_head_at<Q, GAUGE, HOLE>(e.a, e.k, (const double*)e.W, (const double*)e.Hsrc, (double*)e.coef, nullptr,
+                                (double*)e.us, (double*)e.acc, (double*)e.Wn, x, y, z);
 }
 template <int Q>
 __global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRICK_PI_ATTR void k_brick_pi_batch(
@@ -1452,7 +1468,8 @@ __global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRI
   const BatchEntry& e = tab[batch_find(tab, n, L, true)];
   int X0, Y0, Z0;
   tile_at(e.k, PiTile<Q>::X, PiTile<Q>::Y, PiTile<Q>::Z * PiTile<Q>::PZ, L - e.pi0, X0, Y0, Z0);
-  brick_pi_tile<Q>(e.a, e.k, e.a.W, e.a.coef, e.a.du, e.a.us, e.a.acc, e.a.Wn, X0, Y0, Z0);
+  brick_pi_tile<Q>(e.a, e.k, (const double*)e.W, (const double*)e.coef, nullptr, (double*)e.us, (double*)e.acc,
+                   (double*)e.Wn, X0, Y0, Z0);
 }
 
 #ifndef TGH_RHS_ONLY
