@@ -103,6 +103,13 @@ typedef struct tgh_desc {
   /* StateBounds (src/bounds.jl:51-136): the range projection's ranges, applied at
    * r < r_gate from center; r_gate = 0 is a case without bounds (`nothing`). */
   double alpha_min, alpha_max, lambda_min, lambda_max, beta_max, K_max, r_gate;
+  /* The hole's coordinate velocity v (HoleCenter, src/interior.jl:60-93):
+   * with v != 0 a Kerr-Schild or harmonic background is
+   * translate(boost(bg, -v), center) (kerr_schild_case / harmonic_kerr_case's
+   * moving hole, src/initialdata.jl:1283-1298), the hole at center + v t;
+   * every distance to the hole is measured from center + v t, and a :damped
+   * core follows the hole (src/interior.jl:390-549). |v| < 1. */
+  double velocity[3];
 } tgh_desc;
 
 /* Matter's 3+1 projections at every owned point (src/evolution.jl:1295-1311,

@@ -849,7 +849,53 @@ struct BgParams {
   double c[3] = {0, 0, 0};        // translation of the hole
   double A = 0.05, d = 1.0;       // gauge wave amplitude and wavelength / shifted A
   double w = 2.0;                 // shifted Minkowski width
+  double u[3] = {0, 0, 0};        // boost velocity: the hole moves at -u
 };
+
+template <class S>
+inline void bg_metric(const BgParams& p, S t, S x, S y, S z, S g[4][4]);
+
+// boost(m, u) and translate (SpacetimeMetrics, unvendored; the convention the
+// reference states at src/interior.jl:66-72 and src/initialdata.jl:93-96,
+// 625-638): m is evaluated at x' = Lambda (t, x - c), Lambda the symmetric
+// Lorentz boost of velocity u,
+//   Lambda^0_0 = gamma, Lambda^0_i = Lambda^i_0 = gamma u_i,
+//   Lambda^i_j = delta_ij + (gamma - 1) u_i u_j / u^2,
+// and pulled back, g_ab(x) = Lambda^c_a Lambda^d_b m_cd(x'): the rest frame's
+// origin x' = 0 is x = c - u t, so the hole moves at -u.
+inline void lorentz_boost(const double u[3], double L[4][4]) {
+  const double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const double gam = 1.0 / std::sqrt(1.0 - u2);
+  const double f = (gam - 1.0) / u2;
+  L[0][0] = gam;
+  for (int i = 0; i < 3; ++i) {
+    L[0][i + 1] = gam * u[i];
+    L[i + 1][0] = gam * u[i];
+    for (int j = 0; j < 3; ++j) L[i + 1][j + 1] = (i == j ? 1.0 : 0.0) + f * u[i] * u[j];
+  }
+}
+template <class S>
+inline void boosted_metric(const BgParams& p, S t, S x, S y, S z, S g[4][4]) {
+  double L[4][4];
+  lorentz_boost(p.u, L);
+  S xr[4] = {t, x - p.c[0], y - p.c[1], z - p.c[2]};
+  S xp[4];
+  for (int c = 0; c < 4; ++c) xp[c] = L[c][0] * xr[0] + L[c][1] * xr[1] + L[c][2] * xr[2] + L[c][3] * xr[3];
+  BgParams q = p;
+  for (int i = 0; i < 3; ++i) q.u[i] = q.c[i] = 0.0;
+  S gp[4][4];
+  bg_metric(q, xp[0], xp[1], xp[2], xp[3], gp);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      S sum(0.0);
+      for (int c = 0; c < 4; ++c)
+        for (int d = 0; d < 4; ++d) sum = sum + (L[c][a] * L[d][b]) * gp[c][d];
+      g[a][b] = sum;
+    }
+}
+inline bool bg_boosted(const BgParams& p) {
+  return (p.kind == BG_KERR_SCHILD || p.kind == BG_HARMONIC) && (p.u[0] != 0.0 || p.u[1] != 0.0 || p.u[2] != 0.0);
+}
 
 // The Kerr metric in harmonic coordinates (SpacetimeMetrics' Harmonic(M, a),
 // an unvendored dependency; restated from Cook & Scheel, Phys. Rev. D 56,
@@ -912,6 +958,10 @@ inline void harmonic_kerr_metric(const BgParams& p, S X, S Y, S Z, S g[4][4]) {
 // dmetric).
 template <class S>
 inline void bg_metric(const BgParams& p, S t, S x, S y, S z, S g[4][4]) {
+  if (bg_boosted(p)) {
+    boosted_metric(p, t, x, y, z, g);
+    return;
+  }
   for (int a = 0; a < 4; ++a)
     for (int b = 0; b < 4; ++b) g[a][b] = S(eta(a, b));
   if (p.kind == BG_MINKOWSKI) return;
@@ -1089,13 +1139,30 @@ struct Case {
   int diss = DISS_CONSTANT;
   double eps_in = 0, eps_r1 = 0, eps_rh = 0;  // HorizonDissipation (gauge.jl:1221-1227)
   int interior = INT_NONE;
-  double center[3] = {0, 0, 0};
+  double center[3] = {0, 0, 0};  // the hole's center at t = 0
+  double vel[3] = {0, 0, 0};     // and its velocity: c(t) = c0 + v t
   double r0 = 0, r1 = 0, rho_max = 0, w_ramp = 0.5, rho_ramp = 0.5;
+  double delta = 0;              // layer_rate_step: cbrt(eps) r_0
 };
 
-inline double damping_rate(const Case& c, const double x[3]) {
+// center_at (interior.jl:60-93): c(t) = c0 + v t.
+inline void center_at(const Case& c, double t, double out[3]) {
+  for (int i = 0; i < 3; ++i) out[i] = c.center[i] + t * c.vel[i];
+}
+inline bool hole_moves(const Case& c) { return c.vel[0] != 0.0 || c.vel[1] != 0.0 || c.vel[2] != 0.0; }
+// core_follows (interior.jl:520-523): a moving :damped sphere.
+inline bool core_follows(const Case& c) { return c.interior == INT_DAMPED && hole_moves(c); }
+// core_rate (interior.jl:480-484): max(rho_max, 12 |v| / r_0).
+inline double core_rate(const Case& c) {
+  const double s = std::sqrt(c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2]);
+  return std::max(c.rho_max, 12.0 * s / c.r0);
+}
+
+inline double damping_rate(const Case& c, double t, const double x[3]) {
   if (c.damping == DAMP_CONSTANT) return c.g0;
-  double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
+  double cc[3];
+  center_at(c, t, cc);
+  double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
   double r2 = d1 * d1 + d2 * d2 + d3 * d3;
   return c.g0_far + (c.g0_near - c.g0_far) * std::exp(-r2 / (2.0 * c.g0_width * c.g0_width));
 }
@@ -1103,9 +1170,11 @@ inline double damping_rate(const Case& c, const double x[3]) {
 // HorizonDissipation profile about the (static) center: eps_out at and
 // outside r_h, eps_in at and inside r_1, each through a branch, and the
 // quintic smoothstep of (r_h - r)/(r_h - r_1) between.
-inline double dissipation_rate(const Case& c, const double x[3]) {
+inline double dissipation_rate(const Case& c, double t, const double x[3]) {
   if (c.diss == DISS_CONSTANT) return c.eps_ko;
-  double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
+  double cc[3];
+  center_at(c, t, cc);
+  double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
   double r = std::sqrt(d1 * d1 + d2 * d2 + d3 * d3);
   if (r >= c.eps_rh) return c.eps_ko;
   if (r <= c.eps_r1) return c.eps_in;
@@ -1116,27 +1185,137 @@ inline double dissipation_rate(const Case& c, const double x[3]) {
 inline bool has_dissipation(const Case& c) {
   return c.diss == DISS_CONSTANT ? c.eps_ko != 0.0 : !(c.eps_ko == 0.0 && c.eps_in == 0.0);
 }
-inline double interior_radius(const Case& c, const double x[3]) {
-  double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
+inline double interior_radius(const Case& c, double t, const double x[3]) {
+  double cc[3];
+  center_at(c, t, cc);
+  double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
   return std::sqrt(d1 * d1 + d2 * d2 + d3 * d3);
 }
-// core_position (interior.jl:615-625), static center.
-inline void core_position(const Case& c, const double x[3], double out[3]) {
+// core_position (interior.jl:615-625) about c(t).
+inline void core_position(const Case& c, double t, const double x[3], double out[3]) {
   if (c.interior == INT_NONE) { out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; return; }
-  double d[3] = {x[0] - c.center[0], x[1] - c.center[1], x[2] - c.center[2]};
+  double cc[3];
+  center_at(c, t, cc);
+  double d[3] = {x[0] - cc[0], x[1] - cc[1], x[2] - cc[2]};
   double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   if (r >= c.r0) { out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; return; }
   double n[3] = {0, 0, 1};
   if (r != 0) { n[0] = d[0] / r; n[1] = d[1] / r; n[2] = d[2] / r; }
-  for (int i = 0; i < 3; ++i) out[i] = c.center[i] + c.r0 * n[i];
+  for (int i = 0; i < 3; ++i) out[i] = cc[i] + c.r0 * n[i];
 }
 // case_state_tuple (initialdata.jl:1795-1796).
 inline void case_state(const Case& c, double t, const double x[3], double out[20]) {
   double xc[3];
-  core_position(c, x, xc);
+  core_position(c, t, x, xc);
   V10<double> h, P, dh[3];
   background_state(c.bg, t, xc, h, P, dh);
   for (int n = 0; n < NC; ++n) { out[n] = h[n]; out[NC + n] = P[n]; }
+}
+
+// background_state_rate (initialdata.jl:1697-1729): the state at (t, x), its
+// d_t h from the same dual pass, and d_t Pi by the centered difference of Pi
+// at t +- delta divided by the difference of the two times as rounded.
+inline void background_state_rate(const BgParams& p, double t, const double x[3], double delta,
+                                  V10<double>& h, V10<double>& P, V10<double>& dth, V10<double>& dtP) {
+  V10<double> dh[3], Pp, Pm, hx;
+  background_state(p, t, x, h, P, dh, &dth);
+  const double tp = t + delta, tm = t - delta;
+  background_state(p, tp, x, hx, Pp, dh);
+  background_state(p, tm, x, hx, Pm, dh);
+  for (int n = 0; n < NC; ++n) dtP[n] = (Pp[n] - Pm[n]) / (tp - tm);
+}
+// core_rate_weight (interior.jl:539-548): smoothstep(4 r / r_0 - 1).
+inline double core_rate_weight(const Case& c, double r) { return smoothstep(4.0 * r / c.r0 - 1.0); }
+// core_target_rate (initialdata.jl:1731-1764): the rate of the core rule's
+// target t -> u_exact(t, c(t) + r_0 n(t)) at x, by the centered difference at
+// t +- delta, weighted by core_rate_weight; zero (no pass) where that is zero.
+inline void core_target_rate(const Case& c, double t, const double x[3], V10<double>& dth, V10<double>& dtP) {
+  const double s = core_rate_weight(c, interior_radius(c, t, x));
+  for (int n = 0; n < NC; ++n) dth[n] = dtP[n] = 0.0;
+  if (s == 0.0) return;
+  const double tp = t + c.delta, tm = t - c.delta;
+  double xp[3], xm[3];
+  core_position(c, tp, x, xp);
+  core_position(c, tm, x, xm);
+  V10<double> hp, Pp, hm, Pm, dh[3];
+  background_state(c.bg, tp, xp, hp, Pp, dh);
+  background_state(c.bg, tm, xm, hm, Pm, dh);
+  const double f = s / (tp - tm);
+  for (int n = 0; n < NC; ++n) {
+    dth[n] = f * (hp[n] - hm[n]);
+    dtP[n] = f * (Pp[n] - Pm[n]);
+  }
+}
+// _adm_split (interior.jl:505-515): A = alpha^2 = (1 - h_tt) + beta^i beta_i,
+// beta_i = h_ti, beta^i = gamma^-1 beta_i, gamma_ij = delta_ij + h_ij.
+inline void inv3(const double m[3][3], double o[3][3]) {
+  const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1], c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2],
+               c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+  o[0][0] = c00 / det;
+  o[1][0] = c01 / det;
+  o[2][0] = c02 / det;
+  o[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det;
+  o[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
+  o[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det;
+  o[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
+  o[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det;
+  o[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
+}
+inline void mv3(const double m[3][3], const double v[3], double o[3]) {
+  for (int i = 0; i < 3; ++i) o[i] = m[i][0] * v[0] + m[i][1] * v[1] + m[i][2] * v[2];
+}
+inline void adm_split_rate(const V10<double>& h, double& A, double bu[3], double bl[3], double g[3][3]) {
+  g[0][0] = 1.0 + h[4]; g[0][1] = h[5]; g[0][2] = h[6];
+  g[1][0] = h[5]; g[1][1] = 1.0 + h[7]; g[1][2] = h[8];
+  g[2][0] = h[6]; g[2][1] = h[8]; g[2][2] = 1.0 + h[9];
+  bl[0] = h[1]; bl[1] = h[2]; bl[2] = h[3];
+  double gi[3][3];
+  inv3(g, gi);
+  mv3(gi, bl, bu);
+  A = (1.0 - h[0]) + (bu[0] * bl[0] + bu[1] * bl[1] + bu[2] * bl[2]);
+}
+// core_relaxation_rate (interior.jl:450-503): the moving core's metric relaxed
+// toward h_t at rate rho in the ADM variables (alpha^2, beta^i, gamma_ij),
+// plus the target's own ADM rate read off d_t h_t at h_t, mapped back to h by
+// the chain rule of g_tt = -alpha^2 + gamma_ij beta^i beta^j, g_ti = gamma_ij beta^j.
+inline V10<double> core_relaxation_rate(const V10<double>& h, const V10<double>& ht, double rho,
+                                        const V10<double>& dht) {
+  double A, bu[3], bl[3], g[3][3], At, but[3], blt[3], gt[3][3];
+  adm_split_rate(h, A, bu, bl, g);
+  adm_split_rate(ht, At, but, blt, gt);
+  const double dgt[3][3] = {{dht[4], dht[5], dht[6]}, {dht[5], dht[7], dht[8]}, {dht[6], dht[8], dht[9]}};
+  double gti[3][3], dgbt[3], rhs[3], dbt[3];
+  inv3(gt, gti);
+  mv3(dgt, but, dgbt);
+  for (int i = 0; i < 3; ++i) rhs[i] = dht[1 + i] - dgbt[i];
+  mv3(gti, rhs, dbt);
+  const double dAt = -dht[0] + (but[0] * dgbt[0] + but[1] * dgbt[1] + but[2] * dgbt[2]) +
+                     2.0 * (blt[0] * dbt[0] + blt[1] * dbt[1] + blt[2] * dbt[2]);
+  const double dA = dAt - rho * (A - At);
+  double db[3];
+  for (int i = 0; i < 3; ++i) db[i] = dbt[i] - rho * (bu[i] - but[i]);
+  const double dd[3][3] = {{h[4] - ht[4], h[5] - ht[5], h[6] - ht[6]},
+                           {h[5] - ht[5], h[7] - ht[7], h[8] - ht[8]},
+                           {h[6] - ht[6], h[8] - ht[8], h[9] - ht[9]}};
+  double dg[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) dg[i][j] = dgt[i][j] - rho * dd[i][j];
+  double dgb[3], gdb[3];
+  mv3(dg, bu, dgb);
+  mv3(g, db, gdb);
+  const double dtt = -dA + (bu[0] * dgb[0] + bu[1] * dgb[1] + bu[2] * dgb[2]) +
+                     2.0 * (bl[0] * db[0] + bl[1] * db[1] + bl[2] * db[2]);
+  V10<double> o;
+  o[0] = dtt;
+  for (int i = 0; i < 3; ++i) o[1 + i] = dgb[i] + gdb[i];
+  o[4] = dg[0][0];
+  o[5] = dg[1][0];
+  o[6] = dg[2][0];
+  o[7] = dg[1][1];
+  o[8] = dg[2][1];
+  o[9] = dg[2][2];
+  return o;
 }
 
 // SplitMix64 point draw (initialdata.jl:2004-2013).

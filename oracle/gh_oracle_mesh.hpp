@@ -122,7 +122,7 @@ struct Problem {
           for (long i = 0; i <= m.N; ++i) {
             double x[3], xc[3], Hl[4], dHl[4][4];
             m.position(b, i, j, k, x);
-            core_position(cs, x, xc);
+            core_position(cs, t, x, xc);
             gauge_source_grad(cs.bg, t, xc[0], xc[1], xc[2], Hl, dHl);
             for (int a = 0; a < 4; ++a) Hsrc[m.sidx(b, a, i, j, k)] = Hl[a];
             // column-major dHl[c][a] (gauge.jl: entries 5:20 in SMatrix order)
@@ -297,19 +297,38 @@ struct Problem {
     }
   }
 
-  // gh_rhs_point! (evolution.jl:448-695) for the static interior variants.
+  // gh_rhs_point! (evolution.jl:448-695) for the sphere's variants, at rest
+  // or moving (:damped follows a moving hole, :526-583, :652-680).
   void rhs_point(long b, long i, long j, long k, double t, double* du) const {
     const Mesh& m = mesh;
     double x[3];
     m.position(b, i, j, k, x);
-    double g0 = damping_rate(cs, x);
-    double epsh = dissipation_rate(cs, x) * (1.0 / m.h);  // evolution.jl:478
+    double g0 = damping_rate(cs, t, x);
+    double epsh = dissipation_rate(cs, t, x) * (1.0 / m.h);  // evolution.jl:478
     bool diss = has_dissipation(cs);
     V10<double> dth, dtP;
     if (cs.interior != INT_NONE) {
-      double r = interior_radius(cs, x);
+      double r = interior_radius(cs, t, x);
       bool frozen = cs.interior == INT_PASTED ? r < cs.r1 : r < cs.r0;
       if (frozen) {
+        if (core_follows(cs)) {
+          // the moving :damped core relaxes toward the layer's target at the
+          // core rule's point, the metric in the ADM variables
+          const double rc = core_rate(cs);
+          double xc[3];
+          core_position(cs, t, x, xc);
+          V10<double> hd, Pd, dhd[3], dthd, dtPd, hu;
+          background_state(cs.bg, t, xc, hd, Pd, dhd);
+          core_target_rate(cs, t, x, dthd, dtPd);
+          long var = m.widx(b, 0, i, j, k), sv = m.S * m.S * m.S;
+          for (int v = 0; v < NC; ++v) hu[v] = work[var + v * sv];
+          V10<double> dh = core_relaxation_rate(hu, hd, rc, dthd);
+          for (int v = 0; v < NC; ++v) {
+            du[m.sidx(b, v, i, j, k)] = dh[v];
+            du[m.sidx(b, NC + v, i, j, k)] = dtPd[v] - rc * (work[var + (NC + v) * sv] - Pd[v]);
+          }
+          return;
+        }
         for (int v = 0; v < 20; ++v) du[m.sidx(b, v, i, j, k)] = 0.0;
         return;
       }
@@ -324,9 +343,21 @@ struct Problem {
       double w, rho;
       layer_profiles(r, cs.r0, cs.r1, cs.rho_max, cs.w_ramp, cs.rho_ramp, w, rho);
       if (cs.interior == INT_FROZEN) rho = 0.0;
+      long var = m.widx(b, 0, i, j, k), sv = m.S * m.S * m.S;
+      if (core_follows(cs)) {
+        // the moving layer: (1 - w) d_t u_exact keeps the moving solution steady
+        V10<double> hm, Pm, dthm, dtPm;
+        background_state_rate(cs.bg, t, x, cs.delta, hm, Pm, dthm, dtPm);
+        const double om = 1.0 - w;
+        for (int v = 0; v < NC; ++v) {
+          du[m.sidx(b, v, i, j, k)] = w * dth[v] + om * dthm[v] - rho * (work[var + v * sv] - hm[v]);
+          du[m.sidx(b, NC + v, i, j, k)] =
+              w * dtP[v] + om * dtPm[v] - rho * (work[var + (NC + v) * sv] - Pm[v]);
+        }
+        return;
+      }
       V10<double> he, Pe, dhe[3];
       background_state(cs.bg, t, x, he, Pe, dhe);
-      long var = m.widx(b, 0, i, j, k), sv = m.S * m.S * m.S;
       for (int v = 0; v < NC; ++v) {
         du[m.sidx(b, v, i, j, k)] = w * dth[v] - rho * (work[var + v * sv] - he[v]);
         du[m.sidx(b, NC + v, i, j, k)] = w * dtP[v] - rho * (work[var + (NC + v) * sv] - Pe[v]);
@@ -363,7 +394,7 @@ struct Problem {
     const Mesh& m = mesh;
     double x[3];
     m.position(b, i, j, k, x);
-    double g0 = damping_rate(cs, x);
+    double g0 = damping_rate(cs, t, x);
     double hsp = m.h;
     auto at = [&](int v, long ii, long jj, long kk) { return work[m.widx(b, v, ii, jj, kk)]; };
     auto apply = [&](const std::vector<double>& w, int v, int d) {
@@ -426,7 +457,7 @@ struct Problem {
       ks_algebraic_source<double>(cs.bg.M, cs.ks_S, cs.ks_u, hv, dth0, dh, Hl, dHl);
     }
     gh_node_rhs_expanded(hv, Pv, dh, dP, ddh, Hl, dHl, g0, cs.g2, dth, dtP);
-    const double eps = dissipation_rate(cs, x);
+    const double eps = dissipation_rate(cs, t, x);
     const bool diss = has_dissipation(cs);
     if (diss || cs.gauge == GAUGE_KS) {
       V10<double> Qh, QP;
@@ -479,12 +510,12 @@ struct Problem {
   }
 
   // ---- the time step and the analysis reductions ----
-  bool is_evolved(const double x[3]) const {
+  bool is_evolved(double t, const double x[3]) const {
     if (cs.interior == INT_NONE) return true;
-    return interior_radius(cs, x) >= cs.r1;
+    return interior_radius(cs, t, x) >= cs.r1;
   }
   // max_speed (evolution.jl:932-949,1539-1560) over evolved owned points.
-  double max_speed(const double* u) const {
+  double max_speed(const double* u, double t = 0.0) const {
     const Mesh& m = mesh;
     double lam = 0;
     for (long b = 0; b < m.nblocks(); ++b)
@@ -493,7 +524,7 @@ struct Problem {
           for (long i = 0; i <= m.N; ++i) {
             double x[3];
             m.position(b, i, j, k, x);
-            if (!is_evolved(x)) continue;
+            if (!is_evolved(t, x)) continue;
             V10<double> hv;
             for (int v = 0; v < NC; ++v) hv[v] = u[m.sidx(b, v, i, j, k)];
             MQ<double> mq = metric_quantities(hv);
@@ -516,7 +547,7 @@ struct Problem {
           for (long i = 0; i <= m.N; ++i) {
             double x[3], s[20];
             m.position(b, i, j, k, x);
-            if (!is_evolved(x)) continue;
+            if (!is_evolved(t, x)) continue;
             case_state(cs, t, x, s);
             double e2 = 0;
             for (int v = 0; v < 20; ++v) {
@@ -534,7 +565,7 @@ struct Problem {
   // apply_bounds! (bounds.jl:613-660, 733-770): the range projection at every
   // owned point with r < r_gate, written back only where it fired; returns
   // the number of points moved (a no-op without bounds).
-  long apply_bounds(double* u) const {
+  long apply_bounds(double* u, double t = 0.0) const {
     if (!bounds.on) return 0;
     const Mesh& m = mesh;
     long hits = 0;
@@ -544,7 +575,7 @@ struct Problem {
           for (long i = 0; i <= m.N; ++i) {
             double x[3];
             m.position(b, i, j, k, x);
-            if (!(interior_radius(cs, x) < bounds.r_gate)) continue;
+            if (!(interior_radius(cs, t, x) < bounds.r_gate)) continue;
             double h[10], P[10], ho[10], Po[10];
             for (int v = 0; v < 10; ++v) {
               h[v] = u[m.sidx(b, v, i, j, k)];
@@ -565,7 +596,7 @@ struct Problem {
   // gh_limiter! (stepping.jl, bounds.jl:780-800, evolution.jl:888-905): the
   // range projection, then the :pasted overwrite, which has the last word.
   void stage_limit(double* u, double t) const {
-    apply_bounds(u);
+    apply_bounds(u, t);
     paste(u, t);
   }
   void paste(double* u, double t) const {
@@ -577,7 +608,7 @@ struct Problem {
           for (long i = 0; i <= m.N; ++i) {
             double x[3], s[20];
             m.position(b, i, j, k, x);
-            if (interior_radius(cs, x) >= cs.r1) continue;
+            if (interior_radius(cs, t, x) >= cs.r1) continue;
             case_state(cs, t, x, s);
             for (int v = 0; v < 20; ++v) u[m.sidx(b, v, i, j, k)] = s[v];
           }

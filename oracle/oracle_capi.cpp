@@ -4,6 +4,9 @@
 // the same case to the oracle and to the library.
 #include <omp.h>
 
+#include <cfloat>
+#include <cmath>
+
 #include "../include/tgh.h"
 #include "gh_oracle_mesh.hpp"
 
@@ -16,6 +19,7 @@ BgParams bg_from(const tgh_desc* d) {
   p.M = d->M;
   p.a = d->a;
   for (int i = 0; i < 3; ++i) p.c[i] = d->center[i];
+  for (int i = 0; i < 3; ++i) p.u[i] = -d->velocity[i];  // boost(bg, -v): the hole moves at v
   if (d->background == TGH_BG_GAUGE_WAVE) {
     p.A = d->wave_amp;
     p.d = d->wave_len;
@@ -29,8 +33,19 @@ Case case_from(const tgh_desc* d) {
   Case c;
   c.bg = bg_from(d);
   c.gauge = d->gauge;
-  c.ks_S[0] = 0; c.ks_S[1] = 0; c.ks_S[2] = 0; c.ks_S[3] = d->a;  // static hole, spin along z
-  c.ks_u[0] = 1; c.ks_u[1] = 0; c.ks_u[2] = 0; c.ks_u[3] = 0;
+  {
+    // KerrSchildSource(T; M, spin = (0, 0, a), velocity = v) (gauge.jl:263-278):
+    // u = gamma (1, v), S = (gamma v.a, a + (gamma - 1)(v.a)/v^2 v)
+    const double* v = d->velocity;
+    const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double gam = 1.0 / std::sqrt(1.0 - v2);
+    const double av[3] = {0.0, 0.0, d->a};
+    const double va = v[0] * av[0] + v[1] * av[1] + v[2] * av[2];
+    c.ks_S[0] = gam * va;
+    for (int i = 0; i < 3; ++i) c.ks_S[1 + i] = v2 == 0.0 ? av[i] : av[i] + ((gam - 1.0) * va / v2) * v[i];
+    c.ks_u[0] = gam;
+    for (int i = 0; i < 3; ++i) c.ks_u[1 + i] = gam * v[i];
+  }
   c.damping = d->damping;
   c.g0 = d->gamma0;
   c.g0_near = d->gamma0_near;
@@ -44,7 +59,9 @@ Case case_from(const tgh_desc* d) {
   c.eps_rh = d->eps_rh;
   c.interior = d->interior;
   for (int i = 0; i < 3; ++i) c.center[i] = d->center[i];
+  for (int i = 0; i < 3; ++i) c.vel[i] = d->velocity[i];
   c.r0 = d->r0;
+  c.delta = std::cbrt(DBL_EPSILON) * d->r0;  // layer_rate_step (interior.jl:524-537)
   c.r1 = d->r1;
   c.rho_max = d->rho_max;
   c.w_ramp = d->w_ramp;
@@ -232,6 +249,32 @@ void orc_case_state(const tgh_desc* desc, double t, const double* x, double* out
   Case c = case_from(desc);
   case_state(c, t, x, out);
 }
+void orc_background_state_rate(const tgh_desc* desc, double t, const double* x, double* h, double* P,
+                               double* dth, double* dtP) {
+  Case c = case_from(desc);
+  V10<double> a, b, e, f;
+  background_state_rate(c.bg, t, x, c.delta, a, b, e, f);
+  for (int n = 0; n < NC; ++n) {
+    h[n] = a[n];
+    P[n] = b[n];
+    dth[n] = e[n];
+    dtP[n] = f[n];
+  }
+}
+void orc_core_target_rate(const tgh_desc* desc, double t, const double* x, double* dth, double* dtP) {
+  Case c = case_from(desc);
+  V10<double> e, f;
+  core_target_rate(c, t, x, e, f);
+  for (int n = 0; n < NC; ++n) {
+    dth[n] = e[n];
+    dtP[n] = f[n];
+  }
+}
+void orc_core_relaxation_rate(const double* h, const double* ht, double rho, const double* dht, double* out) {
+  V10<double> o = core_relaxation_rate(v10(h), v10(ht), rho, v10(dht));
+  for (int n = 0; n < NC; ++n) out[n] = o[n];
+}
+double orc_layer_rate_step(const tgh_desc* desc) { return case_from(desc).delta; }
 double orc_point_draw(uint64_t base, const double* x, int v) { return point_draw(base, x, v); }
 double orc_smoothstep(double s) { return smoothstep(s); }
 
@@ -285,7 +328,7 @@ void orc_rk4_step(void* p, double* u, double t, double dt, int nthreads) {
 }
 void orc_settle(void* p, double* u, double t) { ((Problem*)p)->settle(u, t); }
 void orc_stage_limit(void* p, double* u, double t) { ((Problem*)p)->stage_limit(u, t); }
-long orc_apply_bounds(void* p, double* u) { return ((Problem*)p)->apply_bounds(u); }
+long orc_apply_bounds(void* p, double* u, double t) { return ((Problem*)p)->apply_bounds(u, t); }
 
 // bounds_project at one point (bounds.jl:510-527), for the tests: h, P in,
 // ho, Po out; returns hit | nonfinite << 1.
@@ -312,7 +355,7 @@ void orc_sym_eigen3(const double* A, double* lam, double* V) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) V[3 * i + j] = v[i][j];
 }
-double orc_max_speed(void* p, const double* u) { return ((Problem*)p)->max_speed(u); }
+double orc_max_speed(void* p, const double* u, double t) { return ((Problem*)p)->max_speed(u, t); }
 void orc_error_norms(void* p, const double* u, double t, double* l2, double* linf) {
   ((Problem*)p)->error_norms(u, t, *l2, *linf);
 }
@@ -329,7 +372,7 @@ double orc_count_rhs_flops(void* pp, long b, long i, long j, long k, int head_on
   p->mesh.position(b, i, j, k, x);
   Cnt::ops = 0;
   V10<Cnt> dth, dtP;
-  Cnt g0(damping_rate(p->cs, x)), g2(p->cs.g2), epsh(p->cs.eps_ko / p->mesh.h);
+  Cnt g0(damping_rate(p->cs, 0.0, x)), g2(p->cs.g2), epsh(p->cs.eps_ko / p->mesh.h);
   p->rhs_point_F<Cnt>(b, i, j, k, 0.0, g0, g2, epsh, p->cs.eps_ko != 0.0, dth, dtP, head_only != 0);
   return (double)Cnt::ops;
 }

@@ -68,10 +68,10 @@ def lib():
             "orc_rk4_step": (None, [P, dp, D, D, C.c_int]),
             "orc_settle": (None, [P, dp, D]),
             "orc_stage_limit": (None, [P, dp, D]),
-            "orc_apply_bounds": (C.c_long, [P, dp]),
+            "orc_apply_bounds": (C.c_long, [P, dp, D]),
             "orc_bounds_project": (C.c_int, [dp, dp, dp, dp, dp]),
             "orc_sym_eigen3": (None, [dp, dp, dp]),
-            "orc_max_speed": (D, [P, dp]),
+            "orc_max_speed": (D, [P, dp, D]),
             "orc_error_norms": (None, [P, dp, D, C.POINTER(D), C.POINTER(D)]),
             "orc_is_primary": (C.c_int, [P, Lg, Lg, Lg, Lg]),
             "orc_count_rhs_flops": (D, [P, Lg, Lg, Lg, Lg, C.c_int]),
@@ -162,11 +162,11 @@ class OracleProblem:
     def stage_limit(self, u, t):
         lib().orc_stage_limit(self.p, u, t)
 
-    def apply_bounds(self, u):
-        return lib().orc_apply_bounds(self.p, u)
+    def apply_bounds(self, u, t=0.0):
+        return lib().orc_apply_bounds(self.p, u, t)
 
-    def max_speed(self, u):
-        return lib().orc_max_speed(self.p, u)
+    def max_speed(self, u, t=0.0):
+        return lib().orc_max_speed(self.p, u, t)
 
     def error_norms(self, u, t):
         l2 = C.c_double()

@@ -113,3 +113,13 @@ def test_per_block_form_equals_fused_block_form(tgh):
     assert torch.equal(u, v), float((u - v).abs().max())
     assert L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, 1,
                      tgh.TGH_RK4_PER_BLOCK | tgh.TGH_RK4_SPLIT) == 1
+
+
+def test_batch_of_moving_holes_steps_each_as_alone(tgh):
+    """Moving harmonic holes (translate(boost(Harmonic, -v), x0)): time-dependent
+    frames and tables, the moving core and layer, in one launch per kernel."""
+    probs = []
+    for a, roots, v in ((0.0, 3, 0.3), (0.3, 4, -0.2)):
+        case = tgh.harmonic_kerr_case(M=1.0, a=a, v=v, x0=0.25, axis=1, halfwidth=roots / 2, r_0=0.35, r_1=0.9)
+        probs.append(tgh.GHProblem(tgh.gh_forest(None, case, N=16, roots=roots), case, q=4))
+    check_batch_equals_alone(tgh, probs, 2, True)

@@ -26,6 +26,7 @@ class TghDesc(C.Structure):
         ("eps_in", C.c_double), ("eps_r1", C.c_double), ("eps_rh", C.c_double),
         ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("lambda_min", C.c_double),
         ("lambda_max", C.c_double), ("beta_max", C.c_double), ("K_max", C.c_double), ("r_gate", C.c_double),
+        ("velocity", C.c_double * 3),
     ]
 
 
@@ -35,7 +36,7 @@ def make_desc(*, q=4, N=16, nblocks=(1, 1, 1), periodic=(1, 1, 1), lo=(0.0, 0.0,
               gamma0_width=3.0, gamma2=0.0, eps_ko=0.0, interior=TGH_INTERIOR_NONE,
               M=1.0, a=0.0, center=(0.0, 0.0, 0.0), wave_amp=0.05, wave_len=1.0,
               r0=0.0, r1=0.0, rho_max=0.0, w_ramp=0.5, rho_ramp=0.5, dissipation=0, eps_in=0.0,
-              eps_r1=0.0, eps_rh=0.0):
+              eps_r1=0.0, eps_rh=0.0, velocity=(0.0, 0.0, 0.0)):
     d = TghDesc()
     d.float_bits = 64
     d.q, d.N = q, N
@@ -51,6 +52,7 @@ def make_desc(*, q=4, N=16, nblocks=(1, 1, 1), periodic=(1, 1, 1), lo=(0.0, 0.0,
     d.gamma2, d.eps_ko = gamma2, eps_ko
     d.r0, d.r1, d.rho_max, d.w_ramp, d.rho_ramp = r0, r1, rho_max, w_ramp, rho_ramp
     d.dissipation, d.eps_in, d.eps_r1, d.eps_rh = dissipation, eps_in, eps_r1, eps_rh
+    d.velocity[:] = list(velocity)
     return d
 
 
@@ -88,3 +90,22 @@ def harmonic_hole_case(*, N=16, nblocks=4, halfwidth=2.0, a=0.3, M=1.0, q=4, r0=
                      background=TGH_BG_HARMONIC, gauge=TGH_GAUGE_NONE, damping=TGH_DAMPING_GAUSSIAN,
                      gamma0_near=1.0 / M, gamma0_far=1.0 / (10 * M), gamma0_width=3.0 * M, gamma2=0.0,
                      eps_ko=0.5, interior=TGH_INTERIOR_DAMPED, M=M, a=a, r0=r0, r1=r1, rho_max=rho_max)
+
+
+def moving_hole_case(*, background=TGH_BG_HARMONIC, a=0.3, v=0.3, x0=0.25, axis=1, N=16, nblocks=4, halfwidth=2.0,
+                     M=1.0, q=4, r0=None, r1=None, rho_max=4.0, interior=TGH_INTERIOR_DAMPED):
+    """moving_hole_case (src/initialdata.jl:1283-1298): the hole at x0 + v t along axis,
+    translate(boost(bg, -v), x0); the Kerr-Schild one with its moving closed-form source."""
+    L = halfwidth
+    harm = background == TGH_BG_HARMONIC
+    r0 = (0.35 if harm else 0.55) if r0 is None else r0
+    r1 = (0.9 if harm else 1.05) if r1 is None else r1
+    c = [0.0, 0.0, 0.0]
+    vel = [0.0, 0.0, 0.0]
+    c[axis - 1] = x0
+    vel[axis - 1] = v
+    return make_desc(q=q, N=N, nblocks=(nblocks,) * 3, periodic=(0, 0, 0), lo=(-L, -L, -L), hi=(L, L, L),
+                     background=background, gauge=TGH_GAUGE_NONE if harm else TGH_GAUGE_KERR_SCHILD,
+                     damping=TGH_DAMPING_GAUSSIAN, gamma0_near=1.0 / M, gamma0_far=1.0 / (10 * M),
+                     gamma0_width=3.0 * M, gamma2=0.0, eps_ko=0.5, interior=interior, M=M, a=a, r0=r0, r1=r1,
+                     rho_max=rho_max, center=tuple(c), velocity=tuple(vel))

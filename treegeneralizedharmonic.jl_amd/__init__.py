@@ -76,6 +76,7 @@ class TghDesc(C.Structure):
         ("eps_in", C.c_double), ("eps_r1", C.c_double), ("eps_rh", C.c_double),
         ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("lambda_min", C.c_double),
         ("lambda_max", C.c_double), ("beta_max", C.c_double), ("K_max", C.c_double), ("r_gate", C.c_double),
+        ("velocity", C.c_double * 3),
     ]
 
 
@@ -345,8 +346,11 @@ class GHCase:
     backgrounds) or "algebraic" (the closed-form Kerr-Schild source)."""
 
     def __init__(self, T, background, *, box, periodic, eps_ko, gamma0, gamma2,
-                 interior=None, gauge_source=None, bounds=None):
+                 interior=None, gauge_source=None, bounds=None, velocity=(0.0, 0.0, 0.0)):
         _check_float(T)
+        self.velocity = tuple(float(x) for x in velocity)
+        if any(self.velocity) and not isinstance(background, (KerrSchild, Harmonic)):
+            raise ArgumentError("only a Kerr-Schild or harmonic hole moves")
         if bounds is not None and not isinstance(bounds, StateBounds):
             raise ArgumentError("bounds is None or a StateBounds")
         if bounds is not None and interior is None:
@@ -389,37 +393,58 @@ def gauge_wave_case(T=None, *, A=1 / 20, d=1.0, eps_ko, gamma0, gamma2):
                   gamma0=gamma0, gamma2=gamma2)
 
 
-def kerr_schild_case(T=None, *, M=1.0, a=0.0, halfwidth, r_0, r_1, rho_max=None, eps_ko=0.5,
-                     gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
+def _moving_hole(v, x0, axis):
+    """moving_hole_case's placement (src/initialdata.jl:1283-1298): the hole at
+    x0 + v t along axis (1, 2 or 3), the background translate(boost(bg, −v ê), x0 ê)."""
+    if axis not in (1, 2, 3):
+        raise ArgumentError(f"axis is the coordinate the hole sits and moves on, 1, 2 or 3, got {axis}.")
+    v, x0 = float(v), float(x0)
+    if not abs(v) < 1:
+        raise ArgumentError(f"the hole's velocity must be below the speed of light, got |v| = {abs(v)}")
+    on_axis = lambda s: tuple(s if d == axis - 1 else 0.0 for d in range(3))  # noqa: E731
+    return on_axis(x0), on_axis(v)
+
+
+def kerr_schild_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, r_0, r_1, rho_max=None,
+                     eps_ko=0.5, gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
                      gauge_source=None, bounds=None):
-    """kerr_schild_case / hole_case (src/initialdata.jl:1044-1174,1274-1281) for a
-    hole at rest at the origin in a Dirichlet box. ρ_max defaults to the
+    """kerr_schild_case / hole_case (src/initialdata.jl:1044-1174,1230-1298) for a
+    hole in a Dirichlet box, at x0 + v t along axis. ρ_max defaults to the
     physical rate 4/M the driver sets per chunk. gauge_source=None samples
-    H_a = −Γ_a of the background once (the reference's default); "algebraic" is
-    the closed-form KerrSchildSource evaluated in the kernel."""
+    H_a = −Γ_a of the background once (the reference's default, a hole at rest
+    only); "algebraic" is the closed-form KerrSchildSource evaluated in the
+    kernel, which moves with the hole."""
+    center, vel = _moving_hole(v, x0, axis)
+    if any(vel) and gauge_source is None:
+        raise ArgumentError("this background is neither harmonic nor static, so its gauge source depends on "
+                            "time and the sampled source cannot represent it: give a moving Kerr-Schild hole "
+                            "gauge_source=\"algebraic\" (the KerrSchildSource that moves with it), or use "
+                            "harmonic_kerr_case.")
     damping = gamma0 if gamma0 is not None else GaussianDamping(near=1 / M, far=1 / (10 * M),
-                                                                width=3 * M)
-    inter = Interior(T, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
+                                                                width=3 * M, center=center)
+    inter = Interior(T, center=center, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
                      variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp)
     L = float(halfwidth)
     return GHCase(T, KerrSchild(M, a), box=[(-L, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
                   gamma0=damping, gamma2=gamma2, interior=inter, gauge_source=gauge_source,
-                  bounds=bounds)
+                  bounds=bounds, velocity=vel)
 
 
-def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, halfwidth, r_0, r_1, rho_max=None, eps_ko=0.5,
-                       gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
+def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, r_0, r_1, rho_max=None,
+                       eps_ko=0.5, gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
                        bounds=None):
-    """harmonic_kerr_case / hole_case (src/initialdata.jl:1300-1326, 1044-1174) for a
-    hole at rest at the origin: Harmonic(M, a) in a Dirichlet box, with no gauge
-    source (the chart is harmonic). Defaults as kerr_schild_case's."""
+    """harmonic_kerr_case / hole_case (src/initialdata.jl:1300-1326, 1044-1174):
+    Harmonic(M, a) in a Dirichlet box, at x0 + v t along axis, with no gauge
+    source (a boost of the harmonic chart is harmonic). Defaults as
+    kerr_schild_case's."""
+    center, vel = _moving_hole(v, x0, axis)
     damping = gamma0 if gamma0 is not None else GaussianDamping(near=1 / M, far=1 / (10 * M),
-                                                                width=3 * M)
-    inter = Interior(T, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
+                                                                width=3 * M, center=center)
+    inter = Interior(T, center=center, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
                      variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp)
     L = float(halfwidth)
     return GHCase(T, Harmonic(M, a), box=[(-L, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
-                  gamma0=damping, gamma2=gamma2, interior=inter, bounds=bounds)
+                  gamma0=damping, gamma2=gamma2, interior=inter, bounds=bounds, velocity=vel)
 
 
 class Forest:
@@ -472,6 +497,7 @@ def _desc(forest, case, q):
         (d.alpha_min, d.alpha_max, d.lambda_min, d.lambda_max, d.beta_max, d.K_max,
          d.r_gate) = (bd.alpha_min, bd.alpha_max, bd.lambda_min, bd.lambda_max, bd.beta_max, bd.K_max,
                       bd.r_gate)
+    d.velocity[:] = list(getattr(case, "velocity", (0.0, 0.0, 0.0)))
     it = case.interior
     if it is not None:
         d.interior = Interior.VARIANTS[it.variant]

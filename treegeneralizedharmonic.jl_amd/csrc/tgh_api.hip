@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -118,6 +119,30 @@ int check_desc(const tgh_desc* d) {
         !(d->r0 > std::fabs(d->a)))
       return fail(TGH_EINVAL, "the frozen core must contain the chart's singular disk |x| <= |a|, z = 0: "
                               "r_0 > |a|");
+  }
+  {
+    const double* v = d->velocity;
+    if (!(std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2])))
+      return fail(TGH_EINVAL, "the hole's velocity must be finite");
+    const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (v2 != 0.0) {
+      if (!(v2 < 1.0))
+        return fail(TGH_EINVAL, "the hole's velocity must be below the speed of light, got |v| = " +
+                                    std::to_string(std::sqrt(v2)));
+      if (d->background != TGH_BG_KERR_SCHILD && d->background != TGH_BG_HARMONIC)
+        return fail(TGH_EINVAL, "only a Kerr-Schild or harmonic hole moves (translate(boost(bg, -v), x0), "
+                                "src/initialdata.jl:1283-1298)");
+      if (d->interior == TGH_INTERIOR_NONE)
+        return fail(TGH_EINVAL, "a moving hole needs its interior (hole_case)");
+      if (d->interior == TGH_INTERIOR_PASTED)
+        return fail(TGH_EUNSUPPORTED, "the :pasted interior of a moving hole is not built");
+      if (d->gauge == TGH_GAUGE_SAMPLED)
+        return fail(TGH_EINVAL, "a moving Kerr-Schild hole's gauge source depends on time, which the sampled "
+                                "source cannot represent: give it the closed-form KerrSchildSource, which moves "
+                                "with the hole (src/initialdata.jl:216-227)");
+      if (d->r_gate != 0)
+        return fail(TGH_EUNSUPPORTED, "the range projection about a moving hole is not built");
+    }
   }
   if (d->dissipation != TGH_DISSIPATION_CONSTANT && d->dissipation != TGH_DISSIPATION_HORIZON)
     return fail(TGH_EINVAL, "unknown dissipation kind");
@@ -294,8 +319,9 @@ void launch_frame(tgh_problem* p, double* W, double t) {
 }
 
 // A background that does not depend on t (the gauge wave is the one that does).
-bool static_background(int kind) {
-  return kind == BG_MINKOWSKI || kind == BG_KERR_SCHILD || kind == BG_SHIFTED || kind == BG_HARMONIC;
+bool static_background(const CaseP& c) {
+  const int kind = c.bg.kind;
+  return !c.moving && (kind == BG_MINKOWSKI || kind == BG_KERR_SCHILD || kind == BG_SHIFTED || kind == BG_HARMONIC);
 }
 
 int ensure_brick(tgh_problem* p) {
@@ -325,7 +351,7 @@ int ensure_brick(tgh_problem* p) {
                                    " MiB each): " + hipGetErrorString(e));
     HIP_TRY(hipMemsetAsync(*a, 0, bytes, p->stream));
   }
-  const bool static_bg = static_background(p->cs.bg.kind);
+  const bool static_bg = static_background(p->cs);
   p->frame_static = static_bg && !g.periodic[0] && !g.periodic[1] && !g.periodic[2];
   if (p->frame_static) {
     launch_frame(p, p->BW[0], 0.0);
@@ -522,6 +548,27 @@ int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
   c.rho_max = d->rho_max;
   c.w_ramp = d->w_ramp;
   c.rho_ramp = d->rho_ramp;
+  {
+    // the moving hole: translate(boost(bg, -v), center), distances from
+    // center + v t, the core's and the layer's rates (src/interior.jl:390-549)
+    const double* v = d->velocity;
+    c.moving = v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0;
+    for (int i = 0; i < 3; ++i) {
+      c.vel[i] = v[i];
+      c.bg.u[i] = c.moving ? -v[i] : 0.0;
+    }
+    c.delta = std::cbrt(DBL_EPSILON) * d->r0;  // layer_rate_step (src/interior.jl:524-537)
+    // KerrSchildSource(T; M, spin = (0, 0, a), velocity = v) (src/gauge.jl:263-278)
+    const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double gam = 1.0 / std::sqrt(1.0 - v2);
+    const double av[3] = {0.0, 0.0, d->a};
+    const double va = v[0] * av[0] + v[1] * av[1] + v[2] * av[2];
+    c.ks_S[0] = gam * va;
+    for (int i = 0; i < 3; ++i) c.ks_S[1 + i] = v2 == 0.0 ? av[i] : av[i] + ((gam - 1.0) * va / v2) * v[i];
+    c.ks_u[0] = gam;
+    for (int i = 0; i < 3; ++i) c.ks_u[1 + i] = gam * v[i];
+    if (c.moving) c.gauge = c.gauge == GAUGE_KS ? GAUGE_KS_MOVING : GAUGE_NONE_MOVING;
+  }
 
   const long P3 = (long)g.P * g.P * g.P, S3 = (long)g.S * g.S * g.S;
   p->state_len = P3 * 20 * g.nblocks;
@@ -589,7 +636,6 @@ int tgh_problem_destroy(tgh_problem* p) {
   (void)hipFree(p->Bacc);
   (void)hipFree(p->BH);
   (void)hipFree(p->Bcoef);
-  (void)hipFree(p->Bd);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   delete p;
@@ -768,8 +814,6 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     if (form == 3 && !p->Bcoef) {
       HIP_TRY(hipMalloc(&p->Bcoef, sizeof(double) * CF_N * (size_t)p->brick.fs));
       HIP_TRY(hipMemsetAsync(p->Bcoef, 0, sizeof(double) * CF_N * (size_t)p->brick.fs, p->stream));
-      HIP_TRY(hipMalloc(&p->Bd, sizeof(double) * 4 * NC * (size_t)p->brick.fs));
-      HIP_TRY(hipMemsetAsync(p->Bd, 0, sizeof(double) * 4 * NC * (size_t)p->brick.fs, p->stream));
     }
     // u_n into the stepper's u_n and first stage input
     BrickCopyArgs ca{p->geom, p->brick, u, p->BW[0], p->Bus, 0};
@@ -805,7 +849,7 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     // one (u, dt, t) triple; the time only matters to the Dirichlet hook of a
     // moving background, so with a static one the graph is replayed across
     // steps and re-captured only when dt, u or the form change.
-    const bool static_bg = static_background(p->cs.bg.kind);
+    const bool static_bg = static_background(p->cs);
     for (int n = 0; n < nsteps; ++n) {
       const double t = t0 + n * dt;
       const bool reuse = p->graph_exec && p->graph_key[0] == dt &&

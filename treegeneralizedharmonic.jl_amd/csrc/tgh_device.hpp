@@ -302,6 +302,37 @@ TGH_HD void ks_source(double M, double a, const double h[NC], const double D[4][
   }
 }
 
+// KerrSchildSource with a velocity (src/gauge.jl:257-359): with w_a = k_ab u^b,
+// K = u^a w_a, s = S^a w_a and D = M + sqrt(M^2 - s^2), H_a = -K w_a / D and
+// its gradient, u = gamma (1, v) and S the boosted spin.
+TGH_HD void ks_source_moving(double M, const double S[4], const double u[4], const double h[NC],
+                             const double D[4][NC], double Hl[4], double dH[4][4]) {
+  double w[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    w[q] = h[pidx(q, 0)] * u[0] + h[pidx(q, 1)] * u[1] + h[pidx(q, 2)] * u[2] + h[pidx(q, 3)] * u[3];
+  const double K = u[0] * w[0] + u[1] * w[1] + u[2] * w[2] + u[3] * w[3];
+  const double sw = S[0] * w[0] + S[1] * w[1] + S[2] * w[2] + S[3] * w[3];
+  const double rad = M * M - sw * sw;
+  const double root = rad > 0.0 ? sqrt(rad) : 0.0;
+  const double Dn = M + root;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) Hl[q] = (-K / Dn) * w[q];
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc) {
+    double dw[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      dw[q] = D[cc][pidx(q, 0)] * u[0] + D[cc][pidx(q, 1)] * u[1] + D[cc][pidx(q, 2)] * u[2] +
+              D[cc][pidx(q, 3)] * u[3];
+    const double dK = u[0] * dw[0] + u[1] * dw[1] + u[2] * dw[2] + u[3] * dw[3];
+    const double Sdw = S[0] * dw[0] + S[1] * dw[1] + S[2] * dw[2] + S[3] * dw[3];
+    const double dD = root > 0.0 ? -sw * Sdw / root : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dH[cc][q] = -(dK * w[q] + K * dw[q]) / Dn + K * w[q] * dD / (Dn * Dn);
+  }
+}
+
 // matter_source (src/pointwise.jl:667-698): 16 pi alpha sqrt(gamma) (T_ab - g_ab T/2).
 TGH_HD void matter_source(const Coef& c, const double h[NC], double rho, const double S[3],
                           const double Sij[6], double out[NC]) {
@@ -417,6 +448,7 @@ struct Bg {
   double M, a;
   double c[3];
   double A, len;  // gauge wave A, d; shifted Minkowski A, w
+  double u[3];    // boost velocity of a Kerr-Schild or harmonic hole (it moves at -u)
 };
 
 // Second-order jets (value, gradient and packed Hessian in the four
@@ -637,21 +669,24 @@ TGH_HD NDual<V, K> operator/(double a, const NDual<V, K>& b) { return nd_div(nd_
 // J the map's Jacobian, J^t_i = 2 M / (r - r_-) d_i R. The map is evaluated
 // in duals over (X, Y, Z) whose components are duals over (t, x, y, z), so
 // each entry of J carries its own first derivatives.
-TGH_HD void bg_harmonic_offset(const Bg& bg, const double x[3], Dual<4> hd[NC]) {
-  using S = NDual<double, 4>;
+using NS = NDual<double, 4>;  // a value with its derivatives in (t, x, y, z)
+TGH_HD NS ns_cst(double x) { return nd_cst(x, (const NS*)nullptr); }
+
+// The harmonic chart's full metric g at the hole-relative harmonic position
+// (X, Y, Z), each carried with its derivatives in (t, x, y, z).
+TGH_HD void harmonic_metric_exact(const Bg& bg, const NS& Xs, const NS& Ys, const NS& Zs, NS g[4][4]) {
+  using S = NS;
   using D3 = NDual<S, 3>;
   const double a = bg.a, M = bg.M;
   const double rm = M - sqrt(M * M - a * a);
   const double aa = a * a;
   D3 P[3];
+  const S* in[3] = {&Xs, &Ys, &Zs};
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    S xi = nd_cst(x[i], (const S*)nullptr);
-    xi.d[i + 1] = 1.0;
-    xi = xi - bg.c[i];
     P[i] = nd_cst(0.0, (const D3*)nullptr);
-    P[i].v = xi;
-    P[i].d[i] = nd_cst(1.0, (const S*)nullptr);
+    P[i].v = *in[i];
+    P[i].d[i] = ns_cst(1.0);
   }
   const D3 &X = P[0], &Y = P[1], &Z = P[2];
   D3 rho2 = X * X + Y * Y + Z * Z;
@@ -670,7 +705,7 @@ TGH_HD void bg_harmonic_offset(const Bg& bg, const double x[3], Dual<4> hd[NC]) 
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) J[c][b] = nd_cst(c == 0 && b == 0 ? 1.0 : 0.0, (const S*)nullptr);
+    for (int b = 0; b < 4; ++b) J[c][b] = ns_cst(c == 0 && b == 0 ? 1.0 : 0.0);
   const S tfac = (2.0 * M) / (r.v - rm);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -683,7 +718,7 @@ TGH_HD void bg_harmonic_offset(const Bg& bg, const double x[3], Dual<4> hd[NC]) 
   const S f = (2.0 * M) * rv * r2v / (r2v * r2v + aa * zv * zv);
   const S dl = r2v + aa;
   S l[4];
-  l[0] = nd_cst(1.0, (const S*)nullptr);
+  l[0] = ns_cst(1.0);
   l[1] = (rv * xv + a * yv) / dl;
   l[2] = (rv * yv - a * xv) / dl;
   l[3] = zv / rv;
@@ -691,19 +726,230 @@ TGH_HD void bg_harmonic_offset(const Bg& bg, const double x[3], Dual<4> hd[NC]) 
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) gk[i][j] = nd_cst(eta(i, j), (const S*)nullptr) + f * l[i] * l[j];
+    for (int j = 0; j < 4; ++j) gk[i][j] = ns_cst(eta(i, j)) + f * l[i] * l[j];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      S sum = ns_cst(0.0);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) sum = sum + J[c][i] * J[e][j] * gk[c][e];
+      g[i][j] = sum;
+    }
+}
+
+// The Kerr-Schild metric g = eta + f l l at the hole-relative position, in the
+// same fixed order (the moving hole's chart).
+TGH_HD void ks_metric_exact(const Bg& bg, const NS& X, const NS& Y, const NS& Z, NS g[4][4]) {
+  const double a = bg.a, M = bg.M;
+  NS rho2 = X * X + Y * Y + Z * Z;
+  NS hb = (rho2 - a * a) / 2.0;
+  NS r2 = hb + nd_sqrt(hb * hb + (a * a) * Z * Z);
+  NS r = nd_sqrt(r2);
+  NS f = (2.0 * M) * r * r2 / (r2 * r2 + (a * a) * Z * Z);
+  NS den = r2 + a * a;
+  NS l[4] = {ns_cst(1.0), (r * X + a * Y) / den, (r * Y - a * X) / den, Z / r};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) g[i][j] = ns_cst(eta(i, j)) + f * l[i] * l[j];
+}
+
+TGH_HD void bg_harmonic_offset(const Bg& bg, const double x[3], Dual<4> hd[NC]) {
+  NS P[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    P[i] = ns_cst(x[i]);
+    P[i].d[i + 1] = 1.0;
+    P[i] = P[i] - bg.c[i];
+  }
+  NS g[4][4];
+  harmonic_metric_exact(bg, P[0], P[1], P[2], g);
 #pragma unroll
   for (int n = 0; n < NC; ++n) {
     const int i = prow(n), j = pcol(n);
-    S sum = nd_cst(0.0, (const S*)nullptr);
+    hd[n].v = nd_sub(g[i][j].v, eta(i, j));
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) sum = sum + J[c][i] * J[e][j] * gk[c][e];
-    hd[n].v = nd_sub(sum.v, eta(i, j));
-#pragma unroll
-    for (int k = 0; k < 4; ++k) hd[n].d[k] = sum.d[k];
+    for (int k = 0; k < 4; ++k) hd[n].d[k] = g[i][j].d[k];
   }
+}
+
+// boost(m, u) and translate (SpacetimeMetrics, as the reference states it at
+// src/interior.jl:66-72, src/initialdata.jl:93-96): m evaluated at
+// x' = Lambda (t, x - c), Lambda the symmetric Lorentz boost of velocity u,
+//   Lambda^0_0 = gamma, Lambda^0_i = Lambda^i_0 = gamma u_i,
+//   Lambda^i_j = delta_ij + (gamma - 1) u_i u_j / u^2,
+// and pulled back, g_ab = Lambda^c_a Lambda^d_b m_cd(x'): the hole moves at -u.
+TGH_HD bool bg_boosted(const Bg& bg) {
+  return (bg.kind == BG_KERR_SCHILD || bg.kind == BG_HARMONIC) && (bg.u[0] != 0.0 || bg.u[1] != 0.0 || bg.u[2] != 0.0);
+}
+TGH_HD void lorentz_boost(const double u[3], double L[4][4]) {
+#pragma clang fp contract(off)
+  const double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const double gam = 1.0 / sqrt(1.0 - u2);
+  const double f = (gam - 1.0) / u2;
+  L[0][0] = gam;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    L[0][i + 1] = gam * u[i];
+    L[i + 1][0] = gam * u[i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) L[i + 1][j + 1] = (i == j ? 1.0 : 0.0) + f * u[i] * u[j];
+  }
+}
+
+// The oracle-order 4x4 helpers behind metric_quantities' alpha, beta and
+// sqrt(gamma) for the moving hole's Pi (see bg_state_exact).
+TGH_HD double ex_det3(double a, double b, double c, double d, double e, double f, double g, double h, double i) {
+#pragma clang fp contract(off)
+  return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+}
+TGH_HD double ex_det4(const double m[4][4]) {
+#pragma clang fp contract(off)
+  const double c0 = ex_det3(m[1][1], m[1][2], m[1][3], m[2][1], m[2][2], m[2][3], m[3][1], m[3][2], m[3][3]);
+  const double c1 = ex_det3(m[1][0], m[1][2], m[1][3], m[2][0], m[2][2], m[2][3], m[3][0], m[3][2], m[3][3]);
+  const double c2 = ex_det3(m[1][0], m[1][1], m[1][3], m[2][0], m[2][1], m[2][3], m[3][0], m[3][1], m[3][3]);
+  const double c3 = ex_det3(m[1][0], m[1][1], m[1][2], m[2][0], m[2][1], m[2][2], m[3][0], m[3][1], m[3][2]);
+  return m[0][0] * c0 - m[0][1] * c1 + m[0][2] * c2 - m[0][3] * c3;
+}
+TGH_HD void ex_matmul(const double A[4][4], const double B[4][4], double C[4][4]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double s = A[i][0] * B[0][j];
+#pragma unroll
+      for (int k = 1; k < 4; ++k) s = s + A[i][k] * B[k][j];
+      C[i][j] = s;
+    }
+}
+TGH_HD void ex_inv4(const double m[4][4], double out[4][4]) {
+#pragma clang fp contract(off)
+  double adj[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int r[3], c[3], nr = 0, nc = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k != j) r[nr++] = k;
+        if (k != i) c[nc++] = k;
+      }
+      const double mn = ex_det3(m[r[0]][c[0]], m[r[0]][c[1]], m[r[0]][c[2]], m[r[1]][c[0]], m[r[1]][c[1]],
+                                m[r[1]][c[2]], m[r[2]][c[0]], m[r[2]][c[1]], m[r[2]][c[2]]);
+      adj[i][j] = ((i + j) % 2 == 0) ? mn : -mn;
+    }
+  const double d = ex_det4(m);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[i][j] = adj[i][j] / d;
+}
+// alpha, beta^i and sqrt(gamma) of metric_quantities (src/pointwise.jl:145-184)
+// in the oracle's order: g^{ab} by the offset -(g^-1 h) eta, sqrt(gamma) from
+// the elementary symmetric polynomials of eta h.
+TGH_HD void ex_lapse_shift(const double hv[NC], double& alpha, double beta[3], double& sqrtg) {
+#pragma clang fp contract(off)
+  double h[4][4], e4[4][4], g4[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      h[a][b] = hv[pidx(a, b)];
+      e4[a][b] = eta(a, b);
+      g4[a][b] = eta(a, b) + h[a][b];
+    }
+  double gi[4][4], gih[4][4], tt[4][4];
+  ex_inv4(g4, gi);
+  ex_matmul(gi, h, gih);
+  ex_matmul(gih, e4, tt);
+  const double q1 = -tt[0][0];
+  const double gutt = eta(0, 0) + q1;
+  alpha = 1.0 / sqrt(1.0 - q1);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) beta[i] = -(eta(0, i + 1) + -tt[0][i + 1]) / gutt;
+  double B[4][4], B2[4][4], B3[4][4];
+  ex_matmul(e4, h, B);
+  ex_matmul(B, B, B2);
+  ex_matmul(B2, B, B3);
+  const double trB = B[0][0] + B[1][1] + B[2][2] + B[3][3];
+  const double trB2 = B2[0][0] + B2[1][1] + B2[2][2] + B2[3][3];
+  const double trB3 = B3[0][0] + B3[1][1] + B3[2][2] + B3[3][3];
+  const double e1 = trB;
+  const double e2 = (trB * trB - trB2) / 2.0;
+  const double e3 = (trB * trB * trB - 3.0 * trB * trB2 + 2.0 * trB3) / 6.0;
+  const double ee4 = ex_det4(B);
+  const double d1 = -(e1 + e2 + e3 + ee4);
+  const double detgm1 = -d1 - q1 + d1 * q1;
+  sqrtg = sqrt(1.0 + detgm1);
+}
+
+// background_state's one dual pass (src/initialdata.jl:1683-1694) for a moving
+// hole: h = g - eta, Pi = (sqrt(gamma)/alpha)(d_t g - beta^i d_i g) and d_t h,
+// every operation in the oracle's order with contraction off. The layer's and
+// the core's target rates are centered differences of these values over
+// 2 delta = 2 cbrt(eps) r_0, which multiplies any difference in their last
+// bits by 1/(2 delta) ~ 1e5; evaluated this way they have none.
+// Kept out of line: a moving hole's core and layer call it up to three times
+// a point, and inlined copies of it would multiply the kernels' size.
+__host__ __device__ __noinline__ inline void bg_state_exact(const Bg& bg, double t, const double x[3], double h[NC],
+                                                     double P[NC], double dtg[NC]) {
+  NS X[4];
+  const double p[4] = {t, x[0], x[1], x[2]};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    X[k] = ns_cst(p[k]);
+    X[k].d[k] = 1.0;
+  }
+  NS g[4][4];
+  if (bg_boosted(bg)) {
+    double L[4][4];
+    lorentz_boost(bg.u, L);
+    const NS xr[4] = {X[0], X[1] - bg.c[0], X[2] - bg.c[1], X[3] - bg.c[2]};
+    NS xp[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xp[c] = L[c][0] * xr[0] + L[c][1] * xr[1] + L[c][2] * xr[2] + L[c][3] * xr[3];
+    NS gp[4][4];
+    if (bg.kind == BG_HARMONIC)
+      harmonic_metric_exact(bg, xp[1] - 0.0, xp[2] - 0.0, xp[3] - 0.0, gp);
+    else
+      ks_metric_exact(bg, xp[1] - 0.0, xp[2] - 0.0, xp[3] - 0.0, gp);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        NS sum = ns_cst(0.0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int d = 0; d < 4; ++d) sum = sum + nd_mul(L[c][a], L[d][b]) * gp[c][d];
+        g[a][b] = sum;
+      }
+  } else if (bg.kind == BG_HARMONIC) {
+    harmonic_metric_exact(bg, X[1] - bg.c[0], X[2] - bg.c[1], X[3] - bg.c[2], g);
+  } else {
+    ks_metric_exact(bg, X[1] - bg.c[0], X[2] - bg.c[1], X[3] - bg.c[2], g);
+  }
+  double dh[3][NC];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
+    const int i = prow(n), j = pcol(n);
+    h[n] = nd_sub(g[i][j].v, eta(i, j));
+    dtg[n] = g[i][j].d[0];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dh[k][n] = g[i][j].d[k + 1];
+  }
+  double alpha, beta[3], sqrtg;
+  ex_lapse_shift(h, alpha, beta, sqrtg);
+  const double s = nd_div(sqrtg, alpha);
+#pragma unroll
+  for (int n = 0; n < NC; ++n)
+    P[n] = nd_mul(s, nd_sub(nd_sub(nd_sub(dtg[n], nd_mul(beta[0], dh[0][n])), nd_mul(beta[1], dh[1][n])),
+                            nd_mul(beta[2], dh[2][n])));
 }
 
 // Sets of backgrounds, as bit masks over the kinds: a kernel compiled for one
@@ -711,7 +957,8 @@ TGH_HD void bg_harmonic_offset(const Bg& bg, const double x[3], Dual<4> hd[NC]) 
 // kind fixes the set: H_a = 0 admits only harmonic charts, and every chart
 // that is not harmonic needs a source.
 constexpr int bg_bit(int kind) { return 1 << kind; }
-constexpr int BGS_ALL = 0x1f;
+constexpr int BGS_MOVING = 1 << 5;  // the boosted (moving) Kerr-Schild and harmonic charts
+constexpr int BGS_ALL = 0x1f | BGS_MOVING;
 constexpr int BGS_HARMONIC = bg_bit(BG_MINKOWSKI) | bg_bit(BG_GAUGE_WAVE) | bg_bit(BG_HARMONIC);
 constexpr int BGS_SOURCED = bg_bit(BG_MINKOWSKI) | bg_bit(BG_KERR_SCHILD) | bg_bit(BG_SHIFTED);
 
@@ -723,7 +970,7 @@ TGH_HD void bg_offset(const Bg& bg, double t, const double x[3], J hd[NC]) {
 #pragma unroll
   for (int n = 0; n < NC; ++n) hd[n] = jet_const<J>(0.0);
   if (bg.kind == BG_MINKOWSKI) return;
-  if (bg.kind < 0 || bg.kind > BG_HARMONIC || !(SET & bg_bit(bg.kind))) {
+  if (bg.kind < 0 || bg.kind > BG_HARMONIC || !(SET & bg_bit(bg.kind)) || bg_boosted(bg)) {
 #pragma unroll
     for (int n = 0; n < NC; ++n) hd[n] = jet_const<J>(__builtin_nan(""));
     return;
@@ -906,6 +1153,13 @@ TGH_HD void gauge_source_grad(const Bg& bg, double t, const double x[3], double 
 // Pi = (sqrt(gamma)/alpha)(d_t g - beta^i d_i g) at (t, x).
 template <int SET = BGS_ALL>
 TGH_HD void background_state(const Bg& bg, double t, const double x[3], double h[NC], double P[NC]) {
+  if constexpr ((SET & BGS_MOVING) != 0) {
+    if (bg_boosted(bg)) {  // a moving hole: every value in the oracle's order
+      double dtg[NC];
+      bg_state_exact(bg, t, x, h, P, dtg);
+      return;
+    }
+  }
   Dual<4> hd[NC];
   bg_offset<Dual<4>, SET>(bg, t, x, hd);
 #pragma unroll

@@ -13,7 +13,10 @@
 namespace tgh {
 
 enum { INT_NONE = 0, INT_DAMPED = 1, INT_FROZEN = 2, INT_PASTED = 3 };
-enum { GAUGE_NONE = 0, GAUGE_SAMPLED = 1, GAUGE_KS = 2 };
+// Gauge-source kinds as the kernels are specialised: the case's kind, and for
+// a moving hole its own (its kernels carry the moving core and layer).
+enum { GAUGE_NONE = 0, GAUGE_SAMPLED = 1, GAUGE_KS = 2, GAUGE_KS_MOVING = 3, GAUGE_NONE_MOVING = 4 };
+constexpr bool gauge_has_source(int gauge) { return gauge == GAUGE_SAMPLED || gauge == GAUGE_KS || gauge == GAUGE_KS_MOVING; }
 enum { EPI_STORE = 0, EPI_RK4 = 1 };
 
 // Everything a kernel needs to know about the mesh and the case: passed by
@@ -36,8 +39,12 @@ struct CaseP {
   double eps_ko;  // the number, or the horizon profile's eps_out
   int diss;       // 0 the number, 1 the horizon profile
   double eps_in, eps_r1, eps_rh;
-  double center[3];
+  double center[3];  // the hole's center at t = 0
   double r0, r1, rho_max, w_ramp, rho_ramp;
+  double vel[3];     // its velocity: c(t) = center + vel t (HoleCenter)
+  int moving;        // vel != 0
+  double delta;      // layer_rate_step: cbrt(eps) r_0, the targets' rate difference step
+  double ks_u[4], ks_S[4];  // the moving KerrSchildSource: u = gamma (1, v), S the boosted spin
 };
 
 TGH_HD void position(const Geom& g, int b, int i, int j, int k, double x[3]) {
@@ -47,9 +54,21 @@ TGH_HD void position(const Geom& g, int b, int i, int j, int k, double x[3]) {
   x[2] = g.lo[2] + (double)(B[2] * g.N + k) * g.h;
 }
 
-TGH_HD double damping_rate(const CaseP& c, const double x[3]) {  // src/gauge.jl:1100,1140-1147
+// center_at (src/interior.jl:86-93): c(t) = c0 + v t. It and the radius and
+// the core rule below round as the oracle does (no contraction): the moving
+// core's target rate is a difference over 2 delta of values taken at them.
+TGH_HD void center_at(const CaseP& c, double t, double cc[3]) {
+#pragma clang fp contract(off)
+  cc[0] = c.center[0] + t * c.vel[0];
+  cc[1] = c.center[1] + t * c.vel[1];
+  cc[2] = c.center[2] + t * c.vel[2];
+}
+
+TGH_HD double damping_rate(const CaseP& c, double t, const double x[3]) {  // src/gauge.jl:1100,1140-1147
   if (c.damping == 0) return c.g0;
-  const double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
+  double cc[3];
+  center_at(c, t, cc);
+  const double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
   const double r2 = d1 * d1 + d2 * d2 + d3 * d3;
   return c.g0_far + (c.g0_near - c.g0_far) * exp(-r2 / (2.0 * c.g0_width * c.g0_width));
 }
@@ -58,9 +77,11 @@ TGH_HD double damping_rate(const CaseP& c, const double x[3]) {  // src/gauge.jl
 // HorizonDissipation profile about the hole's center -- eps_out at and
 // outside r_h, eps_in at and inside r_1, each through a branch, and the
 // quintic smoothstep of (r_h - r)/(r_h - r_1) between.
-TGH_HD double dissipation_rate(const CaseP& c, const double x[3]) {
+TGH_HD double dissipation_rate(const CaseP& c, double t, const double x[3]) {
   if (c.diss == 0) return c.eps_ko;
-  const double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
+  double cc[3];
+  center_at(c, t, cc);
+  const double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
   const double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
   if (r >= c.eps_rh) return c.eps_ko;
   if (r <= c.eps_r1) return c.eps_in;
@@ -68,19 +89,26 @@ TGH_HD double dissipation_rate(const CaseP& c, const double x[3]) {
   return c.eps_ko + (c.eps_in - c.eps_ko) * smoothstep(s);
 }
 
-TGH_HD double hole_radius(const CaseP& c, const double x[3]) {
-  const double d1 = x[0] - c.center[0], d2 = x[1] - c.center[1], d3 = x[2] - c.center[2];
+// interior_radius (src/interior.jl:303-310): |x - c(t)|.
+TGH_HD double hole_radius(const CaseP& c, double t, const double x[3]) {
+#pragma clang fp contract(off)
+  double cc[3];
+  center_at(c, t, cc);
+  const double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
   return sqrt(d1 * d1 + d2 * d2 + d3 * d3);
 }
 
-// case_state_tuple with the core rule (src/initialdata.jl:1795-1796,
-// src/interior.jl:615-625).
-TGH_HD void core_point(const CaseP& c, const double x[3], double xc[3]) {
+// core_position (src/interior.jl:615-625): inside the core, the point on the
+// sphere r_0 about c(t) along the ray (+z at the center).
+TGH_HD void core_point(const CaseP& c, double t, const double x[3], double xc[3]) {
+#pragma clang fp contract(off)
   xc[0] = x[0];
   xc[1] = x[1];
   xc[2] = x[2];
   if (c.interior != INT_NONE) {
-    const double d[3] = {x[0] - c.center[0], x[1] - c.center[1], x[2] - c.center[2]};
+    double cc[3];
+    center_at(c, t, cc);
+    const double d[3] = {x[0] - cc[0], x[1] - cc[1], x[2] - cc[2]};
     const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     if (r < c.r0) {
       double n[3] = {0.0, 0.0, 1.0};
@@ -89,18 +117,165 @@ TGH_HD void core_point(const CaseP& c, const double x[3], double xc[3]) {
         n[1] = d[1] / r;
         n[2] = d[2] / r;
       }
-      for (int q = 0; q < 3; ++q) xc[q] = c.center[q] + c.r0 * n[q];
+      for (int q = 0; q < 3; ++q) xc[q] = cc[q] + c.r0 * n[q];
     }
   }
 }
 template <int SET = BGS_ALL>
 TGH_HD void case_state(const CaseP& c, double t, const double x[3], double out[20]) {
   double xc[3];
-  core_point(c, x, xc);
+  core_point(c, t, x, xc);
   background_state<SET>(c.bg, t, xc, out, out + NC);
 }
-// The backgrounds a right-hand side compiled for a gauge-source kind can meet.
-constexpr int bg_set_for(int gauge) { return gauge == GAUGE_NONE ? BGS_HARMONIC : BGS_SOURCED; }
+// The backgrounds a right-hand side compiled for a gauge-source kind can meet:
+// H_a = 0 the harmonic charts (at rest or moving), a sampled or the static
+// closed-form source the charts at rest that need one, the moving closed form
+// the moving Kerr-Schild hole.
+constexpr int bg_set_for(int gauge) {
+  return gauge == GAUGE_NONE          ? BGS_HARMONIC
+         : gauge == GAUGE_NONE_MOVING ? (bg_bit(BG_HARMONIC) | BGS_MOVING)
+         : gauge == GAUGE_KS_MOVING   ? (bg_bit(BG_KERR_SCHILD) | BGS_MOVING)
+                                      : BGS_SOURCED;
+}
+
+// ---------------------------------------------------------------------------
+// The moving hole's core and layer (src/interior.jl:390-549,
+// src/initialdata.jl:1697-1764), in the oracle's order without contraction.
+
+// background_state_rate: the state at (t, x), its d_t h from the same pass,
+// and d_t Pi as the centered difference of Pi at t +- delta over the
+// difference of the two rounded times.
+TGH_HD void background_state_rate(const Bg& bg, double t, const double x[3], double delta, double h[NC],
+                                  double P[NC], double dth[NC], double dtP[NC]) {
+#pragma clang fp contract(off)
+  bg_state_exact(bg, t, x, h, P, dth);
+  const double tp = t + delta, tm = t - delta;
+  double hx[NC], Pp[NC], Pm[NC], dx[NC];
+  bg_state_exact(bg, tp, x, hx, Pp, dx);
+  bg_state_exact(bg, tm, x, hx, Pm, dx);
+#pragma unroll
+  for (int n = 0; n < NC; ++n) dtP[n] = (Pp[n] - Pm[n]) / (tp - tm);
+}
+// core_target_rate: the rate of t -> u_exact(t, c(t) + r_0 n(t)) at x by the
+// centered difference, weighted by core_rate_weight = smoothstep(4 r / r_0 - 1);
+// no pass where that is zero (r <= r_0 / 4).
+TGH_HD void core_target_rate(const CaseP& c, double t, const double x[3], double dth[NC], double dtP[NC]) {
+#pragma clang fp contract(off)
+  const double s = smoothstep(4.0 * hole_radius(c, t, x) / c.r0 - 1.0);
+#pragma unroll
+  for (int n = 0; n < NC; ++n) dth[n] = dtP[n] = 0.0;
+  if (s == 0.0) return;
+  const double tp = t + c.delta, tm = t - c.delta;
+  double xp[3], xm[3];
+  core_point(c, tp, x, xp);
+  core_point(c, tm, x, xm);
+  double hp[NC], Pp[NC], hm[NC], Pm[NC], dx[NC];
+  bg_state_exact(c.bg, tp, xp, hp, Pp, dx);
+  bg_state_exact(c.bg, tm, xm, hm, Pm, dx);
+  const double f = s / (tp - tm);
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
+    dth[n] = f * (hp[n] - hm[n]);
+    dtP[n] = f * (Pp[n] - Pm[n]);
+  }
+}
+TGH_HD void inv3_ex(const double m[3][3], double o[3][3]) {
+#pragma clang fp contract(off)
+  const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1], c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2],
+               c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+  o[0][0] = c00 / det;
+  o[1][0] = c01 / det;
+  o[2][0] = c02 / det;
+  o[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det;
+  o[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
+  o[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det;
+  o[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
+  o[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det;
+  o[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
+}
+TGH_HD void mv3_ex(const double m[3][3], const double v[3], double o[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[i] = m[i][0] * v[0] + m[i][1] * v[1] + m[i][2] * v[2];
+}
+// _adm_split (src/interior.jl:505-515) without the square root.
+TGH_HD void adm_split_ex(const double h[NC], double& A, double bu[3], double bl[3], double g[3][3]) {
+#pragma clang fp contract(off)
+  g[0][0] = 1.0 + h[4]; g[0][1] = h[5]; g[0][2] = h[6];
+  g[1][0] = h[5]; g[1][1] = 1.0 + h[7]; g[1][2] = h[8];
+  g[2][0] = h[6]; g[2][1] = h[8]; g[2][2] = 1.0 + h[9];
+  bl[0] = h[1];
+  bl[1] = h[2];
+  bl[2] = h[3];
+  double gi[3][3];
+  inv3_ex(g, gi);
+  mv3_ex(gi, bl, bu);
+  A = (1.0 - h[0]) + (bu[0] * bl[0] + bu[1] * bl[1] + bu[2] * bl[2]);
+}
+// core_relaxation_rate (src/interior.jl:450-503): the metric relaxed toward
+// h_t at rate rho in (alpha^2, beta^i, gamma_ij), plus the target's ADM rate
+// read off d_t h_t at h_t, mapped back to h by the chain rule of
+// g_tt = -alpha^2 + gamma_ij beta^i beta^j, g_ti = gamma_ij beta^j.
+TGH_HD void core_relaxation_rate(const double h[NC], const double ht[NC], double rho, const double dht[NC],
+                                 double o[NC]) {
+#pragma clang fp contract(off)
+  double A, bu[3], bl[3], g[3][3], At, but[3], blt[3], gt[3][3];
+  adm_split_ex(h, A, bu, bl, g);
+  adm_split_ex(ht, At, but, blt, gt);
+  const double dgt[3][3] = {{dht[4], dht[5], dht[6]}, {dht[5], dht[7], dht[8]}, {dht[6], dht[8], dht[9]}};
+  double gti[3][3], dgbt[3], rh[3], dbt[3];
+  inv3_ex(gt, gti);
+  mv3_ex(dgt, but, dgbt);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) rh[i] = dht[1 + i] - dgbt[i];
+  mv3_ex(gti, rh, dbt);
+  const double dAt = -dht[0] + (but[0] * dgbt[0] + but[1] * dgbt[1] + but[2] * dgbt[2]) +
+                     2.0 * (blt[0] * dbt[0] + blt[1] * dbt[1] + blt[2] * dbt[2]);
+  const double dA = dAt - rho * (A - At);
+  double db[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) db[i] = dbt[i] - rho * (bu[i] - but[i]);
+  const double dd[3][3] = {{h[4] - ht[4], h[5] - ht[5], h[6] - ht[6]},
+                           {h[5] - ht[5], h[7] - ht[7], h[8] - ht[8]},
+                           {h[6] - ht[6], h[8] - ht[8], h[9] - ht[9]}};
+  double dg[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dg[i][j] = dgt[i][j] - rho * dd[i][j];
+  double dgb[3], gdb[3];
+  mv3_ex(dg, bu, dgb);
+  mv3_ex(g, db, gdb);
+  const double dtt = -dA + (bu[0] * dgb[0] + bu[1] * dgb[1] + bu[2] * dgb[2]) +
+                     2.0 * (bl[0] * db[0] + bl[1] * db[1] + bl[2] * db[2]);
+  o[0] = dtt;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[1 + i] = dgb[i] + gdb[i];
+  o[4] = dg[0][0];
+  o[5] = dg[1][0];
+  o[6] = dg[2][0];
+  o[7] = dg[1][1];
+  o[8] = dg[2][1];
+  o[9] = dg[2][2];
+}
+// The moving :damped core's rates at x (gh_rhs_point!, src/evolution.jl:
+// 526-583): the ADM relaxation of h and Pi's linear one toward the target at
+// the core rule's point, at core_rate = max(rho_max, 12 |v| / r_0), plus the
+// target's rates. hu and Pu are the point's state.
+TGH_HD void moving_core_rates(const CaseP& c, double t, const double x[3], const double hu[NC],
+                              const double Pu[NC], double dh[NC], double dP[NC]) {
+#pragma clang fp contract(off)
+  const double s = sqrt(c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2]);
+  const double rc = fmax(c.rho_max, 12.0 * s / c.r0);
+  double xc[3], hd[NC], Pd[NC], dx[NC], dthd[NC], dtPd[NC];
+  core_point(c, t, x, xc);
+  bg_state_exact(c.bg, t, xc, hd, Pd, dx);
+  core_target_rate(c, t, x, dthd, dtPd);
+  core_relaxation_rate(hu, hd, rc, dthd, dh);
+#pragma unroll
+  for (int v = 0; v < NC; ++v) dP[v] = dtPd[v] - rc * (Pu[v] - Pd[v]);
+}
 
 // Where a point of block b's work array takes its value from (TreeAMR 0.3's
 // closed-block fill on a uniform forest): -2 the Dirichlet hook, -1 an own
@@ -358,10 +533,10 @@ struct GlobalSrc {
 // The frozen core (F is not evaluated there: a hole at rest, du = 0) and the
 // point's radius from the hole's centre.
 template <bool HOLE>
-__device__ __forceinline__ bool frozen_core(const CaseP& c, const double x[3], double& r) {
+__device__ __forceinline__ bool frozen_core(const CaseP& c, double t, const double x[3], double& r) {
   r = 0.0;
   if (!HOLE) return false;
-  r = hole_radius(c, x);
+  r = hole_radius(c, t, x);
   return c.interior == INT_PASTED ? r < c.r1 : r < c.r0;
 }
 
@@ -412,7 +587,7 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
                                             PiCoef& pc, TGH_OUT_PARAMS) {
   const long sv = pr.sv, psz = pr.ssv, si0 = pr.si0, wi0 = pr.wi0;
   const CaseP& c = a.c;
-  const double g0 = damping_rate(c, pr.x);
+  const double g0 = damping_rate(c, a.t, pr.x);
   double msrc[NC];
   {
     double Hl[4] = {0.0, 0.0, 0.0, 0.0}, dH[4][4];
@@ -424,8 +599,10 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
       for (int n = 0; n < 16; ++n) dH[n % 4][n / 4] = Hp[(4 + n) * pr.hs];
     } else if (GAUGE == GAUGE_KS) {
       ks_source(c.bg.M, c.bg.a, hv, D, Hl, dH);
+    } else if (GAUGE == GAUGE_KS_MOVING) {
+      ks_source_moving(c.bg.M, c.ks_S, c.ks_u, hv, D, Hl, dH);
     }
-    gh_source<GAUGE != GAUGE_NONE>(cf, hv, D, Hl, dH, g0, c.g2, msrc);
+    gh_source<gauge_has_source(GAUGE)>(cf, hv, D, Hl, dH, g0, c.g2, msrc);
   }
   if (pr.m_rho != nullptr) {
     const double rr = pr.m_rho[0];
@@ -445,38 +622,76 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
     wgt = smoothstep(sl / c.w_ramp);
     rho = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
   }
-  double ue[20];
-  if (layer) case_state<bg_set_for(GAUGE)>(c, a.t, pr.x, ue);  // r >= r0 here: the core rule is the identity
+  // The layer's target u_e (r >= r0 here: the core rule is the identity) and,
+  // for a moving :damped hole, its rates, which (1 - w) d_t u_e adds so that
+  // the moving solution is a steady state of the layer (src/evolution.jl:652-680).
+  constexpr bool MOVE = (bg_set_for(GAUGE) & BGS_MOVING) != 0;
+  const bool follows = MOVE && c.moving && c.interior == INT_DAMPED;
+  double ue[20], re[20];
+  if (layer) {
+    if (follows)
+      background_state_rate(c.bg, a.t, pr.x, c.delta, ue, ue + NC, re, re + NC);
+    else
+      case_state<bg_set_for(GAUGE)>(c, a.t, pr.x, ue);
+  }
+  const double om = 1.0 - wgt;
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
     double f = D[0][v];
-    if (layer) f = wgt * f - rho * (hv[v] - ue[v]);
+    if (layer) f = follows ? wgt * f + om * re[v] - rho * (hv[v] - ue[v]) : wgt * f - rho * (hv[v] - ue[v]);
     emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, f);
   }
   pc.w = layer ? wgt : 1.0;
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
     double c1 = msrc[v];
-    if (layer) c1 = wgt * c1 - rho * (src.P(v) - ue[NC + v]);
+    if (layer)
+      c1 = follows ? wgt * c1 + om * re[NC + v] - rho * (src.P(v) - ue[NC + v])
+                   : wgt * c1 - rho * (src.P(v) - ue[NC + v]);
     pc.c1[v] = c1;
   }
 }
 
+// The frozen core's rates (gh_rhs_point!, src/evolution.jl:500-583): zero for
+// a hole at rest; a moving :damped core relaxes toward its target.
+template <int GAUGE, class SRC>
+__device__ __forceinline__ void core_rates(const RhsArgs& a, const double x[3], const SRC& src, double dh[NC],
+                                           double dP[NC]) {
+  constexpr bool MOVE = (bg_set_for(GAUGE) & BGS_MOVING) != 0;
+  if constexpr (MOVE) {
+    if (a.c.moving && a.c.interior == INT_DAMPED) {
+      double hu[NC], Pu[NC];
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        hu[v] = src.X(v)[0];
+        Pu[v] = src.P(v);
+      }
+      moving_core_rates(a.c, a.t, x, hu, Pu, dh, dP);
+      return;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NC; ++v) dh[v] = dP[v] = 0.0;
+}
+
 // The whole head at one point (gh_rhs_head, src/evolution.jl:258-297, inside
 // gh_rhs_point!, :448-695,846-863). Returns false in the frozen core, where
-// d_t h = 0 is emitted.
+// the core's d_t h is emitted and its d_t Pi left in pc.c1 (w = -1).
 template <int Q, int GAUGE, bool HOLE, int EPI, class SRC>
 __device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr, const SRC& src, double inv_h,
                                            PiCoef& pc, TGH_OUT_PARAMS) {
   double r;
-  if (frozen_core<HOLE>(a.c, pr.x, r)) {
+  if (frozen_core<HOLE>(a.c, a.t, pr.x, r)) {
+    double dh[NC];
+    core_rates<GAUGE>(a, pr.x, src, dh, pc.c1);
 #pragma unroll 1
-    for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * pr.ssv, pr.wi0 + v * pr.sv, 0.0);
+    for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * pr.ssv, pr.wi0 + v * pr.sv, dh[v]);
+    pc.w = -1.0;
     return false;
   }
   double hv[NC], D[4][NC];
   Coef cf;
-  head_derivs<Q>(a.c, src, inv_h, dissipation_rate(a.c, pr.x) * inv_h, hv, D, cf);
+  head_derivs<Q>(a.c, src, inv_h, dissipation_rate(a.c, a.t, pr.x) * inv_h, hv, D, cf);
   head_source<GAUGE, HOLE, EPI>(a, pr, src, r, cf, hv, D, pc, TGH_OUT_ARGS);
   head_geometry(cf, D, pc);
   return true;
@@ -514,10 +729,11 @@ __device__ __forceinline__ void rhs_point(const RhsArgs& a, const PointRef& pr, 
   const GlobalSrc src{pr.w0, pr.sv, pr.s2, pr.s3};
   if (!head_point<Q, GAUGE, HOLE, EPI>(a, pr, src, inv_h, pc, TGH_OUT_ARGS)) {
 #pragma unroll 1
-    for (int v = NC; v < 20; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * psz, pr.wi0 + v * sv, 0.0);
+    for (int v = 0; v < NC; ++v)
+      emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + (NC + v) * psz, pr.wi0 + (NC + v) * sv, pc.c1[v]);
     return;
   }
-  const double ih = inv_h, ih2 = inv_h * inv_h, epsh = dissipation_rate(a.c, pr.x) * inv_h;
+  const double ih = inv_h, ih2 = inv_h * inv_h, epsh = dissipation_rate(a.c, a.t, pr.x) * inv_h;
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
     const double* ph = pr.w0 + v * sv;
@@ -621,7 +837,7 @@ __global__ __launch_bounds__(256) void k_head(RhsArgs a, const double* __restric
   if (head_point<Q, GAUGE, HOLE, EPI>(a, pr, src, a.g.inv_h, pc, TGH_OUT_ARGS))
     store_coef(coef + pi, cs, pc);
   else
-    coef[CF_W * cs + pi] = -1.0;
+    store_coef_source(coef + pi, cs, pc);  // w = -1 and the core's d_t Pi
 }
 
 // The split form, second half: the ten Pi components at one owned point per
@@ -640,12 +856,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_PI_WAVE
   const double* __restrict__ cf0 = coef + pi;
   PiCoef pc;
   load_coef(cf0, cs, pc);
-  if (pc.w < 0.0) {
+  if (pc.w < 0.0) {  // the frozen core: its d_t Pi is the coefficient c1
 #pragma unroll 1
-    for (int v = NC; v < 20; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * psz, pr.wi0 + v * sv, 0.0);
+    for (int v = 0; v < NC; ++v)
+      emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + (NC + v) * psz, pr.wi0 + (NC + v) * sv, cf0[(CF_C1 + v) * cs]);
     return;
   }
-  const double ih = a.g.inv_h, ih2 = ih * ih, epsh = dissipation_rate(a.c, pr.x) * ih;
+  const double ih = a.g.inv_h, ih2 = ih * ih, epsh = dissipation_rate(a.c, a.t, pr.x) * ih;
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
     const double* ph = pr.w0 + v * sv;
@@ -738,7 +955,7 @@ __global__ __launch_bounds__(256) void k_sample_gauge(SampleArgs a) {
   const long psz = (long)P * P * P;
   double x[3], xc[3];
   position(g, b, i, j, k, x);
-  core_point(a.c, x, xc);
+  core_point(a.c, a.t, x, xc);
   double Hl[4], dH[4][4];
   gauge_source_grad(a.c.bg, a.t, xc, Hl, dH);
   double* out = a.Hsrc + (long)b * 20 * psz + p;
@@ -772,7 +989,7 @@ __global__ __launch_bounds__(256) void k_pointwise(PointArgs a) {
     for (int v = 0; v < 20; ++v) ub[v * psz] += a.amp * (2.0 * point_draw(a.base, x, v + 1) - 1.0);
     return;
   }
-  if (a.mode == 2 && !(hole_radius(a.c, x) < a.c.r1)) return;
+  if (a.mode == 2 && !(hole_radius(a.c, a.t, x) < a.c.r1)) return;
   double s[20];
   case_state(a.c, a.t, x, s);
   for (int v = 0; v < 20; ++v) ub[v * psz] = s[v];
@@ -790,8 +1007,8 @@ struct ReduceArgs {
   double* __restrict__ out;  // per workgroup: [0] max speed | sum e^2, [1] nonfinite | count, [2] linf
   int mode;                  // 0 speed, 1 error
 };
-__device__ __forceinline__ bool is_evolved(const CaseP& c, const double x[3]) {
-  return c.interior == INT_NONE || hole_radius(c, x) >= c.r1;
+__device__ __forceinline__ bool is_evolved(const CaseP& c, double t, const double x[3]) {
+  return c.interior == INT_NONE || hole_radius(c, t, x) >= c.r1;
 }
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
   const Geom& g = a.g;
@@ -806,7 +1023,7 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
     const double* ub = a.u + (long)b * 20 * psz + p;
     double x[3];
     position(g, b, i, j, k, x);
-    if (is_evolved(a.c, x)) {
+    if (is_evolved(a.c, a.t, x)) {
       if (a.mode == 0) {
         double hv[NC];
         bool fin = true;
@@ -1044,9 +1261,14 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
   pr.ms = 0;
   const GlobalSrc src{pr.w0, pr.sv, pr.s2, pr.s3};
   double r;
-  if (frozen_core<HOLE>(a.c, pr.x, r)) {
+  if (frozen_core<HOLE>(a.c, a.t, pr.x, r)) {
+    double dh[NC], dP[NC];
+    core_rates<GAUGE>(a, pr.x, src, dh, dP);
 #pragma unroll 1
-    for (int v = 0; v < NC; ++v) emit<EPI_RK4>(a, TGH_OUT_ARGS, off + v * k.fs, off + v * k.fs, 0.0);
+    for (int v = 0; v < NC; ++v) {
+      emit<EPI_RK4>(a, TGH_OUT_ARGS, off + v * k.fs, off + v * k.fs, dh[v]);
+      coef[(CF_C1 + v) * k.fs + off] = dP[v];  // the core's d_t Pi, for the Pi kernel
+    }
     coef[CF_W * k.fs + off] = -1.0;
     return;
   }
@@ -1055,198 +1277,11 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
   double hv[NC], D[4][NC];
   Coef cf;
   PiCoef pc;
-  head_derivs<Q>(a.c, src, k.inv_h, dissipation_rate(a.c, pr.x) * k.inv_h, hv, D, cf);
+  head_derivs<Q>(a.c, src, k.inv_h, dissipation_rate(a.c, a.t, pr.x) * k.inv_h, hv, D, cf);
   head_geometry(cf, D, pc);
   store_coef_geometry(coef + off, k.fs, pc);
   head_source<GAUGE, HOLE, EPI_RK4>(a, pr, src, r, cf, hv, D, pc, TGH_OUT_ARGS);
   store_coef_source(coef + off, k.fs, pc);
-}
-
-// The head in two launches on the brick (the default). The stencils and the
-// algebra of the head do not fit one kernel at more than one wave per SIMD:
-// held together, the 190 stencil values the compiler issues early and the
-// source's live set take ~400 registers. So k_brick_derivs evaluates the
-// stencil half -- h, the 30 first derivatives, d_t h with its dissipation, the
-// geometric coefficients -- and leaves D = (d_t h, d_x h, d_y h, d_z h), 40
-// values per point, in a brick-layout scratch array; k_brick_source reads
-// them back with 20 contiguous loads of the state and evaluates the source,
-// the interior layer and the RK4 epilogue of d_t h. Every value is computed by
-// the same code in the same order as the one-kernel head, so the two forms
-// agree bit for bit.
-#ifndef TGH_DERIVS_WAVES_PER_EU
-#define TGH_DERIVS_WAVES_PER_EU 2
-#endif
-__device__ __forceinline__ void brick_point_ref(const RhsArgs& a, const Brick& k, const double* W, int x, int y,
-                                                int z, PointRef& pr) {
-  const long off = brick_index(k, x, y, z);
-  pr.w0 = W + off;
-  pr.sv = k.fs;
-  pr.s2 = (int)k.py;
-  pr.s3 = (int)k.pz;
-  pr.x[0] = k.lo[0] + (double)x * k.h;
-  pr.x[1] = k.lo[1] + (double)y * k.h;
-  pr.x[2] = k.lo[2] + (double)z * k.h;
-  pr.si0 = off;
-  pr.ssv = k.fs;
-  pr.wi0 = off;
-  pr.Hp = a.Hsrc ? a.Hsrc + off : nullptr;
-  pr.hs = k.fs;
-  pr.m_rho = nullptr;
-  pr.m_S = nullptr;
-  pr.m_Sij = nullptr;
-  pr.ms = 0;
-}
-template <int Q, bool HOLE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_DERIVS_WAVES_PER_EU))) void k_brick_derivs(
-    RhsArgs a, Brick k, const double* __restrict__ W, double* __restrict__ coef, double* __restrict__ Dbuf) {
-  int x, y, z;
-  if (!brick_map(k, x, y, z)) return;
-  const long off = brick_index(k, x, y, z);
-  const double xp[3] = {k.lo[0] + (double)x * k.h, k.lo[1] + (double)y * k.h, k.lo[2] + (double)z * k.h};
-  double r;
-  if (frozen_core<HOLE>(a.c, xp, r)) return;
-  const GlobalSrc src{W + off, k.fs, (int)k.py, (int)k.pz};
-  double hv[NC], D[4][NC];
-  Coef cf;
-  head_derivs<Q>(a.c, src, k.inv_h, dissipation_rate(a.c, xp) * k.inv_h, hv, D, cf);
-  PiCoef pc;
-  head_geometry(cf, D, pc);
-  store_coef_geometry(coef + off, k.fs, pc);
-  double* __restrict__ d0 = Dbuf + off;
-#pragma unroll
-  for (int n = 0; n < 4 * NC; ++n) d0[n * k.fs] = D[n / NC][n % NC];
-}
-template <int GAUGE, bool HOLE>
-__global__ __launch_bounds__(256) void k_brick_source(RhsArgs a, Brick k, const double* __restrict__ W,
-                                                      const double* __restrict__ Dbuf, double* __restrict__ coef,
-                                                      TGH_OUT_PARAMS) {
-  int x, y, z;
-  if (!brick_map(k, x, y, z)) return;
-  PointRef pr;
-  brick_point_ref(a, k, W, x, y, z, pr);
-  const long off = pr.si0;
-  double r;
-  if (frozen_core<HOLE>(a.c, pr.x, r)) {
-#pragma unroll 1
-    for (int v = 0; v < NC; ++v) emit<EPI_RK4>(a, TGH_OUT_ARGS, off + v * k.fs, off + v * k.fs, 0.0);
-    coef[CF_W * k.fs + off] = -1.0;
-    return;
-  }
-  double hv[NC], D[4][NC];
-#pragma unroll
-  for (int v = 0; v < NC; ++v) hv[v] = W[off + v * k.fs];
-#pragma unroll
-  for (int n = 0; n < 4 * NC; ++n) D[n / NC][n % NC] = Dbuf[off + n * k.fs];
-  Coef cf;
-  metric_quantities(hv, cf);
-  const GlobalSrc src{W + off, k.fs, (int)k.py, (int)k.pz};
-  PiCoef pc;
-  head_source<GAUGE, HOLE, EPI_RK4>(a, pr, src, r, cf, hv, D, pc, TGH_OUT_ARGS);
-  store_coef_source(coef + off, k.fs, pc);
-}
-
-// The head with its inputs staged in LDS (q <= 4): a workgroup takes a tile
-// of 8 x 8 x 4 points, one per thread, and first loads, for the ten h_v, the
-// three lines of points its stencils read -- the tile widened by H = G along
-// x, along y and along z (three arrays, the centre in each) -- and the tile's
-// Pi_v, 70 coalesced loads per thread issued together; then each thread
-// evaluates the head from LDS. 143 KB of LDS at q = 4 (gfx950 allows 160 KB
-// per workgroup), so one workgroup per CU, the head's register count already
-// keeping it at one wave per SIMD.
-template <int Q> struct HeadTile {
-  static constexpr int X = 8, Y = 8, Z = 4, H = ::tgh::W<Q>::R + 1;
-  static constexpr int XA = (X + 2 * H) * Y * Z, YA = X * (Y + 2 * H) * Z, ZA = X * Y * (Z + 2 * H);
-  static constexpr int PA = X * Y * Z;
-  static constexpr int DOUBLES = NC * (XA + YA + ZA + PA);
-};
-struct LdsSrc {
-  const double* xa;
-  const double* ya;
-  const double* za;
-  const double* pa;
-  int fx, fy, fz, fp;
-  int sy, sz;
-  __device__ __forceinline__ const double* X(int v) const { return xa + v * fx; }
-  __device__ __forceinline__ const double* Y(int v) const { return ya + v * fy; }
-  __device__ __forceinline__ const double* Z(int v) const { return za + v * fz; }
-  __device__ __forceinline__ double P(int v) const { return pa[v * fp]; }
-};
-template <int Q, int GAUGE, bool HOLE>
-__global__ __launch_bounds__(256) void k_brick_head_lds(RhsArgs a, Brick k, const double* __restrict__ W,
-                                                        double* __restrict__ coef, TGH_OUT_PARAMS) {
-  using T = HeadTile<Q>;
-  constexpr int TX = T::X, TY = T::Y, TZ = T::Z, H = T::H, NT = TX * TY * TZ;
-  constexpr int WX = TX + 2 * H, WY = TY + 2 * H, WZ = TZ + 2 * H;
-  __shared__ double lds[T::DOUBLES];
-  double* const xa = lds;
-  double* const ya = xa + NC * T::XA;
-  double* const za = ya + NC * T::YA;
-  double* const pa = za + NC * T::ZA;
-  const int ntx = (k.n[0] + TX - 1) / TX, nty = (k.n[1] + TY - 1) / TY, ntz = (k.n[2] + TZ - 1) / TZ;
-  int tile, unused;
-  if (!wg_map(1, ntx * nty * ntz, tile, unused)) return;  // uniform over the workgroup
-  const int X0 = (tile % ntx) * TX, Y0 = ((tile / ntx) % nty) * TY, Z0 = (tile / (ntx * nty)) * TZ;
-  const int t = threadIdx.x;
-  const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
-  // brick points at or beyond n + G on any axis are outside the stored extent
-  // (they are read by no evolved point); below -G none is asked for
-  const int mx = k.n[0] + k.G, my = k.n[1] + k.G, mz = k.n[2] + k.G;
-  for (int e = t; e < T::XA; e += NT) {
-    const int gx = X0 - H + e % WX, gy = Y0 + (e / WX) % TY, gz = Z0 + e / (WX * TY);
-    const bool in = gx < mx && gy < my && gz < mz;
-    const long g = in ? brick_index(k, gx, gy, gz) : 0;
-#pragma unroll
-    for (int v = 0; v < NC; ++v) xa[v * T::XA + e] = in ? W[g + v * k.fs] : 0.0;
-  }
-  for (int e = t; e < T::YA; e += NT) {
-    const int gx = X0 + e % TX, gy = Y0 - H + (e / TX) % WY, gz = Z0 + e / (TX * WY);
-    const bool in = gx < mx && gy < my && gz < mz;
-    const long g = in ? brick_index(k, gx, gy, gz) : 0;
-#pragma unroll
-    for (int v = 0; v < NC; ++v) ya[v * T::YA + e] = in ? W[g + v * k.fs] : 0.0;
-  }
-  for (int e = t; e < T::ZA; e += NT) {
-    const int gx = X0 + e % TX, gy = Y0 + (e / TX) % TY, gz = Z0 - H + e / (TX * TY);
-    const bool in = gx < mx && gy < my && gz < mz;
-    const long g = in ? brick_index(k, gx, gy, gz) : 0;
-#pragma unroll
-    for (int v = 0; v < NC; ++v) za[v * T::ZA + e] = in ? W[g + v * k.fs] : 0.0;
-  }
-  {
-    const int gx = X0 + lx, gy = Y0 + ly, gz = Z0 + lz;
-    const bool in = gx < mx && gy < my && gz < mz;
-    const long g = in ? brick_index(k, gx, gy, gz) : 0;
-#pragma unroll
-    for (int v = 0; v < NC; ++v) pa[v * T::PA + t] = in ? W[g + (NC + v) * k.fs] : 0.0;
-  }
-  __syncthreads();
-  const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-  if (!(x < k.n[0] && y < k.n[1] && z < k.n[2])) return;  // no barrier follows
-  const long off = brick_index(k, x, y, z);
-  PointRef pr;
-  pr.w0 = W + off;
-  pr.sv = k.fs;
-  pr.s2 = (int)k.py;
-  pr.s3 = (int)k.pz;
-  pr.x[0] = k.lo[0] + (double)x * k.h;
-  pr.x[1] = k.lo[1] + (double)y * k.h;
-  pr.x[2] = k.lo[2] + (double)z * k.h;
-  pr.si0 = off;
-  pr.ssv = k.fs;
-  pr.wi0 = off;
-  pr.Hp = a.Hsrc ? a.Hsrc + off : nullptr;
-  pr.hs = k.fs;
-  pr.m_rho = nullptr;
-  pr.m_S = nullptr;
-  pr.m_Sij = nullptr;
-  pr.ms = 0;
-  const LdsSrc src{xa + (lx + H) + WX * (ly + TY * lz), ya + lx + TX * ((ly + H) + WY * lz),
-                   za + lx + TX * (ly + TY * (lz + H)), pa + t, T::XA, T::YA, T::ZA, T::PA, TX, TX * TY};
-  PiCoef pc;
-  if (head_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, src, k.inv_h, pc, TGH_OUT_ARGS))
-    store_coef(coef + off, k.fs, pc);
-  else
-    coef[CF_W * k.fs + off] = -1.0;
 }
 
 // The split form on the brick layout, second half: a workgroup takes a tile
@@ -1324,7 +1359,7 @@ __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, 
     pc[q].w = -1.0;
     if (valid[q]) load_coef(coef + off[q], k.fs, pc[q]);
     const double xp[3] = {k.lo[0] + (double)x * k.h, k.lo[1] + (double)y * k.h, k.lo[2] + (double)z * k.h};
-    epsh[q] = dissipation_rate(a.c, xp) * ih;
+    epsh[q] = dissipation_rate(a.c, a.t, xp) * ih;
     c0[q] = (lx + H) + RX * ((ly + H) + BY * (lz + q * TZ + H));
   }
   // box point (bx, by, bz) is brick point (X0 - H + bx, ...): never below -G,
@@ -1405,7 +1440,7 @@ __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, 
 #pragma unroll
     for (int q = 0; q < PZ; ++q) {
       if (valid[q]) {
-        double f = 0.0;
+        double f = c1[q];  // the frozen core's d_t Pi where w < 0
         if (pc[q].w >= 0.0) {
           const double s = pi_principal<Q>(&sh[0][c0[q]], &sh[1][c0[q]], RX, RX * BY, pc[q].b, pc[q].divb,
                                            pc[q].divA, pc[q].A, ih, ih2, epsh[q], a.c.has_diss);
@@ -1609,7 +1644,7 @@ __global__ __launch_bounds__(256) void k_brick_sample(BrickSampleArgs a) {
   if (!brick_map(k, x, y, z)) return;
   const double xp[3] = {k.lo[0] + (double)x * k.h, k.lo[1] + (double)y * k.h, k.lo[2] + (double)z * k.h};
   double xc[3];
-  core_point(a.c, xp, xc);
+  core_point(a.c, a.t, xp, xc);
   double Hl[4], dH[4][4];
   gauge_source_grad(a.c.bg, a.t, xc, Hl, dH);
   double* out = a.Hsrc + brick_index(k, x, y, z);
@@ -1678,7 +1713,7 @@ __global__ __launch_bounds__(256) void k_bounds(BoundsArgs a) {
   }
   double hit = 0.0, nf = 0.0, rm = 0.0;
   if (valid) {
-    const double r = hole_radius(a.c, x);
+    const double r = hole_radius(a.c, 0.0, x);  // a hole at rest (the library refuses bounds on a moving one)
     if (r < a.bd.r_gate) {
       double h[NC], Pi[NC];
 #pragma unroll

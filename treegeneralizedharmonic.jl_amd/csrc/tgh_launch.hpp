@@ -6,14 +6,6 @@
 
 namespace tgh {
 
-#ifndef TGH_HEAD_LDS
-#define TGH_HEAD_LDS 0
-#endif
-constexpr bool kHeadLds = TGH_HEAD_LDS != 0;
-#ifndef TGH_HEAD_TWO
-#define TGH_HEAD_TWO 0
-#endif
-constexpr bool kHeadTwo = TGH_HEAD_TWO != 0;
 
 // One right-hand side: the fused kernel, or the head and the Pi components as
 // two launches. While profiling, the head is timed as the "rhs" pair and the
@@ -73,6 +65,12 @@ void launch_rhs_g(tgh_problem* p, const RhsArgs& a) {
       if (hole) launch_rhs_q<Q, GAUGE_SAMPLED, true, EPI>(p, a);
       else launch_rhs_q<Q, GAUGE_SAMPLED, false, EPI>(p, a);
       break;
+    case GAUGE_KS_MOVING:  // a moving hole always has an interior
+      launch_rhs_q<Q, GAUGE_KS_MOVING, true, EPI>(p, a);
+      break;
+    case GAUGE_NONE_MOVING:
+      launch_rhs_q<Q, GAUGE_NONE_MOVING, true, EPI>(p, a);
+      break;
     default:
       if (hole) launch_rhs_q<Q, GAUGE_NONE, true, EPI>(p, a);
       else launch_rhs_q<Q, GAUGE_NONE, false, EPI>(p, a);
@@ -96,17 +94,7 @@ void launch_brick_rhs_q(tgh_problem* p, const RhsArgs& a) {
                        a.W, a.du, a.us, a.acc, a.Wn);
     if (prof) p->ev_rhs.push_back({e0, mark(p)});
   } else {
-    if (kHeadTwo) {  // the head as derivatives + source launches
-      hipLaunchKernelGGL((k_brick_derivs<Q, HOLE>), grid_brick(p), dim3(256), 0, p->stream, a, p->brick, a.W,
-                         p->Bcoef, p->Bd);
-      hipLaunchKernelGGL((k_brick_source<GAUGE, HOLE>), grid_brick(p), dim3(256), 0, p->stream, a, p->brick, a.W,
-                         (const double*)p->Bd, p->Bcoef, a.du, a.us, a.acc, a.Wn);
-    } else if constexpr (Q <= 4 && kHeadLds) {  // the head from LDS-staged lines (fits LDS up to q = 4)
-      using HT = HeadTile<Q>;
-      hipLaunchKernelGGL((k_brick_head_lds<Q, GAUGE, HOLE>), grid_brick_tiles(p, HT::X, HT::Y, HT::Z),
-                         dim3(HT::X * HT::Y * HT::Z), 0, p->stream, a, p->brick, a.W, p->Bcoef, a.du, a.us, a.acc,
-                         a.Wn);
-    } else {
+    {
 #ifdef TGH_HEAD_TILE_X
       constexpr int HX = TGH_HEAD_TILE_X, HZ = 4, HY = 256 / (HX * HZ);
       const dim3 hgrid = grid_brick_tiles(p, HX, HY, HZ);
@@ -137,6 +125,12 @@ void launch_brick_rhs_g(tgh_problem* p, const RhsArgs& a) {
     case GAUGE_SAMPLED:
       if (hole) launch_brick_rhs_q<Q, GAUGE_SAMPLED, true>(p, a);
       else launch_brick_rhs_q<Q, GAUGE_SAMPLED, false>(p, a);
+      break;
+    case GAUGE_KS_MOVING:
+      launch_brick_rhs_q<Q, GAUGE_KS_MOVING, true>(p, a);
+      break;
+    case GAUGE_NONE_MOVING:
+      launch_brick_rhs_q<Q, GAUGE_NONE_MOVING, true>(p, a);
       break;
     default:
       if (hole) launch_brick_rhs_q<Q, GAUGE_NONE, true>(p, a);
@@ -169,6 +163,12 @@ void launch_batch(hipStream_t st, const BatchEntry* tab, int n, int head_total, 
     case GAUGE_SAMPLED:
       if (hole) launch_batch_q<Q, GAUGE_SAMPLED, true>(st, tab, n, head_total, pi_total);
       else launch_batch_q<Q, GAUGE_SAMPLED, false>(st, tab, n, head_total, pi_total);
+      break;
+    case GAUGE_KS_MOVING:
+      launch_batch_q<Q, GAUGE_KS_MOVING, true>(st, tab, n, head_total, pi_total);
+      break;
+    case GAUGE_NONE_MOVING:
+      launch_batch_q<Q, GAUGE_NONE_MOVING, true>(st, tab, n, head_total, pi_total);
       break;
     default:
       if (hole) launch_batch_q<Q, GAUGE_NONE, true>(st, tab, n, head_total, pi_total);

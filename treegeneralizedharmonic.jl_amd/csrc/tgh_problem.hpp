@@ -47,7 +47,6 @@ struct tgh_problem {
   double* Bacc = nullptr;
   double* BH = nullptr;
   double* Bcoef = nullptr;            // the split form's coefficients (CF_N brick fields)
-  double* Bd = nullptr;               // the two-launch head's D = (d_t h, d_i h), 40 brick fields
   bool brick_ready = false;
   bool frame_static = false;          // the frame never changes: filled once
   hipStream_t cap_stream = nullptr;   // capture stream when the problem runs on the null stream
