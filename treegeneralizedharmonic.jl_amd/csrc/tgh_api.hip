@@ -193,7 +193,7 @@ int check_desc(const tgh_desc* d) {
 
 void launch_fill(tgh_problem* p, double* W, double t) {
   FillArgs a{p->geom, p->cs, t, W};
-  hipLaunchKernelGGL(k_fill, grid_work(p), dim3(256), 0, p->stream, a);
+  hipLaunchKernelGGL(p->cs.moving ? k_fill<true> : k_fill<false>, grid_work(p), dim3(256), 0, p->stream, a);
 }
 void launch_scatter(tgh_problem* p, const double* u, double* W, int to_work) {
   CopyArgs a{p->geom, to_work ? u : W, to_work ? W : (double*)u, to_work};
@@ -236,7 +236,7 @@ int sync(tgh_problem* p) {
 
 int reduce(tgh_problem* p, const double* u, double t, int mode) {
   ReduceArgs a{p->geom, p->cs, t, u, p->red, mode};
-  hipLaunchKernelGGL(k_reduce, grid_points(p), dim3(256), 0, p->stream, a);
+  hipLaunchKernelGGL(p->cs.moving ? k_reduce<true> : k_reduce<false>, grid_points(p), dim3(256), 0, p->stream, a);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(p->red_host.data(), p->red, sizeof(double) * p->red_host.size(),
                          hipMemcpyDeviceToHost, p->stream));
@@ -315,7 +315,7 @@ void launch_frame(tgh_problem* p, double* W, double t) {
   const Brick& k = p->brick;
   const long E = (long)(k.n[0] + 2 * k.G) * (k.n[1] + 2 * k.G) * (k.n[2] + 2 * k.G);
   FrameArgs a{k, p->cs, t, W};
-  hipLaunchKernelGGL(k_brick_frame, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, p->stream, a);
+  hipLaunchKernelGGL(p->cs.moving ? k_brick_frame<true> : k_brick_frame<false>, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, p->stream, a);
 }
 
 // A background that does not depend on t (the gauge wave is the one that does).
@@ -729,7 +729,7 @@ int tgh_stage_limit(tgh_problem* p, double* u, double t) {
   }
   if (p->cs.interior != INT_PASTED) return TGH_OK;
   PointArgs a{p->geom, p->cs, t, u, 0, 0.0, 2};
-  hipLaunchKernelGGL(k_pointwise, grid_points(p), dim3(256), 0, p->stream, a);
+  hipLaunchKernelGGL(p->cs.moving ? k_pointwise<true> : k_pointwise<false>, grid_points(p), dim3(256), 0, p->stream, a);
   return sync(p);
 }
 int tgh_step_limit(tgh_problem* p, double* u, double t) { return tgh_stage_limit(p, u, t); }
@@ -737,7 +737,7 @@ int tgh_step_limit(tgh_problem* p, double* u, double t) { return tgh_stage_limit
 int tgh_initial_state(tgh_problem* p, double* u, double t) {
   if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
   PointArgs a{p->geom, p->cs, t, u, 0, 0.0, 0};
-  hipLaunchKernelGGL(k_pointwise, grid_points(p), dim3(256), 0, p->stream, a);
+  hipLaunchKernelGGL(p->cs.moving ? k_pointwise<true> : k_pointwise<false>, grid_points(p), dim3(256), 0, p->stream, a);
   int rc = sync(p);
   if (rc) return rc;
   return tgh_stage_limit(p, u, t);
@@ -747,7 +747,7 @@ int tgh_add_noise(tgh_problem* p, double* u, uint64_t base, double amplitude) {
   if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
   if (!(amplitude >= 0)) return fail(TGH_EINVAL, "the noise amplitude must be >= 0");
   PointArgs a{p->geom, p->cs, 0.0, u, base, amplitude, 1};
-  hipLaunchKernelGGL(k_pointwise, grid_points(p), dim3(256), 0, p->stream, a);
+  hipLaunchKernelGGL(p->cs.moving ? k_pointwise<true> : k_pointwise<false>, grid_points(p), dim3(256), 0, p->stream, a);
   return sync(p);
 }
 

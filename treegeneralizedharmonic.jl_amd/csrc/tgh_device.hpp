@@ -959,6 +959,7 @@ __host__ __device__ __noinline__ inline void bg_state_exact(const Bg& bg, double
 constexpr int bg_bit(int kind) { return 1 << kind; }
 constexpr int BGS_MOVING = 1 << 5;  // the boosted (moving) Kerr-Schild and harmonic charts
 constexpr int BGS_ALL = 0x1f | BGS_MOVING;
+constexpr int BGS_STATIC = 0x1f;  // every chart at rest
 constexpr int BGS_HARMONIC = bg_bit(BG_MINKOWSKI) | bg_bit(BG_GAUGE_WAVE) | bg_bit(BG_HARMONIC);
 constexpr int BGS_SOURCED = bg_bit(BG_MINKOWSKI) | bg_bit(BG_KERR_SCHILD) | bg_bit(BG_SHIFTED);
 

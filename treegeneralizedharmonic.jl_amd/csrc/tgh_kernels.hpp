@@ -509,6 +509,7 @@ struct PointRef {
 struct PiCoef {
   double b[3], divb, divA[3], A[6], w;
   double c1[NC];
+  double epsh;  // the Kreiss-Oliger amplitude over h at the point
 };
 
 // The head at one point (gh_rhs_head, src/evolution.jl:258-297, inside
@@ -691,7 +692,8 @@ __device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr,
   }
   double hv[NC], D[4][NC];
   Coef cf;
-  head_derivs<Q>(a.c, src, inv_h, dissipation_rate(a.c, a.t, pr.x) * inv_h, hv, D, cf);
+  pc.epsh = dissipation_rate(a.c, a.t, pr.x) * inv_h;
+  head_derivs<Q>(a.c, src, inv_h, pc.epsh, hv, D, cf);
   head_source<GAUGE, HOLE, EPI>(a, pr, src, r, cf, hv, D, pc, TGH_OUT_ARGS);
   head_geometry(cf, D, pc);
   return true;
@@ -733,7 +735,7 @@ __device__ __forceinline__ void rhs_point(const RhsArgs& a, const PointRef& pr, 
       emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + (NC + v) * psz, pr.wi0 + (NC + v) * sv, pc.c1[v]);
     return;
   }
-  const double ih = inv_h, ih2 = inv_h * inv_h, epsh = dissipation_rate(a.c, a.t, pr.x) * inv_h;
+  const double ih = inv_h, ih2 = inv_h * inv_h, epsh = pc.epsh;
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
     const double* ph = pr.w0 + v * sv;
@@ -745,7 +747,7 @@ __device__ __forceinline__ void rhs_point(const RhsArgs& a, const PointRef& pr, 
 
 // Coefficient slots of the split form, each an array of cs doubles indexed
 // like the points: the PiCoef fields in order (w = -1 marks the frozen core).
-enum { CF_B0 = 0, CF_DIVB = 3, CF_DIVA = 4, CF_A = 7, CF_W = 13, CF_C1 = 14, CF_N = 24 };
+enum { CF_B0 = 0, CF_DIVB = 3, CF_DIVA = 4, CF_A = 7, CF_W = 13, CF_C1 = 14, CF_EPS = 24, CF_N = 25 };
 __device__ __forceinline__ void store_coef(double* __restrict__ cf0, long cs, const PiCoef& pc) {
 #pragma unroll
   for (int n = 0; n < 3; ++n) cf0[(CF_B0 + n) * cs] = pc.b[n];
@@ -757,9 +759,10 @@ __device__ __forceinline__ void store_coef(double* __restrict__ cf0, long cs, co
   cf0[CF_W * cs] = pc.w;
 #pragma unroll
   for (int n = 0; n < NC; ++n) cf0[(CF_C1 + n) * cs] = pc.c1[n];
+  cf0[CF_EPS * cs] = pc.epsh;
 }
-// The geometric and the source halves of the coefficients, for the head in
-// two launches.
+// The geometric and the source halves of the coefficients, stored as each is
+// known.
 __device__ __forceinline__ void store_coef_geometry(double* __restrict__ cf0, long cs, const PiCoef& pc) {
 #pragma unroll
   for (int n = 0; n < 3; ++n) cf0[(CF_B0 + n) * cs] = pc.b[n];
@@ -768,6 +771,7 @@ __device__ __forceinline__ void store_coef_geometry(double* __restrict__ cf0, lo
   for (int n = 0; n < 3; ++n) cf0[(CF_DIVA + n) * cs] = pc.divA[n];
 #pragma unroll
   for (int n = 0; n < 6; ++n) cf0[(CF_A + n) * cs] = pc.A[n];
+  cf0[CF_EPS * cs] = pc.epsh;
 }
 __device__ __forceinline__ void store_coef_source(double* __restrict__ cf0, long cs, const PiCoef& pc) {
   cf0[CF_W * cs] = pc.w;
@@ -783,6 +787,7 @@ __device__ __forceinline__ void load_coef(const double* __restrict__ cf0, long c
 #pragma unroll
   for (int n = 0; n < 6; ++n) pc.A[n] = cf0[(CF_A + n) * cs];
   pc.w = cf0[CF_W * cs];
+  pc.epsh = cf0[CF_EPS * cs];
 }
 
 // The block layout: one owned point of one block per thread.
@@ -862,7 +867,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_PI_WAVE
       emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + (NC + v) * psz, pr.wi0 + (NC + v) * sv, cf0[(CF_C1 + v) * cs]);
     return;
   }
-  const double ih = a.g.inv_h, ih2 = ih * ih, epsh = dissipation_rate(a.c, a.t, pr.x) * ih;
+  const double ih = a.g.inv_h, ih2 = ih * ih, epsh = pc.epsh;
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
     const double* ph = pr.w0 + v * sv;
@@ -882,6 +887,7 @@ struct FillArgs {
   double t;
   double* __restrict__ W;
 };
+template <bool MOVE>
 __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
   const Geom& g = a.g;
   const int S = g.S, G = g.G;
@@ -898,7 +904,7 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
   if (cl == -2) {
     double x[3], s[20];
     position(g, b, l[0], l[1], l[2], x);
-    case_state(a.c, a.t, x, s);
+    case_state<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
 #pragma unroll
     for (int v = 0; v < 20; ++v) dst[v * sv] = s[v];
   } else {
@@ -973,6 +979,7 @@ struct PointArgs {
   double amp;
   int mode;  // 0 exact state, 1 add noise, 2 paste inside r1
 };
+template <bool MOVE>
 __global__ __launch_bounds__(256) void k_pointwise(PointArgs a) {
   const Geom& g = a.g;
   const int P = g.P;
@@ -991,7 +998,7 @@ __global__ __launch_bounds__(256) void k_pointwise(PointArgs a) {
   }
   if (a.mode == 2 && !(hole_radius(a.c, a.t, x) < a.c.r1)) return;
   double s[20];
-  case_state(a.c, a.t, x, s);
+  case_state<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
   for (int v = 0; v < 20; ++v) ub[v * psz] = s[v];
 }
 
@@ -1010,6 +1017,7 @@ struct ReduceArgs {
 __device__ __forceinline__ bool is_evolved(const CaseP& c, double t, const double x[3]) {
   return c.interior == INT_NONE || hole_radius(c, t, x) >= c.r1;
 }
+template <bool MOVE>
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
   const Geom& g = a.g;
   const int P = g.P;
@@ -1040,7 +1048,7 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
              sqrt(cf.beta[0] * cf.beta[0] + cf.beta[1] * cf.beta[1] + cf.beta[2] * cf.beta[2]);
       } else {
         double s[20];
-        case_state(a.c, a.t, x, s);
+        case_state<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
         double e2 = 0.0;
 #pragma unroll
         for (int v = 0; v < 20; ++v) {
@@ -1277,7 +1285,8 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
   double hv[NC], D[4][NC];
   Coef cf;
   PiCoef pc;
-  head_derivs<Q>(a.c, src, k.inv_h, dissipation_rate(a.c, a.t, pr.x) * k.inv_h, hv, D, cf);
+  pc.epsh = dissipation_rate(a.c, a.t, pr.x) * k.inv_h;
+  head_derivs<Q>(a.c, src, k.inv_h, pc.epsh, hv, D, cf);
   head_geometry(cf, D, pc);
   store_coef_geometry(coef + off, k.fs, pc);
   head_source<GAUGE, HOLE, EPI_RK4>(a, pr, src, r, cf, hv, D, pc, TGH_OUT_ARGS);
@@ -1349,7 +1358,6 @@ __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, 
   bool valid[PZ];
   long off[PZ];
   int c0[PZ];
-  double epsh[PZ];
   PiCoef pc[PZ];
 #pragma unroll
   for (int q = 0; q < PZ; ++q) {
@@ -1357,9 +1365,8 @@ __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, 
     valid[q] = x < k.n[0] && y < k.n[1] && z < k.n[2];
     off[q] = valid[q] ? brick_index(k, x, y, z) : 0;
     pc[q].w = -1.0;
+    pc[q].epsh = 0.0;
     if (valid[q]) load_coef(coef + off[q], k.fs, pc[q]);
-    const double xp[3] = {k.lo[0] + (double)x * k.h, k.lo[1] + (double)y * k.h, k.lo[2] + (double)z * k.h};
-    epsh[q] = dissipation_rate(a.c, a.t, xp) * ih;
     c0[q] = (lx + H) + RX * ((ly + H) + BY * (lz + q * TZ + H));
   }
   // box point (bx, by, bz) is brick point (X0 - H + bx, ...): never below -G,
@@ -1440,12 +1447,17 @@ __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, 
 #pragma unroll
     for (int q = 0; q < PZ; ++q) {
       if (valid[q]) {
-        double f = c1[q];  // the frozen core's d_t Pi where w < 0
+        // c1 is added after both branches: the wait for it (which, counted in
+        // order, also waits for the next component's prefetched columns) then
+        // comes after the principal part. In the frozen core (w < 0) the rate
+        // is c1 itself.
+        double ws = 0.0;
         if (pc[q].w >= 0.0) {
           const double s = pi_principal<Q>(&sh[0][c0[q]], &sh[1][c0[q]], RX, RX * BY, pc[q].b, pc[q].divb,
-                                           pc[q].divA, pc[q].A, ih, ih2, epsh[q], a.c.has_diss);
-          f = pc[q].w * s + c1[q];
+                                           pc[q].divA, pc[q].A, ih, ih2, pc[q].epsh, a.c.has_diss);
+          ws = pc[q].w * s;
         }
+        const double f = ws + c1[q];
         const long si = off[q] + (NC + v) * k.fs;
         epi_store(a, us, acc, Wn, si, si, f, u0[q], a0[q]);
       }
@@ -1516,6 +1528,7 @@ struct FrameArgs {
   double t;
   double* __restrict__ W;
 };
+template <bool MOVE>
 __global__ __launch_bounds__(256) void k_brick_frame(FrameArgs a) {
   const Brick& k = a.k;
   const long E0 = k.n[0] + 2 * k.G, E1 = k.n[1] + 2 * k.G, E2 = k.n[2] + 2 * k.G;
@@ -1542,7 +1555,7 @@ __global__ __launch_bounds__(256) void k_brick_frame(FrameArgs a) {
     double x[3], s[20];
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[d] = k.lo[d] + (double)gp[d] * k.h;
-    case_state(a.c, a.t, x, s);
+    case_state<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
 #pragma unroll
     for (int v = 0; v < 20; ++v) dst[v * k.fs] = s[v];
   } else {
