@@ -128,6 +128,25 @@ typedef struct tgh_batch tgh_batch;
  * the reference's constructor checks (order, ghost width, interior radii
  * against the mesh) and allocates the work arrays on the current device. */
 int tgh_problem_create(const tgh_desc* desc, tgh_problem** out);
+/* GHProblem(fs, GhostSchedule(fs, Operators(prolongation = p, ...)), case; q)
+ * on a frozen hierarchy — gh_forest(; refined = true) and hole_forest
+ * (src/initialdata.jl:1370-1515) with TreeAMR's ghost schedule
+ * (test/evolution_cases.jl:34-41). desc->nblocks is the brick of root
+ * blocks; leaves holds nleaves quadruples (level, i, j, k), (i, j, k) the
+ * leaf's coordinates in the brick of its level (nblocks_d 2^level blocks of
+ * N cells of spacing h / 2^level); the state's blocks are the leaves in this
+ * order. The leaves must tile the box, be 2:1 balanced across faces, edges,
+ * corners and periodic seams, and N must be even and at least 2G.
+ * prolongation is the coarse-to-fine interpolation order p (even, 2 <= p <=
+ * q + 2; the interface-order rule of test/interface_tests.jl:64-96 asks for
+ * q + 2). Restriction on a vertex-centred mesh is injection, whatever its
+ * order, and takes no argument. Such a problem steps on the block layout
+ * and cannot join a batch. */
+int tgh_problem_create_forest(const tgh_desc* desc, const int32_t* leaves, int32_t nleaves, int32_t prolongation,
+                              tgh_problem** out);
+/* The level of every block, 1 per block (0 on a uniform forest;
+ * level(k) of TreeAMR's leaves). */
+int tgh_block_levels(const tgh_problem* p, int32_t* out);
 /* No reference counterpart: releases the problem's device memory. */
 int tgh_problem_destroy(tgh_problem* p);
 

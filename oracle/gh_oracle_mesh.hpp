@@ -1,11 +1,17 @@
-// CPU oracle, mesh half: a uniform forest of closed vertex blocks, the ghost
-// fill, the fused right-hand side at every owned point, RK4 and the analysis
-// reductions.
+// CPU oracle, mesh half: a forest of closed vertex blocks (uniform, or a
+// frozen 2:1-balanced hierarchy of levels), the ghost fill, the fused
+// right-hand side at every owned point, RK4 and the analysis reductions.
 //
 // TEST INFRASTRUCTURE ONLY (see gh_oracle.hpp). Restates
 // src/evolution.jl:100-435,448-695,932-949,1327-1560, src/stepping.jl:53-125,
-// src/constraints.jl:311-396 and TreeAMR 0.3's uniform-forest ghost fill.
+// src/constraints.jl:311-396 and TreeAMR 0.3's ghost fill (same-level copies,
+// injection from finer leaves, Lagrange prolongation from coarser ones: the
+// operator set of test/interface_tests.jl:46-52; TreeAMR's own source is not
+// vendored, so the rules below are stated here and pinned by polynomial
+// reproduction and by the interface rates of test/interface_tests.jl:64-96).
 #pragma once
+#include <map>
+
 #include "gh_oracle.hpp"
 #include "gh_oracle_bounds.hpp"
 
@@ -53,9 +59,12 @@ struct Mesh {
   long widx(long b, int v, long i, long j, long k) const {
     return (i + G) + S * ((j + G) + S * ((k + G) + S * (v + 20 * b)));
   }
+  // The spacing of block b: h on a uniform forest, h / 2^level on a forest.
+  double hb(long b) const { return level.empty() ? h : h / (double)(1L << level[b]); }
   void position(long b, long i, long j, long k, double x[3]) const {
     long g[3] = {blocks[b][0] * (long)N + i, blocks[b][1] * (long)N + j, blocks[b][2] * (long)N + k};
-    for (int d = 0; d < 3; ++d) x[d] = lo[d] + (double)g[d] * h;
+    const double hh = hb(b);
+    for (int d = 0; d < 3; ++d) x[d] = lo[d] + (double)g[d] * hh;
   }
   // Source of a work point: -2 Dirichlet, -1 own primary, else owner block and
   // local index.
@@ -91,7 +100,177 @@ struct Mesh {
   // Is owned point l of block b a primary (counted once in sums)?
   bool is_primary(long b, long i, long j, long k) const {
     long l[3] = {i, j, k}, owner, loc[3];
+    if (forest()) {
+      Src s;
+      return forest_classify(b, l, s) == F_PRIMARY;
+    }
     return classify(b, l, owner, loc) == -1;
+  }
+
+  // ---- a frozen hierarchy (gh_forest(; refined), hole_forest:
+  // src/initialdata.jl:1370-1515) ----
+  // blocks[b] holds leaf b's coordinates in the brick of its level (nb_d 2^l
+  // blocks of N cells of spacing h / 2^l per axis); level[b] its level.
+  std::vector<int> level;  // empty: a uniform forest
+  int Lmax = 0;
+  int prolong = 0;         // the prolongation order p (an even number of points)
+  std::map<std::array<long, 4>, long> leaf;
+  std::vector<double> wmid;  // Lagrange weights at the midpoint, p nodes
+  bool forest() const { return !level.empty(); }
+
+  void init_forest(const std::vector<std::array<int, 4>>& leaves, int p) {
+    G = q / 2 + 1;
+    S = N + 1 + 2 * G;
+    P = N + 1;
+    prolong = p;
+    blocks.clear();
+    level.clear();
+    leaf.clear();
+    Lmax = 0;
+    for (const auto& lf : leaves) {
+      level.push_back(lf[0]);
+      blocks.push_back({lf[1], lf[2], lf[3]});
+      leaf[{(long)lf[0], (long)lf[1], (long)lf[2], (long)lf[3]}] = (long)blocks.size() - 1;
+      Lmax = std::max(Lmax, lf[0]);
+    }
+    // w_k = prod_{j != k} (1/2 - x_j) / (x_k - x_j) at nodes x = -p/2+1 .. p/2:
+    // the numerator and the denominator are exact, so one rounding.
+    wmid.assign(p, 0.0);
+    for (int k = 0; k < p; ++k) {
+      double num = 1.0, den = 1.0;
+      const double xk = k - p / 2 + 1;
+      for (int j = 0; j < p; ++j) {
+        if (j == k) continue;
+        const double xj = j - p / 2 + 1;
+        num *= 0.5 - xj;
+        den *= xk - xj;
+      }
+      wmid[k] = num / den;
+    }
+  }
+  enum { F_PRIMARY = 0, F_COPY = 1, F_DIRICHLET = 2, F_PROLONG = 3 };
+  struct Src {
+    long b = -1;
+    long loc[3] = {0, 0, 0};  // COPY: the point; PROLONG: the stencil's first point
+    int half[3] = {0, 0, 0};  // PROLONG: the axes where the point is a coarse midpoint
+  };
+  // The leaf of level lv holding integer point gv (index in that level's
+  // brick) as a vertex: per axis the block where the local index is below N
+  // first, then the lower one where it is N (the uniform rule), in the order
+  // x, y, z of the alternatives' bits.
+  bool find_leaf(int lv, const long gv[3], long& ob, long loc[3]) const {
+    long o0[3], alt[3];
+    for (int d = 0; d < 3; ++d) {
+      const long nbl = (long)nb[d] << lv;
+      o0[d] = std::min(gv[d] / N, nbl - 1);
+      alt[d] = -1;
+      if (gv[d] % N == 0 && o0[d] * N == gv[d]) {
+        if (o0[d] > 0) alt[d] = o0[d] - 1;
+        else if (periodic[d]) alt[d] = nbl - 1;
+      }
+    }
+    for (int mask = 0; mask < 8; ++mask) {
+      long o[3];
+      bool ok = true;
+      for (int d = 0; d < 3; ++d) {
+        if (mask >> d & 1) {
+          if (alt[d] < 0) ok = false;
+          o[d] = alt[d];
+        } else {
+          o[d] = o0[d];
+        }
+      }
+      if (!ok) continue;
+      auto it = leaf.find({(long)lv, o[0], o[1], o[2]});
+      if (it == leaf.end()) continue;
+      ob = it->second;
+      for (int d = 0; d < 3; ++d) {
+        loc[d] = gv[d] - o[d] * N;
+        if (loc[d] < 0) loc[d] += ((long)nb[d] << lv) * N;  // the periodic alternative at 0
+      }
+      return true;
+    }
+    return false;
+  }
+  // Where point l of leaf b's work array takes its value from: its own
+  // primary; a copy of the primary of the finest leaf holding the point as a
+  // vertex (a same-level copy, an injection from a finer leaf, a coincident
+  // coarse vertex); the Dirichlet hook outside a non-periodic axis; or the
+  // order-p Lagrange prolongation from the leaf one level coarser that holds
+  // it, whose midpoint axes take p points and whose other axes one.
+  int forest_classify(long b, const long l[3], Src& s) const {
+    const int L = level[b];
+    long gl[3];
+    for (int d = 0; d < 3; ++d) {
+      const long tot = ((long)nb[d] << L) * N;
+      gl[d] = blocks[b][d] * (long)N + l[d];
+      if (periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
+      else if (gl[d] < 0 || gl[d] > tot) return F_DIRICHLET;
+    }
+    for (int lv = Lmax; lv >= 0; --lv) {
+      long gv[3];
+      bool vertex = true;
+      for (int d = 0; d < 3; ++d) {
+        if (lv >= L) {
+          gv[d] = gl[d] << (lv - L);
+        } else {
+          const long f = 1L << (L - lv);
+          if (gl[d] % f != 0) vertex = false;
+          gv[d] = gl[d] / f;
+        }
+      }
+      if (!vertex) continue;
+      long ob, loc[3];
+      if (!find_leaf(lv, gv, ob, loc)) continue;
+      if (ob == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return F_PRIMARY;
+      s.b = ob;
+      for (int d = 0; d < 3; ++d) s.loc[d] = loc[d];
+      return F_COPY;
+    }
+    if (L == 0) throw std::runtime_error("a level-0 point that no leaf holds: the leaves do not tile the box");
+    // the coarse leaf: integer axes by the uniform rule, midpoint axes in
+    // the cell [m, m + 1] of level L - 1
+    long gc[3];
+    int half[3];
+    for (int d = 0; d < 3; ++d) {
+      half[d] = (int)(gl[d] & 1);
+      gc[d] = half[d] ? (gl[d] - 1) / 2 : gl[d] / 2;
+    }
+    long o0[3], alt[3];
+    for (int d = 0; d < 3; ++d) {
+      const long nbl = (long)nb[d] << (L - 1);
+      o0[d] = std::min(gc[d] / N, nbl - 1);
+      alt[d] = -1;
+      if (!half[d] && gc[d] % N == 0 && o0[d] * N == gc[d]) {
+        if (o0[d] > 0) alt[d] = o0[d] - 1;
+        else if (periodic[d]) alt[d] = nbl - 1;
+      }
+    }
+    for (int mask = 0; mask < 8; ++mask) {
+      long o[3];
+      bool ok = true;
+      for (int d = 0; d < 3; ++d) {
+        if (mask >> d & 1) {
+          if (alt[d] < 0) ok = false;
+          o[d] = alt[d];
+        } else {
+          o[d] = o0[d];
+        }
+      }
+      if (!ok) continue;
+      auto it = leaf.find({(long)(L - 1), o[0], o[1], o[2]});
+      if (it == leaf.end()) continue;
+      s.b = it->second;
+      for (int d = 0; d < 3; ++d) {
+        long loc = gc[d] - o[d] * N;
+        if (loc < 0) loc += ((long)nb[d] << (L - 1)) * N;
+        s.half[d] = half[d];
+        s.loc[d] = half[d] ? loc - prolong / 2 + 1 : loc;
+      }
+      return F_PROLONG;
+    }
+    throw std::runtime_error("a ghost of a level-" + std::to_string(L) +
+                             " leaf lies in no leaf of level L - 1: the forest is not 2:1 balanced");
   }
 };
 
@@ -104,8 +283,10 @@ struct Problem {
   std::vector<double> w1, w2, wD;
 
   void setup() {
-    mesh.init();
-    mesh.build_lookup();
+    if (!mesh.forest()) {
+      mesh.init();
+      mesh.build_lookup();
+    }
     work.assign(mesh.work_size(), 0.0);
     w1 = to_double(rational_derivative_weights(mesh.q, 1));
     w2 = to_double(rational_derivative_weights(mesh.q, 2));
@@ -165,6 +346,10 @@ struct Problem {
         for (long k = 0; k <= m.N; ++k)
           for (long j = 0; j <= m.N; ++j)
             for (long i = 0; i <= m.N; ++i) work[m.widx(b, v, i, j, k)] = u[m.sidx(b, v, i, j, k)];
+    if (m.forest()) {
+      fill_forest(t);
+      return;
+    }
     for (long b = 0; b < m.nblocks(); ++b)
       for (long k = -m.G; k <= m.N + m.G; ++k)
         for (long j = -m.G; j <= m.N + m.G; ++j)
@@ -184,6 +369,56 @@ struct Problem {
                 work[m.widx(b, v, i, j, k)] = work[m.widx(owner, v, loc[0], loc[1], loc[2])];
             }
           }
+  }
+
+  // The fill of a forest, after the scatter: first every copy (each reads a
+  // primary, which no fill writes) and every Dirichlet ghost, then the
+  // prolongations level by level from the coarsest, so that the coarse
+  // ghosts a stencil reads are final before it runs. A point that is not its
+  // block's primary takes its owner's value, the owned face points of a
+  // coarse leaf included (the injection of the finer leaf's values).
+  void fill_forest(double t) {
+    const Mesh& m = mesh;
+    for (int pass = 0; pass <= m.Lmax; ++pass)
+      for (long b = 0; b < m.nblocks(); ++b) {
+        if (pass > 0 && m.level[b] != pass) continue;
+        for (long k = -m.G; k <= m.N + m.G; ++k)
+          for (long j = -m.G; j <= m.N + m.G; ++j)
+            for (long i = -m.G; i <= m.N + m.G; ++i) {
+              long l[3] = {i, j, k};
+              Mesh::Src s;
+              const int c = m.forest_classify(b, l, s);
+              if (c == Mesh::F_PRIMARY) continue;
+              if (pass == 0 && c == Mesh::F_DIRICHLET) {
+                double x[3], st[20];
+                m.position(b, i, j, k, x);
+                case_state(cs, t, x, st);
+                for (int v = 0; v < 20; ++v) work[m.widx(b, v, i, j, k)] = st[v];
+              } else if (pass == 0 && c == Mesh::F_COPY) {
+                for (int v = 0; v < 20; ++v)
+                  work[m.widx(b, v, i, j, k)] = work[m.widx(s.b, v, s.loc[0], s.loc[1], s.loc[2])];
+              } else if (pass > 0 && c == Mesh::F_PROLONG) {
+                // sum-factorised: x innermost, then y, then z
+                const int n[3] = {s.half[0] ? m.prolong : 1, s.half[1] ? m.prolong : 1,
+                                  s.half[2] ? m.prolong : 1};
+                for (int v = 0; v < 20; ++v) {
+                  const double* base = &work[m.widx(s.b, v, s.loc[0], s.loc[1], s.loc[2])];
+                  double sz = 0.0;
+                  for (int c2 = 0; c2 < n[2]; ++c2) {
+                    double sy = 0.0;
+                    for (int c1 = 0; c1 < n[1]; ++c1) {
+                      double sx = 0.0;
+                      for (int c0 = 0; c0 < n[0]; ++c0)
+                        sx += (n[0] > 1 ? m.wmid[c0] : 1.0) * base[c0 + m.S * (c1 + m.S * c2)];
+                      sy += (n[1] > 1 ? m.wmid[c1] : 1.0) * sx;
+                    }
+                    sz += (n[2] > 1 ? m.wmid[c2] : 1.0) * sy;
+                  }
+                  work[m.widx(b, v, i, j, k)] = sz;
+                }
+              }
+            }
+      }
   }
 
   // ---- the fused kernel at one owned point (evolution.jl:258-435,448-695) ----
@@ -230,7 +465,7 @@ struct Problem {
     long st[3] = {1, m.S, m.S * m.S};
     long sv = m.S * m.S * m.S;
     long var = m.widx(b, 0, i, j, k);
-    T inv_h = T(1.0) / T(m.h);
+    T inv_h = T(1.0) / T(m.hb(b));
     // (1) the state, the 30 first derivatives, the coefficient set
     V10<T> hv, Pv, dh[3];
     for (int v = 0; v < NC; ++v) {
@@ -304,7 +539,7 @@ struct Problem {
     double x[3];
     m.position(b, i, j, k, x);
     double g0 = damping_rate(cs, t, x);
-    double epsh = dissipation_rate(cs, t, x) * (1.0 / m.h);  // evolution.jl:478
+    double epsh = dissipation_rate(cs, t, x) * (1.0 / m.hb(b));  // evolution.jl:478
     bool diss = has_dissipation(cs);
     V10<double> dth, dtP;
     if (cs.interior != INT_NONE) {
@@ -395,7 +630,7 @@ struct Problem {
     double x[3];
     m.position(b, i, j, k, x);
     double g0 = damping_rate(cs, t, x);
-    double hsp = m.h;
+    double hsp = m.hb(b);
     auto at = [&](int v, long ii, long jj, long kk) { return work[m.widx(b, v, ii, jj, kk)]; };
     auto apply = [&](const std::vector<double>& w, int v, int d) {
       int n = (int)w.size(), r = (n - 1) / 2;
@@ -503,7 +738,7 @@ struct Problem {
           double t_ = w2[kk] * work[m.widx(b, v, i + (d == 0) * o, j + (d == 1) * o, k + (d == 2) * o)];
           acc = kk == 0 ? t_ : acc + t_;
         }
-        s = std::max(s, std::abs(acc) / (m.h * m.h));
+        s = std::max(s, std::abs(acc) / (m.hb(b) * m.hb(b)));
         s = std::max(s, std::abs(work[m.widx(b, v, i, j, k)]));
       }
     return std::max(s, 1.0);
@@ -548,6 +783,10 @@ struct Problem {
             double x[3], s[20];
             m.position(b, i, j, k, x);
             if (!is_evolved(t, x)) continue;
+            // on a forest only primaries count, each with its volume h^3
+            // relative to level 0's
+            const bool prim = m.is_primary(b, i, j, k);
+            if (m.forest() && !prim) continue;
             case_state(cs, t, x, s);
             double e2 = 0;
             for (int v = 0; v < 20; ++v) {
@@ -555,9 +794,10 @@ struct Problem {
               e2 += e * e;
               linf = std::max(linf, std::abs(e));
             }
-            if (!m.is_primary(b, i, j, k)) continue;
-            num += e2;
-            den += 1.0;
+            if (!prim) continue;
+            const double wv = m.forest() ? 1.0 / (double)(1L << (3 * m.level[b])) : 1.0;
+            num += m.forest() ? e2 * wv : e2;
+            den += wv;
           }
     l2 = den > 0 ? std::sqrt(num / den) : 0.0;
   }

@@ -279,7 +279,7 @@ double orc_point_draw(uint64_t base, const double* x, int v) { return point_draw
 double orc_smoothstep(double s) { return smoothstep(s); }
 
 // ---- the mesh problem ----
-void* orc_problem_create(const tgh_desc* d) {
+static Problem* problem_from(const tgh_desc* d) {
   auto* p = new Problem();
   p->mesh.q = d->q;
   p->mesh.N = d->N;
@@ -300,8 +300,42 @@ void* orc_problem_create(const tgh_desc* d) {
     p->bounds.K_max = d->K_max;
     p->bounds.r_gate = d->r_gate;
   }
+  return p;
+}
+void* orc_problem_create(const tgh_desc* d) {
+  Problem* p = problem_from(d);
   p->setup();
   return p;
+}
+// A frozen hierarchy: leaves as (level, i, j, k) quadruples, i, j, k the
+// leaf's coordinates in the brick of its level; p the prolongation order.
+void* orc_problem_create_forest(const tgh_desc* d, const int* leaves, int nleaves, int prolong) {
+  std::vector<std::array<int, 4>> lv;
+  for (int n = 0; n < nleaves; ++n) lv.push_back({leaves[4 * n], leaves[4 * n + 1], leaves[4 * n + 2], leaves[4 * n + 3]});
+  Problem* p = problem_from(d);
+  p->mesh.init_forest(lv, prolong);
+  p->setup();
+  return p;
+}
+int orc_block_level(void* p, long b) {
+  auto& m = ((Problem*)p)->mesh;
+  return m.forest() ? m.level[b] : 0;
+}
+// Where work point l of block b takes its value from: 0 primary, 1 copy,
+// 2 Dirichlet, 3 prolongation; src gets (block, loc or stencil start, half mask).
+int orc_forest_classify(void* pp, long b, long i, long j, long k, long* src) {
+  auto& m = ((Problem*)pp)->mesh;
+  long l[3] = {i, j, k};
+  Mesh::Src s;
+  int c = m.forest_classify(b, l, s);
+  src[0] = s.b;
+  for (int d = 0; d < 3; ++d) src[1 + d] = s.loc[d];
+  src[4] = s.half[0] | (s.half[1] << 1) | (s.half[2] << 2);
+  return c;
+}
+void orc_prolong_weights(void* pp, double* out) {
+  auto& m = ((Problem*)pp)->mesh;
+  for (size_t k = 0; k < m.wmid.size(); ++k) out[k] = m.wmid[k];
 }
 void orc_problem_destroy(void* p) { delete (Problem*)p; }
 long orc_state_length(void* p) { return ((Problem*)p)->mesh.state_size(); }

@@ -53,6 +53,10 @@ def lib():
             "orc_point_draw": (D, [C.c_uint64, dp, C.c_int]),
             "orc_smoothstep": (D, [D]),
             "orc_problem_create": (P, [desc]),
+            "orc_problem_create_forest": (P, [desc, C.POINTER(C.c_int), C.c_int, C.c_int]),
+            "orc_block_level": (C.c_int, [P, Lg]),
+            "orc_forest_classify": (C.c_int, [P, Lg, Lg, Lg, Lg, C.POINTER(Lg)]),
+            "orc_prolong_weights": (None, [P, dp]),
             "orc_problem_destroy": (None, [P]),
             "orc_state_length": (Lg, [P]),
             "orc_work_length": (Lg, [P]),
@@ -113,10 +117,32 @@ def metric_quantities(h):
 class OracleProblem:
     """The oracle's uniform-forest problem for one case descriptor."""
 
-    def __init__(self, desc):
+    def __init__(self, desc, leaves=None, prolongation=None):
+        """leaves: None (the uniform brick of desc.nblocks) or (level, i, j, k)
+        quadruples of a frozen hierarchy, with the prolongation order."""
         self.desc = desc
-        self.p = lib().orc_problem_create(C.byref(desc))
+        if leaves is None:
+            self.p = lib().orc_problem_create(C.byref(desc))
+        else:
+            flat = [int(x) for lf in leaves for x in lf]
+            arr = (C.c_int * len(flat))(*flat)
+            self.p = lib().orc_problem_create_forest(C.byref(desc), arr, len(leaves), int(prolongation))
         self.n = lib().orc_state_length(self.p)
+
+    def block_level(self, b):
+        return lib().orc_block_level(self.p, b)
+
+    def classify(self, b, i, j, k):
+        """(kind, (block, l0, l1, l2, half mask)): kind 0 primary, 1 copy,
+        2 Dirichlet, 3 prolongation."""
+        src = (C.c_long * 5)()
+        c = lib().orc_forest_classify(self.p, b, i, j, k, src)
+        return c, tuple(src)
+
+    def prolong_weights(self, p):
+        w = np.zeros(p)
+        lib().orc_prolong_weights(self.p, w)
+        return w
 
     def __del__(self):
         if getattr(self, "p", None):

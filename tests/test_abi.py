@@ -118,3 +118,40 @@ def test_float_type_first_refuses_other_types(tgh):
         tgh.gauge_wave_case("Float32", eps_ko=0.0, gamma0=1.0, gamma2=0.0)
     with pytest.raises(tgh.ArgumentError):
         tgh.Interior(r_0=1.0, r_1=0.5)
+
+
+def test_forest_checks_refuse_bad_leaf_lists(tgh):
+    """tgh_problem_create_forest checks the leaves before any device call: they
+    tile the box, are 2:1 balanced (across periodic seams too), N is even and
+    at least 2G, and the prolongation order fits the ghosts."""
+    import tgh_desc as TD
+    L = tgh._load()
+    good = [(1, i, j, k) for k in (0, 1) for j in (0, 1) for i in (0, 1)] + \
+           [(0, i, j, k) for k in (0, 1) for j in (0, 1) for i in (0, 1) if (i, j, k) != (0, 0, 0)]
+
+    def create(leaves, N=8, p=6):
+        d = tgh.TghDesc.from_buffer_copy(TD.gauge_wave_case(N=N, roots=2, q=4))
+        flat = [x for lf in leaves for x in lf]
+        arr = (ctypes.c_int32 * len(flat))(*flat)
+        ptr = ctypes.c_void_p()
+        rc = L.tgh_problem_create_forest(ctypes.byref(d), arr, len(leaves), p, ctypes.byref(ptr))
+        if rc == 0:
+            L.tgh_problem_destroy(ptr)
+        return rc, L.tgh_last_error().decode()
+
+    rc, msg = create(good)
+    assert rc != 1, msg  # passes the checks (no device here: rc 2)
+    overlap = good + [(0, 0, 0, 0)]
+    assert create(overlap) == (1, create(overlap)[1]) and "overlap" in create(overlap)[1]
+    hole = good[1:]
+    assert create(hole)[0] == 1 and "cover" in create(hole)[1]
+    # a level-2 leaf beside a level-0 one across the periodic seam
+    deep = [lf for lf in good if lf != (1, 0, 0, 0)] + \
+           [(2, i, j, k) for k in (0, 1) for j in (0, 1) for i in (0, 1)]
+    rc, msg = create(deep)
+    assert rc == 1 and "balanced" in msg
+    assert create(good, N=9)[0] == 1
+    assert create(good, N=4)[0] == 1
+    assert create(good, p=8)[0] == 1 and "prolongation" in create(good, p=8)[1]
+    assert create(good, p=3)[0] == 1
+    assert create([(0, 2, 0, 0)] + good[1:])[0] == 1

@@ -1,0 +1,214 @@
+"""Frozen hierarchies on the device: the planned ghost fill, the right-hand
+side, RK4 and the error norms against the oracle, through the C ABI, and the
+interface-order rule (test/interface_tests.jl:64-135) run on the GPU.
+
+Per component the right-hand side is compared against rhs_scale
+(test/evolution_tests.jl:97-108) with each block's own spacing: |GPU − oracle|
+/ scale ≤ 1e-12.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib
+import tgh_desc as TD
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+TOL = 1e-12
+RK4_FORMS = {"blocks-split": 4 | 8, "blocks-fused": 4 | 2, "per-block": 16, "default": 0}
+
+
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a visible GPU")
+    return torch
+
+
+def gauge_wave_forest(tgh, N=8, p=6, eps_ko=0.5):
+    case = tgh.gauge_wave_case(eps_ko=eps_ko, gamma0=1.0, gamma2=0.0)
+    f = tgh.gh_forest(None, case, N=N, roots=2, refined=True)
+    return case, f, p
+
+
+def hole_forest_case(tgh, p=6):
+    """A Kerr-Schild hole (a = 0.5, the closed-form source, :damped layer) on a
+    two-level hole_forest in a Dirichlet box: faces with the hook and coarse-
+    fine faces together."""
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    f = tgh.hole_forest(None, case, N=8, roots=4, radii=(1.5,))
+    return case, f, p
+
+
+def make_pair(tgh, case, f, p):
+    torch = torch_cuda()
+    dev = tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=p))
+    desc = TD.TghDesc.from_buffer_copy(dev.desc)
+    orc = oracle_lib.OracleProblem(desc, f.leaves, p)
+    assert dev.state_length == orc.n
+    levels = (tgh.C.c_int32 * f.nleaves())()
+    tgh._check(tgh._load().tgh_block_levels(dev.ptr, levels))
+    assert list(levels) == [lf[0] for lf in f.leaves]
+    return torch, dev, orc
+
+
+def rhs_scale_forest(work, f, q=4):
+    """rhs_scale at every owned point with each block's spacing."""
+    N = f.N
+    G = q // 2 + 1
+    S = N + 1 + 2 * G
+    nb = f.nleaves()
+    W = work.reshape(nb, 20, S, S, S)
+    _, _, w2 = oracle_lib.weights(q, 2)
+    r = q // 2
+    own = slice(G, G + N + 1)
+    scale = np.abs(W[:, :, own, own, own]).max(axis=1)
+    h = np.array([f.root_spacing() / 2 ** lf[0] for lf in f.leaves])[:, None, None, None, None]
+    for ax in (2, 3, 4):
+        acc = 0
+        for kk, wk in enumerate(w2):
+            o = kk - r
+            sl = [slice(None), slice(None), own, own, own]
+            sl[ax] = slice(G + o, G + o + N + 1)
+            acc = acc + wk * W[tuple(sl)]
+        scale = np.maximum(scale, (np.abs(acc) / (h * h)).max(axis=1))
+    return np.maximum(scale, 1.0)
+
+
+@pytest.mark.parametrize("which", ["gauge-wave", "hole"])
+def test_forest_fill_matches_oracle(tgh, which):
+    case, f, p = gauge_wave_forest(tgh) if which == "gauge-wave" else hole_forest_case(tgh)
+    torch, dev, orc = make_pair(tgh, case, f, p)
+    t = 0.05
+    u = orc.fill_exact(t)
+    orc.add_noise(u, 0xF0E57, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    tgh.prepare(dev, ud, t)
+    orc.prepare(u, t)
+    wd = dev.work().cpu().numpy()
+    wo = orc.work()
+    diff = np.abs(wd - wo) / np.maximum(np.abs(wo), 1.0)
+    print(f"{which}: {f.nleaves()} leaves, fill max rel diff {diff.max():.3e}")
+    assert diff.max() <= 1e-14
+
+
+@pytest.mark.parametrize("which", ["gauge-wave", "hole"])
+def test_forest_rhs_matches_oracle(tgh, which):
+    case, f, p = gauge_wave_forest(tgh) if which == "gauge-wave" else hole_forest_case(tgh)
+    torch, dev, orc = make_pair(tgh, case, f, p)
+    t = 0.05
+    u = orc.fill_exact(t)
+    orc.add_noise(u, 0xF0E58, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    du = torch.full_like(ud, float("nan"))
+    tgh.gh_rhs(du, ud, dev, t)
+    ref = orc.gh_rhs(u, t, NTHREADS)
+    scale = rhs_scale_forest(orc.work(), f)
+    P = f.N + 1
+    d = np.abs(du.cpu().numpy() - ref).reshape(f.nleaves(), 20, P, P, P) / scale[:, None]
+    print(f"{which}: rhs max |GPU - oracle| / rhs_scale = {d.max():.3e}")
+    assert np.isfinite(du.cpu().numpy()).all()
+    assert d.max() <= TOL
+
+
+@pytest.mark.parametrize("form", list(RK4_FORMS))
+def test_forest_rk4_matches_oracle(tgh, form):
+    case, f, p = gauge_wave_forest(tgh)
+    torch, dev, orc = make_pair(tgh, case, f, p)
+    u = orc.fill_exact(0.0)
+    orc.add_noise(u, 0xF0E59, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    lam = orc.max_speed(u)
+    assert abs(tgh.max_speed(dev, ud, 0.0) - lam) <= 1e-14 * lam
+    dt = 0.25 * f.minimum_spacing() / lam
+    for n in range(3):
+        orc.rk4_step(u, n * dt, dt, NTHREADS)
+    tgh._check(tgh._load().tgh_rk4(dev.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 3, RK4_FORMS[form]))
+    g = ud.cpu().numpy()
+    diff = float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max())
+    print(f"two-level gauge wave, 3 RK4 steps ({form}): max rel diff = {diff:.3e}")
+    assert diff <= 1e-12
+    l2g, lig = tgh.error_norms(dev, ud, 3 * dt)
+    l2o, lio = orc.error_norms(u, 3 * dt)
+    assert abs(l2g - l2o) <= 1e-9 * l2o and abs(lig - lio) <= 1e-9 * lio
+
+
+def test_forest_rk4_graph_replays_match_launches(tgh):
+    case, f, p = gauge_wave_forest(tgh)
+    torch, dev, orc = make_pair(tgh, case, f, p)
+    u0 = torch.from_numpy(orc.fill_exact(0.0)).cuda()
+    a = u0.clone()
+    tgh._check(tgh._load().tgh_rk4(dev.ptr, tgh.C.c_void_p(a.data_ptr()), 0.0, 0.005, 3, 0))
+    for rep in range(2):
+        b = u0.clone()
+        tgh._check(tgh._load().tgh_rk4(dev.ptr, tgh.C.c_void_p(b.data_ptr()), 0.0, 0.005, 3, 1))
+        torch.cuda.synchronize()
+        assert torch.equal(a, b), f"graph replay {rep} differs"
+
+
+def test_hole_forest_rk4_matches_oracle(tgh):
+    case, f, p = hole_forest_case(tgh)
+    torch, dev, orc = make_pair(tgh, case, f, p)
+    u = orc.fill_exact(0.0)
+    ud = torch.from_numpy(u.copy()).cuda()
+    lam = orc.max_speed(u)
+    dt = 0.25 * f.minimum_spacing() / lam
+    for n in range(2):
+        orc.rk4_step(u, n * dt, dt, NTHREADS)
+    tgh._check(tgh._load().tgh_rk4(dev.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, 0))
+    g = ud.cpu().numpy()
+    diff = float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max())
+    print(f"KS hole on a two-level hole_forest, 2 RK4 steps: max rel diff = {diff:.3e}")
+    assert diff <= 1e-12
+    l2g, lig = tgh.error_norms(dev, ud, 2 * dt)
+    l2o, lio = orc.error_norms(u, 2 * dt)
+    assert abs(l2g - l2o) <= 1e-9 * l2o and abs(lig - lio) <= 1e-9 * lio
+
+
+def _device_rates(tgh, p, refined):
+    torch = torch_cuda()
+    case = tgh.gauge_wave_case(eps_ko=0.0, gamma0=1.0, gamma2=0.0)
+    hs, l2, li = [], [], []
+    for N in (8, 10, 12):
+        f = tgh.gh_forest(None, case, N=N, roots=2, refined=refined)
+        if not refined:  # the control on the forest path: level-0 leaves
+            f = tgh.Forest(case, N, 2, leaves=f.leaves)
+        dev = tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=p))
+        u = dev.statevector()
+        tgh.initial_state(u, dev, 0.0)
+        dt = tgh.gh_dt(dev, u, cfl=0.25)
+        tgh.gh_solve(dev, u, (0.0, 1 / 16), dt=dt)
+        e2, ei = tgh.error_norms(dev, u, 1 / 16)
+        hs.append(f.minimum_spacing())
+        l2.append(e2)
+        li.append(ei)
+    x, y, z = np.log(hs), np.log(l2), np.log(li)
+    rate = lambda a, b: float(((a - a.mean()) * (b - b.mean())).sum() / ((a - a.mean()) ** 2).sum())
+    return dict(l2=l2, linf=li, rate_l2=rate(x, y), rate_linf=rate(x, z))
+
+
+def test_interface_order_rule_on_the_device(tgh):
+    """test/interface_tests.jl:64-96 on the GPU: rate about 3 at p = 4, at
+    least 3.75 at p = 6, errors falling with h and p = 6 the more accurate."""
+    r4 = _device_rates(tgh, 4, True)
+    r6 = _device_rates(tgh, 6, True)
+    print(f"interface order p=4: l2 {r4['l2']} rate {r4['rate_l2']:.3f} / {r4['rate_linf']:.3f}")
+    print(f"interface order p=6: l2 {r6['l2']} rate {r6['rate_l2']:.3f} / {r6['rate_linf']:.3f}")
+    assert 2.5 <= r4["rate_l2"] <= 3.6 and 2.5 <= r4["rate_linf"] <= 3.6
+    assert r6["rate_l2"] >= 3.75 and r6["rate_linf"] >= 3.75
+    assert r4["l2"] == sorted(r4["l2"], reverse=True)
+    assert r6["l2"] == sorted(r6["l2"], reverse=True)
+    assert all(a < b for a, b in zip(r6["l2"], r4["l2"]))
+
+
+def test_unrefined_control_on_the_device(tgh):
+    """test/interface_tests.jl:120-135: no coarse-fine face, no prolongation:
+    the p = 4 and p = 6 runs are the same computation and converge at 4."""
+    c4 = _device_rates(tgh, 4, False)
+    c6 = _device_rates(tgh, 6, False)
+    assert c4["rate_l2"] >= 3.75
+    assert c4["l2"] == c6["l2"]

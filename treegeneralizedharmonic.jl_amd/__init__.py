@@ -100,6 +100,9 @@ def _load():
     D = C.c_double
     sig = {
         "tgh_problem_create": (C.c_int, [C.POINTER(TghDesc), C.POINTER(P)]),
+        "tgh_problem_create_forest": (C.c_int, [C.POINTER(TghDesc), C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                                C.POINTER(P)]),
+        "tgh_block_levels": (C.c_int, [P, C.POINTER(C.c_int32)]),
         "tgh_problem_destroy": (C.c_int, [P]),
         "tgh_state_length": (C.c_int64, [P]),
         "tgh_work_length": (C.c_int64, [P]),
@@ -447,20 +450,254 @@ def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth
                   gamma0=damping, gamma2=gamma2, interior=inter, bounds=bounds, velocity=vel)
 
 
-class Forest:
-    """A uniform brick of root blocks of N cells (gh_forest's roots)."""
+def _morton(i, j, k):
+    key = 0
+    for bit in range(21):
+        key |= ((i >> bit) & 1) << (3 * bit) | ((j >> bit) & 1) << (3 * bit + 1) | ((k >> bit) & 1) << (3 * bit + 2)
+    return key
 
-    def __init__(self, case, N, roots):
+
+class Forest:
+    """gh_forest's forest (src/initialdata.jl:1370-1395): a brick of root blocks
+    of N cells (roots), uniform, or a frozen hierarchy whose leaves are
+    (level, i, j, k), block (i, j, k) of the brick of level l (roots_d 2^l
+    blocks of spacing h / 2^l). `leaves` is None on a uniform forest."""
+
+    def __init__(self, case, N, roots, leaves=None):
         roots = (roots,) * 3 if isinstance(roots, int) else tuple(roots)
         if len(roots) != 3 or any(r < 1 for r in roots):
             raise ArgumentError(f"roots is one count of root blocks for every dimension or a "
                                 f"brick of three counts, got {roots!r}")
         self.N, self.roots, self.case = int(N), roots, case
+        self._leaves = None if leaves is None else set(tuple(int(x) for x in lf) for lf in leaves)
+        if case is not None:
+            widths = [b[1] - b[0] for b in case.box]
+            cell = [widths[d] / roots[d] for d in range(3)]
+            if any(abs(c - cell[0]) > 1e-12 * abs(cell[0]) for c in cell):
+                raise ArgumentError(f"TreeAMR's blocks are cubes with a single spacing per level, so the "
+                                    f"root blocks must be cubes: the box's widths {widths} over the brick "
+                                    f"of roots {roots} give root blocks of widths {cell}")
+
+    @property
+    def refined(self):
+        return self._leaves is not None
+
+    @property
+    def leaves(self):
+        """The leaves in Morton order of their lower corners at the finest level."""
+        if self._leaves is None:
+            return [(0, i, j, k) for k in range(self.roots[2]) for j in range(self.roots[1])
+                    for i in range(self.roots[0])]
+        L = self.maxlevel()
+        return sorted(self._leaves, key=lambda lf: (_morton(*(c << (L - lf[0]) for c in lf[1:])), lf[0]))
+
+    def nleaves(self):
+        return (self.roots[0] * self.roots[1] * self.roots[2]) if self._leaves is None else len(self._leaves)
+
+    def maxlevel(self):
+        return 0 if self._leaves is None else max(lf[0] for lf in self._leaves)
+
+    def root_spacing(self):
+        lo, hi = self.case.box[0]
+        return (hi - lo) / (self.roots[0] * self.N)
+
+    def minimum_spacing(self):
+        return self.root_spacing() / 2 ** self.maxlevel()
+
+    def block_extent(self, leaf):
+        """((lo, hi) per axis) of leaf (level, i, j, k)."""
+        l = leaf[0]
+        h = self.root_spacing() / 2 ** l
+        return tuple((self.case.box[d][0] + leaf[1 + d] * self.N * h,
+                      self.case.box[d][0] + (leaf[1 + d] + 1) * self.N * h) for d in range(3))
+
+    def _containing(self, leafset, l, c):
+        """The leaf holding level-l block c: its ancestor of the lowest depth
+        that is a leaf, or None where the region is refined below l."""
+        for lv in range(l, -1, -1):
+            key = (lv,) + tuple(x >> (l - lv) for x in c)
+            if key in leafset:
+                return key
+        return None
+
+    def _neighbours(self, lf):
+        l = lf[0]
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    if dx == dy == dz == 0:
+                        continue
+                    c = [lf[1] + dx, lf[2] + dy, lf[3] + dz]
+                    ok = True
+                    for d in range(3):
+                        n = self.roots[d] << l
+                        if not 0 <= c[d] < n:
+                            if not self.case.periodic[d]:
+                                ok = False
+                            c[d] %= n
+                    if ok:
+                        yield tuple(c)
+
+    def refine(self, targets):
+        """refine!(forest, targets): each target leaf replaced by its 8 children."""
+        if self._leaves is None:
+            self._leaves = set(self.leaves)
+        for t in list(targets):
+            t = tuple(t)
+            if t not in self._leaves:
+                raise ArgumentError(f"{t} is not a leaf of the forest")
+            self._leaves.remove(t)
+            l, i, j, k = t
+            for c in range(8):
+                self._leaves.add((l + 1, 2 * i + (c & 1), 2 * j + (c >> 1 & 1), 2 * k + (c >> 2 & 1)))
+
+    def _unbalanced(self):
+        leafset = self._leaves or set()
+        bad = set()
+        for lf in leafset:
+            if lf[0] < 2:
+                continue
+            for c in self._neighbours(lf):
+                o = self._containing(leafset, lf[0], c)
+                if o is not None and o[0] < lf[0] - 1:
+                    bad.add(o)
+        return bad
+
+    def balance(self):
+        """balance!(forest): refine until leaves that touch (faces, edges,
+        corners, across periodic seams) differ by at most one level."""
+        while True:
+            bad = self._unbalanced()
+            if not bad:
+                return self
+            self.refine(bad)
+
+    def isbalanced(self):
+        return not self._unbalanced()
+
+    def leaf_array(self):
+        import numpy as np
+        return np.array(self.leaves, dtype=np.int32).reshape(-1, 4)
 
 
-def gh_forest(T=None, case=None, *, N, roots):
+def gh_forest(T=None, case=None, *, N, roots, refined=False):
+    """gh_forest(T, case; N, roots, refined = false) (src/initialdata.jl:1370-1395):
+    the uniform brick, or with `refined` the two-level mesh whose root block
+    holding lo + 3/8 of the box is refined once (every face of it a coarse-fine
+    face)."""
     _check_float(T)
-    return Forest(case, N, roots)
+    f = Forest(case, N, roots)
+    if not refined:
+        return f
+    xref = [case.box[d][0] + 3 * (case.box[d][1] - case.box[d][0]) / 8 for d in range(3)]
+    targets = [lf for lf in f.leaves
+               if all(f.block_extent(lf)[d][0] <= xref[d] < f.block_extent(lf)[d][1] for d in range(3))]
+    if len(targets) != 1:
+        raise ArgumentError(f"the two-level mesh refines exactly the one root block holding {xref}, but "
+                            f"{len(targets)} blocks claim it")
+    f.refine(targets)
+    return f.balance()
+
+
+def _box_meets_ball(ext, c, R):
+    d2 = sum(max(ext[d][0] - c[d], 0.0, c[d] - ext[d][1]) ** 2 for d in range(3))
+    return d2 <= R * R
+
+
+def _box_meets_cube(ext, c, R):
+    return all(ext[d][0] < c[d] + R and ext[d][1] > c[d] - R for d in range(3))
+
+
+def _box_meets_capsule(ext, c, dc, R):
+    # the distance from c + s dc to the box is convex in s: ternary search
+    def dist2(s):
+        return sum(max(ext[d][0] - (c[d] + s * dc[d]), 0.0, (c[d] + s * dc[d]) - ext[d][1]) ** 2
+                   for d in range(3))
+    a, b = 0.0, 1.0
+    for _ in range(200):
+        m1, m2 = a + (b - a) / 3, b - (b - a) / 3
+        if dist2(m1) <= dist2(m2):
+            b = m2
+        else:
+            a = m1
+    return min(dist2(0.0), dist2(1.0), dist2((a + b) / 2)) <= R * R
+
+
+def _box_meets_swept_cube(ext, c, dc, R):
+    # some s in [0, 1] with lo_d < c_d + s dc_d + R and hi_d > c_d + s dc_d - R
+    # on every axis: an intersection of open intervals in s
+    lo, hi = 0.0, 1.0
+    lo_open = hi_open = False
+    for d in range(3):
+        a, b = ext[d][0] - c[d] - R, ext[d][1] - c[d] + R  # need a < s dc < b
+        if dc[d] == 0.0:
+            if not (a < 0.0 < b):
+                return False
+            continue
+        s0, s1 = sorted((a / dc[d], b / dc[d]))
+        if s0 >= lo:
+            lo, lo_open = s0, True
+        if s1 <= hi:
+            hi, hi_open = s1, True
+    return lo < hi or (lo == hi and not (lo_open or hi_open))
+
+
+def hole_forest(T=None, case=None, *, N, roots, center=None, radii, levels=None, shape="ball", path=None):
+    """hole_forest(T, case; N, roots, center, radii, levels, shape, path)
+    (src/initialdata.jl:1469-1515): the uniform forest, then for each level
+    every leaf of the level below that meets shell radii[l] about center (a
+    closed ball, an open cube, or either swept along [center, center + path])
+    refined, and the forest balanced after each level."""
+    _check_float(T)
+    if shape not in ("ball", "cube"):
+        raise ArgumentError(f"a shell is a :ball (|x − c| ≤ R) or a :cube (|x − c|_∞ < R), got :{shape}")
+    if path is not None and len(path) != 3:
+        raise ArgumentError(f"path is None or the displacement Δc of the segment [c, c + Δc], three numbers, "
+                            f"got {path!r}")
+    radii = [float(r) for r in radii]
+    levels = len(radii) if levels is None else int(levels)
+    if len(radii) != levels:
+        raise ArgumentError(f"hole_forest takes one radius per refinement level, but levels = {levels} and "
+                            f"radii has {len(radii)} entries")
+    if not all(r > 0 for r in radii):
+        raise ArgumentError(f"every shell radius must be positive, got radii = {radii}")
+    if not all(radii[l] <= radii[l - 1] for l in range(1, levels)):
+        raise ArgumentError(f"the shells must nest, so the radii must not increase with the level, but "
+                            f"radii = {radii}")
+    if center is None:
+        center = case.interior.center if case.interior is not None else (0.0, 0.0, 0.0)
+    c = [float(x) for x in center]
+    f = Forest(case, N, roots)
+    for l in range(1, levels + 1):
+        R = radii[l - 1]
+        targets = []
+        for lf in f.leaves:
+            if lf[0] != l - 1:
+                continue
+            ext = f.block_extent(lf)
+            if path is None:
+                hit = _box_meets_ball(ext, c, R) if shape == "ball" else _box_meets_cube(ext, c, R)
+            else:
+                dc = [float(x) for x in path]
+                hit = _box_meets_capsule(ext, c, dc, R) if shape == "ball" else _box_meets_swept_cube(ext, c, dc, R)
+            if hit:
+                targets.append(lf)
+        if not targets:
+            raise ArgumentError(f"no leaf of level {l - 1} comes within radii[{l}] = {R} of {c}, so shell {l} "
+                                f"would refine nothing")
+        f.refine(targets)
+        f.balance()
+    return f
+
+
+class Operators:
+    """TreeAMR's Operators(prolongation = p, restriction = r): the ghost
+    schedule's interpolation orders. Restriction on a vertex-centred mesh is
+    injection (a coincident fine point copied), whatever r says."""
+
+    def __init__(self, *, prolongation, restriction=None):
+        self.prolongation = int(prolongation)
+        self.restriction = None if restriction is None else int(restriction)
 
 
 def _desc(forest, case, q):
@@ -510,22 +747,33 @@ class GHProblem:
     """GHProblem(forest, case; q, backend=MI355X()) (src/evolution.jl:1034-1149):
     the device work arrays, the block table and the case, built once."""
 
-    def __init__(self, forest, case, *, q, backend=None):
+    def __init__(self, forest, case, *, q, backend=None, ops=None):
         if not (q >= 2 and q % 2 == 0):
             raise ArgumentError(f"the finite-difference order must be even and at least 2, so "
                                 f"that the centered stencils have an integer half-width q/2, but q={q}")
         self.backend = backend if backend is not None else MI355X()
-        self.forest, self.case, self.q = forest, case, int(q)
+        self.forest, self.case, self.q, self.ops = forest, case, int(q), ops
         self.desc = _desc(forest, case, q)
         self._ptr = C.c_void_p()
         import torch
+        if forest.refined and ops is None:
+            raise ArgumentError(f"a forest with levels needs its ghost operators spelled out, "
+                                f"ops = Operators(prolongation = p): the interface-order rule asks for "
+                                f"p = q + 2 = {q + 2}, and it is never defaulted")
         with torch.cuda.device(self.backend.device):
-            _check(_load().tgh_problem_create(C.byref(self.desc), C.byref(self._ptr)))
+            if forest.refined:
+                leaves = forest.leaf_array()
+                arr = leaves.ravel()
+                _check(_load().tgh_problem_create_forest(
+                    C.byref(self.desc), arr.ctypes.data_as(C.POINTER(C.c_int32)), len(leaves),
+                    ops.prolongation, C.byref(self._ptr)))
+            else:
+                _check(_load().tgh_problem_create(C.byref(self.desc), C.byref(self._ptr)))
             # order the library's work after torch's on the same stream
             _check(_load().tgh_set_stream(self._ptr, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
         self.state_length = int(_load().tgh_state_length(self._ptr))
         self.work_length = int(_load().tgh_work_length(self._ptr))
-        self.nblocks = forest.roots[0] * forest.roots[1] * forest.roots[2]
+        self.nblocks = forest.nleaves()
 
     def __del__(self):
         if getattr(self, "_ptr", None) and self._ptr.value:
@@ -645,9 +893,7 @@ def error_norms(p, u, t):
 def gh_dt(p, u, *, cfl, t=0.0):
     """gh_dt(p, u; cfl) = cfl · h_min / λ_max (src/evolution.jl:1511-1521)."""
     lam = max_speed(p, u, t)
-    lo, hi = p.case.box[0]
-    h = (hi - lo) / (p.forest.roots[0] * p.forest.N)
-    return float(cfl) * h / lam
+    return float(cfl) * p.forest.minimum_spacing() / lam
 
 
 def gh_solve(p, u, tspan, *, dt, graph=False):

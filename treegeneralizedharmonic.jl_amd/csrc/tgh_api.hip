@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "tgh_forest.hpp"
 #include "tgh_problem.hpp"
 
 using namespace tgh;
@@ -62,7 +63,9 @@ dim3 grid_work(const tgh_problem* p) {
 
 namespace {
 
-int check_desc(const tgh_desc* d) {
+// lmax: the finest level of a forest (0 on a uniform one); the radius checks
+// take its spacing, the one at a hole that hole_forest builds around.
+int check_desc(const tgh_desc* d, int lmax = 0) {
   if (!d) return fail(TGH_EINVAL, "desc is NULL");
   if (d->float_bits != 64)
     return fail(TGH_EINVAL, "this library computes in Float64 only (the horizon interior and the "
@@ -85,6 +88,7 @@ int check_desc(const tgh_desc* d) {
       return fail(TGH_EINVAL, "the blocks' cells must be cubic: the box widths over the brick of blocks "
                               "give different spacings per axis");
   }
+  h0 /= (double)(1 << lmax);
   if (d->background < 0 || d->background > 4) return fail(TGH_EINVAL, "unknown background");
   if (d->gauge < 0 || d->gauge > 2) return fail(TGH_EINVAL, "unknown gauge source kind");
   if (d->gauge == TGH_GAUGE_SAMPLED && d->background == TGH_BG_GAUGE_WAVE)
@@ -192,6 +196,25 @@ int check_desc(const tgh_desc* d) {
 }
 
 void launch_fill(tgh_problem* p, double* W, double t) {
+  if (p->forest) {
+    // the plan's passes, in order on the stream
+    PlanArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = p->geom;
+    a.c = p->cs;
+    a.t = t;
+    a.W = W;
+    a.np = p->prolong;
+    for (int k = 0; k < 10; ++k) a.wm[k] = p->wmid[k];
+    for (size_t s = 0; s + 1 < p->plan_off.size(); ++s) {
+      a.n = p->plan_off[s + 1] - p->plan_off[s];
+      if (a.n == 0) continue;
+      a.ops = (const PlanOp*)p->d_plan + p->plan_off[s];
+      hipLaunchKernelGGL(p->cs.moving ? k_fill_plan<true> : k_fill_plan<false>, dim3((a.n + 255) / 256), dim3(256),
+                         0, p->stream, a);
+    }
+    return;
+  }
   FillArgs a{p->geom, p->cs, t, W};
   hipLaunchKernelGGL(p->cs.moving ? k_fill<true> : k_fill<false>, grid_work(p), dim3(256), 0, p->stream, a);
 }
@@ -480,11 +503,28 @@ extern "C" {
 
 const char* tgh_last_error(void) { return g_last_error.c_str(); }
 
-int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
+// The problem on a uniform forest (leaves NULL) or on a forest with levels.
+static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nleaves, int32_t prolong,
+                          tgh_problem** out) {
   if (!out) return fail(TGH_EINVAL, "out is NULL");
   *out = nullptr;
-  int rc = check_desc(d);
+  ForestPlan plan;
+  std::vector<int> lev, fbc;
+  int lmax = 0;
+  if (leaves) {
+    for (int32_t n = 0; n < nleaves; ++n) {
+      lev.push_back(leaves[4 * n]);
+      for (int c = 0; c < 3; ++c) fbc.push_back(leaves[4 * n + 1 + c]);
+      lmax = std::max(lmax, std::min(std::max(leaves[4 * n], 0), 12));
+    }
+  }
+  int rc = check_desc(d, lmax);
   if (rc) return rc;
+  if (leaves) {
+    const int per[3] = {d->periodic[0] ? 1 : 0, d->periodic[1] ? 1 : 0, d->periodic[2] ? 1 : 0};
+    const std::string msg = forest_plan(d->N, d->q / 2 + 1, d->nblocks, per, prolong, lev, fbc, plan);
+    if (!msg.empty()) return fail(TGH_EINVAL, msg);
+  }
   auto* p = new tgh_problem();
   p->desc = *d;
   p->q = d->q;
@@ -498,22 +538,34 @@ int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
     g.periodic[i] = d->periodic[i] ? 1 : 0;
     g.lo[i] = d->lo[i];
   }
-  g.nblocks = d->nblocks[0] * d->nblocks[1] * d->nblocks[2];
+  g.nblocks = leaves ? nleaves : d->nblocks[0] * d->nblocks[1] * d->nblocks[2];
   g.h = (d->hi[0] - d->lo[0]) / ((double)d->nblocks[0] * d->N);
   g.inv_h = 1.0 / g.h;
-  // Blocks in Morton order of the brick.
-  std::vector<std::array<int, 3>> bl;
-  for (int k = 0; k < g.nb[2]; ++k)
-    for (int j = 0; j < g.nb[1]; ++j)
-      for (int i = 0; i < g.nb[0]; ++i) bl.push_back({i, j, k});
-  std::stable_sort(bl.begin(), bl.end(), [](const auto& a, const auto& b) {
-    return morton3(a[0], a[1], a[2]) < morton3(b[0], b[1], b[2]);
-  });
-  p->bcoord_host.resize(3 * bl.size());
-  std::vector<int> lookup((size_t)g.nblocks);
-  for (size_t b = 0; b < bl.size(); ++b) {
-    for (int d2 = 0; d2 < 3; ++d2) p->bcoord_host[3 * b + d2] = bl[b][d2];
-    lookup[bl[b][0] + g.nb[0] * (bl[b][1] + g.nb[1] * bl[b][2])] = (int)b;
+  std::vector<int> lookup;
+  if (leaves) {
+    // a forest: the leaves in the caller's order
+    p->forest = true;
+    p->bcoord_host = fbc;
+    p->level_host = lev;
+    p->prolong = prolong;
+    midpoint_weights(prolong, p->wmid);
+    p->plan_off = plan.pass_off;
+    lookup.assign(1, 0);
+  } else {
+    // Blocks in Morton order of the brick.
+    std::vector<std::array<int, 3>> bl;
+    for (int k = 0; k < g.nb[2]; ++k)
+      for (int j = 0; j < g.nb[1]; ++j)
+        for (int i = 0; i < g.nb[0]; ++i) bl.push_back({i, j, k});
+    std::stable_sort(bl.begin(), bl.end(), [](const auto& a, const auto& b) {
+      return morton3(a[0], a[1], a[2]) < morton3(b[0], b[1], b[2]);
+    });
+    p->bcoord_host.resize(3 * bl.size());
+    lookup.assign((size_t)g.nblocks, 0);
+    for (size_t b = 0; b < bl.size(); ++b) {
+      for (int d2 = 0; d2 < 3; ++d2) p->bcoord_host[3 * b + d2] = bl[b][d2];
+      lookup[bl[b][0] + g.nb[0] * (bl[b][1] + g.nb[1] * bl[b][2])] = (int)b;
+    }
   }
   CaseP& c = p->cs;
   std::memset(&c, 0, sizeof(c));
@@ -602,6 +654,22 @@ int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
     return cleanup(TGH_EDEVICE, hipGetErrorString(e));
   g.bcoord = p->d_bcoord;
   g.lookup = p->d_lookup;
+  g.blevel = nullptr;
+  g.prim = nullptr;
+  if (p->forest) {
+    if ((e = hipMalloc(&p->d_level, sizeof(int) * p->level_host.size())) != hipSuccess ||
+        (e = hipMalloc(&p->d_prim, plan.prim.size())) != hipSuccess ||
+        (e = hipMalloc(&p->d_plan, sizeof(int) * std::max<size_t>(plan.ops.size(), 4))) != hipSuccess)
+      return cleanup(TGH_EDEVICE, std::string("allocating the forest's tables: ") + hipGetErrorString(e));
+    if ((e = hipMemcpy(p->d_level, p->level_host.data(), sizeof(int) * p->level_host.size(),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(p->d_prim, plan.prim.data(), plan.prim.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (!plan.ops.empty() && (e = hipMemcpy(p->d_plan, plan.ops.data(), sizeof(int) * plan.ops.size(),
+                                             hipMemcpyHostToDevice)) != hipSuccess))
+      return cleanup(TGH_EDEVICE, hipGetErrorString(e));
+    g.blevel = p->d_level;
+    g.prim = p->d_prim;
+  }
   if (c.gauge == GAUGE_SAMPLED) {
     // The sample is taken once, here, at t = 0: the background is static (a
     // moving non-harmonic background is refused, src/initialdata.jl GHCase).
@@ -616,6 +684,22 @@ int tgh_problem_create(const tgh_desc* d, tgh_problem** out) {
   return TGH_OK;
 }
 
+int tgh_problem_create(const tgh_desc* d, tgh_problem** out) { return create_problem(d, nullptr, 0, 0, out); }
+
+int tgh_problem_create_forest(const tgh_desc* d, const int32_t* leaves, int32_t nleaves, int32_t prolongation,
+                              tgh_problem** out) {
+  if (!out) return fail(TGH_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!leaves || nleaves < 1) return fail(TGH_EINVAL, "a forest needs a leaf list of at least one leaf");
+  return create_problem(d, leaves, nleaves, prolongation, out);
+}
+
+int tgh_block_levels(const tgh_problem* p, int32_t* out) {
+  if (!p || !out) return fail(TGH_EINVAL, "NULL argument");
+  for (int b = 0; b < p->geom.nblocks; ++b) out[b] = p->forest ? p->level_host[b] : 0;
+  return TGH_OK;
+}
+
 int tgh_problem_destroy(tgh_problem* p) {
   if (!p) return TGH_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
@@ -624,6 +708,9 @@ int tgh_problem_destroy(tgh_problem* p) {
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
   (void)hipFree(p->d_bcoord);
   (void)hipFree(p->d_lookup);
+  (void)hipFree(p->d_level);
+  (void)hipFree(p->d_prim);
+  (void)hipFree(p->d_plan);
   (void)hipFree(p->W[0]);
   (void)hipFree(p->W[1]);
   (void)hipFree(p->acc);
@@ -794,7 +881,7 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
   if (nsteps == 0) return TGH_OK;
   const bool graph = (flags & TGH_RK4_GRAPH) != 0;
-  const bool blocks = (flags & TGH_RK4_BLOCKS) != 0;
+  const bool blocks = (flags & TGH_RK4_BLOCKS) != 0 || p->forest;  // a forest steps on the block layout
   if (flags & ~(TGH_RK4_GRAPH | TGH_RK4_FUSED | TGH_RK4_BLOCKS | TGH_RK4_SPLIT | TGH_RK4_PER_BLOCK))
     return fail(TGH_EINVAL, "unknown tgh_rk4 flags " + std::to_string(flags));
   if ((flags & TGH_RK4_PER_BLOCK) && (flags & TGH_RK4_SPLIT))
@@ -1055,6 +1142,9 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
                                   std::to_string(i) + " differs from problem 0");
     if (p->cs.interior == INT_PASTED)
       return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
+    if (p->forest)
+      return fail(TGH_EUNSUPPORTED, "problem " + std::to_string(i) + " of the batch is a forest with levels, "
+                  "which steps on the block layout; a batch steps bricks");
   }
   tgh_batch* b = new tgh_batch;
   b->probs.assign(probs, probs + n);

@@ -1,0 +1,254 @@
+// A frozen hierarchy of closed vertex blocks (TreeAMR 0.3's forest as
+// gh_forest(; refined) and hole_forest build it, src/initialdata.jl:1370-1515):
+// the host-side checks of a leaf list and the plan of its ghost fill, which
+// k_fill_plan executes.
+//
+// A leaf (l, i, j, k) is block (i, j, k) of the brick of level l: nb_d 2^l
+// blocks of N cells of spacing h / 2^l along axis d. The rules of the fill:
+//  * every point has an owner, the finest leaf that holds it as a vertex;
+//    among leaves of one level, the one where the point's local index is
+//    below N on the most significant axes (the uniform rule: an axis where
+//    it is N ranks after one where it is not, x lowest);
+//  * a work point that is not its block's primary takes the owner's value:
+//    a same-level copy, an injection from a finer leaf, a coincident coarse
+//    vertex; an owned face point of a coarse leaf included;
+//  * a point that no leaf holds as a vertex (a coarse midpoint on one or
+//    more axes) is the order-p Lagrange interpolation of the level-(l-1)
+//    leaf that holds it, p points centred on the midpoint along each such
+//    axis, tensor product;
+//  * outside a non-periodic axis, the Dirichlet hook.
+// The checks: the leaves tile the box exactly, neighbours (faces, edges and
+// corners, across periodic seams) differ by at most one level, N is even and
+// at least 2G, and the prolongation order is even and fits the ghosts.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace tgh {
+
+struct ForestPlan {
+  std::vector<int> ops;            // 4 ints per entry: dst, src, mode, 0
+  std::vector<int> pass_off;       // entry offsets of the passes (Lmax + 2 values)
+  std::vector<unsigned char> prim; // per owned point: 1 if its block's primary
+  int Lmax = 0;
+};
+
+// Builds the plan, or returns an error message.
+inline std::string forest_plan(int N, int G, const int nb[3], const int periodic[3], int np,
+                               const std::vector<int>& lev, const std::vector<int>& bc, ForestPlan& out) {
+  const int nl = (int)lev.size();
+  if (nl < 1) return "a forest needs at least one leaf";
+  if (N % 2 != 0 || N < 2 * G)
+    return "a forest with levels needs an even N of at least 2G = " + std::to_string(2 * G) +
+           " (a coarse vertex is a fine one, and a ghost reaches no further than the next leaf), got N = " +
+           std::to_string(N);
+  if (np < 2 || np % 2 != 0 || np / 2 > G)
+    return "the prolongation order must be even, at least 2 and at most 2G = " + std::to_string(2 * G) +
+           " so that its stencil stays inside the coarse block's ghosts, got " + std::to_string(np);
+  int Lmax = 0;
+  for (int b = 0; b < nl; ++b) {
+    if (lev[b] < 0 || lev[b] > 12) return "leaf " + std::to_string(b) + " has level " + std::to_string(lev[b]) +
+                                          ", outside 0 .. 12";
+    Lmax = lev[b] > Lmax ? lev[b] : Lmax;
+    for (int d = 0; d < 3; ++d)
+      if (bc[3 * b + d] < 0 || bc[3 * b + d] >= (nb[d] << lev[b]))
+        return "leaf " + std::to_string(b) + " lies outside the brick of its level";
+  }
+  // The cell map: the leaf of every block of the finest level.
+  long nc[3];
+  for (int d = 0; d < 3; ++d) nc[d] = (long)nb[d] << Lmax;
+  const long ncell = nc[0] * nc[1] * nc[2];
+  if (ncell > (1L << 26)) return "the forest's finest brick has too many blocks for the host's cell map";
+  std::vector<int> cell(ncell, -1);
+  for (int b = 0; b < nl; ++b) {
+    const long f = 1L << (Lmax - lev[b]);
+    const long c0[3] = {bc[3 * b] * f, bc[3 * b + 1] * f, bc[3 * b + 2] * f};
+    for (long z = c0[2]; z < c0[2] + f; ++z)
+      for (long y = c0[1]; y < c0[1] + f; ++y)
+        for (long x = c0[0]; x < c0[0] + f; ++x) {
+          int& c = cell[x + nc[0] * (y + nc[1] * z)];
+          if (c >= 0) return "leaves " + std::to_string(c) + " and " + std::to_string(b) + " overlap";
+          c = b;
+        }
+  }
+  for (long c = 0; c < ncell; ++c)
+    if (cell[c] < 0) return "the leaves do not cover the box";
+  // 2:1 balance over the 13 forward neighbour directions.
+  for (long z = 0; z < nc[2]; ++z)
+    for (long y = 0; y < nc[1]; ++y)
+      for (long x = 0; x < nc[0]; ++x) {
+        const int a = cell[x + nc[0] * (y + nc[1] * z)];
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+              if (dz * 9 + dy * 3 + dx <= 0) continue;
+              long q[3] = {x + dx, y + dy, z + dz};
+              bool in = true;
+              for (int d = 0; d < 3; ++d) {
+                if (q[d] < 0 || q[d] >= nc[d]) {
+                  if (!periodic[d]) in = false;
+                  q[d] = (q[d] + nc[d]) % nc[d];
+                }
+              }
+              if (!in) continue;
+              const int o = cell[q[0] + nc[0] * (q[1] + nc[1] * q[2])];
+              if (lev[a] - lev[o] > 1 || lev[o] - lev[a] > 1)
+                return "the forest is not 2:1 balanced: leaves " + std::to_string(a) + " (level " +
+                       std::to_string(lev[a]) + ") and " + std::to_string(o) + " (level " + std::to_string(lev[o]) +
+                       ") touch";
+            }
+      }
+
+  const int S = N + 1 + 2 * G, P = N + 1;
+  const long S3 = (long)S * S * S;
+  if ((long)nl * S3 >= (1L << 31)) return "the forest has too many points for the fill plan's 32-bit indices";
+  out.Lmax = Lmax;
+  out.prim.assign((size_t)nl * P * P * P, 0);
+  std::vector<std::vector<int>> pass(Lmax + 1);
+  // The leaves whose closed extent holds finest-level point F: those of the
+  // one or two finest blocks along each axis around it.
+  auto around = [&](const long F[3], int* cand) {
+    long lo[3], n[3];
+    for (int d = 0; d < 3; ++d) {
+      const long tot = nc[d] * N;
+      lo[d] = F[d] / N;
+      n[d] = 1;
+      if (F[d] % N == 0) {
+        if (F[d] == tot) lo[d] = nc[d] - 1;           // the top of a non-periodic axis
+        else if (F[d] > 0 || periodic[d]) { lo[d] -= 1; n[d] = 2; }
+      }
+    }
+    int k = 0;
+    for (long z = 0; z < n[2]; ++z)
+      for (long y = 0; y < n[1]; ++y)
+        for (long x = 0; x < n[0]; ++x) {
+          const long q[3] = {(lo[0] + x + nc[0]) % nc[0], (lo[1] + y + nc[1]) % nc[1], (lo[2] + z + nc[2]) % nc[2]};
+          const int c = cell[q[0] + nc[0] * (q[1] + nc[1] * q[2])];
+          bool seen = false;
+          for (int i = 0; i < k; ++i) seen = seen || cand[i] == c;
+          if (!seen) cand[k++] = c;
+        }
+    return k;
+  };
+  // Local index of level-lv point gv in leaf c (across a periodic seam too).
+  auto local = [&](int c, int d, long gv) {
+    const long tot = ((long)nb[d] << lev[c]) * N;
+    long loc = gv - (long)bc[3 * c + d] * N;
+    if (loc < 0) loc += tot;
+    if (loc > N) loc -= tot;
+    return loc;
+  };
+  for (int b = 0; b < nl; ++b) {
+    const int L = lev[b];
+    for (int k = -G; k <= N + G; ++k)
+      for (int j = -G; j <= N + G; ++j)
+        for (int i = -G; i <= N + G; ++i) {
+          const int l[3] = {i, j, k};
+          const int dst = (int)((long)b * S3 + (i + G) + (long)S * ((j + G) + (long)S * (k + G)));
+          long gl[3];
+          bool outside = false;
+          for (int d = 0; d < 3; ++d) {
+            const long tot = ((long)nb[d] << L) * N;
+            gl[d] = (long)bc[3 * b + d] * N + l[d];
+            if (periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
+            else if (gl[d] < 0 || gl[d] > tot) outside = true;
+          }
+          if (outside) {
+            pass[0].insert(pass[0].end(), {dst, 0, 8, 0});
+            continue;
+          }
+          long F[3];
+          for (int d = 0; d < 3; ++d) F[d] = gl[d] << (Lmax - L);
+          int cand[8];
+          const int nc8 = around(F, cand);
+          // the owner: the finest candidate holding F as a vertex, then the
+          // smallest mask of axes with local index N
+          int own = -1, own_mask = 8;
+          long own_loc[3] = {0, 0, 0};
+          for (int q = 0; q < nc8; ++q) {
+            const int c = cand[q];
+            const long f = 1L << (Lmax - lev[c]);
+            if (F[0] % f || F[1] % f || F[2] % f) continue;
+            long loc[3];
+            int mask = 0;
+            for (int d = 0; d < 3; ++d) {
+              loc[d] = local(c, d, F[d] / f);
+              if (loc[d] == N) mask |= 1 << d;
+            }
+            if (own < 0 || lev[c] > lev[own] || (lev[c] == lev[own] && mask < own_mask)) {
+              own = c;
+              own_mask = mask;
+              for (int d = 0; d < 3; ++d) own_loc[d] = loc[d];
+            }
+          }
+          if (own >= 0) {
+            if (own == b && own_loc[0] == i && own_loc[1] == j && own_loc[2] == k) {
+              if (i >= 0 && i <= N && j >= 0 && j <= N && k >= 0 && k <= N)
+                out.prim[(size_t)b * P * P * P + i + P * (j + (size_t)P * k)] = 1;
+              continue;
+            }
+            const int src = (int)((long)own * S3 + (own_loc[0] + G) + (long)S * ((own_loc[1] + G) +
+                                                                                  (long)S * (own_loc[2] + G)));
+            pass[0].insert(pass[0].end(), {dst, src, 0, 0});
+            continue;
+          }
+          // a coarse midpoint: the level-(L-1) candidate, smallest mask
+          if (L == 0) return "a level-0 point that no leaf holds as a vertex";
+          int cb = -1, cb_mask = 8;
+          long start[3] = {0, 0, 0};
+          int half = 0;
+          for (int d = 0; d < 3; ++d) half |= (int)(gl[d] & 1) << d;
+          for (int q = 0; q < nc8; ++q) {
+            const int c = cand[q];
+            if (lev[c] != L - 1) continue;
+            long st[3];
+            int mask = 0;
+            for (int d = 0; d < 3; ++d) {
+              if (half >> d & 1) {
+                st[d] = local(c, d, (gl[d] - 1) / 2) - np / 2 + 1;
+              } else {
+                st[d] = local(c, d, gl[d] / 2);
+                if (st[d] == N) mask |= 1 << d;
+              }
+            }
+            if (cb < 0 || mask < cb_mask) {
+              cb = c;
+              cb_mask = mask;
+              for (int d = 0; d < 3; ++d) start[d] = st[d];
+            }
+          }
+          if (cb < 0) return "a ghost of a level-" + std::to_string(L) + " leaf lies in no leaf of level " +
+                             std::to_string(L - 1);
+          for (int d = 0; d < 3; ++d) {
+            const long hi = start[d] + ((half >> d & 1) ? np - 1 : 0);
+            if (start[d] < -G || hi > N + G) return "a prolongation stencil leaves its coarse block's ghosts";
+          }
+          const int src = (int)((long)cb * S3 + (start[0] + G) + (long)S * ((start[1] + G) + (long)S * (start[2] + G)));
+          pass[L].insert(pass[L].end(), {dst, src, half, 0});
+        }
+  }
+  out.ops.clear();
+  out.pass_off.assign(1, 0);
+  for (int l = 0; l <= Lmax; ++l) {
+    out.ops.insert(out.ops.end(), pass[l].begin(), pass[l].end());
+    out.pass_off.push_back((int)(out.ops.size() / 4));
+  }
+  return "";
+}
+
+// The midpoint Lagrange weights of p nodes at -p/2+1 .. p/2: each a product
+// of exact factors, divided once.
+inline void midpoint_weights(int np, double* w) {
+  for (int k = 0; k < np; ++k) {
+    double num = 1.0, den = 1.0;
+    for (int j = 0; j < np; ++j) {
+      if (j == k) continue;
+      num *= 0.5 - (double)(j - np / 2 + 1);
+      den *= (double)(k - j);
+    }
+    w[k] = num / den;
+  }
+}
+
+}  // namespace tgh

@@ -27,10 +27,19 @@ struct Geom {
   int periodic[3];
   int nblocks;
   double lo[3];
-  double h, inv_h;
-  const int* bcoord;       // 3 ints per block
-  const int* lookup;       // brick index -> block
+  double h, inv_h;         // level 0's spacing
+  const int* bcoord;       // 3 ints per block (in the brick of its level)
+  const int* lookup;       // brick index -> block (a uniform forest)
+  // A frozen hierarchy: per-block levels (spacing h / 2^level) and per owned
+  // point a primary flag (nblocks P^3 bytes); both null on a uniform forest.
+  const int* blevel;
+  const unsigned char* prim;
 };
+
+TGH_HD int block_level(const Geom& g, int b) { return g.blevel ? g.blevel[b] : 0; }
+// h / 2^l and 2^l / h: scalings by powers of two, exact.
+TGH_HD double block_h(const Geom& g, int b) { return g.h / (double)(1 << block_level(g, b)); }
+TGH_HD double block_inv_h(const Geom& g, int b) { return g.inv_h * (double)(1 << block_level(g, b)); }
 
 struct CaseP {
   Bg bg;
@@ -49,9 +58,10 @@ struct CaseP {
 
 TGH_HD void position(const Geom& g, int b, int i, int j, int k, double x[3]) {
   const int* B = g.bcoord + 3 * b;
-  x[0] = g.lo[0] + (double)(B[0] * g.N + i) * g.h;
-  x[1] = g.lo[1] + (double)(B[1] * g.N + j) * g.h;
-  x[2] = g.lo[2] + (double)(B[2] * g.N + k) * g.h;
+  const double h = block_h(g, b);
+  x[0] = g.lo[0] + (double)(B[0] * g.N + i) * h;
+  x[1] = g.lo[1] + (double)(B[1] * g.N + j) * h;
+  x[2] = g.lo[2] + (double)(B[2] * g.N + k) * h;
 }
 
 // center_at (src/interior.jl:86-93): c(t) = c0 + v t. It and the radius and
@@ -493,6 +503,7 @@ struct PointRef {
   long sv;
   int s2, s3;
   double x[3];
+  double ih;  // 1/h of the point's block (the block layout)
   long si0, ssv, wi0;
   const double* __restrict__ Hp;
   long hs;
@@ -807,6 +818,7 @@ __device__ __forceinline__ bool block_point(const RhsArgs& a, const double* __re
   pr.s2 = S;
   pr.s3 = S * S;
   position(g, b, i, j, k, pr.x);
+  pr.ih = block_inv_h(g, b);
   pr.si0 = (long)b * 20 * psz + p;
   pr.ssv = psz;
   pr.wi0 = (long)b * 20 * sv + off;
@@ -825,7 +837,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict
   PointRef pr;
   long pi;
   if (!block_point(a, W, pr, pi)) return;
-  rhs_point<Q, GAUGE, HOLE, EPI>(a, pr, a.g.inv_h, TGH_OUT_ARGS);
+  rhs_point<Q, GAUGE, HOLE, EPI>(a, pr, pr.ih, TGH_OUT_ARGS);
 }
 
 // The split form, first half: the head at one owned point per thread, leaving
@@ -839,7 +851,7 @@ __global__ __launch_bounds__(256) void k_head(RhsArgs a, const double* __restric
   const long cs = (long)a.g.nblocks * a.g.P * a.g.P * a.g.P;
   PiCoef pc;
   const GlobalSrc src{pr.w0, pr.sv, pr.s2, pr.s3};
-  if (head_point<Q, GAUGE, HOLE, EPI>(a, pr, src, a.g.inv_h, pc, TGH_OUT_ARGS))
+  if (head_point<Q, GAUGE, HOLE, EPI>(a, pr, src, pr.ih, pc, TGH_OUT_ARGS))
     store_coef(coef + pi, cs, pc);
   else
     store_coef_source(coef + pi, cs, pc);  // w = -1 and the core's d_t Pi
@@ -867,7 +879,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_PI_WAVE
       emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + (NC + v) * psz, pr.wi0 + (NC + v) * sv, cf0[(CF_C1 + v) * cs]);
     return;
   }
-  const double ih = a.g.inv_h, ih2 = ih * ih, epsh = pc.epsh;
+  const double ih = pr.ih, ih2 = ih * ih, epsh = pc.epsh;
 #pragma unroll 1
   for (int v = 0; v < NC; ++v) {
     const double* ph = pr.w0 + v * sv;
@@ -911,6 +923,72 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
     const double* src = a.W + (long)owner * 20 * sv + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G));
 #pragma unroll
     for (int v = 0; v < 20; ++v) dst[v * sv] = src[v * sv];
+  }
+}
+
+// The ghost fill of a forest with levels, from a plan the host builds once
+// (one entry per work point that is not its block's primary, grouped in
+// passes that the stream orders): pass 0 holds every copy (same-level,
+// injection from a finer leaf, coincident coarse vertex: each reads a
+// primary, which no entry writes) and every Dirichlet point; pass l >= 1 the
+// prolongations into the ghosts of level-l blocks, each reading one level-
+// (l-1) block's points, filled by the passes before. An entry is
+//   dst = b S^3 + p  (block and work point), src = b' S^3 + p' (the source
+//   point, or the prolongation stencil's first point), mode: bits 0-2 the
+//   axes on which the point is a coarse midpoint (p points with the
+//   midpoint Lagrange weights; none set: a copy), 8 the Dirichlet hook.
+enum { PLAN_DIRICHLET = 8 };
+struct PlanOp {
+  int dst, src, mode, pad;
+};
+struct PlanArgs {
+  Geom g;
+  CaseP c;
+  double t;
+  double* __restrict__ W;
+  const PlanOp* __restrict__ ops;
+  int n;
+  int np;          // the prolongation order: points per midpoint axis
+  double wm[10];   // the midpoint Lagrange weights
+};
+template <bool MOVE>
+__global__ __launch_bounds__(256) void k_fill_plan(PlanArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.n) return;
+  const PlanOp o = a.ops[e];
+  const Geom& g = a.g;
+  const int S = g.S, G = g.G;
+  const long S3 = (long)S * S * S;
+  const int b = o.dst / (int)S3, p = o.dst - b * (int)S3;
+  double* dst = a.W + (long)b * 20 * S3 + p;
+  if (o.mode == PLAN_DIRICHLET) {
+    double x[3], s[20];
+    position(g, b, p % S - G, (p / S) % S - G, p / (S * S) - G, x);
+    case_state<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
+#pragma unroll
+    for (int v = 0; v < 20; ++v) dst[v * S3] = s[v];
+    return;
+  }
+  const int sb = o.src / (int)S3, sp = o.src - sb * (int)S3;
+  const double* src = a.W + (long)sb * 20 * S3 + sp;
+  const int nx = (o.mode & 1) ? a.np : 1, ny = (o.mode & 2) ? a.np : 1, nz = (o.mode & 4) ? a.np : 1;
+  // sum-factorised, x innermost, each sum from its first point (rounded as
+  // written: no contraction)
+#pragma unroll 1
+  for (int v = 0; v < 20; ++v) {
+#pragma clang fp contract(off)
+    const double* f = src + v * S3;
+    double sz = 0.0;
+    for (int c2 = 0; c2 < nz; ++c2) {
+      double sy = 0.0;
+      for (int c1 = 0; c1 < ny; ++c1) {
+        double sx = 0.0;
+        for (int c0 = 0; c0 < nx; ++c0) sx += (nx > 1 ? a.wm[c0] : 1.0) * f[c0 + S * (c1 + S * c2)];
+        sy += (ny > 1 ? a.wm[c1] : 1.0) * sx;
+      }
+      sz += (nz > 1 ? a.wm[c2] : 1.0) * sy;
+    }
+    dst[v * S3] = sz;
   }
 }
 
@@ -1057,7 +1135,17 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
           r2 = fmax(r2, fabs(e));
         }
         int l[3] = {i, j, k}, owner, loc[3];
-        if (classify(g, b, l, owner, loc) == -1) {
+        if (g.prim) {
+          // a forest: primaries only, each weighted by its volume relative
+          // to level 0's, 2^(-3 level)
+          if (g.prim[(long)b * psz + p]) {
+            const double wv = 1.0 / (double)(1 << (3 * block_level(g, b)));
+            r0 = e2 * wv;
+            r1 = wv;
+          } else {
+            r2 = 0.0;
+          }
+        } else if (classify(g, b, l, owner, loc) == -1) {
           r0 = e2;
           r1 = 1.0;
         }

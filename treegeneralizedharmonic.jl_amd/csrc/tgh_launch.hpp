@@ -23,6 +23,7 @@ void launch_rhs_q(tgh_problem* p, const RhsArgs& a) {
       RhsArgs ab = a;
       ab.g.nblocks = 1;
       ab.g.bcoord = a.g.bcoord + 3 * b;
+      if (a.g.blevel) ab.g.blevel = a.g.blevel + b;
       const long ws = (long)b * 20 * S3, ss = (long)b * 20 * P3;
       ab.W = a.W + ws;
       if (a.Wn) ab.Wn = a.Wn + ws;

@@ -24,6 +24,16 @@ struct tgh_problem {
   std::vector<int> bcoord_host;
   int* d_bcoord = nullptr;
   int* d_lookup = nullptr;
+  // A forest with levels (tgh_problem_create_forest): per-block levels, the
+  // primary flags and the fill plan, in passes.
+  bool forest = false;
+  std::vector<int> level_host;
+  int* d_level = nullptr;
+  unsigned char* d_prim = nullptr;
+  int* d_plan = nullptr;
+  std::vector<int> plan_off;
+  int prolong = 0;
+  double wmid[10] = {0};
   double* W[2] = {nullptr, nullptr};  // work arrays (prepare uses W[0])
   double* acc = nullptr;              // RK4 accumulator, state layout
   double* coef = nullptr;             // the split form's per-point coefficients (CF_N x points)
