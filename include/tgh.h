@@ -51,8 +51,10 @@ enum { /* interior variants, src/interior.jl:178-238 */
   TGH_INTERIOR_NONE = 0,
   TGH_INTERIOR_DAMPED = 1,
   TGH_INTERIOR_FROZEN = 2,
-  TGH_INTERIOR_PASTED = 3
+  TGH_INTERIOR_PASTED = 3,
+  TGH_INTERIOR_FITTED = 4  /* :fitted: core and layer relax toward the target cache (tgh_set_target) */
 };
+enum { TGH_SHAPE_MAX_LMAX = 8 }; /* the tracked surface's largest series degree here */
 enum { /* the Kreiss–Oliger amplitude (src/gauge.jl:1172-1264) */
   TGH_DISSIPATION_CONSTANT = 0,
   TGH_DISSIPATION_HORIZON = 1
@@ -110,6 +112,16 @@ typedef struct tgh_desc {
    * every distance to the hole is measured from center + v t, and a :damped
    * core follows the hole (src/interior.jl:390-549). |v| < 1. */
   double velocity[3];
+  /* FittedInterior's tracked geometry (src/interior.jl:1752-1990): with
+   * shaped = 1 the layer's radii along the ray through x are r_1 = r_h(n) -
+   * offset and r_0 = r_1 - thickness instead of r1 and r0, where r_h(n) is
+   * the real spherical-harmonic series of shape (shape_lmax <= 8, the
+   * (lmax + 1)^2 coefficients in real_harmonic_index order, l^2 + l + m)
+   * clamped into [r_in, r_out], its bounding radii. */
+  int32_t shaped;
+  int32_t shape_lmax;
+  double shape[(TGH_SHAPE_MAX_LMAX + 1) * (TGH_SHAPE_MAX_LMAX + 1)];
+  double r_in, r_out, offset, thickness;
 } tgh_desc;
 
 /* Matter's 3+1 projections at every owned point (src/evolution.jl:1295-1311,
@@ -147,6 +159,16 @@ int tgh_problem_create_forest(const tgh_desc* desc, const int32_t* leaves, int32
 /* The level of every block, 1 per block (0 on a uniform forest;
  * level(k) of TreeAMR's leaves). */
 int tgh_block_levels(const tgh_problem* p, int32_t* out);
+/* GHProblem(...; target, t_target, target_rate, trail) (src/evolution.jl:
+ * 1007-1021,1034-1037) for a :fitted interior: target is a DEVICE array of
+ * 40 variables per owned point in the state layout ((N+1)^3 x 40 x nblocks:
+ * A_v then S_v, v = 1..20), read as A + (t - t_target) S
+ * (_cached_target, src/fit.jl:1050-1055) and, with rate != 0, S as the
+ * target's rate (_cached_rate); trail narrows the rho ramp on the trailing
+ * side by 1 - trail max(0, -n.v/|v|) (_trail_profiles,
+ * src/evolution.jl:1256-1272). The library keeps the pointer (the caller owns
+ * the array and may refill it between calls); NULL detaches it. */
+int tgh_set_target(tgh_problem* p, const double* target, double t_target, int32_t rate, double trail);
 /* No reference counterpart: releases the problem's device memory. */
 int tgh_problem_destroy(tgh_problem* p);
 

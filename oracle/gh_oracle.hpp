@@ -1124,7 +1124,7 @@ inline void layer_profiles(double r, double r0, double r1, double rho_max, doubl
   rho = rho_max * smoothstep((1.0 - s) / rho_ramp);
 }
 
-enum Interior { INT_NONE = 0, INT_DAMPED = 1, INT_FROZEN = 2, INT_PASTED = 3 };
+enum Interior { INT_NONE = 0, INT_DAMPED = 1, INT_FROZEN = 2, INT_PASTED = 3, INT_FITTED = 4 };
 enum Gauge { GAUGE_NONE = 0, GAUGE_SAMPLED = 1, GAUGE_KS = 2 };
 enum Damping { DAMP_CONSTANT = 0, DAMP_GAUSSIAN = 1 };
 enum Dissipation { DISS_CONSTANT = 0, DISS_HORIZON = 1 };
@@ -1143,7 +1143,81 @@ struct Case {
   double vel[3] = {0, 0, 0};     // and its velocity: c(t) = c0 + v t
   double r0 = 0, r1 = 0, rho_max = 0, w_ramp = 0.5, rho_ramp = 0.5;
   double delta = 0;              // layer_rate_step: cbrt(eps) r_0
+  // FittedInterior's tracked geometry (interior.jl:1752-1990)
+  bool shaped = false;
+  int lmax = 0;
+  double shape[81] = {0};
+  double r_in = 0, r_out = 0, offset = 0, thickness = 0;
 };
+
+// shape_series (interior.jl:1294-1329): sum_lm a_lm y~_lm(n) with the fully
+// normalised Legendre recurrences in z and (n_x + i n_y)^m for the azimuth
+// (Condon-Shortley phase); _shape_term (:1266-1274) for one term.
+inline double shape_term(const double* shape, int l, int m, double q, double C, double S, double s2) {
+  if (m == 0) return shape[l * l + l] * q;
+  const double ac = shape[l * l + l + m], as = shape[l * l + l - m];
+  return s2 * q * (ac * C - as * S);
+}
+inline double shape_series(const double* shape, int lmax, const double n[3]) {
+  const double x = n[0], y = n[1], z = n[2];
+  const double s2 = std::sqrt(2.0);
+  double acc = 0.0;
+  double qmm = 1.0 / std::sqrt(4.0 * M_PI);
+  double C = 1.0, S = 0.0;
+  for (int m = 0; m <= lmax; ++m) {
+    if (m > 0) {
+      qmm = -std::sqrt((double)(2 * m + 1) / (double)(2 * m)) * qmm;
+      const double Cn = C * x - S * y, Sn = C * y + S * x;
+      C = Cn;
+      S = Sn;
+    }
+    acc += shape_term(shape, m, m, qmm, C, S, s2);
+    if (m + 1 <= lmax) {
+      double qa = qmm;
+      double qb = std::sqrt((double)(2 * m + 3)) * z * qmm;
+      acc += shape_term(shape, m + 1, m, qb, C, S, s2);
+      for (int l = m + 2; l <= lmax; ++l) {
+        const double al = std::sqrt((double)(4 * l * l - 1) / (double)(l * l - m * m));
+        const double be = std::sqrt((double)((l - 1) * (l - 1) - m * m) / (double)(4 * (l - 1) * (l - 1) - 1));
+        const double qc = al * (z * qb - be * qa);
+        acc += shape_term(shape, l, m, qc, C, S, s2);
+        qa = qb;
+        qb = qc;
+      }
+    }
+  }
+  return acc;
+}
+// shape_radius (interior.jl:1846-1853): the series clamped into [r_in, r_out].
+inline double shape_radius(const Case& c, const double n[3]) {
+  return std::min(std::max(shape_series(c.shape, c.lmax, n), c.r_in), c.r_out);
+}
+// interior_point (interior.jl:341, 1868-1886): the distance from c(t) and the
+// layer's radii along the ray -- the sphere's r_0, r_1, or the tracked
+// surface's r_1 = r_h(n) - offset, r_0 = r_1 - thickness, with the two exact
+// fast paths outside r_out - offset and inside r_in - offset - thickness.
+struct IGeo {
+  double r, r0, r1;
+};
+inline IGeo interior_geo(const Case& c, double t, const double x[3]) {
+  double cc[3];
+  for (int i = 0; i < 3; ++i) cc[i] = c.center[i] + t * c.vel[i];
+  const double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
+  const double r = std::sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+  if (!c.shaped) return {r, c.r0, c.r1};
+  const double r1o = c.r_out - c.offset;
+  if (r >= r1o) return {r, r1o - c.thickness, r1o};
+  const double r1i = c.r_in - c.offset, r0i = r1i - c.thickness;
+  if (r < r0i) return {r, r0i, r1i};
+  double n[3] = {0.0, 0.0, 1.0};
+  if (r != 0.0) {
+    n[0] = d1 / r;
+    n[1] = d2 / r;
+    n[2] = d3 / r;
+  }
+  const double r1 = shape_radius(c, n) - c.offset;
+  return {r, r1 - c.thickness, r1};
+}
 
 // center_at (interior.jl:60-93): c(t) = c0 + v t.
 inline void center_at(const Case& c, double t, double out[3]) {
@@ -1191,17 +1265,26 @@ inline double interior_radius(const Case& c, double t, const double x[3]) {
   double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
   return std::sqrt(d1 * d1 + d2 * d2 + d3 * d3);
 }
-// core_position (interior.jl:615-625) about c(t).
+// core_position (interior.jl:615-625) about c(t); on the tracked geometry
+// (:1950-1965) the core surface's own point on the ray, r_0(n) =
+// (r_h(n) - offset) - thickness, the series skipped outside its bounding sphere.
 inline void core_position(const Case& c, double t, const double x[3], double out[3]) {
   if (c.interior == INT_NONE) { out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; return; }
   double cc[3];
   center_at(c, t, cc);
   double d[3] = {x[0] - cc[0], x[1] - cc[1], x[2] - cc[2]};
   double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-  if (r >= c.r0) { out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; return; }
+  double rc = c.r0;
+  if (c.shaped) {
+    if (r >= (c.r_out - c.offset) - c.thickness) { out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; return; }
+    double n[3] = {0, 0, 1};
+    if (r != 0) { n[0] = d[0] / r; n[1] = d[1] / r; n[2] = d[2] / r; }
+    rc = (shape_radius(c, n) - c.offset) - c.thickness;
+  }
+  if (r >= rc) { out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; return; }
   double n[3] = {0, 0, 1};
   if (r != 0) { n[0] = d[0] / r; n[1] = d[1] / r; n[2] = d[2] / r; }
-  for (int i = 0; i < 3; ++i) out[i] = cc[i] + c.r0 * n[i];
+  for (int i = 0; i < 3; ++i) out[i] = cc[i] + rc * n[i];
 }
 // case_state_tuple (initialdata.jl:1795-1796).
 inline void case_state(const Case& c, double t, const double x[3], double out[20]) {

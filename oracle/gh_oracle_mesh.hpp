@@ -543,9 +543,21 @@ struct Problem {
     bool diss = has_dissipation(cs);
     V10<double> dth, dtP;
     if (cs.interior != INT_NONE) {
-      double r = interior_radius(cs, t, x);
-      bool frozen = cs.interior == INT_PASTED ? r < cs.r1 : r < cs.r0;
+      const IGeo g = interior_geo(cs, t, x);
+      const double r = g.r;
+      bool frozen = cs.interior == INT_PASTED ? r < g.r1 : r < g.r0;
+      const long tp = tidx(b, i, j, k);
       if (frozen) {
+        if (cs.interior == INT_FITTED) {
+          // the fitted core (evolution.jl:515-546): du = [S] - rho_max (u - (A + (t - t_f) S))
+          long var = m.widx(b, 0, i, j, k), sv = m.S * m.S * m.S;
+          const long ps = m.P * m.P * m.P;
+          for (int v = 0; v < 20; ++v) {
+            const double A = tw[tp + v * ps], Sl = tw[tp + (20 + v) * ps];
+            du[m.sidx(b, v, i, j, k)] = (rate ? Sl : 0.0) - cs.rho_max * (work[var + v * sv] - (A + (t - t_f) * Sl));
+          }
+          return;
+        }
         if (core_follows(cs)) {
           // the moving :damped core relaxes toward the layer's target at the
           // core rule's point, the metric in the ADM variables
@@ -568,17 +580,40 @@ struct Problem {
         return;
       }
       rhs_point_F<double>(b, i, j, k, t, g0, cs.g2, epsh, diss, dth, dtP);
-      if (r >= cs.r1) {
+      if (r >= g.r1) {
         for (int v = 0; v < NC; ++v) {
           du[m.sidx(b, v, i, j, k)] = dth[v];
           du[m.sidx(b, NC + v, i, j, k)] = dtP[v];
         }
         return;
       }
-      double w, rho;
-      layer_profiles(r, cs.r0, cs.r1, cs.rho_max, cs.w_ramp, cs.rho_ramp, w, rho);
+      double w, rho, rramp = cs.rho_ramp;
+      if (cs.interior == INT_FITTED && trail != 0.0) {
+        // _trail_profiles (evolution.jl:1256-1272): rho's ramp narrowed on
+        // the trailing side by 1 - trail max(0, -n.v/|v|)
+        double cc[3];
+        center_at(cs, t, cc);
+        const double sp = std::sqrt(cs.vel[0] * cs.vel[0] + cs.vel[1] * cs.vel[1] + cs.vel[2] * cs.vel[2]);
+        double zeta = 0.0;
+        if (!(sp == 0.0 || r == 0.0))
+          zeta = std::max(0.0, -((x[0] - cc[0]) * cs.vel[0] + (x[1] - cc[1]) * cs.vel[1] +
+                                 (x[2] - cc[2]) * cs.vel[2]) / (r * sp));
+        rramp = cs.rho_ramp * (1.0 - trail * zeta);
+      }
+      layer_profiles(r, g.r0, g.r1, cs.rho_max, cs.w_ramp, rramp, w, rho);
       if (cs.interior == INT_FROZEN) rho = 0.0;
       long var = m.widx(b, 0, i, j, k), sv = m.S * m.S * m.S;
+      if (cs.interior == INT_FITTED) {
+        // the fitted layer (evolution.jl:600-645): w F + [(1 - w) S] - rho (u - (A + (t - t_f) S))
+        const long ps = m.P * m.P * m.P;
+        const double om = 1.0 - w;
+        for (int v = 0; v < 20; ++v) {
+          const double A = tw[tp + v * ps], Sl = tw[tp + (20 + v) * ps];
+          const double F = v < NC ? dth[v] : dtP[v - NC];
+          du[m.sidx(b, v, i, j, k)] = w * F + (rate ? om * Sl : 0.0) - rho * (work[var + v * sv] - (A + (t - t_f) * Sl));
+        }
+        return;
+      }
       if (core_follows(cs)) {
         // the moving layer: (1 - w) d_t u_exact keeps the moving solution steady
         V10<double> hm, Pm, dthm, dtPm;
@@ -747,7 +782,17 @@ struct Problem {
   // ---- the time step and the analysis reductions ----
   bool is_evolved(double t, const double x[3]) const {
     if (cs.interior == INT_NONE) return true;
-    return interior_radius(cs, t, x) >= cs.r1;
+    const IGeo g = interior_geo(cs, t, x);
+    return g.r >= g.r1;
+  }
+  // The :fitted target cache (evolution.jl:1007-1021): 40 variables per owned
+  // point, A then S, in the state layout; t_f its fill time.
+  std::vector<double> tw;
+  double t_f = 0.0, trail = 0.0;
+  bool rate = false;
+  long tidx(long b, long i, long j, long k) const {
+    const Mesh& m = mesh;
+    return tw.empty() ? 0 : i + m.P * (j + m.P * (k + m.P * (40 * b)));
   }
   // max_speed (evolution.jl:932-949,1539-1560) over evolved owned points.
   double max_speed(const double* u, double t = 0.0) const {

@@ -66,6 +66,13 @@ Case case_from(const tgh_desc* d) {
   c.rho_max = d->rho_max;
   c.w_ramp = d->w_ramp;
   c.rho_ramp = d->rho_ramp;
+  c.shaped = d->shaped != 0;
+  c.lmax = d->shape_lmax;
+  for (int i = 0; i < 81; ++i) c.shape[i] = d->shape[i];
+  c.r_in = d->r_in;
+  c.r_out = d->r_out;
+  c.offset = d->offset;
+  c.thickness = d->thickness;
   return c;
 }
 V10<double> v10(const double* p) {
@@ -316,6 +323,23 @@ void* orc_problem_create_forest(const tgh_desc* d, const int* leaves, int nleave
   p->mesh.init_forest(lv, prolong);
   p->setup();
   return p;
+}
+// The :fitted target: tw host array of 40 (N+1)^3 nblocks doubles (copied).
+void orc_set_target(void* pp, const double* tw, double t_f, int rate, double trail) {
+  Problem* p = (Problem*)pp;
+  const long n = 40 * p->mesh.P * p->mesh.P * p->mesh.P * p->mesh.nblocks();
+  p->tw.assign(tw, tw + n);
+  p->t_f = t_f;
+  p->rate = rate != 0;
+  p->trail = trail;
+}
+double orc_shape_series(const double* shape, int lmax, const double* n) { return shape_series(shape, lmax, n); }
+void orc_interior_geo(const tgh_desc* d, double t, const double* x, double* out) {
+  Case c = case_from(d);
+  IGeo g = interior_geo(c, t, x);
+  out[0] = g.r;
+  out[1] = g.r0;
+  out[2] = g.r1;
 }
 int orc_block_level(void* p, long b) {
   auto& m = ((Problem*)p)->mesh;

@@ -55,6 +55,9 @@ def lib():
             "orc_problem_create": (P, [desc]),
             "orc_problem_create_forest": (P, [desc, C.POINTER(C.c_int), C.c_int, C.c_int]),
             "orc_block_level": (C.c_int, [P, Lg]),
+            "orc_set_target": (None, [P, dp, D, C.c_int, D]),
+            "orc_shape_series": (D, [dp, C.c_int, dp]),
+            "orc_interior_geo": (None, [desc, D, dp, dp]),
             "orc_forest_classify": (C.c_int, [P, Lg, Lg, Lg, Lg, C.POINTER(Lg)]),
             "orc_prolong_weights": (None, [P, dp]),
             "orc_problem_destroy": (None, [P]),
@@ -128,6 +131,12 @@ class OracleProblem:
             arr = (C.c_int * len(flat))(*flat)
             self.p = lib().orc_problem_create_forest(C.byref(desc), arr, len(leaves), int(prolongation))
         self.n = lib().orc_state_length(self.p)
+
+    def set_target(self, tw, t_f, rate=False, trail=0.0):
+        """The :fitted target cache (40 variables per owned point, A then S)."""
+        tw = np.ascontiguousarray(tw, dtype=np.float64)
+        assert tw.size == 2 * self.n
+        lib().orc_set_target(self.p, tw, float(t_f), 1 if rate else 0, float(trail))
 
     def block_level(self, b):
         return lib().orc_block_level(self.p, b)
@@ -225,3 +234,15 @@ def sym_eigen3(A):
     lam, V = np.zeros(3), np.zeros(9)
     lib().orc_sym_eigen3(np.ascontiguousarray(A, dtype=np.float64).ravel(), lam, V)
     return lam, V.reshape(3, 3)
+
+
+def shape_series(shape, lmax, n):
+    sh = np.zeros(81)
+    sh[:len(shape)] = shape
+    return lib().orc_shape_series(sh, int(lmax), np.ascontiguousarray(n, dtype=np.float64))
+
+
+def interior_geo(desc, t, x):
+    out = np.zeros(3)
+    lib().orc_interior_geo(C.byref(desc), float(t), np.ascontiguousarray(x, dtype=np.float64), out)
+    return out

@@ -6,7 +6,7 @@ import ctypes as C
 TGH_BG_MINKOWSKI, TGH_BG_GAUGE_WAVE, TGH_BG_KERR_SCHILD, TGH_BG_SHIFTED_MINKOWSKI, TGH_BG_HARMONIC = 0, 1, 2, 3, 4
 TGH_GAUGE_NONE, TGH_GAUGE_SAMPLED, TGH_GAUGE_KERR_SCHILD = 0, 1, 2
 TGH_DAMPING_CONSTANT, TGH_DAMPING_GAUSSIAN = 0, 1
-TGH_INTERIOR_NONE, TGH_INTERIOR_DAMPED, TGH_INTERIOR_FROZEN, TGH_INTERIOR_PASTED = 0, 1, 2, 3
+TGH_INTERIOR_NONE, TGH_INTERIOR_DAMPED, TGH_INTERIOR_FROZEN, TGH_INTERIOR_PASTED, TGH_INTERIOR_FITTED = 0, 1, 2, 3, 4
 TGH_DISSIPATION_CONSTANT, TGH_DISSIPATION_HORIZON = 0, 1
 
 
@@ -27,6 +27,8 @@ class TghDesc(C.Structure):
         ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("lambda_min", C.c_double),
         ("lambda_max", C.c_double), ("beta_max", C.c_double), ("K_max", C.c_double), ("r_gate", C.c_double),
         ("velocity", C.c_double * 3),
+        ("shaped", C.c_int32), ("shape_lmax", C.c_int32), ("shape", C.c_double * 81),
+        ("r_in", C.c_double), ("r_out", C.c_double), ("offset", C.c_double), ("thickness", C.c_double),
     ]
 
 

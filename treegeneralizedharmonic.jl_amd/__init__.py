@@ -77,6 +77,8 @@ class TghDesc(C.Structure):
         ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("lambda_min", C.c_double),
         ("lambda_max", C.c_double), ("beta_max", C.c_double), ("K_max", C.c_double), ("r_gate", C.c_double),
         ("velocity", C.c_double * 3),
+        ("shaped", C.c_int32), ("shape_lmax", C.c_int32), ("shape", C.c_double * 81),
+        ("r_in", C.c_double), ("r_out", C.c_double), ("offset", C.c_double), ("thickness", C.c_double),
     ]
 
 
@@ -103,6 +105,7 @@ def _load():
         "tgh_problem_create_forest": (C.c_int, [C.POINTER(TghDesc), C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                                 C.POINTER(P)]),
         "tgh_block_levels": (C.c_int, [P, C.POINTER(C.c_int32)]),
+        "tgh_set_target": (C.c_int, [P, C.c_void_p, C.c_double, C.c_int32, C.c_double]),
         "tgh_problem_destroy": (C.c_int, [P]),
         "tgh_state_length": (C.c_int64, [P]),
         "tgh_work_length": (C.c_int64, [P]),
@@ -274,7 +277,7 @@ def horizon_dissipation(case, *, eps_in, eps_out=None):
 class Interior:
     """Interior(T; center, r_0, r_1, ρ_max, variant, w_ramp, ρ_ramp)
     (src/interior.jl:178-238)."""
-    VARIANTS = {"damped": 1, "frozen": 2, "pasted": 3}
+    VARIANTS = {"damped": 1, "frozen": 2, "pasted": 3, "fitted": 4}
 
     def __init__(self, T=None, *, center=(0.0, 0.0, 0.0), r_0, r_1, rho_max=0.0,
                  variant="damped", w_ramp=0.5, rho_ramp=0.5):
@@ -298,6 +301,103 @@ class Interior:
     def with_rho_max(self, rho_max):
         return Interior(center=self.center, r_0=self.r_0, r_1=self.r_1, rho_max=rho_max,
                         variant=self.variant, w_ramp=self.w_ramp, rho_ramp=self.rho_ramp)
+
+
+def shape_series(shape, lmax, n):
+    """shape_series (src/interior.jl:1294-1329): Σ a_lm ỹ_lm(n̂) for real
+    coefficients in real_harmonic_index order (l² + l + m), by the fully
+    normalised Legendre recurrences (Condon–Shortley phase); numpy, host side."""
+    import numpy as np
+    n = np.asarray(n, dtype=float)
+    x, y, z = n[..., 0], n[..., 1], n[..., 2]
+    s2 = math.sqrt(2.0)
+    acc = np.zeros_like(x)
+    qmm = np.full_like(x, 1.0 / math.sqrt(4.0 * math.pi))
+    C, S = np.ones_like(x), np.zeros_like(x)
+
+    def term(l, m, q):
+        if m == 0:
+            return shape[l * l + l] * q
+        return s2 * q * (shape[l * l + l + m] * C - shape[l * l + l - m] * S)
+
+    for m in range(lmax + 1):
+        if m > 0:
+            qmm = -math.sqrt((2 * m + 1) / (2 * m)) * qmm
+            C, S = C * x - S * y, C * y + S * x
+        acc = acc + term(m, m, qmm)
+        if m + 1 <= lmax:
+            qa, qb = qmm, math.sqrt(2 * m + 3) * z * qmm
+            acc = acc + term(m + 1, m, qb)
+            for l in range(m + 2, lmax + 1):
+                al = math.sqrt((4 * l * l - 1) / (l * l - m * m))
+                be = math.sqrt(((l - 1) ** 2 - m * m) / (4 * (l - 1) ** 2 - 1))
+                qa, qb = qb, al * (z * qb - be * qa)
+                acc = acc + term(l, m, qb)
+    return acc
+
+
+def shape_bounds(shape, lmax):
+    """shape_bounds (src/interior.jl:1331-1375): the extremes of the series over
+    both poles and n_θ = 4 lmax + 3 colatitudes by 2 n_θ longitudes."""
+    import numpy as np
+    nth = 4 * int(lmax) + 3
+    nph = 2 * nth
+    th = np.pi * np.arange(1, nth + 1) / (nth + 1)
+    ph = 2 * np.pi * np.arange(nph) / nph
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    dirs = np.concatenate([np.array([[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]]),
+                           np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)])
+    v = shape_series(np.asarray(shape, dtype=float), lmax, dirs)
+    return float(v.min()), float(v.max())
+
+
+class FittedInterior:
+    """FittedInterior(T; center, shape, lmax, offset, thickness, ρ_max, variant,
+    w_ramp, ρ_ramp, r_in, r_out) (src/interior.jl:1752-1816): the tracked
+    horizon r_h(n̂) as a real spherical-harmonic series about center (a point,
+    or (c0, v) for a moving one), clamped into [r_in, r_out] (its sampled
+    extremes by default); the layer is r_1 = r_h − offset, r_0 = r_1 − thickness
+    along each ray."""
+    VARIANTS = {"damped": 1, "frozen": 2, "fitted": 4}
+
+    def __init__(self, T=None, *, center, shape, lmax=None, offset, thickness, rho_max=0.0,
+                 variant="damped", w_ramp=0.5, rho_ramp=0.5, r_in=None, r_out=None):
+        _check_float(T)
+        if variant not in self.VARIANTS:
+            raise ArgumentError(f"the interior variant of a tracked geometry must be one of "
+                                f"{tuple(self.VARIANTS)} here, got :{variant}")
+        shape = [float(a) for a in shape]
+        L = int(math.isqrt(len(shape)) - 1) if lmax is None else int(lmax)
+        if (L + 1) ** 2 != len(shape):
+            raise ArgumentError(f"a shape of degree lmax = {L} has (lmax + 1)² = {(L + 1) ** 2} real "
+                                f"coefficients, got {len(shape)}")
+        if L > 8:
+            raise ArgumentError(f"the shape's degree is at most 8 here, got lmax = {L}")
+        if r_in is None or r_out is None:
+            r_in, r_out = shape_bounds(shape, L)
+        if not 0 < r_in <= r_out:
+            raise ArgumentError(f"the horizon's bounding radii must satisfy 0 < r_in ≤ r_out, got "
+                                f"r_in = {r_in}, r_out = {r_out}")
+        if not (offset > 0 and thickness > 0):
+            raise ArgumentError(f"the offset and the thickness are lengths and must be positive, got "
+                                f"offset = {offset} and thickness = {thickness}")
+        if not (r_in - offset) - thickness > 0:
+            raise ArgumentError(f"the core surface r_h(n̂) − offset − thickness reaches the center: "
+                                f"r_in = {r_in}, offset = {offset}, thickness = {thickness}")
+        if not (0 < w_ramp <= 1 and 0 < rho_ramp <= 1):
+            raise ArgumentError(f"the ramp fractions must lie in (0, 1], got w_ramp = {w_ramp} and "
+                                f"ρ_ramp = {rho_ramp}")
+        if rho_max < 0:
+            raise ArgumentError(f"the relaxation rate must satisfy ρ_max ≥ 0, got {rho_max}")
+        self.center = tuple(float(c) for c in center)
+        self.shape, self.lmax = shape, L
+        self.r_in, self.r_out = float(r_in), float(r_out)
+        self.offset, self.thickness = float(offset), float(thickness)
+        self.rho_max, self.variant = float(rho_max), variant
+        self.w_ramp, self.rho_ramp = float(w_ramp), float(rho_ramp)
+        # the bounding radii of the layer, for the checks that read r_0 and r_1
+        self.r_0 = (self.r_in - self.offset) - self.thickness
+        self.r_1 = self.r_out - self.offset
 
 
 class StateBounds:
@@ -382,6 +482,15 @@ class GHCase:
         if gauge_source == "algebraic" and not isinstance(background, KerrSchild):
             raise ArgumentError("the algebraic gauge source is the Kerr-Schild hole's own")
         self.gauge_source = gauge_source
+
+
+def with_interior(case, interior):
+    """with_interior (src/evolution.jl:1150-1190 spelling for a case): the same
+    case with another interior (an Interior or a FittedInterior)."""
+    import copy
+    c = copy.copy(case)
+    c.interior = interior
+    return c
 
 
 def minkowski_case(T=None, *, L, eps_ko, gamma0, gamma2):
@@ -740,6 +849,10 @@ def _desc(forest, case, q):
         d.interior = Interior.VARIANTS[it.variant]
         d.center[:] = list(it.center)
         d.r0, d.r1, d.rho_max, d.w_ramp, d.rho_ramp = it.r_0, it.r_1, it.rho_max, it.w_ramp, it.rho_ramp
+        if isinstance(it, FittedInterior):
+            d.shaped, d.shape_lmax = 1, it.lmax
+            d.shape[:len(it.shape)] = it.shape
+            d.r_in, d.r_out, d.offset, d.thickness = it.r_in, it.r_out, it.offset, it.thickness
     return d
 
 
@@ -747,7 +860,8 @@ class GHProblem:
     """GHProblem(forest, case; q, backend=MI355X()) (src/evolution.jl:1034-1149):
     the device work arrays, the block table and the case, built once."""
 
-    def __init__(self, forest, case, *, q, backend=None, ops=None):
+    def __init__(self, forest, case, *, q, backend=None, ops=None, target=None, t_target=0.0,
+                 target_rate=False, trail=0.0):
         if not (q >= 2 and q % 2 == 0):
             raise ArgumentError(f"the finite-difference order must be even and at least 2, so "
                                 f"that the centered stencils have an integer half-width q/2, but q={q}")
@@ -774,6 +888,26 @@ class GHProblem:
         self.state_length = int(_load().tgh_state_length(self._ptr))
         self.work_length = int(_load().tgh_work_length(self._ptr))
         self.nblocks = forest.nleaves()
+        self.target = None
+        if target is not None or (case.interior is not None and case.interior.variant == "fitted"):
+            if target is None:
+                raise ArgumentError("a :fitted interior relaxes toward a target the kernel reads from a "
+                                    "cache: give GHProblem target = a device array of 40 variables per "
+                                    "owned point (A then S)")
+            self.set_target(target, t_target, rate=target_rate, trail=trail)
+
+    def set_target(self, target, t_target, *, rate=False, trail=0.0):
+        """The :fitted target cache (src/evolution.jl:1007-1021): a float64 CUDA
+        tensor of 40 (N+1)^3 nblocks values (A_v then S_v per block), read as
+        A + (t − t_target) S; the problem keeps a reference to it."""
+        import torch
+        if (not isinstance(target, torch.Tensor) or target.dtype != torch.float64 or not target.is_cuda
+                or not target.is_contiguous() or target.numel() != 2 * self.state_length):
+            raise ArgumentError(f"the target cache must be a contiguous float64 CUDA tensor of "
+                                f"{2 * self.state_length} values (40 per owned point)")
+        self.target = target
+        _check(_load().tgh_set_target(self._ptr, C.c_void_p(target.data_ptr()), float(t_target),
+                                      1 if rate else 0, float(trail)))
 
     def __del__(self):
         if getattr(self, "_ptr", None) and self._ptr.value:

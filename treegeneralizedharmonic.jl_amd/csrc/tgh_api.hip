@@ -104,8 +104,37 @@ int check_desc(const tgh_desc* d, int lmax = 0) {
       (d->background == TGH_BG_KERR_SCHILD || d->background == TGH_BG_SHIFTED_MINKOWSKI))
     return fail(TGH_EINVAL, "this background is not harmonic, so it needs a gauge source: H_a = 0 "
                             "would evolve it with a wrong gauge (src/gauge.jl isharmonic)");
-  if (d->interior < 0 || d->interior > 3) return fail(TGH_EINVAL, "unknown interior variant");
-  if (d->interior != TGH_INTERIOR_NONE) {
+  if (d->interior < 0 || d->interior > 4) return fail(TGH_EINVAL, "unknown interior variant");
+  if (d->shaped) {
+    // FittedInterior's checks (src/interior.jl:1770-1816)
+    if (d->interior == TGH_INTERIOR_NONE)
+      return fail(TGH_EINVAL, "a tracked geometry (shaped = 1) belongs to an interior; this case has none");
+    if (d->interior == TGH_INTERIOR_PASTED)
+      return fail(TGH_EUNSUPPORTED, "the :pasted limiter on a tracked geometry is not built");
+    if (d->shape_lmax < 0 || d->shape_lmax > TGH_SHAPE_MAX_LMAX)
+      return fail(TGH_EINVAL, "the shape's degree must lie in 0 .. 8, got lmax = " + std::to_string(d->shape_lmax));
+    for (int i = 0; i < (d->shape_lmax + 1) * (d->shape_lmax + 1); ++i)
+      if (!std::isfinite(d->shape[i])) return fail(TGH_EINVAL, "the shape's coefficients must be finite");
+    if (!(0 < d->r_in && d->r_in <= d->r_out))
+      return fail(TGH_EINVAL, "the horizon's bounding radii must satisfy 0 < r_in <= r_out, got r_in = " +
+                                  std::to_string(d->r_in) + ", r_out = " + std::to_string(d->r_out));
+    if (!(d->offset > 0 && d->thickness > 0))
+      return fail(TGH_EINVAL, "the offset and the thickness are lengths and must be positive");
+    if (!((d->r_in - d->offset) - d->thickness > 0))
+      return fail(TGH_EINVAL, "the core surface r_h(n) - offset - thickness reaches the center");
+    if (!(d->w_ramp > 0 && d->w_ramp <= 1 && d->rho_ramp > 0 && d->rho_ramp <= 1))
+      return fail(TGH_EINVAL, "the ramp fractions must lie in (0, 1]");
+    if (!(d->rho_max >= 0)) return fail(TGH_EINVAL, "the relaxation rate must satisfy rho_max >= 0");
+    if (d->thickness < 2.0 * (G + 1) * h0)
+      return fail(TGH_EINVAL, "the layer's thickness must be at least 2(G+1)h so that no stencil of an evolved "
+                              "point reaches the core (check_interior_radii on the ray's radii)");
+    if ((d->background == TGH_BG_KERR_SCHILD || d->background == TGH_BG_HARMONIC) &&
+        !((d->r_in - d->offset) - d->thickness > std::fabs(d->a)))
+      return fail(TGH_EINVAL, "the core surface must contain the chart's singular disk |x| <= |a|, z = 0");
+    const double* v = d->velocity;
+    if ((v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) && d->interior != TGH_INTERIOR_FITTED)
+      return fail(TGH_EUNSUPPORTED, "a moving tracked geometry is built for the :fitted variant only");
+  } else if (d->interior != TGH_INTERIOR_NONE) {
     if (!(d->r0 > 0))
       return fail(TGH_EINVAL, "the frozen core's radius must be positive, got r_0 = " + std::to_string(d->r0));
     if (!(d->r1 > d->r0)) return fail(TGH_EINVAL, "the layer needs r_1 > r_0");
@@ -600,6 +629,18 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
   c.rho_max = d->rho_max;
   c.w_ramp = d->w_ramp;
   c.rho_ramp = d->rho_ramp;
+  c.shaped = d->shaped ? 1 : 0;
+  c.lmax = d->shape_lmax;
+  c.shape = nullptr;
+  c.r_in = d->r_in;
+  c.r_out = d->r_out;
+  c.offset = d->offset;
+  c.thickness = d->thickness;
+  c.tw = nullptr;
+  c.t_f = 0.0;
+  c.trail = 0.0;
+  c.rate = 0;
+  c.general = d->interior != TGH_INTERIOR_NONE && (d->shaped || d->interior == TGH_INTERIOR_FITTED);
   {
     // the moving hole: translate(boost(bg, -v), center), distances from
     // center + v t, the core's and the layer's rates (src/interior.jl:390-549)
@@ -652,6 +693,12 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
   if ((e = hipMemset(p->W[0], 0, sizeof(double) * p->work_len)) != hipSuccess ||
       (e = hipMemset(p->W[1], 0, sizeof(double) * p->work_len)) != hipSuccess)
     return cleanup(TGH_EDEVICE, hipGetErrorString(e));
+  if (c.shaped) {
+    if ((e = hipMalloc(&p->d_shape, sizeof(d->shape))) != hipSuccess ||
+        (e = hipMemcpy(p->d_shape, d->shape, sizeof(d->shape), hipMemcpyHostToDevice)) != hipSuccess)
+      return cleanup(TGH_EDEVICE, std::string("uploading the shape: ") + hipGetErrorString(e));
+    c.shape = p->d_shape;
+  }
   g.bcoord = p->d_bcoord;
   g.lookup = p->d_lookup;
   g.blevel = nullptr;
@@ -694,6 +741,24 @@ int tgh_problem_create_forest(const tgh_desc* d, const int32_t* leaves, int32_t 
   return create_problem(d, leaves, nleaves, prolongation, out);
 }
 
+int tgh_set_target(tgh_problem* p, const double* target, double t_target, int32_t rate, double trail) {
+  if (!p) return fail(TGH_EINVAL, "NULL problem");
+  if (!std::isfinite(t_target) || !std::isfinite(trail))
+    return fail(TGH_EINVAL, "the target's time and the trail factor must be finite");
+  if (!(trail >= 0 && trail <= 1))
+    return fail(TGH_EINVAL, "the trail factor narrows the ramp by 1 - trail zeta and must lie in [0, 1]");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->cs.tw = target;
+  p->cs.t_f = t_target;
+  p->cs.rate = rate != 0;
+  p->cs.trail = trail;
+  if (p->graph_exec) {  // a captured graph holds the old arguments
+    HIP_TRY(hipGraphExecDestroy(p->graph_exec));
+    p->graph_exec = nullptr;
+  }
+  return TGH_OK;
+}
+
 int tgh_block_levels(const tgh_problem* p, int32_t* out) {
   if (!p || !out) return fail(TGH_EINVAL, "NULL argument");
   for (int b = 0; b < p->geom.nblocks; ++b) out[b] = p->forest ? p->level_host[b] : 0;
@@ -709,6 +774,7 @@ int tgh_problem_destroy(tgh_problem* p) {
   (void)hipFree(p->d_bcoord);
   (void)hipFree(p->d_lookup);
   (void)hipFree(p->d_level);
+  (void)hipFree(p->d_shape);
   (void)hipFree(p->d_prim);
   (void)hipFree(p->d_plan);
   (void)hipFree(p->W[0]);
@@ -759,6 +825,9 @@ int tgh_prepare(tgh_problem* p, const double* u, double t) {
 
 int tgh_rhs(tgh_problem* p, double* du, double t, const tgh_matter* matter) {
   if (!p || !du) return fail(TGH_EINVAL, "NULL argument");
+  if (p->cs.interior == INT_FITTED && !p->cs.tw)
+    return fail(TGH_EINVAL, "a :fitted interior relaxes toward a target the kernel reads from a cache: give "
+                            "the problem one with tgh_set_target (GHProblem's target)");
   if (matter && matter->rho && (!matter->S || !matter->Sij))
     return fail(TGH_EINVAL, "a matter adapter needs rho, S and Sij");
   RhsArgs a = rhs_args(p, p->W[0], t, matter);
@@ -879,9 +948,13 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
   if (nsteps < 0 || !(dt > 0)) return fail(TGH_EINVAL, "tgh_rk4 needs dt > 0 and nsteps >= 0");
   if (p->cs.interior == INT_PASTED)
     return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
+  if (p->cs.interior == INT_FITTED && !p->cs.tw)
+    return fail(TGH_EINVAL, "a :fitted interior relaxes toward a target the kernel reads from a cache: give "
+                            "the problem one with tgh_set_target (GHProblem's target)");
   if (nsteps == 0) return TGH_OK;
   const bool graph = (flags & TGH_RK4_GRAPH) != 0;
-  const bool blocks = (flags & TGH_RK4_BLOCKS) != 0 || p->forest;  // a forest steps on the block layout
+  // a forest with levels and the general interior step on the block layout
+  const bool blocks = (flags & TGH_RK4_BLOCKS) != 0 || p->forest || p->cs.general;
   if (flags & ~(TGH_RK4_GRAPH | TGH_RK4_FUSED | TGH_RK4_BLOCKS | TGH_RK4_SPLIT | TGH_RK4_PER_BLOCK))
     return fail(TGH_EINVAL, "unknown tgh_rk4 flags " + std::to_string(flags));
   if ((flags & TGH_RK4_PER_BLOCK) && (flags & TGH_RK4_SPLIT))
@@ -936,7 +1009,8 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     // one (u, dt, t) triple; the time only matters to the Dirichlet hook of a
     // moving background, so with a static one the graph is replayed across
     // steps and re-captured only when dt, u or the form change.
-    const bool static_bg = static_background(p->cs);
+    // (a :fitted target moves with t as well: A + (t - t_f) S)
+    const bool static_bg = static_background(p->cs) && p->cs.interior != INT_FITTED;
     for (int n = 0; n < nsteps; ++n) {
       const double t = t0 + n * dt;
       const bool reuse = p->graph_exec && p->graph_key[0] == dt &&
@@ -1142,9 +1216,9 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
                                   std::to_string(i) + " differs from problem 0");
     if (p->cs.interior == INT_PASTED)
       return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
-    if (p->forest)
-      return fail(TGH_EUNSUPPORTED, "problem " + std::to_string(i) + " of the batch is a forest with levels, "
-                  "which steps on the block layout; a batch steps bricks");
+    if (p->forest || p->cs.general)
+      return fail(TGH_EUNSUPPORTED, "problem " + std::to_string(i) + " of the batch is a forest with levels or "
+                  "has a tracked or :fitted interior, which step on the block layout; a batch steps bricks");
   }
   tgh_batch* b = new tgh_batch;
   b->probs.assign(probs, probs + n);

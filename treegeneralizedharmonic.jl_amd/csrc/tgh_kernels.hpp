@@ -12,7 +12,7 @@
 
 namespace tgh {
 
-enum { INT_NONE = 0, INT_DAMPED = 1, INT_FROZEN = 2, INT_PASTED = 3 };
+enum { INT_NONE = 0, INT_DAMPED = 1, INT_FROZEN = 2, INT_PASTED = 3, INT_FITTED = 4 };
 // Gauge-source kinds as the kernels are specialised: the case's kind, and for
 // a moving hole its own (its kernels carry the moving core and layer).
 enum { GAUGE_NONE = 0, GAUGE_SAMPLED = 1, GAUGE_KS = 2, GAUGE_KS_MOVING = 3, GAUGE_NONE_MOVING = 4 };
@@ -54,7 +54,61 @@ struct CaseP {
   int moving;        // vel != 0
   double delta;      // layer_rate_step: cbrt(eps) r_0, the targets' rate difference step
   double ks_u[4], ks_S[4];  // the moving KerrSchildSource: u = gamma (1, v), S the boosted spin
+  // FittedInterior's tracked geometry: the layer's radii along the ray from
+  // the surface series (device array of (lmax + 1)^2 coefficients) clamped
+  // into [r_in, r_out], less offset and thickness.
+  int shaped, lmax;
+  const double* shape;
+  double r_in, r_out, offset, thickness;
+  // The :fitted target cache (40 variables per owned point, A then S, in
+  // the state layout), its fill time, whether its rate is fed forward, and
+  // the trailing-side narrowing of the rho ramp.
+  const double* tw;
+  double t_f, trail;
+  int rate;
+  int general;  // shaped or :fitted: the kernels' general interior (HOLE = 2)
 };
+
+// shape_series (src/interior.jl:1294-1329): sum_lm a_lm y~_lm(n), the fully
+// normalised Legendre recurrences in z and (n_x + i n_y)^m for the azimuth,
+// in the oracle's order (no contraction).
+TGH_HD double shape_term(const double* sh, int l, int m, double q, double C, double S, double s2) {
+#pragma clang fp contract(off)
+  if (m == 0) return sh[l * l + l] * q;
+  const double ac = sh[l * l + l + m], as = sh[l * l + l - m];
+  return s2 * q * (ac * C - as * S);
+}
+TGH_HD double shape_series(const double* sh, int lmax, const double n[3]) {
+#pragma clang fp contract(off)
+  const double x = n[0], y = n[1], z = n[2];
+  const double s2 = sqrt(2.0);
+  double acc = 0.0;
+  double qmm = 1.0 / sqrt(4.0 * M_PI);
+  double C = 1.0, S = 0.0;
+  for (int m = 0; m <= lmax; ++m) {
+    if (m > 0) {
+      qmm = -sqrt((double)(2 * m + 1) / (double)(2 * m)) * qmm;
+      const double Cn = C * x - S * y, Sn = C * y + S * x;
+      C = Cn;
+      S = Sn;
+    }
+    acc += shape_term(sh, m, m, qmm, C, S, s2);
+    if (m + 1 <= lmax) {
+      double qa = qmm;
+      double qb = sqrt((double)(2 * m + 3)) * z * qmm;
+      acc += shape_term(sh, m + 1, m, qb, C, S, s2);
+      for (int l = m + 2; l <= lmax; ++l) {
+        const double al = sqrt((double)(4 * l * l - 1) / (double)(l * l - m * m));
+        const double be = sqrt((double)((l - 1) * (l - 1) - m * m) / (double)(4 * (l - 1) * (l - 1) - 1));
+        const double qc = al * (z * qb - be * qa);
+        acc += shape_term(sh, l, m, qc, C, S, s2);
+        qa = qb;
+        qb = qc;
+      }
+    }
+  }
+  return acc;
+}
 
 TGH_HD void position(const Geom& g, int b, int i, int j, int k, double x[3]) {
   const int* B = g.bcoord + 3 * b;
@@ -108,6 +162,57 @@ TGH_HD double hole_radius(const CaseP& c, double t, const double x[3]) {
   return sqrt(d1 * d1 + d2 * d2 + d3 * d3);
 }
 
+// interior_point (src/interior.jl:341,1868-1886): the distance from c(t) and
+// the layer's radii along the ray -- the sphere's r_0, r_1, or the tracked
+// surface's r_1 = r_h(n) - offset and r_0 = r_1 - thickness with the two
+// exact fast paths (outside r_out - offset, inside r_in - offset - thickness).
+struct IGeo {
+  double r, r0, r1;
+};
+TGH_HD IGeo interior_geo(const CaseP& c, double t, const double x[3]) {
+#pragma clang fp contract(off)
+  double cc[3];
+  center_at(c, t, cc);
+  const double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
+  const double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+  if (!c.shaped) return {r, c.r0, c.r1};
+  const double r1o = c.r_out - c.offset;
+  if (r >= r1o) return {r, r1o - c.thickness, r1o};
+  const double r1i = c.r_in - c.offset, r0i = r1i - c.thickness;
+  if (r < r0i) return {r, r0i, r1i};
+  double n[3] = {0.0, 0.0, 1.0};
+  if (r != 0.0) {
+    n[0] = d1 / r;
+    n[1] = d2 / r;
+    n[2] = d3 / r;
+  }
+  const double r1 = fmin(fmax(shape_series(c.shape, c.lmax, n), c.r_in), c.r_out) - c.offset;
+  return {r, r1 - c.thickness, r1};
+}
+// core_position on the tracked geometry (src/interior.jl:1950-1965): the core
+// surface's point on the ray, r_0(n) = (r_h(n) - offset) - thickness, where
+// r < r_0(n); x itself elsewhere.
+TGH_HD void shaped_core_point(const CaseP& c, double t, const double x[3], double xc[3]) {
+#pragma clang fp contract(off)
+  xc[0] = x[0];
+  xc[1] = x[1];
+  xc[2] = x[2];
+  double cc[3];
+  center_at(c, t, cc);
+  const double d[3] = {x[0] - cc[0], x[1] - cc[1], x[2] - cc[2]};
+  const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (r >= (c.r_out - c.offset) - c.thickness) return;
+  double n[3] = {0.0, 0.0, 1.0};
+  if (r != 0.0) {
+    n[0] = d[0] / r;
+    n[1] = d[1] / r;
+    n[2] = d[2] / r;
+  }
+  const double r0 = (fmin(fmax(shape_series(c.shape, c.lmax, n), c.r_in), c.r_out) - c.offset) - c.thickness;
+  if (r >= r0) return;
+  for (int q = 0; q < 3; ++q) xc[q] = cc[q] + r0 * n[q];
+}
+
 // core_position (src/interior.jl:615-625): inside the core, the point on the
 // sphere r_0 about c(t) along the ray (+z at the center).
 TGH_HD void core_point(const CaseP& c, double t, const double x[3], double xc[3]) {
@@ -136,6 +241,18 @@ TGH_HD void case_state(const CaseP& c, double t, const double x[3], double out[2
   double xc[3];
   core_point(c, t, x, xc);
   background_state<SET>(c.bg, t, xc, out, out + NC);
+}
+// case_state with the tracked geometry's core rule where the case has one
+// (the fill, the initial state, the error norms and the sampled source).
+template <int SET>
+TGH_HD void case_state_any(const CaseP& c, double t, const double x[3], double out[20]) {
+  if (c.shaped && c.interior != INT_NONE) {
+    double xc[3];
+    shaped_core_point(c, t, x, xc);
+    background_state<SET>(c.bg, t, xc, out, out + NC);
+    return;
+  }
+  case_state<SET>(c, t, x, out);
 }
 // The backgrounds a right-hand side compiled for a gauge-source kind can meet:
 // H_a = 0 the harmonic charts (at rest or moving), a sampled or the static
@@ -504,6 +621,7 @@ struct PointRef {
   int s2, s3;
   double x[3];
   double ih;  // 1/h of the point's block (the block layout)
+  long ti0;   // the :fitted target cache's index of variable 0 (the block layout)
   long si0, ssv, wi0;
   const double* __restrict__ Hp;
   long hs;
@@ -544,7 +662,7 @@ struct GlobalSrc {
 
 // The frozen core (F is not evaluated there: a hole at rest, du = 0) and the
 // point's radius from the hole's centre.
-template <bool HOLE>
+template <int HOLE>
 __device__ __forceinline__ bool frozen_core(const CaseP& c, double t, const double x[3], double& r) {
   r = 0.0;
   if (!HOLE) return false;
@@ -593,7 +711,7 @@ __device__ __forceinline__ void head_geometry(const Coef& cf, const double D[4][
 // matter, and the interior layer -- d_t h through the epilogue, and the
 // weight w and c1_v = w msrc_v - rho (Pi_v - Pi_exact,v) into pc. Pi_v is
 // read through src.P.
-template <int GAUGE, bool HOLE, int EPI, class SRC>
+template <int GAUGE, int HOLE, int EPI, class SRC>
 __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr, const SRC& src, double r,
                                             const Coef& cf, const double hv[NC], const double D[4][NC],
                                             PiCoef& pc, TGH_OUT_PARAMS) {
@@ -686,12 +804,145 @@ __device__ __forceinline__ void core_rates(const RhsArgs& a, const double x[3], 
   for (int v = 0; v < NC; ++v) dh[v] = dP[v] = 0.0;
 }
 
+// The general interior (HOLE = 2, the block layout): the tracked geometry's
+// radii along the ray or the sphere's, and the :fitted variant, whose core and
+// layer relax toward the target cache, A + (t - t_f) S, with its rate S fed
+// forward if asked and rho's ramp narrowed on the trailing side by trail
+// (gh_rhs_point!, src/evolution.jl:500-700; _trail_profiles :1256-1272;
+// _cached_target and _cached_rate, src/fit.jl:1050-1055).
+__device__ __forceinline__ double fitted_target(const CaseP& c, const PointRef& pr, double t, int v, double& S) {
+#pragma clang fp contract(off)
+  const double A = c.tw[pr.ti0 + (long)v * pr.ssv];
+  S = c.tw[pr.ti0 + (long)(20 + v) * pr.ssv];
+  return A + (t - c.t_f) * S;
+}
+template <int GAUGE, int EPI, class SRC>
+__device__ __forceinline__ void head_source_general(const RhsArgs& a, const PointRef& pr, const SRC& src,
+                                                    const IGeo& g, const Coef& cf, const double hv[NC],
+                                                    const double D[4][NC], PiCoef& pc, TGH_OUT_PARAMS) {
+  const long sv = pr.sv, psz = pr.ssv, si0 = pr.si0, wi0 = pr.wi0;
+  const CaseP& c = a.c;
+  const double g0 = damping_rate(c, a.t, pr.x);
+  double msrc[NC];
+  {
+    double Hl[4] = {0.0, 0.0, 0.0, 0.0}, dH[4][4];
+    if (GAUGE == GAUGE_SAMPLED) {
+      const double* Hp = pr.Hp;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Hl[q] = Hp[q * pr.hs];
+#pragma unroll
+      for (int n = 0; n < 16; ++n) dH[n % 4][n / 4] = Hp[(4 + n) * pr.hs];
+    } else if (GAUGE == GAUGE_KS) {
+      ks_source(c.bg.M, c.bg.a, hv, D, Hl, dH);
+    } else if (GAUGE == GAUGE_KS_MOVING) {
+      ks_source_moving(c.bg.M, c.ks_S, c.ks_u, hv, D, Hl, dH);
+    }
+    gh_source<gauge_has_source(GAUGE)>(cf, hv, D, Hl, dH, g0, c.g2, msrc);
+  }
+  if (pr.m_rho != nullptr) {
+    const double rr = pr.m_rho[0];
+    const double Sm[3] = {pr.m_S[0], pr.m_S[pr.ms], pr.m_S[2 * pr.ms]};
+    double Sij[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) Sij[n] = pr.m_Sij[n * pr.ms];
+    double ms[NC];
+    matter_source(cf, hv, rr, Sm, Sij, ms);
+#pragma unroll
+    for (int n = 0; n < NC; ++n) msrc[n] += ms[n];
+  }
+  const bool layer = g.r < g.r1;
+  const bool fitted = c.interior == INT_FITTED;
+  double wgt = 1.0, rho = 0.0;
+  if (layer) {
+#pragma clang fp contract(off)
+    double rramp = c.rho_ramp;
+    if (fitted && c.trail != 0.0) {
+      double cc[3];
+      center_at(c, a.t, cc);
+      const double sp = sqrt(c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2]);
+      double zeta = 0.0;
+      if (!(sp == 0.0 || g.r == 0.0))
+        zeta = fmax(0.0, -((pr.x[0] - cc[0]) * c.vel[0] + (pr.x[1] - cc[1]) * c.vel[1] +
+                           (pr.x[2] - cc[2]) * c.vel[2]) / (g.r * sp));
+      rramp = c.rho_ramp * (1.0 - c.trail * zeta);
+    }
+    const double sl = (g.r - g.r0) / (g.r1 - g.r0);
+    wgt = smoothstep(sl / c.w_ramp);
+    rho = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / rramp);
+  }
+  // the :damped / :frozen layer's target: the background at x (r >= r_0 here)
+  double ue[20];
+  if (layer && !fitted) background_state<bg_set_for(GAUGE)>(c.bg, a.t, pr.x, ue, ue + NC);
+  const double om = 1.0 - wgt;
+#pragma unroll 1
+  for (int v = 0; v < NC; ++v) {
+    double f = D[0][v];
+    if (layer) {
+      if (fitted) {
+        double S;
+        const double tg = fitted_target(c, pr, a.t, v, S);
+        f = wgt * f + (c.rate ? om * S : 0.0) - rho * (hv[v] - tg);
+      } else {
+        f = wgt * f - rho * (hv[v] - ue[v]);
+      }
+    }
+    emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, f);
+  }
+  pc.w = layer ? wgt : 1.0;
+#pragma unroll 1
+  for (int v = 0; v < NC; ++v) {
+    double c1 = msrc[v];
+    if (layer) {
+      if (fitted) {
+        double S;
+        const double tg = fitted_target(c, pr, a.t, NC + v, S);
+        c1 = wgt * c1 + (c.rate ? om * S : 0.0) - rho * (src.P(v) - tg);
+      } else {
+        c1 = wgt * c1 - rho * (src.P(v) - ue[NC + v]);
+      }
+    }
+    pc.c1[v] = c1;
+  }
+}
+
 // The whole head at one point (gh_rhs_head, src/evolution.jl:258-297, inside
 // gh_rhs_point!, :448-695,846-863). Returns false in the frozen core, where
 // the core's d_t h is emitted and its d_t Pi left in pc.c1 (w = -1).
-template <int Q, int GAUGE, bool HOLE, int EPI, class SRC>
+template <int Q, int GAUGE, int HOLE, int EPI, class SRC>
 __device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr, const SRC& src, double inv_h,
                                            PiCoef& pc, TGH_OUT_PARAMS) {
+  if constexpr (HOLE == 2) {
+    const CaseP& c = a.c;
+    const IGeo g = interior_geo(c, a.t, pr.x);
+    if (c.interior == INT_PASTED ? g.r < g.r1 : g.r < g.r0) {
+      // the core: zero for a hole at rest, the relaxation toward the target
+      // at rho_max for :fitted, du = [S] - rho_max (u - (A + (t - t_f) S))
+      double dh[NC];
+#pragma unroll 1
+      for (int v = 0; v < NC; ++v) {
+        dh[v] = 0.0;
+        pc.c1[v] = 0.0;
+        if (c.interior == INT_FITTED) {
+#pragma clang fp contract(off)
+          double S;
+          double tg = fitted_target(c, pr, a.t, v, S);
+          dh[v] = (c.rate ? S : 0.0) - c.rho_max * (src.X(v)[0] - tg);
+          tg = fitted_target(c, pr, a.t, NC + v, S);
+          pc.c1[v] = (c.rate ? S : 0.0) - c.rho_max * (src.P(v) - tg);
+        }
+        emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * pr.ssv, pr.wi0 + v * pr.sv, dh[v]);
+      }
+      pc.w = -1.0;
+      return false;
+    }
+    double hv[NC], D[4][NC];
+    Coef cf;
+    pc.epsh = dissipation_rate(c, a.t, pr.x) * inv_h;
+    head_derivs<Q>(c, src, inv_h, pc.epsh, hv, D, cf);
+    head_source_general<GAUGE, EPI>(a, pr, src, g, cf, hv, D, pc, TGH_OUT_ARGS);
+    head_geometry(cf, D, pc);
+    return true;
+  }
   double r;
   if (frozen_core<HOLE>(a.c, a.t, pr.x, r)) {
     double dh[NC];
@@ -735,7 +986,7 @@ __device__ __forceinline__ double pi_principal(const double* ph, const double* p
 
 // The fused right-hand side at one point in the streaming order of
 // src/evolution.jl:258-388: the head, then the ten Pi components one at a time.
-template <int Q, int GAUGE, bool HOLE, int EPI>
+template <int Q, int GAUGE, int HOLE, int EPI>
 __device__ __forceinline__ void rhs_point(const RhsArgs& a, const PointRef& pr, double inv_h, TGH_OUT_PARAMS) {
   PiCoef pc;
   const long sv = pr.sv, psz = pr.ssv;
@@ -819,6 +1070,7 @@ __device__ __forceinline__ bool block_point(const RhsArgs& a, const double* __re
   pr.s3 = S * S;
   position(g, b, i, j, k, pr.x);
   pr.ih = block_inv_h(g, b);
+  pr.ti0 = (long)b * 40 * psz + p;
   pr.si0 = (long)b * 20 * psz + p;
   pr.ssv = psz;
   pr.wi0 = (long)b * 20 * sv + off;
@@ -832,7 +1084,7 @@ __device__ __forceinline__ bool block_point(const RhsArgs& a, const double* __re
   return true;
 }
 
-template <int Q, int GAUGE, bool HOLE, int EPI>
+template <int Q, int GAUGE, int HOLE, int EPI>
 __global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict__ W, TGH_OUT_PARAMS) {
   PointRef pr;
   long pi;
@@ -842,7 +1094,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict
 
 // The split form, first half: the head at one owned point per thread, leaving
 // the Pi components' coefficients in coef (CF_N arrays of nblocks (N+1)^3).
-template <int Q, int GAUGE, bool HOLE, int EPI>
+template <int Q, int GAUGE, int HOLE, int EPI>
 __global__ __launch_bounds__(256) void k_head(RhsArgs a, const double* __restrict__ W,
                                               double* __restrict__ coef, TGH_OUT_PARAMS) {
   PointRef pr;
@@ -916,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
   if (cl == -2) {
     double x[3], s[20];
     position(g, b, l[0], l[1], l[2], x);
-    case_state<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
+    case_state_any<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
 #pragma unroll
     for (int v = 0; v < 20; ++v) dst[v * sv] = s[v];
   } else {
@@ -964,7 +1216,7 @@ __global__ __launch_bounds__(256) void k_fill_plan(PlanArgs a) {
   if (o.mode == PLAN_DIRICHLET) {
     double x[3], s[20];
     position(g, b, p % S - G, (p / S) % S - G, p / (S * S) - G, x);
-    case_state<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
+    case_state_any<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
 #pragma unroll
     for (int v = 0; v < 20; ++v) dst[v * S3] = s[v];
     return;
@@ -1039,7 +1291,8 @@ __global__ __launch_bounds__(256) void k_sample_gauge(SampleArgs a) {
   const long psz = (long)P * P * P;
   double x[3], xc[3];
   position(g, b, i, j, k, x);
-  core_point(a.c, a.t, x, xc);
+  if (a.c.shaped && a.c.interior != INT_NONE) shaped_core_point(a.c, a.t, x, xc);
+  else core_point(a.c, a.t, x, xc);
   double Hl[4], dH[4][4];
   gauge_source_grad(a.c.bg, a.t, xc, Hl, dH);
   double* out = a.Hsrc + (long)b * 20 * psz + p;
@@ -1076,7 +1329,7 @@ __global__ __launch_bounds__(256) void k_pointwise(PointArgs a) {
   }
   if (a.mode == 2 && !(hole_radius(a.c, a.t, x) < a.c.r1)) return;
   double s[20];
-  case_state<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
+  case_state_any<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
   for (int v = 0; v < 20; ++v) ub[v * psz] = s[v];
 }
 
@@ -1093,7 +1346,12 @@ struct ReduceArgs {
   int mode;                  // 0 speed, 1 error
 };
 __device__ __forceinline__ bool is_evolved(const CaseP& c, double t, const double x[3]) {
-  return c.interior == INT_NONE || hole_radius(c, t, x) >= c.r1;
+  if (c.interior == INT_NONE) return true;
+  if (c.shaped) {
+    const IGeo g = interior_geo(c, t, x);
+    return g.r >= g.r1;
+  }
+  return hole_radius(c, t, x) >= c.r1;
 }
 template <bool MOVE>
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
@@ -1126,7 +1384,7 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
              sqrt(cf.beta[0] * cf.beta[0] + cf.beta[1] * cf.beta[1] + cf.beta[2] * cf.beta[2]);
       } else {
         double s[20];
-        case_state<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
+        case_state_any<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
         double e2 = 0.0;
 #pragma unroll
         for (int v = 0; v < 20; ++v) {
@@ -1275,7 +1533,7 @@ __device__ __forceinline__ bool tile_map(const Brick& k, int TX, int TY, int TZ,
 // The fused right-hand side with the RK4 epilogue on the brick layout: the
 // stage input W, its successor Wn, u_n and the accumulator all in the brick
 // layout, so one index addresses a point in every one of them.
-template <int Q, int GAUGE, bool HOLE>
+template <int Q, int GAUGE, int HOLE>
 __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const double* __restrict__ W,
                                                    TGH_OUT_PARAMS) {
   int x, y, z;
@@ -1309,11 +1567,11 @@ __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const dou
 #else
 #define TGH_HEAD_ATTR
 #endif
-template <int Q, int GAUGE, bool HOLE>
+template <int Q, int GAUGE, int HOLE>
 __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
                                               const double* __restrict__ Hsrc, double* __restrict__ coef,
                                               TGH_OUT_PARAMS, int x, int y, int z);
-template <int Q, int GAUGE, bool HOLE>
+template <int Q, int GAUGE, int HOLE>
 __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head(RhsArgs a, Brick k, const double* __restrict__ W,
                                                     double* __restrict__ coef, TGH_OUT_PARAMS) {
   int x, y, z;
@@ -1333,7 +1591,7 @@ __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head(RhsArgs a, Bri
 #endif
   brick_head_at<Q, GAUGE, HOLE>(a, k, W, a.Hsrc, coef, TGH_OUT_ARGS, x, y, z);
 }
-template <int Q, int GAUGE, bool HOLE>
+template <int Q, int GAUGE, int HOLE>
 __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
                                               const double* __restrict__ Hsrc, double* __restrict__ coef,
                                               TGH_OUT_PARAMS, int x, int y, int z) {
@@ -1584,7 +1842,7 @@ __device__ __forceinline__ int batch_find(const BatchEntry* __restrict__ tab, in
   while (p + 1 < n && L >= (pi ? tab[p + 1].pi0 : tab[p + 1].head0)) ++p;
   return p;
 }
-template <int Q, int GAUGE, bool HOLE>
+template <int Q, int GAUGE, int HOLE>
 __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head_batch(const BatchEntry* __restrict__ tab, int n,
                                                                         int total) {
   const int L = wg_logical(total);

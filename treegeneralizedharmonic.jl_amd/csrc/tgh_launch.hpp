@@ -10,7 +10,7 @@ namespace tgh {
 // One right-hand side: the fused kernel, or the head and the Pi components as
 // two launches. While profiling, the head is timed as the "rhs" pair and the
 // Pi kernel as its own pair.
-template <int Q, int GAUGE, bool HOLE, int EPI>
+template <int Q, int GAUGE, int HOLE, int EPI>
 void launch_rhs_q(tgh_problem* p, const RhsArgs& a) {
   const bool prof = p->profiling && EPI == EPI_RK4;
   if (p->per_block) {
@@ -31,6 +31,7 @@ void launch_rhs_q(tgh_problem* p, const RhsArgs& a) {
       if (a.acc) ab.acc = a.acc + ss;
       if (a.du) ab.du = a.du + ss;
       if (a.Hsrc) ab.Hsrc = a.Hsrc + ss;
+      if (a.c.tw) ab.c.tw = a.c.tw + 2 * ss;  // the target cache: 40 variables a point
       hipLaunchKernelGGL((k_rhs<Q, GAUGE, HOLE, EPI>), grid, dim3(256), 0, p->stream, ab, ab.W, ab.du, ab.us,
                          ab.acc, ab.Wn);
     }
@@ -57,6 +58,17 @@ void launch_rhs_q(tgh_problem* p, const RhsArgs& a) {
 template <int Q, int EPI>
 void launch_rhs_g(tgh_problem* p, const RhsArgs& a) {
   const bool hole = p->cs.interior != INT_NONE;
+  if (hole && p->cs.general) {
+    // the general interior (tracked geometry, :fitted): its own specialisation
+    switch (p->cs.gauge) {
+      case GAUGE_KS: launch_rhs_q<Q, GAUGE_KS, 2, EPI>(p, a); break;
+      case GAUGE_SAMPLED: launch_rhs_q<Q, GAUGE_SAMPLED, 2, EPI>(p, a); break;
+      case GAUGE_KS_MOVING: launch_rhs_q<Q, GAUGE_KS_MOVING, 2, EPI>(p, a); break;
+      case GAUGE_NONE_MOVING: launch_rhs_q<Q, GAUGE_NONE_MOVING, 2, EPI>(p, a); break;
+      default: launch_rhs_q<Q, GAUGE_NONE, 2, EPI>(p, a); break;
+    }
+    return;
+  }
   switch (p->cs.gauge) {
     case GAUGE_KS:
       if (hole) launch_rhs_q<Q, GAUGE_KS, true, EPI>(p, a);
@@ -86,7 +98,7 @@ void launch_rhs_blocks(tgh_problem* p, const RhsArgs& a, int epi) {
     launch_rhs_g<Q, EPI_STORE>(p, a);
 }
 
-template <int Q, int GAUGE, bool HOLE>
+template <int Q, int GAUGE, int HOLE>
 void launch_brick_rhs_q(tgh_problem* p, const RhsArgs& a) {
   const bool prof = p->profiling;
   int e0 = prof ? mark(p) : -1;
@@ -147,7 +159,7 @@ void launch_rhs_brick(tgh_problem* p, const RhsArgs& a) {
 
 // One RK4 stage of a batch of problems: the head and the Pi kernel, each one
 // launch over every problem's runs or tiles (the table's ranges).
-template <int Q, int GAUGE, bool HOLE>
+template <int Q, int GAUGE, int HOLE>
 void launch_batch_q(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total) {
   using T = PiTile<Q>;
   hipLaunchKernelGGL((k_brick_head_batch<Q, GAUGE, HOLE>), xcd_grid(head_total), dim3(256), 0, st, tab, n,

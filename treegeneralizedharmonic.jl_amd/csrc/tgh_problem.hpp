@@ -34,6 +34,7 @@ struct tgh_problem {
   std::vector<int> plan_off;
   int prolong = 0;
   double wmid[10] = {0};
+  double* d_shape = nullptr;          // the tracked surface's coefficients
   double* W[2] = {nullptr, nullptr};  // work arrays (prepare uses W[0])
   double* acc = nullptr;              // RK4 accumulator, state layout
   double* coef = nullptr;             // the split form's per-point coefficients (CF_N x points)
