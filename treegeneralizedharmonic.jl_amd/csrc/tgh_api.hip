@@ -224,6 +224,10 @@ int check_desc(const tgh_desc* d, int lmax = 0) {
   return TGH_OK;
 }
 
+template <int NP>
+void launch_plan(dim3 grid, hipStream_t st, const PlanArgs& a) {
+  hipLaunchKernelGGL((k_fill_plan<NP>), grid, dim3(256), 0, st, a);
+}
 void launch_fill(tgh_problem* p, double* W, double t) {
   if (p->forest) {
     // the plan's passes, in order on the stream
@@ -233,14 +237,26 @@ void launch_fill(tgh_problem* p, double* W, double t) {
     a.c = p->cs;
     a.t = t;
     a.W = W;
-    a.np = p->prolong;
-    for (int k = 0; k < 10; ++k) a.wm[k] = p->wmid[k];
+    // pass 0's Dirichlet tail on its own kernel (it evaluates the background)
+    if (p->plan_dir[1] > p->plan_dir[0]) {
+      a.n = p->plan_dir[1] - p->plan_dir[0];
+      a.ops = (const PlanOp*)p->d_plan + p->plan_dir[0];
+      hipLaunchKernelGGL(p->cs.moving ? k_fill_plan_dirichlet<true> : k_fill_plan_dirichlet<false>,
+                         dim3((a.n + 255) / 256), dim3(256), 0, p->stream, a);
+    }
     for (size_t s = 0; s + 1 < p->plan_off.size(); ++s) {
-      a.n = p->plan_off[s + 1] - p->plan_off[s];
+      const int end = s == 0 ? p->plan_dir[0] : p->plan_off[s + 1];
+      a.n = end - p->plan_off[s];
       if (a.n == 0) continue;
       a.ops = (const PlanOp*)p->d_plan + p->plan_off[s];
-      hipLaunchKernelGGL(p->cs.moving ? k_fill_plan<true> : k_fill_plan<false>, dim3((a.n + 255) / 256), dim3(256),
-                         0, p->stream, a);
+      const dim3 grid((a.n + 255) / 256, 20);
+      switch (p->prolong) {
+        case 2: launch_plan<2>(grid, p->stream, a); break;
+        case 4: launch_plan<4>(grid, p->stream, a); break;
+        case 6: launch_plan<6>(grid, p->stream, a); break;
+        case 8: launch_plan<8>(grid, p->stream, a); break;
+        default: launch_plan<10>(grid, p->stream, a); break;
+      }
     }
     return;
   }
@@ -577,8 +593,9 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
     p->bcoord_host = fbc;
     p->level_host = lev;
     p->prolong = prolong;
-    midpoint_weights(prolong, p->wmid);
     p->plan_off = plan.pass_off;
+    p->plan_dir[0] = plan.dir_begin;
+    p->plan_dir[1] = plan.dir_end;
     lookup.assign(1, 0);
   } else {
     // Blocks in Morton order of the brick.

@@ -32,6 +32,7 @@ struct ForestPlan {
   std::vector<int> pass_off;       // entry offsets of the passes (Lmax + 2 values)
   std::vector<unsigned char> prim; // per owned point: 1 if its block's primary
   int Lmax = 0;
+  int dir_begin = 0, dir_end = 0;  // pass 0's Dirichlet entries (its tail)
 };
 
 // Builds the plan, or returns an error message.
@@ -228,27 +229,20 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
           pass[L].insert(pass[L].end(), {dst, src, half, 0});
         }
   }
+  // Within a pass the entries are independent: order them by mode (stable,
+  // so each mode's entries stay in destination order) for wave-uniform branches.
   out.ops.clear();
   out.pass_off.assign(1, 0);
   for (int l = 0; l <= Lmax; ++l) {
-    out.ops.insert(out.ops.end(), pass[l].begin(), pass[l].end());
+    for (int mode = 0; mode <= 8; ++mode) {
+      if (l == 0 && mode == 8) out.dir_begin = (int)(out.ops.size() / 4);
+      for (size_t e = 0; e < pass[l].size(); e += 4)
+        if (pass[l][e + 2] == mode) out.ops.insert(out.ops.end(), pass[l].begin() + e, pass[l].begin() + e + 4);
+    }
     out.pass_off.push_back((int)(out.ops.size() / 4));
   }
+  out.dir_end = out.pass_off[1];
   return "";
-}
-
-// The midpoint Lagrange weights of p nodes at -p/2+1 .. p/2: each a product
-// of exact factors, divided once.
-inline void midpoint_weights(int np, double* w) {
-  for (int k = 0; k < np; ++k) {
-    double num = 1.0, den = 1.0;
-    for (int j = 0; j < np; ++j) {
-      if (j == k) continue;
-      num *= 0.5 - (double)(j - np / 2 + 1);
-      den *= (double)(k - j);
-    }
-    w[k] = num / den;
-  }
 }
 
 }  // namespace tgh

@@ -1200,48 +1200,93 @@ struct PlanArgs {
   double* __restrict__ W;
   const PlanOp* __restrict__ ops;
   int n;
-  int np;          // the prolongation order: points per midpoint axis
-  double wm[10];   // the midpoint Lagrange weights
 };
-template <bool MOVE>
+// The midpoint Lagrange weights of NP nodes at -NP/2+1 .. NP/2 (exact
+// dyadic rationals, equal to the host's product formula).
+template <int NP> struct MidW;
+template <> struct MidW<2> { static constexpr double w[2] = {1.0 / 2, 1.0 / 2}; };
+template <> struct MidW<4> { static constexpr double w[4] = {-1.0 / 16, 9.0 / 16, 9.0 / 16, -1.0 / 16}; };
+template <> struct MidW<6> {
+  static constexpr double w[6] = {3.0 / 256, -25.0 / 256, 150.0 / 256, 150.0 / 256, -25.0 / 256, 3.0 / 256};
+};
+template <> struct MidW<8> {
+  static constexpr double w[8] = {-5.0 / 2048,    49.0 / 2048,  -245.0 / 2048, 1225.0 / 2048,
+                                  1225.0 / 2048, -245.0 / 2048, 49.0 / 2048,   -5.0 / 2048};
+};
+template <> struct MidW<10> {
+  static constexpr double w[10] = {35.0 / 65536,    -405.0 / 65536, 2268.0 / 65536,  -8820.0 / 65536,
+                                   39690.0 / 65536, 39690.0 / 65536, -8820.0 / 65536, 2268.0 / 65536,
+                                   -405.0 / 65536,  35.0 / 65536};
+};
+// One variable's tensor-product interpolation from the stencil's first point f
+// (strides 1, S, S^2): NX, NY, NZ points per axis (1 or NP), sum-factorised
+// with x innermost, each sum from 0 in stencil order, no contraction.
+template <int NX, int NY, int NZ, int NP>
+__device__ __forceinline__ double prolong_sum(const double* __restrict__ f, int S) {
+#pragma clang fp contract(off)
+  double sz = 0.0;
+#pragma unroll
+  for (int c2 = 0; c2 < NZ; ++c2) {
+    double sy = 0.0;
+#pragma unroll
+    for (int c1 = 0; c1 < NY; ++c1) {
+      double sx = 0.0;
+#pragma unroll
+      for (int c0 = 0; c0 < NX; ++c0) sx += (NX > 1 ? MidW<NP>::w[c0] : 1.0) * f[c0 + S * (c1 + S * c2)];
+      sy += (NY > 1 ? MidW<NP>::w[c1] : 1.0) * sx;
+    }
+    sz += (NZ > 1 ? MidW<NP>::w[c2] : 1.0) * sy;
+  }
+  return sz;
+}
+// The copies and prolongations: one thread per entry and variable
+// (blockIdx.y = v); a pass's entries are sorted by mode on the host, so a
+// wave takes one branch.
+template <int NP>
 __global__ __launch_bounds__(256) void k_fill_plan(PlanArgs a) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.n) return;
+  const int v = blockIdx.y;
   const PlanOp o = a.ops[e];
+  if (o.mode == PLAN_DIRICHLET) return;  // k_fill_plan_dirichlet's
+  const Geom& g = a.g;
+  const int S = g.S;
+  const long S3 = (long)S * S * S;
+  const int b = o.dst / (int)S3, p = o.dst - b * (int)S3;
+  double* dst = a.W + (long)b * 20 * S3 + p;
+  const int sb = o.src / (int)S3, sp = o.src - sb * (int)S3;
+  const double* f = a.W + (long)sb * 20 * S3 + sp + (long)v * S3;
+  double r;
+  switch (o.mode) {
+    case 0: r = prolong_sum<1, 1, 1, NP>(f, S); break;
+    case 1: r = prolong_sum<NP, 1, 1, NP>(f, S); break;
+    case 2: r = prolong_sum<1, NP, 1, NP>(f, S); break;
+    case 3: r = prolong_sum<NP, NP, 1, NP>(f, S); break;
+    case 4: r = prolong_sum<1, 1, NP, NP>(f, S); break;
+    case 5: r = prolong_sum<NP, 1, NP, NP>(f, S); break;
+    case 6: r = prolong_sum<1, NP, NP, NP>(f, S); break;
+    default: r = prolong_sum<NP, NP, NP, NP>(f, S); break;
+  }
+  dst[v * S3] = r;
+}
+// The Dirichlet entries: one thread per entry, the exact state at the
+// ghost's own position.
+template <bool MOVE>
+__global__ __launch_bounds__(256) void k_fill_plan_dirichlet(PlanArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.n) return;
+  const PlanOp o = a.ops[e];
+  if (o.mode != PLAN_DIRICHLET) return;
   const Geom& g = a.g;
   const int S = g.S, G = g.G;
   const long S3 = (long)S * S * S;
   const int b = o.dst / (int)S3, p = o.dst - b * (int)S3;
   double* dst = a.W + (long)b * 20 * S3 + p;
-  if (o.mode == PLAN_DIRICHLET) {
-    double x[3], s[20];
-    position(g, b, p % S - G, (p / S) % S - G, p / (S * S) - G, x);
-    case_state_any<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
+  double x[3], s[20];
+  position(g, b, p % S - G, (p / S) % S - G, p / (S * S) - G, x);
+  case_state_any<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
 #pragma unroll
-    for (int v = 0; v < 20; ++v) dst[v * S3] = s[v];
-    return;
-  }
-  const int sb = o.src / (int)S3, sp = o.src - sb * (int)S3;
-  const double* src = a.W + (long)sb * 20 * S3 + sp;
-  const int nx = (o.mode & 1) ? a.np : 1, ny = (o.mode & 2) ? a.np : 1, nz = (o.mode & 4) ? a.np : 1;
-  // sum-factorised, x innermost, each sum from its first point (rounded as
-  // written: no contraction)
-#pragma unroll 1
-  for (int v = 0; v < 20; ++v) {
-#pragma clang fp contract(off)
-    const double* f = src + v * S3;
-    double sz = 0.0;
-    for (int c2 = 0; c2 < nz; ++c2) {
-      double sy = 0.0;
-      for (int c1 = 0; c1 < ny; ++c1) {
-        double sx = 0.0;
-        for (int c0 = 0; c0 < nx; ++c0) sx += (nx > 1 ? a.wm[c0] : 1.0) * f[c0 + S * (c1 + S * c2)];
-        sy += (ny > 1 ? a.wm[c1] : 1.0) * sx;
-      }
-      sz += (nz > 1 ? a.wm[c2] : 1.0) * sy;
-    }
-    dst[v * S3] = sz;
-  }
+  for (int q = 0; q < 20; ++q) dst[q * S3] = s[q];
 }
 
 // scatter! / gather! between the state and a work array's owned points.

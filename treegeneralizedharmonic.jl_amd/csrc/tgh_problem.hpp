@@ -32,8 +32,8 @@ struct tgh_problem {
   unsigned char* d_prim = nullptr;
   int* d_plan = nullptr;
   std::vector<int> plan_off;
+  int plan_dir[2] = {0, 0};           // pass 0's Dirichlet entries
   int prolong = 0;
-  double wmid[10] = {0};
   double* d_shape = nullptr;          // the tracked surface's coefficients
   double* W[2] = {nullptr, nullptr};  // work arrays (prepare uses W[0])
   double* acc = nullptr;              // RK4 accumulator, state layout
