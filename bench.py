@@ -60,6 +60,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain (verbs) form")
     ap.add_argument("--no-per-block", action="store_true", help="skip the plain per-block form")
     ap.add_argument("--no-batch", action="store_true", help="skip the four-hole batch")
+    ap.add_argument("--no-forest", action="store_true", help="skip the three-level forest")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a 1 GiB device-to-device copy, the known-size reference for "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -189,6 +190,51 @@ def batch_bench(tgh, torch, steps):
             "blocks": sum(r ** 3 for _, r in BATCH_HOLES), "points": points, "steps": steps,
             "batched_graph": batched, "sequential_graphs": sequential, "per_block": per_block,
             "batched_over_per_block": batched / per_block}
+
+
+def forest_bench(tgh, torch, steps):
+    """The refined hierarchy of the batching workload: hole_forest around the
+    a = 0.7 hole (three levels, finest h = 5M/256, 2416 leaves of 16^3), RK4 on
+    the block layout with the planned fill, as one hipGraph per step (one RHS
+    launch over all levels and one fill launch per pass) against the plain
+    per-block form (one launch per leaf and stage); the fill's and the right-
+    hand side's share from hipEvents."""
+    L = tgh._load()
+    case = tgh.kerr_schild_case(M=1.0, a=0.7, halfwidth=5.0, r_0=0.71, r_1=1.21, gauge_source="algebraic")
+    forest = tgh.hole_forest(None, case, N=16, roots=8, radii=(3.0, 1.5))
+    p = tgh.GHProblem(forest, case, q=4, ops=tgh.Operators(prolongation=6))
+    u = p.statevector()
+    tgh.initial_state(u, p, 0.0)
+    dt = tgh.gh_dt(p, u, cfl=0.25)
+    points = forest.nleaves() * 16 ** 3
+    ptr = C.c_void_p(u.data_ptr())
+    tgh._check(L.tgh_rk4(p.ptr, ptr, 0.0, dt, 1, 1))  # warm-up and capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tgh._check(L.tgh_rk4(p.ptr, ptr, dt, dt, steps, 1))
+    torch.cuda.synchronize()
+    graph = points * steps / (time.perf_counter() - t0)
+    tgh._check(L.tgh_profile(p.ptr, 1))
+    tgh._check(L.tgh_rk4(p.ptr, ptr, 0.0, dt, 1, 0))
+    prof = (C.c_double * 6)()
+    tgh._check(L.tgh_profile_read(p.ptr, prof))
+    tgh._check(L.tgh_profile(p.ptr, 0))
+    rhs_ms = prof[0] / max(prof[1], 1)
+    pi_ms = prof[4] / max(prof[5], 1)
+    fill_ms = prof[2] / max(prof[3], 1)
+    t0 = time.perf_counter()
+    tgh._check(L.tgh_rk4(p.ptr, ptr, 0.0, dt, 1, tgh.TGH_RK4_PER_BLOCK))
+    torch.cuda.synchronize()
+    per_block = points / (time.perf_counter() - t0)
+    assert bool(torch.isfinite(u).all()), "the forest's state is not finite"
+    levels = [sum(1 for lf in forest.leaves if lf[0] == k) for k in range(forest.maxlevel() + 1)]
+    return {"workload": "static Kerr-Schild hole M=1 a=0.7 (closed-form gauge source, damped interior) on "
+                        "hole_forest(N=16, roots=8 over [-5M,5M]^3, radii=(3M, 1.5M)): three levels, finest "
+                        "h = 5M/256, prolongation p = 6, block layout",
+            "leaves": forest.nleaves(), "leaves_per_level": levels, "points": points, "steps": steps,
+            "graph": graph, "per_block": per_block, "graph_over_per_block": graph / per_block,
+            "rhs_ms": rhs_ms + pi_ms, "fill_ms": fill_ms,
+            "fill_fraction": fill_ms / (fill_ms + rhs_ms + pi_ms)}
 
 
 def main():
@@ -326,6 +372,9 @@ def main():
     }
     L.tgh_problem_destroy(ptr)
     del k, y
+    if not args.no_forest:
+        torch.cuda.empty_cache()
+        result["forest"] = forest_bench(tgh, torch, max(1, min(5, args.steps)))
     if not args.no_batch:
         torch.cuda.empty_cache()
         result["batch"] = batch_bench(tgh, torch, max(1, min(5, args.steps)))

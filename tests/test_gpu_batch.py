@@ -123,3 +123,53 @@ def test_batch_of_moving_holes_steps_each_as_alone(tgh):
         case = tgh.harmonic_kerr_case(M=1.0, a=a, v=v, x0=0.25, axis=1, halfwidth=roots / 2, r_0=0.35, r_1=0.9)
         probs.append(tgh.GHProblem(tgh.gh_forest(None, case, N=16, roots=roots), case, q=4))
     check_batch_equals_alone(tgh, probs, 2, True)
+
+
+def _forest_problem(tgh, a, halfwidth, roots, radii):
+    case = tgh.kerr_schild_case(M=1.0, a=a, halfwidth=halfwidth, r_0=0.71, r_1=1.21, gauge_source="algebraic")
+    f = tgh.hole_forest(None, case, N=16, roots=roots, radii=radii)
+    return tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=6))
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_block_batch_of_forests_and_a_uniform_hole_steps_each_as_alone(tgh, graph):
+    """Forests with levels among the problems: the batch runs on the block
+    layout, one fused launch over every block of every problem per stage (all
+    levels of all holes), and each problem's result is bit for bit its own
+    fused block-layout step."""
+    torch = torch_cuda()
+    probs = [_forest_problem(tgh, 0.5, 2.0, 4, (1.2,)), _forest_problem(tgh, 0.7, 1.5, 3, (0.9,)),
+             ks_problem(tgh, 0.3, 3)]
+    assert probs[0].forest.maxlevel() == 1 and probs[1].forest.maxlevel() == 1
+    us = noisy_states(tgh, probs)
+    ref = [u.clone() for u in us]
+    dt = min(tgh.gh_dt(p, u, cfl=0.25) for p, u in zip(probs, us))
+    L = tgh._load()
+    flags = (4 | 2) | (1 if graph else 0)  # the block layout, fused, as the batch runs it
+    for p, r in zip(probs, ref):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(r.data_ptr()), 0.0, dt, 3, flags))
+    batch = tgh.GHBatch(probs)
+    uarr = (C.c_void_p * 3)(*[u.data_ptr() for u in us])
+    tgh._check(L.tgh_batch_rk4(batch.ptr, uarr, 0.0, dt, 3, 1 if graph else 0))
+    torch.cuda.synchronize()
+    for i, (u, r) in enumerate(zip(us, ref)):
+        d = float((u - r).abs().max())
+        print(f"problem {i} ({probs[i].nblocks} blocks): max |batch - alone| = {d:.3e}")
+        assert torch.isfinite(u).all()
+        assert torch.equal(u, r), f"problem {i} differs by {d}"
+    # again on the same batch (tables and graph reused), and on swapped states
+    tgh._check(L.tgh_batch_rk4(batch.ptr, uarr, 3 * dt, dt, 1, 1 if graph else 0))
+    for p, r in zip(probs, ref):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(r.data_ptr()), 3 * dt, dt, 1, flags))
+    torch.cuda.synchronize()
+    for u, r in zip(us, ref):
+        assert torch.equal(u, r)
+
+
+def test_block_batch_refuses_moving_holes(tgh):
+    torch_cuda()
+    moving = tgh.kerr_schild_case(M=1.0, a=0.3, v=0.2, halfwidth=2.0, r_0=0.71, r_1=1.21, gauge_source="algebraic")
+    fm = tgh.hole_forest(None, moving, N=16, roots=4, radii=(1.2,))
+    pm = tgh.GHProblem(fm, moving, q=4, ops=tgh.Operators(prolongation=6))
+    with pytest.raises(NotImplementedError):
+        tgh.GHBatch([pm])

@@ -1115,6 +1115,13 @@ struct tgh_batch {
   std::vector<tgh_problem*> probs;
   int q = 4, gauge = 0;
   bool hole = false, all_static = true;
+  // The block layout (a problem on a forest with levels or with the general
+  // interior among them): one fused launch over every block of every problem
+  // per stage; the RK4 epilogue writes the states in place, so the tables and
+  // the graph hold the states' pointers.
+  bool blocks = false;
+  int hole_kind = 0;                 // 0 none, 1 the sphere, 2 the general interior
+  std::vector<double*> tab_u, graph_u;
   BatchEntry* d_tab = nullptr;  // 4 stage tables of n entries
   BatchEntry* h_tab = nullptr;  // pinned staging of the tables
   int head_total = 0, pi_total = 0;
@@ -1185,7 +1192,71 @@ int batch_tables(tgh_batch* b, double t, double dt) {
   b->tab_valid = true;
   return TGH_OK;
 }
+// The block layout's stage tables: problem i's stage-s arguments as
+// rk4_step_launches sets them for the problem alone, u_i its state.
+int batch_tables_blocks(tgh_batch* b, double* const* u, double t, double dt) {
+  const int n = (int)b->probs.size();
+  const double c_acc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+  const double c_next[4] = {dt / 2, dt / 2, dt, 0.0};
+  const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
+  int w0 = 0;
+  for (int i = 0; i < n; ++i) {
+    tgh_problem* p = b->probs[i];
+    const int wgpb = (p->geom.P * p->geom.P * p->geom.P + 255) / 256;
+    for (int s = 0; s < 4; ++s) {
+      BatchEntry& e = b->h_tab[s * n + i];
+      std::memset(&e, 0, sizeof(e));
+      e.a = rhs_args(p, p->W[s % 2], t_stage[s], nullptr);
+      e.a.us = u[i];
+      e.a.acc = p->acc;
+      e.a.Wn = p->W[(s + 1) % 2];
+      e.a.du = nullptr;
+      e.a.cacc = c_acc[s];
+      e.a.cnext = c_next[s];
+      e.a.stage = s + 1;
+      e.W = (decltype(e.W))e.a.W;
+      e.Hsrc = (decltype(e.Hsrc))e.a.Hsrc;
+      e.us = (decltype(e.us))e.a.us;
+      e.acc = (decltype(e.acc))e.a.acc;
+      e.Wn = (decltype(e.Wn))e.a.Wn;
+      e.head0 = w0;
+      e.pi0 = 0;
+    }
+    w0 += wgpb * p->geom.nblocks;
+  }
+  b->head_total = w0;
+  b->pi_total = 0;
+  HIP_TRY(hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(BatchEntry) * 4 * n, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->tab_key[0] = dt;
+  b->tab_key[1] = t;
+  b->tab_u.assign(u, u + n);
+  b->tab_valid = true;
+  return TGH_OK;
+}
+void batch_step_launches_blocks(tgh_batch* b, double t, double dt) {
+  const int n = (int)b->probs.size();
+  const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
+  for (int s = 0; s < 4; ++s) {
+    for (tgh_problem* p : b->probs) launch_fill(p, p->W[s % 2], t_stage[s]);
+    const BatchEntry* tab = b->d_tab + s * n;
+    switch (b->q) {
+      case 2: launch_batch_blocks<2>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
+      case 4: launch_batch_blocks<4>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
+      case 6: launch_batch_blocks<6>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
+      default: launch_batch_blocks<8>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
+    }
+    for (int i = 0; i < n; ++i) {
+      tgh_problem* p = b->probs[i];
+      if (p->has_bounds) launch_bounds_work(p, p->W[(s + 1) % 2], s == 3 ? b->tab_u[i] : nullptr);
+    }
+  }
+}
 void batch_step_launches(tgh_batch* b, double t, double dt) {
+  if (b->blocks) {
+    batch_step_launches_blocks(b, t, dt);
+    return;
+  }
   const int n = (int)b->probs.size();
   const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
   for (int s = 0; s < 4; ++s) {
@@ -1224,29 +1295,38 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
                                 "would be stepped twice per stage");
   }
   const tgh_problem* p0 = probs[0];
+  auto hole_kind = [](const tgh_problem* p) { return p->cs.interior == INT_NONE ? 0 : p->cs.general ? 2 : 1; };
+  bool blocks = false;
+  for (int i = 0; i < n; ++i) blocks = blocks || probs[i]->forest || probs[i]->cs.general;
   for (int i = 0; i < n; ++i) {
     const tgh_problem* p = probs[i];
-    if (p->q != p0->q || p->cs.gauge != p0->cs.gauge ||
-        (p->cs.interior != INT_NONE) != (p0->cs.interior != INT_NONE))
-      return fail(TGH_EINVAL, "the problems of a batch share the order, the gauge-source kind and whether they "
-                              "have an interior (one kernel specialisation runs them all); problem " +
+    if (p->q != p0->q || p->cs.gauge != p0->cs.gauge || hole_kind(p) != hole_kind(p0))
+      return fail(TGH_EINVAL, "the problems of a batch share the order, the gauge-source kind and the kind of "
+                              "interior (one kernel specialisation runs them all); problem " +
                                   std::to_string(i) + " differs from problem 0");
     if (p->cs.interior == INT_PASTED)
       return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
-    if (p->forest || p->cs.general)
-      return fail(TGH_EUNSUPPORTED, "problem " + std::to_string(i) + " of the batch is a forest with levels or "
-                  "has a tracked or :fitted interior, which step on the block layout; a batch steps bricks");
+    if (blocks && (p->cs.gauge == GAUGE_KS_MOVING || p->cs.gauge == GAUGE_NONE_MOVING))
+      return fail(TGH_EUNSUPPORTED, "a batch on the block layout (forests with levels, tracked or :fitted "
+                                    "interiors) of moving holes is not built; problem " + std::to_string(i) + " moves");
   }
   tgh_batch* b = new tgh_batch;
   b->probs.assign(probs, probs + n);
   b->q = p0->q;
   b->gauge = p0->cs.gauge;
   b->hole = p0->cs.interior != INT_NONE;
+  b->blocks = blocks;
+  b->hole_kind = hole_kind(p0);
   auto bail = [&](int rc) {
     tgh_batch_destroy(b);
     return rc;
   };
   for (tgh_problem* p : b->probs) {
+    if (blocks) {
+      // static backgrounds and no :fitted target (which moves with t) keep a graph across steps
+      b->all_static = b->all_static && static_background(p->cs) && p->cs.interior != INT_FITTED;
+      continue;
+    }
     int rc = ensure_brick(p);
     if (rc) return bail(rc);
     if (!p->Bcoef) {
@@ -1287,22 +1367,34 @@ int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t 
   int rc = TGH_OK;
   for (int i = 0; i < n; ++i) {
     tgh_problem* p = b->probs[i];
+    if (b->blocks) {
+      if (p->cs.interior == INT_FITTED && !p->cs.tw) {
+        rc = fail(TGH_EINVAL, "problem " + std::to_string(i) + " has a :fitted interior and no target "
+                              "(tgh_set_target)");
+        break;
+      }
+      launch_scatter(p, u[i], p->W[0], 1);  // u_n into the first stage's work array
+      continue;
+    }
     BrickCopyArgs ca{p->geom, p->brick, u[i], p->BW[0], p->Bus, 0};
     hipLaunchKernelGGL(k_brick_in, grid_brick(p), dim3(256), 0, b->stream, ca);
   }
+  const std::vector<double*> uv(u, u + n);
   for (int s = 0; s < nsteps && rc == TGH_OK; ++s) {
     const double t = t0 + s * dt;
     // static backgrounds ignore t: the tables (and the graph) are kept across steps
-    const bool stale = !b->tab_valid || b->tab_key[0] != dt || (!b->all_static && b->tab_key[1] != t);
+    const bool stale = !b->tab_valid || b->tab_key[0] != dt || (!b->all_static && b->tab_key[1] != t) ||
+                       (b->blocks && b->tab_u != uv);
     if (stale) {
-      rc = batch_tables(b, t, dt);
+      rc = b->blocks ? batch_tables_blocks(b, u, t, dt) : batch_tables(b, t, dt);
       if (rc) break;
     }
     if (!graph) {
       batch_step_launches(b, t, dt);
       continue;
     }
-    const bool reuse = b->exec && b->graph_key[0] == dt && (b->all_static || b->graph_key[1] == t);
+    const bool reuse = b->exec && b->graph_key[0] == dt && (b->all_static || b->graph_key[1] == t) &&
+                       (!b->blocks || b->graph_u == uv);
     if (!reuse) {
       batch_drop_graph(b);
       hipError_t ec = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
@@ -1318,6 +1410,7 @@ int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t 
       }
       b->graph_key[0] = dt;
       b->graph_key[1] = t;
+      b->graph_u = uv;
     }
     if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) {
       rc = fail(TGH_EDEVICE, "launching the batch's RK4 graph");
@@ -1325,7 +1418,7 @@ int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t 
     }
   }
   if (rc == TGH_OK) {
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n && !b->blocks; ++i) {  // (the block layout wrote u in place)
       tgh_problem* p = b->probs[i];
       for (int pass = 0; pass < 2; ++pass) {
         BrickCopyArgs ca{p->geom, p->brick, p->Bus, u[i], nullptr, pass};

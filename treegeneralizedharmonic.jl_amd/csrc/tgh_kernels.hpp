@@ -1052,13 +1052,12 @@ __device__ __forceinline__ void load_coef(const double* __restrict__ cf0, long c
   pc.epsh = cf0[CF_EPS * cs];
 }
 
-// The block layout: one owned point of one block per thread.
-__device__ __forceinline__ bool block_point(const RhsArgs& a, const double* __restrict__ W, PointRef& pr,
-                                            long& pidx) {
+// The block layout: one owned point of one block per thread (workgroup wx of
+// block b).
+__device__ __forceinline__ bool block_point_at(const RhsArgs& a, const double* __restrict__ W, int b, int wx,
+                                               PointRef& pr, long& pidx) {
   const Geom& g = a.g;
   const int P = g.P, S = g.S, G = g.G;
-  int b, wx;
-  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return false;
   const int p = wx * blockDim.x + threadIdx.x;
   if (p >= P * P * P) return false;
   const int i = p % P, j = (p / P) % P, k = p / (P * P);
@@ -1082,6 +1081,12 @@ __device__ __forceinline__ bool block_point(const RhsArgs& a, const double* __re
   pr.ms = psz;
   pidx = (long)b * psz + p;
   return true;
+}
+__device__ __forceinline__ bool block_point(const RhsArgs& a, const double* __restrict__ W, PointRef& pr,
+                                            long& pidx) {
+  int b, wx;
+  if (!wg_map((a.g.P * a.g.P * a.g.P + 255) / 256, a.g.nblocks, b, wx)) return false;
+  return block_point_at(a, W, b, wx, pr, pidx);
 }
 
 template <int Q, int GAUGE, int HOLE, int EPI>
@@ -1908,6 +1913,25 @@ __global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRI
   tile_at(e.k, PiTile<Q>::X, PiTile<Q>::Y, PiTile<Q>::Z * PiTile<Q>::PZ, L - e.pi0, X0, Y0, Z0);
   brick_pi_tile<Q>(e.a, e.k, (const double*)e.W, (const double*)e.coef, nullptr, (double*)e.us, (double*)e.acc,
                    (double*)e.Wn, X0, Y0, Z0);
+}
+
+// One RK4 stage of a batch on the block layout: the fused kernel over every
+// block of every problem in the table (forests of any depth and problems with
+// the general interior among them), each entry's blocks a contiguous range of
+// logical workgroups from head0.
+template <int Q, int GAUGE, int HOLE>
+__global__ __launch_bounds__(256) void k_rhs_batch(const BatchEntry* __restrict__ tab, int n, int total) {
+  const int L = wg_logical(total);
+  if (L >= total) return;
+  const BatchEntry& e = tab[batch_find(tab, n, L, false)];
+  const RhsArgs& a = e.a;
+  const int wgpb = (a.g.P * a.g.P * a.g.P + 255) / 256;
+  const int loc = L - e.head0;
+  const int b = loc / wgpb;
+  PointRef pr;
+  long pi;
+  if (!block_point_at(a, (const double*)e.W, b, loc - b * wgpb, pr, pi)) return;
+  rhs_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, pr.ih, nullptr, (double*)e.us, (double*)e.acc, (double*)e.Wn);
 }
 
 #ifndef TGH_RHS_ONLY

@@ -190,9 +190,31 @@ void launch_batch(hipStream_t st, const BatchEntry* tab, int n, int head_total, 
   }
 }
 
+// One RK4 stage of a batch on the block layout (static gauge-source kinds;
+// hole 0 none, 1 the sphere, 2 the general interior).
+template <int Q, int GAUGE>
+void launch_batch_blocks_g(hipStream_t st, const BatchEntry* tab, int n, int total, int hole) {
+  const dim3 grid = xcd_grid(total);
+  if (hole == 2)
+    hipLaunchKernelGGL((k_rhs_batch<Q, GAUGE, 2>), grid, dim3(256), 0, st, tab, n, total);
+  else if (hole == 1)
+    hipLaunchKernelGGL((k_rhs_batch<Q, GAUGE, 1>), grid, dim3(256), 0, st, tab, n, total);
+  else
+    hipLaunchKernelGGL((k_rhs_batch<Q, GAUGE, 0>), grid, dim3(256), 0, st, tab, n, total);
+}
+template <int Q>
+void launch_batch_blocks(hipStream_t st, const BatchEntry* tab, int n, int total, int gauge, int hole) {
+  switch (gauge) {
+    case GAUGE_KS: launch_batch_blocks_g<Q, GAUGE_KS>(st, tab, n, total, hole); break;
+    case GAUGE_SAMPLED: launch_batch_blocks_g<Q, GAUGE_SAMPLED>(st, tab, n, total, hole); break;
+    default: launch_batch_blocks_g<Q, GAUGE_NONE>(st, tab, n, total, hole); break;
+  }
+}
+
 }  // namespace tgh
 
 #define TGH_INSTANTIATE_ORDER(Q)                                                            \
   template void tgh::launch_rhs_blocks<Q>(tgh_problem*, const tgh::RhsArgs&, int);          \
   template void tgh::launch_rhs_brick<Q>(tgh_problem*, const tgh::RhsArgs&);                \
+  template void tgh::launch_batch_blocks<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int); \
   template void tgh::launch_batch<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int, bool);
