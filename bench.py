@@ -16,8 +16,10 @@ Prints one JSON line (rank 0) with the metric, a roofline object for the
 dominant kernel, the other forms of the same step (the plain verbs, and the
 plain per-block form: the fused block kernel launched once per block and
 stage), a batch of four holes of uneven block counts stepped by one launch per
-kernel against the same holes stepped alone and per block, and a CPU baseline
-(the oracle, OpenMP, on a bounded sample).
+kernel against the same holes stepped alone and per block, the three-level
+octree around the hole (hole_forest, finest h = 5M/256, planned fill with an
+order-6 prolongation) alone and batched with the four holes on the block
+layout, and a CPU baseline (the oracle, OpenMP, on a bounded sample).
 """
 import argparse
 import ctypes as C
@@ -237,6 +239,59 @@ def forest_bench(tgh, torch, steps):
             "fill_fraction": fill_ms / (fill_ms + rhs_ms + pi_ms)}
 
 
+def octree_batch_bench(tgh, torch, steps):
+    """The batching workload whole: the three-level octree around the a = 0.7
+    hole (forest_bench's hierarchy) and the four holes of batch_bench in one
+    batch on the block layout (one fused launch over all 7016 blocks and every
+    level per stage, the fills per problem, the step a hipGraph), against the
+    same problems stepped one after another (each its own graph) and by the
+    plain per-block form."""
+    L = tgh._load()
+    case = tgh.kerr_schild_case(M=1.0, a=0.7, halfwidth=5.0, r_0=0.71, r_1=1.21, gauge_source="algebraic")
+    forest = tgh.hole_forest(None, case, N=16, roots=8, radii=(3.0, 1.5))
+    probs = [tgh.GHProblem(forest, case, q=4, ops=tgh.Operators(prolongation=6))]
+    for a, roots in BATCH_HOLES:
+        c = tgh.kerr_schild_case(M=1.0, a=a, halfwidth=roots / 2, r_0=0.71, r_1=1.21, gauge_source="algebraic")
+        probs.append(tgh.GHProblem(tgh.gh_forest(None, c, N=16, roots=roots), c, q=4))
+    us = []
+    for p in probs:
+        u = p.statevector()
+        tgh.initial_state(u, p, 0.0)
+        us.append(u)
+    dt = min(tgh.gh_dt(p, u, cfl=0.25) for p, u in zip(probs, us))
+    blocks = sum(p.nblocks for p in probs)
+    points = blocks * 16 ** 3
+    batch = tgh.GHBatch(probs)
+    uarr = (C.c_void_p * len(us))(*[u.data_ptr() for u in us])
+    tgh._check(L.tgh_batch_rk4(batch.ptr, uarr, 0.0, dt, 1, 1))  # warm-up and capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tgh._check(L.tgh_batch_rk4(batch.ptr, uarr, dt, dt, steps, 1))
+    torch.cuda.synchronize()
+    batched = points * steps / (time.perf_counter() - t0)
+    flags = 4 | 2 | 1  # each alone as the batch runs it: block layout, fused, graph
+    for p, u in zip(probs, us):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, 1, flags))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p, u in zip(probs, us):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, steps, flags))
+    torch.cuda.synchronize()
+    sequential = points * steps / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for p, u in zip(probs, us):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, 1, tgh.TGH_RK4_PER_BLOCK))
+    torch.cuda.synchronize()
+    per_block = points / (time.perf_counter() - t0)
+    assert all(bool(torch.isfinite(u).all()) for u in us), "a batch state is not finite"
+    del batch
+    return {"workload": "the three-level octree around the a = 0.7 hole (2416 leaves, finest h = 5M/256) and "
+                        "four static Kerr-Schild holes a = 0, 0.3, 0.5, 0.7 on 7^3 .. 13^3 blocks, one batch on "
+                        "the block layout",
+            "blocks": blocks, "points": points, "steps": steps, "batched_graph": batched,
+            "sequential_graphs": sequential, "per_block": per_block, "batched_over_per_block": batched / per_block}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -378,6 +433,9 @@ def main():
     if not args.no_batch:
         torch.cuda.empty_cache()
         result["batch"] = batch_bench(tgh, torch, max(1, min(5, args.steps)))
+        if not args.no_forest:
+            torch.cuda.empty_cache()
+            result["octree_batch"] = octree_batch_bench(tgh, torch, max(1, min(3, args.steps)))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
     if args.calibrate_copy:

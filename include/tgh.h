@@ -152,8 +152,8 @@ int tgh_problem_create(const tgh_desc* desc, tgh_problem** out);
  * prolongation is the coarse-to-fine interpolation order p (even, 2 <= p <=
  * q + 2; the interface-order rule of test/interface_tests.jl:64-96 asks for
  * q + 2). Restriction on a vertex-centred mesh is injection, whatever its
- * order, and takes no argument. Such a problem steps on the block layout
- * and cannot join a batch. */
+ * order, and takes no argument. Such a problem steps on the block layout,
+ * and a batch it joins runs on the block layout. */
 int tgh_problem_create_forest(const tgh_desc* desc, const int32_t* leaves, int32_t nleaves, int32_t prolongation,
                               tgh_problem** out);
 /* The level of every block, 1 per block (0 on a uniform forest;
@@ -234,10 +234,14 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
  * of each problem's ranges and arguments). The reference steps each problem
  * by its own gh_solve (src/stepping.jl:208-239); a problem in a batch computes
  * the bits tgh_rk4 computes for it alone in the brick layout's split form.
- * The problems share the order q, the gauge-source kind and whether they have
- * an interior (one kernel specialisation), and none is :pasted; each keeps
- * its own brick arrays (allocated here if missing). The batch holds the
- * problems, which must outlive it. TGH_EINVAL on a mismatch. */
+ * With a forest with levels or a tracked or :fitted interior among them the
+ * batch runs on the block layout instead (one fused launch over every block
+ * of every problem per stage; each problem computes the bits of tgh_rk4 with
+ * TGH_RK4_BLOCKS | TGH_RK4_FUSED), and moving holes are refused there
+ * (TGH_EUNSUPPORTED). The problems share the order q, the gauge-source kind
+ * and the kind of interior (one kernel specialisation), and none is :pasted;
+ * each keeps its own arrays (brick arrays allocated here if missing). The
+ * batch holds the problems, which must outlive it. TGH_EINVAL on a mismatch. */
 int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out);
 /* gh_solve for every problem of the batch over the same (t0, dt, nsteps):
  * u[i] is problem i's state (in place). flags: 0 or TGH_RK4_GRAPH. */

@@ -212,3 +212,49 @@ def test_unrefined_control_on_the_device(tgh):
     c6 = _device_rates(tgh, 6, False)
     assert c4["rate_l2"] >= 3.75
     assert c4["l2"] == c6["l2"]
+
+
+@pytest.mark.parametrize("q,p", [(2, 4), (6, 8), (4, 4)])
+def test_forest_rhs_other_orders_match_oracle(tgh, q, p):
+    """The other finite-difference orders on the two-level mesh, each with
+    its largest (or a lower) prolongation order: the fill reaches p/2 - 1
+    coarse ghosts, the stencils q/2 + 1."""
+    torch = torch_cuda()
+    case = tgh.gauge_wave_case(eps_ko=0.5, gamma0=1.0, gamma2=0.0)
+    f = tgh.gh_forest(None, case, N=8, roots=2, refined=True)
+    dev = tgh.GHProblem(f, case, q=q, ops=tgh.Operators(prolongation=p))
+    orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc), f.leaves, p)
+    t = 0.05
+    u = orc.fill_exact(t)
+    orc.add_noise(u, 0xF0E5A, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    du = torch.full_like(ud, float("nan"))
+    tgh.gh_rhs(du, ud, dev, t)
+    ref = orc.gh_rhs(u, t, NTHREADS)
+    scale = rhs_scale_forest(orc.work(), f, q=q)
+    P = f.N + 1
+    d = np.abs(du.cpu().numpy() - ref).reshape(f.nleaves(), 20, P, P, P) / scale[:, None]
+    print(f"q={q} p={p}: rhs max |GPU - oracle| / rhs_scale = {d.max():.3e}")
+    assert d.max() <= TOL
+
+
+def test_sampled_source_hole_forest_rhs_matches_oracle(tgh):
+    """The sampled gauge source on a forest: H_a and its derivatives taken at
+    every leaf's own points (its level's spacing)."""
+    torch = torch_cuda()
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7)
+    f = tgh.hole_forest(None, case, N=8, roots=4, radii=(1.5,))
+    dev = tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=6))
+    orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc), f.leaves, 6)
+    t = 0.0
+    u = orc.fill_exact(t)
+    orc.add_noise(u, 0xF0E5B, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    du = torch.full_like(ud, float("nan"))
+    tgh.gh_rhs(du, ud, dev, t)
+    ref = orc.gh_rhs(u, t, NTHREADS)
+    scale = rhs_scale_forest(orc.work(), f)
+    P = f.N + 1
+    d = np.abs(du.cpu().numpy() - ref).reshape(f.nleaves(), 20, P, P, P) / scale[:, None]
+    print(f"sampled source on a hole_forest: rhs max |GPU - oracle| / rhs_scale = {d.max():.3e}")
+    assert d.max() <= TOL

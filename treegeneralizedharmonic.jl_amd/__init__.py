@@ -910,8 +910,8 @@ class GHProblem:
                                       1 if rate else 0, float(trail)))
 
     def __del__(self):
-        if getattr(self, "_ptr", None) and self._ptr.value:
-            _load().tgh_problem_destroy(self._ptr)
+        if getattr(self, "_ptr", None) and self._ptr.value and _lib is not None:
+            _lib.tgh_problem_destroy(self._ptr)
             self._ptr = C.c_void_p()
 
     @property
@@ -1075,8 +1075,8 @@ class GHBatch:
         _check(_load().tgh_batch_create(arr, n, C.byref(self._ptr)))
 
     def __del__(self):
-        if getattr(self, "_ptr", None) and self._ptr.value:
-            _load().tgh_batch_destroy(self._ptr)
+        if getattr(self, "_ptr", None) and self._ptr.value and _lib is not None:
+            _lib.tgh_batch_destroy(self._ptr)
             self._ptr = C.c_void_p()
 
     @property
