@@ -63,6 +63,7 @@ def parse():
     ap.add_argument("--no-per-block", action="store_true", help="skip the plain per-block form")
     ap.add_argument("--no-batch", action="store_true", help="skip the four-hole batch")
     ap.add_argument("--no-forest", action="store_true", help="skip the three-level forest")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default Pi tile shape (no autotuner)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a 1 GiB device-to-device copy, the known-size reference for "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -322,6 +323,15 @@ def main():
     points = nb * 16 ** 3
 
     form = FORMS[args.form]
+    # the autotuner: each Pi tile shape timed on a scratch copy of the state,
+    # the fastest kept for the steps below (all shapes give the same bits)
+    tiles_ms = None
+    if args.form == "brick-split" and not args.no_tune:
+        scratch = u.clone()
+        ms = (C.c_double * tgh.TGH_PI_TILES)()
+        tgh._check(L.tgh_autotune(ptr, C.c_void_p(scratch.data_ptr()), 0.0, dt, ms))
+        tiles_ms = list(ms)
+        del scratch
     # warm-up (also captures the graph)
     tgh._check(L.tgh_rk4(ptr, uptr, 0.0, dt, args.warmup, 1 | form))
     torch.cuda.synchronize()
@@ -421,7 +431,9 @@ def main():
                      "head_ms": head_ms, "head_tflops": head_tf, "pi_ms": pi_ms, "pi_tflops": pi_tf,
                      "fill_ms": fill_ms, "flops_per_point_rhs": f_rhs, "flops_per_point_head": f_head,
                      "flops_per_point_update": 4 * f_rhs + f_combine,
-                     "step_roofline_frac": t_roof / (step_ms * 1e-3 / points)},
+                     "step_roofline_frac": t_roof / (step_ms * 1e-3 / points),
+                     "pi_tiles_ms": tiles_ms,
+                     "pi_tile": (min(range(len(tiles_ms)), key=lambda i: tiles_ms[i]) if tiles_ms else 0)},
         "forms": {f"{args.form}_graph": value, "plain_verbs": plain_value, "per_block": per_block_value,
                   "graph_over_per_block": value / per_block_value if per_block_value else None},
     }

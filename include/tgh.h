@@ -229,6 +229,19 @@ int tgh_add_noise(tgh_problem* p, double* u, uint64_t base, double amplitude);
  * arrays (4 x 20 doubles per point of the ghosted brick, and 24 more with
  * TGH_RK4_SPLIT). */
 int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t flags);
+/* No reference counterpart (the reference's kernels take no launch shape):
+ * the autotuner of the brick stepper's tile shapes. Steps u three times per
+ * candidate shape of the Pi kernel's LDS tile (TGH_PI_TILES of them; every
+ * shape computes the same bits, so u is advanced 3 TGH_PI_TILES steps from
+ * t0 as the default form would), times each shape's Pi launches with
+ * hipEvents, writes the average ms per launch into ms (TGH_PI_TILES doubles,
+ * or NULL) and keeps the fastest for the problem's later tgh_rk4 calls.
+ * Pass a scratch copy of the state. TGH_EUNSUPPORTED on the block layout. */
+enum { TGH_PI_TILES = 4 };
+int tgh_autotune(tgh_problem* p, double* u, double t0, double dt, double* ms);
+/* No reference counterpart: pins the Pi kernel's tile shape (0 .. TGH_PI_TILES - 1;
+ * 0 is the default). */
+int tgh_set_tile(tgh_problem* p, int32_t tile);
 /* Several independent problems stepped together (the batched form: holes of
  * uneven block counts in one launch per kernel, driven by a device-side table
  * of each problem's ranges and arguments). The reference steps each problem

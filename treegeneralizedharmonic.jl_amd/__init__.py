@@ -106,6 +106,8 @@ def _load():
                                                 C.POINTER(P)]),
         "tgh_block_levels": (C.c_int, [P, C.POINTER(C.c_int32)]),
         "tgh_set_target": (C.c_int, [P, C.c_void_p, C.c_double, C.c_int32, C.c_double]),
+        "tgh_autotune": (C.c_int, [P, C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double)]),
+        "tgh_set_tile": (C.c_int, [P, C.c_int32]),
         "tgh_problem_destroy": (C.c_int, [P]),
         "tgh_state_length": (C.c_int64, [P]),
         "tgh_work_length": (C.c_int64, [P]),
@@ -1022,6 +1024,19 @@ def error_norms(p, u, t):
     l2, li = C.c_double(), C.c_double()
     _check(_load().tgh_error_norms(p.ptr, _dev(u, p, "u"), float(t), C.byref(l2), C.byref(li)))
     return l2.value, li.value
+
+
+TGH_PI_TILES = 4
+
+
+def autotune(p, u, dt, t=0.0):
+    """The brick stepper's autotuner (no reference counterpart): times each
+    tile shape of the Π kernel on a scratch copy of u and keeps the fastest
+    for p's later steps; returns the average ms per Π launch of each shape."""
+    w = u.clone()
+    ms = (C.c_double * TGH_PI_TILES)()
+    _check(_load().tgh_autotune(p.ptr, _dev(w, p, "u"), float(t), float(dt), ms))
+    return list(ms)
 
 
 def gh_dt(p, u, *, cfl, t=0.0):

@@ -1081,6 +1081,50 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
   return rc;
 }
 
+int tgh_set_tile(tgh_problem* p, int32_t tile) {
+  if (!p) return fail(TGH_EINVAL, "NULL problem");
+  if (tile < 0 || tile >= PI_TILES)
+    return fail(TGH_EINVAL, "the Pi kernel's tile shape is one of 0 .. " + std::to_string(PI_TILES - 1) + ", got " +
+                                std::to_string(tile));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->pi_tile = tile;
+  if (p->graph_exec) {  // a captured graph launches the old shape
+    HIP_TRY(hipGraphExecDestroy(p->graph_exec));
+    p->graph_exec = nullptr;
+  }
+  return TGH_OK;
+}
+
+int tgh_autotune(tgh_problem* p, double* u, double t0, double dt, double* ms) {
+  if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
+  if (p->forest || p->cs.general)
+    return fail(TGH_EUNSUPPORTED, "the tile shapes are the brick stepper's; this problem steps on the block layout");
+  const bool was_profiling = p->profiling;
+  int best = 0;
+  double best_ms = 0.0;
+  for (int tc = 0; tc < PI_TILES; ++tc) {
+    int rc = tgh_set_tile(p, tc);
+    if (rc) return rc;
+    // one step to warm the shape, then two timed (hipEvents around each Pi launch)
+    rc = tgh_rk4(p, u, t0, dt, 1, TGH_RK4_SPLIT);
+    if (rc) return rc;
+    rc = tgh_profile(p, 1);
+    if (rc) return rc;
+    rc = tgh_rk4(p, u, t0 + dt, dt, 2, TGH_RK4_SPLIT);
+    if (rc) return rc;
+    const double pi_ms = p->prof[5] > 0 ? p->prof[4] / p->prof[5] : 0.0;
+    if (ms) ms[tc] = pi_ms;
+    if (tc == 0 || pi_ms < best_ms) {
+      best = tc;
+      best_ms = pi_ms;
+    }
+    t0 += 3 * dt;
+  }
+  int rc = tgh_profile(p, was_profiling ? 1 : 0);
+  if (rc) return rc;
+  return tgh_set_tile(p, best);
+}
+
 int tgh_profile(tgh_problem* p, int32_t enable) {
   if (!p) return fail(TGH_EINVAL, "NULL problem");
   HIP_TRY(hipStreamSynchronize(p->stream));

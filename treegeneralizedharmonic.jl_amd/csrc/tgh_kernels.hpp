@@ -1712,10 +1712,17 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
 // The box's LDS row pitch: 24 doubles (at least the row's BX), so that the
 // eight rows a wave reads fall on distinct banks of the 64 four-byte LDS
 // banks (24 = -8 mod 32: row r starts at bank 48 r mod 64).
-template <int Q> struct PiTile {
-  static constexpr int X = TGH_PI_TILE_X, Y = TGH_PI_TILE_Y;
-  static constexpr int Z = Q == 8 ? 2 : TGH_PI_TILE_Z;
-  static constexpr int PZ = Q == 8 ? 1 : TGH_PI_POINTS_Z;
+// The Pi kernel's tile shapes the autotuner chooses among (tgh_autotune):
+// TC 0 the measured default, TC 1 a taller tile, TC 2 a wider square one,
+// TC 3 a flat one whose box no longer fits one column per thread. Every shape
+// computes each point with the same arithmetic in the same order: only the
+// staging differs, so the results are the same bits.
+constexpr int PI_TILES = 4;
+template <int Q, int TC = 0> struct PiTile {
+  static constexpr int X = TC == 0 ? TGH_PI_TILE_X : TC == 1 ? 8 : TC == 2 ? 10 : 16;
+  static constexpr int Y = TC == 0 ? TGH_PI_TILE_Y : TC == 1 ? 12 : TC == 2 ? 10 : 8;
+  static constexpr int Z = Q == 8 ? 2 : TC == 0 ? TGH_PI_TILE_Z : TC == 3 ? 2 : 4;
+  static constexpr int PZ = Q == 8 || TC != 0 ? 1 : TGH_PI_POINTS_Z;
   static constexpr int pitch(int bx) { return bx <= 24 ? 24 : bx; }
 };
 #ifdef TGH_BRICK_PI_WAVES_PER_EU
@@ -1723,29 +1730,30 @@ template <int Q> struct PiTile {
 #else
 #define TGH_BRICK_PI_ATTR
 #endif
-template <int Q>
+template <int Q, int TC = 0>
 __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
                                               const double* __restrict__ coef, TGH_OUT_PARAMS, int X0, int Y0,
                                               int Z0);
-template <int Q>
-__global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRICK_PI_ATTR void k_brick_pi(
+template <int Q, int TC>
+__global__ __launch_bounds__((PiTile<Q, TC>::X * PiTile<Q, TC>::Y * PiTile<Q, TC>::Z)) TGH_BRICK_PI_ATTR void k_brick_pi(
     RhsArgs a, Brick k, const double* __restrict__ W, const double* __restrict__ coef, TGH_OUT_PARAMS) {
+  using T = PiTile<Q, TC>;
   int X0, Y0, Z0;
-  if (!tile_map(k, PiTile<Q>::X, PiTile<Q>::Y, PiTile<Q>::Z * PiTile<Q>::PZ, X0, Y0, Z0)) return;  // uniform
-  brick_pi_tile<Q>(a, k, W, coef, TGH_OUT_ARGS, X0, Y0, Z0);
+  if (!tile_map(k, T::X, T::Y, T::Z * T::PZ, X0, Y0, Z0)) return;  // uniform
+  brick_pi_tile<Q, TC>(a, k, W, coef, TGH_OUT_ARGS, X0, Y0, Z0);
 }
 // One tile of the Pi kernel, corner (X0, Y0, Z0), one workgroup.
-template <int Q>
+template <int Q, int TC>
 __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
                                               const double* __restrict__ coef, TGH_OUT_PARAMS, int X0, int Y0,
                                               int Z0) {
-  constexpr int TX = PiTile<Q>::X, TY = PiTile<Q>::Y, TZ = PiTile<Q>::Z, PZ = PiTile<Q>::PZ;
+  constexpr int TX = PiTile<Q, TC>::X, TY = PiTile<Q, TC>::Y, TZ = PiTile<Q, TC>::Z, PZ = PiTile<Q, TC>::PZ;
   constexpr int NT = TX * TY * TZ, DZ = TZ * PZ;  // threads; the tile's depth in points
   constexpr int H = ::tgh::W<Q>::R + 1;
   // the box: BX x BY x BZ points, stored with row pitch RX >= BX (a pitch
   // that spreads a wave's rows over the LDS banks) and plane pitch RX * BY
   constexpr int BX = TX + 2 * H, BY = TY + 2 * H, BZ = DZ + 2 * H, BV = BX * BY * BZ;
-  constexpr int RX = PiTile<Q>::pitch(BX), RV = RX * BY * BZ;
+  constexpr int RX = PiTile<Q, TC>::pitch(BX), RV = RX * BY * BZ;
   __shared__ double sh[2][RV];
   const int t = threadIdx.x;
   const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);

@@ -7,6 +7,14 @@
 namespace tgh {
 
 
+// The brick Pi kernel with tile shape TC.
+template <int Q, int TC>
+void launch_brick_pi(tgh_problem* p, const RhsArgs& a) {
+  using T = PiTile<Q, TC>;
+  hipLaunchKernelGGL((k_brick_pi<Q, TC>), grid_brick_tiles(p, T::X, T::Y, T::Z * T::PZ), dim3(T::X * T::Y * T::Z),
+                     0, p->stream, a, p->brick, a.W, (const double*)p->Bcoef, a.du, a.us, a.acc, a.Wn);
+}
+
 // One right-hand side: the fused kernel, or the head and the Pi components as
 // two launches. While profiling, the head is timed as the "rhs" pair and the
 // Pi kernel as its own pair.
@@ -118,9 +126,12 @@ void launch_brick_rhs_q(tgh_problem* p, const RhsArgs& a) {
                          a.W, p->Bcoef, a.du, a.us, a.acc, a.Wn);
     }
     int e1 = prof ? mark(p) : -1;
-    using T = PiTile<Q>;
-    hipLaunchKernelGGL((k_brick_pi<Q>), grid_brick_tiles(p, T::X, T::Y, T::Z * T::PZ), dim3(T::X * T::Y * T::Z), 0,
-                       p->stream, a, p->brick, a.W, (const double*)p->Bcoef, a.du, a.us, a.acc, a.Wn);
+    switch (p->pi_tile) {
+      case 1: launch_brick_pi<Q, 1>(p, a); break;
+      case 2: launch_brick_pi<Q, 2>(p, a); break;
+      case 3: launch_brick_pi<Q, 3>(p, a); break;
+      default: launch_brick_pi<Q, 0>(p, a); break;
+    }
     if (prof) {
       p->ev_rhs.push_back({e0, e1});
       p->ev_pi.push_back({e1, mark(p)});

@@ -49,6 +49,7 @@ struct tgh_problem {
   bool profiling = false;
   bool fused = false;                 // one kernel per RHS instead of head + Pi
   bool per_block = false;             // the fused block kernel launched once per block
+  int pi_tile = 0;                    // the brick Pi kernel's tile shape (tgh_autotune, tgh_set_tile)
   int graph_form = -1;                // the form the captured graph launches
   // The stepper's brick layout (allocated at the first tgh_rk4 that uses it):
   // two stage arrays, u_n and the accumulator, and the sampled gauge source.
