@@ -644,6 +644,9 @@ struct Problem {
   // rhs! (evolution.jl:1386-1390): every owned point of every block.
   void rhs(double* du, double t, int nthreads = 1) const {
     const Mesh& m = mesh;
+    // (checked here, outside the threads: a :fitted interior reads its cache)
+    if (cs.interior == INT_FITTED && tw.empty())
+      throw std::runtime_error("a :fitted interior needs its target cache (orc_set_target)");
     long nb = m.nblocks();
 #pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
     for (long b = 0; b < nb; ++b)

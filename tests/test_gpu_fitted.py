@@ -110,3 +110,37 @@ def test_fitted_without_target_is_refused(tgh):
     du = torch.empty_like(u)
     rc = tgh._load().tgh_gh_rhs(p.ptr, C.c_void_p(du.data_ptr()), C.c_void_p(u.data_ptr()), 0.0, None)
     assert rc == 1 and b"target" in tgh._load().tgh_last_error()
+
+
+def test_batch_of_fitted_problems_follows_new_targets(tgh):
+    """Two :fitted holes in one batch (the block layout) step as each alone,
+    and a target refilled between calls is what the next batch call reads."""
+    torch = torch_cuda()
+    rng = np.random.default_rng(13)
+    L = tgh._load()
+    d = _desc(TD.TGH_INTERIOR_FITTED)
+    o = oracle_lib.OracleProblem(d)
+    u0 = o.fill_exact(0.0)
+    probs, tws, us = [], [], []
+    for i in range(2):
+        p = tgh_problem_from_desc(tgh, d)
+        tw = torch.from_numpy(_target(o, u0, rng, 0.05)).cuda()
+        tgh._check(L.tgh_set_target(p.ptr, C.c_void_p(tw.data_ptr()), 0.0, 1, 0.0))
+        probs.append(p)
+        tws.append(tw)
+        us.append(torch.from_numpy(u0.copy()).cuda())
+    refs = [u.clone() for u in us]
+    dt = 0.25 * 0.0625 / o.max_speed(u0)
+    batch = tgh.GHBatch(probs)
+    uarr = (C.c_void_p * 2)(*[u.data_ptr() for u in us])
+    for rnd in range(2):
+        if rnd == 1:  # new targets for both
+            for p, tw in zip(probs, tws):
+                tw.mul_(0.5)
+                tgh._check(L.tgh_set_target(p.ptr, C.c_void_p(tw.data_ptr()), 0.0, 1, 0.0))
+        for p, r in zip(probs, refs):
+            tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(r.data_ptr()), rnd * dt, dt, 1, 4 | 2 | 1))
+        tgh._check(L.tgh_batch_rk4(batch.ptr, uarr, rnd * dt, dt, 1, 1))
+        torch.cuda.synchronize()
+        for u, r in zip(us, refs):
+            assert torch.equal(u, r)

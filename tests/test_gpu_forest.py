@@ -258,3 +258,32 @@ def test_sampled_source_hole_forest_rhs_matches_oracle(tgh):
     d = np.abs(du.cpu().numpy() - ref).reshape(f.nleaves(), 20, P, P, P) / scale[:, None]
     print(f"sampled source on a hole_forest: rhs max |GPU - oracle| / rhs_scale = {d.max():.3e}")
     assert d.max() <= TOL
+
+
+def test_three_level_forest_fill_and_rhs_match_oracle(tgh):
+    """Three levels in a Dirichlet box: level-2 ghosts prolongated from
+    level-1 blocks whose own ghosts were prolongated from level 0 in the pass
+    before, next to the hook's outer ghosts."""
+    torch = torch_cuda()
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    f = tgh.hole_forest(None, case, N=8, roots=4, radii=(2.5, 1.0))
+    assert f.maxlevel() == 2
+    dev = tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=6))
+    orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc), f.leaves, 6)
+    t = 0.02
+    u = orc.fill_exact(t)
+    orc.add_noise(u, 0xF0E5C, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    tgh.prepare(dev, ud, t)
+    orc.prepare(u, t)
+    wd, wo = dev.work().cpu().numpy(), orc.work()
+    fd = float((np.abs(wd - wo) / np.maximum(np.abs(wo), 1.0)).max())
+    du = torch.full_like(ud, float("nan"))
+    tgh.gh_rhs(du, ud, dev, t)
+    ref = orc.gh_rhs(u, t, NTHREADS)
+    scale = rhs_scale_forest(orc.work(), f)
+    P = f.N + 1
+    d = np.abs(du.cpu().numpy() - ref).reshape(f.nleaves(), 20, P, P, P) / scale[:, None]
+    print(f"three levels, {f.nleaves()} leaves: fill max rel diff {fd:.3e}, rhs max / rhs_scale {d.max():.3e}")
+    assert fd <= 1e-14
+    assert d.max() <= TOL

@@ -560,7 +560,7 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
     for (int32_t n = 0; n < nleaves; ++n) {
       lev.push_back(leaves[4 * n]);
       for (int c = 0; c < 3; ++c) fbc.push_back(leaves[4 * n + 1 + c]);
-      lmax = std::max(lmax, std::min(std::max(leaves[4 * n], 0), 12));
+      lmax = std::max(lmax, std::min(std::max(leaves[4 * n], 0), 10));
     }
   }
   int rc = check_desc(d, lmax);
@@ -1424,6 +1424,9 @@ int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t 
     hipLaunchKernelGGL(k_brick_in, grid_brick(p), dim3(256), 0, b->stream, ca);
   }
   const std::vector<double*> uv(u, u + n);
+  // on the block layout the tables carry each problem's case, whose :fitted
+  // target (tgh_set_target) may have changed since the last call
+  if (b->blocks) b->tab_valid = false;
   for (int s = 0; s < nsteps && rc == TGH_OK; ++s) {
     const double t = t0 + s * dt;
     // static backgrounds ignore t: the tables (and the graph) are kept across steps

@@ -49,8 +49,8 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
            " so that its stencil stays inside the coarse block's ghosts, got " + std::to_string(np);
   int Lmax = 0;
   for (int b = 0; b < nl; ++b) {
-    if (lev[b] < 0 || lev[b] > 12) return "leaf " + std::to_string(b) + " has level " + std::to_string(lev[b]) +
-                                          ", outside 0 .. 12";
+    if (lev[b] < 0 || lev[b] > 10) return "leaf " + std::to_string(b) + " has level " + std::to_string(lev[b]) +
+                                          ", outside 0 .. 10";
     Lmax = lev[b] > Lmax ? lev[b] : Lmax;
     for (int d = 0; d < 3; ++d)
       if (bc[3 * b + d] < 0 || bc[3 * b + d] >= (nb[d] << lev[b]))

@@ -1447,7 +1447,7 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
           // a forest: primaries only, each weighted by its volume relative
           // to level 0's, 2^(-3 level)
           if (g.prim[(long)b * psz + p]) {
-            const double wv = 1.0 / (double)(1 << (3 * block_level(g, b)));
+            const double wv = ldexp(1.0, -3 * block_level(g, b));
             r0 = e2 * wv;
             r1 = wv;
           } else {
