@@ -229,6 +229,13 @@ int tgh_add_noise(tgh_problem* p, double* u, uint64_t base, double amplitude);
  * arrays (4 x 20 doubles per point of the ghosted brick, and 24 more with
  * TGH_RK4_SPLIT). */
 int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t flags);
+/* gh_constraint!(p, u, t) then constraint_norms(p)'s gauge rows
+ * (src/constraints.jl:211-309,559-566,848-876): prepares u at t and evaluates
+ * the gauge constraint C_a = g_ab (Gamma^b + g^{bc} H_c) at every owned
+ * point, d_t h without the dissipation, H the case's source; l2 and linf get
+ * the masked norms of the four components over the evolved points
+ * (4 doubles each, host memory), counted as tgh_error_norms counts them. */
+int tgh_gauge_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf);
 /* No reference counterpart (the reference's kernels take no launch shape):
  * the autotuner of the brick stepper's tile shapes. Steps u three times per
  * candidate shape of the Pi kernel's LDS tile (TGH_PI_TILES of them; every

@@ -902,6 +902,83 @@ struct Problem {
           }
   }
 
+  // gh_constraint! + constraint_norms' gauge rows (constraints.jl:211-309,
+  // 559-566, 848-876): C_a = g_ab (Gamma^b + g^{bc} H_c) at every evolved owned
+  // point, Gamma^b = g^{cd} Gamma^b_cd (gauge_constraint_at_node,
+  // pointwise.jl:947-960), d_t h from the state without dissipation, H the
+  // case's source; masked L2 and L-inf per component over the points
+  // error_norms counts.
+  void gauge_constraint(const double* u, double t, double l2[4], double linf[4]) {
+    prepare(u, t);
+    const Mesh& m = mesh;
+    double num[4] = {0, 0, 0, 0}, den = 0;
+    for (int a = 0; a < 4; ++a) linf[a] = 0;
+    const long sv = m.S * m.S * m.S;
+    const long st[3] = {1, m.S, m.S * m.S};
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3];
+            m.position(b, i, j, k, x);
+            if (!is_evolved(t, x)) continue;
+            const bool prim = m.is_primary(b, i, j, k);
+            if (m.forest() && !prim) continue;
+            const long var = m.widx(b, 0, i, j, k);
+            const double inv_h = 1.0 / m.hb(b);
+            V10<double> hv, Pv, dh[3], dth;
+            for (int v = 0; v < NC; ++v) {
+              hv[v] = work[var + v * sv];
+              Pv[v] = work[var + (NC + v) * sv];
+            }
+            for (int d = 0; d < 3; ++d)
+              for (int v = 0; v < NC; ++v) dh[d][v] = inv_h * axis_stencil<double>(w1, var + v * sv, st[d]);
+            MQ<double> mq = metric_quantities(hv);
+            for (int n = 0; n < NC; ++n)
+              dth[n] = mq.beta[0] * dh[0][n] + mq.beta[1] * dh[1][n] + mq.beta[2] * dh[2][n] +
+                       (mq.alpha / mq.sqrtg) * Pv[n];
+            double Hl[4] = {0, 0, 0, 0}, dHl[4][4] = {{0}};
+            if (cs.gauge == GAUGE_SAMPLED)
+              for (int a = 0; a < 4; ++a) Hl[a] = Hsrc[m.sidx(b, a, i, j, k)];
+            else if (cs.gauge == GAUGE_KS)
+              ks_algebraic_source<double>(cs.bg.M, cs.ks_S, cs.ks_u, hv, dth, dh, Hl, dHl);
+            double dg[4][4][4];  // dg[c][a][b] = d_c g_ab
+            dg4(dth, dh, dg);
+            // Gamma_{c a b} = (d_b g_ca + d_a g_cb - d_c g_ab) / 2, then
+            // Gamma^m_ab = g^{mc} Gamma_cab and Gamma^m = g^{ab} Gamma^m_ab
+            double Gl[4][4][4], Gm[4][4][4], Gup[4], Hup[4], Cu[4];
+            for (int c = 0; c < 4; ++c)
+              for (int a = 0; a < 4; ++a)
+                for (int bb = 0; bb < 4; ++bb) Gl[c][a][bb] = (dg[bb][c][a] + dg[a][c][bb] - dg[c][a][bb]) / 2;
+            for (int mu = 0; mu < 4; ++mu)
+              for (int a = 0; a < 4; ++a)
+                for (int bb = 0; bb < 4; ++bb) {
+                  double s = 0;
+                  for (int c = 0; c < 4; ++c) s += mq.gu4[mu][c] * Gl[c][a][bb];
+                  Gm[mu][a][bb] = s;
+                }
+            for (int mu = 0; mu < 4; ++mu) {
+              double s = 0;
+              for (int a = 0; a < 4; ++a)
+                for (int bb = 0; bb < 4; ++bb) s += mq.gu4[a][bb] * Gm[mu][a][bb];
+              Gup[mu] = s;
+              double hu = 0;
+              for (int c = 0; c < 4; ++c) hu += mq.gu4[mu][c] * Hl[c];
+              Hup[mu] = hu;
+              Cu[mu] = Gup[mu] + Hup[mu];
+            }
+            const double wv = m.forest() ? 1.0 / (double)(1L << (3 * m.level[b])) : 1.0;
+            for (int a = 0; a < 4; ++a) {
+              double C = 0;
+              for (int mu = 0; mu < 4; ++mu) C += mq.g4[a][mu] * Cu[mu];
+              linf[a] = std::max(linf[a], std::abs(C));
+              if (prim) num[a] += m.forest() ? C * C * wv : C * C;
+            }
+            if (prim) den += wv;
+          }
+    for (int a = 0; a < 4; ++a) l2[a] = den > 0 ? std::sqrt(num[a] / den) : 0.0;
+  }
+
   // One classical RK4 step with the limiter on each stage value and on the
   // result (stepping.jl:53-73, IMEXRungeKutta RK4).
   void rk4_step(double* u, double t, double dt, int nthreads = 1) {

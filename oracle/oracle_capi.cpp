@@ -417,6 +417,9 @@ double orc_max_speed(void* p, const double* u, double t) { return ((Problem*)p)-
 void orc_error_norms(void* p, const double* u, double t, double* l2, double* linf) {
   ((Problem*)p)->error_norms(u, t, *l2, *linf);
 }
+void orc_gauge_constraint(void* p, const double* u, double t, double* l2, double* linf) {
+  ((Problem*)p)->gauge_constraint(u, t, l2, linf);
+}
 int orc_is_primary(void* p, long b, long i, long j, long k) {
   return ((Problem*)p)->mesh.is_primary(b, i, j, k) ? 1 : 0;
 }

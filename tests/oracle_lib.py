@@ -56,6 +56,7 @@ def lib():
             "orc_problem_create_forest": (P, [desc, C.POINTER(C.c_int), C.c_int, C.c_int]),
             "orc_block_level": (C.c_int, [P, Lg]),
             "orc_set_target": (None, [P, dp, D, C.c_int, D]),
+            "orc_gauge_constraint": (None, [P, dp, D, dp, dp]),
             "orc_shape_series": (D, [dp, C.c_int, dp]),
             "orc_interior_geo": (None, [desc, D, dp, dp]),
             "orc_forest_classify": (C.c_int, [P, Lg, Lg, Lg, Lg, C.POINTER(Lg)]),
@@ -137,6 +138,12 @@ class OracleProblem:
         tw = np.ascontiguousarray(tw, dtype=np.float64)
         assert tw.size == 2 * self.n
         lib().orc_set_target(self.p, tw, float(t_f), 1 if rate else 0, float(trail))
+
+    def gauge_constraint(self, u, t):
+        """(L2[4], Linf[4]) of the gauge constraint C_a over the evolved points."""
+        l2, li = np.zeros(4), np.zeros(4)
+        lib().orc_gauge_constraint(self.p, np.ascontiguousarray(u), float(t), l2, li)
+        return l2, li
 
     def block_level(self, b):
         return lib().orc_block_level(self.p, b)

@@ -1,0 +1,83 @@
+"""The gauge constraint monitor on the device against the oracle, through the
+C ABI (uniform and refined meshes, closed-form and sampled sources, a
+harmonic background), and its masked norms' convergence on the frozen
+hierarchy."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib
+import tgh_desc as TD
+
+pytestmark = pytest.mark.gpu
+
+
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a visible GPU")
+    return torch
+
+
+def _compare(tgh, dev, orc, u, t):
+    torch = torch_cuda()
+    ud = torch.from_numpy(u.copy()).cuda()
+    l2g, lig = tgh.gauge_constraint_norms(dev, ud, t)
+    l2o, lio = orc.gauge_constraint(u, t)
+    print(f"gauge constraint: GPU l2 {l2g} linf {lig}; oracle l2 {list(l2o)} linf {list(lio)}")
+    np.testing.assert_allclose(l2g, l2o, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(lig, lio, rtol=1e-9, atol=1e-15)
+
+
+@pytest.mark.parametrize("gauge_source", [None, "algebraic"])
+def test_gauge_constraint_matches_oracle_on_a_uniform_hole(tgh, gauge_source):
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=2.0, r_0=0.71, r_1=1.21, gauge_source=gauge_source)
+    dev = tgh.GHProblem(tgh.gh_forest(None, case, N=16, roots=4), case, q=4)
+    orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc))
+    u = orc.fill_exact(0.0)
+    orc.add_noise(u, 0xC0A5, 1e-6)
+    _compare(tgh, dev, orc, u, 0.0)
+
+
+def test_gauge_constraint_matches_oracle_on_a_forest(tgh):
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    f = tgh.hole_forest(None, case, N=8, roots=4, radii=(2.5, 1.0))
+    dev = tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=6))
+    orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc), f.leaves, 6)
+    u = orc.fill_exact(0.0)
+    orc.add_noise(u, 0xC0A6, 1e-6)
+    _compare(tgh, dev, orc, u, 0.0)
+
+
+def test_gauge_constraint_matches_oracle_on_the_gauge_wave_forest(tgh):
+    case = tgh.gauge_wave_case(eps_ko=0.0, gamma0=1.0, gamma2=0.0)
+    f = tgh.gh_forest(None, case, N=8, roots=2, refined=True)
+    dev = tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=6))
+    orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc), f.leaves, 6)
+    u = orc.fill_exact(0.1)
+    orc.add_noise(u, 0xC0A7, 1e-6)
+    _compare(tgh, dev, orc, u, 0.1)
+
+
+def test_gauge_constraint_converges_on_the_frozen_hierarchy(tgh):
+    """test/driver_tests.jl:209-245's rate_gauge (the record's gauge_l2, the
+    largest component's L2, src/driver.jl:1545) on the static-hole fixture
+    at q = 2: evolved to t = 3/20, falling at order q."""
+    torch = torch_cuda()
+    case = tgh.kerr_schild_case(M=1.0, a=0.0, halfwidth=2.5, r_0=0.4, r_1=1.15)
+    hs, gl = [], []
+    for N in (6, 8, 10):
+        f = tgh.hole_forest(None, case, N=N, roots=1, radii=(3.0, 3.0, 1.0))
+        p = tgh.GHProblem(f, case, q=2, ops=tgh.Operators(prolongation=4))
+        u = p.statevector()
+        tgh.initial_state(u, p, 0.0)
+        dt = tgh.gh_dt(p, u, cfl=0.25)
+        tgh.gh_solve(p, u, (0.0, 3 / 20), dt=dt, graph=True)
+        l2, _ = tgh.gauge_constraint_norms(p, u, 3 / 20)
+        hs.append(f.minimum_spacing())
+        gl.append(max(l2))
+    x, y = np.log(hs), np.log(gl)
+    rate = float(((x - x.mean()) * (y - y.mean())).sum() / ((x - x.mean()) ** 2).sum())
+    print(f"gauge constraint on the frozen hierarchy, q=2: {gl}, rate {rate:.3f}")
+    assert rate > 2 - 0.25

@@ -108,6 +108,8 @@ def _load():
         "tgh_set_target": (C.c_int, [P, C.c_void_p, C.c_double, C.c_int32, C.c_double]),
         "tgh_autotune": (C.c_int, [P, C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double)]),
         "tgh_set_tile": (C.c_int, [P, C.c_int32]),
+        "tgh_gauge_constraint": (C.c_int, [P, C.c_void_p, C.c_double, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double)]),
         "tgh_problem_destroy": (C.c_int, [P]),
         "tgh_state_length": (C.c_int64, [P]),
         "tgh_work_length": (C.c_int64, [P]),
@@ -1037,6 +1039,15 @@ def autotune(p, u, dt, t=0.0):
     ms = (C.c_double * TGH_PI_TILES)()
     _check(_load().tgh_autotune(p.ptr, _dev(w, p, "u"), float(t), float(dt), ms))
     return list(ms)
+
+
+def gauge_constraint_norms(p, u, t):
+    """gh_constraint!(p, u, t) + constraint_norms(p)'s gauge rows
+    (src/constraints.jl:211-309,559-566,848-876): the masked (L2, L∞) of the
+    four components of C_a = g_ab (Γ^b + g^{bc} H_c)."""
+    l2, li = (C.c_double * 4)(), (C.c_double * 4)()
+    _check(_load().tgh_gauge_constraint(p.ptr, _dev(u, p, "u"), float(t), l2, li))
+    return list(l2), list(li)
 
 
 def gh_dt(p, u, *, cfl, t=0.0):

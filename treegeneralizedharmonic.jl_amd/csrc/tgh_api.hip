@@ -792,6 +792,7 @@ int tgh_problem_destroy(tgh_problem* p) {
   (void)hipFree(p->d_lookup);
   (void)hipFree(p->d_level);
   (void)hipFree(p->d_shape);
+  (void)hipFree(p->gc);
   (void)hipFree(p->d_prim);
   (void)hipFree(p->d_plan);
   (void)hipFree(p->W[0]);
@@ -957,6 +958,39 @@ int tgh_error_norms(tgh_problem* p, const double* u, double t, double* l2, doubl
   }
   *l2 = den > 0 ? std::sqrt(num / den) : 0.0;
   *linf = li;
+  return TGH_OK;
+}
+
+int tgh_gauge_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf) {
+  if (!p || !u || !l2 || !linf) return fail(TGH_EINVAL, "NULL argument");
+  const Geom& g = p->geom;
+  const long nwg = (long)g.nblocks * ((g.P * g.P * g.P + 255) / 256);
+  if (!p->gc) HIP_TRY(hipMalloc(&p->gc, sizeof(double) * 9 * nwg));
+  launch_scatter(p, u, p->W[0], 1);
+  launch_fill(p, p->W[0], t);
+  GCArgs a{g, p->cs, t, p->W[0], p->Hsrc, p->gc};
+  switch (p->q) {
+    case 2: hipLaunchKernelGGL(k_gauge_constraint<2>, grid_points(p), dim3(256), 0, p->stream, a); break;
+    case 4: hipLaunchKernelGGL(k_gauge_constraint<4>, grid_points(p), dim3(256), 0, p->stream, a); break;
+    case 6: hipLaunchKernelGGL(k_gauge_constraint<6>, grid_points(p), dim3(256), 0, p->stream, a); break;
+    default: hipLaunchKernelGGL(k_gauge_constraint<8>, grid_points(p), dim3(256), 0, p->stream, a); break;
+  }
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h(9 * nwg);
+  HIP_TRY(hipMemcpyAsync(h.data(), p->gc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  double num[4] = {0, 0, 0, 0}, den = 0, mx[4] = {0, 0, 0, 0};
+  for (long w = 0; w < nwg; ++w) {
+    for (int q = 0; q < 4; ++q) {
+      num[q] += h[9 * w + q];
+      mx[q] = std::max(mx[q], h[9 * w + 4 + q]);
+    }
+    den += h[9 * w + 8];
+  }
+  for (int q = 0; q < 4; ++q) {
+    l2[q] = den > 0 ? std::sqrt(num[q] / den) : 0.0;
+    linf[q] = mx[q];
+  }
   return TGH_OK;
 }
 

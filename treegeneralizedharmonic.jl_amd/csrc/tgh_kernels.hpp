@@ -1486,6 +1486,110 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
   }
 }
 
+// The gauge constraint monitor (gh_constraint_kernel!, src/constraints.jl:
+// 211-309; gauge_constraint_at_node, src/pointwise.jl:947-960) at every owned
+// point of a prepared work array: C_a = g_ab (Gamma^b + g^{bc} H_c), Gamma^b =
+// g^{cd} g^{be} Gamma_{ecd}, with d_t h = beta^i d_i h + (alpha / sqrt gamma) Pi
+// (no dissipation) and the case's source; each workgroup folds its points'
+// sums of w C_a^2 (w the volume weight on a forest, 1 on a uniform one), the
+// maxima of |C_a| and the weights into 9 partials, over the points the
+// error norms count.
+struct GCArgs {
+  Geom g;
+  CaseP c;
+  double t;
+  const double* __restrict__ W;
+  const double* __restrict__ Hsrc;
+  double* __restrict__ out;
+};
+template <int Q>
+__global__ __launch_bounds__(256) void k_gauge_constraint(GCArgs a) {
+  const Geom& g = a.g;
+  const int P = g.P, S = g.S, G = g.G;
+  int b, wx;
+  if (!wg_map((P * P * P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
+  const long psz = (long)P * P * P, sv = (long)S * S * S;
+  double r[9];
+#pragma unroll
+  for (int n = 0; n < 9; ++n) r[n] = 0.0;
+  if (p < P * P * P) {
+    const int i = p % P, j = (p / P) % P, k = p / (P * P);
+    double x[3];
+    position(g, b, i, j, k, x);
+    const CaseP& c = a.c;
+    if (is_evolved(c, a.t, x)) {
+      int l[3] = {i, j, k}, owner, loc[3];
+      const bool prim = g.prim ? g.prim[(long)b * psz + p] != 0 : classify(g, b, l, owner, loc) == -1;
+      if (!(g.prim && !prim)) {
+        const double* w0 = a.W + (long)b * 20 * sv + (i + G) + (long)S * ((j + G) + (long)S * (k + G));
+        const double ih = block_inv_h(g, b);
+        double hv[NC], D[4][NC];
+        for (int v = 0; v < NC; ++v) {
+          const double* wv = w0 + v * sv;
+          hv[v] = wv[0];
+          D[1][v] = ih * st_d1<Q>(wv, 1);
+          D[2][v] = ih * st_d1<Q>(wv, S);
+          D[3][v] = ih * st_d1<Q>(wv, S * S);
+        }
+        Coef cf;
+        metric_quantities(hv, cf);
+        const double ad = cf.alpha / cf.sqrtg;
+        for (int v = 0; v < NC; ++v)
+          D[0][v] = cf.beta[0] * D[1][v] + cf.beta[1] * D[2][v] + cf.beta[2] * D[3][v] + ad * w0[(NC + v) * sv];
+        double Hl[4] = {0.0, 0.0, 0.0, 0.0}, dH[4][4];
+        if (c.gauge == GAUGE_SAMPLED) {
+          for (int q = 0; q < 4; ++q) Hl[q] = a.Hsrc[(long)b * 20 * psz + p + q * psz];
+        } else if (c.gauge == GAUGE_KS) {
+          ks_source(c.bg.M, c.bg.a, hv, D, Hl, dH);
+        } else if (c.gauge == GAUGE_KS_MOVING) {
+          ks_source_moving(c.bg.M, c.ks_S, c.ks_u, hv, D, Hl, dH);
+        }
+        // T_e = g^{cd} Gamma_{ecd} = g^{cd} (d_d g_ec - d_e g_cd / 2)
+        double Te[4];
+        for (int e = 0; e < 4; ++e) {
+          double s = 0.0;
+          for (int cc = 0; cc < 4; ++cc)
+            for (int d = 0; d < 4; ++d)
+              s += cf.gu[pidx(cc, d)] * (0.5 * (D[d][pidx(e, cc)] + D[cc][pidx(e, d)]) - 0.5 * D[e][pidx(cc, d)]);
+          Te[e] = s;
+        }
+        double Cu[4];
+        for (int m = 0; m < 4; ++m) {
+          double gam = 0.0, hu = 0.0;
+          for (int e = 0; e < 4; ++e) {
+            gam += cf.gu[pidx(m, e)] * Te[e];
+            hu += cf.gu[pidx(m, e)] * Hl[e];
+          }
+          Cu[m] = gam + hu;
+        }
+        const double wgt = g.blevel ? ldexp(1.0, -3 * block_level(g, b)) : 1.0;
+        for (int q = 0; q < 4; ++q) {
+          double C = 0.0;
+          for (int m = 0; m < 4; ++m) C += ((q == m ? (q == 0 ? -1.0 : 1.0) : 0.0) + hv[pidx(q, m)]) * Cu[m];
+          r[4 + q] = fabs(C);
+          if (prim) r[q] = C * C * wgt;
+        }
+        if (prim) r[8] = wgt;
+      }
+    }
+  }
+  __shared__ double sh[9][256];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int n = 0; n < 9; ++n) sh[n][tid] = r[n];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+#pragma unroll
+      for (int n = 0; n < 9; ++n)
+        sh[n][tid] = (n >= 4 && n < 8) ? fmax(sh[n][tid], sh[n][tid + w]) : sh[n][tid] + sh[n][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid < 9) a.out[9 * ((long)b * ((P * P * P + 255) / 256) + wx) + tid] = sh[tid][0];
+}
+
 #endif  // TGH_RHS_ONLY
 
 // ---------------------------------------------------------------------------

@@ -35,6 +35,7 @@ struct tgh_problem {
   int plan_dir[2] = {0, 0};           // pass 0's Dirichlet entries
   int prolong = 0;
   double* d_shape = nullptr;          // the tracked surface's coefficients
+  double* gc = nullptr;               // the gauge constraint's partials (9 per workgroup)
   double* W[2] = {nullptr, nullptr};  // work arrays (prepare uses W[0])
   double* acc = nullptr;              // RK4 accumulator, state layout
   double* coef = nullptr;             // the split form's per-point coefficients (CF_N x points)
