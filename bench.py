@@ -437,6 +437,17 @@ def main():
         "forms": {f"{args.form}_graph": value, "plain_verbs": plain_value, "per_block": per_block_value,
                   "graph_over_per_block": value / per_block_value if per_block_value else None},
     }
+    # the two constraint monitors on the same state (each call: scatter, fill,
+    # the monitor kernel and the fold), after one warm call
+    monitors = {}
+    l2m, lim = (C.c_double * 4)(), (C.c_double * 4)()
+    for name, fn in (("gauge_constraint_ms", L.tgh_gauge_constraint), ("adm_constraint_ms", L.tgh_adm_constraint)):
+        tgh._check(fn(ptr, uptr, t, l2m, lim))
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tgh._check(fn(ptr, uptr, t, l2m, lim))
+        monitors[name] = (time.perf_counter() - t2) * 1e3
+    result["monitors"] = monitors
     L.tgh_problem_destroy(ptr)
     del k, y
     if not args.no_forest:

@@ -236,6 +236,14 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
  * the masked norms of the four components over the evolved points
  * (4 doubles each, host memory), counted as tgh_error_norms counts them. */
 int tgh_gauge_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf);
+/* adm_constraint!(p, u, t) then constraint_norms(p)'s Hamiltonian and
+ * momentum rows (src/constraints.jl:100-210,432-534,584-605,848-876): the
+ * ADM constraints H = 2 G_nn and M_i = -G_n i at every owned point, from the
+ * state's first and second derivatives and the time derivatives the
+ * evolution equations give (no dissipation); l2 and linf get the masked
+ * norms of (H, M_x, M_y, M_z) (4 doubles each, host memory), counted as
+ * tgh_error_norms counts them. Vacuum only (no matter adapter). */
+int tgh_adm_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf);
 /* No reference counterpart (the reference's kernels take no launch shape):
  * the autotuner of the brick stepper's tile shapes. Steps u three times per
  * candidate shape of the Pi kernel's LDS tile (TGH_PI_TILES of them; every

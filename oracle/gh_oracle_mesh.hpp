@@ -979,6 +979,181 @@ struct Problem {
     for (int a = 0; a < 4; ++a) l2[a] = den > 0 ? std::sqrt(num[a] / den) : 0.0;
   }
 
+  // adm_constraint! + constraint_norms' Hamiltonian and momentum rows
+  // (constraints.jl:100-210,432-534,584-605,848-876): at every evolved owned
+  // point the time derivatives of g from the evolution equations (d_t g and
+  // d_t Pi from gh_node_rhs_expanded without dissipation, d_i d_t g and
+  // d_t d_t g by differentiating the first equation, metric_derivatives_along,
+  // pointwise.jl:530-545), the textbook Ricci tensor of the 4-metric and its
+  // projections H = 2 R_nn + R, M_i = -n^a (R_ai - g_ai R / 2). ham = {L2, Linf},
+  // mom_l2 / mom_linf per component, counted as error_norms counts.
+  static void along(const MQ<double>& m, const V10<double>& dgv, double& da, double db[3], double& dsq) {
+    M4<double> dgl = sym4(dgv);
+    M4<double> t = matmul(matmul(m.gu4, dgl), m.gu4);
+    const double dgu00 = -t[0][0];
+    da = m.alpha * m.alpha * m.alpha * dgu00 / 2.0;
+    for (int j = 0; j < 3; ++j) db[j] = -(-t[0][j + 1] + m.beta[j] * dgu00) / m.gu4[0][0];
+    double s = 0;
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) s += m.gamu[j][k] * dgl[j + 1][k + 1];
+    dsq = m.sqrtg * s / 2.0;
+  }
+  static void adm_at_node(const MQ<double>& m, const double dg[4][4][4], const double ddg[4][4][4][4],
+                          double& H, double Mo[3]) {
+    // dg[a][b][c] = d_a g_bc, ddg[a][b][c][d] = d_a d_b g_cd
+    double dgu[4][4][4];
+    for (int e = 0; e < 4; ++e) {
+      M4<double> de;
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) de[a][b] = dg[e][a][b];
+      M4<double> t = matmul(matmul(m.gu4, de), m.gu4);
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) dgu[e][a][b] = -t[a][b];
+    }
+    double Gl[4][4][4], G4[4][4][4], Gtr[4];
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
+        for (int c = 0; c < 4; ++c) Gl[a][b][c] = (dg[b][a][c] + dg[c][a][b] - dg[a][b][c]) / 2;
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
+        for (int c = 0; c < 4; ++c) {
+          double s = 0;
+          for (int x = 0; x < 4; ++x) s += m.gu4[a][x] * Gl[x][b][c];
+          G4[a][b][c] = s;
+        }
+    for (int d = 0; d < 4; ++d) {
+      double s = 0;
+      for (int c = 0; c < 4; ++c) s += G4[c][c][d];
+      Gtr[d] = s;
+    }
+    double R[4][4];
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        double A = 0, B = 0, C = 0, D = 0;
+        for (int c = 0; c < 4; ++c)
+          for (int d = 0; d < 4; ++d) {
+            A += dgu[c][c][d] * Gl[d][a][b] + m.gu4[c][d] * (ddg[c][a][d][b] + ddg[c][b][d][a] - ddg[c][d][a][b]) / 2;
+            B += dgu[b][c][d] * Gl[d][c][a] + m.gu4[c][d] * (ddg[b][c][d][a] + ddg[b][a][d][c] - ddg[b][d][c][a]) / 2;
+            D += G4[c][b][d] * G4[d][c][a];
+          }
+        for (int d = 0; d < 4; ++d) C += Gtr[d] * G4[d][a][b];
+        R[a][b] = A - B + C - D;
+      }
+    double Rs = 0;
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) Rs += m.gu4[a][b] * R[a][b];
+    const double n[4] = {1.0 / m.alpha, -m.beta[0] / m.alpha, -m.beta[1] / m.alpha, -m.beta[2] / m.alpha};
+    double Rnn = 0;
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) Rnn += R[a][b] * n[a] * n[b];
+    H = 2 * Rnn + Rs;
+    for (int i = 0; i < 3; ++i) {
+      double s = 0;
+      for (int a = 0; a < 4; ++a) s += n[a] * (R[a][i + 1] - m.g4[a][i + 1] * Rs / 2);
+      Mo[i] = -s;
+    }
+  }
+  void adm_constraint(const double* u, double t, double ham[2], double mom_l2[3], double mom_linf[3]) {
+    prepare(u, t);
+    const Mesh& m = mesh;
+    double num[4] = {0, 0, 0, 0}, den = 0, mx[4] = {0, 0, 0, 0};
+    const long sv = m.S * m.S * m.S;
+    const long st[3] = {1, m.S, m.S * m.S};
+    const int pr[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    auto pair3 = [](int i, int j) {
+      const int a = i < j ? i : j, b = i < j ? j : i;
+      return a == 0 ? b : (a == 1 ? 2 + b : 5);
+    };
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3];
+            m.position(b, i, j, k, x);
+            if (!is_evolved(t, x)) continue;
+            const bool prim = m.is_primary(b, i, j, k);
+            if (m.forest() && !prim) continue;
+            const long var = m.widx(b, 0, i, j, k);
+            const double inv_h = 1.0 / m.hb(b), inv_h2 = inv_h * inv_h;
+            V10<double> hv, Pv, dh[3], dP[3], ddh[6];
+            for (int v = 0; v < NC; ++v) {
+              hv[v] = work[var + v * sv];
+              Pv[v] = work[var + (NC + v) * sv];
+            }
+            for (int d = 0; d < 3; ++d)
+              for (int v = 0; v < NC; ++v) {
+                dh[d][v] = inv_h * axis_stencil<double>(w1, var + v * sv, st[d]);
+                dP[d][v] = inv_h * axis_stencil<double>(w1, var + (NC + v) * sv, st[d]);
+              }
+            for (int n = 0; n < 6; ++n)
+              for (int v = 0; v < NC; ++v) {
+                const int d1 = pr[n][0], d2 = pr[n][1];
+                ddh[n][v] = inv_h2 * (d1 == d2 ? axis_stencil<double>(w2, var + v * sv, st[d1])
+                                               : mixed_stencil<double>(w1, var + v * sv, st[d1], st[d2]));
+              }
+            MQ<double> mq = metric_quantities(hv);
+            V10<double> dth0;
+            for (int n = 0; n < NC; ++n)
+              dth0[n] = mq.beta[0] * dh[0][n] + mq.beta[1] * dh[1][n] + mq.beta[2] * dh[2][n] +
+                        (mq.alpha / mq.sqrtg) * Pv[n];
+            double Hl[4] = {0, 0, 0, 0}, dHl[4][4] = {{0}};
+            if (cs.gauge == GAUGE_SAMPLED) {
+              for (int a = 0; a < 4; ++a) Hl[a] = Hsrc[m.sidx(b, a, i, j, k)];
+              for (int n = 0; n < 16; ++n) dHl[n % 4][n / 4] = Hsrc[m.sidx(b, 4 + n, i, j, k)];
+            } else if (cs.gauge == GAUGE_KS) {
+              ks_algebraic_source<double>(cs.bg.M, cs.ks_S, cs.ks_u, hv, dth0, dh, Hl, dHl);
+            }
+            V10<double> dth, dtP;
+            gh_node_rhs_expanded(hv, Pv, dh, dP, ddh, Hl, dHl, damping_rate(cs, t, x), cs.g2, dth, dtP);
+            const double phi = mq.alpha / mq.sqrtg;
+            V10<double> dtd[3], dtt;
+            for (int ii = 0; ii < 3; ++ii) {
+              double da, db[3], dsq;
+              along(mq, dh[ii], da, db, dsq);
+              const double dphi = (da - phi * dsq) / mq.sqrtg;
+              for (int v = 0; v < NC; ++v)
+                dtd[ii][v] = (dphi * Pv[v] + phi * dP[ii][v]) + (db[0] * dh[0][v] + db[1] * dh[1][v] + db[2] * dh[2][v]) +
+                             (mq.beta[0] * ddh[pair3(ii, 0)][v] + mq.beta[1] * ddh[pair3(ii, 1)][v] +
+                              mq.beta[2] * ddh[pair3(ii, 2)][v]);
+            }
+            {
+              double da, db[3], dsq;
+              along(mq, dth, da, db, dsq);
+              const double dphi = (da - phi * dsq) / mq.sqrtg;
+              for (int v = 0; v < NC; ++v)
+                dtt[v] = dphi * Pv[v] + phi * dtP[v] + (db[0] * dh[0][v] + db[1] * dh[1][v] + db[2] * dh[2][v]) +
+                         (mq.beta[0] * dtd[0][v] + mq.beta[1] * dtd[1][v] + mq.beta[2] * dtd[2][v]);
+            }
+            double dg[4][4][4], ddg[4][4][4][4];
+            dg4(dth, dh, dg);
+            for (int mu = 0; mu < 4; ++mu)
+              for (int nu = 0; nu < 4; ++nu) {
+                const V10<double>* src;
+                if (mu == 0 && nu == 0) src = &dtt;
+                else if (mu == 0) src = &dtd[nu - 1];
+                else if (nu == 0) src = &dtd[mu - 1];
+                else src = &ddh[pair3(mu - 1, nu - 1)];
+                for (int a = 0; a < 4; ++a)
+                  for (int c = 0; c < 4; ++c) ddg[mu][nu][a][c] = (*src)[pairindex(a, c)];
+              }
+            double H, Mo[3];
+            adm_at_node(mq, dg, ddg, H, Mo);
+            const double wv = m.forest() ? 1.0 / (double)(1L << (3 * m.level[b])) : 1.0;
+            const double vals[4] = {H, Mo[0], Mo[1], Mo[2]};
+            for (int q = 0; q < 4; ++q) {
+              mx[q] = std::max(mx[q], std::abs(vals[q]));
+              if (prim) num[q] += m.forest() ? vals[q] * vals[q] * wv : vals[q] * vals[q];
+            }
+            if (prim) den += wv;
+          }
+    ham[0] = den > 0 ? std::sqrt(num[0] / den) : 0.0;
+    ham[1] = mx[0];
+    for (int q = 0; q < 3; ++q) {
+      mom_l2[q] = den > 0 ? std::sqrt(num[1 + q] / den) : 0.0;
+      mom_linf[q] = mx[1 + q];
+    }
+  }
+
   // One classical RK4 step with the limiter on each stage value and on the
   // result (stepping.jl:53-73, IMEXRungeKutta RK4).
   void rk4_step(double* u, double t, double dt, int nthreads = 1) {

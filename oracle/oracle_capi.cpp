@@ -420,6 +420,9 @@ void orc_error_norms(void* p, const double* u, double t, double* l2, double* lin
 void orc_gauge_constraint(void* p, const double* u, double t, double* l2, double* linf) {
   ((Problem*)p)->gauge_constraint(u, t, l2, linf);
 }
+void orc_adm_constraint(void* p, const double* u, double t, double* ham, double* mom_l2, double* mom_linf) {
+  ((Problem*)p)->adm_constraint(u, t, ham, mom_l2, mom_linf);
+}
 int orc_is_primary(void* p, long b, long i, long j, long k) {
   return ((Problem*)p)->mesh.is_primary(b, i, j, k) ? 1 : 0;
 }

@@ -57,6 +57,7 @@ def lib():
             "orc_block_level": (C.c_int, [P, Lg]),
             "orc_set_target": (None, [P, dp, D, C.c_int, D]),
             "orc_gauge_constraint": (None, [P, dp, D, dp, dp]),
+            "orc_adm_constraint": (None, [P, dp, D, dp, dp, dp]),
             "orc_shape_series": (D, [dp, C.c_int, dp]),
             "orc_interior_geo": (None, [desc, D, dp, dp]),
             "orc_forest_classify": (C.c_int, [P, Lg, Lg, Lg, Lg, C.POINTER(Lg)]),
@@ -144,6 +145,12 @@ class OracleProblem:
         l2, li = np.zeros(4), np.zeros(4)
         lib().orc_gauge_constraint(self.p, np.ascontiguousarray(u), float(t), l2, li)
         return l2, li
+
+    def adm_constraint(self, u, t):
+        """((L2, Linf) of H, L2[3], Linf[3] of M_i) over the evolved points."""
+        ham, ml2, mli = np.zeros(2), np.zeros(3), np.zeros(3)
+        lib().orc_adm_constraint(self.p, np.ascontiguousarray(u), float(t), ham, ml2, mli)
+        return ham, ml2, mli
 
     def block_level(self, b):
         return lib().orc_block_level(self.p, b)

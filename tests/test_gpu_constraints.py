@@ -81,3 +81,34 @@ def test_gauge_constraint_converges_on_the_frozen_hierarchy(tgh):
     rate = float(((x - x.mean()) * (y - y.mean())).sum() / ((x - x.mean()) ** 2).sum())
     print(f"gauge constraint on the frozen hierarchy, q=2: {gl}, rate {rate:.3f}")
     assert rate > 2 - 0.25
+
+
+def _compare_adm(tgh, dev, orc, u, t):
+    torch = torch_cuda()
+    ud = torch.from_numpy(u.copy()).cuda()
+    l2g, lig = tgh.adm_constraint_norms(dev, ud, t)
+    ham, ml2, mli = orc.adm_constraint(u, t)
+    l2o, lio = [ham[0]] + list(ml2), [ham[1]] + list(mli)
+    print(f"ADM constraints: GPU l2 {l2g} linf {lig}; oracle l2 {l2o} linf {lio}")
+    np.testing.assert_allclose(l2g, l2o, rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(lig, lio, rtol=1e-8, atol=1e-14)
+
+
+@pytest.mark.parametrize("gauge_source", [None, "algebraic"])
+def test_adm_constraints_match_oracle_on_a_uniform_hole(tgh, gauge_source):
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=2.0, r_0=0.71, r_1=1.21, gauge_source=gauge_source)
+    dev = tgh.GHProblem(tgh.gh_forest(None, case, N=16, roots=4), case, q=4)
+    orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc))
+    u = orc.fill_exact(0.0)
+    orc.add_noise(u, 0xADA1, 1e-6)
+    _compare_adm(tgh, dev, orc, u, 0.0)
+
+
+def test_adm_constraints_match_oracle_on_a_forest(tgh):
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    f = tgh.hole_forest(None, case, N=8, roots=4, radii=(2.5, 1.0))
+    dev = tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=6))
+    orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc), f.leaves, 6)
+    u = orc.fill_exact(0.0)
+    orc.add_noise(u, 0xADA2, 1e-6)
+    _compare_adm(tgh, dev, orc, u, 0.0)

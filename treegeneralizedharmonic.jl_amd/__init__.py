@@ -110,6 +110,8 @@ def _load():
         "tgh_set_tile": (C.c_int, [P, C.c_int32]),
         "tgh_gauge_constraint": (C.c_int, [P, C.c_void_p, C.c_double, C.POINTER(C.c_double),
                                            C.POINTER(C.c_double)]),
+        "tgh_adm_constraint": (C.c_int, [P, C.c_void_p, C.c_double, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]),
         "tgh_problem_destroy": (C.c_int, [P]),
         "tgh_state_length": (C.c_int64, [P]),
         "tgh_work_length": (C.c_int64, [P]),
@@ -1047,6 +1049,14 @@ def gauge_constraint_norms(p, u, t):
     four components of C_a = g_ab (Γ^b + g^{bc} H_c)."""
     l2, li = (C.c_double * 4)(), (C.c_double * 4)()
     _check(_load().tgh_gauge_constraint(p.ptr, _dev(u, p, "u"), float(t), l2, li))
+    return list(l2), list(li)
+
+
+def adm_constraint_norms(p, u, t):
+    """adm_constraint!(p, u, t) + constraint_norms(p)'s ham and mom rows
+    (src/constraints.jl:432-534,584-605,848-876): (L2, L∞) of (ℋ, ℳ_x, ℳ_y, ℳ_z)."""
+    l2, li = (C.c_double * 4)(), (C.c_double * 4)()
+    _check(_load().tgh_adm_constraint(p.ptr, _dev(u, p, "u"), float(t), l2, li))
     return list(l2), list(li)
 
 

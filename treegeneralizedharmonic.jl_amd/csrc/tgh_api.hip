@@ -961,19 +961,30 @@ int tgh_error_norms(tgh_problem* p, const double* u, double t, double* l2, doubl
   return TGH_OK;
 }
 
-int tgh_gauge_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf) {
-  if (!p || !u || !l2 || !linf) return fail(TGH_EINVAL, "NULL argument");
+// The two constraint monitors: prepare u at t, launch the monitor (adm
+// selects the ADM one), and fold its 9 partials per workgroup into the masked
+// norms of its four components.
+static int monitor(tgh_problem* p, const double* u, double t, bool adm, double* l2, double* linf) {
   const Geom& g = p->geom;
   const long nwg = (long)g.nblocks * ((g.P * g.P * g.P + 255) / 256);
   if (!p->gc) HIP_TRY(hipMalloc(&p->gc, sizeof(double) * 9 * nwg));
   launch_scatter(p, u, p->W[0], 1);
   launch_fill(p, p->W[0], t);
   GCArgs a{g, p->cs, t, p->W[0], p->Hsrc, p->gc};
-  switch (p->q) {
-    case 2: hipLaunchKernelGGL(k_gauge_constraint<2>, grid_points(p), dim3(256), 0, p->stream, a); break;
-    case 4: hipLaunchKernelGGL(k_gauge_constraint<4>, grid_points(p), dim3(256), 0, p->stream, a); break;
-    case 6: hipLaunchKernelGGL(k_gauge_constraint<6>, grid_points(p), dim3(256), 0, p->stream, a); break;
-    default: hipLaunchKernelGGL(k_gauge_constraint<8>, grid_points(p), dim3(256), 0, p->stream, a); break;
+  if (adm) {
+    switch (p->q) {
+      case 2: hipLaunchKernelGGL(k_adm_constraint<2>, grid_points(p), dim3(256), 0, p->stream, a); break;
+      case 4: hipLaunchKernelGGL(k_adm_constraint<4>, grid_points(p), dim3(256), 0, p->stream, a); break;
+      case 6: hipLaunchKernelGGL(k_adm_constraint<6>, grid_points(p), dim3(256), 0, p->stream, a); break;
+      default: hipLaunchKernelGGL(k_adm_constraint<8>, grid_points(p), dim3(256), 0, p->stream, a); break;
+    }
+  } else {
+    switch (p->q) {
+      case 2: hipLaunchKernelGGL(k_gauge_constraint<2>, grid_points(p), dim3(256), 0, p->stream, a); break;
+      case 4: hipLaunchKernelGGL(k_gauge_constraint<4>, grid_points(p), dim3(256), 0, p->stream, a); break;
+      case 6: hipLaunchKernelGGL(k_gauge_constraint<6>, grid_points(p), dim3(256), 0, p->stream, a); break;
+      default: hipLaunchKernelGGL(k_gauge_constraint<8>, grid_points(p), dim3(256), 0, p->stream, a); break;
+    }
   }
   HIP_TRY(hipGetLastError());
   std::vector<double> h(9 * nwg);
@@ -992,6 +1003,16 @@ int tgh_gauge_constraint(tgh_problem* p, const double* u, double t, double* l2, 
     linf[q] = mx[q];
   }
   return TGH_OK;
+}
+
+int tgh_gauge_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf) {
+  if (!p || !u || !l2 || !linf) return fail(TGH_EINVAL, "NULL argument");
+  return monitor(p, u, t, false, l2, linf);
+}
+
+int tgh_adm_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf) {
+  if (!p || !u || !l2 || !linf) return fail(TGH_EINVAL, "NULL argument");
+  return monitor(p, u, t, true, l2, linf);
 }
 
 int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t flags) {

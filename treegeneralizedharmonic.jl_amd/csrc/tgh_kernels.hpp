@@ -1590,6 +1590,197 @@ __global__ __launch_bounds__(256) void k_gauge_constraint(GCArgs a) {
   if (tid < 9) a.out[9 * ((long)b * ((P * P * P + 255) / 256) + wx) + tid] = sh[tid][0];
 }
 
+// The ADM monitor (adm_constraint_kernel!, src/constraints.jl:432-534, and
+// adm_constraints_at_node, :100-210) at every owned point: d_t g and d_t Pi
+// from the evolution equations without dissipation, d_i d_t g and d_t d_t g
+// by differentiating the first one (metric_derivatives_along,
+// src/pointwise.jl:530-545), the textbook Ricci tensor of the 4-metric, and
+// H = 2 R_nn + R, M_i = -n^a (R_ai - g_ai R / 2). Partials as the gauge
+// monitor's, 9 per workgroup: sums of w H^2 and w M_i^2, maxima of |H| and
+// |M_i|, the weights.
+__device__ __forceinline__ double g4_at(const double h[NC], int a, int b) {
+  return (a == b ? (a == 0 ? -1.0 : 1.0) : 0.0) + h[pidx(a, b)];
+}
+// d alpha, d beta^j, d sqrt(gamma) along a derivative dg of g (packed).
+__device__ __forceinline__ void adm_along(const Coef& cf, const double dg[NC], double& da, double db[3],
+                                          double& dsq) {
+  double t00 = 0.0, t0[3] = {0.0, 0.0, 0.0};
+  for (int c = 0; c < 4; ++c)
+    for (int d = 0; d < 4; ++d) {
+      const double x = cf.gu[pidx(0, c)] * dg[pidx(c, d)];
+      t00 += x * cf.gu[pidx(d, 0)];
+      for (int j = 0; j < 3; ++j) t0[j] += x * cf.gu[pidx(d, j + 1)];
+    }
+  const double dgu00 = -t00;
+  da = cf.alpha * cf.alpha * cf.alpha * dgu00 / 2.0;
+  for (int j = 0; j < 3; ++j) db[j] = -(-t0[j] + cf.beta[j] * dgu00) / cf.gu[0];
+  double sq = 0.0;
+  for (int j = 0; j < 3; ++j)
+    for (int k = 0; k < 3; ++k) sq += cf.gamu[pidx(j + 1, k + 1) - 4] * dg[pidx(j + 1, k + 1)];
+  dsq = cf.sqrtg * sq / 2.0;
+}
+template <int Q>
+__global__ __launch_bounds__(256) void k_adm_constraint(GCArgs a) {
+  const Geom& g = a.g;
+  const int P = g.P, S = g.S, G = g.G;
+  int b, wx;
+  if (!wg_map((P * P * P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
+  const long psz = (long)P * P * P, sv = (long)S * S * S;
+  double r[9];
+  for (int n = 0; n < 9; ++n) r[n] = 0.0;
+  if (p < P * P * P) {
+    const int i = p % P, j = (p / P) % P, k = p / (P * P);
+    double x[3];
+    position(g, b, i, j, k, x);
+    const CaseP& c = a.c;
+    if (is_evolved(c, a.t, x)) {
+      int l[3] = {i, j, k}, owner, loc[3];
+      const bool prim = g.prim ? g.prim[(long)b * psz + p] != 0 : classify(g, b, l, owner, loc) == -1;
+      if (!(g.prim && !prim)) {
+        const double* w0 = a.W + (long)b * 20 * sv + (i + G) + (long)S * ((j + G) + (long)S * (k + G));
+        const double ih = block_inv_h(g, b), ih2 = ih * ih;
+        const int st[3] = {1, S, S * S};
+        double hv[NC], Pv[NC], D[4][NC], dP[3][NC], dd[10][NC];  // dd: tt, tx, ty, tz, xx, xy, xz, yy, yz, zz
+        for (int v = 0; v < NC; ++v) {
+          const double* wh = w0 + v * sv;
+          const double* wp = w0 + (NC + v) * sv;
+          hv[v] = wh[0];
+          Pv[v] = wp[0];
+          for (int d = 0; d < 3; ++d) {
+            D[1 + d][v] = ih * st_d1<Q>(wh, st[d]);
+            dP[d][v] = ih * st_d1<Q>(wp, st[d]);
+          }
+          for (int d1 = 0; d1 < 3; ++d1)
+            for (int d2 = d1; d2 < 3; ++d2)
+              dd[pidx(d1 + 1, d2 + 1)][v] =
+                  ih2 * (d1 == d2 ? st_d2<Q>(wh, st[d1]) : st_mixed<Q>(wh, st[d1], st[d2]));
+        }
+        Coef cf;
+        metric_quantities(hv, cf);
+        const double phi = cf.alpha / cf.sqrtg;
+        for (int v = 0; v < NC; ++v)
+          D[0][v] = cf.beta[0] * D[1][v] + cf.beta[1] * D[2][v] + cf.beta[2] * D[3][v] + phi * Pv[v];
+        double Hl[4] = {0.0, 0.0, 0.0, 0.0}, dH[4][4];
+        for (int q = 0; q < 16; ++q) dH[q % 4][q / 4] = 0.0;
+        if (c.gauge == GAUGE_SAMPLED) {
+          const double* Hp = a.Hsrc + (long)b * 20 * psz + p;
+          for (int q = 0; q < 4; ++q) Hl[q] = Hp[q * psz];
+          for (int n = 0; n < 16; ++n) dH[n % 4][n / 4] = Hp[(4 + n) * psz];
+        } else if (c.gauge == GAUGE_KS) {
+          ks_source(c.bg.M, c.bg.a, hv, D, Hl, dH);
+        } else if (c.gauge == GAUGE_KS_MOVING) {
+          ks_source_moving(c.bg.M, c.ks_S, c.ks_u, hv, D, Hl, dH);
+        }
+        double msrc[NC];
+        gh_source<true>(cf, hv, D, Hl, dH, damping_rate(c, a.t, x), c.g2, msrc);
+        PiCoef pc;
+        head_geometry(cf, D, pc);
+        double dtP[NC];
+        for (int v = 0; v < NC; ++v)
+          dtP[v] = pc.b[0] * dP[0][v] + pc.b[1] * dP[1][v] + pc.b[2] * dP[2][v] + pc.divb * Pv[v] +
+                   pc.divA[0] * D[1][v] + pc.divA[1] * D[2][v] + pc.divA[2] * D[3][v] + pc.A[0] * dd[4][v] +
+                   pc.A[3] * dd[7][v] + pc.A[5] * dd[9][v] +
+                   2.0 * (pc.A[1] * dd[5][v] + pc.A[2] * dd[6][v] + pc.A[4] * dd[8][v]) + msrc[v];
+        // d_i d_t g, then d_t d_t g
+        for (int ii = 0; ii < 3; ++ii) {
+          double da, db[3], dsq;
+          adm_along(cf, D[1 + ii], da, db, dsq);
+          const double dphi = (da - phi * dsq) / cf.sqrtg;
+          for (int v = 0; v < NC; ++v)
+            dd[1 + ii][v] = (dphi * Pv[v] + phi * dP[ii][v]) +
+                            (db[0] * D[1][v] + db[1] * D[2][v] + db[2] * D[3][v]) +
+                            (cf.beta[0] * dd[pidx(ii + 1, 1)][v] + cf.beta[1] * dd[pidx(ii + 1, 2)][v] +
+                             cf.beta[2] * dd[pidx(ii + 1, 3)][v]);
+        }
+        {
+          double da, db[3], dsq;
+          adm_along(cf, D[0], da, db, dsq);
+          const double dphi = (da - phi * dsq) / cf.sqrtg;
+          for (int v = 0; v < NC; ++v)
+            dd[0][v] = dphi * Pv[v] + phi * dtP[v] + (db[0] * D[1][v] + db[1] * D[2][v] + db[2] * D[3][v]) +
+                       (cf.beta[0] * dd[1][v] + cf.beta[1] * dd[2][v] + cf.beta[2] * dd[3][v]);
+        }
+        // dgu[e] = -g^-1 (d_e g) g^-1, Gamma_{a,bc}, Gamma^a_bc, Gamma^c_cd
+        double dgu[4][NC], Gl[4][NC], G4[4][NC], Gtr[4];
+        for (int e = 0; e < 4; ++e)
+          for (int a2 = 0; a2 < 4; ++a2)
+            for (int b2 = a2; b2 < 4; ++b2) {
+              double s2 = 0.0;
+              for (int cc = 0; cc < 4; ++cc)
+                for (int d = 0; d < 4; ++d) s2 += cf.gu[pidx(a2, cc)] * D[e][pidx(cc, d)] * cf.gu[pidx(d, b2)];
+              dgu[e][pidx(a2, b2)] = -s2;
+            }
+        for (int a2 = 0; a2 < 4; ++a2)
+          for (int b2 = 0; b2 < 4; ++b2)
+            for (int c2 = b2; c2 < 4; ++c2)
+              Gl[a2][pidx(b2, c2)] = (D[b2][pidx(a2, c2)] + D[c2][pidx(a2, b2)] - D[a2][pidx(b2, c2)]) / 2;
+        for (int a2 = 0; a2 < 4; ++a2)
+          for (int n = 0; n < NC; ++n) {
+            double s2 = 0.0;
+            for (int xx = 0; xx < 4; ++xx) s2 += cf.gu[pidx(a2, xx)] * Gl[xx][n];
+            G4[a2][n] = s2;
+          }
+        for (int d = 0; d < 4; ++d) {
+          double s2 = 0.0;
+          for (int cc = 0; cc < 4; ++cc) s2 += G4[cc][pidx(cc, d)];
+          Gtr[d] = s2;
+        }
+        // d_m d_n g_ab
+        auto DD = [&](int m, int n, int a2, int b2) { return dd[pidx(m, n)][pidx(a2, b2)]; };
+        double R[4][4];
+        for (int a2 = 0; a2 < 4; ++a2)
+          for (int b2 = 0; b2 < 4; ++b2) {
+            double A = 0.0, B = 0.0, Cc = 0.0, Dd = 0.0;
+            for (int cc = 0; cc < 4; ++cc)
+              for (int d = 0; d < 4; ++d) {
+                const double gcd = cf.gu[pidx(cc, d)];
+                A += dgu[cc][pidx(cc, d)] * Gl[d][pidx(a2, b2)] +
+                     gcd * (DD(cc, a2, d, b2) + DD(cc, b2, d, a2) - DD(cc, d, a2, b2)) / 2;
+                B += dgu[b2][pidx(cc, d)] * Gl[d][pidx(cc, a2)] +
+                     gcd * (DD(b2, cc, d, a2) + DD(b2, a2, d, cc) - DD(b2, d, cc, a2)) / 2;
+                Dd += G4[cc][pidx(b2, d)] * G4[d][pidx(cc, a2)];
+              }
+            for (int d = 0; d < 4; ++d) Cc += Gtr[d] * G4[d][pidx(a2, b2)];
+            R[a2][b2] = A - B + Cc - Dd;
+          }
+        double Rs = 0.0;
+        for (int a2 = 0; a2 < 4; ++a2)
+          for (int b2 = 0; b2 < 4; ++b2) Rs += cf.gu[pidx(a2, b2)] * R[a2][b2];
+        const double nv[4] = {1.0 / cf.alpha, -cf.beta[0] / cf.alpha, -cf.beta[1] / cf.alpha, -cf.beta[2] / cf.alpha};
+        double Rnn = 0.0;
+        for (int a2 = 0; a2 < 4; ++a2)
+          for (int b2 = 0; b2 < 4; ++b2) Rnn += R[a2][b2] * nv[a2] * nv[b2];
+        double vals[4];
+        vals[0] = 2 * Rnn + Rs;
+        for (int q = 0; q < 3; ++q) {
+          double s2 = 0.0;
+          for (int a2 = 0; a2 < 4; ++a2) s2 += nv[a2] * (R[a2][q + 1] - g4_at(hv, a2, q + 1) * Rs / 2);
+          vals[1 + q] = -s2;
+        }
+        const double wgt = g.blevel ? ldexp(1.0, -3 * block_level(g, b)) : 1.0;
+        for (int q = 0; q < 4; ++q) {
+          r[4 + q] = fabs(vals[q]);
+          if (prim) r[q] = vals[q] * vals[q] * wgt;
+        }
+        if (prim) r[8] = wgt;
+      }
+    }
+  }
+  __shared__ double sh[9][256];
+  const int tid = threadIdx.x;
+  for (int n = 0; n < 9; ++n) sh[n][tid] = r[n];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      for (int n = 0; n < 9; ++n)
+        sh[n][tid] = (n >= 4 && n < 8) ? fmax(sh[n][tid], sh[n][tid + w]) : sh[n][tid] + sh[n][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid < 9) a.out[9 * ((long)b * ((P * P * P + 255) / 256) + wx) + tid] = sh[tid][0];
+}
+
 #endif  // TGH_RHS_ONLY
 
 // ---------------------------------------------------------------------------
