@@ -252,7 +252,7 @@ int tgh_adm_constraint(tgh_problem* p, const double* u, double t, double* l2, do
  * hipEvents, writes the average ms per launch into ms (TGH_PI_TILES doubles,
  * or NULL) and keeps the fastest for the problem's later tgh_rk4 calls.
  * Pass a scratch copy of the state. TGH_EUNSUPPORTED on the block layout. */
-enum { TGH_PI_TILES = 4 };
+enum { TGH_PI_TILES = 6 };
 int tgh_autotune(tgh_problem* p, double* u, double t0, double dt, double* ms);
 /* No reference counterpart: pins the Pi kernel's tile shape (0 .. TGH_PI_TILES - 1;
  * 0 is the default). */

@@ -1030,7 +1030,7 @@ def error_norms(p, u, t):
     return l2.value, li.value
 
 
-TGH_PI_TILES = 4
+TGH_PI_TILES = 6
 
 
 def autotune(p, u, dt, t=0.0):

@@ -2009,16 +2009,20 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
 // banks (24 = -8 mod 32: row r starts at bank 48 r mod 64).
 // The Pi kernel's tile shapes the autotuner chooses among (tgh_autotune):
 // TC 0 the measured default, TC 1 a taller tile, TC 2 a wider square one,
-// TC 3 a flat one whose box no longer fits one column per thread. Every shape
-// computes each point with the same arithmetic in the same order: only the
-// staging differs, so the results are the same bits.
-constexpr int PI_TILES = 4;
+// TC 3 a flat one whose box no longer fits one column per thread, TC 4 and 5
+// the default tile with an LDS row pitch of 25 and 26 doubles instead of 24
+// (the banks a wave's rows land on). Every shape computes each point with the
+// same arithmetic in the same order: only the staging differs, so the
+// results are the same bits.
+constexpr int PI_TILES = 6;
 template <int Q, int TC = 0> struct PiTile {
-  static constexpr int X = TC == 0 ? TGH_PI_TILE_X : TC == 1 ? 8 : TC == 2 ? 10 : 16;
-  static constexpr int Y = TC == 0 ? TGH_PI_TILE_Y : TC == 1 ? 12 : TC == 2 ? 10 : 8;
-  static constexpr int Z = Q == 8 ? 2 : TC == 0 ? TGH_PI_TILE_Z : TC == 3 ? 2 : 4;
-  static constexpr int PZ = Q == 8 || TC != 0 ? 1 : TGH_PI_POINTS_Z;
-  static constexpr int pitch(int bx) { return bx <= 24 ? 24 : bx; }
+  static constexpr int X = TC == 0 || TC >= 4 ? TGH_PI_TILE_X : TC == 1 ? 8 : TC == 2 ? 10 : 16;
+  static constexpr int Y = TC == 0 || TC >= 4 ? TGH_PI_TILE_Y : TC == 1 ? 12 : TC == 2 ? 10 : 8;
+  static constexpr int Z = Q == 8 ? 2 : TC == 0 || TC >= 4 ? TGH_PI_TILE_Z : TC == 3 ? 2 : 4;
+  static constexpr int PZ = Q == 8 || !(TC == 0 || TC >= 4) ? 1 : TGH_PI_POINTS_Z;
+  static constexpr int pitch(int bx) {
+    return TC == 4 ? (bx <= 25 ? 25 : bx) : TC == 5 ? (bx <= 26 ? 26 : bx) : (bx <= 24 ? 24 : bx);
+  }
 };
 #ifdef TGH_BRICK_PI_WAVES_PER_EU
 #define TGH_BRICK_PI_ATTR __attribute__((amdgpu_waves_per_eu(TGH_BRICK_PI_WAVES_PER_EU)))

@@ -130,6 +130,8 @@ void launch_brick_rhs_q(tgh_problem* p, const RhsArgs& a) {
       case 1: launch_brick_pi<Q, 1>(p, a); break;
       case 2: launch_brick_pi<Q, 2>(p, a); break;
       case 3: launch_brick_pi<Q, 3>(p, a); break;
+      case 4: launch_brick_pi<Q, 4>(p, a); break;
+      case 5: launch_brick_pi<Q, 5>(p, a); break;
       default: launch_brick_pi<Q, 0>(p, a); break;
     }
     if (prof) {
