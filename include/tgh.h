@@ -244,6 +244,17 @@ int tgh_gauge_constraint(tgh_problem* p, const double* u, double t, double* l2, 
  * norms of (H, M_x, M_y, M_z) (4 doubles each, host memory), counted as
  * tgh_error_norms counts them. Vacuum only (no matter adapter). */
 int tgh_adm_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf);
+/* gh_tau!(...; scale, eps) then each block's maximum (src/refinement.jl:
+ * 152-219,312-378): prepares u at t and writes into tau_blocks (nblocks
+ * doubles, host memory) each block's largest Loehner indicator over its
+ * evolved owned points, the ten h components and the three axes, with the
+ * noise floor 4 eps |scale|; tau_max is their maximum. The regrid that reads
+ * it is not built. */
+int tgh_refinement_indicator(tgh_problem* p, const double* u, double t, double scale, double eps,
+                             double* tau_blocks);
+/* field_scale(U, mask, ...) (src/refinement.jl:221-300): the largest |h_v|
+ * over the evolved owned points, the indicator's reference amplitude. */
+int tgh_field_scale(tgh_problem* p, const double* u, double t, double* scale);
 /* No reference counterpart (the reference's kernels take no launch shape):
  * the autotuner of the brick stepper's tile shapes. Steps u three times per
  * candidate shape of the Pi kernel's LDS tile (TGH_PI_TILES of them; every

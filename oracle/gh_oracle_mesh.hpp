@@ -1154,6 +1154,58 @@ struct Problem {
     }
   }
 
+  // The refinement indicator (refinement.jl:152-183 lohner, :184-219
+  // cell_tau, :312-365 gh_tau_kernel!, :367-378 tau_max): at every evolved
+  // owned point the largest Loehner ratio |u+ - 2u0 + u-| / (|u+ - u0| +
+  // |u0 - u-| + 4 eps |scale|) over the ten h components and the three axes;
+  // per block the largest over its points (0 where none is evolved).
+  // field_scale (refinement.jl:221-300): the largest |h_v| over the evolved
+  // owned points.
+  static double lohner(double um, double u0, double up, double scale, double eps) {
+    const double num = std::abs(up - 2 * u0 + um);
+    const double den = std::abs(up - u0) + std::abs(u0 - um) + 4 * eps * std::abs(scale);
+    return den == 0.0 ? 0.0 : num / den;
+  }
+  void tau_blocks(const double* u, double t, double scale, double eps, double* out) {
+    prepare(u, t);
+    const Mesh& m = mesh;
+    const long sv = m.S * m.S * m.S;
+    const long st[3] = {1, m.S, m.S * m.S};
+    for (long b = 0; b < m.nblocks(); ++b) {
+      double tb = 0.0;
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3];
+            m.position(b, i, j, k, x);
+            if (!is_evolved(t, x)) continue;
+            const long base = m.widx(b, 0, i, j, k);
+            double tau = 0.0;
+            for (int v = 0; v < NC; ++v) {
+              const long i0 = base + v * sv;
+              for (int d = 0; d < 3; ++d)
+                tau = std::max(tau, lohner(work[i0 - st[d]], work[i0], work[i0 + st[d]], scale, eps));
+            }
+            tb = std::max(tb, tau);
+          }
+      out[b] = tb;
+    }
+  }
+  double field_scale(const double* u, double t) const {
+    const Mesh& m = mesh;
+    double s = 0.0;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            double x[3];
+            m.position(b, i, j, k, x);
+            if (!is_evolved(t, x)) continue;
+            for (int v = 0; v < NC; ++v) s = std::max(s, std::abs(u[m.sidx(b, v, i, j, k)]));
+          }
+    return s;
+  }
+
   // One classical RK4 step with the limiter on each stage value and on the
   // result (stepping.jl:53-73, IMEXRungeKutta RK4).
   void rk4_step(double* u, double t, double dt, int nthreads = 1) {

@@ -423,6 +423,10 @@ void orc_gauge_constraint(void* p, const double* u, double t, double* l2, double
 void orc_adm_constraint(void* p, const double* u, double t, double* ham, double* mom_l2, double* mom_linf) {
   ((Problem*)p)->adm_constraint(u, t, ham, mom_l2, mom_linf);
 }
+void orc_tau_blocks(void* p, const double* u, double t, double scale, double eps, double* out) {
+  ((Problem*)p)->tau_blocks(u, t, scale, eps, out);
+}
+double orc_field_scale(void* p, const double* u, double t) { return ((Problem*)p)->field_scale(u, t); }
 int orc_is_primary(void* p, long b, long i, long j, long k) {
   return ((Problem*)p)->mesh.is_primary(b, i, j, k) ? 1 : 0;
 }

@@ -58,6 +58,8 @@ def lib():
             "orc_set_target": (None, [P, dp, D, C.c_int, D]),
             "orc_gauge_constraint": (None, [P, dp, D, dp, dp]),
             "orc_adm_constraint": (None, [P, dp, D, dp, dp, dp]),
+            "orc_tau_blocks": (None, [P, dp, D, D, D, dp]),
+            "orc_field_scale": (D, [P, dp, D]),
             "orc_shape_series": (D, [dp, C.c_int, dp]),
             "orc_interior_geo": (None, [desc, D, dp, dp]),
             "orc_forest_classify": (C.c_int, [P, Lg, Lg, Lg, Lg, C.POINTER(Lg)]),
@@ -151,6 +153,15 @@ class OracleProblem:
         ham, ml2, mli = np.zeros(2), np.zeros(3), np.zeros(3)
         lib().orc_adm_constraint(self.p, np.ascontiguousarray(u), float(t), ham, ml2, mli)
         return ham, ml2, mli
+
+    def tau_blocks(self, u, t, scale, eps=0.01):
+        nb = self.n // ((self.desc.N + 1) ** 3 * 20)
+        out = np.zeros(nb)
+        lib().orc_tau_blocks(self.p, np.ascontiguousarray(u), float(t), float(scale), float(eps), out)
+        return out
+
+    def field_scale(self, u, t):
+        return lib().orc_field_scale(self.p, np.ascontiguousarray(u), float(t))
 
     def block_level(self, b):
         return lib().orc_block_level(self.p, b)

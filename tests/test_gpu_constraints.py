@@ -112,3 +112,31 @@ def test_adm_constraints_match_oracle_on_a_forest(tgh):
     u = orc.fill_exact(0.0)
     orc.add_noise(u, 0xADA2, 1e-6)
     _compare_adm(tgh, dev, orc, u, 0.0)
+
+
+def test_refinement_indicator_matches_oracle(tgh):
+    """gh_tau! + per-block maxima on the device against the oracle, on a noisy
+    hole (uniform) and on a forest, with the field scale as the reference
+    amplitude."""
+    torch = torch_cuda()
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    small = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=2.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    for case, forest in ((small, tgh.gh_forest(None, small, N=16, roots=4)),
+                         (case, tgh.hole_forest(None, case, N=8, roots=4, radii=(2.5, 1.0)))):
+        if forest.refined:
+            dev = tgh.GHProblem(forest, case, q=4, ops=tgh.Operators(prolongation=6))
+            orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc), forest.leaves, 6)
+        else:
+            dev = tgh.GHProblem(forest, case, q=4)
+            orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc))
+        u = orc.fill_exact(0.0)
+        orc.add_noise(u, 0x7A0, 1e-4)
+        ud = torch.from_numpy(u.copy()).cuda()
+        s_d = tgh.field_scale(dev, ud, 0.0)
+        s_o = orc.field_scale(u, 0.0)
+        assert s_d == s_o
+        tg = np.array(tgh.refinement_indicator(dev, ud, 0.0, scale=s_o, eps=0.01))
+        to = orc.tau_blocks(u, 0.0, s_o, 0.01)
+        print(f"{forest.nleaves()} leaves: tau_max GPU {tg.max():.6e} oracle {to.max():.6e}, "
+              f"max |diff| {np.abs(tg - to).max():.3e}")
+        np.testing.assert_allclose(tg, to, rtol=1e-13, atol=0)

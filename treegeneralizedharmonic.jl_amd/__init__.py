@@ -112,6 +112,9 @@ def _load():
                                            C.POINTER(C.c_double)]),
         "tgh_adm_constraint": (C.c_int, [P, C.c_void_p, C.c_double, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),
+        "tgh_refinement_indicator": (C.c_int, [P, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                               C.POINTER(C.c_double)]),
+        "tgh_field_scale": (C.c_int, [P, C.c_void_p, C.c_double, C.POINTER(C.c_double)]),
         "tgh_problem_destroy": (C.c_int, [P]),
         "tgh_state_length": (C.c_int64, [P]),
         "tgh_work_length": (C.c_int64, [P]),
@@ -1058,6 +1061,21 @@ def adm_constraint_norms(p, u, t):
     l2, li = (C.c_double * 4)(), (C.c_double * 4)()
     _check(_load().tgh_adm_constraint(p.ptr, _dev(u, p, "u"), float(t), l2, li))
     return list(l2), list(li)
+
+
+def field_scale(p, u, t):
+    """field_scale (src/refinement.jl:221-300): the largest |h_v| over the evolved points."""
+    out = C.c_double()
+    _check(_load().tgh_field_scale(p.ptr, _dev(u, p, "u"), float(t), C.byref(out)))
+    return out.value
+
+
+def refinement_indicator(p, u, t, *, scale, eps=0.01):
+    """gh_tau! + each block's maximum (src/refinement.jl:152-219,312-378): the
+    Löhner indicator's largest value per block, in the state's block order."""
+    out = (C.c_double * p.nblocks)()
+    _check(_load().tgh_refinement_indicator(p.ptr, _dev(u, p, "u"), float(t), float(scale), float(eps), out))
+    return list(out)
 
 
 def gh_dt(p, u, *, cfl, t=0.0):

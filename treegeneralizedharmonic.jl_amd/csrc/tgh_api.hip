@@ -1005,6 +1005,52 @@ static int monitor(tgh_problem* p, const double* u, double t, bool adm, double* 
   return TGH_OK;
 }
 
+// The indicator's (scale >= 0) or the field scale's (scale < 0) partials,
+// folded per block into out (nblocks doubles).
+static int tau_pass(tgh_problem* p, const double* u, double t, double scale, double eps, double* out) {
+  const Geom& g = p->geom;
+  const int wgpb = (g.P * g.P * g.P + 255) / 256;
+  const long nwg = (long)g.nblocks * wgpb;
+  if (!p->gc) HIP_TRY(hipMalloc(&p->gc, sizeof(double) * 9 * nwg));
+  if (scale < 0.0) {
+    launch_scatter(p, u, p->W[0], 1);  // owned points only: no fill needed
+  } else {
+    launch_scatter(p, u, p->W[0], 1);
+    launch_fill(p, p->W[0], t);
+  }
+  TauArgs a{g, p->cs, t, scale, eps, p->W[0], p->gc};
+  hipLaunchKernelGGL(k_tau, grid_points(p), dim3(256), 0, p->stream, a);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h(nwg);
+  HIP_TRY(hipMemcpyAsync(h.data(), p->gc, sizeof(double) * nwg, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  for (int b = 0; b < g.nblocks; ++b) {
+    double m = 0.0;
+    for (int w = 0; w < wgpb; ++w) m = std::max(m, h[(long)b * wgpb + w]);
+    out[b] = m;
+  }
+  return TGH_OK;
+}
+
+int tgh_refinement_indicator(tgh_problem* p, const double* u, double t, double scale, double eps,
+                             double* tau_blocks) {
+  if (!p || !u || !tau_blocks) return fail(TGH_EINVAL, "NULL argument");
+  if (!(scale >= 0 && std::isfinite(scale)) || !(eps >= 0 && std::isfinite(eps)))
+    return fail(TGH_EINVAL, "the indicator's reference amplitude and noise fraction must be finite and >= 0");
+  return tau_pass(p, u, t, scale, eps, tau_blocks);
+}
+
+int tgh_field_scale(tgh_problem* p, const double* u, double t, double* scale) {
+  if (!p || !u || !scale) return fail(TGH_EINVAL, "NULL argument");
+  std::vector<double> per(p->geom.nblocks);
+  int rc = tau_pass(p, u, t, -1.0, 0.0, per.data());
+  if (rc) return rc;
+  double s = 0.0;
+  for (double v : per) s = std::max(s, v);
+  *scale = s;
+  return TGH_OK;
+}
+
 int tgh_gauge_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf) {
   if (!p || !u || !l2 || !linf) return fail(TGH_EINVAL, "NULL argument");
   return monitor(p, u, t, false, l2, linf);

@@ -1781,6 +1781,61 @@ __global__ __launch_bounds__(256) void k_adm_constraint(GCArgs a) {
   if (tid < 9) a.out[9 * ((long)b * ((P * P * P + 255) / 256) + wx) + tid] = sh[tid][0];
 }
 
+// The refinement indicator (lohner, cell_tau, gh_tau_kernel!, tau_max,
+// src/refinement.jl:152-219,312-378) on a prepared work array: the largest
+// Loehner ratio over the ten h components and three axes at every evolved
+// owned point, folded per workgroup into its maximum (the host takes each
+// block's over its workgroups); with scale < 0 the kernel folds the largest
+// |h_v| over the evolved owned points instead (field_scale, :221-300).
+struct TauArgs {
+  Geom g;
+  CaseP c;
+  double t, scale, eps;
+  const double* __restrict__ W;
+  double* __restrict__ out;
+};
+__global__ __launch_bounds__(256) void k_tau(TauArgs a) {
+  const Geom& g = a.g;
+  const int P = g.P, S = g.S, G = g.G;
+  int b, wx;
+  if (!wg_map((P * P * P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
+  const long sv = (long)S * S * S;
+  double tau = 0.0;
+  if (p < P * P * P) {
+    const int i = p % P, j = (p / P) % P, k = p / (P * P);
+    double x[3];
+    position(g, b, i, j, k, x);
+    if (is_evolved(a.c, a.t, x)) {
+      const double* w0 = a.W + (long)b * 20 * sv + (i + G) + (long)S * ((j + G) + (long)S * (k + G));
+      const long st[3] = {1, S, (long)S * S};
+      for (int v = 0; v < NC; ++v) {
+        const double* wv = w0 + v * sv;
+        const double u0 = wv[0];
+        if (a.scale < 0.0) {
+          tau = fmax(tau, fabs(u0));
+          continue;
+        }
+        for (int d = 0; d < 3; ++d) {
+          const double up = wv[st[d]], um = wv[-st[d]];
+          const double num = fabs(up - 2 * u0 + um);
+          const double den = fabs(up - u0) + fabs(u0 - um) + 4 * a.eps * fabs(a.scale);
+          tau = fmax(tau, den == 0.0 ? 0.0 : num / den);
+        }
+      }
+    }
+  }
+  __shared__ double sh[256];
+  const int tid = threadIdx.x;
+  sh[tid] = tau;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] = fmax(sh[tid], sh[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) a.out[(long)b * ((P * P * P + 255) / 256) + wx] = sh[0];
+}
+
 #endif  // TGH_RHS_ONLY
 
 // ---------------------------------------------------------------------------
