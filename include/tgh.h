@@ -14,7 +14,9 @@
  * Every function returns TGH_OK or an error code; tgh_last_error() gives the
  * message of the last failure on the calling thread. All work is ordered on
  * the problem's HIP stream and every verb synchronises before returning.
- * Nothing here allocates on the device except tgh_problem_create.
+ * Device memory is allocated by tgh_problem_create and tgh_batch_create,
+ * by the first brick-layout step and the first constraint monitor, and by
+ * tgh_transfer for its plan (freed before it returns).
  */
 #ifndef TGH_H
 #define TGH_H
@@ -159,6 +161,17 @@ int tgh_problem_create_forest(const tgh_desc* desc, const int32_t* leaves, int32
 /* The level of every block, 1 per block (0 on a uniform forest;
  * level(k) of TreeAMR's leaves). */
 int tgh_block_levels(const tgh_problem* p, int32_t* out);
+/* The regrid's state transfer (src/refinement.jl:973-1021: refine!/coarsen!
+ * then the state carried over onto the new leaves): u_to (to's state, DEVICE)
+ * takes at every owned point of to's forest the value that from's ghost
+ * fill at time t gives there -- a copy of from's owner of the point (an
+ * unchanged leaf, injection from a finer one) or from's order-prolongation
+ * interpolation out of the leaf one level coarser that holds it (to's
+ * order where from is uniform). Both cover the same root brick (N,
+ * nblocks, periodic, lo, hi); a uniform problem is the forest of its level-0
+ * blocks; to's leaves may be at most one level finer than from's where they
+ * lie. from's work array is overwritten. Synchronous. */
+int tgh_transfer(tgh_problem* from, const double* u_from, double t, tgh_problem* to, double* u_to);
 /* GHProblem(...; target, t_target, target_rate, trail) (src/evolution.jl:
  * 1007-1021,1034-1037) for a :fitted interior: target is a DEVICE array of
  * 40 variables per owned point in the state layout ((N+1)^3 x 40 x nblocks:

@@ -207,6 +207,14 @@ struct Mesh {
       if (periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
       else if (gl[d] < 0 || gl[d] > tot) return F_DIRICHLET;
     }
+    const int c = classify_point(L, gl, s);
+    if (c == F_COPY && s.b == b && s.loc[0] == l[0] && s.loc[1] == l[1] && s.loc[2] == l[2]) return F_PRIMARY;
+    return c;
+  }
+  // Where the point gl (index in the brick of level L, wrapped) takes its
+  // value from in this forest: F_COPY with its owner, or F_PROLONG from the
+  // level-(L-1) leaf holding it.
+  int classify_point(int L, const long gl[3], Src& s) const {
     for (int lv = Lmax; lv >= 0; --lv) {
       long gv[3];
       bool vertex = true;
@@ -222,7 +230,6 @@ struct Mesh {
       if (!vertex) continue;
       long ob, loc[3];
       if (!find_leaf(lv, gv, ob, loc)) continue;
-      if (ob == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return F_PRIMARY;
       s.b = ob;
       for (int d = 0; d < 3; ++d) s.loc[d] = loc[d];
       return F_COPY;
@@ -1204,6 +1211,61 @@ struct Problem {
             for (int v = 0; v < NC; ++v) s = std::max(s, std::abs(u[m.sidx(b, v, i, j, k)]));
           }
     return s;
+  }
+
+  // The regrid's transfer onto this problem's forest from another forest of
+  // the same root brick (refine!/coarsen! then the state carried over,
+  // src/refinement.jl:973-1021 with TreeAMR's transfer): every owned point of
+  // this forest takes its value as the other forest's fill would give it --
+  // a copy of the other forest's owner of the point (an unchanged leaf, or
+  // injection from a finer one), or the order-p prolongation from the leaf
+  // one level coarser that holds it -- from the other problem's work array
+  // prepared at t.
+  void transfer_from(Problem& from, const double* u_from, double t, double* u_new) {
+    const Mesh& m = mesh;
+    const Mesh& o = from.mesh;
+    if (!m.forest() || !o.forest()) throw std::runtime_error("the transfer runs between forests with levels");
+    from.prepare(u_from, t);
+    const long S = o.S;
+    for (long b = 0; b < m.nblocks(); ++b) {
+      const int L = m.level[b];
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            const long l[3] = {i, j, k};
+            long gl[3];
+            for (int d = 0; d < 3; ++d) {
+              const long tot = ((long)m.nb[d] << L) * m.N;
+              gl[d] = m.blocks[b][d] * (long)m.N + l[d];
+              if (m.periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
+            }
+            Mesh::Src sr;
+            const int c = o.classify_point(L, gl, sr);
+            for (int v = 0; v < 20; ++v) {
+              double val;
+              if (c == Mesh::F_COPY) {
+                val = from.work[o.widx(sr.b, v, sr.loc[0], sr.loc[1], sr.loc[2])];
+              } else {
+                const int n[3] = {sr.half[0] ? o.prolong : 1, sr.half[1] ? o.prolong : 1,
+                                  sr.half[2] ? o.prolong : 1};
+                const double* base = &from.work[o.widx(sr.b, v, sr.loc[0], sr.loc[1], sr.loc[2])];
+                double sz = 0.0;
+                for (int c2 = 0; c2 < n[2]; ++c2) {
+                  double sy = 0.0;
+                  for (int c1 = 0; c1 < n[1]; ++c1) {
+                    double sx = 0.0;
+                    for (int c0 = 0; c0 < n[0]; ++c0)
+                      sx += (n[0] > 1 ? o.wmid[c0] : 1.0) * base[c0 + S * (c1 + S * c2)];
+                    sy += (n[1] > 1 ? o.wmid[c1] : 1.0) * sx;
+                  }
+                  sz += (n[2] > 1 ? o.wmid[c2] : 1.0) * sy;
+                }
+                val = sz;
+              }
+              u_new[m.sidx(b, v, i, j, k)] = val;
+            }
+          }
+    }
   }
 
   // One classical RK4 step with the limiter on each stage value and on the

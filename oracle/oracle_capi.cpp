@@ -427,6 +427,15 @@ void orc_tau_blocks(void* p, const double* u, double t, double scale, double eps
   ((Problem*)p)->tau_blocks(u, t, scale, eps, out);
 }
 double orc_field_scale(void* p, const double* u, double t) { return ((Problem*)p)->field_scale(u, t); }
+// 0, or 1 when the forests admit no transfer (a jump of two levels).
+int orc_transfer(void* to, void* from, const double* u_from, double t, double* u_new) {
+  try {
+    ((Problem*)to)->transfer_from(*(Problem*)from, u_from, t, u_new);
+  } catch (const std::exception&) {
+    return 1;
+  }
+  return 0;
+}
 int orc_is_primary(void* p, long b, long i, long j, long k) {
   return ((Problem*)p)->mesh.is_primary(b, i, j, k) ? 1 : 0;
 }

@@ -60,6 +60,7 @@ def lib():
             "orc_adm_constraint": (None, [P, dp, D, dp, dp, dp]),
             "orc_tau_blocks": (None, [P, dp, D, D, D, dp]),
             "orc_field_scale": (D, [P, dp, D]),
+            "orc_transfer": (C.c_int, [P, P, dp, D, dp]),
             "orc_shape_series": (D, [dp, C.c_int, dp]),
             "orc_interior_geo": (None, [desc, D, dp, dp]),
             "orc_forest_classify": (C.c_int, [P, Lg, Lg, Lg, Lg, C.POINTER(Lg)]),
@@ -162,6 +163,13 @@ class OracleProblem:
 
     def field_scale(self, u, t):
         return lib().orc_field_scale(self.p, np.ascontiguousarray(u), float(t))
+
+    def transfer_from(self, other, u_other, t):
+        """This forest's state carried over from other's (the regrid's transfer)."""
+        out = self.zeros()
+        if lib().orc_transfer(self.p, other.p, np.ascontiguousarray(u_other), float(t), out):
+            raise RuntimeError("no transfer between these forests (a leaf two levels finer than the old forest)")
+        return out
 
     def block_level(self, b):
         return lib().orc_block_level(self.p, b)

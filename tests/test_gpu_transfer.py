@@ -1,0 +1,146 @@
+"""The regrid's state transfer on the device (tgh_transfer) against the oracle,
+and regrid end to end (refine_flags then refine!/coarsen!/balance! and the
+transfer, src/refinement.jl:738-795,973-1021), through the C ABI.
+
+Per component |GPU − oracle| / max(1, |oracle|) ≤ 1e-12: the copies and the
+interpolation sums are the oracle's own order, and the Dirichlet ghosts a
+stencil reads near a non-periodic face are the background evaluated on
+both sides.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+import tgh_desc as TD
+from test_oracle_transfer import _forests
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a visible GPU")
+    return torch
+
+
+def _problem(tgh, f, case, p):
+    dev = tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=p))
+    desc = TD.TghDesc.from_buffer_copy(dev.desc)
+    return dev, oracle_lib.OracleProblem(desc, dev.leaves(), p)
+
+
+def _noisy_state(tgh, torch, dev, t, seed):
+    u = dev.statevector()
+    tgh.initial_state(u, dev, t)
+    tgh.add_noise(u, dev, seed, 1e-3)
+    return u
+
+
+def _close(got, want):
+    err = np.abs(got - want) / np.maximum(1.0, np.abs(want))
+    return float(err.max())
+
+
+@pytest.mark.parametrize("p", [4, 6])
+def test_transfer_matches_the_oracle_on_the_gauge_wave(tgh, p):
+    """Refinement, coarsening and the identity between two periodic forests."""
+    torch = torch_cuda()
+    f0, f1 = _forests(tgh)
+    case = tgh.gauge_wave_case(eps_ko=0.0, gamma0=1.0, gamma2=0.0)
+    d0, o0 = _problem(tgh, f0, case, p)
+    d1, o1 = _problem(tgh, f1, case, p)
+    t = 0.1
+    u0 = _noisy_state(tgh, torch, d0, t, 11)
+    u1 = _noisy_state(tgh, torch, d1, t, 12)
+    h0, h1 = u0.cpu().numpy(), u1.cpu().numpy()
+    for src_d, src_o, u_src, h_src, dst_d, dst_o in ((d0, o0, u0, h0, d1, o1), (d1, o1, u1, h1, d0, o0),
+                                                     (d0, o0, u0, h0, d0, o0)):
+        got = dst_d.statevector()
+        tgh.transfer(src_d, u_src, t, dst_d, got)
+        want = dst_o.transfer_from(src_o, h_src, t)
+        assert _close(got.cpu().numpy(), want) <= TOL, (src_d.nblocks, dst_d.nblocks)
+
+
+def test_transfer_from_a_uniform_problem_to_a_hole_forest(tgh):
+    """A Kerr-Schild hole in a Dirichlet box (N = 16: the layer's width is at
+    least 2(G+1)h on the root level): the uniform problem's blocks (its
+    own order) to a two-level hole_forest, prolongation stencils reading the
+    hook's ghosts at the faces."""
+    torch = torch_cuda()
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    fu = tgh.gh_forest(None, case, N=16, roots=4)
+    fh = tgh.hole_forest(None, case, N=16, roots=4, radii=(1.5,))
+    du, ou = _problem(tgh, fu, case, 6)
+    dh, oh = _problem(tgh, fh, case, 6)
+    u = _noisy_state(tgh, torch, du, 0.0, 5)
+    got = dh.statevector()
+    tgh.transfer(du, u, 0.0, dh, got)
+    want = oh.transfer_from(ou, u.cpu().numpy(), 0.0)
+    assert _close(got.cpu().numpy(), want) <= TOL
+    # back again: injection onto the uniform blocks
+    back = du.statevector()
+    tgh.transfer(dh, got, 0.0, du, back)
+    assert _close(back.cpu().numpy(), ou.transfer_from(oh, want, 0.0)) <= TOL
+
+
+def test_transfer_refuses_other_bricks_and_two_level_jumps(tgh):
+    torch_cuda()
+    case = tgh.gauge_wave_case(eps_ko=0.0, gamma0=1.0, gamma2=0.0)
+    f0, f1 = _forests(tgh)
+    flat = tgh.gh_forest(None, case, N=8, roots=4)
+    dflat = tgh.GHProblem(flat, case, q=4)
+    d1 = tgh.GHProblem(f1, case, q=4, ops=tgh.Operators(prolongation=6))
+    with pytest.raises(tgh.ArgumentError, match="more than one level finer"):
+        tgh.transfer(dflat, dflat.statevector(), 0.0, d1, d1.statevector())
+    other = tgh.GHProblem(tgh.gh_forest(None, case, N=8, roots=2), case, q=4)
+    with pytest.raises(tgh.ArgumentError, match="same root brick"):
+        tgh.transfer(other, other.statevector(), 0.0, dflat, dflat.statevector())
+
+
+def test_regrid_refines_where_the_indicator_fires_and_carries_the_state(tgh):
+    """A Kerr-Schild hole on a uniform brick: the blocks whose Löhner maximum
+    passes refine_tol refine once, the state on the new leaves is the
+    oracle's transfer, and the new problem steps."""
+    torch = torch_cuda()
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    f = tgh.gh_forest(None, case, N=16, roots=4)
+    dev, orc = _problem(tgh, f, case, 6)
+    u = dev.statevector()
+    tgh.initial_state(u, dev, 0.0)
+    flags, tau = tgh.regrid_flags(dev, u, 0.0, refine_tol=1.0, coarsen_tol=0.0, maxlevel_cap=1)
+    thr = sorted(tau)[len(tau) * 3 // 4]
+    new, u_new, flags = tgh.regrid(dev, u, 0.0, refine_tol=thr, coarsen_tol=0.0, maxlevel_cap=1)
+    fired = [lf for lf, fl in zip(dev.leaves(), flags) if fl > 0]
+    assert 0 < len(fired) < f.nleaves()
+    for lf in fired:
+        assert lf not in new.forest.leaves
+        assert (1, 2 * lf[1], 2 * lf[2], 2 * lf[3]) in new.forest.leaves
+    assert new.forest.isbalanced()
+    desc = TD.TghDesc.from_buffer_copy(new.desc)
+    o_new = oracle_lib.OracleProblem(desc, new.forest.leaves, 6)
+    want = o_new.transfer_from(orc, u.cpu().numpy(), 0.0)
+    assert _close(u_new.cpu().numpy(), want) <= TOL
+    dt = 0.25 * new.forest.minimum_spacing()
+    tgh.gh_solve(new, u_new, (0.0, 2 * dt), dt=dt)
+    assert torch.isfinite(u_new).all()
+
+
+def test_regrid_coarsens_quiet_sibling_groups(tgh):
+    """The same hole on the two-level hole_forest with coarsen_tol above
+    every block's indicator: every complete sibling group whose parent keeps
+    2:1 balance merges, and the state is injected."""
+    torch = torch_cuda()
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    fh = tgh.hole_forest(None, case, N=16, roots=4, radii=(1.5,))
+    dev, orc = _problem(tgh, fh, case, 6)
+    u = dev.statevector()
+    tgh.initial_state(u, dev, 0.0)
+    new, u_new, flags = tgh.regrid(dev, u, 0.0, refine_tol=1e300, coarsen_tol=1e299, maxlevel_cap=1)
+    assert all(fl == (-1 if lf[0] > 0 else 0) for lf, fl in zip(dev.leaves(), flags))
+    assert new.forest.maxlevel() == 0
+    desc = TD.TghDesc.from_buffer_copy(new.desc)
+    o_new = oracle_lib.OracleProblem(desc, new.forest.leaves, 6)
+    assert _close(u_new.cpu().numpy(), o_new.transfer_from(orc, u.cpu().numpy(), 0.0)) <= TOL

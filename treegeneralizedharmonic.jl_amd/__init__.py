@@ -30,7 +30,8 @@ __all__ = [
     "gauge_wave_case", "kerr_schild_case", "harmonic_kerr_case", "gh_forest", "GHProblem", "gh_rhs", "prepare",
     "rhs", "settle", "stage_limit", "step_limit", "max_speed", "initial_state",
     "evolved_sets", "gh_dt", "gh_solve", "add_noise", "error_norms", "library_path", "GHBatch",
-    "gh_solve_batch",
+    "gh_solve_batch", "Forest", "hole_forest", "Operators", "transfer", "regrid_flags", "regrid",
+    "field_scale", "refinement_indicator",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -115,6 +116,7 @@ def _load():
         "tgh_refinement_indicator": (C.c_int, [P, C.c_void_p, C.c_double, C.c_double, C.c_double,
                                                C.POINTER(C.c_double)]),
         "tgh_field_scale": (C.c_int, [P, C.c_void_p, C.c_double, C.POINTER(C.c_double)]),
+        "tgh_transfer": (C.c_int, [P, C.c_void_p, C.c_double, P, C.c_void_p]),
         "tgh_problem_destroy": (C.c_int, [P]),
         "tgh_state_length": (C.c_int64, [P]),
         "tgh_work_length": (C.c_int64, [P]),
@@ -669,6 +671,19 @@ class Forest:
             for c in range(8):
                 self._leaves.add((l + 1, 2 * i + (c & 1), 2 * j + (c >> 1 & 1), 2 * k + (c >> 2 & 1)))
 
+    def coarsen(self, parents):
+        """coarsen!(forest, parents): each parent (level, i, j, k) whose eight
+        children are all leaves becomes a leaf again."""
+        if self._leaves is None:
+            raise ArgumentError("a uniform forest has nothing to coarsen")
+        for t in list(parents):
+            l, i, j, k = (int(x) for x in t)
+            kids = [(l + 1, 2 * i + (c & 1), 2 * j + (c >> 1 & 1), 2 * k + (c >> 2 & 1)) for c in range(8)]
+            if l < 0 or any(c not in self._leaves for c in kids):
+                raise ArgumentError(f"{tuple(t)} is not the parent of eight leaves")
+            self._leaves.difference_update(kids)
+            self._leaves.add((l, i, j, k))
+
     def _unbalanced(self):
         leafset = self._leaves or set()
         bad = set()
@@ -936,6 +951,12 @@ class GHProblem:
         _check(_load().tgh_block_coords(self._ptr, out))
         return [tuple(out[3 * b:3 * b + 3]) for b in range(self.nblocks)]
 
+    def leaves(self):
+        """The leaves (level, i, j, k) in the state's block order."""
+        if self.forest.refined:
+            return self.forest.leaves
+        return [(0,) + bc for bc in self.block_coords()]
+
     def work(self):
         import torch
         w = torch.empty(self.work_length, dtype=torch.float64, device=self.backend.device)
@@ -1076,6 +1097,72 @@ def refinement_indicator(p, u, t, *, scale, eps=0.01):
     out = (C.c_double * p.nblocks)()
     _check(_load().tgh_refinement_indicator(p.ptr, _dev(u, p, "u"), float(t), float(scale), float(eps), out))
     return list(out)
+
+
+def transfer(p_from, u_from, t, p_to, u_to):
+    """The regrid's state transfer (src/refinement.jl:973-1021, refine!/coarsen!
+    and the state carried over): u_to on p_to's leaves takes at every owned
+    point what p_from's ghost fill at t gives there -- a copy of the owner
+    (injection from a finer leaf) or the prolongation from the leaf one
+    level coarser."""
+    _check(_load().tgh_transfer(p_from.ptr, _dev(u_from, p_from, "u_from"), float(t), p_to.ptr,
+                                _dev(u_to, p_to, "u_to")))
+    return u_to
+
+
+def regrid_flags(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=None):
+    """refine_flags (src/refinement.jl:738-795) on the Löhner indicator's block
+    maxima: per leaf of p.leaves() +1 (refine: τ above refine_tol below the
+    cap), 0 (keep: τ above coarsen_tol) or -1 (coarsen, above level 0)."""
+    if not (0 <= coarsen_tol < refine_tol):
+        raise ArgumentError(f"the tolerances need 0 <= coarsen_tol < refine_tol, got "
+                            f"coarsen_tol={coarsen_tol}, refine_tol={refine_tol}")
+    if not 0 <= maxlevel_cap <= 10:
+        raise ArgumentError(f"maxlevel_cap must lie in 0 .. 10, got {maxlevel_cap}")
+    if scale is None:
+        scale = field_scale(p, u, t)
+    tau = refinement_indicator(p, u, t, scale=scale, eps=eps)
+    flags = []
+    for lf, tb in zip(p.leaves(), tau):
+        if tb > refine_tol and lf[0] < maxlevel_cap:
+            flags.append(1)
+        elif tb > coarsen_tol or lf[0] == 0:
+            flags.append(0)
+        else:
+            flags.append(-1)
+    return flags, tau
+
+
+def regrid(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=None):
+    """regrid!(forest, U; flags) with the flags of regrid_flags: flagged leaves
+    refine once, sibling groups all flagged to coarsen merge where the parent
+    stays 2:1 balanced, balance! closes the rest, and the state moves to the
+    new leaves (transfer). Returns (the new problem, its state, the flags)."""
+    flags, _ = regrid_flags(p, u, t, refine_tol=refine_tol, coarsen_tol=coarsen_tol,
+                            maxlevel_cap=maxlevel_cap, eps=eps, scale=scale)
+    old = p.forest
+    leaves = p.leaves()
+    f = Forest(old.case, old.N, old.roots, leaves)
+    f.refine([lf for lf, fl in zip(leaves, flags) if fl > 0])
+    f.balance()
+    want = {lf for lf, fl in zip(leaves, flags) if fl < 0}
+    parents = {(lf[0] - 1, lf[1] >> 1, lf[2] >> 1, lf[3] >> 1) for lf in want}
+    for par in sorted(parents):
+        l = par[0] + 1
+        kids = [(l, 2 * par[1] + (c & 1), 2 * par[2] + (c >> 1 & 1), 2 * par[3] + (c >> 2 & 1)) for c in range(8)]
+        if not all(c in want and c in f._leaves for c in kids):
+            continue
+        # the parent keeps 2:1 balance if no neighbour region of its level
+        # is refined below its children's level
+        if any(f._containing(f._leaves, l, (2 * n[0] + (c & 1), 2 * n[1] + (c >> 1 & 1), 2 * n[2] + (c >> 2 & 1)))
+               is None for n in f._neighbours(par) for c in range(8)):
+            continue
+        f.coarsen([par])
+    new = GHProblem(f, p.case, q=p.q, backend=p.backend,
+                    ops=p.ops if p.ops is not None else Operators(prolongation=p.q + 2))
+    u_new = new.statevector()
+    transfer(p, u, t, new, u_new)
+    return new, u_new, flags
 
 
 def gh_dt(p, u, *, cfl, t=0.0):

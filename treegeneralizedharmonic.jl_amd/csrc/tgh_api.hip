@@ -228,6 +228,10 @@ template <int NP>
 void launch_plan(dim3 grid, hipStream_t st, const PlanArgs& a) {
   hipLaunchKernelGGL((k_fill_plan<NP>), grid, dim3(256), 0, st, a);
 }
+template <int NP>
+void launch_transfer(dim3 grid, hipStream_t st, const TransferArgs& a) {
+  hipLaunchKernelGGL((k_transfer<NP>), grid, dim3(256), 0, st, a);
+}
 void launch_fill(tgh_problem* p, double* W, double t) {
   if (p->forest) {
     // the plan's passes, in order on the stream
@@ -773,6 +777,52 @@ int tgh_set_target(tgh_problem* p, const double* target, double t_target, int32_
     HIP_TRY(hipGraphExecDestroy(p->graph_exec));
     p->graph_exec = nullptr;
   }
+  return TGH_OK;
+}
+
+int tgh_transfer(tgh_problem* from, const double* u_from, double t, tgh_problem* to, double* u_to) {
+  if (!from || !to || !u_from || !u_to) return fail(TGH_EINVAL, "NULL argument");
+  if (!std::isfinite(t)) return fail(TGH_EINVAL, "the transfer's time must be finite");
+  const tgh_desc &a = from->desc, &b = to->desc;
+  bool same = a.N == b.N;
+  for (int d = 0; d < 3; ++d)
+    same = same && a.nblocks[d] == b.nblocks[d] && (a.periodic[d] != 0) == (b.periodic[d] != 0) &&
+           a.lo[d] == b.lo[d] && a.hi[d] == b.hi[d];
+  if (!same)
+    return fail(TGH_EINVAL, "the transfer needs two forests of the same root brick: N, nblocks, periodic, lo and hi");
+  // a uniform problem is the forest of its level-0 blocks; the interpolation
+  // order is from's, or to's where from has no levels
+  const std::vector<int> flev = from->forest ? from->level_host : std::vector<int>(from->geom.nblocks, 0);
+  const std::vector<int> tlev = to->forest ? to->level_host : std::vector<int>(to->geom.nblocks, 0);
+  const int np = from->forest ? from->prolong : to->forest ? to->prolong : 2;
+  ForestIndex ix;
+  std::string msg = ix.build(a.N, from->geom.nb, from->geom.periodic, flev, from->bcoord_host);
+  std::vector<int> ops;
+  if (msg.empty()) msg = transfer_plan(ix, from->geom.G, np, tlev, to->bcoord_host, ops);
+  if (!msg.empty()) return fail(TGH_EINVAL, msg);
+  HIP_TRY(hipStreamSynchronize(to->stream));  // the caller's work on u_to
+  void* d_ops = nullptr;
+  HIP_TRY(hipMalloc(&d_ops, sizeof(int) * ops.size()));
+  hipError_t e = hipMemcpy(d_ops, ops.data(), sizeof(int) * ops.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_scatter(from, u_from, from->W[0], 1);
+    launch_fill(from, from->W[0], t);
+    TransferArgs ta{from->W[0], u_to, (const PlanOp*)d_ops, (int)(ops.size() / 4), from->geom.S,
+                    from->geom.P * from->geom.P * from->geom.P};
+    const dim3 grid((ta.n + 255) / 256, 20);
+    switch (np) {
+      case 2: launch_transfer<2>(grid, from->stream, ta); break;
+      case 4: launch_transfer<4>(grid, from->stream, ta); break;
+      case 6: launch_transfer<6>(grid, from->stream, ta); break;
+      case 8: launch_transfer<8>(grid, from->stream, ta); break;
+      default: launch_transfer<10>(grid, from->stream, ta); break;
+    }
+    e = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(from->stream);
+    if (e == hipSuccess) e = e2;
+  }
+  (void)hipFree(d_ops);
+  if (e != hipSuccess) return fail(TGH_EDEVICE, std::string("tgh_transfer: ") + hipGetErrorString(e));
   return TGH_OK;
 }
 

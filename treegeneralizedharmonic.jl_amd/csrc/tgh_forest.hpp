@@ -35,89 +35,93 @@ struct ForestPlan {
   int dir_begin = 0, dir_end = 0;  // pass 0's Dirichlet entries (its tail)
 };
 
-// Builds the plan, or returns an error message.
-inline std::string forest_plan(int N, int G, const int nb[3], const int periodic[3], int np,
-                               const std::vector<int>& lev, const std::vector<int>& bc, ForestPlan& out) {
-  const int nl = (int)lev.size();
-  if (nl < 1) return "a forest needs at least one leaf";
-  if (N % 2 != 0 || N < 2 * G)
-    return "a forest with levels needs an even N of at least 2G = " + std::to_string(2 * G) +
-           " (a coarse vertex is a fine one, and a ghost reaches no further than the next leaf), got N = " +
-           std::to_string(N);
-  if (np < 2 || np % 2 != 0 || np / 2 > G)
-    return "the prolongation order must be even, at least 2 and at most 2G = " + std::to_string(2 * G) +
-           " so that its stencil stays inside the coarse block's ghosts, got " + std::to_string(np);
-  int Lmax = 0;
-  for (int b = 0; b < nl; ++b) {
-    if (lev[b] < 0 || lev[b] > 10) return "leaf " + std::to_string(b) + " has level " + std::to_string(lev[b]) +
-                                          ", outside 0 .. 10";
-    Lmax = lev[b] > Lmax ? lev[b] : Lmax;
-    for (int d = 0; d < 3; ++d)
-      if (bc[3 * b + d] < 0 || bc[3 * b + d] >= (nb[d] << lev[b]))
-        return "leaf " + std::to_string(b) + " lies outside the brick of its level";
-  }
-  // The cell map: the leaf of every block of the finest level.
-  long nc[3];
-  for (int d = 0; d < 3; ++d) nc[d] = (long)nb[d] << Lmax;
-  const long ncell = nc[0] * nc[1] * nc[2];
-  if (ncell > (1L << 26)) return "the forest's finest brick has too many blocks for the host's cell map";
-  std::vector<int> cell(ncell, -1);
-  for (int b = 0; b < nl; ++b) {
-    const long f = 1L << (Lmax - lev[b]);
-    const long c0[3] = {bc[3 * b] * f, bc[3 * b + 1] * f, bc[3 * b + 2] * f};
-    for (long z = c0[2]; z < c0[2] + f; ++z)
-      for (long y = c0[1]; y < c0[1] + f; ++y)
-        for (long x = c0[0]; x < c0[0] + f; ++x) {
-          int& c = cell[x + nc[0] * (y + nc[1] * z)];
-          if (c >= 0) return "leaves " + std::to_string(c) + " and " + std::to_string(b) + " overlap";
-          c = b;
-        }
-  }
-  for (long c = 0; c < ncell; ++c)
-    if (cell[c] < 0) return "the leaves do not cover the box";
-  // 2:1 balance over the 13 forward neighbour directions.
-  for (long z = 0; z < nc[2]; ++z)
-    for (long y = 0; y < nc[1]; ++y)
-      for (long x = 0; x < nc[0]; ++x) {
-        const int a = cell[x + nc[0] * (y + nc[1] * z)];
-        for (int dz = -1; dz <= 1; ++dz)
-          for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-              if (dz * 9 + dy * 3 + dx <= 0) continue;
-              long q[3] = {x + dx, y + dy, z + dz};
-              bool in = true;
-              for (int d = 0; d < 3; ++d) {
-                if (q[d] < 0 || q[d] >= nc[d]) {
-                  if (!periodic[d]) in = false;
-                  q[d] = (q[d] + nc[d]) % nc[d];
-                }
-              }
-              if (!in) continue;
-              const int o = cell[q[0] + nc[0] * (q[1] + nc[1] * q[2])];
-              if (lev[a] - lev[o] > 1 || lev[o] - lev[a] > 1)
-                return "the forest is not 2:1 balanced: leaves " + std::to_string(a) + " (level " +
-                       std::to_string(lev[a]) + ") and " + std::to_string(o) + " (level " + std::to_string(lev[o]) +
-                       ") touch";
-            }
-      }
+// A checked leaf list by position: the cell map of its finest level and the
+// rules above that name where a point takes its value from.
+struct ForestIndex {
+  int N = 0, Lmax = 0;
+  int nb[3] = {0, 0, 0}, periodic[3] = {0, 0, 0};
+  long nc[3] = {0, 0, 0};
+  std::vector<int> lev, bc, cell;
 
-  const int S = N + 1 + 2 * G, P = N + 1;
-  const long S3 = (long)S * S * S;
-  if ((long)nl * S3 >= (1L << 31)) return "the forest has too many points for the fill plan's 32-bit indices";
-  out.Lmax = Lmax;
-  out.prim.assign((size_t)nl * P * P * P, 0);
-  std::vector<std::vector<int>> pass(Lmax + 1);
-  // The leaves whose closed extent holds finest-level point F: those of the
-  // one or two finest blocks along each axis around it.
-  auto around = [&](const long F[3], int* cand) {
+  // Checks that the leaves tile the box with 2:1 balance; returns an error
+  // message or "".
+  std::string build(int N_, const int nb_[3], const int periodic_[3], const std::vector<int>& lev_,
+                    const std::vector<int>& bc_) {
+    N = N_;
+    lev = lev_;
+    bc = bc_;
+    for (int d = 0; d < 3; ++d) {
+      nb[d] = nb_[d];
+      periodic[d] = periodic_[d];
+    }
+    const int nl = (int)lev.size();
+    if (nl < 1) return "a forest needs at least one leaf";
+    Lmax = 0;
+    for (int b = 0; b < nl; ++b) {
+      if (lev[b] < 0 || lev[b] > 10) return "leaf " + std::to_string(b) + " has level " + std::to_string(lev[b]) +
+                                            ", outside 0 .. 10";
+      Lmax = lev[b] > Lmax ? lev[b] : Lmax;
+      for (int d = 0; d < 3; ++d)
+        if (bc[3 * b + d] < 0 || bc[3 * b + d] >= (nb[d] << lev[b]))
+          return "leaf " + std::to_string(b) + " lies outside the brick of its level";
+    }
+    // The cell map: the leaf of every block of the finest level.
+    for (int d = 0; d < 3; ++d) nc[d] = (long)nb[d] << Lmax;
+    const long ncell = nc[0] * nc[1] * nc[2];
+    if (ncell > (1L << 26)) return "the forest's finest brick has too many blocks for the host's cell map";
+    cell.assign(ncell, -1);
+    for (int b = 0; b < nl; ++b) {
+      const long f = 1L << (Lmax - lev[b]);
+      const long c0[3] = {bc[3 * b] * f, bc[3 * b + 1] * f, bc[3 * b + 2] * f};
+      for (long z = c0[2]; z < c0[2] + f; ++z)
+        for (long y = c0[1]; y < c0[1] + f; ++y)
+          for (long x = c0[0]; x < c0[0] + f; ++x) {
+            int& c = cell[x + nc[0] * (y + nc[1] * z)];
+            if (c >= 0) return "leaves " + std::to_string(c) + " and " + std::to_string(b) + " overlap";
+            c = b;
+          }
+    }
+    for (long c = 0; c < ncell; ++c)
+      if (cell[c] < 0) return "the leaves do not cover the box";
+    // 2:1 balance over the 13 forward neighbour directions.
+    for (long z = 0; z < nc[2]; ++z)
+      for (long y = 0; y < nc[1]; ++y)
+        for (long x = 0; x < nc[0]; ++x) {
+          const int a = cell[x + nc[0] * (y + nc[1] * z)];
+          for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+              for (int dx = -1; dx <= 1; ++dx) {
+                if (dz * 9 + dy * 3 + dx <= 0) continue;
+                long q[3] = {x + dx, y + dy, z + dz};
+                bool in = true;
+                for (int d = 0; d < 3; ++d) {
+                  if (q[d] < 0 || q[d] >= nc[d]) {
+                    if (!periodic[d]) in = false;
+                    q[d] = (q[d] + nc[d]) % nc[d];
+                  }
+                }
+                if (!in) continue;
+                const int o = cell[q[0] + nc[0] * (q[1] + nc[1] * q[2])];
+                if (lev[a] - lev[o] > 1 || lev[o] - lev[a] > 1)
+                  return "the forest is not 2:1 balanced: leaves " + std::to_string(a) + " (level " +
+                         std::to_string(lev[a]) + ") and " + std::to_string(o) + " (level " +
+                         std::to_string(lev[o]) + ") touch";
+              }
+        }
+    return "";
+  }
+
+  // The leaves whose closed extent holds point X of level Lw >= Lmax: those
+  // of the one or two finest blocks along each axis around it.
+  int around(int Lw, const long X[3], int* cand) const {
     long lo[3], n[3];
     for (int d = 0; d < 3; ++d) {
-      const long tot = nc[d] * N;
-      lo[d] = F[d] / N;
+      const long csz = (long)N << (Lw - Lmax), tot = nc[d] * csz;
+      lo[d] = X[d] / csz;
       n[d] = 1;
-      if (F[d] % N == 0) {
-        if (F[d] == tot) lo[d] = nc[d] - 1;           // the top of a non-periodic axis
-        else if (F[d] > 0 || periodic[d]) { lo[d] -= 1; n[d] = 2; }
+      if (X[d] % csz == 0) {
+        if (X[d] == tot) lo[d] = nc[d] - 1;  // the top of a non-periodic axis
+        else if (X[d] > 0 || periodic[d]) { lo[d] -= 1; n[d] = 2; }
       }
     }
     int k = 0;
@@ -131,15 +135,107 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
           if (!seen) cand[k++] = c;
         }
     return k;
-  };
-  // Local index of level-lv point gv in leaf c (across a periodic seam too).
-  auto local = [&](int c, int d, long gv) {
+  }
+
+  // Local index of point gv of leaf c's level in leaf c (across a periodic
+  // seam too).
+  long local(int c, int d, long gv) const {
     const long tot = ((long)nb[d] << lev[c]) * N;
     long loc = gv - (long)bc[3 * c + d] * N;
     if (loc < 0) loc += tot;
     if (loc > N) loc -= tot;
     return loc;
-  };
+  }
+
+  // Where point gl of level L (wrapped on periodic axes, inside the box)
+  // takes its value from: 0, the owner -- the finest leaf holding it as a
+  // vertex, then the smallest mask of axes with local index N -- and its
+  // local index in loc; 1, the order-np prolongation from the level-(L-1)
+  // leaf holding it (the smallest such mask), the stencil's first point in
+  // loc and the midpoint axes in half; -1, no leaf of level L - 1 holds it.
+  // L may exceed Lmax by one (a point of a forest refined from this one).
+  int source(int L, const long gl[3], int np, int& leaf, long loc[3], int& half) const {
+    const int Lw = L > Lmax ? L : Lmax;
+    long X[3];
+    for (int d = 0; d < 3; ++d) X[d] = gl[d] << (Lw - L);
+    int cand[8];
+    const int ncand = around(Lw, X, cand);
+    int own = -1, own_mask = 8;
+    for (int q = 0; q < ncand; ++q) {
+      const int c = cand[q];
+      const int s = Lw - lev[c];
+      if ((X[0] | X[1] | X[2]) & ((1L << s) - 1)) continue;
+      long l[3];
+      int mask = 0;
+      for (int d = 0; d < 3; ++d) {
+        l[d] = local(c, d, X[d] >> s);
+        if (l[d] == N) mask |= 1 << d;
+      }
+      if (own < 0 || lev[c] > lev[own] || (lev[c] == lev[own] && mask < own_mask)) {
+        own = c;
+        own_mask = mask;
+        for (int d = 0; d < 3; ++d) loc[d] = l[d];
+      }
+    }
+    half = 0;
+    if (own >= 0) {
+      leaf = own;
+      return 0;
+    }
+    if (L == 0) return -1;
+    for (int d = 0; d < 3; ++d) half |= (int)(gl[d] & 1) << d;
+    int cb = -1, cb_mask = 8;
+    for (int q = 0; q < ncand; ++q) {
+      const int c = cand[q];
+      if (lev[c] != L - 1) continue;
+      long st[3];
+      int mask = 0;
+      for (int d = 0; d < 3; ++d) {
+        if (half >> d & 1) {
+          st[d] = local(c, d, (gl[d] - 1) / 2) - np / 2 + 1;
+        } else {
+          st[d] = local(c, d, gl[d] / 2);
+          if (st[d] == N) mask |= 1 << d;
+        }
+      }
+      if (cb < 0 || mask < cb_mask) {
+        cb = c;
+        cb_mask = mask;
+        for (int d = 0; d < 3; ++d) loc[d] = st[d];
+      }
+    }
+    if (cb < 0) return -1;
+    leaf = cb;
+    return 1;
+  }
+};
+
+inline std::string check_forest_shape(int N, int G, int np) {
+  if (N % 2 != 0 || N < 2 * G)
+    return "a forest with levels needs an even N of at least 2G = " + std::to_string(2 * G) +
+           " (a coarse vertex is a fine one, and a ghost reaches no further than the next leaf), got N = " +
+           std::to_string(N);
+  if (np < 2 || np % 2 != 0 || np / 2 > G)
+    return "the prolongation order must be even, at least 2 and at most 2G = " + std::to_string(2 * G) +
+           " so that its stencil stays inside the coarse block's ghosts, got " + std::to_string(np);
+  return "";
+}
+
+// Builds the plan, or returns an error message.
+inline std::string forest_plan(int N, int G, const int nb[3], const int periodic[3], int np,
+                               const std::vector<int>& lev, const std::vector<int>& bc, ForestPlan& out) {
+  std::string msg = check_forest_shape(N, G, np);
+  if (!msg.empty()) return msg;
+  ForestIndex ix;
+  msg = ix.build(N, nb, periodic, lev, bc);
+  if (!msg.empty()) return msg;
+  const int nl = (int)lev.size(), Lmax = ix.Lmax;
+  const int S = N + 1 + 2 * G, P = N + 1;
+  const long S3 = (long)S * S * S;
+  if ((long)nl * S3 >= (1L << 31)) return "the forest has too many points for the fill plan's 32-bit indices";
+  out.Lmax = Lmax;
+  out.prim.assign((size_t)nl * P * P * P, 0);
+  std::vector<std::vector<int>> pass(Lmax + 1);
   for (int b = 0; b < nl; ++b) {
     const int L = lev[b];
     for (int k = -G; k <= N + G; ++k)
@@ -159,74 +255,26 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
             pass[0].insert(pass[0].end(), {dst, 0, 8, 0});
             continue;
           }
-          long F[3];
-          for (int d = 0; d < 3; ++d) F[d] = gl[d] << (Lmax - L);
-          int cand[8];
-          const int nc8 = around(F, cand);
-          // the owner: the finest candidate holding F as a vertex, then the
-          // smallest mask of axes with local index N
-          int own = -1, own_mask = 8;
-          long own_loc[3] = {0, 0, 0};
-          for (int q = 0; q < nc8; ++q) {
-            const int c = cand[q];
-            const long f = 1L << (Lmax - lev[c]);
-            if (F[0] % f || F[1] % f || F[2] % f) continue;
-            long loc[3];
-            int mask = 0;
-            for (int d = 0; d < 3; ++d) {
-              loc[d] = local(c, d, F[d] / f);
-              if (loc[d] == N) mask |= 1 << d;
-            }
-            if (own < 0 || lev[c] > lev[own] || (lev[c] == lev[own] && mask < own_mask)) {
-              own = c;
-              own_mask = mask;
-              for (int d = 0; d < 3; ++d) own_loc[d] = loc[d];
-            }
-          }
-          if (own >= 0) {
-            if (own == b && own_loc[0] == i && own_loc[1] == j && own_loc[2] == k) {
-              if (i >= 0 && i <= N && j >= 0 && j <= N && k >= 0 && k <= N)
-                out.prim[(size_t)b * P * P * P + i + P * (j + (size_t)P * k)] = 1;
-              continue;
-            }
-            const int src = (int)((long)own * S3 + (own_loc[0] + G) + (long)S * ((own_loc[1] + G) +
-                                                                                  (long)S * (own_loc[2] + G)));
-            pass[0].insert(pass[0].end(), {dst, src, 0, 0});
+          int leaf, half;
+          long loc[3];
+          const int kind = ix.source(L, gl, np, leaf, loc, half);
+          if (kind < 0)
+            return L == 0 ? std::string("a level-0 point that no leaf holds as a vertex")
+                          : "a ghost of a level-" + std::to_string(L) + " leaf lies in no leaf of level " +
+                                std::to_string(L - 1);
+          if (kind == 0 && leaf == b && loc[0] == i && loc[1] == j && loc[2] == k) {
+            if (i >= 0 && i <= N && j >= 0 && j <= N && k >= 0 && k <= N)
+              out.prim[(size_t)b * P * P * P + i + P * (j + (size_t)P * k)] = 1;
             continue;
           }
-          // a coarse midpoint: the level-(L-1) candidate, smallest mask
-          if (L == 0) return "a level-0 point that no leaf holds as a vertex";
-          int cb = -1, cb_mask = 8;
-          long start[3] = {0, 0, 0};
-          int half = 0;
-          for (int d = 0; d < 3; ++d) half |= (int)(gl[d] & 1) << d;
-          for (int q = 0; q < nc8; ++q) {
-            const int c = cand[q];
-            if (lev[c] != L - 1) continue;
-            long st[3];
-            int mask = 0;
+          if (kind == 1)
             for (int d = 0; d < 3; ++d) {
-              if (half >> d & 1) {
-                st[d] = local(c, d, (gl[d] - 1) / 2) - np / 2 + 1;
-              } else {
-                st[d] = local(c, d, gl[d] / 2);
-                if (st[d] == N) mask |= 1 << d;
-              }
+              const long hi = loc[d] + ((half >> d & 1) ? np - 1 : 0);
+              if (loc[d] < -G || hi > N + G) return "a prolongation stencil leaves its coarse block's ghosts";
             }
-            if (cb < 0 || mask < cb_mask) {
-              cb = c;
-              cb_mask = mask;
-              for (int d = 0; d < 3; ++d) start[d] = st[d];
-            }
-          }
-          if (cb < 0) return "a ghost of a level-" + std::to_string(L) + " leaf lies in no leaf of level " +
-                             std::to_string(L - 1);
-          for (int d = 0; d < 3; ++d) {
-            const long hi = start[d] + ((half >> d & 1) ? np - 1 : 0);
-            if (start[d] < -G || hi > N + G) return "a prolongation stencil leaves its coarse block's ghosts";
-          }
-          const int src = (int)((long)cb * S3 + (start[0] + G) + (long)S * ((start[1] + G) + (long)S * (start[2] + G)));
-          pass[L].insert(pass[L].end(), {dst, src, half, 0});
+          const int src = (int)((long)leaf * S3 + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G)));
+          if (kind == 0) pass[0].insert(pass[0].end(), {dst, src, 0, 0});
+          else pass[L].insert(pass[L].end(), {dst, src, half, 0});
         }
   }
   // Within a pass the entries are independent: order them by mode (stable,
@@ -242,6 +290,53 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
     out.pass_off.push_back((int)(out.ops.size() / 4));
   }
   out.dir_end = out.pass_off[1];
+  return "";
+}
+
+// The regrid's transfer (refine!/coarsen! and the state carried over,
+// src/refinement.jl:973-1021): every owned point of the new forest (leaves
+// nlev/nbc, same brick and N) takes its value as the old forest's fill would
+// give it -- a copy of the old owner (an unchanged leaf, an injection from a
+// finer one) or the order-np prolongation from the old level-(L-1) leaf --
+// read from the old forest's prepared work array (ghost width G). Entries
+// are 4 ints: dst (new block * P^3 + point), src (old block * S^3 + point),
+// mode (0 copy, else the midpoint axes), 0; sorted by mode.
+inline std::string transfer_plan(const ForestIndex& old, int G, int np, const std::vector<int>& nlev,
+                                 const std::vector<int>& nbc, std::vector<int>& ops) {
+  const int N = old.N, S = N + 1 + 2 * G, P = N + 1;
+  const long S3 = (long)S * S * S, P3 = (long)P * P * P;
+  if ((long)old.lev.size() * S3 >= (1L << 31) || (long)nlev.size() * P3 >= (1L << 31))
+    return "the forests have too many points for the transfer plan's 32-bit indices";
+  std::vector<int> by_mode[8];
+  for (size_t b = 0; b < nlev.size(); ++b) {
+    const int L = nlev[b];
+    for (int k = 0; k <= N; ++k)
+      for (int j = 0; j <= N; ++j)
+        for (int i = 0; i <= N; ++i) {
+          const int l[3] = {i, j, k};
+          long gl[3];
+          for (int d = 0; d < 3; ++d) {
+            const long tot = ((long)old.nb[d] << L) * N;
+            gl[d] = (long)nbc[3 * b + d] * N + l[d];
+            if (old.periodic[d]) gl[d] %= tot;
+          }
+          int leaf, half;
+          long loc[3];
+          const int kind = old.source(L, gl, np, leaf, loc, half);
+          if (kind < 0)
+            return "new leaf " + std::to_string(b) + " (level " + std::to_string(L) +
+                   ") is more than one level finer than the old forest where it lies";
+          for (int d = 0; d < 3; ++d) {
+            const long hi = loc[d] + ((half >> d & 1) ? np - 1 : 0);
+            if (loc[d] < -G || hi > N + G) return "a prolongation stencil leaves its coarse block's ghosts";
+          }
+          const int dst = (int)((long)b * P3 + i + (long)P * (j + (long)P * k));
+          const int src = (int)((long)leaf * S3 + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G)));
+          by_mode[half].insert(by_mode[half].end(), {dst, src, half, 0});
+        }
+  }
+  ops.clear();
+  for (int m = 0; m < 8; ++m) ops.insert(ops.end(), by_mode[m].begin(), by_mode[m].end());
   return "";
 }
 

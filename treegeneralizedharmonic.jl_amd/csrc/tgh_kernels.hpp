@@ -1244,6 +1244,21 @@ __device__ __forceinline__ double prolong_sum(const double* __restrict__ f, int 
   }
   return sz;
 }
+// A plan entry's value: a copy (mode 0) or the prolongation over the
+// midpoint axes that the mode's bits name (x 1, y 2, z 4).
+template <int NP>
+__device__ __forceinline__ double prolong_mode(int mode, const double* __restrict__ f, int S) {
+  switch (mode) {
+    case 0: return prolong_sum<1, 1, 1, NP>(f, S);
+    case 1: return prolong_sum<NP, 1, 1, NP>(f, S);
+    case 2: return prolong_sum<1, NP, 1, NP>(f, S);
+    case 3: return prolong_sum<NP, NP, 1, NP>(f, S);
+    case 4: return prolong_sum<1, 1, NP, NP>(f, S);
+    case 5: return prolong_sum<NP, 1, NP, NP>(f, S);
+    case 6: return prolong_sum<1, NP, NP, NP>(f, S);
+    default: return prolong_sum<NP, NP, NP, NP>(f, S);
+  }
+}
 // The copies and prolongations: one thread per entry and variable
 // (blockIdx.y = v); a pass's entries are sorted by mode on the host, so a
 // wave takes one branch.
@@ -1261,18 +1276,28 @@ __global__ __launch_bounds__(256) void k_fill_plan(PlanArgs a) {
   double* dst = a.W + (long)b * 20 * S3 + p;
   const int sb = o.src / (int)S3, sp = o.src - sb * (int)S3;
   const double* f = a.W + (long)sb * 20 * S3 + sp + (long)v * S3;
-  double r;
-  switch (o.mode) {
-    case 0: r = prolong_sum<1, 1, 1, NP>(f, S); break;
-    case 1: r = prolong_sum<NP, 1, 1, NP>(f, S); break;
-    case 2: r = prolong_sum<1, NP, 1, NP>(f, S); break;
-    case 3: r = prolong_sum<NP, NP, 1, NP>(f, S); break;
-    case 4: r = prolong_sum<1, 1, NP, NP>(f, S); break;
-    case 5: r = prolong_sum<NP, 1, NP, NP>(f, S); break;
-    case 6: r = prolong_sum<1, NP, NP, NP>(f, S); break;
-    default: r = prolong_sum<NP, NP, NP, NP>(f, S); break;
-  }
-  dst[v * S3] = r;
+  dst[v * S3] = prolong_mode<NP>(o.mode, f, S);
+}
+// The regrid's transfer: one thread per (entry, variable) of a transfer
+// plan, the new forest's state point from the old forest's prepared work
+// array -- a copy or a prolongation, as the old forest's fill computes it.
+struct TransferArgs {
+  const double* __restrict__ W;
+  double* __restrict__ U;
+  const PlanOp* __restrict__ ops;
+  int n, S, P3;
+};
+template <int NP>
+__global__ __launch_bounds__(256) void k_transfer(TransferArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.n) return;
+  const int v = blockIdx.y;
+  const PlanOp o = a.ops[e];
+  const long S3 = (long)a.S * a.S * a.S;
+  const int b = o.dst / a.P3, p = o.dst - b * a.P3;
+  const int sb = o.src / (int)S3, sp = o.src - sb * (int)S3;
+  const double* f = a.W + (long)sb * 20 * S3 + sp + (long)v * S3;
+  a.U[((long)b * 20 + v) * a.P3 + p] = prolong_mode<NP>(o.mode, f, a.S);
 }
 // The Dirichlet entries: one thread per entry, the exact state at the
 // ghost's own position.
