@@ -30,7 +30,7 @@ __all__ = [
     "gauge_wave_case", "kerr_schild_case", "harmonic_kerr_case", "gh_forest", "GHProblem", "gh_rhs", "prepare",
     "rhs", "settle", "stage_limit", "step_limit", "max_speed", "initial_state",
     "evolved_sets", "gh_dt", "gh_solve", "add_noise", "error_norms", "library_path", "GHBatch",
-    "gh_solve_batch", "Forest", "hole_forest", "Operators", "transfer", "regrid_flags", "regrid",
+    "gh_solve_batch", "Forest", "hole_forest", "Operators", "transfer", "regrid_flags", "regrid", "gh_solve_regrid",
     "field_scale", "refinement_indicator",
 ]
 
@@ -1137,7 +1137,8 @@ def regrid(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=No
     """regrid!(forest, U; flags) with the flags of regrid_flags: flagged leaves
     refine once, sibling groups all flagged to coarsen merge where the parent
     stays 2:1 balanced, balance! closes the rest, and the state moves to the
-    new leaves (transfer). Returns (the new problem, its state, the flags)."""
+    new leaves (transfer). Returns (the new problem, its state, the flags); a
+    regrid that leaves the mesh as it was returns p and u themselves."""
     flags, _ = regrid_flags(p, u, t, refine_tol=refine_tol, coarsen_tol=coarsen_tol,
                             maxlevel_cap=maxlevel_cap, eps=eps, scale=scale)
     old = p.forest
@@ -1158,6 +1159,8 @@ def regrid(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=No
                is None for n in f._neighbours(par) for c in range(8)):
             continue
         f.coarsen([par])
+    if set(f.leaves) == set(leaves):  # the mesh as it was: keep the problem
+        return p, u, flags
     new = GHProblem(f, p.case, q=p.q, backend=p.backend,
                     ops=p.ops if p.ops is not None else Operators(prolongation=p.q + 2))
     u_new = new.statevector()
@@ -1197,6 +1200,32 @@ def gh_solve(p, u, tspan, *, dt, graph=False):
         return u
     _check(_load().tgh_rk4(p.ptr, _dev(u, p, "u"), t0, step, nsteps, TGH_RK4_GRAPH if graph else 0))
     return u
+
+
+def gh_solve_regrid(p, u, tspan, *, dt, every, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01,
+                    graph=False):
+    """The driver's regrid cadence (src/driver.jl:1640-1700): gh_solve over
+    chunks of `every` steps of the one step size gh_solve would take over
+    tspan, each chunk but the last followed by regrid at its end time (the
+    hook's t). Returns (the final problem, its state, [(t, nblocks)] per
+    regrid)."""
+    if every < 1:
+        raise ArgumentError(f"the regrid cadence is a positive number of steps, got {every}")
+    t0, t1 = float(tspan[0]), float(tspan[1])
+    nsteps = max(1, math.ceil((t1 - t0) / dt * (1 - 4 * 2.0 ** -52)))
+    step = (t1 - t0) / nsteps
+    done, log = 0, []
+    while done < nsteps:
+        k = min(every, nsteps - done)
+        t = t0 + done * step
+        _check(_load().tgh_rk4(p.ptr, _dev(u, p, "u"), t, step, k, TGH_RK4_GRAPH if graph else 0))
+        done += k
+        if done < nsteps:
+            t = t0 + done * step
+            p, u, _ = regrid(p, u, t, refine_tol=refine_tol, coarsen_tol=coarsen_tol,
+                             maxlevel_cap=maxlevel_cap, eps=eps)
+            log.append((t, p.nblocks))
+    return p, u, log
 
 
 class GHBatch:

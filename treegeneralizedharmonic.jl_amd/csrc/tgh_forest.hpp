@@ -21,11 +21,29 @@
 // corners, across periodic seams) differ by at most one level, N is even and
 // at least 2G, and the prolongation order is even and fits the ghosts.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace tgh {
+
+// Runs fn(chunk, b0, b1) over contiguous block ranges [b0, b1) on up to 16
+// host threads; chunk c covers the c-th range, so per-chunk outputs joined in
+// chunk order are the serial loop's.
+template <class F>
+inline int for_block_chunks(int nl, F fn) {
+  unsigned hw = std::thread::hardware_concurrency();
+  const int nt = (int)std::max(1u, std::min(16u, hw ? hw : 1u));
+  const int nchunk = std::max(1, std::min(nt, nl / 8));
+  std::vector<std::thread> pool;
+  for (int c = 1; c < nchunk; ++c)
+    pool.emplace_back(fn, c, (int)((long)nl * c / nchunk), (int)((long)nl * (c + 1) / nchunk));
+  fn(0, 0, (int)((long)nl / nchunk));
+  for (auto& t : pool) t.join();
+  return nchunk;
+}
 
 struct ForestPlan {
   std::vector<int> ops;            // 4 ints per entry: dst, src, mode, 0
@@ -235,59 +253,96 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
   if ((long)nl * S3 >= (1L << 31)) return "the forest has too many points for the fill plan's 32-bit indices";
   out.Lmax = Lmax;
   out.prim.assign((size_t)nl * P * P * P, 0);
-  std::vector<std::vector<int>> pass(Lmax + 1);
-  for (int b = 0; b < nl; ++b) {
-    const int L = lev[b];
-    for (int k = -G; k <= N + G; ++k)
-      for (int j = -G; j <= N + G; ++j)
-        for (int i = -G; i <= N + G; ++i) {
-          const int l[3] = {i, j, k};
-          const int dst = (int)((long)b * S3 + (i + G) + (long)S * ((j + G) + (long)S * (k + G)));
-          long gl[3];
-          bool outside = false;
-          for (int d = 0; d < 3; ++d) {
-            const long tot = ((long)nb[d] << L) * N;
-            gl[d] = (long)bc[3 * b + d] * N + l[d];
-            if (periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
-            else if (gl[d] < 0 || gl[d] > tot) outside = true;
-          }
-          if (outside) {
-            pass[0].insert(pass[0].end(), {dst, 0, 8, 0});
-            continue;
-          }
-          int leaf, half;
-          long loc[3];
-          const int kind = ix.source(L, gl, np, leaf, loc, half);
-          if (kind < 0)
-            return L == 0 ? std::string("a level-0 point that no leaf holds as a vertex")
-                          : "a ghost of a level-" + std::to_string(L) + " leaf lies in no leaf of level " +
-                                std::to_string(L - 1);
-          if (kind == 0 && leaf == b && loc[0] == i && loc[1] == j && loc[2] == k) {
-            if (i >= 0 && i <= N && j >= 0 && j <= N && k >= 0 && k <= N)
+  // per chunk of blocks: its passes' entries and its first error
+  std::vector<std::vector<std::vector<int>>> cpass(16, std::vector<std::vector<int>>(Lmax + 1));
+  std::vector<std::string> cerr(16);
+  const int nchunk = for_block_chunks(nl, [&](int c, int b0, int b1) {
+    std::vector<std::vector<int>>& pass = cpass[c];
+    for (int b = b0; b < b1; ++b) {
+      const int L = lev[b];
+      for (int k = -G; k <= N + G; ++k)
+        for (int j = -G; j <= N + G; ++j)
+          for (int i = -G; i <= N + G; ++i) {
+            const int dst = (int)((long)b * S3 + (i + G) + (long)S * ((j + G) + (long)S * (k + G)));
+            // a point inside the block's extent lies in no other leaf: its primary
+            if (i > 0 && i < N && j > 0 && j < N && k > 0 && k < N) {
               out.prim[(size_t)b * P * P * P + i + P * (j + (size_t)P * k)] = 1;
-            continue;
-          }
-          if (kind == 1)
-            for (int d = 0; d < 3; ++d) {
-              const long hi = loc[d] + ((half >> d & 1) ? np - 1 : 0);
-              if (loc[d] < -G || hi > N + G) return "a prolongation stencil leaves its coarse block's ghosts";
+              continue;
             }
-          const int src = (int)((long)leaf * S3 + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G)));
-          if (kind == 0) pass[0].insert(pass[0].end(), {dst, src, 0, 0});
-          else pass[L].insert(pass[L].end(), {dst, src, half, 0});
-        }
-  }
-  // Within a pass the entries are independent: order them by mode (stable,
-  // so each mode's entries stay in destination order) for wave-uniform branches.
-  out.ops.clear();
-  out.pass_off.assign(1, 0);
-  for (int l = 0; l <= Lmax; ++l) {
-    for (int mode = 0; mode <= 8; ++mode) {
-      if (l == 0 && mode == 8) out.dir_begin = (int)(out.ops.size() / 4);
-      for (size_t e = 0; e < pass[l].size(); e += 4)
-        if (pass[l][e + 2] == mode) out.ops.insert(out.ops.end(), pass[l].begin() + e, pass[l].begin() + e + 4);
+            const int l[3] = {i, j, k};
+            long gl[3];
+            bool outside = false;
+            for (int d = 0; d < 3; ++d) {
+              const long tot = ((long)nb[d] << L) * N;
+              gl[d] = (long)bc[3 * b + d] * N + l[d];
+              if (periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
+              else if (gl[d] < 0 || gl[d] > tot) outside = true;
+            }
+            if (outside) {
+              pass[0].insert(pass[0].end(), {dst, 0, 8, 0});
+              continue;
+            }
+            int leaf, half;
+            long loc[3];
+            const int kind = ix.source(L, gl, np, leaf, loc, half);
+            if (kind < 0) {
+              cerr[c] = L == 0 ? std::string("a level-0 point that no leaf holds as a vertex")
+                               : "a ghost of a level-" + std::to_string(L) + " leaf lies in no leaf of level " +
+                                     std::to_string(L - 1);
+              return;
+            }
+            if (kind == 0 && leaf == b && loc[0] == i && loc[1] == j && loc[2] == k) {
+              if (i >= 0 && i <= N && j >= 0 && j <= N && k >= 0 && k <= N)
+                out.prim[(size_t)b * P * P * P + i + P * (j + (size_t)P * k)] = 1;
+              continue;
+            }
+            if (kind == 1)
+              for (int d = 0; d < 3; ++d) {
+                const long hi = loc[d] + ((half >> d & 1) ? np - 1 : 0);
+                if (loc[d] < -G || hi > N + G) {
+                  cerr[c] = "a prolongation stencil leaves its coarse block's ghosts";
+                  return;
+                }
+              }
+            const int src =
+                (int)((long)leaf * S3 + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G)));
+            if (kind == 0) pass[0].insert(pass[0].end(), {dst, src, 0, 0});
+            else pass[L].insert(pass[L].end(), {dst, src, half, 0});
+          }
     }
-    out.pass_off.push_back((int)(out.ops.size() / 4));
+  });
+  for (int c = 0; c < nchunk; ++c)
+    if (!cerr[c].empty()) return cerr[c];
+  // Within a pass the entries are independent: order them by mode (stable,
+  // so each mode's entries stay in destination order) for wave-uniform
+  // branches -- a counting sort over the chunks in block order.
+  size_t total = 0;
+  for (int c = 0; c < nchunk; ++c)
+    for (int l = 0; l <= Lmax; ++l) total += cpass[c][l].size();
+  out.ops.assign(total, 0);
+  out.pass_off.assign(1, 0);
+  size_t pos = 0;
+  for (int l = 0; l <= Lmax; ++l) {
+    size_t off[9] = {0};
+    for (int c = 0; c < nchunk; ++c)
+      for (size_t e = 0; e < cpass[c][l].size(); e += 4) off[cpass[c][l][e + 2]] += 4;
+    size_t run = pos;
+    for (int m = 0; m <= 8; ++m) {
+      const size_t n = off[m];
+      off[m] = run;
+      run += n;
+    }
+    if (l == 0) out.dir_begin = (int)(off[8] / 4);
+    for (int c = 0; c < nchunk; ++c) {
+      const std::vector<int>& v = cpass[c][l];
+      for (size_t e = 0; e < v.size(); e += 4) {
+        size_t& o = off[v[e + 2]];
+        for (int q = 0; q < 4; ++q) out.ops[o + q] = v[e + q];
+        o += 4;
+      }
+    }
+    pos = run;
+    out.pass_off.push_back((int)(pos / 4));
   }
   out.dir_end = out.pass_off[1];
   return "";
@@ -305,38 +360,68 @@ inline std::string transfer_plan(const ForestIndex& old, int G, int np, const st
                                  const std::vector<int>& nbc, std::vector<int>& ops) {
   const int N = old.N, S = N + 1 + 2 * G, P = N + 1;
   const long S3 = (long)S * S * S, P3 = (long)P * P * P;
-  if ((long)old.lev.size() * S3 >= (1L << 31) || (long)nlev.size() * P3 >= (1L << 31))
+  const int nl = (int)nlev.size();
+  if ((long)old.lev.size() * S3 >= (1L << 31) || (long)nl * P3 >= (1L << 31))
     return "the forests have too many points for the transfer plan's 32-bit indices";
-  std::vector<int> by_mode[8];
-  for (size_t b = 0; b < nlev.size(); ++b) {
-    const int L = nlev[b];
-    for (int k = 0; k <= N; ++k)
-      for (int j = 0; j <= N; ++j)
-        for (int i = 0; i <= N; ++i) {
-          const int l[3] = {i, j, k};
-          long gl[3];
-          for (int d = 0; d < 3; ++d) {
-            const long tot = ((long)old.nb[d] << L) * N;
-            gl[d] = (long)nbc[3 * b + d] * N + l[d];
-            if (old.periodic[d]) gl[d] %= tot;
+  std::vector<std::vector<int>> cmode(16 * 8);
+  std::vector<std::string> cerr(16);
+  const int nchunk = for_block_chunks(nl, [&](int c, int b0, int b1) {
+    std::vector<int>* by_mode = &cmode[8 * c];
+    for (int b = b0; b < b1; ++b) {
+      const int L = nlev[b];
+      // a leaf of both forests: after the old fill its owned points hold their
+      // owners' values, so they copy across in place
+      int same = -1;
+      if (L <= old.Lmax) {
+        const int f = old.Lmax - L;
+        const long x = (long)nbc[3 * b] << f, y = (long)nbc[3 * b + 1] << f, z = (long)nbc[3 * b + 2] << f;
+        const int o = old.cell[x + old.nc[0] * (y + old.nc[1] * z)];
+        if (old.lev[o] == L && old.bc[3 * o] == nbc[3 * b] && old.bc[3 * o + 1] == nbc[3 * b + 1] &&
+            old.bc[3 * o + 2] == nbc[3 * b + 2])
+          same = o;
+      }
+      for (int k = 0; k <= N; ++k)
+        for (int j = 0; j <= N; ++j)
+          for (int i = 0; i <= N; ++i) {
+            const int dst = (int)((long)b * P3 + i + (long)P * (j + (long)P * k));
+            if (same >= 0) {
+              const int src = (int)((long)same * S3 + (i + G) + (long)S * ((j + G) + (long)S * (k + G)));
+              by_mode[0].insert(by_mode[0].end(), {dst, src, 0, 0});
+              continue;
+            }
+            const int l[3] = {i, j, k};
+            long gl[3];
+            for (int d = 0; d < 3; ++d) {
+              const long tot = ((long)old.nb[d] << L) * N;
+              gl[d] = (long)nbc[3 * b + d] * N + l[d];
+              if (old.periodic[d]) gl[d] %= tot;
+            }
+            int leaf, half;
+            long loc[3];
+            const int kind = old.source(L, gl, np, leaf, loc, half);
+            if (kind < 0) {
+              cerr[c] = "new leaf " + std::to_string(b) + " (level " + std::to_string(L) +
+                        ") is more than one level finer than the old forest where it lies";
+              return;
+            }
+            for (int d = 0; d < 3; ++d) {
+              const long hi = loc[d] + ((half >> d & 1) ? np - 1 : 0);
+              if (loc[d] < -G || hi > N + G) {
+                cerr[c] = "a prolongation stencil leaves its coarse block's ghosts";
+                return;
+              }
+            }
+            const int src =
+                (int)((long)leaf * S3 + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G)));
+            by_mode[half].insert(by_mode[half].end(), {dst, src, half, 0});
           }
-          int leaf, half;
-          long loc[3];
-          const int kind = old.source(L, gl, np, leaf, loc, half);
-          if (kind < 0)
-            return "new leaf " + std::to_string(b) + " (level " + std::to_string(L) +
-                   ") is more than one level finer than the old forest where it lies";
-          for (int d = 0; d < 3; ++d) {
-            const long hi = loc[d] + ((half >> d & 1) ? np - 1 : 0);
-            if (loc[d] < -G || hi > N + G) return "a prolongation stencil leaves its coarse block's ghosts";
-          }
-          const int dst = (int)((long)b * P3 + i + (long)P * (j + (long)P * k));
-          const int src = (int)((long)leaf * S3 + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G)));
-          by_mode[half].insert(by_mode[half].end(), {dst, src, half, 0});
-        }
-  }
+    }
+  });
+  for (int c = 0; c < nchunk; ++c)
+    if (!cerr[c].empty()) return cerr[c];
   ops.clear();
-  for (int m = 0; m < 8; ++m) ops.insert(ops.end(), by_mode[m].begin(), by_mode[m].end());
+  for (int m = 0; m < 8; ++m)
+    for (int c = 0; c < nchunk; ++c) ops.insert(ops.end(), cmode[8 * c + m].begin(), cmode[8 * c + m].end());
   return "";
 }
 
