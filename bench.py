@@ -230,6 +230,22 @@ def forest_bench(tgh, torch, steps):
     torch.cuda.synchronize()
     per_block = points / (time.perf_counter() - t0)
     assert bool(torch.isfinite(u).all()), "the forest's state is not finite"
+    # one regrid: the leaves in the indicator's upper sixteenth refine once
+    # (marking, the new problem with its fill plan, the transfer), and the
+    # transfer alone (its plan and kernel) onto the new forest again
+    _, tau = tgh.regrid_flags(p, u, dt, refine_tol=1.0, coarsen_tol=0.0, maxlevel_cap=3)
+    thr = sorted(tau)[len(tau) * 15 // 16]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    q, uq, _ = tgh.regrid(p, u, dt, refine_tol=thr, coarsen_tol=0.0, maxlevel_cap=3)
+    torch.cuda.synchronize()
+    regrid_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    tgh.transfer(p, u, dt, q, uq)
+    torch.cuda.synchronize()
+    transfer_ms = (time.perf_counter() - t0) * 1e3
+    regrid_leaves = q.nblocks
+    del q, uq
     levels = [sum(1 for lf in forest.leaves if lf[0] == k) for k in range(forest.maxlevel() + 1)]
     return {"workload": "static Kerr-Schild hole M=1 a=0.7 (closed-form gauge source, damped interior) on "
                         "hole_forest(N=16, roots=8 over [-5M,5M]^3, radii=(3M, 1.5M)): three levels, finest "
@@ -237,7 +253,8 @@ def forest_bench(tgh, torch, steps):
             "leaves": forest.nleaves(), "leaves_per_level": levels, "points": points, "steps": steps,
             "graph": graph, "per_block": per_block, "graph_over_per_block": graph / per_block,
             "rhs_ms": rhs_ms + pi_ms, "fill_ms": fill_ms,
-            "fill_fraction": fill_ms / (fill_ms + rhs_ms + pi_ms)}
+            "fill_fraction": fill_ms / (fill_ms + rhs_ms + pi_ms),
+            "regrid_ms": regrid_ms, "regrid_transfer_ms": transfer_ms, "regrid_leaves": regrid_leaves}
 
 
 def octree_batch_bench(tgh, torch, steps):
