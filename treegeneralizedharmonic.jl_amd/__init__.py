@@ -1139,6 +1139,10 @@ def regrid(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=No
     stays 2:1 balanced, balance! closes the rest, and the state moves to the
     new leaves (transfer). Returns (the new problem, its state, the flags); a
     regrid that leaves the mesh as it was returns p and u themselves."""
+    if p.target is not None:
+        raise ArgumentError("a :fitted problem relaxes toward a target cache that the fit computes on its "
+                            "own leaves; after a regrid the fit must fill the new mesh's cache, and the "
+                            "fit is not built here: regrid a problem without a target")
     flags, _ = regrid_flags(p, u, t, refine_tol=refine_tol, coarsen_tol=coarsen_tol,
                             maxlevel_cap=maxlevel_cap, eps=eps, scale=scale)
     old = p.forest

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -571,7 +572,12 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
   if (rc) return rc;
   if (leaves) {
     const int per[3] = {d->periodic[0] ? 1 : 0, d->periodic[1] ? 1 : 0, d->periodic[2] ? 1 : 0};
-    const std::string msg = forest_plan(d->N, d->q / 2 + 1, d->nblocks, per, prolong, lev, fbc, plan);
+    std::string msg;
+    try {
+      msg = forest_plan(d->N, d->q / 2 + 1, d->nblocks, per, prolong, lev, fbc, plan);
+    } catch (const std::exception& ex) {  // host threads or memory
+      return fail(TGH_EDEVICE, std::string("building the fill plan: ") + ex.what());
+    }
     if (!msg.empty()) return fail(TGH_EINVAL, msg);
   }
   auto* p = new tgh_problem();
@@ -796,9 +802,14 @@ int tgh_transfer(tgh_problem* from, const double* u_from, double t, tgh_problem*
   const std::vector<int> tlev = to->forest ? to->level_host : std::vector<int>(to->geom.nblocks, 0);
   const int np = from->forest ? from->prolong : to->forest ? to->prolong : 2;
   ForestIndex ix;
-  std::string msg = ix.build(a.N, from->geom.nb, from->geom.periodic, flev, from->bcoord_host);
+  std::string msg;
   std::vector<int> ops;
-  if (msg.empty()) msg = transfer_plan(ix, from->geom.G, np, tlev, to->bcoord_host, ops);
+  try {
+    msg = ix.build(a.N, from->geom.nb, from->geom.periodic, flev, from->bcoord_host);
+    if (msg.empty()) msg = transfer_plan(ix, from->geom.G, np, tlev, to->bcoord_host, ops);
+  } catch (const std::exception& ex) {  // host threads or memory
+    return fail(TGH_EDEVICE, std::string("building the transfer plan: ") + ex.what());
+  }
   if (!msg.empty()) return fail(TGH_EINVAL, msg);
   HIP_TRY(hipStreamSynchronize(to->stream));  // the caller's work on u_to
   void* d_ops = nullptr;
