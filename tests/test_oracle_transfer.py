@@ -122,3 +122,40 @@ def test_forest_coarsen_undoes_refine(tgh):
     assert sorted(g.leaves) == sorted(f0.leaves)
     with pytest.raises(tgh.ArgumentError):
         g.coarsen([(1, 2, 2, 2)])
+
+
+def test_regrid_forest_refines_marked_leaves_and_balances(tgh):
+    f0, _ = _forests(tgh)
+    leaves = f0.leaves
+    flags = [1 if lf == (1, 2, 2, 2) else 0 for lf in leaves]
+    g = tgh.regrid_forest(f0, leaves, flags)
+    assert g.isbalanced() and g.maxlevel() == 2
+    assert all((2, 4 + (c & 1), 4 + (c >> 1 & 1), 4 + (c >> 2 & 1)) in g.leaves for c in range(8))
+    with pytest.raises(tgh.ArgumentError):
+        tgh.regrid_forest(f0, leaves, flags[:-1])
+
+
+def test_regrid_forest_coarsens_only_where_the_parent_stays_balanced(tgh):
+    """Every level-1 leaf of the three-level forest marked to coarsen: each
+    complete group there touches the level-2 corner (balance! made them), so
+    none merges -- merging any one unbalances the forest -- while on the
+    two-level forest every group merges."""
+    _, f1 = _forests(tgh)
+    leaves = f1.leaves
+    g = tgh.regrid_forest(f1, leaves, [-1 if lf[0] == 1 else 0 for lf in leaves])
+    assert g.isbalanced()
+    assert {lf for lf in leaves if lf[0] == 2} <= set(g.leaves)
+    assert sorted(g.leaves) == sorted(leaves)
+    kept = {(lf[0] - 1, lf[1] >> 1, lf[2] >> 1, lf[3] >> 1) for lf in g.leaves if lf[0] == 1}
+    assert kept
+    for par in kept:
+        kids = [(1, 2 * par[1] + (c & 1), 2 * par[2] + (c >> 1 & 1), 2 * par[3] + (c >> 2 & 1)) for c in range(8)]
+        if not all(k in g.leaves for k in kids):
+            continue
+        h = tgh.Forest(g.case, g.N, g.roots, g.leaves)
+        h.coarsen([par])
+        assert not h.isbalanced(), par
+    # the whole coarse level goes where nothing finer is marked to stay
+    f0, _ = _forests(tgh)
+    u = tgh.regrid_forest(f0, f0.leaves, [-1 if lf[0] == 1 else 0 for lf in f0.leaves])
+    assert u.maxlevel() == 0 and u.nleaves() == ROOTS ** 3

@@ -30,7 +30,7 @@ __all__ = [
     "gauge_wave_case", "kerr_schild_case", "harmonic_kerr_case", "gh_forest", "GHProblem", "gh_rhs", "prepare",
     "rhs", "settle", "stage_limit", "step_limit", "max_speed", "initial_state",
     "evolved_sets", "gh_dt", "gh_solve", "add_noise", "error_norms", "library_path", "GHBatch",
-    "gh_solve_batch", "Forest", "hole_forest", "Operators", "transfer", "regrid_flags", "regrid", "gh_solve_regrid",
+    "gh_solve_batch", "Forest", "hole_forest", "Operators", "transfer", "regrid_flags", "regrid_forest", "regrid", "gh_solve_regrid",
     "field_scale", "refinement_indicator",
 ]
 
@@ -1133,6 +1133,32 @@ def regrid_flags(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, sc
     return flags, tau
 
 
+def regrid_forest(forest, leaves, flags):
+    """The new mesh of regrid! (src/refinement.jl:973-1021 with TreeAMR's
+    refine!/coarsen!/balance!) from per-leaf flags (+1 refine, 0 keep, -1
+    coarsen) over `leaves`: the marked leaves refine once, balance! closes
+    2:1, and each sibling group whose eight leaves are all marked to coarsen
+    merges where no neighbour region of the parent's level is refined below
+    the children's level (so the parent stays balanced)."""
+    if len(flags) != len(leaves):
+        raise ArgumentError(f"one flag per leaf: {len(flags)} flags for {len(leaves)} leaves")
+    f = Forest(forest.case, forest.N, forest.roots, leaves)
+    f.refine([lf for lf, fl in zip(leaves, flags) if fl > 0])
+    f.balance()
+    want = {lf for lf, fl in zip(leaves, flags) if fl < 0}
+    parents = {(lf[0] - 1, lf[1] >> 1, lf[2] >> 1, lf[3] >> 1) for lf in want if lf[0] > 0}
+    for par in sorted(parents):
+        l = par[0] + 1
+        kids = [(l, 2 * par[1] + (c & 1), 2 * par[2] + (c >> 1 & 1), 2 * par[3] + (c >> 2 & 1)) for c in range(8)]
+        if not all(c in want and c in f._leaves for c in kids):
+            continue
+        if any(f._containing(f._leaves, l, (2 * n[0] + (c & 1), 2 * n[1] + (c >> 1 & 1), 2 * n[2] + (c >> 2 & 1)))
+               is None for n in f._neighbours(par) for c in range(8)):
+            continue
+        f.coarsen([par])
+    return f
+
+
 def regrid(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=None):
     """regrid!(forest, U; flags) with the flags of regrid_flags: flagged leaves
     refine once, sibling groups all flagged to coarsen merge where the parent
@@ -1145,24 +1171,8 @@ def regrid(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=No
                             "fit is not built here: regrid a problem without a target")
     flags, _ = regrid_flags(p, u, t, refine_tol=refine_tol, coarsen_tol=coarsen_tol,
                             maxlevel_cap=maxlevel_cap, eps=eps, scale=scale)
-    old = p.forest
     leaves = p.leaves()
-    f = Forest(old.case, old.N, old.roots, leaves)
-    f.refine([lf for lf, fl in zip(leaves, flags) if fl > 0])
-    f.balance()
-    want = {lf for lf, fl in zip(leaves, flags) if fl < 0}
-    parents = {(lf[0] - 1, lf[1] >> 1, lf[2] >> 1, lf[3] >> 1) for lf in want}
-    for par in sorted(parents):
-        l = par[0] + 1
-        kids = [(l, 2 * par[1] + (c & 1), 2 * par[2] + (c >> 1 & 1), 2 * par[3] + (c >> 2 & 1)) for c in range(8)]
-        if not all(c in want and c in f._leaves for c in kids):
-            continue
-        # the parent keeps 2:1 balance if no neighbour region of its level
-        # is refined below its children's level
-        if any(f._containing(f._leaves, l, (2 * n[0] + (c & 1), 2 * n[1] + (c >> 1 & 1), 2 * n[2] + (c >> 2 & 1)))
-               is None for n in f._neighbours(par) for c in range(8)):
-            continue
-        f.coarsen([par])
+    f = regrid_forest(p.forest, leaves, flags)
     if set(f.leaves) == set(leaves):  # the mesh as it was: keep the problem
         return p, u, flags
     new = GHProblem(f, p.case, q=p.q, backend=p.backend,
