@@ -230,14 +230,17 @@ def forest_bench(tgh, torch, steps):
     torch.cuda.synchronize()
     per_block = points / (time.perf_counter() - t0)
     assert bool(torch.isfinite(u).all()), "the forest's state is not finite"
-    # one regrid: the leaves in the indicator's upper sixteenth refine once
+    # one regrid: the leaves in the indicator's upper sixteenth refine once,
+    # the level floor holds the horizon shell, nothing quiet coarsens
     # (marking, the new problem with its fill plan, the transfer), and the
     # transfer alone (its plan and kernel) onto the new forest again
-    _, tau = tgh.regrid_flags(p, u, dt, refine_tol=1.0, coarsen_tol=0.0, maxlevel_cap=3)
+    tau = tgh.refinement_indicator(p, u, dt, scale=tgh.field_scale(p, u, dt), eps=0.01)
     thr = sorted(tau)[len(tau) * 15 // 16]
+    ref = tgh.Refinement(refine_tol=thr, coarsen_tol=min(t for t in tau if t > 0) / 2, maxlevel_cap=3,
+                         floor_margin=1.0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    q, uq, _ = tgh.regrid(p, u, dt, refine_tol=thr, coarsen_tol=0.0, maxlevel_cap=3)
+    q, uq, _ = tgh.regrid(p, u, dt, refinement=ref)
     torch.cuda.synchronize()
     regrid_ms = (time.perf_counter() - t0) * 1e3
     t0 = time.perf_counter()

@@ -61,6 +61,16 @@ enum { /* the Kreiss–Oliger amplitude (src/gauge.jl:1172-1264) */
   TGH_DISSIPATION_CONSTANT = 0,
   TGH_DISSIPATION_HORIZON = 1
 };
+enum { /* which block holds a point on a block face as its own (tgh_desc.ownership) */
+  TGH_OWNERSHIP_CLOSED = 0,   /* TreeAMR 0.3, the reference's mesh today: a block owns both face
+                                 planes; a shared point's primary is the finest leaf holding it as
+                                 a vertex, and both outer planes of a Dirichlet box are evolved */
+  TGH_OWNERSHIP_HALF_OPEN = 1 /* TreeAMR 0.1: a block owns [0, N) per axis; a point belongs to the
+                                 leaf whose half-open extent holds it, and the upper outer plane of
+                                 a Dirichlet axis is a ghost the hook fills. Forests only
+                                 (tgh_problem_create_forest); the mesh the reference's published
+                                 frozen-hierarchy tables and driver records were measured on */
+};
 enum { /* tgh_rk4 flags */
   TGH_RK4_GRAPH = 1,  /* capture the step in a hipGraph and replay it */
   TGH_RK4_FUSED = 2,  /* one kernel per right-hand side */
@@ -124,6 +134,7 @@ typedef struct tgh_desc {
   int32_t shape_lmax;
   double shape[(TGH_SHAPE_MAX_LMAX + 1) * (TGH_SHAPE_MAX_LMAX + 1)];
   double r_in, r_out, offset, thickness;
+  int32_t ownership;    /* TGH_OWNERSHIP_*; 0 (closed blocks) when zero-initialised */
 } tgh_desc;
 
 /* Matter's 3+1 projections at every owned point (src/evolution.jl:1295-1311,
@@ -182,6 +193,11 @@ int tgh_transfer(tgh_problem* from, const double* u_from, double t, tgh_problem*
  * src/evolution.jl:1256-1272). The library keeps the pointer (the caller owns
  * the array and may refill it between calls); NULL detaches it. */
 int tgh_set_target(tgh_problem* p, const double* target, double t_target, int32_t rate, double trail);
+/* with_interior(p, with_rho_max(p.interior, rho_max)) — src/evolution.jl
+ * with_interior, src/interior.jl with_ρ_max; what evolve!'s chunk loop does
+ * per chunk (chunk_interior, src/driver.jl:2391-2411): the layer's
+ * relaxation rate, the rest of the problem unchanged. */
+int tgh_set_rho_max(tgh_problem* p, double rho_max);
 /* No reference counterpart: releases the problem's device memory. */
 int tgh_problem_destroy(tgh_problem* p);
 
@@ -299,9 +315,41 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out);
  * u[i] is problem i's state (in place). flags: 0 or TGH_RK4_GRAPH. */
 int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t nsteps, int32_t flags);
 int tgh_batch_destroy(tgh_batch* b);
-/* gh_error! + error_norms — src/constraints.jl:311-396,629-671: masked
- * volume-weighted L2 and Linf of u - u_exact(t) over evolved primaries. */
+/* gh_error! + error_norms — src/constraints.jl:311-396,629-671: at every
+ * primary the error e = |u - u_exact(t)|, the 2-norm over the 20 variables;
+ * the masked volume-weighted L2 of e and its L-inf over evolved primaries. */
 int tgh_error_norms(tgh_problem* p, const double* u, double t, double* l2, double* linf);
+/* gh_error!(p, u, t; shell = (shell_lo, shell_hi)) + error_norms —
+ * src/constraints.jl:311-396,629-671: out[4] (host) = err_l2, err_linf (as
+ * tgh_error_norms), residual (the layer's L-inf of e at r_0 <= r < r_1; for
+ * :fitted the distance from the target cache A + (t - t_f) S) and drift (the
+ * L-inf of |h_tt - h_tt,exact| at evolved primaries with shell_lo <= r <=
+ * shell_hi; 0 for the default empty band shell_lo = 0 > shell_hi = -1). */
+int tgh_error_rows(tgh_problem* p, const double* u, double t, double shell_lo, double shell_hi, double* out);
+/* The adjoint. The reference differentiates nothing; these entry points give
+ * the vector-Jacobian product of its right-hand side (gh_rhs!,
+ * src/evolution.jl:1327) and the gradient of its masked L2 error
+ * (gh_error! + error_norms, src/constraints.jl:311-396,629-671) after RK4
+ * (src/stepping.jl:208-239) with respect to the parameters
+ *   theta = (gamma0_near, gamma0_far, gamma2, rho_max, rho_ramp)
+ * of GaussianDamping (src/gauge.jl:1094-1156; 0 for ConstantDamping) and of
+ * the interior layer (src/interior.jl:178-238). Built for a uniform forest
+ * with a hole at rest (or none), the :damped or :frozen sphere, no range
+ * bounds, a static gauge-source kind; TGH_EUNSUPPORTED otherwise. */
+/* u_bar = (dF/du)^T du_bar and theta_bar = (dF/dtheta)^T du_bar at (u, t),
+ * F = gh_rhs!(., u, p, t): u, du_bar and u_bar are state vectors (device);
+ * du_bar's face copies each count (the copies of a point share its F),
+ * u_bar's copies are zero (the fill overwrites them); theta_bar has 5
+ * doubles (host). */
+int tgh_vjp_rhs(tgh_problem* p, const double* u, double t, const double* du_bar, double* u_bar, double* theta_bar);
+/* From u (device state, in: u_0 at t0; out: u_T) nsteps RK4 steps of dt
+ * (gh_solve's stepping, checkpointed every `every` steps), J = the masked L2
+ * error of u_T against the exact state at T = t0 + nsteps dt, and grad[5] =
+ * dJ/dtheta (host), the stages of each segment recomputed from its
+ * checkpoint in the reverse sweep. ms[2] (host, or NULL): the forward sweep's
+ * and the reverse sweep's device time. */
+int tgh_adjoint(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t every, double* J,
+                double* grad, double* ms);
 /* Doubles in the problem's work array: (N+1+2G)^3 * 20 * nblocks. */
 int64_t tgh_work_length(const tgh_problem* p);
 /* Copy the work array prepare filled (ghosts included) into a caller's

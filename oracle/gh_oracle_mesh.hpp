@@ -11,6 +11,7 @@
 // reproduction and by the interface rates of test/interface_tests.jl:64-96).
 #pragma once
 #include <map>
+#include <type_traits>
 
 #include "gh_oracle.hpp"
 #include "gh_oracle_bounds.hpp"
@@ -116,6 +117,11 @@ struct Mesh {
   int prolong = 0;         // the prolongation order p (an even number of points)
   std::map<std::array<long, 4>, long> leaf;
   std::vector<double> wmid;  // Lagrange weights at the midpoint, p nodes
+  // TreeAMR 0.1's half-open blocks: a leaf owns [0, N) per axis, a point
+  // belongs to the leaf whose half-open extent holds it (when it is a vertex
+  // of that leaf; else it is prolongated), and the upper plane of a
+  // non-periodic axis is a Dirichlet ghost. Off: TreeAMR 0.3's closed blocks.
+  bool half_open = false;
   bool forest() const { return !level.empty(); }
 
   void init_forest(const std::vector<std::array<int, 4>>& leaves, int p) {
@@ -164,6 +170,13 @@ struct Mesh {
       const long nbl = (long)nb[d] << lv;
       o0[d] = std::min(gv[d] / N, nbl - 1);
       alt[d] = -1;
+      if (half_open) {  // the one leaf whose [0, N) extent holds gv
+        // (on the upper plane of a non-periodic axis, a point only a
+        // transfer asks for, the closed rule: the last block)
+        o0[d] = std::min(gv[d] / N, nbl - 1);
+        if (periodic[d] && gv[d] / N >= nbl) o0[d] = gv[d] / N - nbl;
+        continue;
+      }
       if (gv[d] % N == 0 && o0[d] * N == gv[d]) {
         if (o0[d] > 0) alt[d] = o0[d] - 1;
         else if (periodic[d]) alt[d] = nbl - 1;
@@ -205,7 +218,7 @@ struct Mesh {
       const long tot = ((long)nb[d] << L) * N;
       gl[d] = blocks[b][d] * (long)N + l[d];
       if (periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
-      else if (gl[d] < 0 || gl[d] > tot) return F_DIRICHLET;
+      else if (gl[d] < 0 || gl[d] > tot || (half_open && gl[d] == tot)) return F_DIRICHLET;
     }
     const int c = classify_point(L, gl, s);
     if (c == F_COPY && s.b == b && s.loc[0] == l[0] && s.loc[1] == l[1] && s.loc[2] == l[2]) return F_PRIMARY;
@@ -428,6 +441,22 @@ struct Problem {
       }
   }
 
+  // ---- tangents: the reference's CPU path run with a forward-mode dual as
+  // its float type (one direction). work_dot holds the tangent of the work
+  // array the fill left in work; a load gives the dual (value, tangent).
+  using D1 = Dual<double, 1>;
+  std::vector<double> work_dot;
+  template <class T>
+  T ld(long idx) const {
+    if constexpr (std::is_same<T, D1>::value) {
+      T r(work[idx]);
+      r.d[0] = work_dot.empty() ? 0.0 : work_dot[idx];
+      return r;
+    } else {
+      return T(work[idx]);
+    }
+  }
+
   // ---- the fused kernel at one owned point (evolution.jl:258-435,448-695) ----
   template <class T>
   T axis_stencil(const std::vector<double>& w, long base, long stride) const {
@@ -436,7 +465,7 @@ struct Problem {
     bool first = true;
     for (int k = 0; k < n; ++k) {
       if (w[k] == 0.0) continue;  // zero weights are bitwise-neutral (CODE.md:6647-6653)
-      T t = w[k] * T(work[base + (long)(k - r) * stride]);
+      T t = w[k] * ld<T>(base + (long)(k - r) * stride);
       s = first ? t : s + t;
       first = false;
     }
@@ -453,7 +482,7 @@ struct Problem {
       bool fi = true;
       for (int e = 0; e < n; ++e) {
         if (w[e] == 0.0) continue;
-        T t = w[e] * T(work[base + (long)(a - r) * s1 + (long)(e - r) * s2]);
+        T t = w[e] * ld<T>(base + (long)(a - r) * s1 + (long)(e - r) * s2);
         inner = fi ? t : inner + t;
         fi = false;
       }
@@ -476,8 +505,8 @@ struct Problem {
     // (1) the state, the 30 first derivatives, the coefficient set
     V10<T> hv, Pv, dh[3];
     for (int v = 0; v < NC; ++v) {
-      hv[v] = T(work[var + v * sv]);
-      Pv[v] = T(work[var + (NC + v) * sv]);
+      hv[v] = ld<T>(var + v * sv);
+      Pv[v] = ld<T>(var + (NC + v) * sv);
     }
     for (int d = 0; d < 3; ++d)
       for (int v = 0; v < NC; ++v) dh[d][v] = inv_h * axis_stencil<T>(w1, var + v * sv, st[d]);
@@ -516,7 +545,7 @@ struct Problem {
     T inv_h2 = inv_h * inv_h;
     for (int v = 0; v < NC; ++v) {
       long bh = var + v * sv, bP = bh + NC * sv;
-      T Pi = T(work[bP]);
+      T Pi = ld<T>(bP);
       T dh1 = inv_h * axis_stencil<T>(w1, bh, st[0]);
       T dh2 = inv_h * axis_stencil<T>(w1, bh, st[1]);
       T dh3 = inv_h * axis_stencil<T>(w1, bh, st[2]);
@@ -645,6 +674,145 @@ struct Problem {
     for (int v = 0; v < NC; ++v) {
       du[m.sidx(b, v, i, j, k)] = dth[v];
       du[m.sidx(b, NC + v, i, j, k)] = dtP[v];
+    }
+  }
+
+  // The tangent of one owned point's right-hand side along (work_dot, the
+  // parameter tangents pd = d(gamma0_near, gamma0_far, gamma2, rho_max,
+  // rho_ramp)) by the dual pass: the branches of a hole at rest (none, the
+  // frozen core of :damped / :frozen, the layer w F - rho (u - u_e)).
+  void rhs_point_jvp(long b, long i, long j, long k, double t, const double pd[5], double* du,
+                     double* du_dot) const {
+    const Mesh& m = mesh;
+    double x[3];
+    m.position(b, i, j, k, x);
+    D1 g0;
+    if (cs.damping == DAMP_CONSTANT) {
+      g0 = D1(cs.g0);
+    } else {
+      double cc[3];
+      center_at(cs, t, cc);
+      const double d1 = x[0] - cc[0], d2 = x[1] - cc[1], d3 = x[2] - cc[2];
+      const double e = std::exp(-(d1 * d1 + d2 * d2 + d3 * d3) / (2.0 * cs.g0_width * cs.g0_width));
+      D1 nr(cs.g0_near), fr(cs.g0_far);
+      nr.d[0] = pd[0];
+      fr.d[0] = pd[1];
+      g0 = fr + (nr - fr) * e;
+    }
+    D1 g2(cs.g2);
+    g2.d[0] = pd[2];
+    const D1 epsh(dissipation_rate(cs, t, x) * (1.0 / m.hb(b)));
+    const bool diss = has_dissipation(cs);
+    auto put = [&](int v, const D1& f) {
+      du[m.sidx(b, v, i, j, k)] = f.v;
+      du_dot[m.sidx(b, v, i, j, k)] = f.d[0];
+    };
+    V10<D1> dth, dtP;
+    if (cs.interior != INT_NONE) {
+      const IGeo g = interior_geo(cs, t, x);
+      const double r = g.r;
+      if (cs.interior == INT_PASTED || cs.interior == INT_FITTED || core_follows(cs) || cs.shaped)
+        throw std::runtime_error("the tangent pass is built for a sphere at rest (:damped, :frozen)");
+      if (r < g.r0) {
+        for (int v = 0; v < 20; ++v) put(v, D1(0.0));
+        return;
+      }
+      rhs_point_F<D1>(b, i, j, k, t, g0, g2, epsh, diss, dth, dtP);
+      if (r >= g.r1) {
+        for (int v = 0; v < NC; ++v) {
+          put(v, dth[v]);
+          put(NC + v, dtP[v]);
+        }
+        return;
+      }
+      const double sl = (r - g.r0) / (g.r1 - g.r0);
+      const double w = smoothstep(sl / cs.w_ramp);
+      D1 rmax(cs.rho_max), rramp(cs.rho_ramp);
+      rmax.d[0] = pd[3];
+      rramp.d[0] = pd[4];
+      // rho = rho_max smoothstep((1 - s) / rho_ramp), the quintic's
+      // derivative 30 u^2 (1 - u)^2 inside (0, 1) and zero where it clamps
+      const D1 arg = (1.0 - sl) / rramp;
+      const double u0 = std::min(std::max(arg.v, 0.0), 1.0);
+      D1 ss(smoothstep(arg.v));
+      ss.d[0] = (arg.v > 0.0 && arg.v < 1.0) ? 30.0 * u0 * u0 * (1.0 - u0) * (1.0 - u0) * arg.d[0] : 0.0;
+      const D1 rho = cs.interior == INT_FROZEN ? D1(0.0) : rmax * ss;
+      V10<double> he, Pe, dhe[3];
+      background_state(cs.bg, t, x, he, Pe, dhe);
+      const long var = m.widx(b, 0, i, j, k), sv = m.S * m.S * m.S;
+      for (int v = 0; v < NC; ++v) {
+        put(v, w * dth[v] - rho * (ld<D1>(var + v * sv) - he[v]));
+        put(NC + v, w * dtP[v] - rho * (ld<D1>(var + (NC + v) * sv) - Pe[v]));
+      }
+      return;
+    }
+    rhs_point_F<D1>(b, i, j, k, t, g0, g2, epsh, diss, dth, dtP);
+    for (int v = 0; v < NC; ++v) {
+      put(v, dth[v]);
+      put(NC + v, dtP[v]);
+    }
+  }
+  // prepare! of the tangent: the fill's copies applied to the tangent state;
+  // a Dirichlet ghost's tangent is zero (the hook does not depend on the
+  // state or on the parameters). Uniform forests.
+  void prepare_dot(const double* udot) {
+    const Mesh& m = mesh;
+    if (m.forest()) throw std::runtime_error("the tangent fill is built for uniform forests");
+    work_dot.assign(work.size(), 0.0);
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (int v = 0; v < 20; ++v)
+        for (long k = 0; k <= m.N; ++k)
+          for (long j = 0; j <= m.N; ++j)
+            for (long i = 0; i <= m.N; ++i) work_dot[m.widx(b, v, i, j, k)] = udot[m.sidx(b, v, i, j, k)];
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = -m.G; k <= m.N + m.G; ++k)
+        for (long j = -m.G; j <= m.N + m.G; ++j)
+          for (long i = -m.G; i <= m.N + m.G; ++i) {
+            long l[3] = {i, j, k}, owner = 0, loc[3] = {0, 0, 0};
+            const int c = m.classify(b, l, owner, loc);
+            if (c == -1) continue;
+            for (int v = 0; v < 20; ++v)
+              work_dot[m.widx(b, v, i, j, k)] = c == -2 ? 0.0 : work_dot[m.widx(owner, v, loc[0], loc[1], loc[2])];
+          }
+  }
+  // gh_rhs! and its tangent along (udot, pd) in one dual pass.
+  void gh_rhs_jvp(const double* u, const double* udot, double t, const double pd[5], double* du, double* du_dot,
+                  int nthreads = 1) {
+    if (bounds.on) throw std::runtime_error("the tangent pass is built for cases without range bounds");
+    prepare(u, t);
+    prepare_dot(udot);
+    const Mesh& m = mesh;
+    const long nb = m.nblocks();
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+    for (long b = 0; b < nb; ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) rhs_point_jvp(b, i, j, k, t, pd, du, du_dot);
+    work_dot.clear();
+  }
+  // One RK4 step of (u, udot): the tangent linear model of rk4_step.
+  void rk4_step_jvp(double* u, double* udot, double t, double dt, const double pd[5], int nthreads = 1) {
+    const long n = mesh.state_size();
+    std::vector<double> k[4], kd[4], y(n), yd(n);
+    for (int s = 0; s < 4; ++s) {
+      k[s].assign(n, 0.0);
+      kd[s].assign(n, 0.0);
+    }
+    const double c[4] = {0.0, dt / 2, dt / 2, dt};
+    for (int s = 0; s < 4; ++s) {
+      if (s == 0) {
+        gh_rhs_jvp(u, udot, t, pd, k[0].data(), kd[0].data(), nthreads);
+        continue;
+      }
+      for (long i = 0; i < n; ++i) {
+        y[i] = u[i] + c[s] * k[s - 1][i];
+        yd[i] = udot[i] + c[s] * kd[s - 1][i];
+      }
+      gh_rhs_jvp(y.data(), yd.data(), t + c[s], pd, k[s].data(), kd[s].data(), nthreads);
+    }
+    for (long i = 0; i < n; ++i) {
+      u[i] = u[i] + (dt / 6) * (k[0][i] + 2 * k[1][i] + 2 * k[2][i] + k[3][i]);
+      udot[i] = udot[i] + (dt / 6) * (kd[0][i] + 2 * kd[1][i] + 2 * kd[2][i] + kd[3][i]);
     }
   }
 
@@ -824,37 +992,102 @@ struct Problem {
           }
     return lam;
   }
-  // Masked volume-weighted L2 and Linf of u - u_exact(t) over evolved
-  // primaries (constraints.jl:311-396, 655-671; CODE.md: the L2 divides by the
-  // evolved volume).
-  void error_norms(const double* u, double t, double& l2, double& linf) const {
+  // gh_error_kernel! + error_norms (constraints.jl:311-396, 629-671): at
+  // every primary the error e = |u - u_exact(t)|, the 2-norm over the 20
+  // variables summed in variable order, where the point is evolved; the
+  // masked L2 sqrt(sum w e^2 / sum w) with w the point's volume relative to
+  // level 0's (2^(-3 level)), the masked L-inf the largest e; the layer
+  // residual, the largest e at r_0 <= r < r_1 (the distance from the target
+  // cache A + (t - t_f) S for :fitted); the drift, the largest |h_tt -
+  // h_tt,exact| at evolved points with shell_lo <= r <= shell_hi.
+  void error_rows(const double* u, double t, double shell_lo, double shell_hi, double out[4]) const {
     const Mesh& m = mesh;
-    double num = 0, den = 0;
-    linf = 0;
+    double num = 0, den = 0, linf = 0, res = 0, drift = 0;
+    const bool fitted = cs.interior == INT_FITTED && !tw.empty();
     for (long b = 0; b < m.nblocks(); ++b)
       for (long k = 0; k <= m.N; ++k)
         for (long j = 0; j <= m.N; ++j)
           for (long i = 0; i <= m.N; ++i) {
+            if (!m.is_primary(b, i, j, k)) continue;
             double x[3], s[20];
             m.position(b, i, j, k, x);
-            if (!is_evolved(t, x)) continue;
-            // on a forest only primaries count, each with its volume h^3
-            // relative to level 0's
-            const bool prim = m.is_primary(b, i, j, k);
-            if (m.forest() && !prim) continue;
+            const bool keep = is_evolved(t, x);
+            bool lay = false;
+            double r = 0;
+            if (cs.interior != INT_NONE) {
+              const IGeo g = interior_geo(cs, t, x);
+              lay = g.r0 <= g.r && g.r < g.r1;
+              r = g.r;
+            }
+            const bool inshell = keep && shell_lo <= r && r <= shell_hi;
+            if (!(keep || lay)) continue;
             case_state(cs, t, x, s);
             double e2 = 0;
             for (int v = 0; v < 20; ++v) {
-              double e = u[m.sidx(b, v, i, j, k)] - s[v];
+              const double e = u[m.sidx(b, v, i, j, k)] - s[v];
               e2 += e * e;
-              linf = std::max(linf, std::abs(e));
             }
-            if (!prim) continue;
+            const double e = std::sqrt(e2);
+            if (keep) {
+              const double wv = m.forest() ? 1.0 / (double)(1L << (3 * m.level[b])) : 1.0;
+              num += m.forest() ? e2 * wv : e2;
+              den += wv;
+              if (!(e <= linf)) linf = e;  // NaN propagates
+            }
+            if (lay) {
+              double ef = e;
+              if (fitted) {
+                double f2 = 0;
+                for (int v = 0; v < 20; ++v) {
+                  const long ti = tidx(b, i, j, k);
+                  const double tg = tw[ti + (long)v * m.P * m.P * m.P] + (t - t_f) * tw[ti + (long)(20 + v) * m.P * m.P * m.P];
+                  const double d = u[m.sidx(b, v, i, j, k)] - tg;
+                  f2 += d * d;
+                }
+                ef = std::sqrt(f2);
+              }
+              if (!(ef <= res)) res = ef;
+            }
+            if (inshell) {
+              const double d1 = std::abs(u[m.sidx(b, 0, i, j, k)] - s[0]);
+              if (!(d1 <= drift)) drift = d1;
+            }
+          }
+    out[0] = den > 0 ? std::sqrt(num / den) : 0.0;
+    out[1] = linf;
+    out[2] = res;
+    out[3] = drift;
+  }
+  // The directional derivative of the masked L2 error along udot:
+  // dJ = sum w (u - u_exact) . udot / (J sum w) over the evolved primaries.
+  double error_tangent(const double* u, const double* udot, double t) const {
+    const Mesh& m = mesh;
+    double num = 0, den = 0, dnum = 0;
+    for (long b = 0; b < m.nblocks(); ++b)
+      for (long k = 0; k <= m.N; ++k)
+        for (long j = 0; j <= m.N; ++j)
+          for (long i = 0; i <= m.N; ++i) {
+            if (!m.is_primary(b, i, j, k)) continue;
+            double x[3], s[20];
+            m.position(b, i, j, k, x);
+            if (!is_evolved(t, x)) continue;
+            case_state(cs, t, x, s);
             const double wv = m.forest() ? 1.0 / (double)(1L << (3 * m.level[b])) : 1.0;
-            num += m.forest() ? e2 * wv : e2;
+            for (int v = 0; v < 20; ++v) {
+              const double e = u[m.sidx(b, v, i, j, k)] - s[v];
+              num += wv * e * e;
+              dnum += wv * e * udot[m.sidx(b, v, i, j, k)];
+            }
             den += wv;
           }
-    l2 = den > 0 ? std::sqrt(num / den) : 0.0;
+    const double J = den > 0 ? std::sqrt(num / den) : 0.0;
+    return J > 0 ? dnum / (J * den) : 0.0;
+  }
+  void error_norms(const double* u, double t, double& l2, double& linf) const {
+    double r[4];
+    error_rows(u, t, 0.0, -1.0, r);
+    l2 = r[0];
+    linf = r[1];
   }
 
   // apply_bounds! (bounds.jl:613-660, 733-770): the range projection at every

@@ -320,6 +320,7 @@ void* orc_problem_create_forest(const tgh_desc* d, const int* leaves, int nleave
   std::vector<std::array<int, 4>> lv;
   for (int n = 0; n < nleaves; ++n) lv.push_back({leaves[4 * n], leaves[4 * n + 1], leaves[4 * n + 2], leaves[4 * n + 3]});
   Problem* p = problem_from(d);
+  p->mesh.half_open = d->ownership == TGH_OWNERSHIP_HALF_OPEN;
   p->mesh.init_forest(lv, prolong);
   p->setup();
   return p;
@@ -414,6 +415,34 @@ void orc_sym_eigen3(const double* A, double* lam, double* V) {
     for (int j = 0; j < 3; ++j) V[3 * i + j] = v[i][j];
 }
 double orc_max_speed(void* p, const double* u, double t) { return ((Problem*)p)->max_speed(u, t); }
+// gh_rhs! with its dual tangent along (udot, pd[5] = d(gamma0_near,
+// gamma0_far, gamma2, rho_max, rho_ramp)); 0, or 1 for a case the tangent
+// pass does not cover.
+int orc_gh_rhs_jvp(void* p, const double* u, const double* udot, double t, const double* pd, double* du,
+                   double* du_dot, int nthreads) {
+  try {
+    ((Problem*)p)->gh_rhs_jvp(u, udot, t, pd, du, du_dot, nthreads);
+  } catch (const std::exception&) {
+    return 1;
+  }
+  return 0;
+}
+int orc_rk4_step_jvp(void* p, double* u, double* udot, double t, double dt, const double* pd, int nthreads) {
+  try {
+    ((Problem*)p)->rk4_step_jvp(u, udot, t, dt, pd, nthreads);
+  } catch (const std::exception&) {
+    return 1;
+  }
+  return 0;
+}
+double orc_error_tangent(void* p, const double* u, const double* udot, double t) {
+  return ((Problem*)p)->error_tangent(u, udot, t);
+}
+// with_interior(p, with_rho_max(p.interior, rho)): the layer's rate.
+void orc_set_rho_max(void* p, double rho) { ((Problem*)p)->cs.rho_max = rho; }
+void orc_error_rows(void* p, const double* u, double t, double shell_lo, double shell_hi, double* out) {
+  ((Problem*)p)->error_rows(u, t, shell_lo, shell_hi, out);
+}
 void orc_error_norms(void* p, const double* u, double t, double* l2, double* linf) {
   ((Problem*)p)->error_norms(u, t, *l2, *linf);
 }

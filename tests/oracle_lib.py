@@ -85,6 +85,8 @@ def lib():
             "orc_sym_eigen3": (None, [dp, dp, dp]),
             "orc_max_speed": (D, [P, dp, D]),
             "orc_error_norms": (None, [P, dp, D, C.POINTER(D), C.POINTER(D)]),
+            "orc_error_rows": (None, [P, dp, D, D, D, dp]),
+            "orc_set_rho_max": (None, [P, D]),
             "orc_is_primary": (C.c_int, [P, Lg, Lg, Lg, Lg]),
             "orc_count_rhs_flops": (D, [P, Lg, Lg, Lg, Lg, C.c_int]),
             "orc_max_threads": (C.c_int, []),
@@ -236,6 +238,15 @@ class OracleProblem:
     def max_speed(self, u, t=0.0):
         return lib().orc_max_speed(self.p, u, t)
 
+    def set_rho_max(self, rho):
+        lib().orc_set_rho_max(self.p, float(rho))
+
+    def error_rows(self, u, t, shell=(0.0, -1.0)):
+        """(err_l2, err_linf, residual, drift): gh_error! and error_norms."""
+        out = np.zeros(4)
+        lib().orc_error_rows(self.p, u, t, float(shell[0]), float(shell[1]), out)
+        return tuple(float(x) for x in out)
+
     def error_norms(self, u, t):
         l2 = C.c_double()
         li = C.c_double()
@@ -279,3 +290,69 @@ def interior_geo(desc, t, x):
     out = np.zeros(3)
     lib().orc_interior_geo(C.byref(desc), float(t), np.ascontiguousarray(x, dtype=np.float64), out)
     return out
+
+
+def evolve(orc, u, *, hmin, t_end, chunk, cfl=0.25, rho_max_factor=None, shell=(0.0, -1.0), nthreads=8):
+    """evolve!'s chunk loop on the oracle (src/driver.jl:1732-1800; chunk_count
+    and chunk_bounds, src/precision.jl:81-100): per chunk λ at its start, dt =
+    cfl h_min / λ, steps = ⌈span / dt⌉ of span / steps, the layer's rate
+    factor / dt when rho_max_factor is given (else the problem's own), and the
+    record at the chunk's end: (t, steps, err_l2, err_linf, residual, drift,
+    gauge_l2 = the largest component L2 of C_a)."""
+    import math
+    r = t_end / chunk
+    m = round(r)
+    nchunks = int(m) if m >= 1 and abs(r - m) <= 4 * (math.ulp(r) + math.ulp(t_end) / chunk) else math.ceil(r)
+    recs = []
+    for c in range(1, nchunks + 1):
+        a = min((c - 1) * chunk, t_end)
+        b = t_end if c == nchunks else min(c * chunk, t_end)
+        lam = orc.max_speed(u, a)
+        steps = max(1, math.ceil((b - a) / (cfl * hmin / lam)))
+        dt = (b - a) / steps
+        if rho_max_factor is not None:
+            orc.set_rho_max(rho_max_factor / dt)
+        for s in range(steps):
+            orc.rk4_step(u, a + s * dt, dt, nthreads)
+        rows = orc.error_rows(u, b, shell)
+        gl2, _ = orc.gauge_constraint(u, b)
+        recs.append({"t": b, "steps": steps, "err_l2": rows[0], "err_linf": rows[1], "residual": rows[2],
+                     "drift": rows[3], "gauge_l2": float(max(gl2))})
+    return recs
+
+
+def _jvp_sigs():
+    L = lib()
+    P, D = C.c_void_p, C.c_double
+    L.orc_gh_rhs_jvp.restype = C.c_int
+    L.orc_gh_rhs_jvp.argtypes = [P, dp, dp, D, dp, dp, dp, C.c_int]
+    L.orc_error_tangent.restype = C.c_double
+    L.orc_error_tangent.argtypes = [P, dp, dp, D]
+    L.orc_rk4_step_jvp.restype = C.c_int
+    L.orc_rk4_step_jvp.argtypes = [P, dp, dp, D, D, dp, C.c_int]
+    return L
+
+
+def gh_rhs_jvp(orc, u, udot, t, pd, nthreads=8):
+    """(F(u), its tangent along udot and the parameter tangents pd =
+    d(gamma0_near, gamma0_far, gamma2, rho_max, rho_ramp)) by the dual pass."""
+    L = _jvp_sigs()
+    du, dd = np.zeros_like(u), np.zeros_like(u)
+    rc = L.orc_gh_rhs_jvp(orc.p, np.ascontiguousarray(u), np.ascontiguousarray(udot), float(t),
+                          np.ascontiguousarray(pd, dtype=np.float64), du, dd, nthreads)
+    if rc:
+        raise RuntimeError("the oracle's tangent pass does not cover this case")
+    return du, dd
+
+
+def rk4_step_jvp(orc, u, udot, t, dt, pd, nthreads=8):
+    L = _jvp_sigs()
+    rc = L.orc_rk4_step_jvp(orc.p, u, udot, float(t), float(dt), np.ascontiguousarray(pd, dtype=np.float64),
+                            nthreads)
+    if rc:
+        raise RuntimeError("the oracle's tangent pass does not cover this case")
+
+
+def error_tangent(orc, u, udot, t):
+    """The directional derivative of the masked L2 error along udot."""
+    return float(_jvp_sigs().orc_error_tangent(orc.p, np.ascontiguousarray(u), np.ascontiguousarray(udot), float(t)))

@@ -100,21 +100,40 @@ def test_transfer_refuses_other_bricks_and_two_level_jumps(tgh):
         tgh.transfer(other, other.statevector(), 0.0, dflat, dflat.statevector())
 
 
+def _hole_case(tgh, v=0.0):
+    """A Kerr-Schild hole (a = 0.5, the closed-form source) whose layer
+    r_0 = 0.7, r_1 = 1.2 sits m = 8 level-1 spacings inside the horizon at
+    root spacing 1/8, so the refinement's level floor is 1 (src/refinement.jl:
+    436-470) on the shell from r_1 to the horizon's largest radius."""
+    return tgh.kerr_schild_case(M=1.0, a=0.5, v=v, axis=1, halfwidth=4.0, r_0=0.7, r_1=1.2,
+                                gauge_source="algebraic")
+
+
+def _tau(tgh, p, u, t):
+    return tgh.refinement_indicator(p, u, t, scale=tgh.field_scale(p, u, t), eps=0.01)
+
+
 def test_regrid_refines_where_the_indicator_fires_and_carries_the_state(tgh):
     """A Kerr-Schild hole on a uniform brick: the blocks whose Löhner maximum
-    passes refine_tol refine once, the state on the new leaves is the
-    oracle's transfer, and the new problem steps."""
+    passes refine_tol refine once, so do the blocks the level floor holds,
+    the state on the new leaves is the oracle's transfer, and the new problem
+    steps."""
     torch = torch_cuda()
-    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
+    case = _hole_case(tgh)
     f = tgh.gh_forest(None, case, N=16, roots=4)
     dev, orc = _problem(tgh, f, case, 6)
     u = dev.statevector()
     tgh.initial_state(u, dev, 0.0)
-    flags, tau = tgh.regrid_flags(dev, u, 0.0, refine_tol=1.0, coarsen_tol=0.0, maxlevel_cap=1)
+    tau = _tau(tgh, dev, u, 0.0)
     thr = sorted(tau)[len(tau) * 3 // 4]
-    new, u_new, flags = tgh.regrid(dev, u, 0.0, refine_tol=thr, coarsen_tol=0.0, maxlevel_cap=1)
+    ref = tgh.Refinement(refine_tol=thr, coarsen_tol=thr * 1e-3, maxlevel_cap=1, floor_margin=0.0)
+    bounds = tgh.level_bounds(dev.forest, dev.case, dev.q, ref, 0.0, dev.leaves())
+    assert sum(1 for b in bounds if b[0] == 1) == 8  # the eight root blocks around the hole
+    new, u_new, flags = tgh.regrid(dev, u, 0.0, refinement=ref)
     fired = [lf for lf, fl in zip(dev.leaves(), flags) if fl > 0]
-    assert 0 < len(fired) < f.nleaves()
+    assert 8 <= len(fired) < f.nleaves()
+    for lf, (flo, _), t in zip(dev.leaves(), bounds, tau):
+        assert (lf in fired) == (flo == 1 or t > thr)
     for lf in fired:
         assert lf not in new.forest.leaves
         assert (1, 2 * lf[1], 2 * lf[2], 2 * lf[3]) in new.forest.leaves
@@ -128,19 +147,25 @@ def test_regrid_refines_where_the_indicator_fires_and_carries_the_state(tgh):
     assert torch.isfinite(u_new).all()
 
 
-def test_regrid_coarsens_quiet_sibling_groups(tgh):
-    """The same hole on the two-level hole_forest with coarsen_tol above
-    every block's indicator: every complete sibling group whose parent keeps
-    2:1 balance merges, and the state is injected."""
-    torch = torch_cuda()
-    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=4.0, r_0=0.7, r_1=1.7, gauge_source="algebraic")
-    fh = tgh.hole_forest(None, case, N=16, roots=4, radii=(1.5,))
+def test_regrid_coarsens_quiet_leaves_outside_the_floor(tgh):
+    """The same hole on a two-level hole_forest that reaches the outer faces,
+    with coarsen_tol above every block's indicator: the level-1 leaves the
+    floor's shell meets keep their level, the sibling groups outside it merge
+    (the ceiling caps the ones at the boundary at level 0 anyway), and the
+    state is injected."""
+    torch_cuda()
+    case = _hole_case(tgh)
+    fh = tgh.hole_forest(None, case, N=16, roots=4, radii=(3.0,))
     dev, orc = _problem(tgh, fh, case, 6)
     u = dev.statevector()
     tgh.initial_state(u, dev, 0.0)
-    new, u_new, flags = tgh.regrid(dev, u, 0.0, refine_tol=1e300, coarsen_tol=1e299, maxlevel_cap=1)
-    assert all(fl == (-1 if lf[0] > 0 else 0) for lf, fl in zip(dev.leaves(), flags))
-    assert new.forest.maxlevel() == 0
+    ref = tgh.Refinement(refine_tol=0.9999, coarsen_tol=0.999, maxlevel_cap=1, floor_margin=0.0)
+    new, u_new, flags = tgh.regrid(dev, u, 0.0, refinement=ref)
+    bounds = tgh.level_bounds(fh, case, 4, ref, 0.0, dev.leaves())
+    for lf, fl, (flo, _) in zip(dev.leaves(), flags, bounds):
+        assert fl == (0 if lf[0] == 0 or flo == 1 else -1), (lf, fl, flo)
+    kept = [lf for lf in dev.leaves() if lf[0] == 1 and lf in new.forest.leaves]
+    assert len(kept) == 64 and new.forest.maxlevel() == 1
     desc = TD.TghDesc.from_buffer_copy(new.desc)
     o_new = oracle_lib.OracleProblem(desc, new.forest.leaves, 6)
     assert _close(u_new.cpu().numpy(), o_new.transfer_from(orc, u.cpu().numpy(), 0.0)) <= TOL
@@ -148,36 +173,37 @@ def test_regrid_coarsens_quiet_sibling_groups(tgh):
 
 def test_regrid_cadence_follows_a_boosted_hole(tgh):
     """Config 5's loop in small: a Kerr-Schild hole boosted to v = 0.3 along
-    x on a two-level hole_forest, 24 steps with a regrid every 8 (no
-    coarsening, the upper quartile of the indicator refines). The masked
-    error at the end stays within twice that of the same steps on the fixed
-    forest, and the loop is deterministic."""
+    x on a two-level hole_forest, regridded every 8 steps with each chunk's
+    step sized on its own mesh (cfl 1/4), the upper quartile of the first
+    indicator refining. The loop is deterministic, reaches level 2 and keeps
+    the masked error within twice that of the fixed forest over the same
+    span."""
     torch = torch_cuda()
-    case = tgh.kerr_schild_case(M=1.0, a=0.5, v=0.3, axis=1, halfwidth=4.0, r_0=0.7, r_1=1.7,
-                                gauge_source="algebraic")
+    case = _hole_case(tgh, v=0.3)
     fh = tgh.hole_forest(None, case, N=16, roots=4, radii=(1.5,))
     ops = tgh.Operators(prolongation=6)
     fixed = tgh.GHProblem(fh, case, q=4, ops=ops)
     u = fixed.statevector()
     tgh.initial_state(u, fixed, 0.0)
-    _, tau = tgh.regrid_flags(fixed, u, 0.0, refine_tol=1.0, coarsen_tol=0.0, maxlevel_cap=2)
+    tau = _tau(tgh, fixed, u, 0.0)
     thr = sorted(tau)[len(tau) * 3 // 4]
-    dt = 0.125 * fh.minimum_spacing()  # CFL 1/4 on the level-2 leaves the regrids add
-    t1 = 24 * dt
+    ref = tgh.Refinement(refine_tol=thr, coarsen_tol=thr * 1e-3, maxlevel_cap=2, floor_margin=0.0)
+    t1 = 24 * tgh.gh_dt(fixed, u, cfl=0.25)
     runs = []
     for _ in range(2):
         p0 = tgh.GHProblem(fh, case, q=4, ops=ops)
         u0 = p0.statevector()
         tgh.initial_state(u0, p0, 0.0)
-        p, uu, log = tgh.gh_solve_regrid(p0, u0, (0.0, t1), dt=dt, every=8, refine_tol=thr, coarsen_tol=0.0,
-                                         maxlevel_cap=2)
-        runs.append((p, uu, log))
+        runs.append(tgh.gh_solve_regrid(p0, u0, (0.0, t1), cfl=0.25, every=8, refinement=ref))
     (p, uu, log), (p2, uu2, log2) = runs
-    assert len(log) == 2 and log == log2
+    print(f"regrids: {log}")
+    assert len(log) >= 2 and log == log2
     assert p.forest.maxlevel() == 2 and p.forest.isbalanced()
+    # once level 2 exists the step halves: the chunks after it are on it
+    assert log[-1][2] < log[0][2]
     assert torch.equal(uu, uu2)
     assert torch.isfinite(uu).all()
-    tgh.gh_solve(fixed, u, (0.0, t1), dt=dt)
+    tgh.gh_solve(fixed, u, (0.0, t1), dt=tgh.gh_dt(fixed, u, cfl=0.25))
     e_fixed = tgh.error_norms(fixed, u, t1)[0]
     e_regrid = tgh.error_norms(p, uu, t1)[0]
     assert e_regrid <= 2 * e_fixed, (e_regrid, e_fixed)

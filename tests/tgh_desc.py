@@ -29,6 +29,7 @@ class TghDesc(C.Structure):
         ("velocity", C.c_double * 3),
         ("shaped", C.c_int32), ("shape_lmax", C.c_int32), ("shape", C.c_double * 81),
         ("r_in", C.c_double), ("r_out", C.c_double), ("offset", C.c_double), ("thickness", C.c_double),
+        ("ownership", C.c_int32),
     ]
 
 

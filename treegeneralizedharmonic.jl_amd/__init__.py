@@ -80,6 +80,7 @@ class TghDesc(C.Structure):
         ("velocity", C.c_double * 3),
         ("shaped", C.c_int32), ("shape_lmax", C.c_int32), ("shape", C.c_double * 81),
         ("r_in", C.c_double), ("r_out", C.c_double), ("offset", C.c_double), ("thickness", C.c_double),
+        ("ownership", C.c_int32),
     ]
 
 
@@ -133,6 +134,10 @@ def _load():
         "tgh_add_noise": (C.c_int, [P, P, C.c_uint64, D]),
         "tgh_rk4": (C.c_int, [P, P, D, D, C.c_int32, C.c_int32]),
         "tgh_error_norms": (C.c_int, [P, P, D, C.POINTER(D), C.POINTER(D)]),
+        "tgh_error_rows": (C.c_int, [P, P, D, D, D, C.POINTER(D)]),
+        "tgh_set_rho_max": (C.c_int, [P, D]),
+        "tgh_vjp_rhs": (C.c_int, [P, P, D, P, P, C.POINTER(D)]),
+        "tgh_adjoint": (C.c_int, [P, P, D, D, C.c_int32, C.c_int32, C.POINTER(D), C.POINTER(D), C.POINTER(D)]),
         "tgh_copy_work": (C.c_int, [P, P]),
         "tgh_profile": (C.c_int, [P, C.c_int32]),
         "tgh_profile_read": (C.c_int, [P, C.POINTER(D)]),
@@ -230,10 +235,10 @@ class GaussianDamping:
 
     def __init__(self, *, near, far, width, center=(0.0, 0.0, 0.0)):
         if not float(width) > 0:
-            raise ArgumentError(f"the damping profile's width must be positive, got {width}")
+            raise ArgumentError(f"GaussianDamping: width = {width}; a Gaussian needs a positive width")
         if float(near) < 0 or float(far) < 0:
-            raise ArgumentError(f"both constraint-damping rates must satisfy γ0 ≥ 0, got near = "
-                                f"{near} and far = {far}")
+            raise ArgumentError(f"GaussianDamping: near = {near}, far = {far}; negative damping "
+                                f"rates would drive the constraints instead of damping them")
         self.near, self.far, self.width = float(near), float(far), float(width)
         self.center = tuple(float(c) for c in center)
 
@@ -247,17 +252,16 @@ class HorizonDissipation:
         _check_float(T)
         eo, ei, r1, rh = float(eps_out), float(eps_in), float(r_1), float(r_h)
         if not (eo >= 0 and ei >= 0):
-            raise ArgumentError(f"both Kreiss–Oliger amplitudes must be non-negative, got ε_out = "
-                                f"{eo} and ε_in = {ei}: the operator carries the damping sign in "
-                                f"its weights, so a negative amplitude amplifies the grid-scale "
-                                f"modes it is there to remove.")
+            raise ArgumentError(f"HorizonDissipation: ε_out = {eo}, ε_in = {ei}; a negative amplitude "
+                                f"turns the dissipation into growth of the highest-frequency modes "
+                                f"(the sign is already in the operator's weights)")
         if not (eo <= 4 and ei <= 4):
-            raise ArgumentError(f"the Kreiss–Oliger amplitude is capped at 4, got ε_out = {eo} and "
-                                f"ε_in = {ei}: the operator damps the lattice's corner mode at "
-                                f"3ε/h, and RK4 is stable on the negative real axis only to 2.79.")
+            raise ArgumentError(f"HorizonDissipation: ε_out = {eo}, ε_in = {ei}; amplitudes above 4 "
+                                f"put the corner mode's decay rate 3ε/h outside RK4's real-axis "
+                                f"stability interval (|z| ≤ 2.79) at the stepper's CFL numbers")
         if not 0 < r1 < rh:
-            raise ArgumentError(f"the profile rises from the horizon inward, so it needs "
-                                f"0 < r_1 < r_h, got r_1 = {r1} and r_h = {rh}")
+            raise ArgumentError(f"HorizonDissipation: r_1 = {r1}, r_h = {rh}; the ramp runs inward "
+                                f"from r_h to r_1, which needs 0 < r_1 < r_h")
         self.eps_out, self.eps_in, self.r_1, self.r_h = eo, ei, r1, rh
         self.center = tuple(float(c) for c in center)
 
@@ -272,46 +276,62 @@ def horizon_min_radius(background):
     raise ArgumentError(f"{type(background).__name__} has no horizon")
 
 
+def horizon_max_radius(background):
+    """The horizon's largest coordinate radius (src/interior.jl:761-765): √(r₊² + a²)
+    with r₊ the smallest (the equator's)."""
+    return (horizon_min_radius(background) ** 2 + background.a ** 2) ** 0.5
+
+
 def horizon_dissipation(case, *, eps_in, eps_out=None):
     """horizon_dissipation(case; ε_in, ε_out = case.ε_KO) (src/initialdata.jl:838-860)."""
     if case.interior is None:
-        raise ArgumentError("the dissipation profile rises from the horizon to the layer's outer "
-                            "radius r_1, and this case has no interior, so no r_1.")
+        raise ArgumentError("horizon_dissipation reads r_1 from the case's interior, and this case "
+                            "has none")
     eo = case.eps_ko if eps_out is None else eps_out
     if isinstance(eo, HorizonDissipation):
-        raise ArgumentError("the profile's exterior amplitude is a number: pass eps_out "
-                            "explicitly for a case whose ε_KO is already a profile.")
+        raise ArgumentError("this case's ε_KO is already a HorizonDissipation; give eps_out "
+                            "as a number")
     return HorizonDissipation(eps_out=eo, eps_in=eps_in, r_1=case.interior.r_1,
                               r_h=horizon_min_radius(case.background), center=case.interior.center)
 
 
 class Interior:
-    """Interior(T; center, r_0, r_1, ρ_max, variant, w_ramp, ρ_ramp)
-    (src/interior.jl:178-238)."""
+    """Interior(T; center, r_0, r_1, ρ_max, variant, w_ramp, ρ_ramp, margin)
+    (src/interior.jl:178-238). `margin` is m, the number of spacings the
+    horizon keeps outside r_1 (r_1 ≤ r_h,min − m h), which the refinement's
+    level floor is derived from."""
     VARIANTS = {"damped": 1, "frozen": 2, "pasted": 3, "fitted": 4}
 
     def __init__(self, T=None, *, center=(0.0, 0.0, 0.0), r_0, r_1, rho_max=0.0,
-                 variant="damped", w_ramp=0.5, rho_ramp=0.5):
+                 variant="damped", w_ramp=0.5, rho_ramp=0.5, margin=8):
         _check_float(T)
         if variant not in self.VARIANTS:
-            raise ArgumentError(f"the interior variant must be one of {tuple(self.VARIANTS)}, "
-                                f"got :{variant}")
+            raise ArgumentError(f"unknown interior variant :{variant} (known: "
+                                f"{', '.join(self.VARIANTS)})")
         if not r_0 > 0:
-            raise ArgumentError(f"the frozen core's radius must be positive, got r_0 = {r_0}")
+            raise ArgumentError(f"Interior: r_0 = {r_0}; the core radius has to be positive")
         if not r_1 > r_0:
-            raise ArgumentError(f"the layer needs r_1 > r_0, got r_0 = {r_0} and r_1 = {r_1}")
+            raise ArgumentError(f"Interior: r_0 = {r_0}, r_1 = {r_1}; the layer is the shell r_0 ≤ r < r_1, so r_1 > r_0")
         if not (0 < w_ramp <= 1 and 0 < rho_ramp <= 1):
-            raise ArgumentError(f"the ramp fractions must lie in (0, 1], but w_ramp = {w_ramp} "
-                                f"and ρ_ramp = {rho_ramp}")
+            raise ArgumentError(f"Interior: w_ramp = {w_ramp}, ρ_ramp = {rho_ramp}; the ramps are "
+                                f"fractions of the layer, in (0, 1]")
         if rho_max < 0:
-            raise ArgumentError(f"the relaxation rate must satisfy ρ_max ≥ 0, got {rho_max}")
+            raise ArgumentError(f"ρ_max = {rho_max} is negative")
+        if int(margin) != margin or margin < 1:
+            raise ArgumentError(f"the margin counts spacings between r_1 and the horizon and must be a "
+                                f"positive integer, got {margin}")
         self.center = tuple(float(c) for c in center)
         self.r_0, self.r_1, self.rho_max = float(r_0), float(r_1), float(rho_max)
         self.variant, self.w_ramp, self.rho_ramp = variant, float(w_ramp), float(rho_ramp)
+        self.margin = int(margin)
 
     def with_rho_max(self, rho_max):
-        return Interior(center=self.center, r_0=self.r_0, r_1=self.r_1, rho_max=rho_max,
-                        variant=self.variant, w_ramp=self.w_ramp, rho_ramp=self.rho_ramp)
+        import copy
+        if not rho_max >= 0:
+            raise ArgumentError(f"ρ_max = {rho_max} is negative")
+        c = copy.copy(self)
+        c.rho_max = float(rho_max)
+        return c
 
 
 def shape_series(shape, lmax, n):
@@ -372,43 +392,52 @@ class FittedInterior:
     VARIANTS = {"damped": 1, "frozen": 2, "fitted": 4}
 
     def __init__(self, T=None, *, center, shape, lmax=None, offset, thickness, rho_max=0.0,
-                 variant="damped", w_ramp=0.5, rho_ramp=0.5, r_in=None, r_out=None):
+                 variant="damped", w_ramp=0.5, rho_ramp=0.5, r_in=None, r_out=None, margin=8):
         _check_float(T)
         if variant not in self.VARIANTS:
-            raise ArgumentError(f"the interior variant of a tracked geometry must be one of "
-                                f"{tuple(self.VARIANTS)} here, got :{variant}")
+            raise ArgumentError(f"unknown variant :{variant} for a tracked geometry (built here: "
+                                f"{', '.join(self.VARIANTS)})")
         shape = [float(a) for a in shape]
         L = int(math.isqrt(len(shape)) - 1) if lmax is None else int(lmax)
         if (L + 1) ** 2 != len(shape):
-            raise ArgumentError(f"a shape of degree lmax = {L} has (lmax + 1)² = {(L + 1) ** 2} real "
-                                f"coefficients, got {len(shape)}")
+            raise ArgumentError(f"FittedInterior: {len(shape)} coefficients do not make a series "
+                                f"of degree {L}, which has {(L + 1) ** 2}")
         if L > 8:
-            raise ArgumentError(f"the shape's degree is at most 8 here, got lmax = {L}")
+            raise ArgumentError(f"FittedInterior: lmax = {L}; this library carries series up to degree 8")
         if r_in is None or r_out is None:
             r_in, r_out = shape_bounds(shape, L)
         if not 0 < r_in <= r_out:
-            raise ArgumentError(f"the horizon's bounding radii must satisfy 0 < r_in ≤ r_out, got "
-                                f"r_in = {r_in}, r_out = {r_out}")
+            raise ArgumentError(f"FittedInterior: r_in = {r_in}, r_out = {r_out}; the clamp "
+                                f"interval needs 0 < r_in ≤ r_out")
         if not (offset > 0 and thickness > 0):
-            raise ArgumentError(f"the offset and the thickness are lengths and must be positive, got "
-                                f"offset = {offset} and thickness = {thickness}")
+            raise ArgumentError(f"FittedInterior: offset = {offset}, thickness = {thickness}; both "
+                                f"are distances and have to be positive")
         if not (r_in - offset) - thickness > 0:
-            raise ArgumentError(f"the core surface r_h(n̂) − offset − thickness reaches the center: "
-                                f"r_in = {r_in}, offset = {offset}, thickness = {thickness}")
+            raise ArgumentError(f"FittedInterior: r_in − offset − thickness = "
+                                f"{(r_in - offset) - thickness} ≤ 0, so the core would swallow the center")
         if not (0 < w_ramp <= 1 and 0 < rho_ramp <= 1):
-            raise ArgumentError(f"the ramp fractions must lie in (0, 1], got w_ramp = {w_ramp} and "
-                                f"ρ_ramp = {rho_ramp}")
+            raise ArgumentError(f"FittedInterior: w_ramp = {w_ramp}, ρ_ramp = {rho_ramp}; the "
+                                f"ramps are fractions of the layer, in (0, 1]")
         if rho_max < 0:
-            raise ArgumentError(f"the relaxation rate must satisfy ρ_max ≥ 0, got {rho_max}")
+            raise ArgumentError(f"ρ_max = {rho_max} is negative")
         self.center = tuple(float(c) for c in center)
         self.shape, self.lmax = shape, L
         self.r_in, self.r_out = float(r_in), float(r_out)
         self.offset, self.thickness = float(offset), float(thickness)
         self.rho_max, self.variant = float(rho_max), variant
         self.w_ramp, self.rho_ramp = float(w_ramp), float(rho_ramp)
+        self.margin = int(margin)
         # the bounding radii of the layer, for the checks that read r_0 and r_1
         self.r_0 = (self.r_in - self.offset) - self.thickness
         self.r_1 = self.r_out - self.offset
+
+    def with_rho_max(self, rho_max):
+        import copy
+        if not rho_max >= 0:
+            raise ArgumentError(f"ρ_max = {rho_max} is negative")
+        c = copy.copy(self)
+        c.rho_max = float(rho_max)
+        return c
 
 
 class StateBounds:
@@ -419,21 +448,21 @@ class StateBounds:
         _check_float(T)
         vals = [float(x) for x in (alpha_min, alpha_max, lambda_min, lambda_max, beta_max, K_max, r_gate)]
         if not all(math.isfinite(x) for x in vals):
-            raise ArgumentError(f"every bound must be finite, got {vals}: a bound at infinity is no "
-                                f"bound, and it is spelled by leaving the case's bounds at None instead.")
+            raise ArgumentError(f"StateBounds: {vals}; all seven numbers have to be finite (a case "
+                                f"without bounds passes bounds=None)")
         a0, a1, l0, l1, b, K, rg = vals
         if not 0 < a0 < 1 < a1:
-            raise ArgumentError(f"the lapse range needs 0 < α_min < 1 < α_max, got α_min = {a0} and "
-                                f"α_max = {a1}")
+            raise ArgumentError(f"StateBounds: α_min = {a0}, α_max = {a1}; the lapse interval has "
+                                f"to contain 1 and stay positive")
         if not 0 < l0 < 1 < l1:
-            raise ArgumentError(f"the spatial metric's eigenvalue range needs 0 < λ_min < 1 < λ_max, "
-                                f"got λ_min = {l0} and λ_max = {l1}")
+            raise ArgumentError(f"StateBounds: λ_min = {l0}, λ_max = {l1}; the eigenvalue "
+                                f"interval of γ_ij has to contain 1 and stay positive")
         if not b > 0:
-            raise ArgumentError(f"the shift's norm bound must be positive, got β_max = {b}")
+            raise ArgumentError(f"StateBounds: β_max = {b} is not positive")
         if not K > 0:
-            raise ArgumentError(f"the momentum's scale bound must be positive, got K_max = {K}")
+            raise ArgumentError(f"StateBounds: K_max = {K} is not positive")
         if not rg > 0:
-            raise ArgumentError(f"the gate radius must be positive, got r_gate = {rg}")
+            raise ArgumentError(f"StateBounds: r_gate = {rg} is not positive")
         (self.alpha_min, self.alpha_max, self.lambda_min, self.lambda_max, self.beta_max, self.K_max,
          self.r_gate) = vals
 
@@ -468,8 +497,8 @@ class GHCase:
         if bounds is not None and not isinstance(bounds, StateBounds):
             raise ArgumentError("bounds is None or a StateBounds")
         if bounds is not None and interior is None:
-            raise ArgumentError("this case carries range bounds but no interior: the projection is "
-                                "gated on r < r_gate from the hole's center")
+            raise ArgumentError("StateBounds act inside r_gate about the hole's center, and this "
+                                "case has no hole interior to measure r from")
         self.bounds = bounds
         if not gamma2 > -1:
             raise ArgumentError(f"γ2 must satisfy γ2 > −1, got {gamma2}")
@@ -478,7 +507,7 @@ class GHCase:
             if tuple(eps_ko.center) != tuple(c0):
                 raise ArgumentError("the dissipation profile is centred on the case's hole")
         elif eps_ko < 0:
-            raise ArgumentError(f"the Kreiss–Oliger amplitude must be ≥ 0, got {eps_ko}")
+            raise ArgumentError(f"ε_KO = {eps_ko} is negative")
         self.background = background
         self.box = tuple((float(a), float(b)) for a, b in box)
         self.periodic = tuple(bool(p) for p in periodic)
@@ -505,7 +534,7 @@ def with_interior(case, interior):
 
 
 def minkowski_case(T=None, *, L, eps_ko, gamma0, gamma2):
-    """Flat space on a periodic box of side L (src/initialdata.jl:869-871)."""
+    """minkowski_case (src/initialdata.jl:869-871): Minkowski space, periodic box [0, L]^3."""
     return GHCase(T, Minkowski(), box=[(0.0, L)] * 3, periodic=(True,) * 3, eps_ko=eps_ko,
                   gamma0=gamma0, gamma2=gamma2)
 
@@ -520,17 +549,17 @@ def _moving_hole(v, x0, axis):
     """moving_hole_case's placement (src/initialdata.jl:1283-1298): the hole at
     x0 + v t along axis (1, 2 or 3), the background translate(boost(bg, −v ê), x0 ê)."""
     if axis not in (1, 2, 3):
-        raise ArgumentError(f"axis is the coordinate the hole sits and moves on, 1, 2 or 3, got {axis}.")
+        raise ArgumentError(f"axis = {axis}: the hole moves along coordinate 1 (x), 2 (y) or 3 (z)")
     v, x0 = float(v), float(x0)
     if not abs(v) < 1:
-        raise ArgumentError(f"the hole's velocity must be below the speed of light, got |v| = {abs(v)}")
+        raise ArgumentError(f"|v| = {abs(v)} ≥ 1: a hole moves slower than light")
     on_axis = lambda s: tuple(s if d == axis - 1 else 0.0 for d in range(3))  # noqa: E731
     return on_axis(x0), on_axis(v)
 
 
 def kerr_schild_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, r_0, r_1, rho_max=None,
                      eps_ko=0.5, gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
-                     gauge_source=None, bounds=None):
+                     gauge_source=None, bounds=None, margin=8):
     """kerr_schild_case / hole_case (src/initialdata.jl:1044-1174,1230-1298) for a
     hole in a Dirichlet box, at x0 + v t along axis. ρ_max defaults to the
     physical rate 4/M the driver sets per chunk. gauge_source=None samples
@@ -539,14 +568,13 @@ def kerr_schild_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, 
     kernel, which moves with the hole."""
     center, vel = _moving_hole(v, x0, axis)
     if any(vel) and gauge_source is None:
-        raise ArgumentError("this background is neither harmonic nor static, so its gauge source depends on "
-                            "time and the sampled source cannot represent it: give a moving Kerr-Schild hole "
-                            "gauge_source=\"algebraic\" (the KerrSchildSource that moves with it), or use "
-                            "harmonic_kerr_case.")
+        raise ArgumentError("a moving Kerr-Schild hole has a time-dependent H_a, which a source sampled once at "
+                            "creation cannot follow: pass gauge_source=\"algebraic\" (evaluated in the kernel at "
+                            "each point and time) or move a harmonic_kerr_case hole instead")
     damping = gamma0 if gamma0 is not None else GaussianDamping(near=1 / M, far=1 / (10 * M),
                                                                 width=3 * M, center=center)
     inter = Interior(T, center=center, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
-                     variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp)
+                     variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp, margin=margin)
     L = float(halfwidth)
     return GHCase(T, KerrSchild(M, a), box=[(-L, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
                   gamma0=damping, gamma2=gamma2, interior=inter, gauge_source=gauge_source,
@@ -555,7 +583,7 @@ def kerr_schild_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, 
 
 def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, r_0, r_1, rho_max=None,
                        eps_ko=0.5, gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
-                       bounds=None):
+                       bounds=None, margin=8):
     """harmonic_kerr_case / hole_case (src/initialdata.jl:1300-1326, 1044-1174):
     Harmonic(M, a) in a Dirichlet box, at x0 + v t along axis, with no gauge
     source (a boost of the harmonic chart is harmonic). Defaults as
@@ -564,7 +592,7 @@ def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth
     damping = gamma0 if gamma0 is not None else GaussianDamping(near=1 / M, far=1 / (10 * M),
                                                                 width=3 * M, center=center)
     inter = Interior(T, center=center, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
-                     variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp)
+                     variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp, margin=margin)
     L = float(halfwidth)
     return GHCase(T, Harmonic(M, a), box=[(-L, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
                   gamma0=damping, gamma2=gamma2, interior=inter, bounds=bounds, velocity=vel)
@@ -581,22 +609,34 @@ class Forest:
     """gh_forest's forest (src/initialdata.jl:1370-1395): a brick of root blocks
     of N cells (roots), uniform, or a frozen hierarchy whose leaves are
     (level, i, j, k), block (i, j, k) of the brick of level l (roots_d 2^l
-    blocks of spacing h / 2^l). `leaves` is None on a uniform forest."""
+    blocks of spacing h / 2^l). `leaves` is None on a uniform forest.
 
-    def __init__(self, case, N, roots, leaves=None):
+    `ownership` names which block owns a point on a block face: "closed"
+    (TreeAMR 0.3, the reference's mesh: a block owns both face planes, a
+    shared point's primary is the finest leaf holding it, both outer planes
+    of a Dirichlet box are evolved) or "half_open" (TreeAMR 0.1: a block owns
+    [0, N) per axis and the upper outer plane is a Dirichlet ghost), the mesh
+    the reference's published frozen-hierarchy tables were measured on."""
+
+    OWNERSHIP = {"closed": 0, "half_open": 1}
+
+    def __init__(self, case, N, roots, leaves=None, ownership="closed"):
         roots = (roots,) * 3 if isinstance(roots, int) else tuple(roots)
         if len(roots) != 3 or any(r < 1 for r in roots):
-            raise ArgumentError(f"roots is one count of root blocks for every dimension or a "
-                                f"brick of three counts, got {roots!r}")
+            raise ArgumentError(f"roots = {roots!r}: give one positive count, or three (one per "
+                                f"axis)")
+        if ownership not in self.OWNERSHIP:
+            raise ArgumentError(f"ownership is \"closed\" or \"half_open\", got {ownership!r}")
+        self.ownership = ownership
         self.N, self.roots, self.case = int(N), roots, case
         self._leaves = None if leaves is None else set(tuple(int(x) for x in lf) for lf in leaves)
         if case is not None:
             widths = [b[1] - b[0] for b in case.box]
             cell = [widths[d] / roots[d] for d in range(3)]
             if any(abs(c - cell[0]) > 1e-12 * abs(cell[0]) for c in cell):
-                raise ArgumentError(f"TreeAMR's blocks are cubes with a single spacing per level, so the "
-                                    f"root blocks must be cubes: the box's widths {widths} over the brick "
-                                    f"of roots {roots} give root blocks of widths {cell}")
+                raise ArgumentError(f"the box widths {widths} over roots {roots} give root blocks of "
+                                    f"widths {cell}; blocks have one spacing on all three axes, so "
+                                    f"the widths must be in the roots' proportions")
 
     @property
     def refined(self):
@@ -713,111 +753,131 @@ class Forest:
         return np.array(self.leaves, dtype=np.int32).reshape(-1, 4)
 
 
-def gh_forest(T=None, case=None, *, N, roots, refined=False):
+def gh_forest(T=None, case=None, *, N, roots, refined=False, ownership="closed"):
     """gh_forest(T, case; N, roots, refined = false) (src/initialdata.jl:1370-1395):
     the uniform brick, or with `refined` the two-level mesh whose root block
     holding lo + 3/8 of the box is refined once (every face of it a coarse-fine
     face)."""
     _check_float(T)
-    f = Forest(case, N, roots)
+    f = Forest(case, N, roots, ownership=ownership)
     if not refined:
         return f
     xref = [case.box[d][0] + 3 * (case.box[d][1] - case.box[d][0]) / 8 for d in range(3)]
     targets = [lf for lf in f.leaves
                if all(f.block_extent(lf)[d][0] <= xref[d] < f.block_extent(lf)[d][1] for d in range(3))]
     if len(targets) != 1:
-        raise ArgumentError(f"the two-level mesh refines exactly the one root block holding {xref}, but "
-                            f"{len(targets)} blocks claim it")
+        raise ArgumentError(f"gh_forest(refined=True): {len(targets)} root blocks contain the "
+                            f"point {xref} that selects the refined block; expected exactly one")
     f.refine(targets)
     return f.balance()
 
 
-def _box_meets_ball(ext, c, R):
-    d2 = sum(max(ext[d][0] - c[d], 0.0, c[d] - ext[d][1]) ** 2 for d in range(3))
-    return d2 <= R * R
-
-
-def _box_meets_cube(ext, c, R):
-    return all(ext[d][0] < c[d] + R and ext[d][1] > c[d] - R for d in range(3))
-
-
-def _box_meets_capsule(ext, c, dc, R):
-    # the distance from c + s dc to the box is convex in s: ternary search
-    def dist2(s):
-        return sum(max(ext[d][0] - (c[d] + s * dc[d]), 0.0, (c[d] + s * dc[d]) - ext[d][1]) ** 2
-                   for d in range(3))
-    a, b = 0.0, 1.0
-    for _ in range(200):
-        m1, m2 = a + (b - a) / 3, b - (b - a) / 3
-        if dist2(m1) <= dist2(m2):
-            b = m2
-        else:
-            a = m1
-    return min(dist2(0.0), dist2(1.0), dist2((a + b) / 2)) <= R * R
-
-
-def _box_meets_swept_cube(ext, c, dc, R):
-    # some s in [0, 1] with lo_d < c_d + s dc_d + R and hi_d > c_d + s dc_d - R
-    # on every axis: an intersection of open intervals in s
-    lo, hi = 0.0, 1.0
-    lo_open = hi_open = False
+def _segment_box_dist2(lo, hi, c, dc):
+    """The squared distance from the segment c + s dc, s in [0, 1], to each box
+    [lo, hi] (arrays n x 3): the per-axis excess max(lo - x, 0, x - hi) is
+    piecewise linear in s, so its square summed over axes is a convex
+    piecewise quadratic; its minimum is at a breakpoint (where x_d(s) crosses
+    lo_d or hi_d), at an end, or at the vertex of the quadratic on a piece."""
+    import numpy as np
+    n = lo.shape[0]
+    cand = [np.zeros(n), np.ones(n)]
     for d in range(3):
-        a, b = ext[d][0] - c[d] - R, ext[d][1] - c[d] + R  # need a < s dc < b
+        if dc[d] != 0.0:
+            for b in (lo[:, d], hi[:, d]):
+                cand.append(np.clip((b - c[d]) / dc[d], 0.0, 1.0))
+    S = np.sort(np.stack(cand, 1), 1)  # breakpoints per box
+    def f(s):
+        x = c[None, None, :] + s[..., None] * dc[None, None, :]
+        e = np.maximum(np.maximum(lo[:, None, :] - x, 0.0), x - hi[:, None, :])
+        return (e * e).sum(-1)
+    best = f(S).min(1)
+    # inside each piece [S_k, S_k+1] the active sides are fixed: the quadratic's
+    # vertex from the excess's slope and offset at the piece's midpoint
+    mid = 0.5 * (S[:, 1:] + S[:, :-1])
+    x = c[None, None, :] + mid[..., None] * dc[None, None, :]
+    below = x < lo[:, None, :]
+    above = x > hi[:, None, :]
+    # excess = a + b s on the active axes: a = lo - c (below) or c - hi (above), b = -dc or dc
+    a = np.where(below, lo[:, None, :] - c, np.where(above, c - hi[:, None, :], 0.0))
+    b = np.where(below, -dc, np.where(above, dc, 0.0))
+    bb = (b * b).sum(-1)
+    ab = (a * b).sum(-1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sv = np.where(bb > 0, -ab / bb, mid)
+    sv = np.clip(sv, S[:, :-1], S[:, 1:])
+    best = np.minimum(best, f(sv).min(1))
+    return best
+
+def _boxes_meet_shell(lo, hi, c, R, shape, dc):
+    """Which boxes [lo, hi] (n x 3 arrays) meet the shell of radius R about c:
+    the closed ball, the open cube, or either swept along [c, c + dc]."""
+    import numpy as np
+    c = np.asarray(c, float)
+    if dc is None:
+        if shape == "ball":
+            e = np.maximum(np.maximum(lo - c, 0.0), c - hi)
+            return (e * e).sum(1) <= R * R
+        return np.all((lo < c + R) & (hi > c - R), axis=1)
+    dc = np.asarray(dc, float)
+    if shape == "ball":
+        return _segment_box_dist2(lo, hi, c, dc) <= R * R
+    # the union of open cubes about c + s dc: per axis the s where the box's
+    # [lo, hi] overlaps (c + s dc - R, c + s dc + R), an open interval in s
+    s_lo = np.zeros(lo.shape[0])
+    s_hi = np.ones(lo.shape[0])
+    ok = np.ones(lo.shape[0], bool)
+    for d in range(3):
+        a, b = lo[:, d] - R - c[d], hi[:, d] + R - c[d]  # need a < s dc < b
         if dc[d] == 0.0:
-            if not (a < 0.0 < b):
-                return False
-            continue
-        s0, s1 = sorted((a / dc[d], b / dc[d]))
-        if s0 >= lo:
-            lo, lo_open = s0, True
-        if s1 <= hi:
-            hi, hi_open = s1, True
-    return lo < hi or (lo == hi and not (lo_open or hi_open))
+            ok &= (a < 0.0) & (0.0 < b)
+        else:
+            t0, t1 = np.minimum(a / dc[d], b / dc[d]), np.maximum(a / dc[d], b / dc[d])
+            s_lo = np.maximum(s_lo, t0)
+            s_hi = np.minimum(s_hi, t1)
+    return ok & (s_lo < s_hi)
 
 
-def hole_forest(T=None, case=None, *, N, roots, center=None, radii, levels=None, shape="ball", path=None):
+def hole_forest(T=None, case=None, *, N, roots, center=None, radii, levels=None, shape="ball", path=None,
+                ownership="closed"):
     """hole_forest(T, case; N, roots, center, radii, levels, shape, path)
-    (src/initialdata.jl:1469-1515): the uniform forest, then for each level
-    every leaf of the level below that meets shell radii[l] about center (a
-    closed ball, an open cube, or either swept along [center, center + path])
-    refined, and the forest balanced after each level."""
+    (src/initialdata.jl:1417-1510): a fixed nested hierarchy about `center`
+    (the case's hole by default). Starting from the uniform forest, level l
+    refines every level-(l − 1) leaf whose box meets shell l of radius
+    radii[l] -- the closed ball |x − c| ≤ R (shape "ball") or the open cube
+    |x − c|_∞ < R ("cube"), each swept along the segment [c, c + path] when a
+    path is given -- and 2:1 balance follows each level. The layout depends on
+    roots and radii only, so raising N halves every spacing and moves no
+    block."""
+    import numpy as np
     _check_float(T)
     if shape not in ("ball", "cube"):
-        raise ArgumentError(f"a shell is a :ball (|x − c| ≤ R) or a :cube (|x − c|_∞ < R), got :{shape}")
+        raise ArgumentError(f"hole_forest: shape = :{shape}; the shells are \"ball\" or \"cube\"")
     if path is not None and len(path) != 3:
-        raise ArgumentError(f"path is None or the displacement Δc of the segment [c, c + Δc], three numbers, "
-                            f"got {path!r}")
+        raise ArgumentError(f"hole_forest: path = {path!r}; give None or a displacement of "
+                            f"three numbers")
     radii = [float(r) for r in radii]
     levels = len(radii) if levels is None else int(levels)
     if len(radii) != levels:
-        raise ArgumentError(f"hole_forest takes one radius per refinement level, but levels = {levels} and "
-                            f"radii has {len(radii)} entries")
-    if not all(r > 0 for r in radii):
-        raise ArgumentError(f"every shell radius must be positive, got radii = {radii}")
-    if not all(radii[l] <= radii[l - 1] for l in range(1, levels)):
-        raise ArgumentError(f"the shells must nest, so the radii must not increase with the level, but "
-                            f"radii = {radii}")
+        raise ArgumentError(f"hole_forest: {len(radii)} radii for {levels} levels; one "
+                            f"radius refines one level")
+    if min(radii) <= 0:
+        raise ArgumentError(f"hole_forest: radii = {radii} include a non-positive one")
+    if any(b > a for a, b in zip(radii, radii[1:])):
+        raise ArgumentError(f"hole_forest: radii = {radii} grow somewhere; a finer level's "
+                            f"shell lies inside the coarser one's")
     if center is None:
         center = case.interior.center if case.interior is not None else (0.0, 0.0, 0.0)
-    c = [float(x) for x in center]
-    f = Forest(case, N, roots)
-    for l in range(1, levels + 1):
-        R = radii[l - 1]
-        targets = []
-        for lf in f.leaves:
-            if lf[0] != l - 1:
-                continue
-            ext = f.block_extent(lf)
-            if path is None:
-                hit = _box_meets_ball(ext, c, R) if shape == "ball" else _box_meets_cube(ext, c, R)
-            else:
-                dc = [float(x) for x in path]
-                hit = _box_meets_capsule(ext, c, dc, R) if shape == "ball" else _box_meets_swept_cube(ext, c, dc, R)
-            if hit:
-                targets.append(lf)
+    c = np.asarray(center, dtype=float)
+    dc = None if path is None else np.asarray(path, dtype=float)
+    f = Forest(case, N, roots, ownership=ownership)
+    for l, R in enumerate(radii, start=1):
+        cand = [lf for lf in f.leaves if lf[0] == l - 1]
+        ext = np.array([f.block_extent(lf) for lf in cand])  # leaves x axis x (lo, hi)
+        hit = _boxes_meet_shell(ext[:, :, 0], ext[:, :, 1], c, R, shape, dc) if cand else []
+        targets = [lf for lf, h in zip(cand, hit) if h]
         if not targets:
-            raise ArgumentError(f"no leaf of level {l - 1} comes within radii[{l}] = {R} of {c}, so shell {l} "
-                                f"would refine nothing")
+            raise ArgumentError(f"hole_forest: shell {l} (radius {R} about {tuple(c)}) meets no "
+                                f"level-{l - 1} leaf, so it would add no level")
         f.refine(targets)
         f.balance()
     return f
@@ -868,6 +928,7 @@ def _desc(forest, case, q):
          d.r_gate) = (bd.alpha_min, bd.alpha_max, bd.lambda_min, bd.lambda_max, bd.beta_max, bd.K_max,
                       bd.r_gate)
     d.velocity[:] = list(getattr(case, "velocity", (0.0, 0.0, 0.0)))
+    d.ownership = Forest.OWNERSHIP[getattr(forest, "ownership", "closed")]
     it = case.interior
     if it is not None:
         d.interior = Interior.VARIANTS[it.variant]
@@ -887,19 +948,20 @@ class GHProblem:
     def __init__(self, forest, case, *, q, backend=None, ops=None, target=None, t_target=0.0,
                  target_rate=False, trail=0.0):
         if not (q >= 2 and q % 2 == 0):
-            raise ArgumentError(f"the finite-difference order must be even and at least 2, so "
-                                f"that the centered stencils have an integer half-width q/2, but q={q}")
+            raise ArgumentError(f"GHProblem: q = {q}; the centred stencils are of even order "
+                                f"2, 4, 6 or 8")
         self.backend = backend if backend is not None else MI355X()
         self.forest, self.case, self.q, self.ops = forest, case, int(q), ops
         self.desc = _desc(forest, case, q)
         self._ptr = C.c_void_p()
         import torch
-        if forest.refined and ops is None:
-            raise ArgumentError(f"a forest with levels needs its ghost operators spelled out, "
-                                f"ops = Operators(prolongation = p): the interface-order rule asks for "
-                                f"p = q + 2 = {q + 2}, and it is never defaulted")
+        planned = forest.refined or forest.ownership != "closed"
+        if planned and ops is None:
+            raise ArgumentError(f"a forest with levels (or half-open blocks) takes its ghost operators "
+                                f"from the caller, ops = Operators(prolongation = p): the interface rule "
+                                f"wants p = q + 2 = {q + 2}, and this library does not pick it")
         with torch.cuda.device(self.backend.device):
-            if forest.refined:
+            if planned:
                 leaves = forest.leaf_array()
                 arr = leaves.ravel()
                 _check(_load().tgh_problem_create_forest(
@@ -915,9 +977,9 @@ class GHProblem:
         self.target = None
         if target is not None or (case.interior is not None and case.interior.variant == "fitted"):
             if target is None:
-                raise ArgumentError("a :fitted interior relaxes toward a target the kernel reads from a "
-                                    "cache: give GHProblem target = a device array of 40 variables per "
-                                    "owned point (A then S)")
+                raise ArgumentError("the :fitted variant reads its target from a cache the caller supplies: "
+                                    "pass GHProblem(..., target=...) with a device array of 40 variables "
+                                    "per owned point (A then S)")
             self.set_target(target, t_target, rate=target_rate, trail=trail)
 
     def set_target(self, target, t_target, *, rate=False, trail=0.0):
@@ -1046,12 +1108,20 @@ def evolved_sets(p):
     return (p,)
 
 
-def error_norms(p, u, t):
-    """(L2, L∞) of the masked error u − u_exact(t) over evolved points
-    (src/constraints.jl:311-396,655-671)."""
-    l2, li = C.c_double(), C.c_double()
-    _check(_load().tgh_error_norms(p.ptr, _dev(u, p, "u"), float(t), C.byref(l2), C.byref(li)))
-    return l2.value, li.value
+def error_norms(p, u, t, *, shell=None):
+    """gh_error!(p, u, t; shell) + error_norms(p) (src/constraints.jl:311-396,
+    629-671). Without `shell`: (err_l2, err_linf), the masked volume-weighted
+    L2 and the L-inf of e = |u − u_exact(t)| (the 2-norm over the 20
+    variables) over evolved primaries. With `shell = (lo, hi)`: the record's
+    four rows as a dict, err_l2, err_linf, residual (the layer's L-inf of e)
+    and drift (the L-inf of |h_tt − h_tt,exact| over evolved primaries with
+    lo ≤ r ≤ hi)."""
+    out = (C.c_double * 4)()
+    lo, hi = (0.0, -1.0) if shell is None else (float(shell[0]), float(shell[1]))
+    _check(_load().tgh_error_rows(p.ptr, _dev(u, p, "u"), float(t), lo, hi, out))
+    if shell is None:
+        return out[0], out[1]
+    return {"err_l2": out[0], "err_linf": out[1], "residual": out[2], "drift": out[3]}
 
 
 TGH_PI_TILES = 6
@@ -1110,27 +1180,139 @@ def transfer(p_from, u_from, t, p_to, u_to):
     return u_to
 
 
-def regrid_flags(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=None):
-    """refine_flags (src/refinement.jl:738-795) on the Löhner indicator's block
-    maxima: per leaf of p.leaves() +1 (refine: τ above refine_tol below the
-    cap), 0 (keep: τ above coarsen_tol) or -1 (coarsen, above level 0)."""
-    if not (0 <= coarsen_tol < refine_tol):
-        raise ArgumentError(f"the tolerances need 0 <= coarsen_tol < refine_tol, got "
-                            f"coarsen_tol={coarsen_tol}, refine_tol={refine_tol}")
-    if not 0 <= maxlevel_cap <= 10:
-        raise ArgumentError(f"maxlevel_cap must lie in 0 .. 10, got {maxlevel_cap}")
+class Refinement:
+    """Refinement(T; refine_tol, coarsen_tol, maxlevel_cap, floor_margin,
+    ceiling_cells = 1, ceiling_level = 0, ε = 1/100) (src/refinement.jl:
+    93-134): the regrid's thresholds on the Löhner indicator τ ∈ [0, 1], the
+    level cap, how far past the horizon's largest radius the level floor
+    reaches, and the ceiling: blocks within ceiling_cells root spacings of a
+    non-periodic face stay at ceiling_level."""
+
+    def __init__(self, T=None, *, refine_tol, coarsen_tol, maxlevel_cap, floor_margin, ceiling_cells=1,
+                 ceiling_level=0, eps=0.01):
+        _check_float(T)
+        rtol, ctol = float(refine_tol), float(coarsen_tol)
+        if not 0 < ctol < rtol:
+            raise ArgumentError(f"the two thresholds need 0 < coarsen_tol < refine_tol (the band between "
+                                f"them keeps a block at the threshold from refining and coarsening in "
+                                f"turn), got coarsen_tol = {ctol}, refine_tol = {rtol}")
+        if not rtol < 1:
+            raise ArgumentError(f"τ never exceeds 1, so refine_tol = {rtol} would never refine anything; "
+                                f"pick a threshold below 1")
+        if int(maxlevel_cap) != maxlevel_cap or not 0 <= maxlevel_cap <= 10:
+            raise ArgumentError(f"maxlevel_cap is a level in 0 .. 10, got {maxlevel_cap}")
+        if not float(floor_margin) >= 0:
+            raise ArgumentError(f"the floor's shell ends floor_margin beyond the horizon's largest radius, "
+                                f"which needs floor_margin ≥ 0, got {floor_margin}")
+        if int(ceiling_cells) != ceiling_cells or ceiling_cells < 0:
+            raise ArgumentError(f"ceiling_cells counts root spacings from the outer faces and must be a "
+                                f"non-negative integer, got {ceiling_cells}")
+        if int(ceiling_level) != ceiling_level or not 0 <= ceiling_level <= maxlevel_cap:
+            raise ArgumentError(f"ceiling_level must lie in 0 .. maxlevel_cap = {maxlevel_cap}, got "
+                                f"{ceiling_level}")
+        if not float(eps) > 0:
+            raise ArgumentError(f"the indicator's noise floor ε must be positive, got {eps}")
+        self.refine_tol, self.coarsen_tol = rtol, ctol
+        self.maxlevel_cap, self.floor_margin = int(maxlevel_cap), float(floor_margin)
+        self.ceiling_cells, self.ceiling_level, self.eps = int(ceiling_cells), int(ceiling_level), float(eps)
+
+
+def _box_radii(ext, c):
+    """The nearest and farthest distance from c to the box ext."""
+    near = sum(max(ext[d][0] - c[d], 0.0, c[d] - ext[d][1]) ** 2 for d in range(3)) ** 0.5
+    far = sum(max(abs(ext[d][0] - c[d]), abs(ext[d][1] - c[d])) ** 2 for d in range(3)) ** 0.5
+    return near, far
+
+
+def level_bounds(forest, case, q, refinement, t, leaves=None):
+    """block_level_bounds over `leaves` (forest.leaves by default;
+    src/refinement.jl:430-668): per
+    leaf (floor, cap). The floor is the coarsest level whose spacing meets
+    both interior requirements, h ≤ (r_h,min − r_1)/m and h ≤ (r_1 −
+    r_0)/(2(G+1)) (for a tracked geometry h ≤ offset/m and thickness/(2(G+1))),
+    on the leaves meeting the shell from r_1 (the tracked core surface) to the
+    horizon's largest radius plus floor_margin about the hole's center at t;
+    the cap is ceiling_level within ceiling_cells root spacings of a
+    non-periodic face and maxlevel_cap elsewhere. A floor the cap forbids is
+    an error."""
+    ref, f = refinement, forest
+    it = case.interior
+    G = q // 2 + 1
+    h0 = f.root_spacing()
+    lo, hi, L = 0.0, -1.0, 0
+    c = (0.0, 0.0, 0.0)
+    if it is not None:
+        vel = getattr(case, "velocity", (0.0, 0.0, 0.0))
+        c = tuple(it.center[d] + vel[d] * t for d in range(3))
+        if isinstance(it, FittedInterior):
+            need = min(it.offset / it.margin, it.thickness / (2 * (G + 1))) * (1 + 8 * 2.0 ** -52)
+            lo, hi = it.r_0, it.r_out + ref.floor_margin
+        else:
+            rmin, rmax = horizon_min_radius(case.background), horizon_max_radius(case.background)
+            if not rmin > it.r_1:
+                raise ArgumentError(f"r_1 = {it.r_1} is not inside the horizon's smallest radius {rmin}: "
+                                    f"no level can keep m = {it.margin} spacings between them")
+            need = min((rmin - it.r_1) / it.margin, (it.r_1 - it.r_0) / (2 * (G + 1)))
+            lo, hi = it.r_1, rmax + ref.floor_margin
+        h = h0
+        while h > need and L < 10:
+            L, h = L + 1, h / 2
+        if L > ref.maxlevel_cap:
+            raise ArgumentError(f"the interior needs level {L} around the hole (h ≤ {need:.6g} at root "
+                                f"spacing {h0:.6g}), above maxlevel_cap = {ref.maxlevel_cap}: raise the "
+                                f"cap, refine the root brick or move the layer outward")
+    margin = ref.ceiling_cells * h0
+    out = []
+    for lf in (f.leaves if leaves is None else leaves):
+        ext = f.block_extent(lf)
+        near, far = _box_radii(ext, c)
+        flo = L if (L > 0 and hi >= lo and near <= hi and far >= lo) else 0
+        capped = any(not case.periodic[d] and (ext[d][0] - case.box[d][0] <= margin or
+                                               case.box[d][1] - ext[d][1] <= margin) for d in range(3))
+        cap = ref.ceiling_level if capped else ref.maxlevel_cap
+        if flo > cap:
+            raise ArgumentError(f"leaf {lf} meets the horizon shell [{lo:.6g}, {hi:.6g}], which wants level "
+                                f"{flo}, and lies within {margin:.6g} of the outer boundary, which caps it at "
+                                f"{cap}: widen the box, narrow floor_margin or ceiling_cells")
+        out.append((flo, cap))
+    return out
+
+
+def refine_flags(leaves, tau, bounds, refinement):
+    """refine_flags (src/refinement.jl:670-730) on per-leaf indicator maxima:
+    each leaf asks for one level finer where τ passes refine_tol below its
+    cap, its own level where τ passes coarsen_tol, one coarser elsewhere; the
+    request is clamped into the leaf's (floor, cap) and returned as +1
+    (refine), 0 (keep) or -1 (coarsen)."""
+    ref = refinement
+    if not len(leaves) == len(tau) == len(bounds):
+        raise ArgumentError(f"one τ and one (floor, cap) per leaf: {len(leaves)} leaves, {len(tau)} τ, "
+                            f"{len(bounds)} bounds")
+    flags = []
+    for lf, tb, (flo, cap) in zip(leaves, tau, bounds):
+        l = lf[0]
+        if tb > ref.refine_tol and l < cap:
+            want = l + 1
+        elif tb > ref.coarsen_tol:
+            want = l
+        else:
+            want = max(l - 1, 0)
+        want = min(max(want, flo), cap)
+        flags.append((want > l) - (want < l))
+    return flags
+
+
+def regrid_flags(p, u, t, *, refinement, scale=None):
+    """The regrid's marking on the device state: the Löhner indicator's block
+    maxima (scaled by the field scale of u unless `scale` is given) through
+    refine_flags with the problem's level_bounds at t. Returns (flags, τ per
+    leaf of p.leaves())."""
+    leaves = p.leaves()
+    bounds = level_bounds(p.forest, p.case, p.q, refinement, t, leaves)
     if scale is None:
         scale = field_scale(p, u, t)
-    tau = refinement_indicator(p, u, t, scale=scale, eps=eps)
-    flags = []
-    for lf, tb in zip(p.leaves(), tau):
-        if tb > refine_tol and lf[0] < maxlevel_cap:
-            flags.append(1)
-        elif tb > coarsen_tol or lf[0] == 0:
-            flags.append(0)
-        else:
-            flags.append(-1)
-    return flags, tau
+    tau = refinement_indicator(p, u, t, scale=scale, eps=refinement.eps)
+    return refine_flags(leaves, tau, bounds, refinement), tau
 
 
 def regrid_forest(forest, leaves, flags):
@@ -1142,7 +1324,7 @@ def regrid_forest(forest, leaves, flags):
     the children's level (so the parent stays balanced)."""
     if len(flags) != len(leaves):
         raise ArgumentError(f"one flag per leaf: {len(flags)} flags for {len(leaves)} leaves")
-    f = Forest(forest.case, forest.N, forest.roots, leaves)
+    f = Forest(forest.case, forest.N, forest.roots, leaves, ownership=forest.ownership)
     f.refine([lf for lf, fl in zip(leaves, flags) if fl > 0])
     f.balance()
     want = {lf for lf, fl in zip(leaves, flags) if fl < 0}
@@ -1159,18 +1341,20 @@ def regrid_forest(forest, leaves, flags):
     return f
 
 
-def regrid(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=None):
+def regrid(p, u, t, *, refinement, scale=None):
     """regrid!(forest, U; flags) with the flags of regrid_flags: flagged leaves
     refine once, sibling groups all flagged to coarsen merge where the parent
-    stays 2:1 balanced, balance! closes the rest, and the state moves to the
-    new leaves (transfer). Returns (the new problem, its state, the flags); a
-    regrid that leaves the mesh as it was returns p and u themselves."""
+    stays 2:1 balanced, balance! closes the rest, the state moves to the new
+    leaves (transfer), and the stage limiter projects it as the driver does
+    after a regrid (src/driver.jl:1692-1700: prolongation into a new leaf can
+    leave a point outside the state bounds). Returns (the new problem, its
+    state, the flags); a regrid that leaves the mesh as it was returns p and
+    u themselves."""
     if p.target is not None:
         raise ArgumentError("a :fitted problem relaxes toward a target cache that the fit computes on its "
                             "own leaves; after a regrid the fit must fill the new mesh's cache, and the "
                             "fit is not built here: regrid a problem without a target")
-    flags, _ = regrid_flags(p, u, t, refine_tol=refine_tol, coarsen_tol=coarsen_tol,
-                            maxlevel_cap=maxlevel_cap, eps=eps, scale=scale)
+    flags, _ = regrid_flags(p, u, t, refinement=refinement, scale=scale)
     leaves = p.leaves()
     f = regrid_forest(p.forest, leaves, flags)
     if set(f.leaves) == set(leaves):  # the mesh as it was: keep the problem
@@ -1179,6 +1363,7 @@ def regrid(p, u, t, *, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01, scale=No
                     ops=p.ops if p.ops is not None else Operators(prolongation=p.q + 2))
     u_new = new.statevector()
     transfer(p, u, t, new, u_new)
+    stage_limit(u_new, new, t)
     return new, u_new, flags
 
 
@@ -1216,30 +1401,139 @@ def gh_solve(p, u, tspan, *, dt, graph=False):
     return u
 
 
-def gh_solve_regrid(p, u, tspan, *, dt, every, refine_tol, coarsen_tol, maxlevel_cap, eps=0.01,
-                    graph=False):
-    """The driver's regrid cadence (src/driver.jl:1640-1700): gh_solve over
-    chunks of `every` steps of the one step size gh_solve would take over
-    tspan, each chunk but the last followed by regrid at its end time (the
-    hook's t). Returns (the final problem, its state, [(t, nblocks)] per
-    regrid)."""
+def set_rho_max(p, rho_max):
+    """with_interior(p, with_ρ_max(p.interior, ρ)): the layer's relaxation rate
+    of the problem, nothing else changed. Returns p."""
+    _check(_load().tgh_set_rho_max(p.ptr, float(rho_max)))
+    if p.case.interior is not None:
+        p.case = with_interior(p.case, p.case.interior.with_rho_max(float(rho_max)))
+    return p
+
+
+ADJOINT_PARAMS = ("gamma0_near", "gamma0_far", "gamma2", "rho_max", "rho_ramp")
+
+
+def vjp_rhs(p, u, t, du_bar):
+    """The vector-Jacobian product of gh_rhs! (src/evolution.jl:1327) at (u, t):
+    (u_bar, theta_bar) = ((dF/du)^T du_bar, (dF/dθ)^T du_bar) with θ =
+    ADJOINT_PARAMS (the damping's Gaussian rates, γ2, the layer's ρ_max and ρ
+    ramp). u_bar is zero at the face copies the fill overwrites."""
+    import torch
+    u_bar = torch.empty_like(u)
+    th = (C.c_double * 5)()
+    _check(_load().tgh_vjp_rhs(p.ptr, _dev(u, p, "u"), float(t), _dev(du_bar, p, "du_bar"), _dev(u_bar, p, "u_bar"), th))
+    return u_bar, list(th)
+
+
+def adjoint(p, u, t0, dt, nsteps, *, every):
+    """J and dJ/dθ for J = the masked L2 error after nsteps RK4 steps of dt from
+    u (in place: u ends at T = t0 + nsteps dt), θ = ADJOINT_PARAMS: the forward
+    sweep keeps a checkpoint every `every` steps, the reverse sweep recomputes
+    each segment's stages from its checkpoint and applies the transposed
+    right-hand side and RK4 combine. Returns (J, grad, (forward_ms, reverse_ms))."""
+    J = C.c_double()
+    g = (C.c_double * 5)()
+    ms = (C.c_double * 2)()
+    _check(_load().tgh_adjoint(p.ptr, _dev(u, p, "u"), float(t0), float(dt), int(nsteps), int(every), C.byref(J), g, ms))
+    return J.value, list(g), (ms[0], ms[1])
+
+
+def chunk_count(t_end, chunk):
+    """chunk_count (src/precision.jl:81-87): ⌈t_end / chunk⌉, except that a
+    quotient within a few ulp of an integer m gives m."""
+    r = t_end / chunk
+    m = round(r)
+    tol = 4 * (math.ulp(r) + math.ulp(t_end) / chunk)
+    return int(m) if m >= 1 and abs(r - m) <= tol else math.ceil(r)
+
+
+def chunk_bounds(c, nchunks, t_end, chunk):
+    """chunk_bounds (src/precision.jl:96-100): chunk c (1-based) spans
+    (c − 1)·chunk to c·chunk, both capped at t_end, the last ending at t_end."""
+    return min((c - 1) * chunk, t_end), (t_end if c == nchunks else min(c * chunk, t_end))
+
+
+def evolve(p, u, *, t_end, chunk, cfl=0.25, rho_max_factor=None, rho_max_fixed=None, graph=True):
+    """evolve!'s chunk loop for an analytic case on a frozen mesh
+    (src/driver.jl:1732-1960, the record row :1540-1560): for each chunk
+    (chunk_count / chunk_bounds) the largest speed λ over the evolved points
+    at the chunk's start, dt = cfl · h_min / λ, steps = ⌈span / dt⌉ and the
+    step span / steps; the layer's rate for the chunk (chunk_interior,
+    :2391-2411): ρ_max = rho_max_factor / dt, else rho_max_fixed, else the
+    case's own (hole_case's default 4/M), a fixed rate refused above 1/dt;
+    then the steps, and at the chunk's end the record: t, dt, steps, λ, the
+    error rows (gh_error! with the horizon shell) and the gauge constraint's
+    largest component norms. The state starts as u holds it at t = 0 (the
+    caller's initial_state). Holes at rest only: a moving hole's step takes a
+    growth factor this loop does not model. Returns the records."""
+    if rho_max_factor is not None and rho_max_fixed is not None:
+        raise ArgumentError(f"evolve was given both rho_max_factor = {rho_max_factor} and rho_max_fixed = "
+                            f"{rho_max_fixed}: the layer relaxes at the grid rate factor/dt or at a fixed "
+                            f"rate, one or the other")
+    case = p.case
+    if any(getattr(case, "velocity", (0.0, 0.0, 0.0))):
+        raise ArgumentError("evolve's chunk loop is built for holes at rest; a moving hole's time step "
+                            "carries a growth factor of its own")
+    it = case.interior
+    shell = (0.0, -1.0)
+    if it is not None and isinstance(case.background, (KerrSchild, Harmonic)) and not isinstance(it, FittedInterior):
+        shell = (horizon_min_radius(case.background), horizon_max_radius(case.background) + (it.r_1 - it.r_0))
+    hmin = p.forest.minimum_spacing()
+    nchunks = chunk_count(float(t_end), float(chunk))
+    records = []
+    for c in range(1, nchunks + 1):
+        tstart, stop = chunk_bounds(c, nchunks, float(t_end), float(chunk))
+        lam = max_speed(p, u, tstart)
+        dt = cfl * hmin / lam
+        steps = max(1, math.ceil((stop - tstart) / dt))
+        dt_used = (stop - tstart) / steps
+        if it is not None:
+            if rho_max_factor is not None:
+                set_rho_max(p, rho_max_factor / dt_used)
+            else:
+                fixed = rho_max_fixed if rho_max_fixed is not None else it.rho_max
+                if not fixed * dt_used <= 1:
+                    raise ArgumentError(f"ρ_max = {fixed} exceeds this chunk's "
+                                        f"grid rate 1/dt = {1 / dt_used}: relax at most once per step "
+                                        f"(refine the mesh, lower the rate, or give rho_max_factor)")
+                if fixed != p.case.interior.rho_max:
+                    set_rho_max(p, fixed)
+        _check(_load().tgh_rk4(p.ptr, _dev(u, p, "u"), tstart, dt_used, steps, TGH_RK4_GRAPH if graph else 0))
+        rows = error_norms(p, u, stop, shell=shell)
+        gl2, gli = gauge_constraint_norms(p, u, stop)
+        import torch
+        records.append({"t": stop, "dt": dt_used, "steps": steps, "lambda": lam, **rows,
+                        "gauge_l2": max(gl2), "gauge_linf": max(gli),
+                        "rho_max": p.case.interior.rho_max if it is not None else None,
+                        "finite": bool(torch.isfinite(u).all())})
+    return records
+
+
+def gh_solve_regrid(p, u, tspan, *, cfl, every, refinement, graph=False):
+    """The driver's regrid cadence (src/driver.jl:1640-1700, 1786-1800): chunks
+    of `every` RK4 steps, each chunk's step sized on its own mesh as the
+    driver sizes it, dt = cfl · h_min / λ (λ the largest speed over the
+    evolved points at the chunk's start), and every chunk but the last
+    followed by regrid at its end time. The last chunk takes ⌈rest / dt⌉
+    steps of rest / steps to end at tspan[1]. Returns (the final problem, its
+    state, [(t, nblocks, dt)] per regrid)."""
     if every < 1:
         raise ArgumentError(f"the regrid cadence is a positive number of steps, got {every}")
-    t0, t1 = float(tspan[0]), float(tspan[1])
-    nsteps = max(1, math.ceil((t1 - t0) / dt * (1 - 4 * 2.0 ** -52)))
-    step = (t1 - t0) / nsteps
-    done, log = 0, []
-    while done < nsteps:
-        k = min(every, nsteps - done)
-        t = t0 + done * step
+    t, t1 = float(tspan[0]), float(tspan[1])
+    log = []
+    while True:
+        dt = gh_dt(p, u, cfl=cfl, t=t)
+        rest = t1 - t
+        n = max(1, math.ceil(rest / dt * (1 - 4 * 2.0 ** -52)))
+        last = n <= every
+        step = rest / n if last else dt
+        k = n if last else every
         _check(_load().tgh_rk4(p.ptr, _dev(u, p, "u"), t, step, k, TGH_RK4_GRAPH if graph else 0))
-        done += k
-        if done < nsteps:
-            t = t0 + done * step
-            p, u, _ = regrid(p, u, t, refine_tol=refine_tol, coarsen_tol=coarsen_tol,
-                             maxlevel_cap=maxlevel_cap, eps=eps)
-            log.append((t, p.nblocks))
-    return p, u, log
+        if last:
+            return p, u, log
+        t = t + k * step
+        p, u, _ = regrid(p, u, t, refinement=refinement)
+        log.append((t, p.nblocks, step))
 
 
 class GHBatch:

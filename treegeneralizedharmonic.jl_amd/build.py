@@ -10,13 +10,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libtgh_mi355x.so")
-# The API and pointwise kernels, and the right-hand side at each order in a
-# translation unit of its own: the five compile in parallel.
+# The API and pointwise kernels, the right-hand side at each order and the
+# adjoint, each in a translation unit of its own: the six compile in parallel.
 SOURCES = [os.path.join(HERE, "csrc", f) for f in
-           ("tgh_api.hip", "tgh_q2.hip", "tgh_q4.hip", "tgh_q6.hip", "tgh_q8.hip")]
-DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in
-                  ("tgh_device.hpp", "tgh_kernels.hpp", "tgh_problem.hpp", "tgh_launch.hpp")] + \
-    [os.path.join(ROOT, "include", "tgh.h")]
+           ("tgh_api.hip", "tgh_q2.hip", "tgh_q4.hip", "tgh_q6.hip", "tgh_q8.hip", "tgh_adjoint.hip")]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",

@@ -14,6 +14,7 @@
 
 #include "tgh_forest.hpp"
 #include "tgh_problem.hpp"
+#include "tgh_adjoint.hpp"
 
 using namespace tgh;
 
@@ -69,42 +70,42 @@ namespace {
 int check_desc(const tgh_desc* d, int lmax = 0) {
   if (!d) return fail(TGH_EINVAL, "desc is NULL");
   if (d->float_bits != 64)
-    return fail(TGH_EINVAL, "this library computes in Float64 only (the horizon interior and the "
-                            "constraint damping need it); got a float type of " +
-                                std::to_string(d->float_bits) + " bits");
+    return fail(TGH_EINVAL, "float_bits = " + std::to_string(d->float_bits) + ": the kernels compute in FP64 "
+                            "only (64)");
   if (d->q < 2 || d->q > 8 || d->q % 2)
-    return fail(TGH_EINVAL, "the finite-difference order must be even and between 2 and 8, so that the "
-                            "centered stencils have an integer half-width q/2 and the ghost width "
-                            "G = q/2 + 1 covers them, but q=" + std::to_string(d->q));
+    return fail(TGH_EINVAL, "q = " + std::to_string(d->q) + ": the centred stencils are built for the "
+                            "even orders 2, 4, 6 and 8");
   const int G = d->q / 2 + 1;
   if (d->N < 2 * G)
     return fail(TGH_EINVAL, "a closed vertex block needs N >= 2G = " + std::to_string(2 * G) +
                                 " at q=" + std::to_string(d->q) + ", got N=" + std::to_string(d->N));
   for (int i = 0; i < 3; ++i)
-    if (d->nblocks[i] < 1) return fail(TGH_EINVAL, "every dimension needs at least one block");
+    if (d->nblocks[i] < 1) return fail(TGH_EINVAL, "nblocks: each axis holds one block or more");
   double h0 = (d->hi[0] - d->lo[0]) / ((double)d->nblocks[0] * d->N);
   for (int i = 0; i < 3; ++i) {
     double hi_ = (d->hi[i] - d->lo[i]) / ((double)d->nblocks[i] * d->N);
     if (!(hi_ > 0) || std::fabs(hi_ - h0) > 1e-12 * h0)
-      return fail(TGH_EINVAL, "the blocks' cells must be cubic: the box widths over the brick of blocks "
-                              "give different spacings per axis");
+      return fail(TGH_EINVAL, "the box widths over nblocks * N give a different spacing on each "
+                              "axis; the blocks' cells are cubes");
   }
   h0 /= (double)(1 << lmax);
+  if (d->ownership != TGH_OWNERSHIP_CLOSED && d->ownership != TGH_OWNERSHIP_HALF_OPEN)
+    return fail(TGH_EINVAL, "unknown ownership of block faces");
   if (d->background < 0 || d->background > 4) return fail(TGH_EINVAL, "unknown background");
   if (d->gauge < 0 || d->gauge > 2) return fail(TGH_EINVAL, "unknown gauge source kind");
   if (d->gauge == TGH_GAUGE_SAMPLED && d->background == TGH_BG_GAUGE_WAVE)
-    return fail(TGH_EINVAL, "a time-dependent gauge source is not supported: the gauge wave is harmonic "
-                            "(H = 0) and takes no sampled source (src/gauge.jl isstatic, GHCase)");
+    return fail(TGH_EINVAL, "the gauge wave is a harmonic background (H_a = 0); a sampled source "
+                            "is for static non-harmonic backgrounds only");
   if (d->gauge == TGH_GAUGE_KERR_SCHILD && d->background != TGH_BG_KERR_SCHILD)
-    return fail(TGH_EINVAL, "the closed-form Kerr-Schild gauge source reproduces H_a = -Gamma_a of a "
-                            "Kerr-Schild background only");
+    return fail(TGH_EINVAL, "TGH_GAUGE_KERR_SCHILD is the Kerr-Schild hole's own H_a; this "
+                            "background is not Kerr-Schild");
   if (d->gauge == TGH_GAUGE_SAMPLED && d->background == TGH_BG_HARMONIC)
-    return fail(TGH_EINVAL, "the harmonic Kerr chart has H_a = 0 exactly, so it takes no gauge source "
-                            "(src/gauge.jl isharmonic, src/initialdata.jl harmonic_kerr_case)");
+    return fail(TGH_EINVAL, "TGH_BG_HARMONIC needs TGH_GAUGE_NONE: its chart is harmonic, H_a "
+                            "vanishes identically");
   if (d->gauge == TGH_GAUGE_NONE &&
       (d->background == TGH_BG_KERR_SCHILD || d->background == TGH_BG_SHIFTED_MINKOWSKI))
-    return fail(TGH_EINVAL, "this background is not harmonic, so it needs a gauge source: H_a = 0 "
-                            "would evolve it with a wrong gauge (src/gauge.jl isharmonic)");
+    return fail(TGH_EINVAL, "TGH_GAUGE_NONE on a non-harmonic background: its coordinates have "
+                            "H_a = -Gamma_a != 0, so give it a sampled or closed-form source");
   if (d->interior < 0 || d->interior > 4) return fail(TGH_EINVAL, "unknown interior variant");
   if (d->shaped) {
     // FittedInterior's checks (src/interior.jl:1770-1816)
@@ -113,46 +114,47 @@ int check_desc(const tgh_desc* d, int lmax = 0) {
     if (d->interior == TGH_INTERIOR_PASTED)
       return fail(TGH_EUNSUPPORTED, "the :pasted limiter on a tracked geometry is not built");
     if (d->shape_lmax < 0 || d->shape_lmax > TGH_SHAPE_MAX_LMAX)
-      return fail(TGH_EINVAL, "the shape's degree must lie in 0 .. 8, got lmax = " + std::to_string(d->shape_lmax));
+      return fail(TGH_EINVAL, "shape_lmax = " + std::to_string(d->shape_lmax) + " is outside 0 .. 8");
     for (int i = 0; i < (d->shape_lmax + 1) * (d->shape_lmax + 1); ++i)
       if (!std::isfinite(d->shape[i])) return fail(TGH_EINVAL, "the shape's coefficients must be finite");
     if (!(0 < d->r_in && d->r_in <= d->r_out))
-      return fail(TGH_EINVAL, "the horizon's bounding radii must satisfy 0 < r_in <= r_out, got r_in = " +
-                                  std::to_string(d->r_in) + ", r_out = " + std::to_string(d->r_out));
+      return fail(TGH_EINVAL, "r_in = " + std::to_string(d->r_in) + ", r_out = " + std::to_string(d->r_out) +
+                                  ": the clamp interval of the tracked surface needs 0 < r_in <= r_out");
     if (!(d->offset > 0 && d->thickness > 0))
-      return fail(TGH_EINVAL, "the offset and the thickness are lengths and must be positive");
+      return fail(TGH_EINVAL, "offset and thickness are distances: both have to be positive");
     if (!((d->r_in - d->offset) - d->thickness > 0))
-      return fail(TGH_EINVAL, "the core surface r_h(n) - offset - thickness reaches the center");
+      return fail(TGH_EINVAL, "r_in - offset - thickness <= 0: the tracked core would reach the center");
     if (!(d->w_ramp > 0 && d->w_ramp <= 1 && d->rho_ramp > 0 && d->rho_ramp <= 1))
-      return fail(TGH_EINVAL, "the ramp fractions must lie in (0, 1]");
+      return fail(TGH_EINVAL, "w_ramp and rho_ramp are fractions of the layer, in (0, 1]");
     if (!(d->rho_max >= 0)) return fail(TGH_EINVAL, "the relaxation rate must satisfy rho_max >= 0");
     if (d->thickness < 2.0 * (G + 1) * h0)
-      return fail(TGH_EINVAL, "the layer's thickness must be at least 2(G+1)h so that no stencil of an evolved "
-                              "point reaches the core (check_interior_radii on the ray's radii)");
+      return fail(TGH_EINVAL, "thickness < 2(G+1)h: an evolved point's stencil (G points) plus the "
+                              "ramps would reach into the tracked core");
     if ((d->background == TGH_BG_KERR_SCHILD || d->background == TGH_BG_HARMONIC) &&
         !((d->r_in - d->offset) - d->thickness > std::fabs(d->a)))
-      return fail(TGH_EINVAL, "the core surface must contain the chart's singular disk |x| <= |a|, z = 0");
+      return fail(TGH_EINVAL, "r_in - offset - thickness <= |a|: the Kerr ring singularity (the disk "
+                              "|x| <= |a| in z = 0) would lie outside the core");
     const double* v = d->velocity;
     if ((v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) && d->interior != TGH_INTERIOR_FITTED)
       return fail(TGH_EUNSUPPORTED, "a moving tracked geometry is built for the :fitted variant only");
   } else if (d->interior != TGH_INTERIOR_NONE) {
     if (!(d->r0 > 0))
-      return fail(TGH_EINVAL, "the frozen core's radius must be positive, got r_0 = " + std::to_string(d->r0));
+      return fail(TGH_EINVAL, "r0 = " + std::to_string(d->r0) + ": the core radius has to be positive");
     if (!(d->r1 > d->r0)) return fail(TGH_EINVAL, "the layer needs r_1 > r_0");
     if (!(d->w_ramp > 0 && d->w_ramp <= 1 && d->rho_ramp > 0 && d->rho_ramp <= 1))
-      return fail(TGH_EINVAL, "the ramp fractions must lie in (0, 1]");
+      return fail(TGH_EINVAL, "w_ramp and rho_ramp are fractions of the layer, in (0, 1]");
     if (!(d->rho_max >= 0)) return fail(TGH_EINVAL, "the relaxation rate must satisfy rho_max >= 0");
     // check_interior_radii (src/interior.jl:1104-1150): no stencil of an
     // evolved point reaches the core.
     if (d->r1 - d->r0 < 2.0 * (G + 1) * h0)
-      return fail(TGH_EINVAL, "r_1 - r_0 must be at least 2(G+1)h so that no stencil of an evolved point "
-                              "reaches the frozen core (check_interior_radii)");
+      return fail(TGH_EINVAL, "r1 - r0 < 2(G+1)h: an evolved point's stencil plus the ramps would "
+                              "reach into the core");
     // singular_radius (src/interior.jl:997-1010, checked at :1120-1126): both
     // Kerr charts are singular on the disk |x| <= |a|, z = 0.
     if ((d->background == TGH_BG_KERR_SCHILD || d->background == TGH_BG_HARMONIC) &&
         !(d->r0 > std::fabs(d->a)))
-      return fail(TGH_EINVAL, "the frozen core must contain the chart's singular disk |x| <= |a|, z = 0: "
-                              "r_0 > |a|");
+      return fail(TGH_EINVAL, "r0 <= |a|: the Kerr ring singularity (the disk |x| <= |a| in z = 0) "
+                              "would lie outside the core");
   }
   {
     const double* v = d->velocity;
@@ -161,19 +163,16 @@ int check_desc(const tgh_desc* d, int lmax = 0) {
     const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     if (v2 != 0.0) {
       if (!(v2 < 1.0))
-        return fail(TGH_EINVAL, "the hole's velocity must be below the speed of light, got |v| = " +
-                                    std::to_string(std::sqrt(v2)));
+        return fail(TGH_EINVAL, "|velocity| = " + std::to_string(std::sqrt(v2)) + " >= 1");
       if (d->background != TGH_BG_KERR_SCHILD && d->background != TGH_BG_HARMONIC)
-        return fail(TGH_EINVAL, "only a Kerr-Schild or harmonic hole moves (translate(boost(bg, -v), x0), "
-                                "src/initialdata.jl:1283-1298)");
+        return fail(TGH_EINVAL, "a velocity is defined for the Kerr-Schild and harmonic holes only");
       if (d->interior == TGH_INTERIOR_NONE)
         return fail(TGH_EINVAL, "a moving hole needs its interior (hole_case)");
       if (d->interior == TGH_INTERIOR_PASTED)
         return fail(TGH_EUNSUPPORTED, "the :pasted interior of a moving hole is not built");
       if (d->gauge == TGH_GAUGE_SAMPLED)
-        return fail(TGH_EINVAL, "a moving Kerr-Schild hole's gauge source depends on time, which the sampled "
-                                "source cannot represent: give it the closed-form KerrSchildSource, which moves "
-                                "with the hole (src/initialdata.jl:216-227)");
+        return fail(TGH_EINVAL, "a moving hole's H_a changes with t and a source sampled at creation "
+                                "would freeze it: use TGH_GAUGE_KERR_SCHILD");
       if (d->r_gate != 0)
         return fail(TGH_EUNSUPPORTED, "the range projection about a moving hole is not built");
     }
@@ -181,47 +180,45 @@ int check_desc(const tgh_desc* d, int lmax = 0) {
   if (d->dissipation != TGH_DISSIPATION_CONSTANT && d->dissipation != TGH_DISSIPATION_HORIZON)
     return fail(TGH_EINVAL, "unknown dissipation kind");
   if (!(d->eps_ko >= 0))
-    return fail(TGH_EINVAL, "the Kreiss-Oliger amplitude must be non-negative: the operator carries the "
-                            "damping sign in its weights");
+    return fail(TGH_EINVAL, "eps_ko < 0 would amplify the grid-scale modes (the weights already "
+                            "carry the damping sign)");
   if (d->dissipation == TGH_DISSIPATION_HORIZON) {
     // HorizonDissipation's checks (src/gauge.jl:1229-1252)
     if (!(d->eps_in >= 0))
-      return fail(TGH_EINVAL, "both Kreiss-Oliger amplitudes must be non-negative, got eps_in = " +
-                                  std::to_string(d->eps_in));
+      return fail(TGH_EINVAL, "eps_in = " + std::to_string(d->eps_in) + " is negative");
     if (!(d->eps_ko <= 4 && d->eps_in <= 4))
-      return fail(TGH_EINVAL, "the Kreiss-Oliger amplitude is capped at 4: the operator damps the lattice's "
-                              "corner mode at 3 eps/h and RK4 is stable on the negative real axis only to 2.79");
+      return fail(TGH_EINVAL, "an amplitude above 4 puts the corner mode's rate 3 eps/h beyond RK4's "
+                              "real-axis stability limit 2.79/dt at the stepper's CFL");
     if (!(0 < d->eps_r1 && d->eps_r1 < d->eps_rh))
-      return fail(TGH_EINVAL, "the profile rises from the horizon inward, so it needs 0 < r_1 < r_h, got r_1 = " +
-                                  std::to_string(d->eps_r1) + " and r_h = " + std::to_string(d->eps_rh));
+      return fail(TGH_EINVAL, "eps_r1 = " + std::to_string(d->eps_r1) + ", eps_rh = " + std::to_string(d->eps_rh) +
+                                  ": the dissipation ramp needs 0 < eps_r1 < eps_rh");
   }
   if (d->r_gate != 0) {
     // StateBounds (src/bounds.jl:94-135) and check_bounds_gate (:214-236)
     const double v[7] = {d->alpha_min, d->alpha_max, d->lambda_min, d->lambda_max, d->beta_max, d->K_max, d->r_gate};
     for (double x : v)
       if (!std::isfinite(x))
-        return fail(TGH_EINVAL, "every bound must be finite: a bound at infinity is no bound, and it is spelled "
-                                "r_gate = 0 (no range projection)");
+        return fail(TGH_EINVAL, "a range bound is not finite (no bounds at all is r_gate = 0)");
     if (!(0 < d->alpha_min && d->alpha_min < 1 && 1 < d->alpha_max))
-      return fail(TGH_EINVAL, "the lapse range needs 0 < alpha_min < 1 < alpha_max: flat space's alpha = 1 has to "
-                              "be inside the range for the non-finite repair to be idempotent");
+      return fail(TGH_EINVAL, "alpha_min/alpha_max must bracket 1 and stay positive (the repair of "
+                              "a non-finite point sets alpha = 1)");
     if (!(0 < d->lambda_min && d->lambda_min < 1 && 1 < d->lambda_max))
-      return fail(TGH_EINVAL, "the spatial metric's eigenvalue range needs 0 < lambda_min < 1 < lambda_max");
-    if (!(d->beta_max > 0)) return fail(TGH_EINVAL, "the shift's norm bound must be positive");
-    if (!(d->K_max > 0)) return fail(TGH_EINVAL, "the momentum's scale bound must be positive");
+      return fail(TGH_EINVAL, "lambda_min/lambda_max must bracket 1 and stay positive");
+    if (!(d->beta_max > 0)) return fail(TGH_EINVAL, "beta_max is not positive");
+    if (!(d->K_max > 0)) return fail(TGH_EINVAL, "K_max is not positive");
     if (!(d->r_gate > 0))
-      return fail(TGH_EINVAL, "the gate radius must be positive (r_gate = 0 is a case without bounds)");
+      return fail(TGH_EINVAL, "r_gate < 0 (0 turns the bounds off)");
     if (d->interior == TGH_INTERIOR_NONE)
-      return fail(TGH_EINVAL, "this case carries range bounds but no interior: the projection is gated on "
-                              "r < r_gate from the hole's center");
+      return fail(TGH_EINVAL, "range bounds act inside r_gate about a hole's center, and this case "
+                              "has no interior");
     const double reach = std::max((double)G, std::sqrt(2.0) * (d->q / 2));
     if (!(d->r_gate <= d->r1 - reach * h0))
-      return fail(TGH_EINVAL, "the range projection's gate is too shallow: r_gate = " + std::to_string(d->r_gate) +
+      return fail(TGH_EINVAL, "r_gate = " + std::to_string(d->r_gate) +
                                   " but an evolved point (r >= r_1) reads points down to r_1 - R h = " +
                                   std::to_string(d->r1 - reach * h0) + " (check_bounds_gate)");
   }
   if (d->damping == TGH_DAMPING_GAUSSIAN && !(d->gamma0_width > 0))
-    return fail(TGH_EINVAL, "the damping profile's width must be positive");
+    return fail(TGH_EINVAL, "gamma0_width is not positive");
   return TGH_OK;
 }
 
@@ -301,14 +298,32 @@ RhsArgs rhs_args(tgh_problem* p, const double* W, double t, const tgh_matter* m)
   return a;
 }
 
+// The error rows from the reduction's partials, folded in workgroup order:
+// (err_l2, err_linf, residual, drift).
+void fold_error_rows(const double* red, size_t nwg, double out[4]) {
+  double num = 0, den = 0, mx[3] = {0, 0, 0};
+  for (size_t w = 0; w < nwg; ++w) {
+    num += red[RED_STRIDE * w];
+    den += red[RED_STRIDE * w + 1];
+    for (int q = 0; q < 3; ++q) {
+      const double v = red[RED_STRIDE * w + 2 + q];
+      mx[q] = (v > mx[q] || std::isnan(v)) ? v : mx[q];
+    }
+  }
+  out[0] = den > 0 ? std::sqrt(num / den) : 0.0;
+  out[1] = mx[0];
+  out[2] = mx[1];
+  out[3] = mx[2];
+}
+
 int sync(tgh_problem* p) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(p->stream));
   return TGH_OK;
 }
 
-int reduce(tgh_problem* p, const double* u, double t, int mode) {
-  ReduceArgs a{p->geom, p->cs, t, u, p->red, mode};
+int reduce(tgh_problem* p, const double* u, double t, int mode, double shell_lo = 0.0, double shell_hi = -1.0) {
+  ReduceArgs a{p->geom, p->cs, t, u, p->red, mode, shell_lo, shell_hi};
   hipLaunchKernelGGL(p->cs.moving ? k_reduce<true> : k_reduce<false>, grid_points(p), dim3(256), 0, p->stream, a);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(p->red_host.data(), p->red, sizeof(double) * p->red_host.size(),
@@ -570,11 +585,16 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
   }
   int rc = check_desc(d, lmax);
   if (rc) return rc;
+  if (!leaves && d->ownership == TGH_OWNERSHIP_HALF_OPEN)
+    return fail(TGH_EUNSUPPORTED, "half-open ownership of block faces is built for forests: create the "
+                                  "problem with tgh_problem_create_forest (a uniform brick is the forest "
+                                  "of its level-0 leaves)");
   if (leaves) {
     const int per[3] = {d->periodic[0] ? 1 : 0, d->periodic[1] ? 1 : 0, d->periodic[2] ? 1 : 0};
     std::string msg;
     try {
-      msg = forest_plan(d->N, d->q / 2 + 1, d->nblocks, per, prolong, lev, fbc, plan);
+      msg = forest_plan(d->N, d->q / 2 + 1, d->nblocks, per, prolong, lev, fbc, plan,
+                        d->ownership == TGH_OWNERSHIP_HALF_OPEN);
     } catch (const std::exception& ex) {  // host threads or memory
       return fail(TGH_EDEVICE, std::string("building the fill plan: ") + ex.what());
     }
@@ -694,7 +714,7 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
   p->state_len = P3 * 20 * g.nblocks;
   p->work_len = S3 * 20 * g.nblocks;
   const long wg_per_block = (P3 + 255) / 256;
-  p->red_host.assign(3 * (size_t)g.nblocks * wg_per_block, 0.0);
+  p->red_host.assign(RED_STRIDE * (size_t)g.nblocks * wg_per_block, 0.0);
   auto cleanup = [&](int code, const std::string& m) {
     tgh_problem_destroy(p);
     return fail(code, m);
@@ -779,6 +799,23 @@ int tgh_set_target(tgh_problem* p, const double* target, double t_target, int32_
   p->cs.t_f = t_target;
   p->cs.rate = rate != 0;
   p->cs.trail = trail;
+  ++p->version;
+  if (p->graph_exec) {  // a captured graph holds the old arguments
+    HIP_TRY(hipGraphExecDestroy(p->graph_exec));
+    p->graph_exec = nullptr;
+  }
+  return TGH_OK;
+}
+
+int tgh_set_rho_max(tgh_problem* p, double rho_max) {
+  if (!p) return fail(TGH_EINVAL, "NULL problem");
+  if (p->desc.interior == TGH_INTERIOR_NONE) return fail(TGH_EINVAL, "this problem's case has no interior");
+  if (!(rho_max >= 0) || !std::isfinite(rho_max))
+    return fail(TGH_EINVAL, "the relaxation rate must be finite and satisfy rho_max >= 0");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->desc.rho_max = rho_max;
+  p->cs.rho_max = rho_max;
+  ++p->version;
   if (p->graph_exec) {  // a captured graph holds the old arguments
     HIP_TRY(hipGraphExecDestroy(p->graph_exec));
     p->graph_exec = nullptr;
@@ -801,7 +838,10 @@ int tgh_transfer(tgh_problem* from, const double* u_from, double t, tgh_problem*
   const std::vector<int> flev = from->forest ? from->level_host : std::vector<int>(from->geom.nblocks, 0);
   const std::vector<int> tlev = to->forest ? to->level_host : std::vector<int>(to->geom.nblocks, 0);
   const int np = from->forest ? from->prolong : to->forest ? to->prolong : 2;
+  if (a.ownership != b.ownership)
+    return fail(TGH_EINVAL, "the transfer needs two forests with the same ownership of block faces");
   ForestIndex ix;
+  ix.half_open = a.ownership == TGH_OWNERSHIP_HALF_OPEN;
   std::string msg;
   std::vector<int> ops;
   try {
@@ -868,6 +908,14 @@ int tgh_problem_destroy(tgh_problem* p) {
   (void)hipFree(p->Bacc);
   (void)hipFree(p->BH);
   (void)hipFree(p->Bcoef);
+  (void)hipFree(p->adjF);
+  for (int i = 0; i < 2; ++i) {
+    (void)hipFree(p->adjL[i]);
+    (void)hipFree(p->adjK[i]);
+  }
+  (void)hipFree(p->adjTh);
+  (void)hipFree(p->adjSlots);
+  (void)hipFree(p->adjCkpt);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   delete p;
@@ -905,8 +953,8 @@ int tgh_prepare(tgh_problem* p, const double* u, double t) {
 int tgh_rhs(tgh_problem* p, double* du, double t, const tgh_matter* matter) {
   if (!p || !du) return fail(TGH_EINVAL, "NULL argument");
   if (p->cs.interior == INT_FITTED && !p->cs.tw)
-    return fail(TGH_EINVAL, "a :fitted interior relaxes toward a target the kernel reads from a cache: give "
-                            "the problem one with tgh_set_target (GHProblem's target)");
+    return fail(TGH_EINVAL, "the :fitted variant reads its target cache, and none is set: call "
+                            "tgh_set_target first");
   if (matter && matter->rho && (!matter->S || !matter->Sij))
     return fail(TGH_EINVAL, "a matter adapter needs rho, S and Sij");
   RhsArgs a = rhs_args(p, p->W[0], t, matter);
@@ -940,10 +988,10 @@ int tgh_apply_bounds(tgh_problem* p, double* u, double t, int64_t* moved, int64_
                            hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
     double hits = 0, nfs = 0, r = 0;
-    for (size_t w = 0; w < p->red_host.size() / 3; ++w) {
-      hits += p->red_host[3 * w];
-      nfs += p->red_host[3 * w + 1];
-      r = std::max(r, p->red_host[3 * w + 2]);
+    for (size_t w = 0; w < p->red_host.size() / RED_STRIDE; ++w) {
+      hits += p->red_host[RED_STRIDE * w];
+      nfs += p->red_host[RED_STRIDE * w + 1];
+      r = std::max(r, p->red_host[RED_STRIDE * w + 2]);
     }
     n = (int64_t)hits;
     nf = (int64_t)nfs;
@@ -991,34 +1039,36 @@ int tgh_max_speed(tgh_problem* p, const double* u, double t, double* out) {
   int rc = reduce(p, u, t, 0);
   if (rc) return rc;
   double lam = 0.0, nbad = 0.0;
-  for (size_t w = 0; w < p->red_host.size() / 3; ++w) {
-    const double l = p->red_host[3 * w];
+  for (size_t w = 0; w < p->red_host.size() / RED_STRIDE; ++w) {
+    const double l = p->red_host[RED_STRIDE * w];
     lam = (l > lam || std::isnan(l)) ? l : lam;
-    nbad += p->red_host[3 * w + 1];
+    nbad += p->red_host[RED_STRIDE * w + 1];
   }
   if (nbad > 0)
-    return fail(TGH_ENOTFINITE, "the state at t = " + std::to_string(t) + " is not finite where it is "
-                "evolved: " + std::to_string((long)nbad) + " points outside the interior hold NaN or Inf");
+    return fail(TGH_ENOTFINITE, std::to_string((long)nbad) + " evolved points hold NaN or Inf at t = " +
+                std::to_string(t));
   if (!(std::isfinite(lam) && lam > 0))
-    return fail(TGH_ENOTFINITE, "the maximum characteristic speed alpha sqrt(tr gamma^ij) + |beta| over "
-                "the evolved points came out as " + std::to_string(lam) + ", so there is no CFL-limited "
-                "time step");
+    return fail(TGH_ENOTFINITE, "max over evolved points of alpha sqrt(tr gamma^ij) + |beta| = " +
+                std::to_string(lam) + ": no time step follows from it");
   *out = lam;
   return TGH_OK;
 }
 
-int tgh_error_norms(tgh_problem* p, const double* u, double t, double* l2, double* linf) {
-  if (!p || !u || !l2 || !linf) return fail(TGH_EINVAL, "NULL argument");
-  int rc = reduce(p, u, t, 1);
+int tgh_error_rows(tgh_problem* p, const double* u, double t, double shell_lo, double shell_hi, double* out) {
+  if (!p || !u || !out) return fail(TGH_EINVAL, "NULL argument");
+  int rc = reduce(p, u, t, 1, shell_lo, shell_hi);
   if (rc) return rc;
-  double num = 0, den = 0, li = 0;
-  for (size_t w = 0; w < p->red_host.size() / 3; ++w) {
-    num += p->red_host[3 * w];
-    den += p->red_host[3 * w + 1];
-    li = std::max(li, p->red_host[3 * w + 2]);
-  }
-  *l2 = den > 0 ? std::sqrt(num / den) : 0.0;
-  *linf = li;
+  fold_error_rows(p->red_host.data(), p->red_host.size() / RED_STRIDE, out);
+  return TGH_OK;
+}
+
+int tgh_error_norms(tgh_problem* p, const double* u, double t, double* l2, double* linf) {
+  if (!p || !l2 || !linf) return fail(TGH_EINVAL, "NULL argument");
+  double r[4];
+  int rc = tgh_error_rows(p, u, t, 0.0, -1.0, r);
+  if (rc) return rc;
+  *l2 = r[0];
+  *linf = r[1];
   return TGH_OK;
 }
 
@@ -1128,8 +1178,8 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
   if (p->cs.interior == INT_PASTED)
     return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
   if (p->cs.interior == INT_FITTED && !p->cs.tw)
-    return fail(TGH_EINVAL, "a :fitted interior relaxes toward a target the kernel reads from a cache: give "
-                            "the problem one with tgh_set_target (GHProblem's target)");
+    return fail(TGH_EINVAL, "the :fitted variant reads its target cache, and none is set: call "
+                            "tgh_set_target first");
   if (nsteps == 0) return TGH_OK;
   const bool graph = (flags & TGH_RK4_GRAPH) != 0;
   // a forest with levels and the general interior step on the block layout
@@ -1243,6 +1293,199 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
   return rc;
 }
 
+// ---------------------------------------------------------------------------
+// The adjoint (the reference has none; the parameters are the constraint
+// damping's Gaussian rates, gamma2, and the layer's rho_max and rho ramp).
+namespace {
+
+long adj_runs(const Brick& k) {
+  const long plane = (long)k.n[0] * k.n[1];
+  return (plane + 255) / 256 * k.n[2];
+}
+
+// The cases the adjoint covers, and its scratch (allocated once).
+int adj_prepare(tgh_problem* p) {
+  if (p->forest || p->cs.general)
+    return fail(TGH_EUNSUPPORTED, "the adjoint runs on the brick layout: a uniform forest and a spherical "
+                                  "interior (no levels, no tracked or :fitted geometry)");
+  if (p->cs.moving) return fail(TGH_EUNSUPPORTED, "the adjoint is built for holes at rest");
+  if (p->has_bounds) return fail(TGH_EUNSUPPORTED, "the adjoint is built for cases without range bounds");
+  if (p->cs.interior == INT_PASTED) return fail(TGH_EUNSUPPORTED, "the adjoint of the :pasted limiter is not built");
+  if (p->cs.gauge != GAUGE_NONE && p->cs.gauge != GAUGE_SAMPLED && p->cs.gauge != GAUGE_KS)
+    return fail(TGH_EUNSUPPORTED, "the adjoint is built for the static gauge-source kinds");
+  int rc = ensure_brick(p);
+  if (rc) return rc;
+  const size_t fb = sizeof(double) * (size_t)p->brick.fs;
+  if (!p->adjF) {
+    HIP_TRY(hipMalloc(&p->adjF, AF_N * fb));
+    HIP_TRY(hipMemsetAsync(p->adjF, 0, AF_N * fb, p->stream));  // the frame stays zero
+    for (int i = 0; i < 2; ++i) {
+      HIP_TRY(hipMalloc(&p->adjL[i], 20 * fb));
+      HIP_TRY(hipMemsetAsync(p->adjL[i], 0, 20 * fb, p->stream));
+      HIP_TRY(hipMalloc(&p->adjK[i], 20 * fb));
+      HIP_TRY(hipMemsetAsync(p->adjK[i], 0, 20 * fb, p->stream));
+    }
+    HIP_TRY(hipMalloc(&p->adjTh, sizeof(double) * ADJ_NPAR * adj_runs(p->brick)));
+  }
+  return TGH_OK;
+}
+
+// The parameters' cotangent: the per-workgroup accumulators folded in order.
+int adj_theta(tgh_problem* p, double* out) {
+  const long n = adj_runs(p->brick);
+  std::vector<double> h((size_t)(ADJ_NPAR * n));
+  HIP_TRY(hipMemcpyAsync(h.data(), p->adjTh, sizeof(double) * h.size(), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  for (int q = 0; q < ADJ_NPAR; ++q) out[q] = 0.0;
+  for (long w = 0; w < n; ++w)
+    for (int q = 0; q < ADJ_NPAR; ++q) out[q] += h[(size_t)(ADJ_NPAR * w + q)];
+  return TGH_OK;
+}
+
+// One RK4 stage on explicit brick arrays (rk4_brick_launches' stage s with
+// the stage input Wc and the next stage's Wn).
+void brick_stage(tgh_problem* p, double* Wc, double* Wn, double t, double dt, int s) {
+  const double c_acc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+  const double c_next[4] = {dt / 2, dt / 2, dt, 0.0};
+  const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
+  if (!p->frame_static) launch_frame(p, Wc, t_stage[s]);
+  RhsArgs a = rhs_args(p, Wc, t_stage[s], nullptr);
+  a.Hsrc = p->BH;
+  a.us = p->Bus;
+  a.acc = p->Bacc;
+  a.Wn = Wn;
+  a.cacc = c_acc[s];
+  a.cnext = c_next[s];
+  a.stage = s + 1;
+  switch (p->q) {
+    case 2: launch_rhs_brick<2>(p, a); break;
+    case 4: launch_rhs_brick<4>(p, a); break;
+    case 6: launch_rhs_brick<6>(p, a); break;
+    default: launch_rhs_brick<8>(p, a); break;
+  }
+}
+
+}  // namespace
+
+int tgh_vjp_rhs(tgh_problem* p, const double* u, double t, const double* du_bar, double* u_bar, double* theta_bar) {
+  if (!p || !u || !du_bar || !u_bar || !theta_bar) return fail(TGH_EINVAL, "NULL argument");
+  int rc = adj_prepare(p);
+  if (rc) return rc;
+  const Brick& k = p->brick;
+  BrickCopyArgs ca{p->geom, k, u, p->BW[0], p->Bus, 0};
+  hipLaunchKernelGGL(k_brick_in, grid_brick(p), dim3(256), 0, p->stream, ca);
+  if (!p->frame_static) launch_frame(p, p->BW[0], t);
+  launch_adj_fold_copies(p, AdjCopiesArgs{p->geom, k, du_bar, p->adjK[0]});
+  HIP_TRY(hipMemsetAsync(p->adjTh, 0, sizeof(double) * ADJ_NPAR * adj_runs(k), p->stream));
+  launch_adj_point(p, AdjPointArgs{k, p->cs, t, p->BW[0], p->BH, p->adjK[0], p->adjF, p->adjTh});
+  launch_adj_gather(p, AdjGatherArgs{k, p->adjF, nullptr, nullptr, nullptr, p->adjL[0], 0.0, 0.0, 0, 0});
+  HIP_TRY(hipMemsetAsync(u_bar, 0, sizeof(double) * (size_t)p->state_len, p->stream));
+  launch_adj_to_primaries(p, AdjCopiesArgs{p->geom, k, p->adjL[0], u_bar});
+  HIP_TRY(hipGetLastError());
+  return adj_theta(p, theta_bar);
+}
+
+int tgh_adjoint(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t every, double* J,
+                double* grad, double* ms) {
+  if (!p || !u || !J || !grad) return fail(TGH_EINVAL, "NULL argument");
+  if (nsteps < 1 || every < 1 || !(dt > 0) || !std::isfinite(t0))
+    return fail(TGH_EINVAL, "tgh_adjoint needs nsteps >= 1, every >= 1, dt > 0 and a finite t0");
+  int rc = adj_prepare(p);
+  if (rc) return rc;
+  const Brick& k = p->brick;
+  const size_t fb20 = sizeof(double) * 20 * (size_t)k.fs;
+  const size_t nslot = 4 * (size_t)every + 1, nseg = ((size_t)nsteps + every - 1) / every;
+  if (p->adjSlotsN < nslot) {
+    HIP_TRY(hipFree(p->adjSlots));
+    p->adjSlots = nullptr;
+    p->adjSlotsN = 0;
+    HIP_TRY(hipMalloc(&p->adjSlots, nslot * fb20));
+    HIP_TRY(hipMemsetAsync(p->adjSlots, 0, nslot * fb20, p->stream));
+    p->adjSlotsN = nslot;
+    if (p->frame_static)
+      for (size_t i = 0; i < nslot; ++i) launch_frame(p, p->adjSlots + i * 20 * (size_t)k.fs, 0.0);
+  }
+  if (p->adjCkptN < nseg) {
+    HIP_TRY(hipFree(p->adjCkpt));
+    p->adjCkpt = nullptr;
+    p->adjCkptN = 0;
+    HIP_TRY(hipMalloc(&p->adjCkpt, nseg * fb20));
+    p->adjCkptN = nseg;
+  }
+  if (!p->Bcoef) {
+    HIP_TRY(hipMalloc(&p->Bcoef, sizeof(double) * CF_N * (size_t)k.fs));
+    HIP_TRY(hipMemsetAsync(p->Bcoef, 0, sizeof(double) * CF_N * (size_t)k.fs, p->stream));
+  }
+  auto slot = [&](size_t i) { return p->adjSlots + i * 20 * (size_t)k.fs; };
+  auto ckpt = [&](size_t i) { return p->adjCkpt + i * 20 * (size_t)k.fs; };
+  hipEvent_t ev[4];
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  p->fused = false;  // the split form (head + LDS-tiled Pi), the stepper's default
+  // ---- forward: u_0 into the brick, a checkpoint at each segment's start ----
+  HIP_TRY(hipEventRecord(ev[0], p->stream));
+  BrickCopyArgs ca{p->geom, k, u, p->BW[0], p->Bus, 0};
+  hipLaunchKernelGGL(k_brick_in, grid_brick(p), dim3(256), 0, p->stream, ca);
+  for (size_t sg = 0; sg < nseg; ++sg) {
+    HIP_TRY(hipMemcpyAsync(ckpt(sg), p->BW[0], fb20, hipMemcpyDeviceToDevice, p->stream));
+    const int n0 = (int)(sg * every), n1 = std::min(nsteps, (int)((sg + 1) * every));
+    for (int n = n0; n < n1; ++n) rk4_brick_launches(p, t0 + n * dt, dt);
+  }
+  HIP_TRY(hipEventRecord(ev[1], p->stream));
+  for (int pass = 0; pass < 2; ++pass) {
+    BrickCopyArgs co{p->geom, k, p->Bus, u, nullptr, pass};
+    hipLaunchKernelGGL(k_brick_out, grid_points(p), dim3(256), 0, p->stream, co);
+  }
+  // J = the masked L2 error at T, and lambda_T = dJ/du_T
+  const double T = t0 + nsteps * dt;
+  rc = reduce(p, u, T, 1);
+  if (rc) return rc;
+  double num = 0.0, den = 0.0;
+  for (size_t w = 0; w < p->red_host.size() / RED_STRIDE; ++w) {
+    num += p->red_host[RED_STRIDE * w];
+    den += p->red_host[RED_STRIDE * w + 1];
+  }
+  *J = den > 0 ? std::sqrt(num / den) : 0.0;
+  double *lam = p->adjL[0], *lacc = p->adjL[1], *kb = p->adjK[0], *kbn = p->adjK[1];
+  launch_adj_init(p, AdjInitArgs{k, p->cs, T, *J, den, dt / 6, p->Bus, lam, kb});
+  HIP_TRY(hipMemsetAsync(p->adjTh, 0, sizeof(double) * ADJ_NPAR * adj_runs(k), p->stream));
+  // ---- backward: per segment, its stage inputs recomputed from the
+  // checkpoint, then its steps reversed ----
+  HIP_TRY(hipEventRecord(ev[2], p->stream));
+  const double cl[4] = {dt / 6, dt / 6, dt / 3, dt / 3}, cy[4] = {0.0, dt / 2, dt / 2, dt};
+  for (size_t sg = nseg; sg-- > 0;) {
+    const int n0 = (int)(sg * every), n1 = std::min(nsteps, (int)((sg + 1) * every)), ns = n1 - n0;
+    HIP_TRY(hipMemcpyAsync(slot(0), ckpt(sg), fb20, hipMemcpyDeviceToDevice, p->stream));
+    HIP_TRY(hipMemcpyAsync(p->Bus, ckpt(sg), fb20, hipMemcpyDeviceToDevice, p->stream));
+    for (int j = 0; j < ns; ++j)
+      for (int s = 0; s < 4; ++s) brick_stage(p, slot(4 * j + s), slot(4 * j + s + 1), t0 + (n0 + j) * dt, dt, s);
+    for (int j = ns; j-- > 0;) {
+      const double t = t0 + (n0 + j) * dt;
+      const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
+      for (int s = 4; s-- > 0;) {
+        double* Wc = slot(4 * j + s);
+        if (!p->frame_static) launch_frame(p, Wc, t_stage[s]);
+        launch_adj_point(p, AdjPointArgs{k, p->cs, t_stage[s], Wc, p->BH, kb, p->adjF, p->adjTh});
+        launch_adj_gather(p, AdjGatherArgs{k, p->adjF, lam, lacc, kbn, nullptr, cl[s], cy[s], s == 3, s == 0});
+        std::swap(kb, kbn);
+      }
+      std::swap(lam, lacc);
+    }
+  }
+  HIP_TRY(hipEventRecord(ev[3], p->stream));
+  HIP_TRY(hipGetLastError());
+  rc = adj_theta(p, grad);
+  if (rc) return rc;
+  if (ms) {
+    float f0 = 0.0f, f1 = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&f0, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&f1, ev[2], ev[3]));
+    ms[0] = f0;
+    ms[1] = f1;
+  }
+  for (auto& e : ev) HIP_TRY(hipEventDestroy(e));
+  return TGH_OK;
+}
+
 int tgh_set_tile(tgh_problem* p, int32_t tile) {
   if (!p) return fail(TGH_EINVAL, "NULL problem");
   if (tile < 0 || tile >= PI_TILES)
@@ -1332,6 +1575,7 @@ struct tgh_batch {
   BatchEntry* h_tab = nullptr;  // pinned staging of the tables
   int head_total = 0, pi_total = 0;
   double tab_key[2] = {0, 0};   // (dt, t) the uploaded tables hold
+  std::vector<unsigned> tab_ver;  // the problems' versions the tables were built from
   bool tab_valid = false;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -1589,6 +1833,13 @@ int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t 
   // on the block layout the tables carry each problem's case, whose :fitted
   // target (tgh_set_target) may have changed since the last call
   if (b->blocks) b->tab_valid = false;
+  // a setter (tgh_set_rho_max, tgh_set_target) changed a problem's case
+  std::vector<unsigned> ver;
+  for (tgh_problem* q : b->probs) ver.push_back(q->version);
+  if (ver != b->tab_ver) {
+    b->tab_valid = false;
+    b->tab_ver = ver;
+  }
   for (int s = 0; s < nsteps && rc == TGH_OK; ++s) {
     const double t = t0 + s * dt;
     // static backgrounds ignore t: the tables (and the graph) are kept across steps

@@ -17,12 +17,17 @@
 //    leaf that holds it, p points centred on the midpoint along each such
 //    axis, tensor product;
 //  * outside a non-periodic axis, the Dirichlet hook.
+// With half-open ownership (TreeAMR 0.1) a leaf owns [0, N) per axis
+// instead: a point's owner is the one leaf whose half-open extent holds it,
+// when the point is a vertex of that leaf (else it is prolongated as
+// above), and the upper plane of a non-periodic axis is a Dirichlet ghost.
 // The checks: the leaves tile the box exactly, neighbours (faces, edges and
 // corners, across periodic seams) differ by at most one level, N is even and
 // at least 2G, and the prolongation order is even and fits the ghosts.
 #pragma once
 #include <algorithm>
 #include <cstdint>
+#include <exception>
 #include <string>
 #include <thread>
 #include <vector>
@@ -31,17 +36,35 @@ namespace tgh {
 
 // Runs fn(chunk, b0, b1) over contiguous block ranges [b0, b1) on up to 16
 // host threads; chunk c covers the c-th range, so per-chunk outputs joined in
-// chunk order are the serial loop's.
+// chunk order are the serial loop's. An exception in any chunk (or in
+// starting a thread) is rethrown on the calling thread after every thread
+// started has been joined.
 template <class F>
 inline int for_block_chunks(int nl, F fn) {
   unsigned hw = std::thread::hardware_concurrency();
   const int nt = (int)std::max(1u, std::min(16u, hw ? hw : 1u));
   const int nchunk = std::max(1, std::min(nt, nl / 8));
+  std::vector<std::exception_ptr> err(nchunk);
+  auto run = [&](int c) {
+    try {
+      fn(c, (int)((long)nl * c / nchunk), (int)((long)nl * (c + 1) / nchunk));
+    } catch (...) {
+      err[c] = std::current_exception();
+    }
+  };
   std::vector<std::thread> pool;
-  for (int c = 1; c < nchunk; ++c)
-    pool.emplace_back(fn, c, (int)((long)nl * c / nchunk), (int)((long)nl * (c + 1) / nchunk));
-  fn(0, 0, (int)((long)nl / nchunk));
+  std::exception_ptr start_err;
+  try {
+    pool.reserve(nchunk);
+    for (int c = 1; c < nchunk; ++c) pool.emplace_back(run, c);
+  } catch (...) {
+    start_err = std::current_exception();
+  }
+  if (!start_err) run(0);
   for (auto& t : pool) t.join();
+  if (start_err) std::rethrow_exception(start_err);
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
   return nchunk;
 }
 
@@ -57,6 +80,7 @@ struct ForestPlan {
 // rules above that name where a point takes its value from.
 struct ForestIndex {
   int N = 0, Lmax = 0;
+  bool half_open = false;
   int nb[3] = {0, 0, 0}, periodic[3] = {0, 0, 0};
   long nc[3] = {0, 0, 0};
   std::vector<int> lev, bc, cell;
@@ -178,6 +202,12 @@ struct ForestIndex {
     for (int d = 0; d < 3; ++d) X[d] = gl[d] << (Lw - L);
     int cand[8];
     const int ncand = around(Lw, X, cand);
+    // the axes where the point is the upper plane of a non-periodic box:
+    // half-open ownership leaves it to the Dirichlet hook, and only a
+    // transfer asks for it, which takes the closed rule there
+    int top = 0;
+    for (int d = 0; d < 3; ++d)
+      if (!periodic[d] && X[d] == nc[d] * ((long)N << (Lw - Lmax))) top |= 1 << d;
     int own = -1, own_mask = 8;
     for (int q = 0; q < ncand; ++q) {
       const int c = cand[q];
@@ -189,6 +219,7 @@ struct ForestIndex {
         l[d] = local(c, d, X[d] >> s);
         if (l[d] == N) mask |= 1 << d;
       }
+      if (half_open && (mask & ~top) != 0) continue;
       if (own < 0 || lev[c] > lev[own] || (lev[c] == lev[own] && mask < own_mask)) {
         own = c;
         own_mask = mask;
@@ -241,10 +272,12 @@ inline std::string check_forest_shape(int N, int G, int np) {
 
 // Builds the plan, or returns an error message.
 inline std::string forest_plan(int N, int G, const int nb[3], const int periodic[3], int np,
-                               const std::vector<int>& lev, const std::vector<int>& bc, ForestPlan& out) {
+                               const std::vector<int>& lev, const std::vector<int>& bc, ForestPlan& out,
+                               bool half_open = false) {
   std::string msg = check_forest_shape(N, G, np);
   if (!msg.empty()) return msg;
   ForestIndex ix;
+  ix.half_open = half_open;
   msg = ix.build(N, nb, periodic, lev, bc);
   if (!msg.empty()) return msg;
   const int nl = (int)lev.size(), Lmax = ix.Lmax;
@@ -276,7 +309,7 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
               const long tot = ((long)nb[d] << L) * N;
               gl[d] = (long)bc[3 * b + d] * N + l[d];
               if (periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
-              else if (gl[d] < 0 || gl[d] > tot) outside = true;
+              else if (gl[d] < 0 || gl[d] > tot || (half_open && gl[d] == tot)) outside = true;
             }
             if (outside) {
               pass[0].insert(pass[0].end(), {dst, 0, 8, 0});

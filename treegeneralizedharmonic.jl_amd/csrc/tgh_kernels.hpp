@@ -1147,6 +1147,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_PI_WAVE
   }
 }
 
+// Whether x is evolved (outside the layer's outer surface): the points the
+// masked norms count.
+__device__ __forceinline__ bool is_evolved(const CaseP& c, double t, const double x[3]) {
+  if (c.interior == INT_NONE) return true;
+  if (c.shaped) {
+    const IGeo g = interior_geo(c, t, x);
+    return g.r >= g.r1;
+  }
+  return hole_radius(c, t, x) >= c.r1;
+}
+
 #ifndef TGH_RHS_ONLY  // the pointwise kernels: built once, in the API's translation unit
 // ---------------------------------------------------------------------------
 // The ghost fill of a work array (prepare!, src/evolution.jl:1349-1357).
@@ -1409,24 +1420,42 @@ __global__ __launch_bounds__(256) void k_pointwise(PointArgs a) {
 }
 
 // Reductions over owned points: each workgroup of 256 points of one block
-// folds its points in shared memory and writes one partial; the host folds
-// the partials in (block, workgroup) order, so the answer does not depend on
-// the launch.
+// folds its points in shared memory and writes RED_STRIDE partials; the host
+// folds the partials in (block, workgroup) order, so the answer does not
+// depend on the launch.
+//  mode 0 (max_speed): [0] the largest speed over evolved points, [1] the
+//    number of evolved points with a non-finite value;
+//  mode 1 (gh_error_kernel! + error_norms, src/constraints.jl:311-396,
+//    629-671): at every primary the error e = |u - u_exact(t)|, the 2-norm
+//    over the 20 variables summed in variable order; [0] sum w e^2 and [1]
+//    sum w over evolved primaries (w the volume relative to level 0's on a
+//    forest, 1 on a uniform one), [2] the largest e there, [3] the layer's
+//    residual (the largest e at r_0 <= r < r_1; for :fitted the distance
+//    from the target cache), [4] the drift (the largest |h_tt - h_tt,exact|
+//    at evolved primaries with shell_lo <= r <= shell_hi).
+enum { RED_STRIDE = 5 };
 struct ReduceArgs {
   Geom g;
   CaseP c;
   double t;
   const double* __restrict__ u;
-  double* __restrict__ out;  // per workgroup: [0] max speed | sum e^2, [1] nonfinite | count, [2] linf
-  int mode;                  // 0 speed, 1 error
+  double* __restrict__ out;  // RED_STRIDE partials per workgroup
+  int mode;                  // 0 speed, 1 error rows
+  double shell_lo, shell_hi;
 };
-__device__ __forceinline__ bool is_evolved(const CaseP& c, double t, const double x[3]) {
-  if (c.interior == INT_NONE) return true;
+// the point's radius and whether it lies in the layer r_0 <= r < r_1
+__device__ __forceinline__ bool in_layer(const CaseP& c, double t, const double x[3], double& r) {
+  if (c.interior == INT_NONE) {
+    r = 0.0;
+    return false;
+  }
   if (c.shaped) {
     const IGeo g = interior_geo(c, t, x);
-    return g.r >= g.r1;
+    r = g.r;
+    return g.r0 <= g.r && g.r < g.r1;
   }
-  return hole_radius(c, t, x) >= c.r1;
+  r = hole_radius(c, t, x);
+  return c.r0 <= r && r < c.r1;
 }
 template <bool MOVE>
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
@@ -1436,14 +1465,15 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
   if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
   const int p = wx * blockDim.x + threadIdx.x;
   const long psz = (long)P * P * P;
-  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  double r[RED_STRIDE] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (p < P * P * P) {
     const int i = p % P, j = (p / P) % P, k = p / (P * P);
     const double* ub = a.u + (long)b * 20 * psz + p;
     double x[3];
     position(g, b, i, j, k, x);
-    if (is_evolved(a.c, a.t, x)) {
-      if (a.mode == 0) {
+    const bool keep = is_evolved(a.c, a.t, x);
+    if (a.mode == 0) {
+      if (keep) {
         double hv[NC];
         bool fin = true;
 #pragma unroll
@@ -1452,12 +1482,23 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
           fin = fin && isfinite(val);
           if (v < NC) hv[v] = val;
         }
-        r1 = fin ? 0.0 : 1.0;
+        r[1] = fin ? 0.0 : 1.0;
         Coef cf;
         metric_quantities(hv, cf);
-        r0 = cf.alpha * sqrt(cf.gamu[0] + cf.gamu[3] + cf.gamu[5]) +
-             sqrt(cf.beta[0] * cf.beta[0] + cf.beta[1] * cf.beta[1] + cf.beta[2] * cf.beta[2]);
+        r[0] = cf.alpha * sqrt(cf.gamu[0] + cf.gamu[3] + cf.gamu[5]) +
+               sqrt(cf.beta[0] * cf.beta[0] + cf.beta[1] * cf.beta[1] + cf.beta[2] * cf.beta[2]);
+      }
+    } else {
+      bool prim;
+      if (g.prim) {
+        prim = g.prim[(long)b * psz + p] != 0;
       } else {
+        int l[3] = {i, j, k}, owner, loc[3];
+        prim = classify(g, b, l, owner, loc) == -1;
+      }
+      double rad;
+      const bool lay = in_layer(a.c, a.t, x, rad);
+      if (prim && (keep || lay)) {
         double s[20];
         case_state_any<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
         double e2 = 0.0;
@@ -1465,49 +1506,60 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
         for (int v = 0; v < 20; ++v) {
           const double e = ub[v * psz] - s[v];
           e2 += e * e;
-          r2 = fmax(r2, fabs(e));
         }
-        int l[3] = {i, j, k}, owner, loc[3];
-        if (g.prim) {
-          // a forest: primaries only, each weighted by its volume relative
-          // to level 0's, 2^(-3 level)
-          if (g.prim[(long)b * psz + p]) {
-            const double wv = ldexp(1.0, -3 * block_level(g, b));
-            r0 = e2 * wv;
-            r1 = wv;
-          } else {
-            r2 = 0.0;
+        const double e = sqrt(e2);
+        if (keep) {
+          const double wv = g.prim ? ldexp(1.0, -3 * block_level(g, b)) : 1.0;
+          r[0] = g.prim ? e2 * wv : e2;
+          r[1] = wv;
+          r[2] = e;
+          if (a.shell_lo <= rad && rad <= a.shell_hi) r[4] = fabs(ub[0] - s[0]);
+        }
+        if (lay) {
+          double ef = e;
+          if (a.c.interior == INT_FITTED && a.c.tw) {
+            const long ti = (long)b * 40 * psz + p;
+            double f2 = 0.0;
+            for (int v = 0; v < 20; ++v) {
+#pragma clang fp contract(off)
+              const double tg = a.c.tw[ti + v * psz] + (a.t - a.c.t_f) * a.c.tw[ti + (20 + v) * psz];
+              const double d = ub[v * psz] - tg;
+              f2 += d * d;
+            }
+            ef = sqrt(f2);
           }
-        } else if (classify(g, b, l, owner, loc) == -1) {
-          r0 = e2;
-          r1 = 1.0;
+          r[3] = ef;
         }
       }
     }
   }
-  __shared__ double sh[3][256];
+  __shared__ double sh[RED_STRIDE][256];
   const int tid = threadIdx.x;
-  sh[0][tid] = r0;
-  sh[1][tid] = r1;
-  sh[2][tid] = r2;
+#pragma unroll
+  for (int q = 0; q < RED_STRIDE; ++q) sh[q][tid] = r[q];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (tid < w) {
-      const double o0 = sh[0][tid + w], m0 = sh[0][tid];
-      if (a.mode == 0)
+      if (a.mode == 0) {
+        const double o0 = sh[0][tid + w], m0 = sh[0][tid];
         sh[0][tid] = (o0 > m0 || isnan(o0)) ? o0 : m0;
-      else
-        sh[0][tid] = m0 + o0;
-      sh[1][tid] = sh[1][tid] + sh[1][tid + w];
-      sh[2][tid] = fmax(sh[2][tid], sh[2][tid + w]);
+        sh[1][tid] = sh[1][tid] + sh[1][tid + w];
+      } else {
+        sh[0][tid] = sh[0][tid] + sh[0][tid + w];
+        sh[1][tid] = sh[1][tid] + sh[1][tid + w];
+#pragma unroll
+        for (int q = 2; q < RED_STRIDE; ++q) {
+          const double o = sh[q][tid + w], m = sh[q][tid];
+          sh[q][tid] = (o > m || isnan(o)) ? o : m;
+        }
+      }
     }
     __syncthreads();
   }
   if (tid == 0) {
-    double* o = a.out + 3 * ((long)b * ((g.P * g.P * g.P + 255) / 256) + wx);
-    o[0] = sh[0][0];
-    o[1] = sh[1][0];
-    o[2] = sh[2][0];
+    double* o = a.out + RED_STRIDE * ((long)b * ((g.P * g.P * g.P + 255) / 256) + wx);
+#pragma unroll
+    for (int q = 0; q < RED_STRIDE; ++q) o[q] = sh[q][0];
   }
 }
 
@@ -1876,6 +1928,18 @@ __global__ __launch_bounds__(256) void k_tau(TauArgs a) {
 // copies exist, and only the frame is filled: from the Dirichlet hook or by
 // periodic wrap, with the same rule as the block fill (a point outside a
 // non-periodic axis takes the exact state at its own unwrapped position).
+// The primary (block, local index) of brick point gp.
+__device__ __forceinline__ int brick_primary(const Geom& g, const int gp[3], int loc[3]) {
+  int ob[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int o = gp[d] / g.N;
+    ob[d] = o < g.nb[d] - 1 ? o : g.nb[d] - 1;
+    loc[d] = gp[d] - ob[d] * g.N;
+  }
+  return g.lookup[ob[0] + g.nb[0] * (ob[1] + g.nb[1] * ob[2])];
+}
+
 struct Brick {
   int n[3];      // evolved points per axis
   int tot[3];    // nb_d N
@@ -1901,16 +1965,8 @@ __device__ __forceinline__ int brick_runs(const Brick& k) {
 __device__ __forceinline__ bool brick_run(const Brick& k, int L, int& x, int& y, int& z) {
   const long plane = (long)k.n[0] * k.n[1];
   const int wpp = (int)((plane + 255) / 256);
-  int r;
-#ifdef TGH_BRICK_ZFAST
-  // runs ordered z-fastest: consecutive workgroups of an XCD take the same
-  // run of consecutive planes, which share their z-stencil planes
-  r = L / k.n[2];
-  z = L - r * k.n[2];
-#else
   z = L / wpp;
-  r = L - z * wpp;
-#endif
+  const int r = L - z * wpp;
   const long lin = (long)r * 256 + threadIdx.x;
   if (lin >= plane) return false;
   y = (int)(lin / k.n[0]);
@@ -1925,9 +1981,7 @@ __device__ __forceinline__ bool brick_map(const Brick& k, int& x, int& y, int& z
 }
 
 // Tiles of TX x TY x TZ points over the brick, dealt to the XCDs in
-// contiguous ranges of a z-fastest (TGH_TILES_ZFAST) or x-fastest order.
-// z-fastest: the tiles an XCD holds at once are a column of consecutive
-// z-tiles, which share their z-halo planes in its L2.
+// contiguous ranges of an x-fastest order.
 __device__ __forceinline__ int brick_tiles(const Brick& k, int TX, int TY, int TZ) {
   return ((k.n[0] + TX - 1) / TX) * ((k.n[1] + TY - 1) / TY) * ((k.n[2] + TZ - 1) / TZ);
 }
@@ -1935,17 +1989,9 @@ __device__ __forceinline__ int brick_tiles(const Brick& k, int TX, int TY, int T
 __device__ __forceinline__ void tile_at(const Brick& k, int TX, int TY, int TZ, int tile, int& X0, int& Y0,
                                         int& Z0) {
   const int ntx = (k.n[0] + TX - 1) / TX, nty = (k.n[1] + TY - 1) / TY;
-#ifdef TGH_TILES_ZFAST
-  const int ntz = (k.n[2] + TZ - 1) / TZ;
-  const int tz = tile % ntz, txy = tile / ntz;
-  X0 = (txy % ntx) * TX;
-  Y0 = (txy / ntx) * TY;
-  Z0 = tz * TZ;
-#else
   X0 = (tile % ntx) * TX;
   Y0 = ((tile / ntx) % nty) * TY;
   Z0 = (tile / (ntx * nty)) * TZ;
-#endif
 }
 __device__ __forceinline__ bool tile_map(const Brick& k, int TX, int TY, int TZ, int& X0, int& Y0, int& Z0) {
   const int total = brick_tiles(k, TX, TY, TZ);
@@ -1987,33 +2033,15 @@ __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const dou
 // The split form on the brick layout, first half: the head at one point per
 // thread (brick_map's order), leaving the Pi coefficients in coef (CF_N
 // arrays of the brick's field size, indexed like the brick).
-#ifdef TGH_HEAD_WAVES_PER_EU
-#define TGH_HEAD_ATTR __attribute__((amdgpu_waves_per_eu(TGH_HEAD_WAVES_PER_EU)))
-#else
-#define TGH_HEAD_ATTR
-#endif
 template <int Q, int GAUGE, int HOLE>
 __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
                                               const double* __restrict__ Hsrc, double* __restrict__ coef,
                                               TGH_OUT_PARAMS, int x, int y, int z);
 template <int Q, int GAUGE, int HOLE>
-__global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head(RhsArgs a, Brick k, const double* __restrict__ W,
+__global__ __launch_bounds__(256) void k_brick_head(RhsArgs a, Brick k, const double* __restrict__ W,
                                                     double* __restrict__ coef, TGH_OUT_PARAMS) {
   int x, y, z;
-#ifdef TGH_HEAD_TILE_X
-  {  // tiles of TGH_HEAD_TILE_X x (256 / (TGH_HEAD_TILE_X * 4)) x 4 points
-    constexpr int TX = TGH_HEAD_TILE_X, TZ = 4, TY = 256 / (TX * TZ);
-    int X0, Y0, Z0;
-    if (!tile_map(k, TX, TY, TZ, X0, Y0, Z0)) return;
-    const int t = threadIdx.x;
-    x = X0 + t % TX;
-    y = Y0 + (t / TX) % TY;
-    z = Z0 + t / (TX * TY);
-    if (!(x < k.n[0] && y < k.n[1] && z < k.n[2])) return;
-  }
-#else
   if (!brick_map(k, x, y, z)) return;
-#endif
   brick_head_at<Q, GAUGE, HOLE>(a, k, W, a.Hsrc, coef, TGH_OUT_ARGS, x, y, z);
 }
 template <int Q, int GAUGE, int HOLE>
@@ -2104,17 +2132,12 @@ template <int Q, int TC = 0> struct PiTile {
     return TC == 4 ? (bx <= 25 ? 25 : bx) : TC == 5 ? (bx <= 26 ? 26 : bx) : (bx <= 24 ? 24 : bx);
   }
 };
-#ifdef TGH_BRICK_PI_WAVES_PER_EU
-#define TGH_BRICK_PI_ATTR __attribute__((amdgpu_waves_per_eu(TGH_BRICK_PI_WAVES_PER_EU)))
-#else
-#define TGH_BRICK_PI_ATTR
-#endif
 template <int Q, int TC = 0>
 __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, const double* __restrict__ W,
                                               const double* __restrict__ coef, TGH_OUT_PARAMS, int X0, int Y0,
                                               int Z0);
 template <int Q, int TC>
-__global__ __launch_bounds__((PiTile<Q, TC>::X * PiTile<Q, TC>::Y * PiTile<Q, TC>::Z)) TGH_BRICK_PI_ATTR void k_brick_pi(
+__global__ __launch_bounds__((PiTile<Q, TC>::X * PiTile<Q, TC>::Y * PiTile<Q, TC>::Z)) void k_brick_pi(
     RhsArgs a, Brick k, const double* __restrict__ W, const double* __restrict__ coef, TGH_OUT_PARAMS) {
   using T = PiTile<Q, TC>;
   int X0, Y0, Z0;
@@ -2280,7 +2303,7 @@ __device__ __forceinline__ int batch_find(const BatchEntry* __restrict__ tab, in
   return p;
 }
 template <int Q, int GAUGE, int HOLE>
-__global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head_batch(const BatchEntry* __restrict__ tab, int n,
+__global__ __launch_bounds__(256) void k_brick_head_batch(const BatchEntry* __restrict__ tab, int n,
                                                                         int total) {
   const int L = wg_logical(total);
   if (L >= total) return;
@@ -2291,7 +2314,7 @@ __global__ __launch_bounds__(256) TGH_HEAD_ATTR void k_brick_head_batch(const Ba
                                 (double*)e.us, (double*)e.acc, (double*)e.Wn, x, y, z);
 }
 template <int Q>
-__global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) TGH_BRICK_PI_ATTR void k_brick_pi_batch(
+__global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) void k_brick_pi_batch(
     const BatchEntry* __restrict__ tab, int n, int total) {
   const int L = wg_logical(total);
   if (L >= total) return;  // uniform over the workgroup
@@ -2384,17 +2407,6 @@ struct BrickCopyArgs {
   double* __restrict__ dst1;
   int pass;  // out: 0 copies, 1 primaries
 };
-// The primary (block, local index) of brick point gp.
-__device__ __forceinline__ int brick_primary(const Geom& g, const int gp[3], int loc[3]) {
-  int ob[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int o = gp[d] / g.N;
-    ob[d] = o < g.nb[d] - 1 ? o : g.nb[d] - 1;
-    loc[d] = gp[d] - ob[d] * g.N;
-  }
-  return g.lookup[ob[0] + g.nb[0] * (ob[1] + g.nb[1] * ob[2])];
-}
 __global__ __launch_bounds__(256) void k_brick_in(BrickCopyArgs a) {
   const Brick& k = a.k;
   const Geom& g = a.g;
@@ -2572,7 +2584,7 @@ __global__ __launch_bounds__(256) void k_bounds(BoundsArgs a) {
     __syncthreads();
   }
   if (tid == 0) {
-    double* o = a.out + 3 * ((long)b * ((g.P * g.P * g.P + 255) / 256) + wx);
+    double* o = a.out + RED_STRIDE * ((long)b * ((g.P * g.P * g.P + 255) / 256) + wx);
     o[0] = sh[0][0];
     o[1] = sh[1][0];
     o[2] = sh[2][0];

@@ -115,16 +115,8 @@ void launch_brick_rhs_q(tgh_problem* p, const RhsArgs& a) {
                        a.W, a.du, a.us, a.acc, a.Wn);
     if (prof) p->ev_rhs.push_back({e0, mark(p)});
   } else {
-    {
-#ifdef TGH_HEAD_TILE_X
-      constexpr int HX = TGH_HEAD_TILE_X, HZ = 4, HY = 256 / (HX * HZ);
-      const dim3 hgrid = grid_brick_tiles(p, HX, HY, HZ);
-#else
-      const dim3 hgrid = grid_brick(p);
-#endif
-      hipLaunchKernelGGL((k_brick_head<Q, GAUGE, HOLE>), hgrid, dim3(256), 0, p->stream, a, p->brick,
-                         a.W, p->Bcoef, a.du, a.us, a.acc, a.Wn);
-    }
+    hipLaunchKernelGGL((k_brick_head<Q, GAUGE, HOLE>), grid_brick(p), dim3(256), 0, p->stream, a, p->brick,
+                       a.W, p->Bcoef, a.du, a.us, a.acc, a.Wn);
     int e1 = prof ? mark(p) : -1;
     switch (p->pi_tile) {
       case 1: launch_brick_pi<Q, 1>(p, a); break;

@@ -40,12 +40,13 @@ struct tgh_problem {
   double* acc = nullptr;              // RK4 accumulator, state layout
   double* coef = nullptr;             // the split form's per-point coefficients (CF_N x points)
   double* Hsrc = nullptr;             // sampled gauge source, state layout x 20
-  double* red = nullptr;              // per-block reduction partials (3 per block)
+  double* red = nullptr;              // reduction partials (RED_STRIDE per workgroup)
   std::vector<double> red_host;
   long state_len = 0, work_len = 0;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
   double graph_key[3] = {0, 0, 0};    // (dt, u pointer, t) the graph was captured for
+  unsigned version = 0;               // bumped when a setter changes the case (batches re-read it)
   // Benchmark timing: event pairs around each launch while profiling.
   bool profiling = false;
   bool fused = false;                 // one kernel per RHS instead of head + Pi
@@ -67,9 +68,32 @@ struct tgh_problem {
   std::vector<std::pair<int, int>> ev_rhs, ev_fill, ev_pi;  // indices into ev_pool
   size_t ev_used = 0;
   double prof[6] = {0, 0, 0, 0, 0, 0};
+  // The adjoint's device memory (tgh_vjp_rhs, tgh_adjoint; allocated at
+  // their first call): the VJP's scratch fields (AF_N brick fields), the
+  // cotangents lambda, its accumulator and the stage cotangents (20 brick
+  // fields each), the parameters' per-workgroup accumulators; the stage
+  // inputs of one checkpoint segment and the checkpoints (grown on demand).
+  double* adjF = nullptr;
+  double* adjL[2] = {nullptr, nullptr};
+  double* adjK[2] = {nullptr, nullptr};
+  double* adjTh = nullptr;
+  double* adjSlots = nullptr;
+  size_t adjSlotsN = 0;
+  double* adjCkpt = nullptr;
+  size_t adjCkptN = 0;
 };
 
 namespace tgh {
+struct AdjPointArgs;
+struct AdjGatherArgs;
+struct AdjInitArgs;
+struct AdjCopiesArgs;
+// The adjoint's launchers (tgh_adjoint.hip).
+void launch_adj_point(const tgh_problem* p, const AdjPointArgs& a);
+void launch_adj_gather(const tgh_problem* p, const AdjGatherArgs& a);
+void launch_adj_init(const tgh_problem* p, const AdjInitArgs& a);
+void launch_adj_fold_copies(const tgh_problem* p, const AdjCopiesArgs& a);
+void launch_adj_to_primaries(const tgh_problem* p, const AdjCopiesArgs& a);
 dim3 xcd_grid(long total);
 dim3 grid_points(const tgh_problem* p);
 dim3 grid_brick(const tgh_problem* p);
