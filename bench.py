@@ -118,29 +118,76 @@ def flop_count():
     return f_rhs, f_head, f_combine
 
 
-def cpu_baseline(seconds_target=15.0):
-    """The oracle's RK4 step on a bounded sample of the same case: the hole's
-    own 4x4x4 blocks ([-2, 2]^3 at h = 1/16: core, layer and exterior),
-    OpenMP over blocks on this host's cores."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_native():
+    """The oracle built for this host's CPU (g++ -O3 -march=native, OpenMP) into
+    a temporary directory, as the baseline is specified; the prebuilt
+    oracle/liboracle.so (-O3, generic x86-64) when no compiler is at hand.
+    Returns (library path, flags)."""
+    import shutil
+    import subprocess
+    import tempfile
+    here = os.path.dirname(os.path.abspath(__file__))
+    gxx = shutil.which("g++")
+    if gxx:
+        out = os.path.join(tempfile.mkdtemp(prefix="tgh_cpu_"), "liboracle_native.so")
+        flags = ["-O3", "-march=native", "-std=c++17", "-fPIC", "-fopenmp", "-w", "-shared"]
+        r = subprocess.run([gxx] + flags + ["-o", out, os.path.join(here, "oracle", "oracle_capi.cpp")],
+                           capture_output=True, text=True, timeout=180)
+        if r.returncode == 0:
+            return out, " ".join(flags[:2])
+    return os.path.join(here, "oracle", "liboracle.so"), "-O3 generic x86-64 (prebuilt)"
+
+
+def cpu_baseline():
+    """The CPU baseline: the oracle's RK4 step (the reference's algorithm,
+    OpenMP over blocks, one block per thread by owner) built for this host
+    (-O3 -march=native) and run at the host's thread budget (OMP_NUM_THREADS,
+    else its core count), one warm-up and the minimum of 10 timed steps, on
+    two samples: the CPU-runnable workload (one 16^3 periodic Minkowski block
+    with noise, q = 4) and the Kerr-Schild hole's own 64 blocks ([-2,2]^3 at
+    h = 1/16: core, layer and exterior of the 4096-block workload's hole)."""
     import oracle_lib
     import tgh_desc as TD
-    threads = min(16, os.cpu_count() or 1)
-    d = TD.kerr_schild_hole_case(N=16, nblocks=4, halfwidth=2.0, a=0.7)
-    p = oracle_lib.OracleProblem(d)
-    u = p.fill_exact(0.0)
-    dt = 0.25 * (1 / 16) / p.max_speed(u)
-    p.rk4_step(u, 0.0, dt, threads)  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        p.rk4_step(u, (n + 1) * dt, dt, threads)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > seconds_target or n >= 50:
-            break
-    points = 64 * 16 ** 3
-    return {"value": points * n / el, "unit": "point-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle RK4 steps of the same case on the hole's 64 blocks ([-2,2]^3, h=1/16, "
-                      f"{points} points), {n} steps in {el:.1f} s, {threads} OpenMP threads"}
+    path, flags = _oracle_native()
+    oracle_lib.LIB_PATH = path
+    oracle_lib._lib = None
+    threads = int(os.environ.get("OMP_NUM_THREADS") or (os.cpu_count() or 1))
+
+    def min_step(desc, points, noise=False):
+        p = oracle_lib.OracleProblem(desc)
+        u = p.fill_exact(0.0)
+        if noise:
+            p.add_noise(u, 0x5EED20260917, 1e-6)
+        dt = 0.25 * (desc.hi[0] - desc.lo[0]) / (desc.nblocks[0] * desc.N) / p.max_speed(u)
+        p.rk4_step(u, 0.0, dt, threads)  # warm-up
+        best = float("inf")
+        for n in range(10):
+            t0 = time.perf_counter()
+            p.rk4_step(u, (n + 1) * dt, dt, threads)
+            best = min(best, time.perf_counter() - t0)
+        return points / best, best
+
+    v1, t1 = min_step(TD.minkowski_noise_case(N=16, q=4), 16 ** 3, noise=True)
+    out = {"cpu_runnable_block": {"value": v1, "step_ms": 1e3 * t1}}
+    hole = TD.kerr_schild_hole_case(N=16, nblocks=4, halfwidth=2.0, a=0.7)
+    v2, t2 = min_step(hole, 64 * 16 ** 3)
+    return {"value": v2, "unit": "point-updates/s", "cores": threads, "kind": "port",
+            "cpu": _cpu_model(), "flags": flags, "timing": "min of 10 steps after one warm-up",
+            "configs": out,
+            "sample": f"oracle RK4 step of the same case on the hole's 64 blocks ([-2,2]^3, h=1/16, "
+                      f"{64 * 16 ** 3} points): min {1e3 * t2:.0f} ms over 10 steps, {threads} OpenMP "
+                      f"threads, {flags}"}
 
 
 # The batch workload: four Kerr-Schild holes a = 0, 0.3, 0.5, 0.7 (closed-form

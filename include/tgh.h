@@ -193,6 +193,13 @@ int tgh_transfer(tgh_problem* from, const double* u_from, double t, tgh_problem*
  * src/evolution.jl:1256-1272). The library keeps the pointer (the caller owns
  * the array and may refill it between calls); NULL detaches it. */
 int tgh_set_target(tgh_problem* p, const double* target, double t_target, int32_t rate, double trail);
+/* gh_integrator(p, u, tspan; matter) — src/stepping.jl:150-154,208-225
+ * (MatterRHS) with the in-kernel adapter of src/evolution.jl:1295-1311: the
+ * matter fields every right-hand side tgh_rk4 evaluates adds to the source,
+ * held fixed over the steps (device arrays owned by the caller, laid out as
+ * tgh_matter says; NULL or rho NULL: vacuum again). A problem with matter
+ * steps on the block layout. */
+int tgh_set_matter(tgh_problem* p, const tgh_matter* m);
 /* with_interior(p, with_rho_max(p.interior, rho_max)) — src/evolution.jl
  * with_interior, src/interior.jl with_ρ_max; what evolve!'s chunk loop does
  * per chunk (chunk_interior, src/driver.jl:2391-2411): the layer's

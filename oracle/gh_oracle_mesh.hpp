@@ -300,6 +300,9 @@ struct Problem {
   StateBounds bounds;
   std::vector<double> work;
   std::vector<double> Hsrc;  // sampled gauge source at owned points, 20 per point
+  // matter's 3+1 projections at owned points (empty: vacuum): rho P^3 x nb,
+  // S P^3 x 3 x nb, S_ij P^3 x 6 x nb (xx, xy, xz, yy, yz, zz)
+  std::vector<double> mat_rho, mat_S, mat_Sij;
   std::vector<double> w1, w2, wD;
 
   void setup() {
@@ -534,6 +537,21 @@ struct Problem {
       ks_algebraic_source<T>(cs.bg.M, cs.ks_S, cs.ks_u, hv, dth, dh, Hl, dHl);
     }
     V10<T> msrc = gh_node_source_lean(mq, dth, dh, Hl, dHl, g0, g2);
+    if (!mat_rho.empty()) {
+      // the matter adapter's (rho, S_i, S_ij) at the point (evolution.jl:1295-1311)
+      const long P3 = m.P * m.P * m.P, pi = i + m.P * (j + m.P * k);
+      const T rho = T(mat_rho[b * P3 + pi]);
+      T S[3], Sij[3][3];
+      for (int c = 0; c < 3; ++c) S[c] = T(mat_S[(b * 3 + c) * P3 + pi]);
+      const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+      for (int n = 0; n < 6; ++n) {
+        const T v = T(mat_Sij[(b * 6 + n) * P3 + pi]);
+        Sij[ij[n][0]][ij[n][1]] = v;
+        Sij[ij[n][1]][ij[n][0]] = v;
+      }
+      const V10<T> ms = matter_source(mq, rho, S, Sij);
+      for (int v = 0; v < NC; ++v) msrc[v] = msrc[v] + ms[v];
+    }
     // (4) the coefficients of the Pi components
     T A[3][3];
     for (int a = 0; a < 3; ++a)

@@ -438,6 +438,20 @@ int orc_rk4_step_jvp(void* p, double* u, double* udot, double t, double dt, cons
 double orc_error_tangent(void* p, const double* u, const double* udot, double t) {
   return ((Problem*)p)->error_tangent(u, udot, t);
 }
+// The matter fields every right-hand side adds (NULL rho: vacuum again).
+void orc_set_matter(void* pp, const double* rho, const double* S, const double* Sij) {
+  Problem* p = (Problem*)pp;
+  const long n = p->mesh.P * p->mesh.P * p->mesh.P * p->mesh.nblocks();
+  if (!rho) {
+    p->mat_rho.clear();
+    p->mat_S.clear();
+    p->mat_Sij.clear();
+    return;
+  }
+  p->mat_rho.assign(rho, rho + n);
+  p->mat_S.assign(S, S + 3 * n);
+  p->mat_Sij.assign(Sij, Sij + 6 * n);
+}
 // with_interior(p, with_rho_max(p.interior, rho)): the layer's rate.
 void orc_set_rho_max(void* p, double rho) { ((Problem*)p)->cs.rho_max = rho; }
 void orc_error_rows(void* p, const double* u, double t, double shell_lo, double shell_hi, double* out) {

@@ -87,6 +87,7 @@ def lib():
             "orc_error_norms": (None, [P, dp, D, C.POINTER(D), C.POINTER(D)]),
             "orc_error_rows": (None, [P, dp, D, D, D, dp]),
             "orc_set_rho_max": (None, [P, D]),
+            "orc_set_matter": (None, [P, P, P, P]),
             "orc_is_primary": (C.c_int, [P, Lg, Lg, Lg, Lg]),
             "orc_count_rhs_flops": (D, [P, Lg, Lg, Lg, Lg, C.c_int]),
             "orc_max_threads": (C.c_int, []),
@@ -237,6 +238,15 @@ class OracleProblem:
 
     def max_speed(self, u, t=0.0):
         return lib().orc_max_speed(self.p, u, t)
+
+    def set_matter(self, rho, S, Sij):
+        """Matter fields (host arrays in the device layout) every right-hand
+        side adds; None clears them."""
+        if rho is None:
+            lib().orc_set_matter(self.p, None, None, None)
+            return
+        self._mat = [np.ascontiguousarray(x, dtype=np.float64) for x in (rho, S, Sij)]
+        lib().orc_set_matter(self.p, *[x.ctypes.data for x in self._mat])
 
     def set_rho_max(self, rho):
         lib().orc_set_rho_max(self.p, float(rho))

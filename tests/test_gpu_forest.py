@@ -394,3 +394,25 @@ def test_driver_record_of_the_gauge_wave(tgh):
     print("gauge wave driver records: " + " ".join(f"{r['err_l2']:.6e}" for r in recs))
     assert len(recs) == 3
     assert abs(recs[-1]["err_l2"] - 7.683712e-9) <= 1.0e-15
+
+
+def test_driver_record_of_the_hole_fixture(tgh):
+    """bench/stepping.jl's driver mode on the hole fixture (hole_fixture with
+    q = 4 and halfwidth 5: the Schwarzschild hole M = 1 with the sampled
+    gauge source, :damped layer r_0 = 2/5, r_1 = 23/20, margin 8; hole_forest
+    with roots = 2, radii (6, 3, 3/2), N = 16, prolongation of order 6; three
+    chunks of 1/10): the record's err_l2 = 1.690514e-06 (CODE.md:6478-6480),
+    to one unit in its last printed digit. The blocks are half-open (TreeAMR
+    0.1's ownership, under which the published frozen-hierarchy table is
+    reproduced)."""
+    torch_cuda()
+    case = tgh.kerr_schild_case(M=1.0, a=0.0, halfwidth=5.0, r_0=0.4, r_1=1.15, margin=8)
+    f = tgh.hole_forest(None, case, N=16, roots=2, radii=(6.0, 3.0, 1.5), ownership="half_open")
+    p = tgh.GHProblem(f, case, q=4, ops=tgh.Operators(prolongation=6))
+    u = p.statevector()
+    tgh.initial_state(u, p, 0.0)
+    recs = tgh.evolve(p, u, t_end=3 / 10, chunk=1 / 10)
+    print(f"hole fixture ({f.nleaves()} leaves) driver records: " +
+          " ".join(f"{r['err_l2']:.6e}" for r in recs))
+    assert len(recs) == 3
+    assert abs(recs[-1]["err_l2"] - 1.690514e-6) <= 1.0e-12

@@ -100,12 +100,13 @@ def test_transfer_refuses_other_bricks_and_two_level_jumps(tgh):
         tgh.transfer(other, other.statevector(), 0.0, dflat, dflat.statevector())
 
 
-def _hole_case(tgh, v=0.0):
+def _hole_case(tgh, v=0.0, halfwidth=4.0):
     """A Kerr-Schild hole (a = 0.5, the closed-form source) whose layer
-    r_0 = 0.7, r_1 = 1.2 sits m = 8 level-1 spacings inside the horizon at
-    root spacing 1/8, so the refinement's level floor is 1 (src/refinement.jl:
-    436-470) on the shell from r_1 to the horizon's largest radius."""
-    return tgh.kerr_schild_case(M=1.0, a=0.5, v=v, axis=1, halfwidth=4.0, r_0=0.7, r_1=1.2,
+    r_0 = 0.7, r_1 = 1.2 needs level-1 spacing 1/16 at root spacing 1/8
+    (r_1 - r_0 >= 2(G+1)h with G = 3), so the refinement's level floor is 1
+    (src/refinement.jl:436-470) on the shell from r_1 to the horizon's
+    largest radius (1.93) plus the floor's margin."""
+    return tgh.kerr_schild_case(M=1.0, a=0.5, v=v, axis=1, halfwidth=halfwidth, r_0=0.7, r_1=1.2,
                                 gauge_source="algebraic")
 
 
@@ -114,26 +115,31 @@ def _tau(tgh, p, u, t):
 
 
 def test_regrid_refines_where_the_indicator_fires_and_carries_the_state(tgh):
-    """A Kerr-Schild hole on a uniform brick: the blocks whose Löhner maximum
-    passes refine_tol refine once, so do the blocks the level floor holds,
-    the state on the new leaves is the oracle's transfer, and the new problem
-    steps."""
+    """A Kerr-Schild hole in [-6, 6]^3 on 6^3 root blocks of 16^3 whose eight
+    central blocks hold the layer at level 1 (hole_forest): with the floor's
+    shell reaching 1/2 past the horizon's largest radius, the 24 root blocks
+    face-adjacent to the centre are held at level 1 by the floor, so they
+    refine, and so do the root blocks whose Löhner maximum passes refine_tol
+    (the upper quartile); the state on the new leaves is the oracle's
+    transfer, and the new problem steps."""
     torch = torch_cuda()
-    case = _hole_case(tgh)
-    f = tgh.gh_forest(None, case, N=16, roots=4)
+    case = _hole_case(tgh, halfwidth=6.0)
+    f = tgh.hole_forest(None, case, N=16, roots=6, radii=(1.5,))
+    assert f.nleaves() == 272 and f.maxlevel() == 1
     dev, orc = _problem(tgh, f, case, 6)
     u = dev.statevector()
     tgh.initial_state(u, dev, 0.0)
     tau = _tau(tgh, dev, u, 0.0)
     thr = sorted(tau)[len(tau) * 3 // 4]
-    ref = tgh.Refinement(refine_tol=thr, coarsen_tol=thr * 1e-3, maxlevel_cap=1, floor_margin=0.0)
+    ref = tgh.Refinement(refine_tol=thr, coarsen_tol=thr * 1e-3, maxlevel_cap=1, floor_margin=0.5)
     bounds = tgh.level_bounds(dev.forest, dev.case, dev.q, ref, 0.0, dev.leaves())
-    assert sum(1 for b in bounds if b[0] == 1) == 8  # the eight root blocks around the hole
+    held = [lf for lf, (flo, _) in zip(dev.leaves(), bounds) if flo == 1 and lf[0] == 0]
+    assert len(held) == 24  # the root blocks face-adjacent to the central eight
     new, u_new, flags = tgh.regrid(dev, u, 0.0, refinement=ref)
     fired = [lf for lf, fl in zip(dev.leaves(), flags) if fl > 0]
-    assert 8 <= len(fired) < f.nleaves()
+    assert 24 <= len(fired) < f.nleaves()
     for lf, (flo, _), t in zip(dev.leaves(), bounds, tau):
-        assert (lf in fired) == (flo == 1 or t > thr)
+        assert (lf in fired) == (lf[0] == 0 and (flo == 1 or t > thr)), (lf, flo, t, thr)
     for lf in fired:
         assert lf not in new.forest.leaves
         assert (1, 2 * lf[1], 2 * lf[2], 2 * lf[3]) in new.forest.leaves

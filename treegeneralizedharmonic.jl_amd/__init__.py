@@ -136,6 +136,7 @@ def _load():
         "tgh_error_norms": (C.c_int, [P, P, D, C.POINTER(D), C.POINTER(D)]),
         "tgh_error_rows": (C.c_int, [P, P, D, D, D, C.POINTER(D)]),
         "tgh_set_rho_max": (C.c_int, [P, D]),
+        "tgh_set_matter": (C.c_int, [P, C.POINTER(TghMatter)]),
         "tgh_vjp_rhs": (C.c_int, [P, P, D, P, P, C.POINTER(D)]),
         "tgh_adjoint": (C.c_int, [P, P, D, D, C.c_int32, C.c_int32, C.POINTER(D), C.POINTER(D), C.POINTER(D)]),
         "tgh_copy_work": (C.c_int, [P, P]),
@@ -1380,25 +1381,18 @@ def gh_solve(p, u, tspan, *, dt, graph=False):
     t0, t1 = float(tspan[0]), float(tspan[1])
     nsteps = max(1, math.ceil((t1 - t0) / dt * (1 - 4 * 2.0 ** -52)))
     step = (t1 - t0) / nsteps
-    case = getattr(p, "case", None)
-    if case is not None and case.interior is not None and case.interior.variant == "pasted":
-        # The fused stepper does not run the paste; use the verbs.
-        import torch
-        k = [torch.empty_like(u) for _ in range(4)]
-        for n in range(nsteps):
-            t = t0 + n * step
-            y = u.clone()
-            gh_rhs(k[0], u, p, t)
-            torch.add(u, k[0], alpha=step / 2, out=y); stage_limit(y, p, t + step / 2)
-            gh_rhs(k[1], y, p, t + step / 2)
-            torch.add(u, k[1], alpha=step / 2, out=y); stage_limit(y, p, t + step / 2)
-            gh_rhs(k[2], y, p, t + step / 2)
-            torch.add(u, k[2], alpha=step, out=y); stage_limit(y, p, t + step)
-            gh_rhs(k[3], y, p, t + step)
-            u.add_((k[0] + 2 * k[1] + 2 * k[2] + k[3]) * (step / 6)); step_limit(u, p, t + step)
-        return u
     _check(_load().tgh_rk4(p.ptr, _dev(u, p, "u"), t0, step, nsteps, TGH_RK4_GRAPH if graph else 0))
     return u
+
+
+def set_matter(p, matter):
+    """gh_integrator(...; matter) (src/stepping.jl:150-154): matter fields
+    (rho, S, Sij as float64 CUDA tensors in the tgh_matter layout, or None for
+    vacuum) that every right-hand side of gh_solve adds, held fixed over the
+    steps. The problem keeps a reference to them."""
+    _check(_load().tgh_set_matter(p.ptr, _matter(matter)))
+    p.matter = matter
+    return p
 
 
 def set_rho_max(p, rho_max):

@@ -458,6 +458,7 @@ int ensure_brick(tgh_problem* p) {
 }
 
 void launch_bounds_brick(tgh_problem* p, double* A, double* B);
+void launch_paste(tgh_problem* p, int mode, double* A, double* B, double t);
 
 // One RK4 step on the brick arrays (u_n in Bus, the stage input in BW[0]).
 void rk4_brick_launches(tgh_problem* p, double t, double dt) {
@@ -487,8 +488,10 @@ void rk4_brick_launches(tgh_problem* p, double t, double dt) {
       case 6: launch_rhs_brick<6>(p, a); break;
       default: launch_rhs_brick<8>(p, a); break;
     }
-    // the stage limiter on the stage value (and on the step's result)
+    // the stage limiter on the stage value (and on the step's result): the
+    // range projection, then the :pasted overwrite
     if (p->has_bounds) launch_bounds_brick(p, s == 3 ? p->Bus : Wn, s == 3 ? Wn : nullptr);
+    if (p->cs.interior == INT_PASTED) launch_paste(p, 1, s == 3 ? p->Bus : Wn, s == 3 ? Wn : nullptr, t_stage[s < 3 ? s + 1 : 3]);
   }
 }
 
@@ -536,6 +539,21 @@ void launch_bounds_brick(tgh_problem* p, double* A, double* B) {
   if (n > 0) hipLaunchKernelGGL(k_bounds, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, a);
 }
 
+// The :pasted limiter on a stage value at time t (mode 0: a work array's
+// owned points and optionally the state; mode 1: brick arrays).
+void launch_paste(tgh_problem* p, int mode, double* A, double* B, double t) {
+  PasteArgs a{};
+  a.g = p->geom;
+  a.k = p->brick;
+  a.c = p->cs;
+  a.t = t;
+  a.dst0 = A;
+  a.dst1 = B;
+  a.mode = mode;
+  const dim3 grid = mode == 1 ? grid_brick(p) : grid_points(p);
+  hipLaunchKernelGGL(p->cs.moving ? k_paste<true> : k_paste<false>, grid, dim3(256), 0, p->stream, a);
+}
+
 // One RK4 step on device arrays: u (state, in place), stage work arrays
 // W[0] (holding u_n with ghosts unfilled) and W[1].
 void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
@@ -549,7 +567,7 @@ void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
     int e0 = prof ? mark(p) : -1;
     launch_fill(p, Wc, t_stage[s]);
     if (prof) p->ev_fill.push_back({e0, mark(p)});
-    RhsArgs a = rhs_args(p, Wc, t_stage[s], nullptr);
+    RhsArgs a = rhs_args(p, Wc, t_stage[s], p->mat.rho ? &p->mat : nullptr);
     a.us = u;
     a.acc = p->acc;
     a.Wn = Wn;
@@ -557,8 +575,10 @@ void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
     a.cnext = c_next[s];
     a.stage = s + 1;
     launch_rhs(p, a, EPI_RK4);
-    // the stage limiter on the stage value (and on the step's result)
+    // the stage limiter on the stage value (and on the step's result): the
+    // range projection, then the :pasted overwrite
     if (p->has_bounds) launch_bounds_work(p, Wn, s == 3 ? u : nullptr);
+    if (p->cs.interior == INT_PASTED) launch_paste(p, 0, Wn, s == 3 ? u : nullptr, t_stage[s < 3 ? s + 1 : 3]);
   }
 }
 
@@ -799,6 +819,19 @@ int tgh_set_target(tgh_problem* p, const double* target, double t_target, int32_
   p->cs.t_f = t_target;
   p->cs.rate = rate != 0;
   p->cs.trail = trail;
+  ++p->version;
+  if (p->graph_exec) {  // a captured graph holds the old arguments
+    HIP_TRY(hipGraphExecDestroy(p->graph_exec));
+    p->graph_exec = nullptr;
+  }
+  return TGH_OK;
+}
+
+int tgh_set_matter(tgh_problem* p, const tgh_matter* m) {
+  if (!p) return fail(TGH_EINVAL, "NULL problem");
+  if (m && m->rho && (!m->S || !m->Sij)) return fail(TGH_EINVAL, "matter needs rho, S and Sij together");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->mat = (m && m->rho) ? *m : tgh_matter{nullptr, nullptr, nullptr};
   ++p->version;
   if (p->graph_exec) {  // a captured graph holds the old arguments
     HIP_TRY(hipGraphExecDestroy(p->graph_exec));
@@ -1175,15 +1208,13 @@ int tgh_adm_constraint(tgh_problem* p, const double* u, double t, double* l2, do
 int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int32_t flags) {
   if (!p || !u) return fail(TGH_EINVAL, "NULL argument");
   if (nsteps < 0 || !(dt > 0)) return fail(TGH_EINVAL, "tgh_rk4 needs dt > 0 and nsteps >= 0");
-  if (p->cs.interior == INT_PASTED)
-    return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
   if (p->cs.interior == INT_FITTED && !p->cs.tw)
     return fail(TGH_EINVAL, "the :fitted variant reads its target cache, and none is set: call "
                             "tgh_set_target first");
   if (nsteps == 0) return TGH_OK;
   const bool graph = (flags & TGH_RK4_GRAPH) != 0;
-  // a forest with levels and the general interior step on the block layout
-  const bool blocks = (flags & TGH_RK4_BLOCKS) != 0 || p->forest || p->cs.general;
+  // a forest with levels, the general interior and matter step on the block layout
+  const bool blocks = (flags & TGH_RK4_BLOCKS) != 0 || p->forest || p->cs.general || p->mat.rho != nullptr;
   if (flags & ~(TGH_RK4_GRAPH | TGH_RK4_FUSED | TGH_RK4_BLOCKS | TGH_RK4_SPLIT | TGH_RK4_PER_BLOCK))
     return fail(TGH_EINVAL, "unknown tgh_rk4 flags " + std::to_string(flags));
   if ((flags & TGH_RK4_PER_BLOCK) && (flags & TGH_RK4_SPLIT))
@@ -1310,6 +1341,7 @@ int adj_prepare(tgh_problem* p) {
                                   "interior (no levels, no tracked or :fitted geometry)");
   if (p->cs.moving) return fail(TGH_EUNSUPPORTED, "the adjoint is built for holes at rest");
   if (p->has_bounds) return fail(TGH_EUNSUPPORTED, "the adjoint is built for cases without range bounds");
+  if (p->mat.rho) return fail(TGH_EUNSUPPORTED, "the adjoint is built for vacuum (no matter fields)");
   if (p->cs.interior == INT_PASTED) return fail(TGH_EUNSUPPORTED, "the adjoint of the :pasted limiter is not built");
   if (p->cs.gauge != GAUGE_NONE && p->cs.gauge != GAUGE_SAMPLED && p->cs.gauge != GAUGE_KS)
     return fail(TGH_EUNSUPPORTED, "the adjoint is built for the static gauge-source kinds");
@@ -1755,7 +1787,10 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
                               "interior (one kernel specialisation runs them all); problem " +
                                   std::to_string(i) + " differs from problem 0");
     if (p->cs.interior == INT_PASTED)
-      return fail(TGH_EUNSUPPORTED, "the fused stepper does not run the :pasted limiter; step with the verbs");
+      return fail(TGH_EUNSUPPORTED, "a batch does not run the :pasted limiter; step problem " + std::to_string(i) +
+                                        " alone (tgh_rk4)");
+    if (p->mat.rho)
+      return fail(TGH_EUNSUPPORTED, "a batch steps vacuum problems; problem " + std::to_string(i) + " has matter");
     if (blocks && (p->cs.gauge == GAUGE_KS_MOVING || p->cs.gauge == GAUGE_NONE_MOVING))
       return fail(TGH_EUNSUPPORTED, "a batch on the block layout (forests with levels, tracked or :fitted "
                                     "interiors) of moving holes is not built; problem " + std::to_string(i) + " moves");

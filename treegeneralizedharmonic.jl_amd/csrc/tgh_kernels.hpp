@@ -2591,6 +2591,59 @@ __global__ __launch_bounds__(256) void k_bounds(BoundsArgs a) {
   }
 }
 
+// The :pasted limiter (gh_paste_kernel!, src/evolution.jl:880-905) on a
+// stage value: the exact state, with the core rule, at every point inside the
+// layer's outer surface (r < r_1).
+//   mode 0: block layout: the owned points of the work array dst0 and, when
+//           given, of the state dst1;
+//   mode 1: brick layout: every brick point of dst0 and, when given, dst1.
+struct PasteArgs {
+  Geom g;
+  Brick k;
+  CaseP c;
+  double t;
+  double* __restrict__ dst0;
+  double* __restrict__ dst1;
+  int mode;
+};
+template <bool MOVE>
+__global__ __launch_bounds__(256) void k_paste(PasteArgs a) {
+  double x[3];
+  long i0, s0, i1 = 0, s1 = 0;
+  if (a.mode == 1) {
+    const Brick& k = a.k;
+    int bx, by, bz;
+    if (!brick_map(k, bx, by, bz)) return;
+    x[0] = k.lo[0] + (double)bx * k.h;
+    x[1] = k.lo[1] + (double)by * k.h;
+    x[2] = k.lo[2] + (double)bz * k.h;
+    i0 = i1 = brick_index(k, bx, by, bz);
+    s0 = s1 = k.fs;
+  } else {
+    const Geom& g = a.g;
+    const int P = g.P, S = g.S, G = g.G;
+    int b, wx;
+    if (!wg_map((P * P * P + 255) / 256, g.nblocks, b, wx)) return;
+    const int p = wx * blockDim.x + threadIdx.x;
+    if (p >= P * P * P) return;
+    const int i = p % P, j = (p / P) % P, kk = p / (P * P);
+    position(g, b, i, j, kk, x);
+    const long sv = (long)S * S * S, psz = (long)P * P * P;
+    i0 = (long)b * 20 * sv + (i + G) + (long)S * ((j + G) + (long)S * (kk + G));
+    s0 = sv;
+    i1 = (long)b * 20 * psz + p;
+    s1 = psz;
+  }
+  if (!(hole_radius(a.c, a.t, x) < a.c.r1)) return;
+  double st[20];
+  case_state_any<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, st);
+#pragma unroll 4
+  for (int v = 0; v < 20; ++v) {
+    a.dst0[i0 + v * s0] = st[v];
+    if (a.dst1) a.dst1[i1 + v * s1] = st[v];
+  }
+}
+
 #endif  // TGH_RHS_ONLY
 
 }  // namespace tgh
