@@ -122,8 +122,11 @@ def test_rk4_pasted_matches_oracle(tgh, form):
     assert inside.sum() > 1000
     ex = o.fill_exact(2 * dt).reshape(-1, 20, *inside.shape[1:])
     gv = g.reshape(ex.shape)
-    pasted = np.abs(gv - ex).transpose(1, 0, 2, 3, 4)[:, inside]
-    assert float(pasted.max()) <= 1e-13 * max(1.0, float(np.abs(ex).max()))
+    # per component against max(1, |exact|): the paste's exact state and the
+    # oracle's are two dual passes, equal to a few ulps of values up to ~2e3
+    # near the core
+    rel = (np.abs(gv - ex) / np.maximum(np.abs(ex), 1.0)).transpose(1, 0, 2, 3, 4)
+    assert float(rel[:, inside].max()) <= 1e-12
     outside = np.abs(gv - ex).transpose(1, 0, 2, 3, 4)[:, ~inside]
     assert float(outside.max()) > 1e-8  # the noise outside r_1 is evolved, not pasted
 
@@ -261,6 +264,7 @@ def test_verbs_on_a_segment_are_the_stepper(tgh):
     tgh.add_noise(seg, p, 0xBEEF, 1e-7)
     tgh.add_noise(u0, p, 0xBEEF, 1e-7)
     assert torch.equal(seg, u0)
+    start = u0.cpu().numpy()
 
     # the speed of the segment, and the step of a script
     lam = tgh.max_speed(p, seg, 0.0)
@@ -300,7 +304,7 @@ def test_verbs_on_a_segment_are_the_stepper(tgh):
     got = seg.cpu().numpy()
     diff = _rel(got, ref)
     print(f"verbs on a segment vs tgh_rk4, {nsteps} steps: max rel diff {diff:.3e}; "
-          f"|u_T - u_0| up to {float(np.abs(ref - before[n_other:n_other + n_gh].cpu().numpy()).max()):.3e}")
+          f"|u_T - u_0| up to {float(np.abs(ref - start).max()):.3e}")
     assert diff <= 1e-13
     settled = seg.clone()
     tgh.settle(p, seg, nsteps * dt)
@@ -338,7 +342,7 @@ def test_limiters_on_a_segment_are_the_vectors_own(tgh, which):
     assert not torch.equal(seg, before[n_other:n_other + n_gh])  # the planted failures were limited
     want = h.copy()
     o.stage_limit(want, 0.1)
-    assert _rel(seg.cpu().numpy(), want) <= 1e-13
+    assert _rel(seg.cpu().numpy(), want) <= 1e-12
     seg2 = before[n_other:n_other + n_gh].clone()
     tgh.step_limit(seg2, p, 0.1)
     assert torch.equal(seg2, seg)
