@@ -272,6 +272,17 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
  * the masked norms of the four components over the evolved points
  * (4 doubles each, host memory), counted as tgh_error_norms counts them. */
 int tgh_gauge_constraint(tgh_problem* p, const double* u, double t, double* l2, double* linf);
+/* No counterpart in the reference: tgh_gauge_constraint for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The same kernels and the same fold
+ * order, so the same norms to the bit; the per-workgroup partials and their
+ * fold live in the library's static device array (8 MiB on each device it
+ * runs on, held until the process ends) instead of memory the problem
+ * allocates on its first monitor call. Calls no hipMalloc or hipFree. Limit:
+ * 116 506 workgroup partials, i.e. 5825 blocks of 17^3 points (N = 16); a
+ * larger problem gets TGH_EINVAL. The problem's device must be current.
+ * Calls that overlap on one device share the array and race. */
+int tgh_gauge_constraint_noalloc(tgh_problem* p, const double* u, double t, double* l2, double* linf);
 /* adm_constraint!(p, u, t) then constraint_norms(p)'s Hamiltonian and
  * momentum rows (src/constraints.jl:100-210,432-534,584-605,848-876): the
  * ADM constraints H = 2 G_nn and M_i = -G_n i at every owned point, from the

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -620,7 +621,10 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
     }
     if (!msg.empty()) return fail(TGH_EINVAL, msg);
   }
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
   auto* p = new tgh_problem();
+  p->device = device;
   p->desc = *d;
   p->q = d->q;
   Geom& g = p->geom;
@@ -1108,13 +1112,15 @@ int tgh_error_norms(tgh_problem* p, const double* u, double t, double* l2, doubl
 // The two constraint monitors: prepare u at t, launch the monitor (adm
 // selects the ADM one), and fold its 9 partials per workgroup into the masked
 // norms of its four components.
-static int monitor(tgh_problem* p, const double* u, double t, bool adm, double* l2, double* linf) {
+// The monitor's launches: the state into the work array, the ghost fill, and
+// the constraint kernel, which writes 9 partials per workgroup into gc
+// ([0..3] the weighted squares of the four components, [4..7] their largest
+// magnitudes, [8] the weight).
+static int launch_monitor(tgh_problem* p, const double* u, double t, bool adm, double* gc) {
   const Geom& g = p->geom;
-  const long nwg = (long)g.nblocks * ((g.P * g.P * g.P + 255) / 256);
-  if (!p->gc) HIP_TRY(hipMalloc(&p->gc, sizeof(double) * 9 * nwg));
   launch_scatter(p, u, p->W[0], 1);
   launch_fill(p, p->W[0], t);
-  GCArgs a{g, p->cs, t, p->W[0], p->Hsrc, p->gc};
+  GCArgs a{g, p->cs, t, p->W[0], p->Hsrc, gc};
   if (adm) {
     switch (p->q) {
       case 2: hipLaunchKernelGGL(k_adm_constraint<2>, grid_points(p), dim3(256), 0, p->stream, a); break;
@@ -1131,21 +1137,118 @@ static int monitor(tgh_problem* p, const double* u, double t, bool adm, double* 
     }
   }
   HIP_TRY(hipGetLastError());
+  return TGH_OK;
+}
+
+// The norms from the folded partials f: [0..3] the sums of the weighted
+// squares, [4..7] the largest magnitudes, [8] the sum of the weights.
+static void monitor_norms(const double f[9], double* l2, double* linf) {
+  for (int q = 0; q < 4; ++q) {
+    l2[q] = f[8] > 0 ? std::sqrt(f[q] / f[8]) : 0.0;
+    linf[q] = f[4 + q];
+  }
+}
+
+static int monitor(tgh_problem* p, const double* u, double t, bool adm, double* l2, double* linf) {
+  const Geom& g = p->geom;
+  const long nwg = (long)g.nblocks * ((g.P * g.P * g.P + 255) / 256);
+  if (!p->gc) HIP_TRY(hipMalloc(&p->gc, sizeof(double) * 9 * nwg));
+  int rc = launch_monitor(p, u, t, adm, p->gc);
+  if (rc) return rc;
   std::vector<double> h(9 * nwg);
   HIP_TRY(hipMemcpyAsync(h.data(), p->gc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
-  double num[4] = {0, 0, 0, 0}, den = 0, mx[4] = {0, 0, 0, 0};
+  // in workgroup order, so the answer does not depend on the launch
+  double f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (long w = 0; w < nwg; ++w) {
     for (int q = 0; q < 4; ++q) {
-      num[q] += h[9 * w + q];
-      mx[q] = std::max(mx[q], h[9 * w + 4 + q]);
+      f[q] += h[9 * w + q];
+      f[4 + q] = std::max(f[4 + q], h[9 * w + 4 + q]);
     }
-    den += h[9 * w + 8];
+    f[8] += h[9 * w + 8];
   }
-  for (int q = 0; q < 4; ++q) {
-    l2[q] = den > 0 ? std::sqrt(num[q] / den) : 0.0;
-    linf[q] = mx[q];
+  monitor_norms(f, l2, linf);
+  return TGH_OK;
+}
+
+// ---- the gauge monitor without allocation ----------------------------------
+// A file-scope device array holds tgh_gauge_constraint_noalloc's partials
+// (9 per workgroup from offset 0) and their fold (9 doubles at
+// kScratchFold). Each device the library runs on holds its own copy until the
+// process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (scratch_on). Every call on one
+// device shares the array: calls that overlap there race.
+constexpr long kScratchDoubles = 1L << 20;  // 8 MiB
+constexpr long kScratchFold = kScratchDoubles - 16;
+constexpr long kScratchMaxWorkgroups = kScratchFold / 9;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchDevices = 64;
+static std::mutex scratch_mutex;
+static double* scratch_addr[kScratchDevices] = {};
+
+// The array's address on `device` (current), looked up once per device.
+static int scratch_on(int device, double** out) {
+  if (device < 0 || device >= kScratchDevices)
+    return fail(TGH_EUNSUPPORTED, "device " + std::to_string(device) + ": the no-allocation entry points "
+                                  "keep the scratch addresses of devices 0.." +
+                                      std::to_string(kScratchDevices - 1) + " only");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_addr[device]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    scratch_addr[device] = (double*)ptr;
   }
+  *out = scratch_addr[device];
+  return TGH_OK;
+}
+
+// The fold of monitor() on the device: thread q folds column q of the nwg
+// partials in workgroup order (sums for 0..3 and 8, the larger-of for 4..7,
+// the same comparisons as the host's), so the norms are the allocating
+// entry point's to the bit.
+__global__ __launch_bounds__(64) void k_monitor_fold(double* s, long nwg) {
+  const int q = threadIdx.x;
+  if (q >= 9) return;
+  double acc = 0.0;
+  if (q >= 4 && q < 8) {
+    for (long w = 0; w < nwg; ++w) {
+      const double v = s[9 * w + q];
+      acc = acc < v ? v : acc;
+    }
+  } else {
+    for (long w = 0; w < nwg; ++w) acc += s[9 * w + q];
+  }
+  s[kScratchFold + q] = acc;
+}
+
+int tgh_gauge_constraint_noalloc(tgh_problem* p, const double* u, double t, double* l2, double* linf) {
+  if (!p || !u || !l2 || !linf) return fail(TGH_EINVAL, "NULL argument");
+  const Geom& g = p->geom;
+  const long wgpb = (g.P * g.P * g.P + 255) / 256;
+  const long nwg = (long)g.nblocks * wgpb;
+  if (nwg > kScratchMaxWorkgroups)
+    return fail(TGH_EINVAL, "tgh_gauge_constraint_noalloc: " + std::to_string(g.nblocks) + " blocks of " +
+                                std::to_string(g.P) + "^3 points need " + std::to_string(nwg) +
+                                " workgroup partials, above the static scratch's " +
+                                std::to_string(kScratchMaxWorkgroups) + " (" +
+                                std::to_string(kScratchMaxWorkgroups / wgpb) +
+                                " such blocks): use tgh_gauge_constraint");
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  if (device != p->device)
+    return fail(TGH_EINVAL, "the problem's arrays live on device " + std::to_string(p->device) +
+                                " but device " + std::to_string(device) + " is current");
+  double* s = nullptr;
+  int rc = scratch_on(device, &s);
+  if (rc) return rc;
+  rc = launch_monitor(p, u, t, false, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_monitor_fold, dim3(1), dim3(64), 0, p->stream, s, nwg);
+  HIP_TRY(hipGetLastError());
+  double f[9];
+  HIP_TRY(hipMemcpyAsync(f, s + kScratchFold, sizeof(f), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  monitor_norms(f, l2, linf);
   return TGH_OK;
 }
 

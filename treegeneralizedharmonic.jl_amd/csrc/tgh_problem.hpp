@@ -48,6 +48,7 @@ struct tgh_problem {
   double graph_key[3] = {0, 0, 0};    // (dt, u pointer, t) the graph was captured for
   unsigned version = 0;               // bumped when a setter changes the case (batches re-read it)
   tgh_matter mat{nullptr, nullptr, nullptr};  // tgh_set_matter's fields for tgh_rk4 (rho NULL: vacuum)
+  int device = 0;                     // the HIP device current at creation, which holds the arrays
   // Benchmark timing: event pairs around each launch while profiling.
   bool profiling = false;
   bool fused = false;                 // one kernel per RHS instead of head + Pi
