@@ -49,12 +49,19 @@ KERNEL = {"brick-fused": "k_brick_rhs<4, KS, hole>", "blocks-fused": "k_rhs<4, K
           "blocks-split:head": "k_head<4, KS, hole, RK4 epilogue>", "blocks-split:pi": "k_pi<4, RK4 epilogue>"}
 
 
+def note(msg):
+    """A progress line on stderr (the JSON line stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-sampled", action="store_true", help="skip the sampled-gauge-source line")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint's cost line")
     ap.add_argument("--nblocks", type=int, default=16, help="blocks per axis (16 = the full workload)")
     ap.add_argument("--form", choices=tuple(FORMS), default="brick-split",
                     help="the stepper's layout and kernels: the brick layout (default), or the "
@@ -188,6 +195,46 @@ def cpu_baseline():
             "sample": f"oracle RK4 step of the same case on the hole's 64 blocks ([-2,2]^3, h=1/16, "
                       f"{64 * 16 ** 3} points): min {1e3 * t2:.0f} ms over 10 steps, {threads} OpenMP "
                       f"threads, {flags}"}
+
+
+def adjoint_bench(tgh, torch):
+    """The differentiation workload: the a = 0.7 hole on 64 blocks of 16^3
+    ([-2M, 2M]^3, h = M/16, q = 4, closed-form gauge source, Gaussian damping
+    near 1/M far 1/(10M) width 3M, gamma2 = 0.1, the :damped layer), 200 RK4
+    steps of dt = 1/100 to T = 2M, a checkpoint every 20 steps: the gradient
+    of the masked L2 error with respect to (gamma0_near, gamma0_far, gamma2,
+    rho_max, rho_ramp) by the checkpointed reverse sweep. The cost ratio is
+    the forward sweep plus the reverse sweep (segment recomputation
+    included) over the plain forward run (tgh_rk4's hipGraph) of the same
+    200 steps."""
+    case = tgh.kerr_schild_case(M=1.0, a=0.7, halfwidth=2.0, r_0=0.71, r_1=1.21, gauge_source="algebraic",
+                                gamma2=0.1, rho_max=4.0, rho_ramp=0.5)
+    case.gamma0 = tgh.GaussianDamping(near=1.0, far=0.1, width=3.0)
+    p = tgh.GHProblem(tgh.gh_forest(None, case, N=16, roots=4), case, q=4)
+    u = p.statevector()
+    tgh.initial_state(u, p, 0.0)
+    u0 = u.clone()
+    dt, nsteps, every = 0.01, 200, 20
+    tgh.adjoint(p, u, 0.0, dt, nsteps, every=every)  # the first call allocates the checkpoints
+    u.copy_(u0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, grad, (fwd, rev) = tgh.adjoint(p, u, 0.0, dt, nsteps, every=every)
+    total = (time.perf_counter() - t0) * 1e3
+    u.copy_(u0)
+    tgh.gh_solve(p, u, (0.0, nsteps * dt), dt=dt, graph=True)  # graph capture
+    u.copy_(u0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tgh.gh_solve(p, u, (0.0, nsteps * dt), dt=dt, graph=True)
+    torch.cuda.synchronize()
+    plain = (time.perf_counter() - t0) * 1e3
+    return {"workload": "static Kerr-Schild hole M=1 a=0.7 on 64 blocks of 16^3 ([-2M,2M]^3, h=M/16), q=4, "
+                        "closed-form gauge source, 200 RK4 steps of dt=1/100, checkpoints every 20; "
+                        "J = masked L2 error at T=2M",
+            "J": J, "grad": dict(zip(tgh.ADJOINT_PARAMS, grad)),
+            "forward_sweep_ms": fwd, "reverse_sweep_ms": rev, "gradient_wall_ms": total,
+            "plain_forward_ms": plain, "adjoint_over_forward": total / plain}
 
 
 # The batch workload: four Kerr-Schild holes a = 0, 0.3, 0.5, 0.7 (closed-form
@@ -400,6 +447,7 @@ def main():
         tiles_ms = list(ms)
         del scratch
     # warm-up (also captures the graph)
+    note(f"{nb} blocks, {args.form}: warm-up")
     tgh._check(L.tgh_rk4(ptr, uptr, 0.0, dt, args.warmup, 1 | form))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -407,6 +455,7 @@ def main():
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     value = points * args.steps / el
+    note(f"timed {args.steps} steps: {el / args.steps * 1e3:.2f} ms per step")
     assert bool(torch.isfinite(u).all()), "the state is not finite after the timed steps"
 
     # Per-kernel durations from hipEvents on the problem's stream, over a
@@ -490,7 +539,7 @@ def main():
                                f"{nb} blocks of 16^3, q=4, damped interior, KS closed-form gauge "
                                f"source, RK4 hipGraph step",
                    "blocks": nb, "points": points, "q": 4, "dt": dt},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": measured_traffic(KERNEL[dom], nb),
                      "kernel": KERNEL[dom],
@@ -517,16 +566,50 @@ def main():
     result["monitors"] = monitors
     L.tgh_problem_destroy(ptr)
     del k, y
+
+    # The same workload with the gauge source sampled once at creation (the
+    # reference's default H_a = -Gamma_a of the background) instead of the
+    # closed form: the same stepper form, timed the same way.
+    if not args.no_sampled:
+        note("the sampled-source line")
+        torch.cuda.empty_cache()
+        ds = kerr_schild_desc(tgh, args.nblocks)
+        ds.gauge = 1
+        ps = C.c_void_p()
+        tgh._check(L.tgh_problem_create(C.byref(ds), C.byref(ps)))
+        tgh._check(L.tgh_set_stream(ps, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        tgh._check(L.tgh_initial_state(ps, uptr, 0.0))
+        if tiles_ms:
+            tgh._check(L.tgh_set_tile(ps, result["roofline"]["pi_tile"]))
+        tgh._check(L.tgh_rk4(ps, uptr, 0.0, dt, args.warmup, 1 | form))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        tgh._check(L.tgh_rk4(ps, uptr, args.warmup * dt, dt, args.steps, 1 | form))
+        torch.cuda.synchronize()
+        el_s = time.perf_counter() - t1
+        assert bool(torch.isfinite(u).all()), "the sampled-source state is not finite after the timed steps"
+        result["sampled_source"] = {"value": points * args.steps / el_s, "ms_per_step": el_s / args.steps * 1e3,
+                                    "workload": "the same hole with H_a sampled at creation (20 more fields "
+                                                "read per point and stage)"}
+        L.tgh_problem_destroy(ps)
+    if not args.no_adjoint:
+        note("the adjoint's cost")
+        torch.cuda.empty_cache()
+        result["adjoint"] = adjoint_bench(tgh, torch)
     if not args.no_forest:
+        note("the octree")
         torch.cuda.empty_cache()
         result["forest"] = forest_bench(tgh, torch, max(1, min(5, args.steps)))
     if not args.no_batch:
+        note("the four-hole batch")
         torch.cuda.empty_cache()
         result["batch"] = batch_bench(tgh, torch, max(1, min(5, args.steps)))
         if not args.no_forest:
+            note("the octree with the four holes")
             torch.cuda.empty_cache()
             result["octree_batch"] = octree_batch_bench(tgh, torch, max(1, min(3, args.steps)))
     if not args.no_cpu:
+        note("the CPU baseline")
         result["cpu_baseline"] = cpu_baseline()
     if args.calibrate_copy:
         x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")

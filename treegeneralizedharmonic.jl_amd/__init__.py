@@ -31,7 +31,11 @@ __all__ = [
     "rhs", "settle", "stage_limit", "step_limit", "max_speed", "initial_state",
     "evolved_sets", "gh_dt", "gh_solve", "add_noise", "error_norms", "library_path", "GHBatch",
     "gh_solve_batch", "Forest", "hole_forest", "Operators", "transfer", "regrid_flags", "regrid_forest", "regrid", "gh_solve_regrid",
-    "field_scale", "refinement_indicator",
+    "field_scale", "refinement_indicator", "Refinement", "level_bounds", "refine_flags", "apply_bounds",
+    "gauge_constraint_norms", "gauge_constraint_norms_noalloc", "adm_constraint_norms", "autotune",
+    "set_matter", "set_rho_max", "vjp_rhs", "adjoint", "ADJOINT_PARAMS", "evolve", "chunk_count",
+    "chunk_bounds", "FittedInterior", "StateBounds", "HorizonDissipation", "horizon_dissipation",
+    "horizon_min_radius", "horizon_max_radius", "default_bounds", "default_gate", "with_interior",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
