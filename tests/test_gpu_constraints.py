@@ -140,3 +140,35 @@ def test_refinement_indicator_matches_oracle(tgh):
         print(f"{forest.nleaves()} leaves: tau_max GPU {tg.max():.6e} oracle {to.max():.6e}, "
               f"max |diff| {np.abs(tg - to).max():.3e}")
         np.testing.assert_allclose(tg, to, rtol=1e-13, atol=0)
+
+
+def test_record_norms_is_the_three_monitors_against_the_oracle(tgh):
+    """record_norms (src/constraints.jl:868-881) on the a = 0.5 hole with noise:
+    its gauge, ham and mom rows are the oracle's monitors' and its error rows
+    the oracle's error rows (the drift over the horizon shell), each to the
+    monitors' own tolerance; without the ADM pass its ham and mom fields are
+    empty, as the driver's rows between ADM chunks are."""
+    torch = torch_cuda()
+    case = tgh.kerr_schild_case(M=1.0, a=0.5, halfwidth=2.0, r_0=0.71, r_1=1.21, gauge_source="algebraic")
+    dev = tgh.GHProblem(tgh.gh_forest(None, case, N=16, roots=4), case, q=4)
+    orc = oracle_lib.OracleProblem(TD.TghDesc.from_buffer_copy(dev.desc))
+    u = orc.fill_exact(0.0)
+    orc.add_noise(u, 0xC0A9, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    shell = (1.21, tgh.horizon_max_radius(case.background))
+    rec = tgh.record_norms(dev, ud, 0.05, shell=shell)
+    l2o, lio = orc.gauge_constraint(u, 0.05)
+    np.testing.assert_allclose(rec["gauge_l2"], l2o, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(rec["gauge_linf"], lio, rtol=1e-9, atol=1e-15)
+    ham, ml2, mli = orc.adm_constraint(u, 0.05)
+    np.testing.assert_allclose([rec["ham_l2"], rec["ham_linf"]], ham, rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(rec["mom_l2"], ml2, rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(rec["mom_linf"], mli, rtol=1e-8, atol=1e-14)
+    rows = orc.error_rows(u, 0.05, shell)
+    print(f"record: {rec}; oracle error rows {rows}")
+    np.testing.assert_allclose([rec["err_l2"], rec["err_linf"], rec["residual"], rec["drift"]], rows,
+                               rtol=1e-9, atol=1e-15)
+    assert rec["drift"] > 0 and rec["residual"] > 0
+    lean = tgh.record_norms(dev, ud, 0.05, adm=False)
+    assert lean["ham_l2"] is None and lean["mom_linf"] is None and lean["drift"] == 0.0
+    assert lean["gauge_l2"] == rec["gauge_l2"] and lean["err_l2"] == rec["err_l2"]

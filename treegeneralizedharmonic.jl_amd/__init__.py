@@ -36,6 +36,7 @@ __all__ = [
     "set_matter", "set_rho_max", "vjp_rhs", "adjoint", "ADJOINT_PARAMS", "evolve", "chunk_count",
     "chunk_bounds", "FittedInterior", "StateBounds", "HorizonDissipation", "horizon_dissipation",
     "horizon_min_radius", "horizon_max_radius", "default_bounds", "default_gate", "with_interior",
+    "record_norms",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -1170,6 +1171,27 @@ def adm_constraint_norms(p, u, t):
     l2, li = (C.c_double * 4)(), (C.c_double * 4)()
     _check(_load().tgh_adm_constraint(p.ptr, _dev(u, p, "u"), float(t), l2, li))
     return list(l2), list(li)
+
+
+def record_norms(p, u, t, *, adm=True, shell=None):
+    """record_norms(p) (src/constraints.jl:868-881): constraint_norms' rows and
+    error_norms' rows together, what the driver's analysis row reads. The
+    reference folds the three monitors' slots of its diagnostics field set in
+    one sweep after running them; here each monitor folds its own partials,
+    so the call runs them on (u, t): the gauge constraint always, the ADM
+    constraints when `adm` (the driver runs them on their own cadence; without
+    them the ham and mom fields are None), and the error rows with the drift
+    over `shell` (lo, hi) when given. Returns a dict: gauge_l2, gauge_linf
+    (4 each), ham_l2, ham_linf, mom_l2, mom_linf (3 each), err_l2, err_linf,
+    residual, drift."""
+    g2, gi = gauge_constraint_norms(p, u, t)
+    rec = {"gauge_l2": g2, "gauge_linf": gi, "ham_l2": None, "ham_linf": None, "mom_l2": None,
+           "mom_linf": None}
+    if adm:
+        a2, ai = adm_constraint_norms(p, u, t)
+        rec.update(ham_l2=a2[0], ham_linf=ai[0], mom_l2=a2[1:], mom_linf=ai[1:])
+    rec.update(error_norms(p, u, t, shell=shell if shell is not None else (0.0, -1.0)))
+    return rec
 
 
 def field_scale(p, u, t):

@@ -42,80 +42,134 @@ TGH_HD void coefbar_zero(CoefBar& c) {
   for (int n = 0; n < 6; ++n) c.gamu[n] = 0.0;
 }
 
-// h-bar += (d Coef / d h)^T cbar: metric_quantities' spelling run on
-// duals seeded with the ten components of h (forward mode over the ten
-// inputs; the Jacobian is 21 x 10), then contracted with the cotangent.
+// h-bar += (d Coef / d h)^T cbar: metric_quantities (tgh_device.hpp) run
+// forward for its intermediate values, then reversed statement by statement
+// (gu, beta and gamma^{ij} from gi, b and 1/alpha^2; alpha^2 from h_tt, b and
+// the shift; b from gi; gi = cof / det; det(I + H), whose derivative in h_ii
+// is the cofactor cof_ii and in h_ij (i != j) twice cof_ij, since h_ij and
+// h_ji are one packed entry).
 TGH_HD void metric_quantities_vjp(const double h[NC], const CoefBar& cb, double hbar[NC]) {
-  typedef Dual<NC> D;
-  D x[NC];
+  const double hxx = h[4], hxy = h[5], hxz = h[6], hyy = h[7], hyz = h[8], hzz = h[9];
+  const double gxx = 1.0 + hxx, gyy = 1.0 + hyy, gzz = 1.0 + hzz;
+  const double e1 = hxx + hyy + hzz;
+  const double e2 = (hxx * hyy - hxy * hxy) + (hxx * hzz - hxz * hxz) + (hyy * hzz - hyz * hyz);
+  const double e3 = hxx * (hyy * hzz - hyz * hyz) - hxy * (hxy * hzz - hyz * hxz) +
+                    hxz * (hxy * hyz - hyy * hxz);
+  const double detg = 1.0 + (e1 + (e2 + e3));
+  const double sqrtg = sqrt(detg);
+  const double idet = 1.0 / detg;
+  const double cof[6] = {gyy * gzz - hyz * hyz, hxz * hyz - hxy * gzz, hxy * hyz - hxz * gyy,
+                         gxx * gzz - hxz * hxz, hxy * hxz - gxx * hyz, gxx * gyy - hxy * hxy};
+  double gi[6];
 #pragma unroll
-  for (int v = 0; v < NC; ++v) {
-    x[v] = dconst<NC>(h[v]);
-    x[v].d[v] = 1.0;
+  for (int n = 0; n < 6; ++n) gi[n] = cof[n] * idet;
+  const double bl[3] = {h[1], h[2], h[3]};
+  // gi's packed index of the spatial pair (i, j), and the pair of packed n
+  const int gidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  const int pi_[6] = {0, 0, 0, 1, 1, 2}, pj_[6] = {0, 1, 2, 1, 2, 2};
+  double b[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) b[i] = gi[gidx[i][0]] * bl[0] + gi[gidx[i][1]] * bl[1] + gi[gidx[i][2]] * bl[2];
+  const double a2 = (1.0 - h[0]) + (b[0] * bl[0] + b[1] * bl[1] + b[2] * bl[2]);
+  const double alpha = sqrt(a2);
+  const double ia2 = 1.0 / a2;
+  // gu[0] = -1/a2, gu[1+i] = b_i / a2, gu[4+n] = gi[n] - b_i b_j / a2; beta = b; gamu = gi
+  double gib[6], bb[3];
+  double ia2b = -cb.gu[0];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    bb[i] = cb.beta[i] + cb.gu[1 + i] * ia2;
+    ia2b += cb.gu[1 + i] * b[i];
   }
-  const D hxx = x[4], hxy = x[5], hxz = x[6], hyy = x[7], hyz = x[8], hzz = x[9];
-  const D e1 = hxx + hyy + hzz;
-  const D e2 = (hxx * hyy - hxy * hxy) + (hxx * hzz - hxz * hxz) + (hyy * hzz - hyz * hyz);
-  const D e3 = hxx * (hyy * hzz - hyz * hyz) - hxy * (hxy * hzz - hyz * hxz) + hxz * (hxy * hyz - hyy * hxz);
-  const D detg = 1.0 + (e1 + (e2 + e3));
-  const D sqrtg = dsqrt(detg);
-  const D gxx = 1.0 + hxx, gyy = 1.0 + hyy, gzz = 1.0 + hzz;
-  const D one = dconst<NC>(1.0);
-  const D idet = one / detg;
-  D gi[6];
-  gi[0] = (gyy * gzz - hyz * hyz) * idet;
-  gi[1] = (hxz * hyz - hxy * gzz) * idet;
-  gi[2] = (hxy * hyz - hxz * gyy) * idet;
-  gi[3] = (gxx * gzz - hxz * hxz) * idet;
-  gi[4] = (hxy * hxz - gxx * hyz) * idet;
-  gi[5] = (gxx * gyy - hxy * hxy) * idet;
-  const D bl0 = x[1], bl1 = x[2], bl2 = x[3];
-  const D b0 = gi[0] * bl0 + gi[1] * bl1 + gi[2] * bl2;
-  const D b1 = gi[1] * bl0 + gi[3] * bl1 + gi[4] * bl2;
-  const D b2 = gi[2] * bl0 + gi[4] * bl1 + gi[5] * bl2;
-  const D a2 = (one - x[0]) + (b0 * bl0 + b1 * bl1 + b2 * bl2);
-  const D alpha = dsqrt(a2);
-  const D ia2 = one / a2;
-  D gu[NC];
-  gu[0] = -ia2;
-  gu[1] = b0 * ia2;
-  gu[2] = b1 * ia2;
-  gu[3] = b2 * ia2;
-  gu[4] = gi[0] - b0 * b0 * ia2;
-  gu[5] = gi[1] - b1 * b0 * ia2;
-  gu[6] = gi[2] - b2 * b0 * ia2;
-  gu[7] = gi[3] - b1 * b1 * ia2;
-  gu[8] = gi[4] - b2 * b1 * ia2;
-  gu[9] = gi[5] - b2 * b2 * ia2;
-  const D be[3] = {b0, b1, b2};
 #pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    double s = cb.alpha * alpha.d[k] + cb.sqrtg * sqrtg.d[k];
-#pragma unroll
-    for (int n = 0; n < NC; ++n) s += cb.gu[n] * gu[n].d[k];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) s += cb.beta[i] * be[i].d[k];
-#pragma unroll
-    for (int n = 0; n < 6; ++n) s += cb.gamu[n] * gi[n].d[k];
-    hbar[k] += s;
+  for (int n = 0; n < 6; ++n) {
+    const int i = pi_[n], j = pj_[n];
+    const double g = cb.gu[4 + n];
+    gib[n] = cb.gamu[n] + g;
+    ia2b -= g * b[i] * b[j];
+    bb[i] -= g * b[j] * ia2;
+    bb[j] -= g * b[i] * ia2;
   }
+  // 1/a2 and alpha = sqrt(a2); a2 = (1 - h_tt) + b . bl
+  const double a2b = cb.alpha * 0.5 / alpha - ia2b * ia2 * ia2;
+  hbar[0] -= a2b;
+  double blb[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    bb[i] += a2b * bl[i];
+    blb[i] = a2b * b[i];
+  }
+  // b_i = sum_j gi(ij) bl_j
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      gib[gidx[i][j]] += bb[i] * bl[j];
+      blb[j] += bb[i] * gi[gidx[i][j]];
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) hbar[1 + i] += blb[i];
+  // gi = cof / det; sqrt(det)
+  double cofb[6], idetb = 0.0;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    cofb[n] = gib[n] * idet;
+    idetb += gib[n] * cof[n];
+  }
+  const double detgb = cb.sqrtg * 0.5 / sqrtg - idetb * idet * idet;
+  double hxxb = detgb * cof[0], hxyb = 2.0 * detgb * cof[1], hxzb = 2.0 * detgb * cof[2];
+  double hyyb = detgb * cof[3], hyzb = 2.0 * detgb * cof[4], hzzb = detgb * cof[5];
+  // the cofactors
+  hyyb += cofb[0] * gzz;
+  hzzb += cofb[0] * gyy;
+  hyzb -= 2.0 * cofb[0] * hyz;
+  hxzb += cofb[1] * hyz;
+  hyzb += cofb[1] * hxz;
+  hxyb -= cofb[1] * gzz;
+  hzzb -= cofb[1] * hxy;
+  hxyb += cofb[2] * hyz;
+  hyzb += cofb[2] * hxy;
+  hxzb -= cofb[2] * gyy;
+  hyyb -= cofb[2] * hxz;
+  hxxb += cofb[3] * gzz;
+  hzzb += cofb[3] * gxx;
+  hxzb -= 2.0 * cofb[3] * hxz;
+  hxyb += cofb[4] * hxz;
+  hxzb += cofb[4] * hxy;
+  hxxb -= cofb[4] * hyz;
+  hyzb -= cofb[4] * gxx;
+  hxxb += cofb[5] * gyy;
+  hyyb += cofb[5] * gxx;
+  hxyb -= 2.0 * cofb[5] * hxy;
+  hbar[4] += hxxb;
+  hbar[5] += hxyb;
+  hbar[6] += hxzb;
+  hbar[7] += hyyb;
+  hbar[8] += hyzb;
+  hbar[9] += hzzb;
 }
 
 // The reverse of gh_source<HAS_H> (the lean source, tgh_device.hpp): given
 // ob = the cotangent of out, accumulate the cotangents of the coefficient
-// set (cb), of h (hb: through eta + h in the damping term), of D (Db),
-// of the gauge source (Hb, dHb) and of g0 and g2. The forward's intermediate
-// values are recomputed here first.
+// set (cb), of h (hb: through eta + h in the damping term), of D (Db), of
+// the lowered gauge source (Hb) and of g0 and g2, and return Sb, the
+// cotangent of the sum S (out = -alpha sqrt(gamma) S), from which that of
+// d_c H_a follows: dH_ab enters S_ab and S_ba with a minus sign, so its
+// cotangent is -Sb_ab off the diagonal and -2 Sb_aa on it. The forward's
+// intermediate values are recomputed here first; S itself is never held:
+// only its contraction with ob (for the cotangent of alpha sqrt(gamma)) and
+// with Sb the damping term's (for g0's) are accumulated as it is formed.
 template <bool HAS_H>
 TGH_HD void gh_source_vjp(const Coef& c, const double h[NC], const double D[4][NC], const double Hl[4],
                           const double dH[4][4], double g0, double g2, const double ob[NC], CoefBar& cb,
-                          double hb[NC], double Db[4][NC], double Hb[4], double dHb[4][4], double& g0b,
+                          double hb[NC], double Db[4][NC], double Hb[4], double Sb[NC], double& g0b,
                           double& g2b) {
   const double* Gp = c.gu;
-  // ---- forward: S, the Christoffels, Gup, and the damping's Cup, Cl, tC ----
-  double S[NC];
+  const double as = c.alpha * c.sqrtg;
 #pragma unroll
-  for (int n = 0; n < NC; ++n) S[n] = 0.0;
+  for (int n = 0; n < NC; ++n) Sb[n] = -as * ob[n];
+  double sob = 0.0;  // sum_n ob[n] S[n]
+  // ---- forward: S's contraction, the Christoffels, Gup, and the damping's Cup, Cl, tC ----
 #pragma unroll 1
   for (int a = 0; a < 4; ++a) {
     double E[4][4];
@@ -135,10 +189,8 @@ TGH_HD void gh_source_vjp(const Coef& c, const double h[NC], const double D[4][N
     for (int b = 0; b < 4; ++b) {
       double s = 0.0;
       for (int m = 0; m < 16; ++m) s += Cuu[pidx(m % 4, m / 4)] * D[m % 4][pidx(m / 4, b)];
-      for (int n = 0; n < NC; ++n) {
-        if (prow(n) == a && pcol(n) == b) S[n] += s;
-        if (pcol(n) == a && prow(n) == b) S[n] += s;
-      }
+      // C2 + C2^T: S_ab and S_ba each take s, the diagonal twice
+      sob += s * ob[pidx(a, b)] * (a == b ? 2.0 : 1.0);
     }
   }
   double Gam[4][NC];
@@ -168,9 +220,9 @@ TGH_HD void gh_source_vjp(const Coef& c, const double h[NC], const double D[4][N
       const double t4 = Gam[0][n] * Hl[0] + Gam[1][n] * Hl[1] + Gam[2][n] * Hl[2] + Gam[3][n] * Hl[3];
       s += 2.0 * t4 - (dH[a][b] + dH[b][a]);
     }
-    S[n] += s;
+    sob += ob[n] * s;
   }
-  double Cup[4], Cl[4], tC = 0.0, dmp[NC];
+  double Cup[4], Cl[4], tC = 0.0;
   const double tl = -c.alpha, k1 = 1.0 + g2;
   for (int cc = 0; cc < 4; ++cc) {
     double gh = 0.0;
@@ -188,17 +240,13 @@ TGH_HD void gh_source_vjp(const Coef& c, const double h[NC], const double D[4][N
     double d = 0.0;
     if (a == 0) d += tl * Cl[b];
     if (b == 0) d += tl * Cl[a];
-    dmp[n] = d - k1 * (eta(a, b) + h[n]) * tC;
-    S[n] += g0 * dmp[n];
+    const double dmp = d - k1 * (eta(a, b) + h[n]) * tC;
+    sob += ob[n] * (g0 * dmp);
+    g0b += Sb[n] * dmp;
   }
   // ---- reverse ----
   // (4) out = -alpha sqrt(gamma) S
-  const double as = c.alpha * c.sqrtg;
-  double Sb[NC], asb = 0.0;
-  for (int n = 0; n < NC; ++n) {
-    Sb[n] = -as * ob[n];
-    asb -= ob[n] * S[n];
-  }
+  const double asb = -sob;
   cb.alpha += asb * c.sqrtg;
   cb.sqrtg += asb * c.alpha;
   // (3) the damping term, g0 dmp
@@ -207,7 +255,6 @@ TGH_HD void gh_source_vjp(const Coef& c, const double h[NC], const double D[4][N
     double Clb[4] = {0.0, 0.0, 0.0, 0.0}, tlb = 0.0, tCb = 0.0;
     for (int n = 0; n < NC; ++n) {
       const int a = prow(n), b = pcol(n);
-      g0b += Sb[n] * dmp[n];
       g2b += Sb[n] * g0 * (-(eta(a, b) + h[n]) * tC);
       const double sg = g0 * Sb[n];
       if (a == 0) {
@@ -263,8 +310,6 @@ TGH_HD void gh_source_vjp(const Coef& c, const double h[NC], const double D[4][N
         Gamb[cc][n] += t4b * Hl[cc];
         Hb[cc] += t4b * Gam[cc][n];
       }
-      dHb[a][b] -= sb;
-      dHb[b][a] -= sb;
     }
     for (int m = 0; m < 16; ++m) {
       const int p1 = pidx(m / 4, a), p2 = pidx(m % 4, b);
@@ -458,7 +503,7 @@ TGH_HD void metric_divergences_vjp(const Coef& c, const double dh[3][NC], double
 // hole at rest, spin along z): the cotangents of h (its tt, tx, ty, tz
 // components) and of D[c] (the same components) from those of H_a and d_c H_a.
 TGH_HD void ks_source_vjp(double M, double a, const double h[NC], const double D[4][NC], const double Hb[4],
-                          const double dHb[4][4], double hb[NC], double Db[4][NC]) {
+                          const double Sb[NC], double hb[NC], double Db[4][NC]) {
   const double w[4] = {h[0], h[1], h[2], h[3]};
   const double K = w[0];
   const double sw = a * w[3];
@@ -479,7 +524,7 @@ TGH_HD void ks_source_vjp(double M, double a, const double h[NC], const double D
     const double dD = root > 0.0 ? -sw * (a * D[cc][3]) / root : 0.0;
     double dKb = 0.0, dDb = 0.0;
     for (int q = 0; q < 4; ++q) {
-      const double g = dHb[cc][q];
+      const double g = -(cc == q ? 2.0 : 1.0) * Sb[pidx(cc, q)];  // the cotangent of d_cc H_q
       const double dwq = D[cc][q];
       dKb -= g * w[q] * iD;
       wb[q] += -g * dK * iD + g * K * dD * iD * iD;
@@ -680,11 +725,9 @@ __device__ __forceinline__ void adj_point(const AdjPointArgs& a, int x, int y, i
   CoefBar cb;
   coefbar_zero(cb);
   // the source: msrc = gh_source(cf, h, D, H, dH, g0, g2)
-  double Hb[4] = {0.0, 0.0, 0.0, 0.0}, dHb[4][4], g0b = 0.0, g2b = 0.0;
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) dHb[i][j] = 0.0;
-  gh_source_vjp<gauge_has_source(GAUGE)>(cf, hv, D, Hl, dH, g0, c.g2, msb, cb, aH, Db, Hb, dHb, g0b, g2b);
-  if (GAUGE == GAUGE_KS) ks_source_vjp(c.bg.M, c.bg.a, hv, D, Hb, dHb, aH, Db);
+  double Hb[4] = {0.0, 0.0, 0.0, 0.0}, Sb[NC], g0b = 0.0, g2b = 0.0;
+  gh_source_vjp<gauge_has_source(GAUGE)>(cf, hv, D, Hl, dH, g0, c.g2, msb, cb, aH, Db, Hb, Sb, g0b, g2b);
+  if (GAUGE == GAUGE_KS) ks_source_vjp(c.bg.M, c.bg.a, hv, D, Hb, Sb, aH, Db);
   // d_t h = D[0] = beta . d h + (alpha / sqrt g) Pi + KO(h); the output
   // for h is w D[0] - rho (h - h_e)
 #pragma unroll 1

@@ -100,9 +100,13 @@ template <bool HAS_H>
 TGH_HD void gh_source(const Coef& c, const double h[NC], const double D[4][NC], const double Hl[4],
                       const double dH[4][4], double g0, double g2, double out[NC]) {
   const double* Gp = c.gu;
+  // The terms are summed in an order that lets the inputs die early: the
+  // gauge source's derivative first, then C2 (D and G), then g^{ab} Gamma^c_ab
+  // and its contraction with D (from D directly), then the forty
+  // Christoffels (D's last use) and their products.
   double S[NC];
 #pragma unroll
-  for (int n = 0; n < NC; ++n) S[n] = 0.0;
+  for (int n = 0; n < NC; ++n) S[n] = HAS_H ? -(dH[prow(n)][pcol(n)] + dH[pcol(n)][prow(n)]) : 0.0;
   // (1) C2 + C2^T, one first index a at a time: E = G D_a, Cuu = E G,
   //     C2[a][b] = sum_{mu,nu} Cuu[mu nu] D[mu](nu, b).
 #pragma unroll
@@ -141,7 +145,27 @@ TGH_HD void gh_source(const Coef& c, const double h[NC], const double D[4][NC], 
       if (j == a) S[n] += C2[i];
     }
   }
-  // (2) the forty Christoffels Gamma^a_bc = G^{ax} (D_b[x,c] + D_c[x,b] - D_x[b,c]) / 2.
+  // (2) Gup^c = g^{ab} Gamma^c_ab = G^{cx} L_x with L_x = g^{ab} Gamma_{x ab}, the
+  //     lowered Christoffels Gamma_{x bc} = (D_b[x,c] + D_c[x,b] - D_x[b,c]) / 2;
+  //     then -Gup^c D_c[n]
+  double Gup[4];
+  {
+    double L[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+      const int i = prow(n), j = pcol(n);
+      const double gn = (i == j ? 1.0 : 2.0) * Gp[n];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) L[x] += gn * (0.5 * (D[i][pidx(x, j)] + D[j][pidx(x, i)] - D[x][n]));
+    }
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+      Gup[cc] = Gp[pidx(cc, 0)] * L[0] + Gp[pidx(cc, 1)] * L[1] + Gp[pidx(cc, 2)] * L[2] + Gp[pidx(cc, 3)] * L[3];
+  }
+#pragma unroll
+  for (int n = 0; n < NC; ++n) S[n] -= Gup[0] * D[0][n] + Gup[1] * D[1][n] + Gup[2] * D[2][n] + Gup[3] * D[3][n];
+  //     the forty Christoffels Gamma^a_bc = G^{ax} Gamma_{x bc}, and
+  //     -2 Gamma^c_{d a} Gamma^d_{c b} + 2 Gamma^c_ab H_c
   double Gam[4][NC];
 #pragma unroll
   for (int n = 0; n < NC; ++n) {
@@ -157,26 +181,14 @@ TGH_HD void gh_source(const Coef& c, const double h[NC], const double D[4][NC], 
       Gam[a][n] = s;
     }
   }
-  double Gup[4];
-#pragma unroll
-  for (int cc = 0; cc < 4; ++cc) {
-    double s = 0.0;
-#pragma unroll
-    for (int n = 0; n < NC; ++n) s += (prow(n) == pcol(n) ? 1.0 : 2.0) * Gp[n] * Gam[cc][n];
-    Gup[cc] = s;
-  }
 #pragma unroll
   for (int n = 0; n < NC; ++n) {
     const int a = prow(n), b = pcol(n);
     double t2 = 0.0;
 #pragma unroll
     for (int m = 0; m < 16; ++m) t2 += Gam[m % 4][pidx(m / 4, a)] * Gam[m / 4][pidx(m % 4, b)];
-    const double t5 = Gup[0] * D[0][n] + Gup[1] * D[1][n] + Gup[2] * D[2][n] + Gup[3] * D[3][n];
-    double s = -2.0 * t2 - t5;
-    if (HAS_H) {
-      const double t4 = Gam[0][n] * Hl[0] + Gam[1][n] * Hl[1] + Gam[2][n] * Hl[2] + Gam[3][n] * Hl[3];
-      s += 2.0 * t4 - (dH[a][b] + dH[b][a]);
-    }
+    double s = -2.0 * t2;
+    if (HAS_H) s += 2.0 * (Gam[0][n] * Hl[0] + Gam[1][n] * Hl[1] + Gam[2][n] * Hl[2] + Gam[3][n] * Hl[3]);
     S[n] += s;
   }
   // (3) Gundlach–Pretorius damping, t_a = -alpha delta_a^t.

@@ -717,6 +717,36 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
                                             PiCoef& pc, TGH_OUT_PARAMS) {
   const long sv = pr.sv, psz = pr.ssv, si0 = pr.si0, wi0 = pr.wi0;
   const CaseP& c = a.c;
+  double wgt = 1.0, rho = 0.0;
+  const bool layer = HOLE && r < c.r1;
+  if (layer) {
+    const double sl = (r - c.r0) / (c.r1 - c.r0);
+    wgt = smoothstep(sl / c.w_ramp);
+    rho = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
+  }
+  // The layer's target u_e (r >= r0 here: the core rule is the identity) and,
+  // for a moving :damped hole, its rates, which (1 - w) d_t u_e adds so that
+  // the moving solution is a steady state of the layer (src/evolution.jl:652-680).
+  // It is evaluated twice in the layer, for d_t h before the source and for
+  // d_t Pi after it, so that no part of it is held through the source.
+  constexpr bool MOVE = (bg_set_for(GAUGE) & BGS_MOVING) != 0;
+  const bool follows = MOVE && c.moving && c.interior == INT_DAMPED;
+  const double om = 1.0 - wgt;
+  {
+    double ue[20], re[20];
+    if (layer) {
+      if (follows)
+        background_state_rate(c.bg, a.t, pr.x, c.delta, ue, ue + NC, re, re + NC);
+      else
+        case_state<bg_set_for(GAUGE)>(c, a.t, pr.x, ue);
+    }
+#pragma unroll
+    for (int v = 0; v < NC; ++v) {
+      double f = D[0][v];
+      if (layer) f = follows ? wgt * f + om * re[v] - rho * (hv[v] - ue[v]) : wgt * f - rho * (hv[v] - ue[v]);
+      emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, f);
+    }
+  }
   const double g0 = damping_rate(c, a.t, pr.x);
   double msrc[NC];
   {
@@ -745,18 +775,7 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
 #pragma unroll
     for (int n = 0; n < NC; ++n) msrc[n] += ms[n];
   }
-  double wgt = 1.0, rho = 0.0;
-  const bool layer = HOLE && r < c.r1;
-  if (layer) {
-    const double sl = (r - c.r0) / (c.r1 - c.r0);
-    wgt = smoothstep(sl / c.w_ramp);
-    rho = c.interior == INT_FROZEN ? 0.0 : c.rho_max * smoothstep((1.0 - sl) / c.rho_ramp);
-  }
-  // The layer's target u_e (r >= r0 here: the core rule is the identity) and,
-  // for a moving :damped hole, its rates, which (1 - w) d_t u_e adds so that
-  // the moving solution is a steady state of the layer (src/evolution.jl:652-680).
-  constexpr bool MOVE = (bg_set_for(GAUGE) & BGS_MOVING) != 0;
-  const bool follows = MOVE && c.moving && c.interior == INT_DAMPED;
+  pc.w = layer ? wgt : 1.0;
   double ue[20], re[20];
   if (layer) {
     if (follows)
@@ -764,14 +783,6 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
     else
       case_state<bg_set_for(GAUGE)>(c, a.t, pr.x, ue);
   }
-  const double om = 1.0 - wgt;
-#pragma unroll
-  for (int v = 0; v < NC; ++v) {
-    double f = D[0][v];
-    if (layer) f = follows ? wgt * f + om * re[v] - rho * (hv[v] - ue[v]) : wgt * f - rho * (hv[v] - ue[v]);
-    emit<EPI>(a, TGH_OUT_ARGS, si0 + v * psz, wi0 + v * sv, f);
-  }
-  pc.w = layer ? wgt : 1.0;
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
     double c1 = msrc[v];
