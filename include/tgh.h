@@ -135,6 +135,16 @@ typedef struct tgh_desc {
   double shape[(TGH_SHAPE_MAX_LMAX + 1) * (TGH_SHAPE_MAX_LMAX + 1)];
   double r_in, r_out, offset, thickness;
   int32_t ownership;    /* TGH_OWNERSHIP_*; 0 (closed blocks) when zero-initialised */
+  /* Reflecting faces (GHCase's reflecting, src/initialdata.jl:30-60; the
+   * mirrors of src/boundaries.jl:84-142): reflecting[d][0] / [d][1] nonzero
+   * makes the low / high face of axis d a symmetry plane. Its ghosts are the
+   * mirror image of the interior, each variable times its parity (component
+   * ab of h and Pi is odd across x^d = 0 when the index d occurs an odd
+   * number of times in ab), the wall plane itself is owned and evolved, and
+   * at every fill its odd variables are set to zero; the Dirichlet hook
+   * fills only the faces that neither reflect nor wrap. Not with periodic on
+   * the same axis; uniform forests only. Zero-initialised: no mirror. */
+  int32_t reflecting[3][2];
 } tgh_desc;
 
 /* Matter's 3+1 projections at every owned point (src/evolution.jl:1295-1311,

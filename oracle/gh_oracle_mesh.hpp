@@ -29,12 +29,28 @@ inline uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
   return k;
 }
 
+// The parity of variable v (h_ab then Pi_ab, packed tt, tx, ty, tz, xx, xy,
+// xz, yy, yz, zz) under the mirror x^d -> -x^d (state_parity,
+// src/boundaries.jl:84-120): bit d is set when the spatial index d occurs an
+// odd number of times in ab, so the component changes sign in that mirror.
+inline int parity_odd_mask(int v) {
+  static const int odd[10] = {0, 1, 2, 4, 0, 3, 5, 0, 6, 0};
+  return odd[v % 10];
+}
+// The sign of variable v under the mirrors of the axes in mask m.
+inline double parity_sign(int v, int m) {
+  const int o = parity_odd_mask(v) & m;
+  return ((o & 1) ^ ((o >> 1) & 1) ^ ((o >> 2) & 1)) ? -1.0 : 1.0;
+}
+
 struct Mesh {
   int N = 16, G = 3, q = 4;
   int nb[3] = {1, 1, 1};
   double lo[3] = {0, 0, 0};
   double h = 1.0 / 16;
   bool periodic[3] = {true, true, true};
+  // reflecting faces: reflect[d][0] the low face of axis d, [d][1] the high one
+  bool reflect[3][2] = {{false, false}, {false, false}, {false, false}};
   std::vector<std::array<int, 3>> blocks;  // block coordinates, Morton order
   long S = 0, P = 0;                       // work and state extents per axis
 
@@ -68,9 +84,13 @@ struct Mesh {
     for (int d = 0; d < 3; ++d) x[d] = lo[d] + (double)g[d] * hh;
   }
   // Source of a work point: -2 Dirichlet, -1 own primary, else owner block and
-  // local index.
-  int classify(long b, const long l[3], long& owner, long loc[3]) const {
+  // local index, with *mirror the axes across whose reflecting face the point
+  // lies (its value is then the source's times the variables' parities). A
+  // point beyond a face that neither wraps nor reflects is the hook's, also
+  // where it lies beyond a reflecting face on another axis.
+  int classify(long b, const long l[3], long& owner, long loc[3], int* mirror = nullptr) const {
     long ob[3];
+    int mm = 0;
     for (int d = 0; d < 3; ++d) {
       long g = blocks[b][d] * (long)N + l[d];
       long tot = (long)nb[d] * N;
@@ -79,15 +99,38 @@ struct Mesh {
         ob[d] = gw / N;
         loc[d] = gw - ob[d] * N;
       } else {
-        if (g < 0 || g > tot) return -2;
+        if (g < 0) {
+          if (!reflect[d][0]) return -2;
+          g = -g;
+          mm |= 1 << d;
+        } else if (g > tot) {
+          if (!reflect[d][1]) return -2;
+          g = 2 * tot - g;
+          mm |= 1 << d;
+        }
         ob[d] = std::min(g / N, (long)nb[d] - 1);
         loc[d] = g - ob[d] * N;
       }
     }
+    if (mirror) *mirror = mm;
     // owner block index (linear search over the Morton order is fine here)
     owner = block_index(ob[0], ob[1], ob[2]);
-    if (owner == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return -1;
+    if (mm == 0 && owner == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return -1;
     return 0;
+  }
+  // The axes whose reflecting wall plane the owned point l of block b lies on.
+  int wall_mask(long b, const long l[3]) const {
+    int m = 0;
+    for (int d = 0; d < 3; ++d) {
+      const long g = blocks[b][d] * (long)N + l[d], tot = (long)nb[d] * N;
+      if ((reflect[d][0] && g == 0) || (reflect[d][1] && g == tot)) m |= 1 << d;
+    }
+    return m;
+  }
+  bool reflects() const {
+    for (int d = 0; d < 3; ++d)
+      if (reflect[d][0] || reflect[d][1]) return true;
+    return false;
   }
   std::vector<long> lookup;
   long block_index(long i, long j, long k) const {
@@ -358,6 +401,13 @@ struct Problem {
               double x[3];
               m.position(b, i, j, k, x);
               u[m.sidx(b, v, i, j, k)] += amp * (2.0 * point_draw(base, x, v + 1) - 1.0);
+              // on a low reflecting wall the noise is projected onto the
+              // parity: the odd variables there are zero (add_noise!,
+              // src/initialdata.jl:1938-1990, which projects the low walls)
+              for (int d = 0; d < 3; ++d) {
+                const long gl = m.blocks[b][d] * (long)m.N + (d == 0 ? i : d == 1 ? j : k);
+                if (m.reflect[d][0] && gl == 0 && (parity_odd_mask(v) >> d & 1)) u[m.sidx(b, v, i, j, k)] = 0.0;
+              }
             }
   }
 
@@ -373,12 +423,27 @@ struct Problem {
       fill_forest(t);
       return;
     }
+    // a reflecting wall plane takes each variable's parity: its odd
+    // variables are zero (TreeAMR's projection of the wall at every fill of
+    // closed blocks, src/constraints.jl:222-225)
+    if (m.reflects())
+      for (long b = 0; b < m.nblocks(); ++b)
+        for (long k = 0; k <= m.N; ++k)
+          for (long j = 0; j <= m.N; ++j)
+            for (long i = 0; i <= m.N; ++i) {
+              const long l[3] = {i, j, k};
+              const int wm = m.wall_mask(b, l);
+              if (!wm) continue;
+              for (int v = 0; v < 20; ++v)
+                if (parity_odd_mask(v) & wm) work[m.widx(b, v, i, j, k)] = 0.0;
+            }
     for (long b = 0; b < m.nblocks(); ++b)
       for (long k = -m.G; k <= m.N + m.G; ++k)
         for (long j = -m.G; j <= m.N + m.G; ++j)
           for (long i = -m.G; i <= m.N + m.G; ++i) {
             long l[3] = {i, j, k}, owner, loc[3];
-            int c = m.classify(b, l, owner, loc);
+            int mm = 0;
+            int c = m.classify(b, l, owner, loc, &mm);
             if (c == -1) continue;
             if (c == -2) {
               long g[3] = {m.blocks[b][0] * (long)m.N + i, m.blocks[b][1] * (long)m.N + j,
@@ -389,7 +454,8 @@ struct Problem {
               for (int v = 0; v < 20; ++v) work[m.widx(b, v, i, j, k)] = s[v];
             } else {
               for (int v = 0; v < 20; ++v)
-                work[m.widx(b, v, i, j, k)] = work[m.widx(owner, v, loc[0], loc[1], loc[2])];
+                work[m.widx(b, v, i, j, k)] =
+                    parity_sign(v, mm) * work[m.widx(owner, v, loc[0], loc[1], loc[2])];
             }
           }
   }

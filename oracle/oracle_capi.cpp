@@ -294,6 +294,8 @@ static Problem* problem_from(const tgh_desc* d) {
     p->mesh.nb[i] = d->nblocks[i];
     p->mesh.lo[i] = d->lo[i];
     p->mesh.periodic[i] = d->periodic[i] != 0;
+    p->mesh.reflect[i][0] = d->reflecting[i][0] != 0;
+    p->mesh.reflect[i][1] = d->reflecting[i][1] != 0;
   }
   p->mesh.h = (d->hi[0] - d->lo[0]) / ((double)d->nblocks[0] * d->N);
   p->cs = case_from(d);
@@ -438,6 +440,8 @@ int orc_rk4_step_jvp(void* p, double* u, double* udot, double t, double dt, cons
 double orc_error_tangent(void* p, const double* u, const double* udot, double t) {
   return ((Problem*)p)->error_tangent(u, udot, t);
 }
+// The mirror parity of variable v: bit d set when it is odd across x^d = 0.
+int orc_parity_odd_mask(int v) { return parity_odd_mask(v); }
 // The matter fields every right-hand side adds (NULL rho: vacuum again).
 void orc_set_matter(void* pp, const double* rho, const double* S, const double* Sij) {
   Problem* p = (Problem*)pp;

@@ -1,0 +1,164 @@
+"""Reflecting faces in the oracle (the mirrors of src/boundaries.jl:84-142 and
+test/reflection_tests.jl): flat space with noise, because noise has every
+component nonzero, so a wrong parity in any of the twenty variables shows.
+
+Pinned by: the reference's own parity table (test/reflection_tests.jl:
+53-69); the octant [0, 4]^3 against the box [-4, 4]^3 with Dirichlet data
+everywhere and the octant's noise extended by parity, which is the one
+problem whose answer is known without a mirror (test/reflection_tests.jl:
+124-187: the octant is the box's symmetric part to roundoff); the odd
+variables staying zero on their walls; and add_noise's projection of the
+low walls (src/initialdata.jl:1938-1990). A plausible slip -- a parity
+taken from the wrong index, a mirror about the wrong plane (-g - 1 instead
+of -g), a missing sign -- moves the octant off the box by the data's own
+size."""
+import numpy as np
+import pytest
+
+import oracle_lib
+import tgh_desc as TD
+
+NTHREADS = 8
+E, O = 0, 1
+
+
+def test_parities_are_the_tensors():
+    """Component ab of h and Pi is odd across x^d = 0 when x^d occurs an odd
+    number of times in ab: the reference's table, entry for entry."""
+    L = oracle_lib.lib()
+    table = {0: (E, E, E), 1: (O, E, E), 2: (E, O, E), 3: (E, E, O), 4: (E, E, E),
+             5: (O, O, E), 6: (O, E, O), 7: (E, E, E), 8: (E, O, O), 9: (E, E, E)}
+    for v in range(20):
+        m = L.orc_parity_odd_mask(v)
+        assert tuple((m >> d) & 1 for d in range(3)) == table[v % 10], v
+
+
+def _minkowski(lo, hi, nb, reflecting):
+    return TD.make_desc(q=4, N=8, nblocks=(nb,) * 3, periodic=(0, 0, 0), lo=(lo,) * 3, hi=(hi,) * 3,
+                        background=TD.TGH_BG_MINKOWSKI, gamma0=1.0, gamma2=0.0, eps_ko=0.5,
+                        reflecting=reflecting)
+
+
+def _points(desc, coords):
+    """(b, k, j, i, 3) positions of the owned points."""
+    P, N = desc.N + 1, desc.N
+    h = (desc.hi[0] - desc.lo[0]) / (desc.nblocks[0] * N)
+    i = np.arange(P)
+    x = np.zeros((len(coords), P, P, P, 3))
+    for b, (bx, by, bz) in enumerate(coords):
+        x[b, ..., 0] = (desc.lo[0] + (bx * N + i) * h)[None, None, :]
+        x[b, ..., 1] = (desc.lo[1] + (by * N + i) * h)[None, :, None]
+        x[b, ..., 2] = (desc.lo[2] + (bz * N + i) * h)[:, None, None]
+    return x
+
+
+def _odd(L):
+    return np.array([[(L.orc_parity_odd_mask(v) >> d) & 1 for d in range(3)] for v in range(20)])
+
+
+def _octant_and_box():
+    oct_d = _minkowski(0.0, 4.0, 2, ((1, 0), (1, 0), (1, 0)))
+    box_d = _minkowski(-4.0, 4.0, 4, ((0, 0), (0, 0), (0, 0)))
+    return oct_d, oracle_lib.OracleProblem(oct_d), box_d, oracle_lib.OracleProblem(box_d)
+
+
+def test_noise_is_projected_onto_the_parity_on_the_low_walls():
+    L = oracle_lib.lib()
+    oct_d, o, _, _ = _octant_and_box()
+    u = o.fill_exact(0.0)
+    o.add_noise(u, 7, 1e-8)
+    P = oct_d.N + 1
+    U = u.reshape(-1, 20, P, P, P)
+    x = _points(oct_d, o.block_coords())
+    odd = _odd(L)
+    wall_odd, wall_even = 0.0, 0.0
+    for d in range(3):
+        on = x[..., d] == 0.0  # (b, k, j, i)
+        for v in range(20):
+            vals = np.abs(U[:, v][on])
+            if odd[v, d]:
+                wall_odd = max(wall_odd, float(vals.max()))
+            else:
+                wall_even = max(wall_even, float(vals.max()))
+    assert wall_odd == 0.0 and wall_even > 9e-9
+    assert float(np.abs(u).max()) <= 1e-8
+
+
+def test_the_octant_is_the_mirrored_box_to_roundoff():
+    """32 steps of dt = h / (4 sqrt 3) from noise put inside [0, 1]^3 (so the
+    box's evolved plane x = -4, which the octant has as its Dirichlet plane,
+    stays out of the data's reach): every owned point of the octant equals
+    the box's at the same position to 64 eps of the data's scale; the odd
+    variables on the walls stay at roundoff; and a control with the parities
+    ignored (every variable extended evenly into the box) is off by the
+    data's own size, so the comparison is not vacuous."""
+    L = oracle_lib.lib()
+    oct_d, o, box_d, bx = _octant_and_box()
+    P = oct_d.N + 1
+    uo = o.fill_exact(0.0)
+    o.add_noise(uo, 1, 1e-8)
+    xo = _points(oct_d, o.block_coords())
+    far = xo.max(axis=-1) > 1.0
+    Uo = uo.reshape(-1, 20, P, P, P)
+    for v in range(20):
+        Uo[:, v][far] = 0.0
+    odd = _odd(L)
+    xb = _points(box_d, bx.block_coords())
+    # the octant's owned point at a position: index by rounded coordinates
+    h = 0.25  # 4 / (2 blocks x 8 cells), the box's too
+    lookup = {}
+    for b in range(xo.shape[0]):
+        for idx in np.ndindex(P, P, P):
+            key = tuple(int(round(c / h)) for c in xo[b][idx])
+            lookup.setdefault(key, (b, idx))
+
+    def box_state(sign_by_parity):
+        ub = bx.fill_exact(0.0)
+        Ub = ub.reshape(-1, 20, P, P, P)
+        for b in range(xb.shape[0]):
+            for idx in np.ndindex(P, P, P):
+                x = xb[b][idx]
+                if np.abs(x).max() >= 4.0:
+                    continue
+                bo, io = lookup[tuple(int(round(abs(c) / h)) for c in x)]
+                for v in range(20):
+                    s = 1.0
+                    if sign_by_parity:
+                        for d in range(3):
+                            if x[d] < 0 and odd[v, d]:
+                                s = -s
+                    Ub[b, v][idx] = s * Uo[bo, v][io]
+        return ub
+
+    ub = box_state(True)
+    uc = box_state(False)
+    dt = h / (4 * np.sqrt(3.0))
+    for n in range(32):
+        o.rk4_step(uo, n * dt, dt, NTHREADS)
+        bx.rk4_step(ub, n * dt, dt, NTHREADS)
+        bx.rk4_step(uc, n * dt, dt, NTHREADS)
+    Ub = ub.reshape(-1, 20, P, P, P)
+    Uc = uc.reshape(-1, 20, P, P, P)
+    box_at = {}
+    for b in range(xb.shape[0]):
+        for idx in np.ndindex(P, P, P):
+            box_at.setdefault(tuple(int(round(c / h)) for c in xb[b][idx]), (b, idx))
+    diff, ctrl = 0.0, 0.0
+    for b in range(xo.shape[0]):
+        for idx in np.ndindex(P, P, P):
+            bb, ib = box_at[tuple(int(round(c / h)) for c in xo[b][idx])]
+            diff = max(diff, float(np.abs(Uo[b][(slice(None),) + idx] - Ub[bb][(slice(None),) + ib]).max()))
+            ctrl = max(ctrl, float(np.abs(Uo[b][(slice(None),) + idx] - Uc[bb][(slice(None),) + ib]).max()))
+    scale = float(np.abs(uo).max())
+    wall = 0.0
+    for d in range(3):
+        on = xo[..., d] == 0.0
+        for v in range(20):
+            if odd[v, d]:
+                wall = max(wall, float(np.abs(Uo[:, v][on]).max()))
+    print(f"octant vs mirrored box: max|diff| {diff:.3e}, scale {scale:.3e}; control {ctrl:.3e}; "
+          f"odd on the walls {wall:.3e}")
+    assert scale > 1e-9
+    assert diff <= 64 * np.finfo(float).eps * scale
+    assert ctrl > 1e-3 * scale
+    assert wall <= 64 * np.finfo(float).eps * scale

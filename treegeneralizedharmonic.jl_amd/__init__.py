@@ -36,7 +36,7 @@ __all__ = [
     "set_matter", "set_rho_max", "vjp_rhs", "adjoint", "ADJOINT_PARAMS", "evolve", "chunk_count",
     "chunk_bounds", "FittedInterior", "StateBounds", "HorizonDissipation", "horizon_dissipation",
     "horizon_min_radius", "horizon_max_radius", "default_bounds", "default_gate", "with_interior",
-    "record_norms",
+    "record_norms", "minkowski_octant_case",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -85,7 +85,7 @@ class TghDesc(C.Structure):
         ("velocity", C.c_double * 3),
         ("shaped", C.c_int32), ("shape_lmax", C.c_int32), ("shape", C.c_double * 81),
         ("r_in", C.c_double), ("r_out", C.c_double), ("offset", C.c_double), ("thickness", C.c_double),
-        ("ownership", C.c_int32),
+        ("ownership", C.c_int32), ("reflecting", (C.c_int32 * 2) * 3),
     ]
 
 
@@ -497,8 +497,20 @@ class GHCase:
     backgrounds) or "algebraic" (the closed-form Kerr-Schild source)."""
 
     def __init__(self, T, background, *, box, periodic, eps_ko, gamma0, gamma2,
-                 interior=None, gauge_source=None, bounds=None, velocity=(0.0, 0.0, 0.0)):
+                 interior=None, gauge_source=None, bounds=None, velocity=(0.0, 0.0, 0.0), reflecting=None):
         _check_float(T)
+        refl = ((False, False),) * 3 if reflecting is None else \
+            tuple((bool(r[0]), bool(r[1])) for r in reflecting)
+        if len(refl) != 3:
+            raise ArgumentError(f"reflecting is one (low, high) pair per axis, got {reflecting!r}")
+        for dd in range(3):
+            if periodic[dd] and any(refl[dd]):
+                raise ArgumentError(f"axis {dd} cannot wrap and mirror at once: periodic[{dd}] is set and "
+                                    f"reflecting[{dd}] = {refl[dd]}")
+        # a symmetry plane (src/boundaries.jl:84-142): the ghosts mirror the
+        # interior with each variable's parity; the hook fills only the faces
+        # that neither wrap nor reflect
+        self.reflecting = refl
         self.velocity = tuple(float(x) for x in velocity)
         if any(self.velocity) and not isinstance(background, (KerrSchild, Harmonic)):
             raise ArgumentError("only a Kerr-Schild or harmonic hole moves")
@@ -547,6 +559,16 @@ def minkowski_case(T=None, *, L, eps_ko, gamma0, gamma2):
                   gamma0=gamma0, gamma2=gamma2)
 
 
+def minkowski_octant_case(T=None, *, L, eps_ko, gamma0, gamma2):
+    """minkowski_octant_case (src/initialdata.jl:873-893): flat space on the
+    octant [0, L]^3 of the box [-L, L]^3, a symmetry plane at each of the three
+    faces through the origin and Minkowski's h = Pi = 0 at the three outer
+    ones -- the robust-stability case on the mesh of a hole at the origin.
+    (The rotating octant's quarter-turn seam is not built.)"""
+    return GHCase(T, Minkowski(), box=[(0.0, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
+                  gamma0=gamma0, gamma2=gamma2, reflecting=((True, False),) * 3)
+
+
 def gauge_wave_case(T=None, *, A=1 / 20, d=1.0, eps_ko, gamma0, gamma2):
     """The AwA gauge wave on one periodic wavelength cubed (src/initialdata.jl:908-911)."""
     return GHCase(T, GaugeWave(A, d), box=[(0.0, d)] * 3, periodic=(True,) * 3, eps_ko=eps_ko,
@@ -565,15 +587,46 @@ def _moving_hole(v, x0, axis):
     return on_axis(x0), on_axis(v)
 
 
+def _octant_box(octant, center, vel, a, L):
+    """The box and the reflecting faces of hole_case's `octant` (src/initialdata.jl:
+    993-1035, 1075-1170): None or False the whole box [-L, L]^3; True or
+    "reflecting" the octant [0, L]^3 mirrored at the three planes through the
+    origin; "quadrant" (-L, L) x (0, L)^2 mirrored at y = 0 and z = 0;
+    "bitant" (-L, L)^2 x (0, L) mirrored at z = 0. Each asks for a hole that is
+    its own image in its planes at all times; the rotating seam's variants
+    are not built."""
+    kind = "none" if octant in (None, False) else "reflecting" if octant is True else octant
+    if kind in ("rotating", "rotating_quadrant"):
+        raise NotImplementedError(f"octant = {kind!r} glues two faces by a quarter turn; that seam is not built")
+    if kind not in ("none", "reflecting", "quadrant", "bitant"):
+        raise ArgumentError(f"octant is None, True, \"reflecting\", \"quadrant\" or \"bitant\", got {octant!r}")
+    c, v = center, vel
+    if kind == "reflecting" and not (not any(c) and not any(v) and a == 0.0):
+        raise ArgumentError(f"the octant mirrors the solution in the three coordinate planes, which holds "
+                            f"only for a hole at the origin, at rest and without spin (a spinning one "
+                            f"keeps only the z mirror); got center {c}, velocity {v}, a = {a}")
+    if kind == "quadrant" and not (c[1] == c[2] == 0.0 and v[1] == v[2] == 0.0 and a == 0.0):
+        raise ArgumentError(f"the quadrant mirrors the solution in y = 0 and z = 0 at every time, which holds "
+                            f"only for a hole without spin on the x axis moving along it; got center {c}, "
+                            f"velocity {v}, a = {a}")
+    if kind == "bitant" and not (c[2] == 0.0 and v[2] == 0.0):
+        raise ArgumentError(f"the bitant mirrors the solution in z = 0 at every time, which holds only for a "
+                            f"hole in that plane moving in it, its spin along z; got center {c}, velocity {v}")
+    half = {"none": (), "reflecting": (0, 1, 2), "quadrant": (1, 2), "bitant": (2,)}[kind]
+    box = [((0.0 if d in half else -L), L) for d in range(3)]
+    return box, tuple((d in half, False) for d in range(3))
+
+
 def kerr_schild_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, r_0, r_1, rho_max=None,
                      eps_ko=0.5, gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
-                     gauge_source=None, bounds=None, margin=8):
+                     gauge_source=None, bounds=None, margin=8, octant=None):
     """kerr_schild_case / hole_case (src/initialdata.jl:1044-1174,1230-1298) for a
     hole in a Dirichlet box, at x0 + v t along axis. ρ_max defaults to the
     physical rate 4/M the driver sets per chunk. gauge_source=None samples
     H_a = −Γ_a of the background once (the reference's default, a hole at rest
     only); "algebraic" is the closed-form KerrSchildSource evaluated in the
-    kernel, which moves with the hole."""
+    kernel, which moves with the hole. `octant` puts the hole on a symmetric
+    part of the box with reflecting faces (_octant_box)."""
     center, vel = _moving_hole(v, x0, axis)
     if any(vel) and gauge_source is None:
         raise ArgumentError("a moving Kerr-Schild hole has a time-dependent H_a, which a source sampled once at "
@@ -584,14 +637,15 @@ def kerr_schild_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, 
     inter = Interior(T, center=center, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
                      variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp, margin=margin)
     L = float(halfwidth)
-    return GHCase(T, KerrSchild(M, a), box=[(-L, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
+    box, refl = _octant_box(octant, center, vel, float(a), L)
+    return GHCase(T, KerrSchild(M, a), box=box, periodic=(False,) * 3, eps_ko=eps_ko,
                   gamma0=damping, gamma2=gamma2, interior=inter, gauge_source=gauge_source,
-                  bounds=bounds, velocity=vel)
+                  bounds=bounds, velocity=vel, reflecting=refl)
 
 
 def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, r_0, r_1, rho_max=None,
                        eps_ko=0.5, gamma0=None, gamma2=0.0, interior="damped", w_ramp=0.5, rho_ramp=0.5,
-                       bounds=None, margin=8):
+                       bounds=None, margin=8, octant=None):
     """harmonic_kerr_case / hole_case (src/initialdata.jl:1300-1326, 1044-1174):
     Harmonic(M, a) in a Dirichlet box, at x0 + v t along axis, with no gauge
     source (a boost of the harmonic chart is harmonic). Defaults as
@@ -602,8 +656,10 @@ def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth
     inter = Interior(T, center=center, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
                      variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp, margin=margin)
     L = float(halfwidth)
-    return GHCase(T, Harmonic(M, a), box=[(-L, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
-                  gamma0=damping, gamma2=gamma2, interior=inter, bounds=bounds, velocity=vel)
+    box, refl = _octant_box(octant, center, vel, float(a), L)
+    return GHCase(T, Harmonic(M, a), box=box, periodic=(False,) * 3, eps_ko=eps_ko,
+                  gamma0=damping, gamma2=gamma2, interior=inter, bounds=bounds, velocity=vel,
+                  reflecting=refl)
 
 
 def _morton(i, j, k):
@@ -907,6 +963,9 @@ def _desc(forest, case, q):
     d.q, d.N = int(q), forest.N
     d.nblocks[:] = list(forest.roots)
     d.periodic[:] = [1 if p else 0 for p in case.periodic]
+    for dd in range(3):
+        r = getattr(case, "reflecting", ((False, False),) * 3)[dd]
+        d.reflecting[dd][0], d.reflecting[dd][1] = int(r[0]), int(r[1])
     bg = case.background
     d.background = bg.kind
     d.lo[:] = [b[0] for b in case.box]

@@ -92,6 +92,10 @@ int check_desc(const tgh_desc* d, int lmax = 0) {
   h0 /= (double)(1 << lmax);
   if (d->ownership != TGH_OWNERSHIP_CLOSED && d->ownership != TGH_OWNERSHIP_HALF_OPEN)
     return fail(TGH_EINVAL, "unknown ownership of block faces");
+  for (int i = 0; i < 3; ++i)
+    if (d->periodic[i] && (d->reflecting[i][0] || d->reflecting[i][1]))
+      return fail(TGH_EINVAL, "axis " + std::to_string(i) + " is marked periodic and reflecting: a wrap "
+                              "joins its two faces, a mirror ends the box at one, so pick one");
   if (d->background < 0 || d->background > 4) return fail(TGH_EINVAL, "unknown background");
   if (d->gauge < 0 || d->gauge > 2) return fail(TGH_EINVAL, "unknown gauge source kind");
   if (d->gauge == TGH_GAUGE_SAMPLED && d->background == TGH_BG_GAUGE_WAVE)
@@ -423,6 +427,8 @@ int ensure_brick(tgh_problem* p) {
   for (int d = 0; d < 3; ++d) {
     k.tot[d] = g.nb[d] * g.N;
     k.periodic[d] = g.periodic[d];
+    k.refl[d][0] = g.refl[d][0];
+    k.refl[d][1] = g.refl[d][1];
     k.n[d] = k.tot[d] + (g.periodic[d] ? 0 : 1);
     k.lo[d] = g.lo[d];
   }
@@ -441,7 +447,8 @@ int ensure_brick(tgh_problem* p) {
     HIP_TRY(hipMemsetAsync(*a, 0, bytes, p->stream));
   }
   const bool static_bg = static_background(p->cs);
-  p->frame_static = static_bg && !g.periodic[0] && !g.periodic[1] && !g.periodic[2];
+  // a wrapped or mirrored frame reads the stage's own values: refilled every stage
+  p->frame_static = static_bg && !g.periodic[0] && !g.periodic[1] && !g.periodic[2] && !reflects(g);
   if (p->frame_static) {
     launch_frame(p, p->BW[0], 0.0);
     launch_frame(p, p->BW[1], 0.0);
@@ -606,6 +613,13 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
   }
   int rc = check_desc(d, lmax);
   if (rc) return rc;
+  const bool mirrors = d->reflecting[0][0] || d->reflecting[0][1] || d->reflecting[1][0] ||
+                       d->reflecting[1][1] || d->reflecting[2][0] || d->reflecting[2][1];
+  if (leaves && mirrors)
+    return fail(TGH_EUNSUPPORTED, "reflecting faces are built for uniform bricks (tgh_problem_create); the "
+                                  "planned fill of a forest does not mirror");
+  if (mirrors && d->ownership == TGH_OWNERSHIP_HALF_OPEN)
+    return fail(TGH_EUNSUPPORTED, "reflecting faces are built for closed blocks, whose wall plane is owned");
   if (!leaves && d->ownership == TGH_OWNERSHIP_HALF_OPEN)
     return fail(TGH_EUNSUPPORTED, "half-open ownership of block faces is built for forests: create the "
                                   "problem with tgh_problem_create_forest (a uniform brick is the forest "
@@ -635,6 +649,8 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
   for (int i = 0; i < 3; ++i) {
     g.nb[i] = d->nblocks[i];
     g.periodic[i] = d->periodic[i] ? 1 : 0;
+    g.refl[i][0] = d->reflecting[i][0] ? 1 : 0;
+    g.refl[i][1] = d->reflecting[i][1] ? 1 : 0;
     g.lo[i] = d->lo[i];
   }
   g.nblocks = leaves ? nleaves : d->nblocks[0] * d->nblocks[1] * d->nblocks[2];
@@ -870,6 +886,8 @@ int tgh_transfer(tgh_problem* from, const double* u_from, double t, tgh_problem*
            a.lo[d] == b.lo[d] && a.hi[d] == b.hi[d];
   if (!same)
     return fail(TGH_EINVAL, "the transfer needs two forests of the same root brick: N, nblocks, periodic, lo and hi");
+  if (reflects(from->geom) || reflects(to->geom))
+    return fail(TGH_EUNSUPPORTED, "the transfer's plan does not mirror: regrid is built without reflecting faces");
   // a uniform problem is the forest of its level-0 blocks; the interpolation
   // order is from's, or to's where from has no levels
   const std::vector<int> flev = from->forest ? from->level_host : std::vector<int>(from->geom.nblocks, 0);
@@ -1420,6 +1438,13 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
       hipLaunchKernelGGL(k_brick_out, grid_points(p), dim3(256), 0, p->stream, ca);
     }
   }
+  if (rc == TGH_OK && reflects(p->geom)) {
+    // gh_solve returns a settled state (src/stepping.jl:235-239): the walls'
+    // odd variables, which the stages carried at roundoff, are zero
+    PointArgs wa{p->geom, p->cs, t0 + nsteps * dt, u, 0, 0.0, 3};
+    hipLaunchKernelGGL(p->cs.moving ? k_pointwise<true> : k_pointwise<false>, grid_points(p), dim3(256), 0,
+                       p->stream, wa);
+  }
   if (rc == TGH_OK) rc = sync(p);
   p->fused = false;
   p->per_block = false;
@@ -1445,6 +1470,7 @@ int adj_prepare(tgh_problem* p) {
   if (p->cs.moving) return fail(TGH_EUNSUPPORTED, "the adjoint is built for holes at rest");
   if (p->has_bounds) return fail(TGH_EUNSUPPORTED, "the adjoint is built for cases without range bounds");
   if (p->mat.rho) return fail(TGH_EUNSUPPORTED, "the adjoint is built for vacuum (no matter fields)");
+  if (reflects(p->geom)) return fail(TGH_EUNSUPPORTED, "the adjoint's transposed fill does not mirror");
   if (p->cs.interior == INT_PASTED) return fail(TGH_EUNSUPPORTED, "the adjoint of the :pasted limiter is not built");
   if (p->cs.gauge != GAUGE_NONE && p->cs.gauge != GAUGE_SAMPLED && p->cs.gauge != GAUGE_KS)
     return fail(TGH_EUNSUPPORTED, "the adjoint is built for the static gauge-source kinds");
@@ -1892,6 +1918,9 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
     if (p->cs.interior == INT_PASTED)
       return fail(TGH_EUNSUPPORTED, "a batch does not run the :pasted limiter; step problem " + std::to_string(i) +
                                         " alone (tgh_rk4)");
+    if (reflects(p->geom))
+      return fail(TGH_EUNSUPPORTED, "a batch steps problems without reflecting faces; step problem " +
+                                        std::to_string(i) + " alone (tgh_rk4)");
     if (p->mat.rho)
       return fail(TGH_EUNSUPPORTED, "a batch steps vacuum problems; problem " + std::to_string(i) + " has matter");
     if (blocks && (p->cs.gauge == GAUGE_KS_MOVING || p->cs.gauge == GAUGE_NONE_MOVING))

@@ -25,6 +25,7 @@ struct Geom {
   int N, G, S, P;          // cells per block, ghost width, work and state extents
   int nb[3];               // brick of blocks
   int periodic[3];
+  int refl[3][2];          // reflecting low / high face per axis
   int nblocks;
   double lo[3];
   double h, inv_h;         // level 0's spacing
@@ -409,27 +410,70 @@ TGH_HD void moving_core_rates(const CaseP& c, double t, const double x[3], const
 // primary (left as scattered), 0 a copy of (owner, loc). A face point is a
 // primary in the block where its local index is below N, and in the last
 // block of a non-periodic axis also at N (the evolved boundary plane).
-TGH_HD int classify(const Geom& g, int b, const int l[3], int& owner, int loc[3]) {
+// The parity of variable v (h_ab then Pi_ab, packed tt, tx, ty, tz, xx, xy, xz,
+// yy, yz, zz) under the mirror x^d -> -x^d (state_parity, src/boundaries.jl:
+// 84-120): bit d is set when the index d occurs an odd number of times in ab.
+TGH_HD int parity_odd_mask(int v) {
+  const int c = v % 10;
+  return c == 1 ? 1 : c == 2 ? 2 : c == 3 ? 4 : c == 5 ? 3 : c == 6 ? 5 : c == 8 ? 6 : 0;
+}
+// -1 when variable v changes sign under the mirrors of the axes in mask m.
+TGH_HD double parity_sign(int v, int m) {
+  const int o = parity_odd_mask(v) & m;
+  return ((o ^ (o >> 1) ^ (o >> 2)) & 1) ? -1.0 : 1.0;
+}
+
+// Where work point l (owned-relative, -G .. N + G) of block b takes its value:
+// -2 the Dirichlet hook, -1 its own primary, 0 a copy of the primary (owner,
+// loc); *mirror gets the axes across whose reflecting face the point lies
+// (the copy then takes each variable's parity). Beyond a face that neither
+// wraps nor reflects the point is the hook's, also where it lies beyond a
+// reflecting face on another axis.
+TGH_HD int classify(const Geom& g, int b, const int l[3], int& owner, int loc[3], int* mirror = nullptr) {
   int ob[3];
+  int mm = 0;
   const int* B = g.bcoord + 3 * b;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int gi = B[d] * g.N + l[d];
+    int gi = B[d] * g.N + l[d];
     const int tot = g.nb[d] * g.N;
     if (g.periodic[d]) {
       const int gw = ((gi % tot) + tot) % tot;
       ob[d] = gw / g.N;
       loc[d] = gw - ob[d] * g.N;
     } else {
-      if (gi < 0 || gi > tot) return -2;
+      if (gi < 0) {
+        if (!g.refl[d][0]) return -2;
+        gi = -gi;
+        mm |= 1 << d;
+      } else if (gi > tot) {
+        if (!g.refl[d][1]) return -2;
+        gi = 2 * tot - gi;
+        mm |= 1 << d;
+      }
       const int o = gi / g.N;
       ob[d] = o < g.nb[d] - 1 ? o : g.nb[d] - 1;
       loc[d] = gi - ob[d] * g.N;
     }
   }
+  if (mirror) *mirror = mm;
   owner = g.lookup[ob[0] + g.nb[0] * (ob[1] + g.nb[1] * ob[2])];
-  if (owner == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return -1;
+  if (mm == 0 && owner == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return -1;
   return 0;
+}
+// The axes whose reflecting wall plane holds owned point l of block b.
+TGH_HD int wall_mask(const Geom& g, int b, const int l[3]) {
+  int m = 0;
+  const int* B = g.bcoord + 3 * b;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int gi = B[d] * g.N + l[d], tot = g.nb[d] * g.N;
+    if ((g.refl[d][0] && gi == 0) || (g.refl[d][1] && gi == tot)) m |= 1 << d;
+  }
+  return m;
+}
+TGH_HD bool reflects(const Geom& g) {
+  return g.refl[0][0] | g.refl[0][1] | g.refl[1][0] | g.refl[1][1] | g.refl[2][0] | g.refl[2][1];
 }
 
 // XCD-aware workgroup order. MI355X dispatches consecutive workgroups of a
@@ -1187,11 +1231,21 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
   const int p = wx * blockDim.x + threadIdx.x;
   if (p >= S * S * S) return;
   const int l[3] = {p % S - G, (p / S) % S - G, p / (S * S) - G};
-  int owner, loc[3];
-  const int cl = classify(g, b, l, owner, loc);
-  if (cl == -1) return;
+  int owner, loc[3], mm = 0;
+  const int cl = classify(g, b, l, owner, loc, &mm);
   const long sv = (long)S * S * S;
   double* dst = a.W + (long)b * 20 * sv + p;
+  if (cl == -1) {
+    // a primary on a reflecting wall takes the parity: its odd variables are
+    // zero (TreeAMR's projection of the wall at every fill of closed blocks)
+    const int wm = wall_mask(g, b, l);
+    if (wm) {
+#pragma unroll
+      for (int v = 0; v < 20; ++v)
+        if (parity_odd_mask(v) & wm) dst[v * sv] = 0.0;
+    }
+    return;
+  }
   if (cl == -2) {
     double x[3], s[20];
     position(g, b, l[0], l[1], l[2], x);
@@ -1199,9 +1253,14 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
 #pragma unroll
     for (int v = 0; v < 20; ++v) dst[v * sv] = s[v];
   } else {
+    // the primary's value, projected as its own wall would be (the thread
+    // that owns it may not have written the projection yet), times the
+    // parities of the mirrors crossed
     const double* src = a.W + (long)owner * 20 * sv + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G));
+    const int sw = wall_mask(g, owner, loc);
 #pragma unroll
-    for (int v = 0; v < 20; ++v) dst[v * sv] = src[v * sv];
+    for (int v = 0; v < 20; ++v)
+      dst[v * sv] = (parity_odd_mask(v) & sw) ? 0.0 : parity_sign(v, mm) * src[v * sv];
   }
 }
 
@@ -1405,7 +1464,7 @@ struct PointArgs {
   double* __restrict__ u;
   uint64_t base;
   double amp;
-  int mode;  // 0 exact state, 1 add noise, 2 paste inside r1
+  int mode;  // 0 exact state, 1 add noise, 2 paste inside r1, 3 project the reflecting walls
 };
 template <bool MOVE>
 __global__ __launch_bounds__(256) void k_pointwise(PointArgs a) {
@@ -1421,7 +1480,28 @@ __global__ __launch_bounds__(256) void k_pointwise(PointArgs a) {
   double x[3];
   position(g, b, i, j, k, x);
   if (a.mode == 1) {
-    for (int v = 0; v < 20; ++v) ub[v * psz] += a.amp * (2.0 * point_draw(a.base, x, v + 1) - 1.0);
+    // on a low reflecting wall the noise is projected onto the parity: the
+    // odd variables there are zero (add_noise!, src/initialdata.jl:1938-1990,
+    // which projects the low walls only)
+    const int l[3] = {i, j, k};
+    int low = 0;
+    const int* B = g.bcoord + 3 * b;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (g.refl[d][0] && B[d] * g.N + l[d] == 0) low |= 1 << d;
+    for (int v = 0; v < 20; ++v) {
+      ub[v * psz] += a.amp * (2.0 * point_draw(a.base, x, v + 1) - 1.0);
+      if (parity_odd_mask(v) & low) ub[v * psz] = 0.0;
+    }
+    return;
+  }
+  if (a.mode == 3) {
+    // settle!'s wall: the odd variables of every reflecting wall plane zero
+    const int l[3] = {i, j, k};
+    const int wm = wall_mask(g, b, l);
+    if (wm)
+      for (int v = 0; v < 20; ++v)
+        if (parity_odd_mask(v) & wm) ub[v * psz] = 0.0;
     return;
   }
   if (a.mode == 2 && !(hole_radius(a.c, a.t, x) < a.c.r1)) return;
@@ -1955,6 +2035,7 @@ struct Brick {
   int n[3];      // evolved points per axis
   int tot[3];    // nb_d N
   int periodic[3];
+  int refl[3][2];  // reflecting low / high face per axis
   int G, XO;     // ghost width; x offset of x = 0 in a row (XO >= G)
   long py, pz, fs;
   double lo[3], h, inv_h;
@@ -2372,21 +2453,38 @@ __global__ __launch_bounds__(256) void k_brick_frame(FrameArgs a) {
   if (t >= E0 * E1 * E2) return;
   const int gp[3] = {(int)(t % E0) - k.G, (int)((t / E0) % E1) - k.G, (int)(t / (E0 * E1)) - k.G};
   bool inside = true, dirichlet = false;
-  int w[3];
+  int w[3], mm = 0, wall = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (gp[d] >= 0 && gp[d] < k.n[d]) {
       w[d] = gp[d];
+      if (!k.periodic[d] && ((k.refl[d][0] && gp[d] == 0) || (k.refl[d][1] && gp[d] == k.n[d] - 1)))
+        wall |= 1 << d;
     } else {
       inside = false;
-      if (k.periodic[d])
+      if (k.periodic[d]) {
         w[d] = ((gp[d] % k.tot[d]) + k.tot[d]) % k.tot[d];
-      else
+      } else if (gp[d] < 0 && k.refl[d][0]) {
+        w[d] = -gp[d];
+        mm |= 1 << d;
+      } else if (gp[d] >= k.n[d] && k.refl[d][1]) {
+        w[d] = 2 * (k.n[d] - 1) - gp[d];
+        mm |= 1 << d;
+      } else {
         dirichlet = true;
+      }
     }
   }
-  if (inside) return;
   double* dst = a.W + brick_index(k, gp[0], gp[1], gp[2]);
+  if (inside) {
+    // a point of a reflecting wall takes the parity: its odd variables are zero
+    if (wall) {
+#pragma unroll
+      for (int v = 0; v < 20; ++v)
+        if (parity_odd_mask(v) & wall) dst[v * k.fs] = 0.0;
+    }
+    return;
+  }
   if (dirichlet) {
     double x[3], s[20];
 #pragma unroll
@@ -2395,9 +2493,16 @@ __global__ __launch_bounds__(256) void k_brick_frame(FrameArgs a) {
 #pragma unroll
     for (int v = 0; v < 20; ++v) dst[v * k.fs] = s[v];
   } else {
+    // the source brick point, projected as its own walls would be, times the
+    // parities of the mirrors crossed
+    int sw = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (!k.periodic[d] && ((k.refl[d][0] && w[d] == 0) || (k.refl[d][1] && w[d] == k.n[d] - 1))) sw |= 1 << d;
     const double* src = a.W + brick_index(k, w[0], w[1], w[2]);
 #pragma unroll
-    for (int v = 0; v < 20; ++v) dst[v * k.fs] = src[v * k.fs];
+    for (int v = 0; v < 20; ++v)
+      dst[v * k.fs] = (parity_odd_mask(v) & sw) ? 0.0 : parity_sign(v, mm) * src[v * k.fs];
   }
 }
 
