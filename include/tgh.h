@@ -145,6 +145,14 @@ typedef struct tgh_desc {
    * fills only the faces that neither reflect nor wrap. Not with periodic on
    * the same axis; uniform forests only. Zero-initialised: no mirror. */
   int32_t reflecting[3][2];
+  /* The rotating seam (GHCase's rotating = (1, 2), src/boundaries.jl:145-205):
+   * nonzero glues the low x face to the low y face by a quarter turn about
+   * the z axis through (lo[0], lo[1]): the ghosts beyond x = lo[0] hold the
+   * state a quarter turn away, u(R p) = Q u(p) with R (x, y) -> (-y, x) and Q
+   * the tensor's signed permutation of the 20 variables; both seam planes
+   * are owned and evolved. Needs x and y non-periodic and non-reflecting,
+   * lo[0] = lo[1] and equal block counts on both. Zero: no seam. */
+  int32_t rotating;
 } tgh_desc;
 
 /* Matter's 3+1 projections at every owned point (src/evolution.jl:1295-1311,

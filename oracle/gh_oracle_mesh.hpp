@@ -43,6 +43,19 @@ inline double parity_sign(int v, int m) {
   return ((o & 1) ^ ((o >> 1) & 1) ^ ((o >> 2) & 1)) ? -1.0 : 1.0;
 }
 
+// The quarter turn about z (state_rotation, src/boundaries.jl:145-205):
+// with R (x, y, z) -> (-y, x, z), u(R p) = Q u(p), (Q u)[v] = rot_sign(v) *
+// u[rot_src(v)], the covariant tensor turned index by index (t and z stay,
+// the index x of the turned tensor is minus the original's y, y is x).
+inline int rot_src(int v) {
+  static const int src[10] = {0, 2, 1, 3, 7, 5, 8, 4, 6, 9};
+  return (v / 10) * 10 + src[v % 10];
+}
+inline double rot_sign(int v) {
+  static const double sg[10] = {1, -1, 1, 1, 1, -1, -1, 1, 1, 1};
+  return sg[v % 10];
+}
+
 struct Mesh {
   int N = 16, G = 3, q = 4;
   int nb[3] = {1, 1, 1};
@@ -51,6 +64,9 @@ struct Mesh {
   bool periodic[3] = {true, true, true};
   // reflecting faces: reflect[d][0] the low face of axis d, [d][1] the high one
   bool reflect[3][2] = {{false, false}, {false, false}, {false, false}};
+  // the rotating seam: the low x face glued to the low y face by a quarter
+  // turn about z through (lo[0], lo[1])
+  bool rotating = false;
   std::vector<std::array<int, 3>> blocks;  // block coordinates, Morton order
   long S = 0, P = 0;                       // work and state extents per axis
 
@@ -88,11 +104,33 @@ struct Mesh {
   // lies (its value is then the source's times the variables' parities). A
   // point beyond a face that neither wraps nor reflects is the hook's, also
   // where it lies beyond a reflecting face on another axis.
-  int classify(long b, const long l[3], long& owner, long loc[3], int* mirror = nullptr) const {
+  int classify(long b, const long l[3], long& owner, long loc[3], int* mirror = nullptr,
+               int* turn = nullptr) const {
     long ob[3];
-    int mm = 0;
+    int mm = 0, tn = 0;
+    long gs[3];
+    for (int d = 0; d < 3; ++d) gs[d] = blocks[b][d] * (long)N + l[d];
+    if (rotating) {
+      // a point beyond the seam's x face is R p, beyond its y face R^-1 p,
+      // beyond both R^2 p, with p the point inside: u = Q^turn u(p)
+      const long x = gs[0], y = gs[1];
+      if (x < 0 && y < 0) {
+        gs[0] = -x;
+        gs[1] = -y;
+        tn = 2;
+      } else if (x < 0) {
+        gs[0] = y;
+        gs[1] = -x;
+        tn = 1;
+      } else if (y < 0) {
+        gs[0] = -y;
+        gs[1] = x;
+        tn = 3;
+      }
+    }
+    if (turn) *turn = tn;
     for (int d = 0; d < 3; ++d) {
-      long g = blocks[b][d] * (long)N + l[d];
+      long g = gs[d];
       long tot = (long)nb[d] * N;
       if (periodic[d]) {
         long gw = ((g % tot) + tot) % tot;
@@ -115,7 +153,7 @@ struct Mesh {
     if (mirror) *mirror = mm;
     // owner block index (linear search over the Morton order is fine here)
     owner = block_index(ob[0], ob[1], ob[2]);
-    if (mm == 0 && owner == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return -1;
+    if (mm == 0 && tn == 0 && owner == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return -1;
     return 0;
   }
   // The axes whose reflecting wall plane the owned point l of block b lies on.
@@ -400,7 +438,10 @@ struct Problem {
             for (long i = 0; i <= m.N; ++i) {
               double x[3];
               m.position(b, i, j, k, x);
-              u[m.sidx(b, v, i, j, k)] += amp * (2.0 * point_draw(base, x, v + 1) - 1.0);
+              // the two seam planes are the same points a quarter turn apart:
+              // they keep their data (src/initialdata.jl:1947-1952)
+              if (!(m.rotating && ((m.blocks[b][0] == 0 && i == 0) || (m.blocks[b][1] == 0 && j == 0))))
+                u[m.sidx(b, v, i, j, k)] += amp * (2.0 * point_draw(base, x, v + 1) - 1.0);
               // on a low reflecting wall the noise is projected onto the
               // parity: the odd variables there are zero (add_noise!,
               // src/initialdata.jl:1938-1990, which projects the low walls)
@@ -442,8 +483,8 @@ struct Problem {
         for (long j = -m.G; j <= m.N + m.G; ++j)
           for (long i = -m.G; i <= m.N + m.G; ++i) {
             long l[3] = {i, j, k}, owner, loc[3];
-            int mm = 0;
-            int c = m.classify(b, l, owner, loc, &mm);
+            int mm = 0, tn = 0;
+            int c = m.classify(b, l, owner, loc, &mm, &tn);
             if (c == -1) continue;
             if (c == -2) {
               long g[3] = {m.blocks[b][0] * (long)m.N + i, m.blocks[b][1] * (long)m.N + j,
@@ -453,9 +494,16 @@ struct Problem {
               case_state(cs, t, x, s);
               for (int v = 0; v < 20; ++v) work[m.widx(b, v, i, j, k)] = s[v];
             } else {
-              for (int v = 0; v < 20; ++v)
-                work[m.widx(b, v, i, j, k)] =
-                    parity_sign(v, mm) * work[m.widx(owner, v, loc[0], loc[1], loc[2])];
+              // Q^tn: (Q w)[v] = rot_sign(v) w[rot_src(v)], applied tn times
+              for (int v = 0; v < 20; ++v) {
+                int sv = v;
+                double sg = parity_sign(v, mm);
+                for (int n = 0; n < tn; ++n) {
+                  sg *= rot_sign(sv);
+                  sv = rot_src(sv);
+                }
+                work[m.widx(b, v, i, j, k)] = sg * work[m.widx(owner, sv, loc[0], loc[1], loc[2])];
+              }
             }
           }
   }

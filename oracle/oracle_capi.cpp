@@ -297,6 +297,7 @@ static Problem* problem_from(const tgh_desc* d) {
     p->mesh.reflect[i][0] = d->reflecting[i][0] != 0;
     p->mesh.reflect[i][1] = d->reflecting[i][1] != 0;
   }
+  p->mesh.rotating = d->rotating != 0;
   p->mesh.h = (d->hi[0] - d->lo[0]) / ((double)d->nblocks[0] * d->N);
   p->cs = case_from(d);
   if (d->r_gate > 0) {
@@ -442,6 +443,9 @@ double orc_error_tangent(void* p, const double* u, const double* udot, double t)
 }
 // The mirror parity of variable v: bit d set when it is odd across x^d = 0.
 int orc_parity_odd_mask(int v) { return parity_odd_mask(v); }
+// The quarter turn's map of variable v: (Q u)[v] = sign * u[src], returned as
+// sign * (src + 1), the reference's state_rotation entry.
+int orc_rotation(int v) { return (int)rot_sign(v) * (rot_src(v) + 1); }
 // The matter fields every right-hand side adds (NULL rho: vacuum again).
 void orc_set_matter(void* pp, const double* rho, const double* S, const double* Sij) {
   Problem* p = (Problem*)pp;

@@ -89,6 +89,7 @@ def lib():
             "orc_set_rho_max": (None, [P, D]),
             "orc_set_matter": (None, [P, P, P, P]),
             "orc_parity_odd_mask": (C.c_int, [C.c_int]),
+            "orc_rotation": (C.c_int, [C.c_int]),
             "orc_is_primary": (C.c_int, [P, Lg, Lg, Lg, Lg]),
             "orc_count_rhs_flops": (D, [P, Lg, Lg, Lg, Lg, C.c_int]),
             "orc_max_threads": (C.c_int, []),

@@ -10,6 +10,9 @@ test/reflection_tests.jl) against the oracle, through the C ABI:
   the device's mirrored box to roundoff after 32 steps;
 * add_noise's projection of the low walls, bit for bit, and the settled
   walls tgh_rk4 returns;
+* the rotating seam (src/boundaries.jl:145-205, test/rotation_tests.jl): the
+  fill and RK4 on the rotating octant of the a = 3/10 hole against the
+  oracle, the device's octant against its box, the seam planes' noise;
 * the refusals: a wrapped and mirrored axis, a forest, a batch.
 Tolerances as the other parity tests: 1e-12 of rhs_scale or of max(1, |u|)."""
 import ctypes as C
@@ -230,3 +233,99 @@ def test_mirrors_are_refused_where_they_are_not_built(tgh):
     p = tgh_problem_from_desc(tgh, minkowski_octant())
     arr = (C.c_void_p * 1)(p.ptr.value)
     assert L.tgh_batch_create(arr, 1, C.byref(out)) == tgh.TGH_EUNSUPPORTED
+
+
+# -- the rotating seam ----------------------------------------------------------
+
+def rotating_hole(nb=2, octant=True):
+    """The a = 3/10 hole (q = 2, the sampled source, h = 1/16) on the rotating
+    octant [0, 2]^3 (x = 0 glued to y = 0 by a quarter turn about z, z = 0
+    mirrored), or in the box [-2, 2]^3."""
+    from test_oracle_reflect import _rotating_hole
+    return _rotating_hole(nb, octant)
+
+
+def test_seam_fill_matches_oracle(tgh):
+    """The fill of the rotating octant (ghosts turned across the seam, twice
+    at the axis's corner, mirrored across z = 0, the hook's) against the
+    oracle's: the turned copies are signed permutations, so equal to the
+    Dirichlet data's roundoff."""
+    torch = torch_cuda()
+    desc = rotating_hole()
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = _noisy(o, 0x5EA, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    tgh.prepare(p, ud, 0.0)
+    o.prepare(u, 0.0)
+    wg, wo = p.work().cpu().numpy(), o.work()
+    err = float((np.abs(wg - wo) / np.maximum(np.abs(wo), 1.0)).max())
+    print(f"rotating octant: fill |GPU - oracle| / max(1, |oracle|) = {err:.3e}")
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("form", ["brick-split", "brick-fused", "blocks-fused", "blocks-split"])
+def test_rk4_on_the_rotating_octant_matches_oracle(tgh, form):
+    torch = torch_cuda()
+    desc = rotating_hole()
+    p = tgh_problem_from_desc(tgh, desc)
+    o = oracle_lib.OracleProblem(desc)
+    u = _noisy(o, 0x5EB, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    dt = 0.25 * (1 / 16) / o.max_speed(u)
+    for n in range(2):
+        o.rk4_step(u, n * dt, dt, NTHREADS)
+    tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, RK4_FORMS[form]))
+    g = ud.cpu().numpy()
+    diff = float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max())
+    print(f"rotating octant, 2 RK4 steps ({form}): max rel diff {diff:.3e}")
+    assert diff <= 1e-12
+
+
+def test_the_rotating_octant_is_the_hole_in_the_box_on_the_device(tgh):
+    """test/rotation_tests.jl:148-190 on the device alone: the spinning hole
+    on the rotating octant and in the whole box, 8 steps from the exact
+    state: every owned point outside the core agrees to 64 eps of the
+    state's scale; and the seam's noise leaves both seam planes alone."""
+    torch = torch_cuda()
+    from test_oracle_reflect import _points
+    do, db = rotating_hole(2, True), rotating_hole(4, False)
+    po, pb = tgh_problem_from_desc(tgh, do), tgh_problem_from_desc(tgh, db)
+    oo, ob = oracle_lib.OracleProblem(do), oracle_lib.OracleProblem(db)
+    go = torch.from_numpy(oo.fill_exact(0.0)).cuda()
+    gb = torch.from_numpy(ob.fill_exact(0.0)).cuda()
+    dt = 0.25 * (1 / 16) / tgh.max_speed(pb, gb, 0.0)
+    for p, g in ((po, go), (pb, gb)):
+        tgh._check(tgh._load().tgh_rk4(p.ptr, tgh.C.c_void_p(g.data_ptr()), 0.0, dt, 8, 0))
+    P, h = do.N + 1, 1 / 16
+    xo, xb = _points(do, oo.block_coords()), _points(db, ob.block_coords())
+    Go, Gb = go.cpu().numpy().reshape(-1, 20, P, P, P), gb.cpu().numpy().reshape(-1, 20, P, P, P)
+    at = {}
+    for b in range(xb.shape[0]):
+        for idx in np.ndindex(P, P, P):
+            at.setdefault(tuple(int(round(c / h)) for c in xb[b][idx]), (b, idx))
+    diff, scale = 0.0, 0.0
+    for b in range(xo.shape[0]):
+        for idx in np.ndindex(P, P, P):
+            x = xo[b][idx]
+            if x @ x < 0.35 ** 2:
+                continue
+            bb, ib = at[tuple(int(round(c / h)) for c in x)]
+            diff = max(diff, float(np.abs(Go[b][(slice(None),) + idx] - Gb[bb][(slice(None),) + ib]).max()))
+            scale = max(scale, float(np.abs(Go[b][(slice(None),) + idx]).max()))
+    print(f"device rotating octant vs device box: max|diff| {diff:.3e}, scale {scale:.3e}")
+    assert diff <= 64 * np.finfo(float).eps * scale
+    # the noise: both seam planes keep their data, the device as the oracle
+    u = oo.fill_exact(0.0)
+    ud = torch.from_numpy(u.copy()).cuda()
+    oo.add_noise(u, 2, 1e-8)
+    tgh.add_noise(ud, po, 2, 1e-8)
+    g = ud.cpu().numpy()
+    # the same draws (the device's sum may be contracted: one rounding apart)
+    assert float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max()) <= 1e-15
+    # (off the z = 0 wall, whose odd variables the projection zeroes)
+    seam = ((xo[..., 0] == 0.0) | (xo[..., 1] == 0.0)) & (xo[..., 2] > 0.0)
+    U0 = oo.fill_exact(0.0).reshape(-1, 20, P, P, P)
+    for arr in (u, g):
+        moved = np.abs(arr.reshape(-1, 20, P, P, P) - U0).transpose(1, 0, 2, 3, 4)
+        assert float(moved[:, seam].max()) == 0.0 and float(moved[:, ~seam].max()) > 9e-9

@@ -162,3 +162,115 @@ def test_the_octant_is_the_mirrored_box_to_roundoff():
     assert diff <= 64 * np.finfo(float).eps * scale
     assert ctrl > 1e-3 * scale
     assert wall <= 64 * np.finfo(float).eps * scale
+
+
+# -- the rotating seam (src/boundaries.jl:145-205, test/rotation_tests.jl) -----
+
+ROT = [1, -3, 2, 4, 8, -6, -9, 5, 7, 10]  # the reference's state_rotation, tt .. zz
+
+
+def test_the_quarter_turn_is_the_tensors():
+    """The map every seam transfer turns the state with: the reference's table
+    (test/rotation_tests.jl:54-58), for h and for Pi, and against the
+    solution itself: u(R p) = Q u(p) for the spinning hole a = 3/10 at random
+    points outside its core, R (x, y, z) = (-y, x, z)."""
+    L = oracle_lib.lib()
+    r = [L.orc_rotation(v) for v in range(20)]
+    assert r[:10] == ROT
+    assert r[10:] == [int(np.sign(x)) * (abs(x) + 10) for x in ROT]
+    d = TD.kerr_schild_hole_case(N=12, nblocks=2, halfwidth=2.5, a=0.3, r0=0.35, r1=1.1,
+                                 gauge=TD.TGH_GAUGE_SAMPLED, q=2)
+    rng = np.random.default_rng(4)
+    worst = 0.0
+    import ctypes as C
+    for _ in range(20):
+        p = np.array([2 * rng.random() + 0.5, 2 * rng.random() + 0.5, 2 * rng.random() - 1.0])
+        u, uR = np.zeros(20), np.zeros(20)
+        L.orc_case_state(C.byref(d), 0.0, p, u)
+        L.orc_case_state(C.byref(d), 0.0, np.array([-p[1], p[0], p[2]]), uR)
+        want = np.array([np.sign(r[v]) * u[abs(r[v]) - 1] for v in range(20)])
+        worst = max(worst, float(np.abs(uR - want).max() / max(1.0, float(np.abs(u).max()))))
+    print(f"u(R p) against Q u(p): {worst:.3e}")
+    assert worst <= 64 * np.finfo(float).eps
+
+
+def _rotating_hole(nb, octant):
+    """The a = 3/10 hole, q = 2, the sampled source, r_0 = 0.35, r_1 = 1.1, at
+    h = 1/16 (exact in binary, so that the octant's and the box's points sit
+    at the same positions to the bit): the octant [0, 2]^3 on 2^3 blocks of
+    16^3, or the box [-2, 2]^3 on 4^3."""
+    d = TD.kerr_schild_hole_case(N=16, nblocks=nb, halfwidth=2.0, a=0.3, r0=0.35, r1=1.1,
+                                 gauge=TD.TGH_GAUGE_SAMPLED, q=2)
+    if octant:
+        d.lo[:] = [0.0] * 3
+        d.reflecting[2][0] = 1
+        d.rotating = 1
+    return d
+
+
+def test_the_seams_ghosts_are_the_solution_a_quarter_turn_away():
+    """After one fill of the exact state every stored point of every block --
+    owned, mirrored across z = 0, turned across the seam, turned twice at the
+    axis's corner, or the hook's -- holds the case's exact state at its own
+    position (test/rotation_tests.jl:115-146), outside the core, to the
+    roundoff of the solution's own symmetry."""
+    import ctypes as C
+    L = oracle_lib.lib()
+    d = _rotating_hole(2, True)
+    o = oracle_lib.OracleProblem(d)
+    o.prepare(o.fill_exact(0.0), 0.0)
+    W = o.work()
+    G, N = d.q // 2 + 1, d.N
+    S = N + 1 + 2 * G
+    h = 1 / 16
+    Wv = W.reshape(-1, 20, S, S, S)
+    worst, scale = 0.0, 0.0
+    e = np.zeros(20)
+    for b, (bx, by, bz) in enumerate(o.block_coords()):
+        for k in range(S):
+            for j in range(S):
+                for i in range(S):
+                    x = np.array([(bx * N + i - G) * h, (by * N + j - G) * h, (bz * N + k - G) * h])
+                    if x @ x < 0.35 ** 2:
+                        continue
+                    L.orc_case_state(C.byref(d), 0.0, x, e)
+                    worst = max(worst, float(np.abs(Wv[b, :, k, j, i] - e).max()))
+                    scale = max(scale, float(np.abs(e).max()))
+    print(f"seam ghosts against the exact state: {worst / scale:.3e} relative")
+    assert worst <= 256 * np.finfo(float).eps * scale
+
+
+def test_a_spinning_hole_on_the_rotating_octant_is_the_hole_in_the_box():
+    """The a = 3/10 hole (q = 2, h = 1/16, the sampled source, :damped layer)
+    on the rotating octant and in the whole box [-2, 2]^3, 8 RK4 steps
+    from the exact state: every owned point of the octant outside the core
+    equals the box's at its position to 64 eps of the state's scale
+    (test/rotation_tests.jl:148-190)."""
+    do, db = _rotating_hole(2, True), _rotating_hole(4, False)
+    o, bx = oracle_lib.OracleProblem(do), oracle_lib.OracleProblem(db)
+    uo, ub = o.fill_exact(0.0), bx.fill_exact(0.0)
+    dt = 0.25 * (1 / 16) / bx.max_speed(ub)
+    for n in range(8):
+        o.rk4_step(uo, n * dt, dt, NTHREADS)
+        bx.rk4_step(ub, n * dt, dt, NTHREADS)
+    P, h = do.N + 1, 1 / 16
+    xo, xb = _points(do, o.block_coords()), _points(db, bx.block_coords())
+    Uo, Ub = uo.reshape(-1, 20, P, P, P), ub.reshape(-1, 20, P, P, P)
+    at = {}
+    for b in range(xb.shape[0]):
+        for idx in np.ndindex(P, P, P):
+            at.setdefault(tuple(int(round(c / h)) for c in xb[b][idx]), (b, idx))
+    diff, scale = 0.0, 0.0
+    for b in range(xo.shape[0]):
+        for idx in np.ndindex(P, P, P):
+            x = xo[b][idx]
+            if x @ x < 0.35 ** 2:
+                continue
+            bb, ib = at[tuple(int(round(c / h)) for c in x)]
+            a, c = Uo[b][(slice(None),) + idx], Ub[bb][(slice(None),) + ib]
+            diff = max(diff, float(np.abs(a - c).max()))
+            scale = max(scale, float(np.abs(a).max()))
+    moved = float(np.abs(uo - o.fill_exact(0.0)).max())
+    print(f"rotating octant vs box after 8 steps: max|diff| {diff:.3e}, scale {scale:.3e}, moved {moved:.3e}")
+    assert moved > 1e-6
+    assert diff <= 64 * np.finfo(float).eps * scale

@@ -29,7 +29,7 @@ class TghDesc(C.Structure):
         ("velocity", C.c_double * 3),
         ("shaped", C.c_int32), ("shape_lmax", C.c_int32), ("shape", C.c_double * 81),
         ("r_in", C.c_double), ("r_out", C.c_double), ("offset", C.c_double), ("thickness", C.c_double),
-        ("ownership", C.c_int32), ("reflecting", (C.c_int32 * 2) * 3),
+        ("ownership", C.c_int32), ("reflecting", (C.c_int32 * 2) * 3), ("rotating", C.c_int32),
     ]
 
 
@@ -39,7 +39,8 @@ def make_desc(*, q=4, N=16, nblocks=(1, 1, 1), periodic=(1, 1, 1), lo=(0.0, 0.0,
               gamma0_width=3.0, gamma2=0.0, eps_ko=0.0, interior=TGH_INTERIOR_NONE,
               M=1.0, a=0.0, center=(0.0, 0.0, 0.0), wave_amp=0.05, wave_len=1.0,
               r0=0.0, r1=0.0, rho_max=0.0, w_ramp=0.5, rho_ramp=0.5, dissipation=0, eps_in=0.0,
-              eps_r1=0.0, eps_rh=0.0, velocity=(0.0, 0.0, 0.0), reflecting=((0, 0), (0, 0), (0, 0))):
+              eps_r1=0.0, eps_rh=0.0, velocity=(0.0, 0.0, 0.0), reflecting=((0, 0), (0, 0), (0, 0)),
+              rotating=0):
     d = TghDesc()
     d.float_bits = 64
     d.q, d.N = q, N
@@ -58,6 +59,7 @@ def make_desc(*, q=4, N=16, nblocks=(1, 1, 1), periodic=(1, 1, 1), lo=(0.0, 0.0,
     d.velocity[:] = list(velocity)
     for dd in range(3):
         d.reflecting[dd][0], d.reflecting[dd][1] = int(reflecting[dd][0]), int(reflecting[dd][1])
+    d.rotating = int(rotating)
     return d
 
 

@@ -85,7 +85,7 @@ class TghDesc(C.Structure):
         ("velocity", C.c_double * 3),
         ("shaped", C.c_int32), ("shape_lmax", C.c_int32), ("shape", C.c_double * 81),
         ("r_in", C.c_double), ("r_out", C.c_double), ("offset", C.c_double), ("thickness", C.c_double),
-        ("ownership", C.c_int32), ("reflecting", (C.c_int32 * 2) * 3),
+        ("ownership", C.c_int32), ("reflecting", (C.c_int32 * 2) * 3), ("rotating", C.c_int32),
     ]
 
 
@@ -497,7 +497,8 @@ class GHCase:
     backgrounds) or "algebraic" (the closed-form Kerr-Schild source)."""
 
     def __init__(self, T, background, *, box, periodic, eps_ko, gamma0, gamma2,
-                 interior=None, gauge_source=None, bounds=None, velocity=(0.0, 0.0, 0.0), reflecting=None):
+                 interior=None, gauge_source=None, bounds=None, velocity=(0.0, 0.0, 0.0), reflecting=None,
+                 rotating=None):
         _check_float(T)
         refl = ((False, False),) * 3 if reflecting is None else \
             tuple((bool(r[0]), bool(r[1])) for r in reflecting)
@@ -511,6 +512,26 @@ class GHCase:
         # interior with each variable's parity; the hook fills only the faces
         # that neither wrap nor reflect
         self.reflecting = refl
+        # the rotating seam (src/boundaries.jl:145-205): the low faces of d1 and
+        # d2 glued by a quarter turn; (0, 0) for none
+        rot = (0, 0) if rotating is None else tuple(int(x) for x in rotating)
+        if rot != (0, 0):
+            if len(rot) != 2 or rot[0] == rot[1] or not all(1 <= x <= 3 for x in rot):
+                raise ArgumentError(f"rotating is a pair of two different axes (1-based), got {rotating!r}")
+            if rot != (1, 2):
+                raise NotImplementedError(f"the rotating seam is built for the axes (1, 2), a quarter turn "
+                                          f"about z; got {rot}")
+            for dd in (0, 1):
+                if periodic[dd]:
+                    raise ArgumentError(f"the seam's axis {dd + 1} is periodic: a seam ends the box there")
+                if any(refl[dd]):
+                    raise ArgumentError(f"the seam's axis {dd + 1} is reflecting: a face is glued or "
+                                        f"mirrored, not both")
+            w = [b[1] - b[0] for b in box]
+            if abs(w[0] - w[1]) > 1e-12 * abs(w[0]) or box[0][0] != box[1][0]:
+                raise ArgumentError(f"the seam turns x into y about the axis through their low faces, so "
+                                    f"the two need one width and one low end; got {box[0]} and {box[1]}")
+        self.rotating = rot
         self.velocity = tuple(float(x) for x in velocity)
         if any(self.velocity) and not isinstance(background, (KerrSchild, Harmonic)):
             raise ArgumentError("only a Kerr-Schild or harmonic hole moves")
@@ -559,14 +580,17 @@ def minkowski_case(T=None, *, L, eps_ko, gamma0, gamma2):
                   gamma0=gamma0, gamma2=gamma2)
 
 
-def minkowski_octant_case(T=None, *, L, eps_ko, gamma0, gamma2):
+def minkowski_octant_case(T=None, *, L, eps_ko, gamma0, gamma2, rotating=False):
     """minkowski_octant_case (src/initialdata.jl:873-893): flat space on the
     octant [0, L]^3 of the box [-L, L]^3, a symmetry plane at each of the three
     faces through the origin and Minkowski's h = Pi = 0 at the three outer
     ones -- the robust-stability case on the mesh of a hole at the origin.
-    (The rotating octant's quarter-turn seam is not built.)"""
+    `rotating` glues x = 0 to y = 0 by a quarter turn about z instead, and
+    only z = 0 reflects."""
     return GHCase(T, Minkowski(), box=[(0.0, L)] * 3, periodic=(False,) * 3, eps_ko=eps_ko,
-                  gamma0=gamma0, gamma2=gamma2, reflecting=((True, False),) * 3)
+                  gamma0=gamma0, gamma2=gamma2,
+                  reflecting=tuple((not rotating or d == 2, False) for d in range(3)),
+                  rotating=(1, 2) if rotating else None)
 
 
 def gauge_wave_case(T=None, *, A=1 / 20, d=1.0, eps_ko, gamma0, gamma2):
@@ -588,18 +612,19 @@ def _moving_hole(v, x0, axis):
 
 
 def _octant_box(octant, center, vel, a, L):
-    """The box and the reflecting faces of hole_case's `octant` (src/initialdata.jl:
-    993-1035, 1075-1170): None or False the whole box [-L, L]^3; True or
-    "reflecting" the octant [0, L]^3 mirrored at the three planes through the
-    origin; "quadrant" (-L, L) x (0, L)^2 mirrored at y = 0 and z = 0;
-    "bitant" (-L, L)^2 x (0, L) mirrored at z = 0. Each asks for a hole that is
-    its own image in its planes at all times; the rotating seam's variants
-    are not built."""
+    """The box, the reflecting faces and the rotating seam of hole_case's
+    `octant` (src/initialdata.jl:993-1035, 1075-1170): None or False the whole
+    box [-L, L]^3; True or "reflecting" the octant [0, L]^3 mirrored at the
+    three planes through the origin; "rotating" the same octant with x = 0
+    glued to y = 0 by a quarter turn about z and a mirror at z = 0;
+    "quadrant" (-L, L) x (0, L)^2 mirrored at y = 0 and z = 0;
+    "rotating_quadrant" (0, L)^2 x (-L, L) with the seam and no mirror;
+    "bitant" (-L, L)^2 x (0, L) mirrored at z = 0. Each asks for a hole that
+    is its own image under its maps at all times."""
     kind = "none" if octant in (None, False) else "reflecting" if octant is True else octant
-    if kind in ("rotating", "rotating_quadrant"):
-        raise NotImplementedError(f"octant = {kind!r} glues two faces by a quarter turn; that seam is not built")
-    if kind not in ("none", "reflecting", "quadrant", "bitant"):
-        raise ArgumentError(f"octant is None, True, \"reflecting\", \"quadrant\" or \"bitant\", got {octant!r}")
+    if kind not in ("none", "reflecting", "rotating", "quadrant", "rotating_quadrant", "bitant"):
+        raise ArgumentError(f"octant is None, True, \"reflecting\", \"rotating\", \"quadrant\", "
+                            f"\"rotating_quadrant\" or \"bitant\", got {octant!r}")
     c, v = center, vel
     if kind == "reflecting" and not (not any(c) and not any(v) and a == 0.0):
         raise ArgumentError(f"the octant mirrors the solution in the three coordinate planes, which holds "
@@ -612,9 +637,19 @@ def _octant_box(octant, center, vel, a, L):
     if kind == "bitant" and not (c[2] == 0.0 and v[2] == 0.0):
         raise ArgumentError(f"the bitant mirrors the solution in z = 0 at every time, which holds only for a "
                             f"hole in that plane moving in it, its spin along z; got center {c}, velocity {v}")
-    half = {"none": (), "reflecting": (0, 1, 2), "quadrant": (1, 2), "bitant": (2,)}[kind]
+    if kind == "rotating" and not (not any(c) and not any(v)):
+        raise ArgumentError(f"the rotating octant turns the solution by quarter turns about z and mirrors it "
+                            f"in z = 0, which holds only for a hole at the origin and at rest, spinning about "
+                            f"z; got center {c}, velocity {v}")
+    if kind == "rotating_quadrant" and not (c[0] == c[1] == 0.0 and v[0] == v[1] == 0.0):
+        raise ArgumentError(f"the rotating quadrant turns the solution by quarter turns about z at every time, "
+                            f"which holds only for a hole on the z axis moving along it; got center {c}, "
+                            f"velocity {v}")
+    half = {"none": (), "reflecting": (0, 1, 2), "rotating": (0, 1, 2), "quadrant": (1, 2),
+            "rotating_quadrant": (0, 1), "bitant": (2,)}[kind]
+    mirror = {"reflecting": (0, 1, 2), "rotating": (2,), "quadrant": (1, 2), "bitant": (2,)}.get(kind, ())
     box = [((0.0 if d in half else -L), L) for d in range(3)]
-    return box, tuple((d in half, False) for d in range(3))
+    return box, tuple((d in mirror, False) for d in range(3)), (1, 2) if kind.startswith("rotating") else None
 
 
 def kerr_schild_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, r_0, r_1, rho_max=None,
@@ -637,10 +672,10 @@ def kerr_schild_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, 
     inter = Interior(T, center=center, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
                      variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp, margin=margin)
     L = float(halfwidth)
-    box, refl = _octant_box(octant, center, vel, float(a), L)
+    box, refl, rot = _octant_box(octant, center, vel, float(a), L)
     return GHCase(T, KerrSchild(M, a), box=box, periodic=(False,) * 3, eps_ko=eps_ko,
                   gamma0=damping, gamma2=gamma2, interior=inter, gauge_source=gauge_source,
-                  bounds=bounds, velocity=vel, reflecting=refl)
+                  bounds=bounds, velocity=vel, reflecting=refl, rotating=rot)
 
 
 def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth, r_0, r_1, rho_max=None,
@@ -656,10 +691,10 @@ def harmonic_kerr_case(T=None, *, M=1.0, a=0.0, v=0.0, x0=0.0, axis=1, halfwidth
     inter = Interior(T, center=center, r_0=r_0, r_1=r_1, rho_max=(4.0 / M if rho_max is None else rho_max),
                      variant=interior, w_ramp=w_ramp, rho_ramp=rho_ramp, margin=margin)
     L = float(halfwidth)
-    box, refl = _octant_box(octant, center, vel, float(a), L)
+    box, refl, rot = _octant_box(octant, center, vel, float(a), L)
     return GHCase(T, Harmonic(M, a), box=box, periodic=(False,) * 3, eps_ko=eps_ko,
                   gamma0=damping, gamma2=gamma2, interior=inter, bounds=bounds, velocity=vel,
-                  reflecting=refl)
+                  reflecting=refl, rotating=rot)
 
 
 def _morton(i, j, k):
@@ -966,6 +1001,7 @@ def _desc(forest, case, q):
     for dd in range(3):
         r = getattr(case, "reflecting", ((False, False),) * 3)[dd]
         d.reflecting[dd][0], d.reflecting[dd][1] = int(r[0]), int(r[1])
+    d.rotating = 1 if getattr(case, "rotating", (0, 0)) == (1, 2) else 0
     bg = case.background
     d.background = bg.kind
     d.lo[:] = [b[0] for b in case.box]

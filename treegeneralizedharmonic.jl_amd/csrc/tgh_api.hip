@@ -96,6 +96,17 @@ int check_desc(const tgh_desc* d, int lmax = 0) {
     if (d->periodic[i] && (d->reflecting[i][0] || d->reflecting[i][1]))
       return fail(TGH_EINVAL, "axis " + std::to_string(i) + " is marked periodic and reflecting: a wrap "
                               "joins its two faces, a mirror ends the box at one, so pick one");
+  if (d->rotating) {
+    // the seam glues the low x face to the low y face (src/initialdata.jl:240-290's checks)
+    for (int i = 0; i < 2; ++i) {
+      if (d->periodic[i]) return fail(TGH_EINVAL, "the rotating seam's axis " + std::to_string(i) + " is periodic");
+      if (d->reflecting[i][0] || d->reflecting[i][1])
+        return fail(TGH_EINVAL, "the rotating seam's axis " + std::to_string(i) + " also reflects");
+    }
+    if (d->lo[0] != d->lo[1] || d->nblocks[0] != d->nblocks[1])
+      return fail(TGH_EINVAL, "the rotating seam turns x into y about the axis x = y = lo: it needs "
+                              "lo[0] = lo[1] and the same number of blocks along x and y");
+  }
   if (d->background < 0 || d->background > 4) return fail(TGH_EINVAL, "unknown background");
   if (d->gauge < 0 || d->gauge > 2) return fail(TGH_EINVAL, "unknown gauge source kind");
   if (d->gauge == TGH_GAUGE_SAMPLED && d->background == TGH_BG_GAUGE_WAVE)
@@ -429,6 +440,7 @@ int ensure_brick(tgh_problem* p) {
     k.periodic[d] = g.periodic[d];
     k.refl[d][0] = g.refl[d][0];
     k.refl[d][1] = g.refl[d][1];
+    k.rot = g.rot;
     k.n[d] = k.tot[d] + (g.periodic[d] ? 0 : 1);
     k.lo[d] = g.lo[d];
   }
@@ -614,10 +626,10 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
   int rc = check_desc(d, lmax);
   if (rc) return rc;
   const bool mirrors = d->reflecting[0][0] || d->reflecting[0][1] || d->reflecting[1][0] ||
-                       d->reflecting[1][1] || d->reflecting[2][0] || d->reflecting[2][1];
+                       d->reflecting[1][1] || d->reflecting[2][0] || d->reflecting[2][1] || d->rotating;
   if (leaves && mirrors)
-    return fail(TGH_EUNSUPPORTED, "reflecting faces are built for uniform bricks (tgh_problem_create); the "
-                                  "planned fill of a forest does not mirror");
+    return fail(TGH_EUNSUPPORTED, "reflecting faces and the rotating seam are built for uniform bricks "
+                                  "(tgh_problem_create); the planned fill of a forest neither mirrors nor turns");
   if (mirrors && d->ownership == TGH_OWNERSHIP_HALF_OPEN)
     return fail(TGH_EUNSUPPORTED, "reflecting faces are built for closed blocks, whose wall plane is owned");
   if (!leaves && d->ownership == TGH_OWNERSHIP_HALF_OPEN)
@@ -651,6 +663,7 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
     g.periodic[i] = d->periodic[i] ? 1 : 0;
     g.refl[i][0] = d->reflecting[i][0] ? 1 : 0;
     g.refl[i][1] = d->reflecting[i][1] ? 1 : 0;
+    g.rot = d->rotating ? 1 : 0;
     g.lo[i] = d->lo[i];
   }
   g.nblocks = leaves ? nleaves : d->nblocks[0] * d->nblocks[1] * d->nblocks[2];

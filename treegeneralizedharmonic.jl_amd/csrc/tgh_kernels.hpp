@@ -26,6 +26,7 @@ struct Geom {
   int nb[3];               // brick of blocks
   int periodic[3];
   int refl[3][2];          // reflecting low / high face per axis
+  int rot;                 // the rotating seam: the low x and y faces glued by a quarter turn about z
   int nblocks;
   double lo[3];
   double h, inv_h;         // level 0's spacing
@@ -422,6 +423,50 @@ TGH_HD double parity_sign(int v, int m) {
   const int o = parity_odd_mask(v) & m;
   return ((o ^ (o >> 1) ^ (o >> 2)) & 1) ? -1.0 : 1.0;
 }
+// The quarter turn about z (state_rotation, src/boundaries.jl:145-205): with
+// R (x, y, z) -> (-y, x, z), u(R p) = Q u(p), (Q u)[v] = rot_sign(v) u[rot_src(v)]
+// -- the covariant tensor turned index by index: t and z stay, the turned
+// tensor's index x is minus the original's y, its y the original's x.
+TGH_HD int rot_src(int v) {
+  const int c = v % 10;
+  const int s = c == 1 ? 2 : c == 2 ? 1 : c == 4 ? 7 : c == 6 ? 8 : c == 7 ? 4 : c == 8 ? 6 : c;
+  return v - c + s;
+}
+TGH_HD double rot_sign(int v) {
+  const int c = v % 10;
+  return (c == 1 || c == 5 || c == 6) ? -1.0 : 1.0;
+}
+// Q^tn applied to variable v: the source variable and the sign.
+TGH_HD void rot_power(int v, int tn, int& sv, double& sg) {
+  sv = v;
+  sg = 1.0;
+  for (int n = 0; n < tn; ++n) {
+    sg *= rot_sign(sv);
+    sv = rot_src(sv);
+  }
+}
+// The seam's map of a global point (x, y) beyond the low x and/or y face: the
+// point inside it is the image of, and the number of quarter turns tn, with
+// u(point) = Q^tn u(inside point). Unchanged (tn = 0) inside the quadrant.
+TGH_HD int rot_unfold(int& x, int& y) {
+  const int x0 = x, y0 = y;
+  if (x0 < 0 && y0 < 0) {
+    x = -x0;
+    y = -y0;
+    return 2;
+  }
+  if (x0 < 0) {
+    x = y0;
+    y = -x0;
+    return 1;
+  }
+  if (y0 < 0) {
+    x = -y0;
+    y = x0;
+    return 3;
+  }
+  return 0;
+}
 
 // Where work point l (owned-relative, -G .. N + G) of block b takes its value:
 // -2 the Dirichlet hook, -1 its own primary, 0 a copy of the primary (owner,
@@ -429,13 +474,17 @@ TGH_HD double parity_sign(int v, int m) {
 // (the copy then takes each variable's parity). Beyond a face that neither
 // wraps nor reflects the point is the hook's, also where it lies beyond a
 // reflecting face on another axis.
-TGH_HD int classify(const Geom& g, int b, const int l[3], int& owner, int loc[3], int* mirror = nullptr) {
+TGH_HD int classify(const Geom& g, int b, const int l[3], int& owner, int loc[3], int* mirror = nullptr,
+                    int* turn = nullptr) {
   int ob[3];
   int mm = 0;
   const int* B = g.bcoord + 3 * b;
+  int gs[3] = {B[0] * g.N + l[0], B[1] * g.N + l[1], B[2] * g.N + l[2]};
+  const int tn = g.rot ? rot_unfold(gs[0], gs[1]) : 0;
+  if (turn) *turn = tn;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    int gi = B[d] * g.N + l[d];
+    int gi = gs[d];
     const int tot = g.nb[d] * g.N;
     if (g.periodic[d]) {
       const int gw = ((gi % tot) + tot) % tot;
@@ -458,7 +507,7 @@ TGH_HD int classify(const Geom& g, int b, const int l[3], int& owner, int loc[3]
   }
   if (mirror) *mirror = mm;
   owner = g.lookup[ob[0] + g.nb[0] * (ob[1] + g.nb[1] * ob[2])];
-  if (mm == 0 && owner == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return -1;
+  if (mm == 0 && tn == 0 && owner == b && loc[0] == l[0] && loc[1] == l[1] && loc[2] == l[2]) return -1;
   return 0;
 }
 // The axes whose reflecting wall plane holds owned point l of block b.
@@ -473,7 +522,7 @@ TGH_HD int wall_mask(const Geom& g, int b, const int l[3]) {
   return m;
 }
 TGH_HD bool reflects(const Geom& g) {
-  return g.refl[0][0] | g.refl[0][1] | g.refl[1][0] | g.refl[1][1] | g.refl[2][0] | g.refl[2][1];
+  return g.refl[0][0] | g.refl[0][1] | g.refl[1][0] | g.refl[1][1] | g.refl[2][0] | g.refl[2][1] | g.rot;
 }
 
 // XCD-aware workgroup order. MI355X dispatches consecutive workgroups of a
@@ -1231,8 +1280,8 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
   const int p = wx * blockDim.x + threadIdx.x;
   if (p >= S * S * S) return;
   const int l[3] = {p % S - G, (p / S) % S - G, p / (S * S) - G};
-  int owner, loc[3], mm = 0;
-  const int cl = classify(g, b, l, owner, loc, &mm);
+  int owner, loc[3], mm = 0, tn = 0;
+  const int cl = classify(g, b, l, owner, loc, &mm, &tn);
   const long sv = (long)S * S * S;
   double* dst = a.W + (long)b * 20 * sv + p;
   if (cl == -1) {
@@ -1258,9 +1307,19 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
     // parities of the mirrors crossed
     const double* src = a.W + (long)owner * 20 * sv + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G));
     const int sw = wall_mask(g, owner, loc);
+    if (tn == 0) {
 #pragma unroll
-    for (int v = 0; v < 20; ++v)
-      dst[v * sv] = (parity_odd_mask(v) & sw) ? 0.0 : parity_sign(v, mm) * src[v * sv];
+      for (int v = 0; v < 20; ++v)
+        dst[v * sv] = (parity_odd_mask(v) & sw) ? 0.0 : parity_sign(v, mm) * src[v * sv];
+    } else {
+      // across the rotating seam: Q^tn of the (projected) source, then the mirrors
+      for (int v = 0; v < 20; ++v) {
+        int q;
+        double sg;
+        rot_power(v, tn, q, sg);
+        dst[v * sv] = (parity_odd_mask(q) & sw) ? 0.0 : parity_sign(v, mm) * sg * src[q * sv];
+      }
+    }
   }
 }
 
@@ -1489,8 +1548,10 @@ __global__ __launch_bounds__(256) void k_pointwise(PointArgs a) {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
       if (g.refl[d][0] && B[d] * g.N + l[d] == 0) low |= 1 << d;
+    // the two seam planes are the same points a quarter turn apart: they keep their data
+    const bool seam = g.rot && ((B[0] * g.N + l[0]) == 0 || (B[1] * g.N + l[1]) == 0);
     for (int v = 0; v < 20; ++v) {
-      ub[v * psz] += a.amp * (2.0 * point_draw(a.base, x, v + 1) - 1.0);
+      if (!seam) ub[v * psz] += a.amp * (2.0 * point_draw(a.base, x, v + 1) - 1.0);
       if (parity_odd_mask(v) & low) ub[v * psz] = 0.0;
     }
     return;
@@ -2036,6 +2097,7 @@ struct Brick {
   int tot[3];    // nb_d N
   int periodic[3];
   int refl[3][2];  // reflecting low / high face per axis
+  int rot;         // the rotating seam (low x and y faces, a quarter turn about z)
   int G, XO;     // ghost width; x offset of x = 0 in a row (XO >= G)
   long py, pz, fs;
   double lo[3], h, inv_h;
@@ -2454,21 +2516,24 @@ __global__ __launch_bounds__(256) void k_brick_frame(FrameArgs a) {
   const int gp[3] = {(int)(t % E0) - k.G, (int)((t / E0) % E1) - k.G, (int)(t / (E0 * E1)) - k.G};
   bool inside = true, dirichlet = false;
   int w[3], mm = 0, wall = 0;
+  // beyond the rotating seam the point inside is the image of, tn quarter turns
+  int gs[3] = {gp[0], gp[1], gp[2]};
+  const int tn = k.rot ? rot_unfold(gs[0], gs[1]) : 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (gp[d] >= 0 && gp[d] < k.n[d]) {
-      w[d] = gp[d];
+    if (gp[d] < 0 || gp[d] >= k.n[d]) inside = false;
+    if (gs[d] >= 0 && gs[d] < k.n[d]) {
+      w[d] = gs[d];
       if (!k.periodic[d] && ((k.refl[d][0] && gp[d] == 0) || (k.refl[d][1] && gp[d] == k.n[d] - 1)))
         wall |= 1 << d;
     } else {
-      inside = false;
       if (k.periodic[d]) {
-        w[d] = ((gp[d] % k.tot[d]) + k.tot[d]) % k.tot[d];
-      } else if (gp[d] < 0 && k.refl[d][0]) {
-        w[d] = -gp[d];
+        w[d] = ((gs[d] % k.tot[d]) + k.tot[d]) % k.tot[d];
+      } else if (gs[d] < 0 && k.refl[d][0]) {
+        w[d] = -gs[d];
         mm |= 1 << d;
-      } else if (gp[d] >= k.n[d] && k.refl[d][1]) {
-        w[d] = 2 * (k.n[d] - 1) - gp[d];
+      } else if (gs[d] >= k.n[d] && k.refl[d][1]) {
+        w[d] = 2 * (k.n[d] - 1) - gs[d];
         mm |= 1 << d;
       } else {
         dirichlet = true;
@@ -2500,9 +2565,18 @@ __global__ __launch_bounds__(256) void k_brick_frame(FrameArgs a) {
     for (int d = 0; d < 3; ++d)
       if (!k.periodic[d] && ((k.refl[d][0] && w[d] == 0) || (k.refl[d][1] && w[d] == k.n[d] - 1))) sw |= 1 << d;
     const double* src = a.W + brick_index(k, w[0], w[1], w[2]);
+    if (tn == 0) {
 #pragma unroll
-    for (int v = 0; v < 20; ++v)
-      dst[v * k.fs] = (parity_odd_mask(v) & sw) ? 0.0 : parity_sign(v, mm) * src[v * k.fs];
+      for (int v = 0; v < 20; ++v)
+        dst[v * k.fs] = (parity_odd_mask(v) & sw) ? 0.0 : parity_sign(v, mm) * src[v * k.fs];
+    } else {
+      for (int v = 0; v < 20; ++v) {
+        int q;
+        double sg;
+        rot_power(v, tn, q, sg);
+        dst[v * k.fs] = (parity_odd_mask(q) & sw) ? 0.0 : parity_sign(v, mm) * sg * src[q * k.fs];
+      }
+    }
   }
 }
 
