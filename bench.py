@@ -275,6 +275,13 @@ def batch_bench(tgh, torch, steps):
         tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, steps, 1))
     torch.cuda.synchronize()
     sequential = points * steps / (time.perf_counter() - t0)
+    # the plain form: each hole alone, its launches without a graph (one per
+    # kernel per stage over all its blocks)
+    t0 = time.perf_counter()
+    for p, u in zip(probs, us):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, steps, 0))
+    torch.cuda.synchronize()
+    plain = points * steps / (time.perf_counter() - t0)
     t0 = time.perf_counter()
     for p, u in zip(probs, us):
         tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, 1, tgh.TGH_RK4_PER_BLOCK))
@@ -285,7 +292,8 @@ def batch_bench(tgh, torch, steps):
     return {"workload": "four static Kerr-Schild holes a = 0, 0.3, 0.5, 0.7 (closed-form gauge "
                         "source, damped interior) on 7^3, 9^3, 11^3, 13^3 blocks of 16^3 at h = M/16",
             "blocks": sum(r ** 3 for _, r in BATCH_HOLES), "points": points, "steps": steps,
-            "batched_graph": batched, "sequential_graphs": sequential, "per_block": per_block,
+            "batched_graph": batched, "sequential_graphs": sequential, "plain_launches": plain,
+            "batched_over_plain": batched / plain, "per_block": per_block,
             "batched_over_per_block": batched / per_block}
 
 
@@ -311,6 +319,12 @@ def forest_bench(tgh, torch, steps):
     tgh._check(L.tgh_rk4(p.ptr, ptr, dt, dt, steps, 1))
     torch.cuda.synchronize()
     graph = points * steps / (time.perf_counter() - t0)
+    # the plain form: the same launches without the graph (one launch per
+    # kernel per stage over every block, as map_blocks! does)
+    t0 = time.perf_counter()
+    tgh._check(L.tgh_rk4(p.ptr, ptr, 0.0, dt, steps, 0))
+    torch.cuda.synchronize()
+    plain = points * steps / (time.perf_counter() - t0)
     tgh._check(L.tgh_profile(p.ptr, 1))
     tgh._check(L.tgh_rk4(p.ptr, ptr, 0.0, dt, 1, 0))
     prof = (C.c_double * 6)()
@@ -348,7 +362,8 @@ def forest_bench(tgh, torch, steps):
                         "hole_forest(N=16, roots=8 over [-5M,5M]^3, radii=(3M, 1.5M)): three levels, finest "
                         "h = 5M/256, prolongation p = 6, block layout",
             "leaves": forest.nleaves(), "leaves_per_level": levels, "points": points, "steps": steps,
-            "graph": graph, "per_block": per_block, "graph_over_per_block": graph / per_block,
+            "graph": graph, "plain_launches": plain, "graph_over_plain": graph / plain,
+            "per_block": per_block, "graph_over_per_block": graph / per_block,
             "rhs_ms": rhs_ms + pi_ms, "fill_ms": fill_ms,
             "fill_fraction": fill_ms / (fill_ms + rhs_ms + pi_ms),
             "regrid_ms": regrid_ms, "regrid_transfer_ms": transfer_ms, "regrid_leaves": regrid_leaves}
@@ -395,6 +410,11 @@ def octree_batch_bench(tgh, torch, steps):
     sequential = points * steps / (time.perf_counter() - t0)
     t0 = time.perf_counter()
     for p, u in zip(probs, us):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, steps, flags & ~1))
+    torch.cuda.synchronize()
+    plain = points * steps / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for p, u in zip(probs, us):
         tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, 1, tgh.TGH_RK4_PER_BLOCK))
     torch.cuda.synchronize()
     per_block = points / (time.perf_counter() - t0)
@@ -404,7 +424,8 @@ def octree_batch_bench(tgh, torch, steps):
                         "four static Kerr-Schild holes a = 0, 0.3, 0.5, 0.7 on 7^3 .. 13^3 blocks, one batch on "
                         "the block layout",
             "blocks": blocks, "points": points, "steps": steps, "batched_graph": batched,
-            "sequential_graphs": sequential, "per_block": per_block, "batched_over_per_block": batched / per_block}
+            "sequential_graphs": sequential, "plain_launches": plain, "batched_over_plain": batched / plain,
+            "per_block": per_block, "batched_over_per_block": batched / per_block}
 
 
 def main():
@@ -457,6 +478,14 @@ def main():
     value = points * args.steps / el
     note(f"timed {args.steps} steps: {el / args.steps * 1e3:.2f} ms per step")
     assert bool(torch.isfinite(u).all()), "the state is not finite after the timed steps"
+
+    # The same launches without the graph (one launch per kernel per stage
+    # over the whole brick, as map_blocks! launches), wall-clock timed.
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tgh._check(L.tgh_rk4(ptr, uptr, (args.warmup + args.steps) * dt, dt, args.steps, form))
+    torch.cuda.synchronize()
+    nograph_value = points * args.steps / (time.perf_counter() - t1)
 
     # Per-kernel durations from hipEvents on the problem's stream, over a
     # second pass of the same steps launched without the graph.
@@ -550,7 +579,9 @@ def main():
                      "step_roofline_frac": t_roof / (step_ms * 1e-3 / points),
                      "pi_tiles_ms": tiles_ms,
                      "pi_tile": (min(range(len(tiles_ms)), key=lambda i: tiles_ms[i]) if tiles_ms else 0)},
-        "forms": {f"{args.form}_graph": value, "plain_verbs": plain_value, "per_block": per_block_value,
+        "forms": {f"{args.form}_graph": value, "launches_without_graph": nograph_value,
+                  "graph_over_launches": value / nograph_value, "plain_verbs": plain_value,
+                  "per_block": per_block_value,
                   "graph_over_per_block": value / per_block_value if per_block_value else None},
     }
     # the two constraint monitors on the same state (each call: scatter, fill,

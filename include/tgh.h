@@ -143,7 +143,9 @@ typedef struct tgh_desc {
    * number of times in ab), the wall plane itself is owned and evolved, and
    * at every fill its odd variables are set to zero; the Dirichlet hook
    * fills only the faces that neither reflect nor wrap. Not with periodic on
-   * the same axis; uniform forests only. Zero-initialised: no mirror. */
+   * the same axis, nor with half-open ownership (the wall plane must be
+   * owned); on uniform forests and on forests with levels, not in batches,
+   * the adjoint or tgh_transfer. Zero-initialised: no mirror. */
   int32_t reflecting[3][2];
   /* The rotating seam (GHCase's rotating = (1, 2), src/boundaries.jl:145-205):
    * nonzero glues the low x face to the low y face by a quarter turn about

@@ -240,6 +240,8 @@ struct Mesh {
     long b = -1;
     long loc[3] = {0, 0, 0};  // COPY: the point; PROLONG: the stencil's first point
     int half[3] = {0, 0, 0};  // PROLONG: the axes where the point is a coarse midpoint
+    int mirror = 0;           // the axes whose reflecting face the point lies beyond
+    int turn = 0;             // quarter turns across the rotating seam
   };
   // The leaf of level lv holding integer point gv (index in that level's
   // brick) as a vertex: per axis the block where the local index is below N
@@ -295,15 +297,57 @@ struct Mesh {
   int forest_classify(long b, const long l[3], Src& s) const {
     const int L = level[b];
     long gl[3];
+    for (int d = 0; d < 3; ++d) gl[d] = blocks[b][d] * (long)N + l[d];
+    // across the rotating seam and the reflecting faces: the point inside the
+    // box whose value this one takes, turned and mirrored (as the uniform fill)
+    int tn = 0, mm = 0;
+    if (rotating) {
+      const long x = gl[0], y = gl[1];
+      if (x < 0 && y < 0) {
+        gl[0] = -x;
+        gl[1] = -y;
+        tn = 2;
+      } else if (x < 0) {
+        gl[0] = y;
+        gl[1] = -x;
+        tn = 1;
+      } else if (y < 0) {
+        gl[0] = -y;
+        gl[1] = x;
+        tn = 3;
+      }
+    }
     for (int d = 0; d < 3; ++d) {
       const long tot = ((long)nb[d] << L) * N;
-      gl[d] = blocks[b][d] * (long)N + l[d];
-      if (periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
-      else if (gl[d] < 0 || gl[d] > tot || (half_open && gl[d] == tot)) return F_DIRICHLET;
+      if (periodic[d]) {
+        gl[d] = ((gl[d] % tot) + tot) % tot;
+      } else if (gl[d] < 0) {
+        if (!reflect[d][0]) return F_DIRICHLET;
+        gl[d] = -gl[d];
+        mm |= 1 << d;
+      } else if (gl[d] > tot) {
+        if (!reflect[d][1]) return F_DIRICHLET;
+        gl[d] = 2 * tot - gl[d];
+        mm |= 1 << d;
+      } else if (half_open && gl[d] == tot) {
+        return F_DIRICHLET;
+      }
     }
     const int c = classify_point(L, gl, s);
-    if (c == F_COPY && s.b == b && s.loc[0] == l[0] && s.loc[1] == l[1] && s.loc[2] == l[2]) return F_PRIMARY;
+    s.mirror = mm;
+    s.turn = tn;
+    if (c == F_COPY && mm == 0 && tn == 0 && s.b == b && s.loc[0] == l[0] && s.loc[1] == l[1] && s.loc[2] == l[2])
+      return F_PRIMARY;
     return c;
+  }
+  // The axes whose reflecting wall plane holds owned point l of leaf b.
+  int forest_wall_mask(long b, const long l[3]) const {
+    int m = 0;
+    for (int d = 0; d < 3; ++d) {
+      const long g = blocks[b][d] * (long)N + l[d], tot = ((long)nb[d] << level[b]) * N;
+      if ((reflect[d][0] && g == 0) || (reflect[d][1] && g == tot)) m |= 1 << d;
+    }
+    return m;
   }
   // Where the point gl (index in the brick of level L, wrapped) takes its
   // value from in this forest: F_COPY with its owner, or F_PROLONG from the
@@ -516,6 +560,29 @@ struct Problem {
   // coarse leaf included (the injection of the finer leaf's values).
   void fill_forest(double t) {
     const Mesh& m = mesh;
+    // the reflecting walls' odd variables first, so that every copy and
+    // prolongation reads the projected values (the uniform fill's order)
+    if (m.reflects())
+      for (long b = 0; b < m.nblocks(); ++b)
+        for (long k = 0; k <= m.N; ++k)
+          for (long j = 0; j <= m.N; ++j)
+            for (long i = 0; i <= m.N; ++i) {
+              const long l[3] = {i, j, k};
+              const int wm = m.forest_wall_mask(b, l);
+              if (!wm) continue;
+              for (int v = 0; v < 20; ++v)
+                if (parity_odd_mask(v) & wm) work[m.widx(b, v, i, j, k)] = 0.0;
+            }
+    // the variable a turned and mirrored entry reads, and its sign
+    auto turned = [](const Mesh::Src& s, int v, int& sv) {
+      double sg = parity_sign(v, s.mirror);
+      sv = v;
+      for (int n = 0; n < s.turn; ++n) {
+        sg *= rot_sign(sv);
+        sv = rot_src(sv);
+      }
+      return sg;
+    };
     for (int pass = 0; pass <= m.Lmax; ++pass)
       for (long b = 0; b < m.nblocks(); ++b) {
         if (pass > 0 && m.level[b] != pass) continue;
@@ -532,14 +599,19 @@ struct Problem {
                 case_state(cs, t, x, st);
                 for (int v = 0; v < 20; ++v) work[m.widx(b, v, i, j, k)] = st[v];
               } else if (pass == 0 && c == Mesh::F_COPY) {
-                for (int v = 0; v < 20; ++v)
-                  work[m.widx(b, v, i, j, k)] = work[m.widx(s.b, v, s.loc[0], s.loc[1], s.loc[2])];
+                for (int v = 0; v < 20; ++v) {
+                  int sv;
+                  const double sg = turned(s, v, sv);
+                  work[m.widx(b, v, i, j, k)] = sg * work[m.widx(s.b, sv, s.loc[0], s.loc[1], s.loc[2])];
+                }
               } else if (pass > 0 && c == Mesh::F_PROLONG) {
                 // sum-factorised: x innermost, then y, then z
                 const int n[3] = {s.half[0] ? m.prolong : 1, s.half[1] ? m.prolong : 1,
                                   s.half[2] ? m.prolong : 1};
                 for (int v = 0; v < 20; ++v) {
-                  const double* base = &work[m.widx(s.b, v, s.loc[0], s.loc[1], s.loc[2])];
+                  int sv;
+                  const double sg = turned(s, v, sv);
+                  const double* base = &work[m.widx(s.b, sv, s.loc[0], s.loc[1], s.loc[2])];
                   double sz = 0.0;
                   for (int c2 = 0; c2 < n[2]; ++c2) {
                     double sy = 0.0;
@@ -551,7 +623,7 @@ struct Problem {
                     }
                     sz += (n[2] > 1 ? m.wmid[c2] : 1.0) * sy;
                   }
-                  work[m.widx(b, v, i, j, k)] = sz;
+                  work[m.widx(b, v, i, j, k)] = sg * sz;
                 }
               }
             }

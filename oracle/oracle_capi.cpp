@@ -359,6 +359,7 @@ int orc_forest_classify(void* pp, long b, long i, long j, long k, long* src) {
   src[0] = s.b;
   for (int d = 0; d < 3; ++d) src[1 + d] = s.loc[d];
   src[4] = s.half[0] | (s.half[1] << 1) | (s.half[2] << 2);
+  src[5] = s.mirror | (s.turn << 3);  // the mirrored axes and the quarter turns
   return c;
 }
 void orc_prolong_weights(void* pp, double* out) {

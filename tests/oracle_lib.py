@@ -180,9 +180,9 @@ class OracleProblem:
         return lib().orc_block_level(self.p, b)
 
     def classify(self, b, i, j, k):
-        """(kind, (block, l0, l1, l2, half mask)): kind 0 primary, 1 copy,
-        2 Dirichlet, 3 prolongation."""
-        src = (C.c_long * 5)()
+        """(kind, (block, l0, l1, l2, half mask, mirror | turns << 3)): kind 0
+        primary, 1 copy, 2 Dirichlet, 3 prolongation."""
+        src = (C.c_long * 6)()
         c = lib().orc_forest_classify(self.p, b, i, j, k, src)
         return c, tuple(src)
 

@@ -13,7 +13,9 @@ test/reflection_tests.jl) against the oracle, through the C ABI:
 * the rotating seam (src/boundaries.jl:145-205, test/rotation_tests.jl): the
   fill and RK4 on the rotating octant of the a = 3/10 hole against the
   oracle, the device's octant against its box, the seam planes' noise;
-* the refusals: a wrapped and mirrored axis, a forest, a batch.
+* both on a two-level forest (the plan's mirrored and turned entries): the
+  fill and RK4 against the oracle;
+* the refusals: a wrapped and mirrored axis, half-open blocks, a batch.
 Tolerances as the other parity tests: 1e-12 of rhs_scale or of max(1, |u|)."""
 import ctypes as C
 
@@ -227,9 +229,10 @@ def test_mirrors_are_refused_where_they_are_not_built(tgh):
     assert L.tgh_problem_create(C.byref(tgh.TghDesc.from_buffer_copy(d)), C.byref(out)) == tgh.TGH_EINVAL
     assert "periodic" in L.tgh_last_error().decode()
     d = minkowski_octant()
+    d.ownership = tgh.Forest.OWNERSHIP["half_open"]
     leaves = (C.c_int32 * 32)(*[x for k in range(2) for j in range(2) for i in range(2) for x in (0, i, j, k)])
     rc = L.tgh_problem_create_forest(C.byref(tgh.TghDesc.from_buffer_copy(d)), leaves, 8, 6, C.byref(out))
-    assert rc == tgh.TGH_EUNSUPPORTED
+    assert rc == tgh.TGH_EUNSUPPORTED  # the wall plane must be owned: closed blocks only
     p = tgh_problem_from_desc(tgh, minkowski_octant())
     arr = (C.c_void_p * 1)(p.ptr.value)
     assert L.tgh_batch_create(arr, 1, C.byref(out)) == tgh.TGH_EUNSUPPORTED
@@ -329,3 +332,66 @@ def test_the_rotating_octant_is_the_hole_in_the_box_on_the_device(tgh):
     for arr in (u, g):
         moved = np.abs(arr.reshape(-1, 20, P, P, P) - U0).transpose(1, 0, 2, 3, 4)
         assert float(moved[:, seam].max()) == 0.0 and float(moved[:, ~seam].max()) > 9e-9
+
+
+
+# -- on a forest with levels -----------------------------------------------------
+
+def _forest_case(tgh, kind):
+    """The a = 0 hole on the mirrored octant or the a = 3/10 hole on the
+    rotating octant, [0, 2]^3, on the two-level forest of the oracle's test
+    (the root block at the corner refined), q = 2, the sampled source, p = 4."""
+    from test_oracle_reflect import _octant_leaves
+    a = 0.0 if kind == "reflecting" else 0.3
+    d = TD.kerr_schild_hole_case(N=8, nblocks=2, halfwidth=2.0, a=a, r0=0.35, r1=1.1,
+                                 gauge=TD.TGH_GAUGE_SAMPLED, q=2)
+    d.lo[:] = [0.0] * 3
+    if kind == "reflecting":
+        for dd in range(3):
+            d.reflecting[dd][0] = 1
+    else:
+        d.reflecting[2][0] = 1
+        d.rotating = 1
+    leaves = _octant_leaves()
+    L = tgh._load()
+    arr = (C.c_int32 * (4 * len(leaves)))(*[x for lf in leaves for x in lf])
+    ptr = C.c_void_p()
+    tgh._check(L.tgh_problem_create_forest(C.byref(tgh.TghDesc.from_buffer_copy(d)), arr, len(leaves), 4,
+                                           C.byref(ptr)))
+    import torch
+    tgh._check(L.tgh_set_stream(ptr, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    class Handle:
+        pass
+
+    h = Handle()
+    h.ptr = ptr
+    h.state_length = int(L.tgh_state_length(ptr))
+    h.work_length = int(L.tgh_work_length(ptr))
+    o = oracle_lib.OracleProblem(d, leaves, 4)
+    assert o.n == h.state_length
+    return d, h, o
+
+
+@pytest.mark.parametrize("kind", ["reflecting", "rotating"])
+def test_forest_fill_and_rk4_across_the_walls_match_oracle(tgh, kind):
+    torch = torch_cuda()
+    d, h, o = _forest_case(tgh, kind)
+    L = tgh._load()
+    u = _noisy(o, 0xF0E5, 1e-6)
+    ud = torch.from_numpy(u.copy()).cuda()
+    tgh._check(L.tgh_prepare(h.ptr, C.c_void_p(ud.data_ptr()), 0.0))
+    w = torch.empty(h.work_length, dtype=torch.float64, device="cuda")
+    tgh._check(L.tgh_copy_work(h.ptr, C.c_void_p(w.data_ptr())))
+    o.prepare(u, 0.0)
+    wg, wo = w.cpu().numpy(), o.work()
+    ferr = float((np.abs(wg - wo) / np.maximum(np.abs(wo), 1.0)).max())
+    dt = 0.25 * (2.0 / 32) / o.max_speed(u)
+    for n in range(2):
+        o.rk4_step(u, n * dt, dt, NTHREADS)
+    tgh._check(L.tgh_rk4(h.ptr, C.c_void_p(ud.data_ptr()), 0.0, dt, 2, 0))
+    g = ud.cpu().numpy()
+    L.tgh_problem_destroy(h.ptr)
+    diff = float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max())
+    print(f"{kind} forest: fill {ferr:.3e}, 2 RK4 steps {diff:.3e}")
+    assert ferr <= 1e-13 and diff <= 1e-12

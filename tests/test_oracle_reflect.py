@@ -274,3 +274,114 @@ def test_a_spinning_hole_on_the_rotating_octant_is_the_hole_in_the_box():
     print(f"rotating octant vs box after 8 steps: max|diff| {diff:.3e}, scale {scale:.3e}, moved {moved:.3e}")
     assert moved > 1e-6
     assert diff <= 64 * np.finfo(float).eps * scale
+
+
+# -- mirrors and the seam on forests with levels ---------------------------------
+
+def _forest_problem(desc, leaves, p=4):
+    return oracle_lib.OracleProblem(desc, leaves, p)
+
+
+def _octant_leaves():
+    """Two levels about the corner of a 2^3 root brick: the root block at the
+    origin refined (8 level-1 leaves) and the other seven roots as they are,
+    2:1 balanced (a face, edge and corner of the refined block touch each of
+    the three reflecting walls and the seam)."""
+    leaves = [(0, i, j, k) for k in range(2) for j in range(2) for i in range(2) if (i, j, k) != (0, 0, 0)]
+    leaves += [(1, i, j, k) for k in range(2) for j in range(2) for i in range(2)]
+    return leaves
+
+
+@pytest.mark.parametrize("kind", ["reflecting", "rotating"])
+def test_forest_ghosts_across_the_walls_are_the_exact_state(kind):
+    """The a = 0 (mirrored octant) or a = 3/10 (rotating octant) hole on a
+    two-level forest, one fill of the exact state: every stored point that
+    is a copy -- plain, mirrored, turned, or both -- holds the exact state at
+    its own position to roundoff, every Dirichlet point too, and every
+    prolongation to the order-4 interpolation's error; the classification
+    reports mirrored and turned copies, so the check covers them."""
+    import ctypes as C
+    L = oracle_lib.lib()
+    a = 0.0 if kind == "reflecting" else 0.3
+    d = TD.kerr_schild_hole_case(N=8, nblocks=2, halfwidth=2.0, a=a, r0=0.35, r1=1.1,
+                                 gauge=TD.TGH_GAUGE_SAMPLED, q=2)
+    d.lo[:] = [0.0] * 3
+    if kind == "reflecting":
+        for dd in range(3):
+            d.reflecting[dd][0] = 1
+    else:
+        d.reflecting[2][0] = 1
+        d.rotating = 1
+    leaves = _octant_leaves()
+    o = _forest_problem(d, leaves)
+    o.prepare(o.fill_exact(0.0), 0.0)
+    G, N = d.q // 2 + 1, d.N
+    S = N + 1 + 2 * G
+    Wv = o.work().reshape(-1, 20, S, S, S)
+    h0 = 2.0 / (2 * N)
+    coords = o.block_coords()
+    worst_copy, worst_prol, scale = 0.0, 0.0, 0.0
+    seen = {"mirrored": 0, "turned": 0}
+    e = np.zeros(20)
+    for b in range(len(coords)):
+        lv = o.block_level(b)
+        h = h0 / 2 ** lv
+        for k in range(-G, N + G + 1):
+            for j in range(-G, N + G + 1):
+                for i in range(-G, N + G + 1):
+                    x = np.array([(coords[b][0] * N + i) * h, (coords[b][1] * N + j) * h, (coords[b][2] * N + k) * h])
+                    if x @ x < 0.5 ** 2:
+                        continue
+                    c, src = o.classify(b, i, j, k)
+                    L.orc_case_state(C.byref(d), 0.0, x, e)
+                    err = float(np.abs(Wv[b, :, k + G, j + G, i + G] - e).max())
+                    scale = max(scale, float(np.abs(e).max()))
+                    if c == 3:
+                        worst_prol = max(worst_prol, err)
+                    else:
+                        worst_copy = max(worst_copy, err)
+                    if src[5] & 7:
+                        seen["mirrored"] += 1
+                    if src[5] >> 3:
+                        seen["turned"] += 1
+    print(f"{kind}: copies {worst_copy / scale:.3e}, prolongations {worst_prol / scale:.3e} of {scale:.3e}; {seen}")
+    assert seen["mirrored"] > 0 and (kind == "reflecting" or seen["turned"] > 0)
+    assert worst_copy <= 256 * np.finfo(float).eps * scale
+    assert worst_prol <= 1e-2 * scale
+
+
+def test_on_a_refined_octant_the_odd_components_stay_zero_on_the_walls():
+    """test/reflection_tests.jl:190-231: flat space with noise on a two-level
+    octant, 20 steps: an odd variable that starts zero on its wall stays zero
+    there to roundoff (the mirrored ghosts give u(-h) = -u(h) and centred
+    stencils respect it), while the data themselves move."""
+    L = oracle_lib.lib()
+    d = TD.make_desc(q=4, N=8, nblocks=(2,) * 3, periodic=(0, 0, 0), lo=(0.0,) * 3, hi=(4.0,) * 3,
+                     background=TD.TGH_BG_MINKOWSKI, gamma0=1.0, gamma2=0.0, eps_ko=0.5,
+                     reflecting=((1, 0), (1, 0), (1, 0)))
+    o = _forest_problem(d, _octant_leaves(), 6)
+    u = o.fill_exact(0.0)
+    o.add_noise(u, 3, 1e-8)
+    u0 = u.copy()
+    dt = (0.25 / 2) / (4 * np.sqrt(3.0))
+    for n in range(20):
+        o.rk4_step(u, n * dt, dt, NTHREADS)
+    P = d.N + 1
+    U = u.reshape(-1, 20, P, P, P)
+    coords = o.block_coords()
+    odd = _odd(L)
+    wall = 0.0
+    for b, bc in enumerate(coords):
+        for dd in range(3):
+            if bc[dd] != 0:
+                continue
+            sl = [slice(None)] * 3
+            sl[2 - dd] = 0
+            for v in range(20):
+                if odd[v, dd]:
+                    wall = max(wall, float(np.abs(U[b, v][tuple(sl)]).max()))
+    scale = float(np.abs(u).max())
+    print(f"refined octant after 20 steps: odd on the walls {wall:.3e}, scale {scale:.3e}, "
+          f"moved {float(np.abs(u - u0).max()):.3e}")
+    assert float(np.abs(u - u0).max()) > 1e-9
+    assert wall <= 64 * np.finfo(float).eps * scale

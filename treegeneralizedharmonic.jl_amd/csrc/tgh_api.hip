@@ -248,6 +248,10 @@ void launch_transfer(dim3 grid, hipStream_t st, const TransferArgs& a) {
 }
 void launch_fill(tgh_problem* p, double* W, double t) {
   if (p->forest) {
+    if (reflects(p->geom)) {
+      WallArgs wa{p->geom, W};
+      hipLaunchKernelGGL(k_wall_work, grid_points(p), dim3(256), 0, p->stream, wa);
+    }
     // the plan's passes, in order on the stream
     PlanArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -627,9 +631,6 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
   if (rc) return rc;
   const bool mirrors = d->reflecting[0][0] || d->reflecting[0][1] || d->reflecting[1][0] ||
                        d->reflecting[1][1] || d->reflecting[2][0] || d->reflecting[2][1] || d->rotating;
-  if (leaves && mirrors)
-    return fail(TGH_EUNSUPPORTED, "reflecting faces and the rotating seam are built for uniform bricks "
-                                  "(tgh_problem_create); the planned fill of a forest neither mirrors nor turns");
   if (mirrors && d->ownership == TGH_OWNERSHIP_HALF_OPEN)
     return fail(TGH_EUNSUPPORTED, "reflecting faces are built for closed blocks, whose wall plane is owned");
   if (!leaves && d->ownership == TGH_OWNERSHIP_HALF_OPEN)
@@ -641,7 +642,7 @@ static int create_problem(const tgh_desc* d, const int32_t* leaves, int32_t nlea
     std::string msg;
     try {
       msg = forest_plan(d->N, d->q / 2 + 1, d->nblocks, per, prolong, lev, fbc, plan,
-                        d->ownership == TGH_OWNERSHIP_HALF_OPEN);
+                        d->ownership == TGH_OWNERSHIP_HALF_OPEN, d->reflecting, d->rotating);
     } catch (const std::exception& ex) {  // host threads or memory
       return fail(TGH_EDEVICE, std::string("building the fill plan: ") + ex.what());
     }

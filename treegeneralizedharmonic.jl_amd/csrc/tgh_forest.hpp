@@ -271,9 +271,14 @@ inline std::string check_forest_shape(int N, int G, int np) {
 }
 
 // Builds the plan, or returns an error message.
+// refl[d][0] / [d][1]: the reflecting low / high face of axis d; rot: the
+// rotating seam (the low x and y faces glued by a quarter turn about z). A
+// ghost beyond them takes the value of the point inside it is the image of,
+// turned and mirrored: the entry's mode carries the mirrored axes in bits
+// 4-6 and the number of quarter turns in bits 7-8.
 inline std::string forest_plan(int N, int G, const int nb[3], const int periodic[3], int np,
                                const std::vector<int>& lev, const std::vector<int>& bc, ForestPlan& out,
-                               bool half_open = false) {
+                               bool half_open = false, const int (*refl)[2] = nullptr, int rot = 0) {
   std::string msg = check_forest_shape(N, G, np);
   if (!msg.empty()) return msg;
   ForestIndex ix;
@@ -305,11 +310,45 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
             const int l[3] = {i, j, k};
             long gl[3];
             bool outside = false;
+            int mm = 0, tn = 0;
+            for (int d = 0; d < 3; ++d) gl[d] = (long)bc[3 * b + d] * N + l[d];
+            if (rot) {
+              const long x = gl[0], y = gl[1];
+              if (x < 0 && y < 0) {
+                gl[0] = -x;
+                gl[1] = -y;
+                tn = 2;
+              } else if (x < 0) {
+                gl[0] = y;
+                gl[1] = -x;
+                tn = 1;
+              } else if (y < 0) {
+                gl[0] = -y;
+                gl[1] = x;
+                tn = 3;
+              }
+            }
             for (int d = 0; d < 3; ++d) {
               const long tot = ((long)nb[d] << L) * N;
-              gl[d] = (long)bc[3 * b + d] * N + l[d];
-              if (periodic[d]) gl[d] = ((gl[d] % tot) + tot) % tot;
-              else if (gl[d] < 0 || gl[d] > tot || (half_open && gl[d] == tot)) outside = true;
+              if (periodic[d]) {
+                gl[d] = ((gl[d] % tot) + tot) % tot;
+              } else if (gl[d] < 0) {
+                if (refl && refl[d][0]) {
+                  gl[d] = -gl[d];
+                  mm |= 1 << d;
+                } else {
+                  outside = true;
+                }
+              } else if (gl[d] > tot) {
+                if (refl && refl[d][1]) {
+                  gl[d] = 2 * tot - gl[d];
+                  mm |= 1 << d;
+                } else {
+                  outside = true;
+                }
+              } else if (half_open && gl[d] == tot) {
+                outside = true;
+              }
             }
             if (outside) {
               pass[0].insert(pass[0].end(), {dst, 0, 8, 0});
@@ -324,7 +363,7 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
                                      std::to_string(L - 1);
               return;
             }
-            if (kind == 0 && leaf == b && loc[0] == i && loc[1] == j && loc[2] == k) {
+            if (kind == 0 && mm == 0 && tn == 0 && leaf == b && loc[0] == i && loc[1] == j && loc[2] == k) {
               if (i >= 0 && i <= N && j >= 0 && j <= N && k >= 0 && k <= N)
                 out.prim[(size_t)b * P * P * P + i + P * (j + (size_t)P * k)] = 1;
               continue;
@@ -339,8 +378,9 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
               }
             const int src =
                 (int)((long)leaf * S3 + (loc[0] + G) + (long)S * ((loc[1] + G) + (long)S * (loc[2] + G)));
-            if (kind == 0) pass[0].insert(pass[0].end(), {dst, src, 0, 0});
-            else pass[L].insert(pass[L].end(), {dst, src, half, 0});
+            const int xf = (mm << 4) | (tn << 7);
+            if (kind == 0) pass[0].insert(pass[0].end(), {dst, src, xf, 0});
+            else pass[L].insert(pass[L].end(), {dst, src, half | xf, 0});
           }
     }
   });
@@ -356,9 +396,10 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
   out.pass_off.assign(1, 0);
   size_t pos = 0;
   for (int l = 0; l <= Lmax; ++l) {
+    // bucket: the interpolation axes and the Dirichlet bit (mode & 15, 0..8)
     size_t off[9] = {0};
     for (int c = 0; c < nchunk; ++c)
-      for (size_t e = 0; e < cpass[c][l].size(); e += 4) off[cpass[c][l][e + 2]] += 4;
+      for (size_t e = 0; e < cpass[c][l].size(); e += 4) off[cpass[c][l][e + 2] & 15] += 4;
     size_t run = pos;
     for (int m = 0; m <= 8; ++m) {
       const size_t n = off[m];
@@ -369,7 +410,7 @@ inline std::string forest_plan(int N, int G, const int nb[3], const int periodic
     for (int c = 0; c < nchunk; ++c) {
       const std::vector<int>& v = cpass[c][l];
       for (size_t e = 0; e < v.size(); e += 4) {
-        size_t& o = off[v[e + 2]];
+        size_t& o = off[v[e + 2] & 15];
         for (int q = 0; q < 4; ++q) out.ops[o + q] = v[e + q];
         o += 4;
       }

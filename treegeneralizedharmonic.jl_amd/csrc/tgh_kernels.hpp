@@ -514,9 +514,10 @@ TGH_HD int classify(const Geom& g, int b, const int l[3], int& owner, int loc[3]
 TGH_HD int wall_mask(const Geom& g, int b, const int l[3]) {
   int m = 0;
   const int* B = g.bcoord + 3 * b;
+  const int L = g.blevel ? g.blevel[b] : 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int gi = B[d] * g.N + l[d], tot = g.nb[d] * g.N;
+    const int gi = B[d] * g.N + l[d], tot = (g.nb[d] << L) * g.N;
     if ((g.refl[d][0] && gi == 0) || (g.refl[d][1] && gi == tot)) m |= 1 << d;
   }
   return m;
@@ -1333,7 +1334,10 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
 //   dst = b S^3 + p  (block and work point), src = b' S^3 + p' (the source
 //   point, or the prolongation stencil's first point), mode: bits 0-2 the
 //   axes on which the point is a coarse midpoint (p points with the
-//   midpoint Lagrange weights; none set: a copy), 8 the Dirichlet hook.
+//   midpoint Lagrange weights; none set: a copy), 8 the Dirichlet hook,
+//   bits 4-6 the axes whose reflecting face the point lies beyond and bits
+//   7-8 the quarter turns across the rotating seam (the value is then the
+//   turned and mirrored one of the point inside).
 enum { PLAN_DIRICHLET = 8 };
 struct PlanOp {
   int dst, src, mode, pad;
@@ -1415,8 +1419,16 @@ __global__ __launch_bounds__(256) void k_fill_plan(PlanArgs a) {
   const int b = o.dst / (int)S3, p = o.dst - b * (int)S3;
   double* dst = a.W + (long)b * 20 * S3 + p;
   const int sb = o.src / (int)S3, sp = o.src - sb * (int)S3;
-  const double* f = a.W + (long)sb * 20 * S3 + sp + (long)v * S3;
-  dst[v * S3] = prolong_mode<NP>(o.mode, f, S);
+  const int mm = (o.mode >> 4) & 7, tn = (o.mode >> 7) & 3;
+  int sv = v;
+  double sg = 1.0;
+  if (mm | tn) {
+    rot_power(v, tn, sv, sg);
+    sg *= parity_sign(v, mm);
+  }
+  const double* f = a.W + (long)sb * 20 * S3 + sp + (long)sv * S3;
+  const double val = prolong_mode<NP>(o.mode & 7, f, S);
+  dst[v * S3] = (mm | tn) ? sg * val : val;
 }
 // The regrid's transfer: one thread per (entry, variable) of a transfer
 // plan, the new forest's state point from the old forest's prepared work
@@ -1457,6 +1469,28 @@ __global__ __launch_bounds__(256) void k_fill_plan_dirichlet(PlanArgs a) {
   case_state_any<MOVE ? BGS_ALL : BGS_STATIC>(a.c, a.t, x, s);
 #pragma unroll
   for (int q = 0; q < 20; ++q) dst[q * S3] = s[q];
+}
+
+// The reflecting walls' odd variables in a forest's work array, zeroed at the
+// start of every fill (before the plan's copies and prolongations read them).
+struct WallArgs {
+  Geom g;
+  double* __restrict__ W;
+};
+__global__ __launch_bounds__(256) void k_wall_work(WallArgs a) {
+  const Geom& g = a.g;
+  const int P = g.P, S = g.S, G = g.G;
+  int b, wx;
+  if (!wg_map((g.P * g.P * g.P + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * blockDim.x + threadIdx.x;
+  if (p >= P * P * P) return;
+  const int l[3] = {p % P, (p / P) % P, p / (P * P)};
+  const int wm = wall_mask(g, b, l);
+  if (!wm) return;
+  const long sv = (long)S * S * S;
+  double* w = a.W + (long)b * 20 * sv + (l[0] + G) + (long)S * ((l[1] + G) + (long)S * (l[2] + G));
+  for (int v = 0; v < 20; ++v)
+    if (parity_odd_mask(v) & wm) w[v * sv] = 0.0;
 }
 
 // scatter! / gather! between the state and a work array's owned points.
