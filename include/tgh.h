@@ -213,6 +213,42 @@ int tgh_transfer(tgh_problem* from, const double* u_from, double t, tgh_problem*
  * src/evolution.jl:1256-1272). The library keeps the pointer (the caller owns
  * the array and may refill it between calls); NULL detaches it. */
 int tgh_set_target(tgh_problem* p, const double* target, double t_target, int32_t rate, double trail);
+/* A fitted target as the cache's fill reads it: FitParams (src/fit.jl:492-502)
+ * and InteriorFit's coefficients and time (src/fit.jl:504-539). The fit is
+ *   f_v(x) = sum_lm S_lm(xi) sum_{k=0..cont} C[lm, k, v] |xi|^{2k},
+ *   xi = (x - c(t)) / rbar,  c(t) = center + velocity t,
+ * S_lm the real solid harmonics (shape_series' normalisation, slots l^2 + l + m),
+ * for the twenty fit variables (log alpha, beta^i, gamma_ij - delta_ij, Pi_ab
+ * or, with tilde, (alpha / sqrt(det gamma)) Pi_ab); fit_state reassembles the
+ * packed state from them and projects it into the bounds (src/fit.jl:182-212,
+ * 749-755). */
+typedef struct tgh_fit {
+  int32_t lmax;       /* degree L of the harmonics, 0..32 */
+  int32_t cont;       /* radial order, 0, 1 or 2 */
+  int32_t tilde;      /* nonzero: the momentum slots hold (alpha / sqrt(det gamma)) Pi */
+  int32_t pad;
+  double rbar;        /* the radius the ansatz is scaled by (> 0) */
+  double center[3];   /* HoleCenter: c(t) = center + velocity t */
+  double velocity[3];
+  double alpha_min, alpha_max, lambda_min, lambda_max, beta_max, K_max; /* StateBounds */
+  double t;           /* the time the fit was built at */
+  const double* coeffs; /* DEVICE array of (L+1)^2 x (cont+1) x 20 doubles, column-major
+                           (slot fastest, then k, then the variable) */
+} tgh_fit;
+/* fill_target!(target, origins, spacings, interior, (latest, previous), t;
+ * rate, clear) with fit_target_kernel! (src/fit.jl:1070-1151) on a problem
+ * whose interior is the tracked geometry (tgh_desc.shaped = 1): at every
+ * owned point with |x - c(t)| < r_fill = (r_out - offset) + h (c the
+ * interior's center at t, h the geometry's spacing, FittedInterior's h), the
+ * 40 slots of target ((N+1)^3 x 40 x nblocks, the layout tgh_set_target
+ * reads) take A = F_a + (t - t_a) S and S = (F_a - F_b) / (t_a - t_b), F the
+ * fits' states at t (S = 0 without an older previous fit: previous NULL or
+ * previous->t >= latest->t); with rate != 0 S also carries the latest fit's
+ * translation, the centred difference of F_a over t +- h / (8 |v|). With
+ * clear != 0 every other point is zeroed, else left as it is. target is a
+ * DEVICE array. Synchronous. */
+int tgh_fill_target(tgh_problem* p, double* target, const tgh_fit* latest, const tgh_fit* previous, double t,
+                    double h, int32_t rate, int32_t clear);
 /* gh_integrator(p, u, tspan; matter) — src/stepping.jl:150-154,208-225
  * (MatterRHS) with the in-kernel adapter of src/evolution.jl:1295-1311: the
  * matter fields every right-hand side tgh_rk4 evaluates adds to the source,

@@ -8,6 +8,7 @@
 #include <cmath>
 
 #include "../include/tgh.h"
+#include "gh_oracle_fit.hpp"
 #include "gh_oracle_mesh.hpp"
 
 using namespace orc;
@@ -338,6 +339,52 @@ void orc_set_target(void* pp, const double* tw, double t_f, int rate, double tra
   p->trail = trail;
 }
 double orc_shape_series(const double* shape, int lmax, const double* n) { return shape_series(shape, lmax, n); }
+namespace {
+// tgh_fit (coeffs a HOST pointer here) as the oracle's FitParams.
+FitParams fit_from(const tgh_fit* f) {
+  FitParams p;
+  p.lmax = f->lmax;
+  p.cont = f->cont;
+  p.tilde = f->tilde != 0;
+  p.rbar = f->rbar;
+  for (int d = 0; d < 3; ++d) {
+    p.c0[d] = f->center[d];
+    p.v[d] = f->velocity[d];
+  }
+  p.bounds.on = true;
+  p.bounds.a_min = f->alpha_min;
+  p.bounds.a_max = f->alpha_max;
+  p.bounds.l_min = f->lambda_min;
+  p.bounds.l_max = f->lambda_max;
+  p.bounds.b_max = f->beta_max;
+  p.bounds.K_max = f->K_max;
+  p.coeffs = f->coeffs;
+  p.t = f->t;
+  return p;
+}
+}  // namespace
+// fit_variables_at / state_from_fit / fit_state at one point (src/fit.jl:182-212,
+// 721-755), for the tests.
+void orc_fit_variables_at(const tgh_fit* f, const double* x, double t, double* out) {
+  fit_variables_at(fit_from(f), x, t, out);
+}
+void orc_state_from_fit(const double* v, int tilde, double* h, double* P) { state_from_fit(v, tilde != 0, h, P); }
+void orc_fit_state(const tgh_fit* f, const double* x, double t, double* h, double* P) {
+  fit_state(fit_from(f), x, t, h, P);
+}
+// fill_target! on the problem's owned points (src/fit.jl:1070-1151): r_fill =
+// (r_out - offset) + h about the interior's center; out has 40 P^3 nblocks doubles.
+void orc_fill_target(void* pp, double* out, const tgh_fit* fa, const tgh_fit* fb, double t, double h, int rate,
+                     int clear) {
+  Problem* p = (Problem*)pp;
+  const Mesh& m = p->mesh;
+  const FitParams a = fit_from(fa);
+  FitParams b;
+  if (fb) b = fit_from(fb);
+  const double r_fill = (p->cs.r_out - p->cs.offset) + h;
+  fill_target(a, fb ? &b : nullptr, t, r_fill, h, p->cs.center, p->cs.vel, rate != 0, clear != 0, m.nblocks(),
+              (int)m.P, [&](long bb, int i, int j, int k, double x[3]) { m.position(bb, i, j, k, x); }, out);
+}
 void orc_interior_geo(const tgh_desc* d, double t, const double* x, double* out) {
   Case c = case_from(d);
   IGeo g = interior_geo(c, t, x);

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
 
-from tgh_desc import TghDesc  # noqa: E402  (tests/ is on sys.path under pytest)
+from tgh_desc import TghDesc, TghFit  # noqa: E402  (tests/ is on sys.path under pytest)
 
 _lib = None
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -93,6 +93,10 @@ def lib():
             "orc_is_primary": (C.c_int, [P, Lg, Lg, Lg, Lg]),
             "orc_count_rhs_flops": (D, [P, Lg, Lg, Lg, Lg, C.c_int]),
             "orc_max_threads": (C.c_int, []),
+            "orc_fit_variables_at": (None, [C.POINTER(TghFit), dp, D, dp]),
+            "orc_state_from_fit": (None, [dp, C.c_int, dp, dp]),
+            "orc_fit_state": (None, [C.POINTER(TghFit), dp, D, dp, dp]),
+            "orc_fill_target": (None, [P, dp, C.POINTER(TghFit), C.POINTER(TghFit), D, D, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -147,6 +151,13 @@ class OracleProblem:
         tw = np.ascontiguousarray(tw, dtype=np.float64)
         assert tw.size == 2 * self.n
         lib().orc_set_target(self.p, tw, float(t_f), 1 if rate else 0, float(trail))
+
+    def fill_target(self, out, fa, fb, t, h, rate=False, clear=True):
+        """fill_target! into the host cache out (40 variables per owned point)."""
+        assert out.size == 2 * self.n and out.dtype == np.float64 and out.flags.c_contiguous
+        lib().orc_fill_target(self.p, out, C.byref(fa), C.byref(fb) if fb is not None else None, float(t),
+                              float(h), 1 if rate else 0, 1 if clear else 0)
+        return out
 
     def gauge_constraint(self, u, t):
         """(L2[4], Linf[4]) of the gauge constraint C_a over the evolved points."""
@@ -368,3 +379,21 @@ def rk4_step_jvp(orc, u, udot, t, dt, pd, nthreads=8):
 def error_tangent(orc, u, udot, t):
     """The directional derivative of the masked L2 error along udot."""
     return float(_jvp_sigs().orc_error_tangent(orc.p, np.ascontiguousarray(u), np.ascontiguousarray(udot), float(t)))
+
+
+def fit_variables_at(f, x, t):
+    out = np.zeros(20)
+    lib().orc_fit_variables_at(C.byref(f), np.ascontiguousarray(x, dtype=np.float64), float(t), out)
+    return out
+
+
+def state_from_fit(v, tilde=False):
+    h, P = np.zeros(10), np.zeros(10)
+    lib().orc_state_from_fit(np.ascontiguousarray(v, dtype=np.float64), 1 if tilde else 0, h, P)
+    return h, P
+
+
+def fit_state(f, x, t):
+    h, P = np.zeros(10), np.zeros(10)
+    lib().orc_fit_state(C.byref(f), np.ascontiguousarray(x, dtype=np.float64), float(t), h, P)
+    return h, P

@@ -33,6 +33,39 @@ class TghDesc(C.Structure):
     ]
 
 
+class TghFit(C.Structure):
+    """include/tgh.h's tgh_fit (coeffs: a device pointer for the library, a
+    host pointer for the oracle)."""
+    _fields_ = [
+        ("lmax", C.c_int32), ("cont", C.c_int32), ("tilde", C.c_int32), ("pad", C.c_int32),
+        ("rbar", C.c_double), ("center", C.c_double * 3), ("velocity", C.c_double * 3),
+        ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("lambda_min", C.c_double),
+        ("lambda_max", C.c_double), ("beta_max", C.c_double), ("K_max", C.c_double),
+        ("t", C.c_double), ("coeffs", C.c_void_p),
+    ]
+
+
+WIDE_BOUNDS = (1e-3, 1e3, 1e-4, 1e4, 1e3, 1e6)  # ranges no test state reaches
+
+
+def make_fit(coeffs, *, lmax, cont, rbar=1.0, center=(0.0, 0.0, 0.0), velocity=(0.0, 0.0, 0.0), t=0.0,
+             tilde=False, bounds=WIDE_BOUNDS):
+    """A tgh_fit over a host coefficient array of shape (20, cont+1, (lmax+1)^2)
+    (the reference's ((L+1)^2, cont+1, 20) column-major); the struct keeps
+    the array alive as ._host."""
+    import numpy as np
+    arr = np.ascontiguousarray(coeffs, dtype=np.float64)
+    assert arr.shape == (20, cont + 1, (lmax + 1) ** 2)
+    f = TghFit()
+    f.lmax, f.cont, f.tilde, f.rbar = lmax, cont, 1 if tilde else 0, rbar
+    f.center[:], f.velocity[:] = list(center), list(velocity)
+    (f.alpha_min, f.alpha_max, f.lambda_min, f.lambda_max, f.beta_max, f.K_max) = bounds
+    f.t = t
+    f.coeffs = arr.ctypes.data
+    f._host = arr
+    return f
+
+
 def make_desc(*, q=4, N=16, nblocks=(1, 1, 1), periodic=(1, 1, 1), lo=(0.0, 0.0, 0.0),
               hi=(1.0, 1.0, 1.0), background=TGH_BG_MINKOWSKI, gauge=TGH_GAUGE_NONE,
               damping=TGH_DAMPING_CONSTANT, gamma0=0.0, gamma0_near=1.0, gamma0_far=0.1,

@@ -36,7 +36,7 @@ __all__ = [
     "set_matter", "set_rho_max", "vjp_rhs", "adjoint", "ADJOINT_PARAMS", "evolve", "chunk_count",
     "chunk_bounds", "FittedInterior", "StateBounds", "HorizonDissipation", "horizon_dissipation",
     "horizon_min_radius", "horizon_max_radius", "default_bounds", "default_gate", "with_interior",
-    "record_norms", "minkowski_octant_case",
+    "record_norms", "minkowski_octant_case", "FitParams", "InteriorFit", "target_cache", "fill_target",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -93,6 +93,17 @@ class TghMatter(C.Structure):
     _fields_ = [("rho", C.c_void_p), ("S", C.c_void_p), ("Sij", C.c_void_p)]
 
 
+class TghFit(C.Structure):
+    """include/tgh.h's tgh_fit: a fitted target as the cache's fill reads it."""
+    _fields_ = [
+        ("lmax", C.c_int32), ("cont", C.c_int32), ("tilde", C.c_int32), ("pad", C.c_int32),
+        ("rbar", C.c_double), ("center", C.c_double * 3), ("velocity", C.c_double * 3),
+        ("alpha_min", C.c_double), ("alpha_max", C.c_double), ("lambda_min", C.c_double),
+        ("lambda_max", C.c_double), ("beta_max", C.c_double), ("K_max", C.c_double),
+        ("t", C.c_double), ("coeffs", C.c_void_p),
+    ]
+
+
 _lib = None
 
 
@@ -113,6 +124,8 @@ def _load():
                                                 C.POINTER(P)]),
         "tgh_block_levels": (C.c_int, [P, C.POINTER(C.c_int32)]),
         "tgh_set_target": (C.c_int, [P, C.c_void_p, C.c_double, C.c_int32, C.c_double]),
+        "tgh_fill_target": (C.c_int, [P, C.c_void_p, C.POINTER(TghFit), C.POINTER(TghFit), C.c_double,
+                                      C.c_double, C.c_int32, C.c_int32]),
         "tgh_autotune": (C.c_int, [P, C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double)]),
         "tgh_set_tile": (C.c_int, [P, C.c_int32]),
         "tgh_gauge_constraint": (C.c_int, [P, C.c_void_p, C.c_double, C.POINTER(C.c_double),
@@ -400,7 +413,7 @@ class FittedInterior:
     VARIANTS = {"damped": 1, "frozen": 2, "fitted": 4}
 
     def __init__(self, T=None, *, center, shape, lmax=None, offset, thickness, rho_max=0.0,
-                 variant="damped", w_ramp=0.5, rho_ramp=0.5, r_in=None, r_out=None, margin=8):
+                 variant="damped", w_ramp=0.5, rho_ramp=0.5, r_in=None, r_out=None, margin=8, h=0.0):
         _check_float(T)
         if variant not in self.VARIANTS:
             raise ArgumentError(f"unknown variant :{variant} for a tracked geometry (built here: "
@@ -435,6 +448,9 @@ class FittedInterior:
         self.rho_max, self.variant = float(rho_max), variant
         self.w_ramp, self.rho_ramp = float(w_ramp), float(rho_ramp)
         self.margin = int(margin)
+        if not (math.isfinite(h) and h >= 0):
+            raise ArgumentError(f"FittedInterior: the spacing h = {h} the geometry was built at must be ≥ 0")
+        self.h = float(h)  # read by fill_target only (its fill radius and translation step)
         # the bounding radii of the layer, for the checks that read r_0 and r_1
         self.r_0 = (self.r_in - self.offset) - self.thickness
         self.r_1 = self.r_out - self.offset
@@ -1266,6 +1282,106 @@ def adm_constraint_norms(p, u, t):
     l2, li = (C.c_double * 4)(), (C.c_double * 4)()
     _check(_load().tgh_adm_constraint(p.ptr, _dev(u, p, "u"), float(t), l2, li))
     return list(l2), list(li)
+
+
+class FitParams:
+    """FitParams{T}(lmax, cont, rbar, center, bounds[, tilde]) (src/fit.jl:492-502):
+    what an evaluation of a fit needs besides its coefficients -- the degree,
+    the radial order, the radius the ansatz is scaled by, the center (a point,
+    or (c0, v) for c(t) = c0 + v t), the StateBounds its states are projected
+    into (the gate is not used) and whether the momentum slots hold
+    (α/√γ) Π."""
+
+    def __init__(self, T=None, *, lmax, cont, rbar, center, bounds, tilde=False):
+        _check_float(T)
+        if int(lmax) != lmax or not 0 <= lmax <= 32:
+            raise ArgumentError(f"FitParams: lmax = {lmax}; this library evaluates degrees 0 to 32")
+        if cont not in (0, 1, 2):
+            raise ArgumentError(f"FitParams: cont = {cont}; the radial order is 0, 1 or 2")
+        if not (math.isfinite(rbar) and rbar > 0):
+            raise ArgumentError(f"FitParams: r̄ = {rbar} has to be a positive length")
+        if not isinstance(bounds, StateBounds):
+            raise ArgumentError("FitParams: bounds must be a StateBounds (fit_state always projects)")
+        c = list(center)
+        if len(c) == 2 and all(hasattr(x, "__len__") for x in c):
+            c0, v = (tuple(float(x) for x in c[0]), tuple(float(x) for x in c[1]))
+        else:
+            c0, v = tuple(float(x) for x in c), (0.0, 0.0, 0.0)
+        if len(c0) != 3 or len(v) != 3:
+            raise ArgumentError("FitParams: the center is a point or (c0, v), three coordinates each")
+        self.lmax, self.cont, self.rbar, self.tilde = int(lmax), int(cont), float(rbar), bool(tilde)
+        self.c0, self.v, self.bounds = c0, v, bounds
+
+    @property
+    def nslots(self):
+        return (self.lmax + 1) ** 2
+
+
+class InteriorFit:
+    """The part of InteriorFit (src/fit.jl:504-539) the cache's fill reads:
+    the params, the coefficients C[slot, k, v] as a float64 CUDA tensor of
+    shape (20, cont + 1, (L+1)^2) (the reference's ((L+1)^2, cont+1, 20)
+    column-major array, so the same memory) and the time t it was built at.
+    Building a fit (the least squares of build_fit) is the caller's."""
+
+    def __init__(self, params, coeffs, t):
+        import torch
+        if not isinstance(params, FitParams):
+            raise ArgumentError("InteriorFit: params must be a FitParams")
+        shape = (20, params.cont + 1, params.nslots)
+        if (not isinstance(coeffs, torch.Tensor) or coeffs.dtype != torch.float64 or not coeffs.is_cuda
+                or tuple(coeffs.shape) != shape or not coeffs.is_contiguous()):
+            raise ArgumentError(f"InteriorFit: the coefficients must be a contiguous float64 CUDA tensor of "
+                                f"shape {shape}")
+        if not math.isfinite(t):
+            raise ArgumentError(f"InteriorFit: t = {t} is not finite")
+        self.params, self.coeffs, self.t = params, coeffs, float(t)
+
+    def _struct(self):
+        p, b = self.params, self.params.bounds
+        f = TghFit()
+        f.lmax, f.cont, f.tilde, f.rbar = p.lmax, p.cont, 1 if p.tilde else 0, p.rbar
+        f.center[:], f.velocity[:] = list(p.c0), list(p.v)
+        f.alpha_min, f.alpha_max, f.lambda_min, f.lambda_max = b.alpha_min, b.alpha_max, b.lambda_min, b.lambda_max
+        f.beta_max, f.K_max, f.t = b.beta_max, b.K_max, self.t
+        f.coeffs = self.coeffs.data_ptr()
+        return f
+
+
+def target_cache(p):
+    """target_cache(U) (src/fit.jl:1118-1127): a zeroed cache of 40 variables per
+    owned point (A_h, A_Π, S_h, S_Π) on p's forest, as a float64 CUDA tensor."""
+    import torch
+    return torch.zeros(2 * p.state_length, dtype=torch.float64, device=p.backend.device)
+
+
+def fill_target(target, p, fits, t, *, rate=False, clear=True):
+    """fill_target!(target, origins, spacings, interior, fits, t; rate, clear)
+    (src/fit.jl:1129-1151, fit_target_kernel! :1057-1116) on p's mesh and its
+    tracked geometry (a FittedInterior): fits = (latest, previous), previous
+    None before there are two. At every owned point with |x − c(t)| < r_fill =
+    (r_out − offset) + h the cache takes A = F_a + (t − t_a) S and S = (F_a −
+    F_b)/(t_a − t_b) (0 without an older previous fit), F the fits' states at
+    t; with rate, S also carries the latest fit's translation at the point.
+    With clear the other points are zeroed. Returns target."""
+    it = p.case.interior
+    if not isinstance(it, FittedInterior):
+        raise ArgumentError("fill_target fills the tracked geometry's cache: the problem's interior must be "
+                            "a FittedInterior")
+    fa, fb = fits
+    if not isinstance(fa, InteriorFit) or (fb is not None and not isinstance(fb, InteriorFit)):
+        raise ArgumentError("fill_target: fits = (latest, previous) of InteriorFit, previous None before "
+                            "there are two")
+    _dev(target, p, "the target cache")
+    if target.numel() != 2 * p.state_length:
+        raise ArgumentError(f"the target cache has {target.numel()} values, not {2 * p.state_length} (40 per "
+                            f"owned point)")
+    sa = fa._struct()
+    sb = fb._struct() if fb is not None else None
+    _check(_load().tgh_fill_target(p.ptr, C.c_void_p(target.data_ptr()), C.byref(sa),
+                                   C.byref(sb) if sb is not None else None, float(t), float(it.h),
+                                   1 if rate else 0, 1 if clear else 0))
+    return target
 
 
 def record_norms(p, u, t, *, adm=True, shell=None):

@@ -16,6 +16,7 @@
 #include "tgh_forest.hpp"
 #include "tgh_problem.hpp"
 #include "tgh_adjoint.hpp"
+#include "tgh_fit.hpp"
 
 using namespace tgh;
 
@@ -859,6 +860,77 @@ int tgh_set_target(tgh_problem* p, const double* target, double t_target, int32_
     p->graph_exec = nullptr;
   }
   return TGH_OK;
+}
+
+namespace {
+// A tgh_fit checked and turned into the kernel's view, or an error message.
+std::string fit_dev(const tgh_fit* f, const char* which, FitDev& o) {
+  const std::string w(which);
+  if (!f->coeffs) return w + " fit has no coefficients";
+  if (f->lmax < 0 || f->lmax > 32) return w + " fit's degree lmax = " + std::to_string(f->lmax) + " is outside 0..32";
+  if (f->cont < 0 || f->cont > 2)
+    return w + " fit's radial order cont = " + std::to_string(f->cont) + " is not 0, 1 or 2";
+  const double nums[] = {f->rbar,      f->center[0],   f->center[1],   f->center[2], f->velocity[0],
+                         f->velocity[1], f->velocity[2], f->alpha_min, f->alpha_max,   f->lambda_min,
+                         f->lambda_max, f->beta_max,    f->K_max,      f->t};
+  for (double x : nums)
+    if (!std::isfinite(x)) return w + " fit holds a non-finite number";
+  if (!(f->rbar > 0)) return w + " fit's scale radius rbar must be positive";
+  if (!(0 < f->alpha_min && f->alpha_min < 1 && 1 < f->alpha_max) ||
+      !(0 < f->lambda_min && f->lambda_min < 1 && 1 < f->lambda_max) || !(f->beta_max > 0) || !(f->K_max > 0))
+    return w + " fit's bounds need 0 < alpha_min < 1 < alpha_max, 0 < lambda_min < 1 < lambda_max and "
+               "positive beta_max and K_max";
+  o.lmax = f->lmax;
+  o.cont = f->cont;
+  o.tilde = f->tilde != 0;
+  o.rbar = f->rbar;
+  for (int d = 0; d < 3; ++d) {
+    o.c0[d] = f->center[d];
+    o.v[d] = f->velocity[d];
+  }
+  o.bd = Bounds{f->alpha_min, f->alpha_max, f->lambda_min, f->lambda_max, f->beta_max, f->K_max, 0.0};
+  o.coeffs = f->coeffs;
+  return "";
+}
+}  // namespace
+
+int tgh_fill_target(tgh_problem* p, double* target, const tgh_fit* latest, const tgh_fit* previous, double t,
+                    double h, int32_t rate, int32_t clear) {
+  if (!p || !target || !latest) return fail(TGH_EINVAL, "NULL problem, target or latest fit");
+  if (!p->desc.shaped)
+    return fail(TGH_EINVAL, "fill_target reads the tracked geometry (FittedInterior): this problem's interior "
+                            "is not shaped");
+  if (!std::isfinite(t) || !(h >= 0) || !std::isfinite(h))
+    return fail(TGH_EINVAL, "the fill time must be finite and the geometry's spacing h finite and >= 0");
+  FitTargetArgs a;
+  std::memset(&a, 0, sizeof(a));
+  std::string msg = fit_dev(latest, "the latest", a.fa);
+  if (msg.empty() && previous) msg = fit_dev(previous, "the previous", a.fb);
+  if (!msg.empty()) return fail(TGH_EINVAL, msg);
+  // fill_target! (src/fit.jl:1131-1151): the previous fit counts when it is older
+  a.hasb = previous && latest->t > previous->t;
+  if (!a.hasb) a.fb = a.fa;
+  a.kappa = a.hasb ? 1.0 / (latest->t - previous->t) : 0.0;
+  const double* v = latest->velocity;
+  const double speed = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (rate && speed > 0 && !(h > 0))
+    return fail(TGH_EINVAL, "a moving fit's rate is a difference over h / (8 |v|): it needs the geometry's "
+                            "spacing h > 0");
+  a.dt = speed > 0 ? h / (8.0 * speed) : 1.0;
+  a.g = p->geom;
+  for (int d = 0; d < 3; ++d) {
+    a.ic[d] = p->desc.center[d];
+    a.iv[d] = p->desc.velocity[d];
+  }
+  a.rate = rate != 0;
+  a.clear = clear != 0;
+  a.t_a = latest->t;
+  a.t_f = t;
+  a.r_fill = (p->desc.r_out - p->desc.offset) + h;
+  a.out = target;
+  hipLaunchKernelGGL(k_fit_target, grid_points(p), dim3(256), 0, p->stream, a);
+  HIP_TRY(hipGetLastError());
+  return sync(p);
 }
 
 int tgh_set_matter(tgh_problem* p, const tgh_matter* m) {
