@@ -237,6 +237,35 @@ def adjoint_bench(tgh, torch):
             "plain_forward_ms": plain, "adjoint_over_forward": total / plain}
 
 
+def moving_bench(tgh, torch, steps):
+    """A time-dependent case where launch costs show: the boosted harmonic hole
+    (translate(boost(Harmonic(M=1, a=0.3), -v), x0), v = 0.3 along x, x0 =
+    0.25, the moving :damped core) on 64 blocks of 16^3 ([-2M, 2M]^3, h =
+    M/16): the step's graph, captured once and replayed with the stage clock
+    set before each replay, against the same launches without a graph."""
+    L = tgh._load()
+    case = tgh.harmonic_kerr_case(M=1.0, a=0.3, v=0.3, x0=0.25, axis=1, halfwidth=2.0, r_0=0.35, r_1=0.9)
+    p = tgh.GHProblem(tgh.gh_forest(None, case, N=16, roots=4), case, q=4)
+    u = p.statevector()
+    tgh.initial_state(u, p, 0.0)
+    dt = tgh.gh_dt(p, u, cfl=0.25)
+    points = 64 * 16 ** 3
+    out = {"workload": "boosted harmonic hole M=1 a=0.3 v=0.3 (moving core) on 64 blocks of 16^3, h=M/16, q=4",
+           "steps": steps}
+    for name, flags in (("graph", 1), ("launches", 0)):
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 0.0, dt, 2, flags))  # warm-up, capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(u.data_ptr()), 2 * dt, dt, steps, flags))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[name] = points * steps / el
+        out[name + "_ms_per_step"] = el / steps * 1e3
+    assert bool(torch.isfinite(u).all()), "the moving hole's state is not finite"
+    out["graph_over_launches"] = out["graph"] / out["launches"]
+    return out
+
+
 # The batch workload: four Kerr-Schild holes a = 0, 0.3, 0.5, 0.7 (closed-form
 # gauge source, damped interior r_0 = 0.71, r_1 = 1.21) on uneven forests of
 # 7^3, 9^3, 11^3 and 13^3 blocks of 16^3 at h = M/16 (4600 blocks, 18.8 M points).
@@ -627,6 +656,9 @@ def main():
         note("the adjoint's cost")
         torch.cuda.empty_cache()
         result["adjoint"] = adjoint_bench(tgh, torch)
+    note("the moving hole")
+    torch.cuda.empty_cache()
+    result["moving"] = moving_bench(tgh, torch, max(1, args.steps))
     if not args.no_forest:
         note("the octree")
         torch.cuda.empty_cache()

@@ -247,7 +247,7 @@ template <int NP>
 void launch_transfer(dim3 grid, hipStream_t st, const TransferArgs& a) {
   hipLaunchKernelGGL((k_transfer<NP>), grid, dim3(256), 0, st, a);
 }
-void launch_fill(tgh_problem* p, double* W, double t) {
+void launch_fill(tgh_problem* p, double* W, double t, double toff = 0.0) {
   if (p->forest) {
     if (reflects(p->geom)) {
       WallArgs wa{p->geom, W};
@@ -260,6 +260,8 @@ void launch_fill(tgh_problem* p, double* W, double t) {
     a.c = p->cs;
     a.t = t;
     a.W = W;
+    a.clock = p->clock_use;
+    a.toff = toff;
     // pass 0's Dirichlet tail on its own kernel (it evaluates the background)
     if (p->plan_dir[1] > p->plan_dir[0]) {
       a.n = p->plan_dir[1] - p->plan_dir[0];
@@ -283,7 +285,7 @@ void launch_fill(tgh_problem* p, double* W, double t) {
     }
     return;
   }
-  FillArgs a{p->geom, p->cs, t, W};
+  FillArgs a{p->geom, p->cs, t, W, p->clock_use, toff};
   hipLaunchKernelGGL(p->cs.moving ? k_fill<true> : k_fill<false>, grid_work(p), dim3(256), 0, p->stream, a);
 }
 void launch_scatter(tgh_problem* p, const double* u, double* W, int to_work) {
@@ -420,10 +422,10 @@ namespace {
 // ---------------------------------------------------------------------------
 // The brick stepper. Its default right-hand side form (measured on MI355X):
 constexpr bool kBrickSplitDefault = true;
-void launch_frame(tgh_problem* p, double* W, double t) {
+void launch_frame(tgh_problem* p, double* W, double t, double toff = 0.0) {
   const Brick& k = p->brick;
   const long E = (long)(k.n[0] + 2 * k.G) * (k.n[1] + 2 * k.G) * (k.n[2] + 2 * k.G);
-  FrameArgs a{k, p->cs, t, W};
+  FrameArgs a{k, p->cs, t, W, p->clock_use, toff};
   hipLaunchKernelGGL(p->cs.moving ? k_brick_frame<true> : k_brick_frame<false>, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, p->stream, a);
 }
 
@@ -483,12 +485,13 @@ int ensure_brick(tgh_problem* p) {
 }
 
 void launch_bounds_brick(tgh_problem* p, double* A, double* B);
-void launch_paste(tgh_problem* p, int mode, double* A, double* B, double t);
+void launch_paste(tgh_problem* p, int mode, double* A, double* B, double t, double toff = 0.0);
 
 // One RK4 step on the brick arrays (u_n in Bus, the stage input in BW[0]).
 void rk4_brick_launches(tgh_problem* p, double t, double dt) {
   const double c_acc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
   const double c_next[4] = {dt / 2, dt / 2, dt, 0.0};
+  const double t_off[4] = {0.0, dt / 2, dt / 2, dt};
   const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
   for (int s = 0; s < 4; ++s) {
     double* Wc = p->BW[s % 2];
@@ -496,10 +499,12 @@ void rk4_brick_launches(tgh_problem* p, double t, double dt) {
     if (!p->frame_static) {
       const bool prof = p->profiling;
       int e0 = prof ? mark(p) : -1;
-      launch_frame(p, Wc, t_stage[s]);
+      launch_frame(p, Wc, t_stage[s], t_off[s]);
       if (prof) p->ev_fill.push_back({e0, mark(p)});
     }
     RhsArgs a = rhs_args(p, Wc, t_stage[s], nullptr);
+    a.clock = p->clock_use;
+    a.toff = t_off[s];
     a.Hsrc = p->BH;
     a.us = p->Bus;
     a.acc = p->Bacc;
@@ -516,7 +521,9 @@ void rk4_brick_launches(tgh_problem* p, double t, double dt) {
     // the stage limiter on the stage value (and on the step's result): the
     // range projection, then the :pasted overwrite
     if (p->has_bounds) launch_bounds_brick(p, s == 3 ? p->Bus : Wn, s == 3 ? Wn : nullptr);
-    if (p->cs.interior == INT_PASTED) launch_paste(p, 1, s == 3 ? p->Bus : Wn, s == 3 ? Wn : nullptr, t_stage[s < 3 ? s + 1 : 3]);
+    if (p->cs.interior == INT_PASTED)
+      launch_paste(p, 1, s == 3 ? p->Bus : Wn, s == 3 ? Wn : nullptr, t_stage[s < 3 ? s + 1 : 3],
+                   t_off[s < 3 ? s + 1 : 3]);
   }
 }
 
@@ -566,12 +573,14 @@ void launch_bounds_brick(tgh_problem* p, double* A, double* B) {
 
 // The :pasted limiter on a stage value at time t (mode 0: a work array's
 // owned points and optionally the state; mode 1: brick arrays).
-void launch_paste(tgh_problem* p, int mode, double* A, double* B, double t) {
+void launch_paste(tgh_problem* p, int mode, double* A, double* B, double t, double toff) {
   PasteArgs a{};
   a.g = p->geom;
   a.k = p->brick;
   a.c = p->cs;
   a.t = t;
+  a.clock = p->clock_use;
+  a.toff = toff;
   a.dst0 = A;
   a.dst1 = B;
   a.mode = mode;
@@ -584,15 +593,18 @@ void launch_paste(tgh_problem* p, int mode, double* A, double* B, double t) {
 void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
   const double c_acc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
   const double c_next[4] = {dt / 2, dt / 2, dt, 0.0};
+  const double t_off[4] = {0.0, dt / 2, dt / 2, dt};
   const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
   for (int s = 0; s < 4; ++s) {
     double* Wc = p->W[s % 2];
     double* Wn = p->W[(s + 1) % 2];
     const bool prof = p->profiling;
     int e0 = prof ? mark(p) : -1;
-    launch_fill(p, Wc, t_stage[s]);
+    launch_fill(p, Wc, t_stage[s], t_off[s]);
     if (prof) p->ev_fill.push_back({e0, mark(p)});
     RhsArgs a = rhs_args(p, Wc, t_stage[s], p->mat.rho ? &p->mat : nullptr);
+    a.clock = p->clock_use;
+    a.toff = t_off[s];
     a.us = u;
     a.acc = p->acc;
     a.Wn = Wn;
@@ -603,7 +615,8 @@ void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
     // the stage limiter on the stage value (and on the step's result): the
     // range projection, then the :pasted overwrite
     if (p->has_bounds) launch_bounds_work(p, Wn, s == 3 ? u : nullptr);
-    if (p->cs.interior == INT_PASTED) launch_paste(p, 0, Wn, s == 3 ? u : nullptr, t_stage[s < 3 ? s + 1 : 3]);
+    if (p->cs.interior == INT_PASTED)
+      launch_paste(p, 0, Wn, s == 3 ? u : nullptr, t_stage[s < 3 ? s + 1 : 3], t_off[s < 3 ? s + 1 : 3]);
   }
 }
 
@@ -1030,6 +1043,7 @@ int tgh_problem_destroy(tgh_problem* p) {
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
+  (void)hipFree(p->d_clock);
   (void)hipFree(p->d_bcoord);
   (void)hipFree(p->d_lookup);
   (void)hipFree(p->d_level);
@@ -1472,16 +1486,15 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     }
   } else {
     // A hipGraph of one step (4 x (fill + fused RHS)), replayed nsteps times.
-    // The stage times enter the kernels as arguments, so a graph is valid for
-    // one (u, dt, t) triple; the time only matters to the Dirichlet hook of a
-    // moving background, so with a static one the graph is replayed across
-    // steps and re-captured only when dt, u or the form change.
-    // (a :fitted target moves with t as well: A + (t - t_f) S)
-    const bool static_bg = static_background(p->cs) && p->cs.interior != INT_FITTED;
+    // The captured kernels read the step's time from the stage clock, which a
+    // one-thread kernel sets before each replay, so one graph serves every
+    // step of every case (moving holes, :fitted targets, the gauge wave
+    // included); it is re-captured only when dt, u or the form change.
+    if (!p->d_clock) HIP_TRY(hipMalloc(&p->d_clock, sizeof(double)));
     for (int n = 0; n < nsteps; ++n) {
       const double t = t0 + n * dt;
-      const bool reuse = p->graph_exec && p->graph_key[0] == dt &&
-                         p->graph_key[1] == (double)(uintptr_t)u && (static_bg || p->graph_key[2] == t);
+      const bool reuse =
+          p->graph_exec && p->graph_key[0] == dt && p->graph_key[1] == (double)(uintptr_t)u;
       if (!reuse) {
         if (p->graph_exec) {
           HIP_TRY(hipGraphExecDestroy(p->graph_exec));
@@ -1500,7 +1513,9 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
         }
         hipError_t ec = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
         if (ec == hipSuccess) {
+          p->clock_use = p->d_clock;
           step(t);
+          p->clock_use = nullptr;
           ec = hipStreamEndCapture(p->stream, &p->graph);
         }
         p->stream = saved;
@@ -1512,8 +1527,8 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
         p->graph_form = form;
         p->graph_key[0] = dt;
         p->graph_key[1] = (double)(uintptr_t)u;
-        p->graph_key[2] = t;
       }
+      hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(1), 0, p->stream, p->d_clock, t);
       HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
     }
   }
@@ -1810,7 +1825,7 @@ int tgh_copy_work(tgh_problem* p, double* dst) {
 struct tgh_batch {
   std::vector<tgh_problem*> probs;
   int q = 4, gauge = 0;
-  bool hole = false, all_static = true;
+  bool hole = false;
   // The block layout (a problem on a forest with levels or with the general
   // interior among them): one fused launch over every block of every problem
   // per stage; the RK4 epilogue writes the states in place, so the tables and
@@ -1827,7 +1842,11 @@ struct tgh_batch {
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  double graph_key[2] = {0, 0};  // (dt, t) of the captured graph
+  double graph_dt = 0;  // the step of the captured graph
+  // The stage clock (the step's time on the device, set before each step):
+  // the step's first launch stamps the stage times into the tables, and the
+  // problems' frames and fills read it, so tables and graph outlive a step.
+  double* d_clock = nullptr;
 };
 
 namespace {
@@ -1933,9 +1952,10 @@ int batch_tables_blocks(tgh_batch* b, double* const* u, double t, double dt) {
 }
 void batch_step_launches_blocks(tgh_batch* b, double t, double dt) {
   const int n = (int)b->probs.size();
+  const double t_off[4] = {0.0, dt / 2, dt / 2, dt};
   const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
   for (int s = 0; s < 4; ++s) {
-    for (tgh_problem* p : b->probs) launch_fill(p, p->W[s % 2], t_stage[s]);
+    for (tgh_problem* p : b->probs) launch_fill(p, p->W[s % 2], t_stage[s], t_off[s]);
     const BatchEntry* tab = b->d_tab + s * n;
     switch (b->q) {
       case 2: launch_batch_blocks<2>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
@@ -1950,15 +1970,21 @@ void batch_step_launches_blocks(tgh_batch* b, double t, double dt) {
   }
 }
 void batch_step_launches(tgh_batch* b, double t, double dt) {
+  // the stage times from the clock, into the tables and the problems' launches
+  const int n = (int)b->probs.size();
+  hipLaunchKernelGGL(k_batch_clock, dim3((4 * n + 255) / 256), dim3(256), 0, b->stream, b->d_tab, n, b->d_clock,
+                     dt / 2, dt);
+  for (tgh_problem* p : b->probs) p->clock_use = b->d_clock;
   if (b->blocks) {
     batch_step_launches_blocks(b, t, dt);
+    for (tgh_problem* p : b->probs) p->clock_use = nullptr;
     return;
   }
-  const int n = (int)b->probs.size();
+  const double t_off[4] = {0.0, dt / 2, dt / 2, dt};
   const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
   for (int s = 0; s < 4; ++s) {
     for (tgh_problem* p : b->probs)
-      if (!p->frame_static) launch_frame(p, p->BW[s % 2], t_stage[s]);
+      if (!p->frame_static) launch_frame(p, p->BW[s % 2], t_stage[s], t_off[s]);
     const BatchEntry* tab = b->d_tab + s * n;
     switch (b->q) {
       case 2: launch_batch<2>(b->stream, tab, n, b->head_total, b->pi_total, b->gauge, b->hole); break;
@@ -1969,6 +1995,7 @@ void batch_step_launches(tgh_batch* b, double t, double dt) {
     for (tgh_problem* p : b->probs)
       if (p->has_bounds) launch_bounds_brick(p, s == 3 ? p->Bus : p->BW[(s + 1) % 2], s == 3 ? p->BW[(s + 1) % 2] : nullptr);
   }
+  for (tgh_problem* p : b->probs) p->clock_use = nullptr;
 }
 void batch_drop_graph(tgh_batch* b) {
   if (b->exec) hipGraphExecDestroy(b->exec);
@@ -2025,11 +2052,7 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
     return rc;
   };
   for (tgh_problem* p : b->probs) {
-    if (blocks) {
-      // static backgrounds and no :fitted target (which moves with t) keep a graph across steps
-      b->all_static = b->all_static && static_background(p->cs) && p->cs.interior != INT_FITTED;
-      continue;
-    }
+    if (blocks) continue;
     int rc = ensure_brick(p);
     if (rc) return bail(rc);
     if (!p->Bcoef) {
@@ -2039,12 +2062,12 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
     }
     rc = sync(p);
     if (rc) return bail(rc);
-    b->all_static = b->all_static && p->frame_static;
   }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(TGH_EDEVICE, "creating the batch's stream"));
   if (hipMalloc(&b->d_tab, sizeof(BatchEntry) * 4 * n) != hipSuccess ||
-      hipHostMalloc(&b->h_tab, sizeof(BatchEntry) * 4 * n) != hipSuccess)
+      hipHostMalloc(&b->h_tab, sizeof(BatchEntry) * 4 * n) != hipSuccess ||
+      hipMalloc(&b->d_clock, sizeof(double)) != hipSuccess)
     return bail(fail(TGH_EDEVICE, "allocating the batch's tables"));
   *out = b;
   return TGH_OK;
@@ -2095,19 +2118,19 @@ int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t 
   }
   for (int s = 0; s < nsteps && rc == TGH_OK; ++s) {
     const double t = t0 + s * dt;
-    // static backgrounds ignore t: the tables (and the graph) are kept across steps
-    const bool stale = !b->tab_valid || b->tab_key[0] != dt || (!b->all_static && b->tab_key[1] != t) ||
-                       (b->blocks && b->tab_u != uv);
+    // the stage times come from the clock: the tables (and the graph) are kept
+    // across steps and rebuilt only when dt, the states or a problem's case change
+    const bool stale = !b->tab_valid || b->tab_key[0] != dt || (b->blocks && b->tab_u != uv);
     if (stale) {
       rc = b->blocks ? batch_tables_blocks(b, u, t, dt) : batch_tables(b, t, dt);
       if (rc) break;
     }
+    hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(1), 0, b->stream, b->d_clock, t);
     if (!graph) {
       batch_step_launches(b, t, dt);
       continue;
     }
-    const bool reuse = b->exec && b->graph_key[0] == dt && (b->all_static || b->graph_key[1] == t) &&
-                       (!b->blocks || b->graph_u == uv);
+    const bool reuse = b->exec && b->graph_dt == dt && (!b->blocks || b->graph_u == uv);
     if (!reuse) {
       batch_drop_graph(b);
       hipError_t ec = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
@@ -2121,8 +2144,7 @@ int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t 
         rc = fail(TGH_EDEVICE, std::string("capturing the batch's RK4 step: ") + hipGetErrorString(ec));
         break;
       }
-      b->graph_key[0] = dt;
-      b->graph_key[1] = t;
+      b->graph_dt = dt;
       b->graph_u = uv;
     }
     if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) {
@@ -2154,6 +2176,7 @@ int tgh_batch_destroy(tgh_batch* b) {
   batch_drop_graph(b);
   if (b->d_tab) hipFree(b->d_tab);
   if (b->h_tab) hipHostFree(b->h_tab);
+  if (b->d_clock) hipFree(b->d_clock);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return TGH_OK;

@@ -625,6 +625,13 @@ __device__ __forceinline__ double st_mixed(const double* __restrict__ p, int s1,
 }
 
 // ---------------------------------------------------------------------------
+// The stage time. A kernel captured in a graph that is replayed step after
+// step reads the step's time t_n from a device scalar (clock) that the host
+// sets before each replay, and adds its stage's offset (toff: 0, dt/2 or dt,
+// the same sum t_n + dt/2 the host forms); without a clock it is the
+// argument t itself.
+TGH_HD double stage_time(double t, const double* clock, double toff) { return clock ? *clock + toff : t; }
+
 struct RhsArgs {
   Geom g;
   CaseP c;
@@ -645,6 +652,8 @@ struct RhsArgs {
   int stage;
   // the split form: per-point coefficients the head leaves for the Pi kernel
   double* __restrict__ coef;
+  const double* clock;  // stage_time's clock (NULL: t as given) and offset
+  double toff;
 };
 
 
@@ -1196,6 +1205,7 @@ __device__ __forceinline__ bool block_point(const RhsArgs& a, const double* __re
 
 template <int Q, int GAUGE, int HOLE, int EPI>
 __global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict__ W, TGH_OUT_PARAMS) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   PointRef pr;
   long pi;
   if (!block_point(a, W, pr, pi)) return;
@@ -1207,6 +1217,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a, const double* __restrict
 template <int Q, int GAUGE, int HOLE, int EPI>
 __global__ __launch_bounds__(256) void k_head(RhsArgs a, const double* __restrict__ W,
                                               double* __restrict__ coef, TGH_OUT_PARAMS) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   PointRef pr;
   long pi;
   if (!block_point(a, W, pr, pi)) return;
@@ -1227,6 +1238,7 @@ __global__ __launch_bounds__(256) void k_head(RhsArgs a, const double* __restric
 template <int Q, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TGH_PI_WAVES_PER_EU))) void k_pi(
     RhsArgs a, const double* __restrict__ W, const double* __restrict__ coef, TGH_OUT_PARAMS) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   PointRef pr;
   long pi;
   if (!block_point(a, W, pr, pi)) return;
@@ -1271,9 +1283,12 @@ struct FillArgs {
   CaseP c;
   double t;
   double* __restrict__ W;
+  const double* clock;  // stage_time's clock and offset
+  double toff;
 };
 template <bool MOVE>
 __global__ __launch_bounds__(256) void k_fill(FillArgs a) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   const Geom& g = a.g;
   const int S = g.S, G = g.G;
   int b, wx;
@@ -1349,6 +1364,8 @@ struct PlanArgs {
   double* __restrict__ W;
   const PlanOp* __restrict__ ops;
   int n;
+  const double* clock;  // stage_time's clock and offset
+  double toff;
 };
 // The midpoint Lagrange weights of NP nodes at -NP/2+1 .. NP/2 (exact
 // dyadic rationals, equal to the host's product formula).
@@ -1455,6 +1472,7 @@ __global__ __launch_bounds__(256) void k_transfer(TransferArgs a) {
 // ghost's own position.
 template <bool MOVE>
 __global__ __launch_bounds__(256) void k_fill_plan_dirichlet(PlanArgs a) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.n) return;
   const PlanOp o = a.ops[e];
@@ -2195,6 +2213,7 @@ __device__ __forceinline__ bool tile_map(const Brick& k, int TX, int TY, int TZ,
 template <int Q, int GAUGE, int HOLE>
 __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const double* __restrict__ W,
                                                    TGH_OUT_PARAMS) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   int x, y, z;
   if (!brick_map(k, x, y, z)) return;
   const long off = brick_index(k, x, y, z);
@@ -2228,6 +2247,7 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
 template <int Q, int GAUGE, int HOLE>
 __global__ __launch_bounds__(256) void k_brick_head(RhsArgs a, Brick k, const double* __restrict__ W,
                                                     double* __restrict__ coef, TGH_OUT_PARAMS) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   int x, y, z;
   if (!brick_map(k, x, y, z)) return;
   brick_head_at<Q, GAUGE, HOLE>(a, k, W, a.Hsrc, coef, TGH_OUT_ARGS, x, y, z);
@@ -2327,6 +2347,7 @@ __device__ __forceinline__ void brick_pi_tile(const RhsArgs& a, const Brick& k, 
 template <int Q, int TC>
 __global__ __launch_bounds__((PiTile<Q, TC>::X * PiTile<Q, TC>::Y * PiTile<Q, TC>::Z)) void k_brick_pi(
     RhsArgs a, Brick k, const double* __restrict__ W, const double* __restrict__ coef, TGH_OUT_PARAMS) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   using T = PiTile<Q, TC>;
   int X0, Y0, Z0;
   if (!tile_map(k, T::X, T::Y, T::Z * T::PZ, X0, Y0, Z0)) return;  // uniform
@@ -2533,6 +2554,21 @@ __global__ __launch_bounds__(256) void k_rhs_batch(const BatchEntry* __restrict_
 }
 
 #ifndef TGH_RHS_ONLY
+// The stage clock's kernels: one thread sets the step's time before a graph
+// replay; k_batch_clock gives a batch's stage tables (4 x n entries, stage
+// major) the stage times clock + (0, dt/2, dt/2, dt), the first node of the
+// batch's captured step.
+__global__ void k_set_clock(double* clock, double t) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *clock = t;
+}
+__global__ __launch_bounds__(256) void k_batch_clock(BatchEntry* tab, int n, const double* clock, double half,
+                                                     double dt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 4 * n) return;
+  const int s = e / n;
+  tab[e].a.t = *clock + (s == 0 ? 0.0 : s == 3 ? dt : half);
+}
+
 // The frame of a brick array at time t, one point of the whole extent per
 // thread (interior points return at once).
 struct FrameArgs {
@@ -2540,9 +2576,12 @@ struct FrameArgs {
   CaseP c;
   double t;
   double* __restrict__ W;
+  const double* clock;  // stage_time's clock and offset
+  double toff;
 };
 template <bool MOVE>
 __global__ __launch_bounds__(256) void k_brick_frame(FrameArgs a) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   const Brick& k = a.k;
   const long E0 = k.n[0] + 2 * k.G, E1 = k.n[1] + 2 * k.G, E2 = k.n[2] + 2 * k.G;
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
@@ -2829,9 +2868,12 @@ struct PasteArgs {
   double* __restrict__ dst0;
   double* __restrict__ dst1;
   int mode;
+  const double* clock;  // stage_time's clock and offset
+  double toff;
 };
 template <bool MOVE>
 __global__ __launch_bounds__(256) void k_paste(PasteArgs a) {
+  a.t = stage_time(a.t, a.clock, a.toff);
   double x[3];
   long i0, s0, i1 = 0, s1 = 0;
   if (a.mode == 1) {

@@ -45,7 +45,12 @@ struct tgh_problem {
   long state_len = 0, work_len = 0;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
-  double graph_key[3] = {0, 0, 0};    // (dt, u pointer, t) the graph was captured for
+  double graph_key[2] = {0, 0};       // (dt, u pointer) the graph was captured for
+  // The stage clock: a device scalar holding the step's time, set before each
+  // replay of the graph, which the captured kernels read (stage_time); clock_use
+  // points at it while launches are recorded for a graph, else NULL.
+  double* d_clock = nullptr;
+  const double* clock_use = nullptr;
   unsigned version = 0;               // bumped when a setter changes the case (batches re-read it)
   tgh_matter mat{nullptr, nullptr, nullptr};  // tgh_set_matter's fields for tgh_rk4 (rho NULL: vacuum)
   int device = 0;                     // the HIP device current at creation, which holds the arrays
