@@ -830,11 +830,15 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
   // The layer's target u_e (r >= r0 here: the core rule is the identity) and,
   // for a moving :damped hole, its rates, which (1 - w) d_t u_e adds so that
   // the moving solution is a steady state of the layer (src/evolution.jl:652-680).
-  // It is evaluated twice in the layer, for d_t h before the source and for
-  // d_t Pi after it, so that no part of it is held through the source.
+  // At rest it is evaluated twice in the layer, for d_t h before the source
+  // and for d_t Pi after it, so that no part of it is held through the
+  // source. A moving background's exact state and rates cost three dual
+  // passes each time: its variants evaluate them once and hold the Pi half
+  // through the source (their register budget is spent on those passes).
   constexpr bool MOVE = (bg_set_for(GAUGE) & BGS_MOVING) != 0;
   const bool follows = MOVE && c.moving && c.interior == INT_DAMPED;
   const double om = 1.0 - wgt;
+  double uP[MOVE ? NC : 1], rP[MOVE ? NC : 1];
   {
     double ue[20], re[20];
     if (layer) {
@@ -842,6 +846,13 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
         background_state_rate(c.bg, a.t, pr.x, c.delta, ue, ue + NC, re, re + NC);
       else
         case_state<bg_set_for(GAUGE)>(c, a.t, pr.x, ue);
+    }
+    if constexpr (MOVE) {
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        uP[v] = ue[NC + v];
+        rP[v] = re[NC + v];
+      }
     }
 #pragma unroll
     for (int v = 0; v < NC; ++v) {
@@ -879,6 +890,16 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
     for (int n = 0; n < NC; ++n) msrc[n] += ms[n];
   }
   pc.w = layer ? wgt : 1.0;
+  if constexpr (MOVE) {
+#pragma unroll
+    for (int v = 0; v < NC; ++v) {
+      double c1 = msrc[v];
+      if (layer)
+        c1 = follows ? wgt * c1 + om * rP[v] - rho * (src.P(v) - uP[v]) : wgt * c1 - rho * (src.P(v) - uP[v]);
+      pc.c1[v] = c1;
+    }
+    return;
+  }
   double ue[20], re[20];
   if (layer) {
     if (follows)
