@@ -425,8 +425,13 @@ constexpr bool kBrickSplitDefault = true;
 void launch_frame(tgh_problem* p, double* W, double t, double toff = 0.0) {
   const Brick& k = p->brick;
   const long E = (long)(k.n[0] + 2 * k.G) * (k.n[1] + 2 * k.G) * (k.n[2] + 2 * k.G);
-  FrameArgs a{k, p->cs, t, W, p->clock_use, toff};
-  hipLaunchKernelGGL(p->cs.moving ? k_brick_frame<true> : k_brick_frame<false>, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, p->stream, a);
+  // without mirrors or the seam only the frame's points have work: one thread
+  // each; with them the walls' inside points are projected too
+  const int shell = reflects(p->geom) ? 0 : 1;
+  const long n = shell ? E - (long)k.n[0] * k.n[1] * k.n[2] : E;
+  FrameArgs a{k, p->cs, t, W, p->clock_use, toff, shell};
+  hipLaunchKernelGGL(p->cs.moving ? k_brick_frame<true> : k_brick_frame<false>, dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, p->stream, a);
 }
 
 // A background that does not depend on t (the gauge wave is the one that does).

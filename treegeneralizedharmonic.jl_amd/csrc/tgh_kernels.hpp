@@ -2599,15 +2599,59 @@ struct FrameArgs {
   double* __restrict__ W;
   const double* clock;  // stage_time's clock and offset
   double toff;
+  int shell;  // 1: one thread per frame point (brick_shell_point); 0: per extent point
 };
+// The frame's points (the brick's extent less its n0 x n1 x n2 inside) in x-fastest
+// order: the G planes below and above, then per inner plane the G rows below
+// and above, then per inner row the G points on each side. Point s of
+// shell_size(k) as extent coordinates (0 .. n + 2G - 1).
+__device__ __forceinline__ long shell_size(const Brick& k) {
+  const long E0 = k.n[0] + 2 * k.G, E1 = k.n[1] + 2 * k.G, E2 = k.n[2] + 2 * k.G;
+  return E0 * E1 * E2 - (long)k.n[0] * k.n[1] * k.n[2];
+}
+__device__ __forceinline__ void brick_shell_point(const Brick& k, long s, int e[3]) {
+  const int G = k.G;
+  const long E0 = k.n[0] + 2 * G, E1 = k.n[1] + 2 * G;
+  const long A = 2L * G * E0 * E1, B = 2L * G * E0 * k.n[2];
+  if (s < A) {
+    const long pl = s / (E0 * E1), r = s - pl * E0 * E1;
+    e[2] = (int)(pl < G ? pl : pl + k.n[2]);
+    e[1] = (int)(r / E0);
+    e[0] = (int)(r - (r / E0) * E0);
+  } else if (s < A + B) {
+    const long s2 = s - A, row = s2 / E0;  // row of the 2G per inner plane
+    const long pz = row / (2 * G), q = row - pz * 2 * G;
+    e[2] = (int)(G + pz);
+    e[1] = (int)(q < G ? q : q + k.n[1]);
+    e[0] = (int)(s2 - row * E0);
+  } else {
+    const long s3 = s - A - B, per = 2L * G * k.n[1];
+    const long pz = s3 / per, r = s3 - pz * per, py = r / (2 * G), q = r - py * 2 * G;
+    e[2] = (int)(G + pz);
+    e[1] = (int)(G + py);
+    e[0] = (int)(q < G ? q : q + k.n[0]);
+  }
+}
 template <bool MOVE>
 __global__ __launch_bounds__(256) void k_brick_frame(FrameArgs a) {
   a.t = stage_time(a.t, a.clock, a.toff);
   const Brick& k = a.k;
   const long E0 = k.n[0] + 2 * k.G, E1 = k.n[1] + 2 * k.G, E2 = k.n[2] + 2 * k.G;
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= E0 * E1 * E2) return;
-  const int gp[3] = {(int)(t % E0) - k.G, (int)((t / E0) % E1) - k.G, (int)(t / (E0 * E1)) - k.G};
+  int gp[3];
+  if (a.shell) {
+    // the frame alone: a wave's lanes are frame points, none idle beside an inner row
+    if (t >= shell_size(k)) return;
+    int e[3];
+    brick_shell_point(k, t, e);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gp[d] = e[d] - k.G;
+  } else {
+    if (t >= E0 * E1 * E2) return;
+    gp[0] = (int)(t % E0) - k.G;
+    gp[1] = (int)((t / E0) % E1) - k.G;
+    gp[2] = (int)(t / (E0 * E1)) - k.G;
+  }
   bool inside = true, dirichlet = false;
   int w[3], mm = 0, wall = 0;
   // beyond the rotating seam the point inside is the image of, tn quarter turns
