@@ -477,6 +477,15 @@ int ensure_brick(tgh_problem* p) {
     launch_frame(p, p->BW[0], 0.0);
     launch_frame(p, p->BW[1], 0.0);
   }
+  if (p->cs.moving && p->cs.interior == INT_DAMPED && !p->cs.general) {
+    // the layer box: the layer's bounding box about c(t) with a point of margin
+    // each side, whatever t (slots wrap modulo B)
+    p->layer_B = 2 * (int)std::ceil(p->cs.r1 * k.inv_h) + 4;
+    const size_t lb = sizeof(double) * 40 * (size_t)p->layer_B * p->layer_B * p->layer_B;
+    hipError_t e = hipMalloc(&p->Blayer, lb);
+    if (e != hipSuccess)
+      return fail(TGH_EDEVICE, std::string("allocating the layer box: ") + hipGetErrorString(e));
+  }
   if (p->cs.gauge == GAUGE_SAMPLED) {
     HIP_TRY(hipMalloc(&p->BH, bytes));
     HIP_TRY(hipMemsetAsync(p->BH, 0, bytes, p->stream));
@@ -510,6 +519,14 @@ void rk4_brick_launches(tgh_problem* p, double t, double dt) {
     RhsArgs a = rhs_args(p, Wc, t_stage[s], nullptr);
     a.clock = p->clock_use;
     a.toff = t_off[s];
+    if (p->Blayer) {
+      const long B3 = (long)p->layer_B * p->layer_B * p->layer_B;
+      LayerArgs la{p->brick, p->cs, t_stage[s], p->clock_use, t_off[s], p->Blayer, p->layer_B};
+      hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256)), dim3(256), 0, p->stream, la);
+      a.lbox = p->Blayer;
+      a.lB3 = B3;
+      a.lB = p->layer_B;
+    }
     a.Hsrc = p->BH;
     a.us = p->Bus;
     a.acc = p->Bacc;
@@ -1049,6 +1066,7 @@ int tgh_problem_destroy(tgh_problem* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
   (void)hipFree(p->d_clock);
+  (void)hipFree(p->Blayer);
   (void)hipFree(p->d_bcoord);
   (void)hipFree(p->d_lookup);
   (void)hipFree(p->d_level);

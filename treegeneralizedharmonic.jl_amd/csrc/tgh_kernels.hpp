@@ -654,7 +654,17 @@ struct RhsArgs {
   double* __restrict__ coef;
   const double* clock;  // stage_time's clock (NULL: t as given) and offset
   double toff;
+  // a moving hole's layer targets and rates on the brick, evaluated per stage by
+  // k_brick_layer (40 fields of lB^3 slots, slot = brick coordinates mod lB);
+  // NULL: the head evaluates them itself
+  const double* lbox;
+  long lB3;
+  int lB;
 };
+// The layer box's slot of brick point (x, y, z).
+TGH_HD long layer_slot(int x, int y, int z, int B) {
+  return (long)(x % B) + (long)B * ((long)(y % B) + (long)B * (long)(z % B));
+}
 
 
 // The epilogue: store F (the rhs! verb) or apply RK4 stage s to it: with k = F,
@@ -732,6 +742,7 @@ struct PointRef {
   const double* __restrict__ m_S;
   const double* __restrict__ m_Sij;
   long ms;
+  long lslot;  // the layer box's slot of the point (read only where RhsArgs::lbox is set)
 };
 
 // What the Pi components need from the head at a point: beta^i, d_i beta^i,
@@ -842,10 +853,18 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
   {
     double ue[20], re[20];
     if (layer) {
-      if (follows)
+      if (follows && a.lbox) {
+        const double* lb = a.lbox + pr.lslot;
+#pragma unroll
+        for (int v = 0; v < 20; ++v) {
+          ue[v] = lb[v * a.lB3];
+          re[v] = lb[(20 + v) * a.lB3];
+        }
+      } else if (follows) {
         background_state_rate(c.bg, a.t, pr.x, c.delta, ue, ue + NC, re, re + NC);
-      else
+      } else {
         case_state<bg_set_for(GAUGE)>(c, a.t, pr.x, ue);
+      }
     }
     if constexpr (MOVE) {
 #pragma unroll
@@ -2255,6 +2274,7 @@ __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const dou
   pr.m_S = nullptr;
   pr.m_Sij = nullptr;
   pr.ms = 0;
+  pr.lslot = a.lbox ? layer_slot(x, y, z, a.lB) : 0;
   rhs_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, k.inv_h, TGH_OUT_ARGS);
 }
 
@@ -2295,6 +2315,7 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
   pr.m_S = nullptr;
   pr.m_Sij = nullptr;
   pr.ms = 0;
+  pr.lslot = a.lbox ? layer_slot(x, y, z, a.lB) : 0;
   const GlobalSrc src{pr.w0, pr.sv, pr.s2, pr.s3};
   double r;
   if (frozen_core<HOLE>(a.c, a.t, pr.x, r)) {
@@ -2601,6 +2622,50 @@ struct FrameArgs {
   double toff;
   int shell;  // 1: one thread per frame point (brick_shell_point); 0: per extent point
 };
+// A moving :damped hole's layer on the brick: per stage, one thread per point
+// of the B^3 box that holds the layer (its corner one point below the layer's
+// bounding box about c(t)), the layer points' exact state and rates
+// (background_state_rate) into slot layer_slot(x, y, z, B) of 40 fields, A
+// wave of the head then reads them instead of running the dual passes on
+// every lane where one of its points lies in the layer.
+struct LayerArgs {
+  Brick k;
+  CaseP c;
+  double t;
+  const double* clock;  // stage_time's clock and offset
+  double toff;
+  double* __restrict__ box;
+  int B;
+};
+__global__ __launch_bounds__(256) void k_brick_layer(LayerArgs a) {
+  a.t = stage_time(a.t, a.clock, a.toff);
+  const Brick& k = a.k;
+  const int B = a.B;
+  const long B3 = (long)B * B * B;
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B3) return;
+  const int l[3] = {(int)(s % B), (int)((s / B) % B), (int)(s / ((long)B * B))};
+  double cc[3];
+  center_at(a.c, a.t, cc);
+  int p[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    p[d] = (int)floor((cc[d] - a.c.r1 - k.lo[d]) * k.inv_h) - 1 + l[d];
+    if (p[d] < 0 || p[d] >= k.n[d]) return;
+  }
+  const double x[3] = {k.lo[0] + (double)p[0] * k.h, k.lo[1] + (double)p[1] * k.h, k.lo[2] + (double)p[2] * k.h};
+  double r;
+  if (frozen_core<1>(a.c, a.t, x, r) || !(r < a.c.r1)) return;  // the head reads layer points only
+  double ue[20], re[20];
+  background_state_rate(a.c.bg, a.t, x, a.c.delta, ue, ue + NC, re, re + NC);
+  double* dst = a.box + layer_slot(p[0], p[1], p[2], B);
+#pragma unroll
+  for (int v = 0; v < 20; ++v) {
+    dst[v * B3] = ue[v];
+    dst[(20 + v) * B3] = re[v];
+  }
+}
+
 // The frame's points (the brick's extent less its n0 x n1 x n2 inside) in x-fastest
 // order: the G planes below and above, then per inner plane the G rows below
 // and above, then per inner row the G points on each side. Point s of

@@ -68,6 +68,8 @@ struct tgh_problem {
   double* Bacc = nullptr;
   double* BH = nullptr;
   double* Bcoef = nullptr;            // the split form's coefficients (CF_N brick fields)
+  double* Blayer = nullptr;           // a moving :damped hole's layer box (40 x layer_B^3)
+  int layer_B = 0;
   bool brick_ready = false;
   bool frame_static = false;          // the frame never changes: filled once
   hipStream_t cap_stream = nullptr;   // capture stream when the problem runs on the null stream
