@@ -42,7 +42,10 @@ def noisy_states(tgh, probs, amplitude=1e-8):
     return us
 
 
-def check_batch_equals_alone(tgh, probs, nsteps, graph):
+def check_batch_equals_alone(tgh, probs, nsteps, graph, roundoff=False):
+    """Every problem of the batch computes the bits it computes alone; with
+    roundoff, to an ulp of the state's scale (a moving hole's head is its own
+    kernel alone and in a batch, and their arithmetic may round apart)."""
     torch = torch_cuda()
     us = noisy_states(tgh, probs)
     ref = [u.clone() for u in us]
@@ -56,13 +59,19 @@ def check_batch_equals_alone(tgh, probs, nsteps, graph):
         d = float((u - r).abs().max())
         print(f"problem {i} ({probs[i].nblocks} blocks): max |batch - alone| = {d:.3e}")
         assert torch.isfinite(u).all()
-        assert torch.equal(u, r), f"problem {i} differs by {d}"
+        if roundoff:
+            assert d <= 4e-16 * float(r.abs().max()), f"problem {i} differs by {d}"
+        else:
+            assert torch.equal(u, r), f"problem {i} differs by {d}"
     # a second call on the same batch (tables and graph reused)
     tgh.gh_solve_batch(batch, us, (0.0, dt), dt=dt, graph=graph)
     for p, r in zip(probs, ref):
         tgh.gh_solve(p, r, (0.0, dt), dt=dt, graph=graph)
     for u, r in zip(us, ref):
-        assert torch.equal(u, r)
+        if roundoff:
+            assert float((u - r).abs().max()) <= 4e-16 * float(r.abs().max())
+        else:
+            assert torch.equal(u, r)
 
 
 @pytest.mark.parametrize("graph", [False, True])
@@ -117,12 +126,15 @@ def test_per_block_form_equals_fused_block_form(tgh):
 
 def test_batch_of_moving_holes_steps_each_as_alone(tgh):
     """Moving harmonic holes (translate(boost(Harmonic, -v), x0)): time-dependent
-    frames and tables, the moving core and layer, in one launch per kernel."""
+    frames and tables, the moving core and layer from each problem's layer box,
+    in one launch per kernel; equal to each hole stepped alone to an ulp of
+    the state (measured: two and six entries of a near-zero Pi component one
+    ulp apart, outside the layer)."""
     probs = []
     for a, roots, v in ((0.0, 3, 0.3), (0.3, 4, -0.2)):
         case = tgh.harmonic_kerr_case(M=1.0, a=a, v=v, x0=0.25, axis=1, halfwidth=roots / 2, r_0=0.35, r_1=0.9)
         probs.append(tgh.GHProblem(tgh.gh_forest(None, case, N=16, roots=roots), case, q=4))
-    check_batch_equals_alone(tgh, probs, 2, True)
+    check_batch_equals_alone(tgh, probs, 2, True, roundoff=True)
 
 
 def _forest_problem(tgh, a, halfwidth, roots, radii):

@@ -1848,7 +1848,7 @@ int tgh_copy_work(tgh_problem* p, double* dst) {
 struct tgh_batch {
   std::vector<tgh_problem*> probs;
   int q = 4, gauge = 0;
-  bool hole = false;
+  int hole = 0;  // the head's interior kind: 0, 1 (the sphere) or HOLE_BOXED
   // The block layout (a problem on a forest with levels or with the general
   // interior among them): one fused launch over every block of every problem
   // per stage; the RK4 epilogue writes the states in place, so the tables and
@@ -1908,6 +1908,11 @@ int batch_tables(tgh_batch* b, double t, double dt) {
       e.a.cacc = c_acc[s];
       e.a.cnext = c_next[s];
       e.a.stage = s + 1;
+      if (b->hole == HOLE_BOXED) {
+        e.a.lbox = p->Blayer;
+        e.a.lB3 = (long)p->layer_B * p->layer_B * p->layer_B;
+        e.a.lB = p->layer_B;
+      }
       e.k = k;
       e.W = (decltype(e.W))e.a.W;
       e.Hsrc = (decltype(e.Hsrc))e.a.Hsrc;
@@ -2006,8 +2011,14 @@ void batch_step_launches(tgh_batch* b, double t, double dt) {
   const double t_off[4] = {0.0, dt / 2, dt / 2, dt};
   const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
   for (int s = 0; s < 4; ++s) {
-    for (tgh_problem* p : b->probs)
+    for (tgh_problem* p : b->probs) {
       if (!p->frame_static) launch_frame(p, p->BW[s % 2], t_stage[s], t_off[s]);
+      if (b->hole == HOLE_BOXED) {
+        const long B3 = (long)p->layer_B * p->layer_B * p->layer_B;
+        LayerArgs la{p->brick, p->cs, t_stage[s], p->clock_use, t_off[s], p->Blayer, p->layer_B};
+        hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256)), dim3(256), 0, p->stream, la);
+      }
+    }
     const BatchEntry* tab = b->d_tab + s * n;
     switch (b->q) {
       case 2: launch_batch<2>(b->stream, tab, n, b->head_total, b->pi_total, b->gauge, b->hole); break;
@@ -2067,7 +2078,7 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
   b->probs.assign(probs, probs + n);
   b->q = p0->q;
   b->gauge = p0->cs.gauge;
-  b->hole = p0->cs.interior != INT_NONE;
+  b->hole = p0->cs.interior != INT_NONE ? 1 : 0;
   b->blocks = blocks;
   b->hole_kind = hole_kind(p0);
   auto bail = [&](int rc) {
@@ -2085,6 +2096,13 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
     }
     rc = sync(p);
     if (rc) return bail(rc);
+  }
+  // moving :damped holes with their layer boxes: the head reads the boxes, as
+  // each problem's own steps do, so that every problem computes its own bits
+  if (!blocks && b->hole) {
+    bool boxed = true;
+    for (tgh_problem* p : b->probs) boxed = boxed && p->Blayer;
+    if (boxed) b->hole = HOLE_BOXED;
   }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(TGH_EDEVICE, "creating the batch's stream"));

@@ -16,6 +16,11 @@ enum { INT_NONE = 0, INT_DAMPED = 1, INT_FROZEN = 2, INT_PASTED = 3, INT_FITTED 
 // Gauge-source kinds as the kernels are specialised: the case's kind, and for
 // a moving hole its own (its kernels carry the moving core and layer).
 enum { GAUGE_NONE = 0, GAUGE_SAMPLED = 1, GAUGE_KS = 2, GAUGE_KS_MOVING = 3, GAUGE_NONE_MOVING = 4 };
+// The kernels' interior kinds (template HOLE): 0 none, 1 the sphere, 2 the
+// general interior (tracked geometry, :fitted), 3 the sphere of a moving
+// :damped hole on the brick, whose core and layer targets come from the
+// per-stage box of k_brick_layer instead of the dual passes in the head.
+enum { HOLE_BOXED = 3 };
 constexpr bool gauge_has_source(int gauge) { return gauge == GAUGE_SAMPLED || gauge == GAUGE_KS || gauge == GAUGE_KS_MOVING; }
 enum { EPI_STORE = 0, EPI_RK4 = 1 };
 
@@ -392,18 +397,32 @@ TGH_HD void core_relaxation_rate(const double h[NC], const double ht[NC], double
 // 526-583): the ADM relaxation of h and Pi's linear one toward the target at
 // the core rule's point, at core_rate = max(rho_max, 12 |v| / r_0), plus the
 // target's rates. hu and Pu are the point's state.
-TGH_HD void moving_core_rates(const CaseP& c, double t, const double x[3], const double hu[NC],
-                              const double Pu[NC], double dh[NC], double dP[NC]) {
+// Its two halves: the target (the core rule's point's exact state hd, Pd and
+// the target's rates dthd, dtPd, which need the dual passes) and the
+// relaxation of the point's own state toward it.
+TGH_HD void moving_core_target(const CaseP& c, double t, const double x[3], double hd[NC], double Pd[NC],
+                               double dthd[NC], double dtPd[NC]) {
 #pragma clang fp contract(off)
-  const double s = sqrt(c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2]);
-  const double rc = fmax(c.rho_max, 12.0 * s / c.r0);
-  double xc[3], hd[NC], Pd[NC], dx[NC], dthd[NC], dtPd[NC];
+  double xc[3], dx[NC];
   core_point(c, t, x, xc);
   bg_state_exact(c.bg, t, xc, hd, Pd, dx);
   core_target_rate(c, t, x, dthd, dtPd);
+}
+TGH_HD void moving_core_relax(const CaseP& c, const double hu[NC], const double Pu[NC], const double hd[NC],
+                              const double Pd[NC], const double dthd[NC], const double dtPd[NC], double dh[NC],
+                              double dP[NC]) {
+#pragma clang fp contract(off)
+  const double s = sqrt(c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2]);
+  const double rc = fmax(c.rho_max, 12.0 * s / c.r0);
   core_relaxation_rate(hu, hd, rc, dthd, dh);
 #pragma unroll
   for (int v = 0; v < NC; ++v) dP[v] = dtPd[v] - rc * (Pu[v] - Pd[v]);
+}
+TGH_HD void moving_core_rates(const CaseP& c, double t, const double x[3], const double hu[NC],
+                              const double Pu[NC], double dh[NC], double dP[NC]) {
+  double hd[NC], Pd[NC], dthd[NC], dtPd[NC];
+  moving_core_target(c, t, x, hd, Pd, dthd, dtPd);
+  moving_core_relax(c, hu, Pu, hd, Pd, dthd, dtPd, dh, dP);
 }
 
 // Where a point of block b's work array takes its value from (TreeAMR 0.3's
@@ -853,7 +872,7 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
   {
     double ue[20], re[20];
     if (layer) {
-      if (follows && a.lbox) {
+      if constexpr (HOLE == HOLE_BOXED) {  // a moving :damped hole's box (k_brick_layer)
         const double* lb = a.lbox + pr.lslot;
 #pragma unroll
         for (int v = 0; v < 20; ++v) {
@@ -938,9 +957,9 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
 
 // The frozen core's rates (gh_rhs_point!, src/evolution.jl:500-583): zero for
 // a hole at rest; a moving :damped core relaxes toward its target.
-template <int GAUGE, class SRC>
-__device__ __forceinline__ void core_rates(const RhsArgs& a, const double x[3], const SRC& src, double dh[NC],
-                                           double dP[NC]) {
+template <int GAUGE, int HOLE, class SRC>
+__device__ __forceinline__ void core_rates(const RhsArgs& a, const double x[3], long lslot, const SRC& src,
+                                           double dh[NC], double dP[NC]) {
   constexpr bool MOVE = (bg_set_for(GAUGE) & BGS_MOVING) != 0;
   if constexpr (MOVE) {
     if (a.c.moving && a.c.interior == INT_DAMPED) {
@@ -950,7 +969,20 @@ __device__ __forceinline__ void core_rates(const RhsArgs& a, const double x[3], 
         hu[v] = src.X(v)[0];
         Pu[v] = src.P(v);
       }
-      moving_core_rates(a.c, a.t, x, hu, Pu, dh, dP);
+      if constexpr (HOLE == HOLE_BOXED) {  // the target from the box (k_brick_layer)
+        const double* lb = a.lbox + lslot;
+        double hd[NC], Pd[NC], dthd[NC], dtPd[NC];
+#pragma unroll
+        for (int v = 0; v < NC; ++v) {
+          hd[v] = lb[v * a.lB3];
+          Pd[v] = lb[(NC + v) * a.lB3];
+          dthd[v] = lb[(2 * NC + v) * a.lB3];
+          dtPd[v] = lb[(3 * NC + v) * a.lB3];
+        }
+        moving_core_relax(a.c, hu, Pu, hd, Pd, dthd, dtPd, dh, dP);
+      } else {
+        moving_core_rates(a.c, a.t, x, hu, Pu, dh, dP);
+      }
       return;
     }
   }
@@ -1100,7 +1132,7 @@ __device__ __forceinline__ bool head_point(const RhsArgs& a, const PointRef& pr,
   double r;
   if (frozen_core<HOLE>(a.c, a.t, pr.x, r)) {
     double dh[NC];
-    core_rates<GAUGE>(a, pr.x, src, dh, pc.c1);
+    core_rates<GAUGE, HOLE>(a, pr.x, pr.lslot, src, dh, pc.c1);
 #pragma unroll 1
     for (int v = 0; v < NC; ++v) emit<EPI>(a, TGH_OUT_ARGS, pr.si0 + v * pr.ssv, pr.wi0 + v * pr.sv, dh[v]);
     pc.w = -1.0;
@@ -1233,6 +1265,7 @@ __device__ __forceinline__ bool block_point_at(const RhsArgs& a, const double* _
   pr.m_S = a.m_S ? a.m_S + (long)b * 3 * psz + p : nullptr;
   pr.m_Sij = a.m_Sij ? a.m_Sij + (long)b * 6 * psz + p : nullptr;
   pr.ms = psz;
+  pr.lslot = 0;
   pidx = (long)b * psz + p;
   return true;
 }
@@ -2320,7 +2353,7 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
   double r;
   if (frozen_core<HOLE>(a.c, a.t, pr.x, r)) {
     double dh[NC], dP[NC];
-    core_rates<GAUGE>(a, pr.x, src, dh, dP);
+    core_rates<GAUGE, HOLE>(a, pr.x, pr.lslot, src, dh, dP);
 #pragma unroll 1
     for (int v = 0; v < NC; ++v) {
       emit<EPI_RK4>(a, TGH_OUT_ARGS, off + v * k.fs, off + v * k.fs, dh[v]);
@@ -2622,12 +2655,13 @@ struct FrameArgs {
   double toff;
   int shell;  // 1: one thread per frame point (brick_shell_point); 0: per extent point
 };
-// A moving :damped hole's layer on the brick: per stage, one thread per point
-// of the B^3 box that holds the layer (its corner one point below the layer's
-// bounding box about c(t)), the layer points' exact state and rates
-// (background_state_rate) into slot layer_slot(x, y, z, B) of 40 fields, A
-// wave of the head then reads them instead of running the dual passes on
-// every lane where one of its points lies in the layer.
+// A moving :damped hole's core and layer on the brick: per stage, one thread
+// per point of the B^3 box that holds them (its corner one point below the
+// layer's bounding box about c(t)), into slot layer_slot(x, y, z, B) of 40
+// fields: a layer point's exact state and rates (background_state_rate), a
+// core point's target and the target's rates (moving_core_target). The head
+// then reads them instead of running the dual passes on every lane of a wave
+// where one of its points lies in the core or the layer.
 struct LayerArgs {
   Brick k;
   CaseP c;
@@ -2655,9 +2689,15 @@ __global__ __launch_bounds__(256) void k_brick_layer(LayerArgs a) {
   }
   const double x[3] = {k.lo[0] + (double)p[0] * k.h, k.lo[1] + (double)p[1] * k.h, k.lo[2] + (double)p[2] * k.h};
   double r;
-  if (frozen_core<1>(a.c, a.t, x, r) || !(r < a.c.r1)) return;  // the head reads layer points only
   double ue[20], re[20];
-  background_state_rate(a.c.bg, a.t, x, a.c.delta, ue, ue + NC, re, re + NC);
+  if (frozen_core<1>(a.c, a.t, x, r)) {
+    // a core point: its target and the target's rates (moving_core_target)
+    moving_core_target(a.c, a.t, x, ue, ue + NC, re, re + NC);
+  } else if (r < a.c.r1) {
+    background_state_rate(a.c.bg, a.t, x, a.c.delta, ue, ue + NC, re, re + NC);
+  } else {
+    return;  // the head reads core and layer points only
+  }
   double* dst = a.box + layer_slot(p[0], p[1], p[2], B);
 #pragma unroll
   for (int v = 0; v < 20; ++v) {

@@ -135,6 +135,13 @@ void launch_brick_rhs_q(tgh_problem* p, const RhsArgs& a) {
 template <int Q>
 void launch_brick_rhs_g(tgh_problem* p, const RhsArgs& a) {
   const bool hole = p->cs.interior != INT_NONE;
+  if (a.lbox) {  // a moving :damped hole with its layer box
+    if (p->cs.gauge == GAUGE_KS_MOVING)
+      launch_brick_rhs_q<Q, GAUGE_KS_MOVING, HOLE_BOXED>(p, a);
+    else
+      launch_brick_rhs_q<Q, GAUGE_NONE_MOVING, HOLE_BOXED>(p, a);
+    return;
+  }
   switch (p->cs.gauge) {
     case GAUGE_KS:
       if (hole) launch_brick_rhs_q<Q, GAUGE_KS, true>(p, a);
@@ -172,7 +179,14 @@ void launch_batch_q(hipStream_t st, const BatchEntry* tab, int n, int head_total
   hipLaunchKernelGGL((k_brick_pi_batch<Q>), xcd_grid(pi_total), dim3(T::X * T::Y * T::Z), 0, st, tab, n, pi_total);
 }
 template <int Q>
-void launch_batch(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total, int gauge, bool hole) {
+void launch_batch(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total, int gauge, int hole) {
+  if (hole == HOLE_BOXED) {  // moving :damped holes with their layer boxes
+    if (gauge == GAUGE_KS_MOVING)
+      launch_batch_q<Q, GAUGE_KS_MOVING, HOLE_BOXED>(st, tab, n, head_total, pi_total);
+    else
+      launch_batch_q<Q, GAUGE_NONE_MOVING, HOLE_BOXED>(st, tab, n, head_total, pi_total);
+    return;
+  }
   switch (gauge) {
     case GAUGE_KS:
       if (hole) launch_batch_q<Q, GAUGE_KS, true>(st, tab, n, head_total, pi_total);
@@ -222,4 +236,4 @@ void launch_batch_blocks(hipStream_t st, const BatchEntry* tab, int n, int total
   template void tgh::launch_rhs_blocks<Q>(tgh_problem*, const tgh::RhsArgs&, int);          \
   template void tgh::launch_rhs_brick<Q>(tgh_problem*, const tgh::RhsArgs&);                \
   template void tgh::launch_batch_blocks<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int); \
-  template void tgh::launch_batch<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int, bool);
+  template void tgh::launch_batch<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int, int);

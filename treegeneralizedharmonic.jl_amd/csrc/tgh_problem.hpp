@@ -125,7 +125,7 @@ extern template void launch_rhs_brick<8>(tgh_problem*, const RhsArgs&);
 // One RK4 stage of a batch of problems on the brick layout (the head and the
 // Pi launch over the table's ranges), at order Q.
 template <int Q>
-void launch_batch(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total, int gauge, bool hole);
+void launch_batch(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total, int gauge, int hole);
 // One RK4 stage of a batch on the block layout (the fused kernel over every
 // block of every problem), at order Q.
 template <int Q>
@@ -134,8 +134,8 @@ extern template void launch_batch_blocks<2>(hipStream_t, const BatchEntry*, int,
 extern template void launch_batch_blocks<4>(hipStream_t, const BatchEntry*, int, int, int, int);
 extern template void launch_batch_blocks<6>(hipStream_t, const BatchEntry*, int, int, int, int);
 extern template void launch_batch_blocks<8>(hipStream_t, const BatchEntry*, int, int, int, int);
-extern template void launch_batch<2>(hipStream_t, const BatchEntry*, int, int, int, int, bool);
-extern template void launch_batch<4>(hipStream_t, const BatchEntry*, int, int, int, int, bool);
-extern template void launch_batch<6>(hipStream_t, const BatchEntry*, int, int, int, int, bool);
-extern template void launch_batch<8>(hipStream_t, const BatchEntry*, int, int, int, int, bool);
+extern template void launch_batch<2>(hipStream_t, const BatchEntry*, int, int, int, int, int);
+extern template void launch_batch<4>(hipStream_t, const BatchEntry*, int, int, int, int, int);
+extern template void launch_batch<6>(hipStream_t, const BatchEntry*, int, int, int, int, int);
+extern template void launch_batch<8>(hipStream_t, const BatchEntry*, int, int, int, int, int);
 }  // namespace tgh
