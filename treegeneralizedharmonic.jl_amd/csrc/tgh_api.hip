@@ -481,7 +481,7 @@ int ensure_brick(tgh_problem* p) {
     // the layer box: the layer's bounding box about c(t) with a point of margin
     // each side, whatever t (slots wrap modulo B)
     p->layer_B = 2 * (int)std::ceil(p->cs.r1 * k.inv_h) + 4;
-    const size_t lb = sizeof(double) * 40 * (size_t)p->layer_B * p->layer_B * p->layer_B;
+    const size_t lb = sizeof(double) * LAYER_FIELDS * (size_t)p->layer_B * p->layer_B * p->layer_B;
     hipError_t e = hipMalloc(&p->Blayer, lb);
     if (e != hipSuccess)
       return fail(TGH_EDEVICE, std::string("allocating the layer box: ") + hipGetErrorString(e));
@@ -522,7 +522,7 @@ void rk4_brick_launches(tgh_problem* p, double t, double dt) {
     if (p->Blayer) {
       const long B3 = (long)p->layer_B * p->layer_B * p->layer_B;
       LayerArgs la{p->brick, p->cs, t_stage[s], p->clock_use, t_off[s], p->Blayer, p->layer_B};
-      hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256)), dim3(256), 0, p->stream, la);
+      hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256), 3), dim3(256), 0, p->stream, la);
       a.lbox = p->Blayer;
       a.lB3 = B3;
       a.lB = p->layer_B;
@@ -2016,7 +2016,7 @@ void batch_step_launches(tgh_batch* b, double t, double dt) {
       if (b->hole == HOLE_BOXED) {
         const long B3 = (long)p->layer_B * p->layer_B * p->layer_B;
         LayerArgs la{p->brick, p->cs, t_stage[s], p->clock_use, t_off[s], p->Blayer, p->layer_B};
-        hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256)), dim3(256), 0, p->stream, la);
+        hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256), 3), dim3(256), 0, p->stream, la);
       }
     }
     const BatchEntry* tab = b->d_tab + s * n;

@@ -674,7 +674,7 @@ struct RhsArgs {
   const double* clock;  // stage_time's clock (NULL: t as given) and offset
   double toff;
   // a moving hole's layer targets and rates on the brick, evaluated per stage by
-  // k_brick_layer (40 fields of lB^3 slots, slot = brick coordinates mod lB);
+  // k_brick_layer (LAYER_FIELDS fields of lB^3 slots, slot = brick coordinates mod lB);
   // NULL: the head evaluates them itself
   const double* lbox;
   long lB3;
@@ -872,12 +872,18 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
   {
     double ue[20], re[20];
     if (layer) {
-      if constexpr (HOLE == HOLE_BOXED) {  // a moving :damped hole's box (k_brick_layer)
+      if constexpr (HOLE == HOLE_BOXED) {
+        // a moving :damped hole's box (k_brick_layer): background_state_rate's
+        // values, its d_t Pi formed here from Pi at t +- delta as it forms it
+#pragma clang fp contract(off)
         const double* lb = a.lbox + pr.lslot;
+        const double tp = a.t + c.delta, tm = a.t - c.delta;
 #pragma unroll
-        for (int v = 0; v < 20; ++v) {
+        for (int v = 0; v < NC; ++v) {
           ue[v] = lb[v * a.lB3];
-          re[v] = lb[(20 + v) * a.lB3];
+          ue[NC + v] = lb[(NC + v) * a.lB3];
+          re[v] = lb[(2 * NC + v) * a.lB3];
+          re[NC + v] = (lb[(3 * NC + v) * a.lB3] - lb[(4 * NC + v) * a.lB3]) / (tp - tm);
         }
       } else if (follows) {
         background_state_rate(c.bg, a.t, pr.x, c.delta, ue, ue + NC, re, re + NC);
@@ -969,15 +975,22 @@ __device__ __forceinline__ void core_rates(const RhsArgs& a, const double x[3], 
         hu[v] = src.X(v)[0];
         Pu[v] = src.P(v);
       }
-      if constexpr (HOLE == HOLE_BOXED) {  // the target from the box (k_brick_layer)
+      if constexpr (HOLE == HOLE_BOXED) {
+        // the target from the box (k_brick_layer) and its rates formed here as
+        // core_target_rate forms them
+#pragma clang fp contract(off)
+        const CaseP& c = a.c;
         const double* lb = a.lbox + lslot;
         double hd[NC], Pd[NC], dthd[NC], dtPd[NC];
+        const double w = smoothstep(4.0 * hole_radius(c, a.t, x) / c.r0 - 1.0);
+        const double tp = a.t + c.delta, tm = a.t - c.delta;
+        const double f = w / (tp - tm);
 #pragma unroll
         for (int v = 0; v < NC; ++v) {
           hd[v] = lb[v * a.lB3];
           Pd[v] = lb[(NC + v) * a.lB3];
-          dthd[v] = lb[(2 * NC + v) * a.lB3];
-          dtPd[v] = lb[(3 * NC + v) * a.lB3];
+          dthd[v] = w == 0.0 ? 0.0 : f * (lb[(2 * NC + v) * a.lB3] - lb[(4 * NC + v) * a.lB3]);
+          dtPd[v] = w == 0.0 ? 0.0 : f * (lb[(3 * NC + v) * a.lB3] - lb[(5 * NC + v) * a.lB3]);
         }
         moving_core_relax(a.c, hu, Pu, hd, Pd, dthd, dtPd, dh, dP);
       } else {
@@ -2655,13 +2668,20 @@ struct FrameArgs {
   double toff;
   int shell;  // 1: one thread per frame point (brick_shell_point); 0: per extent point
 };
-// A moving :damped hole's core and layer on the brick: per stage, one thread
-// per point of the B^3 box that holds them (its corner one point below the
-// layer's bounding box about c(t)), into slot layer_slot(x, y, z, B) of 40
-// fields: a layer point's exact state and rates (background_state_rate), a
-// core point's target and the target's rates (moving_core_target). The head
-// then reads them instead of running the dual passes on every lane of a wave
-// where one of its points lies in the core or the layer.
+// A moving :damped hole's core and layer on the brick: per stage, the exact
+// background's dual passes (bg_state_exact) that the head's core and layer
+// targets are made of, for every core and layer point of the B^3 box about the
+// hole (its corner one point below the layer's bounding box about c(t)), into
+// slot layer_slot(x, y, z, B) of LAYER_FIELDS fields; grid.y is the pass, so the
+// three passes of a point run on three threads:
+//   a layer point (background_state_rate): pass 0 the state h, Pi and d_t h
+//     at t (fields 0-29), passes 1 and 2 Pi at t +- delta (30-39, 40-49);
+//   a core point (moving_core_target): pass 0 the state at the core rule's
+//     point at t (0-19), passes 1 and 2 the state at t +- delta at that
+//     time's core point (20-39, 40-59), only where core_target_rate's weight
+//     is not zero.
+// The head (HOLE_BOXED) combines them as those functions do.
+enum { LAYER_FIELDS = 60 };
 struct LayerArgs {
   Brick k;
   CaseP c;
@@ -2674,35 +2694,58 @@ struct LayerArgs {
 __global__ __launch_bounds__(256) void k_brick_layer(LayerArgs a) {
   a.t = stage_time(a.t, a.clock, a.toff);
   const Brick& k = a.k;
-  const int B = a.B;
+  const CaseP& c = a.c;
+  const int B = a.B, pass = blockIdx.y;
   const long B3 = (long)B * B * B;
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B3) return;
   const int l[3] = {(int)(s % B), (int)((s / B) % B), (int)(s / ((long)B * B))};
   double cc[3];
-  center_at(a.c, a.t, cc);
+  center_at(c, a.t, cc);
   int p[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    p[d] = (int)floor((cc[d] - a.c.r1 - k.lo[d]) * k.inv_h) - 1 + l[d];
+    p[d] = (int)floor((cc[d] - c.r1 - k.lo[d]) * k.inv_h) - 1 + l[d];
     if (p[d] < 0 || p[d] >= k.n[d]) return;
   }
   const double x[3] = {k.lo[0] + (double)p[0] * k.h, k.lo[1] + (double)p[1] * k.h, k.lo[2] + (double)p[2] * k.h};
   double r;
-  double ue[20], re[20];
-  if (frozen_core<1>(a.c, a.t, x, r)) {
-    // a core point: its target and the target's rates (moving_core_target)
-    moving_core_target(a.c, a.t, x, ue, ue + NC, re, re + NC);
-  } else if (r < a.c.r1) {
-    background_state_rate(a.c.bg, a.t, x, a.c.delta, ue, ue + NC, re, re + NC);
-  } else {
-    return;  // the head reads core and layer points only
-  }
+  const bool core = frozen_core<1>(c, a.t, x, r);
+  if (!core && !(r < c.r1)) return;  // the head reads core and layer points only
   double* dst = a.box + layer_slot(p[0], p[1], p[2], B);
+  double hh[NC], PP[NC], dd[NC];
+  if (core) {
+    double xq[3];
+    double tq = a.t;
+    if (pass > 0) {
+#pragma clang fp contract(off)
+      const double w = smoothstep(4.0 * hole_radius(c, a.t, x) / c.r0 - 1.0);
+      if (w == 0.0) return;  // no rate there
+      tq = pass == 1 ? a.t + c.delta : a.t - c.delta;
+    }
+    core_point(c, tq, x, xq);
+    bg_state_exact(c.bg, tq, xq, hh, PP, dd);
+    const int f0 = 20 * pass;
 #pragma unroll
-  for (int v = 0; v < 20; ++v) {
-    dst[v * B3] = ue[v];
-    dst[(20 + v) * B3] = re[v];
+    for (int v = 0; v < NC; ++v) {
+      dst[(f0 + v) * B3] = hh[v];
+      dst[(f0 + NC + v) * B3] = PP[v];
+    }
+    return;
+  }
+  const double tq = pass == 0 ? a.t : pass == 1 ? a.t + c.delta : a.t - c.delta;
+  bg_state_exact(c.bg, tq, x, hh, PP, dd);
+  if (pass == 0) {
+#pragma unroll
+    for (int v = 0; v < NC; ++v) {
+      dst[v * B3] = hh[v];
+      dst[(NC + v) * B3] = PP[v];
+      dst[(2 * NC + v) * B3] = dd[v];
+    }
+  } else {
+    const int f0 = pass == 1 ? 3 * NC : 4 * NC;
+#pragma unroll
+    for (int v = 0; v < NC; ++v) dst[(f0 + v) * B3] = PP[v];
   }
 }
 

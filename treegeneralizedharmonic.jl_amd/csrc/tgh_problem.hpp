@@ -68,7 +68,7 @@ struct tgh_problem {
   double* Bacc = nullptr;
   double* BH = nullptr;
   double* Bcoef = nullptr;            // the split form's coefficients (CF_N brick fields)
-  double* Blayer = nullptr;           // a moving :damped hole's layer box (40 x layer_B^3)
+  double* Blayer = nullptr;           // a moving :damped hole's layer box (LAYER_FIELDS x layer_B^3)
   int layer_B = 0;
   bool brick_ready = false;
   bool frame_static = false;          // the frame never changes: filled once
