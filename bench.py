@@ -398,6 +398,52 @@ def forest_bench(tgh, torch, steps):
             "regrid_ms": regrid_ms, "regrid_transfer_ms": transfer_ms, "regrid_leaves": regrid_leaves}
 
 
+def moving_forest_bench(tgh, torch, steps):
+    """The forward path of the regrid-cadence workload: the boosted Kerr-Schild
+    hole (KerrSchildSource) M=1 a=0.7 at v=0.3 along x on hole_forest(N=16,
+    roots=8 over [-16M,16M]^3, radii=(8M, 4M, 2M)): four levels, finest h=M/32,
+    prolongation p=6, the block layout; RK4 steps as a graph and as plain
+    launches, then the driver's cadence (chunks of 8 steps, each followed by
+    a regrid: marking, the new problem and its fill plan, the transfer)."""
+    L = tgh._load()
+    case = tgh.kerr_schild_case(M=1.0, a=0.7, v=0.3, halfwidth=16.0, r_0=0.71, r_1=1.21,
+                                gauge_source="algebraic")
+    forest = tgh.hole_forest(None, case, N=16, roots=8, radii=(8.0, 4.0, 2.0))
+    p = tgh.GHProblem(forest, case, q=4, ops=tgh.Operators(prolongation=6))
+    u = p.statevector()
+    tgh.initial_state(u, p, 0.0)
+    dt = tgh.gh_dt(p, u, cfl=0.25)
+    points = forest.nleaves() * 16 ** 3
+    ptr = C.c_void_p(u.data_ptr())
+    out = {"workload": "boosted Kerr-Schild hole M=1 a=0.7 v=0.3 (KerrSchildSource, moving damped core) on "
+                       "hole_forest(N=16, roots=8 over [-16M,16M]^3, radii=(8M,4M,2M)): four levels, finest "
+                       "h=M/32, p=6, block layout",
+           "leaves": forest.nleaves(), "points": points, "steps": steps}
+    for name, flags in (("graph", 1), ("plain_launches", 0)):
+        tgh._check(L.tgh_rk4(p.ptr, ptr, 0.0, dt, 1, flags))  # warm-up, capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tgh._check(L.tgh_rk4(p.ptr, ptr, dt, dt, steps, flags))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[name] = points * steps / el
+        out[name + "_ms_per_step"] = el / steps * 1e3
+    assert bool(torch.isfinite(u).all()), "the moving forest's state is not finite"
+    tau = tgh.refinement_indicator(p, u, dt, scale=tgh.field_scale(p, u, dt), eps=0.01)
+    thr = sorted(tau)[len(tau) * 15 // 16]
+    ref = tgh.Refinement(refine_tol=thr, coarsen_tol=min(t for t in tau if t > 0) / 2, maxlevel_cap=3,
+                         floor_margin=1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    q, uq, log = tgh.gh_solve_regrid(p, u, (0.0, 16 * dt), cfl=0.25, every=8, refinement=ref, graph=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    assert bool(torch.isfinite(uq).all()), "the regridded state is not finite"
+    out["cadence"] = {"steps": 16, "regrids": len(log), "wall_ms": el * 1e3,
+                      "leaves_after": q.nblocks, "ms_per_step_with_regrids": el / 16 * 1e3}
+    return out
+
+
 def octree_batch_bench(tgh, torch, steps):
     """The batching workload whole: the three-level octree around the a = 0.7
     hole (forest_bench's hierarchy) and the four holes of batch_bench in one
@@ -663,6 +709,10 @@ def main():
         note("the octree")
         torch.cuda.empty_cache()
         result["forest"] = forest_bench(tgh, torch, max(1, min(5, args.steps)))
+    if not args.no_forest:
+        note("the moving hole on its forest")
+        torch.cuda.empty_cache()
+        result["moving_forest"] = moving_forest_bench(tgh, torch, max(1, min(5, args.steps)))
     if not args.no_batch:
         note("the four-hole batch")
         torch.cuda.empty_cache()

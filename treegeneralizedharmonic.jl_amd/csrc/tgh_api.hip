@@ -440,6 +440,47 @@ bool static_background(const CaseP& c) {
   return !c.moving && (kind == BG_MINKOWSKI || kind == BG_KERR_SCHILD || kind == BG_SHIFTED || kind == BG_HARMONIC);
 }
 
+// A moving :damped hole's layer box (k_brick_layer), on the finest level's grid:
+// the layer's bounding box about c(t) with a point of margin each side,
+// whatever t (slots wrap modulo B). Nothing for other cases.
+int ensure_layer_box(tgh_problem* p) {
+  if (p->Blayer || !(p->cs.moving && p->cs.interior == INT_DAMPED && !p->cs.general)) return TGH_OK;
+  int lmax = 0;
+  for (int l : p->level_host) lmax = std::max(lmax, l);
+  p->layer_lmax = lmax;
+  p->layer_B = 2 * (int)std::ceil(p->cs.r1 * p->geom.inv_h * (double)(1 << lmax)) + 4;
+  const size_t lb = sizeof(double) * LAYER_FIELDS * (size_t)p->layer_B * p->layer_B * p->layer_B;
+  hipError_t e = hipMalloc(&p->Blayer, lb);
+  if (e != hipSuccess) {
+    p->Blayer = nullptr;
+    return fail(TGH_EDEVICE, std::string("allocating the layer box: ") + hipGetErrorString(e));
+  }
+  return TGH_OK;
+}
+// The box kernel's view of the finest level's grid (its corner, spacing and
+// closed point counts; the other fields are unused there).
+Brick layer_grid(const tgh_problem* p) {
+  Brick k;
+  std::memset(&k, 0, sizeof(k));
+  const double f = (double)(1 << p->layer_lmax);
+  for (int d = 0; d < 3; ++d) {
+    k.lo[d] = p->geom.lo[d];
+    k.n[d] = (p->geom.nb[d] << p->layer_lmax) * p->geom.N + 1;
+  }
+  k.h = p->geom.h / f;
+  k.inv_h = p->geom.inv_h * f;
+  return k;
+}
+void launch_layer_box(tgh_problem* p, double t, double toff, RhsArgs& a) {
+  const long B3 = (long)p->layer_B * p->layer_B * p->layer_B;
+  LayerArgs la{layer_grid(p), p->cs, t, p->clock_use, toff, p->Blayer, p->layer_B};
+  hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256), 3), dim3(256), 0, p->stream, la);
+  a.lbox = p->Blayer;
+  a.lB3 = B3;
+  a.lB = p->layer_B;
+  a.lLmax = p->layer_lmax;
+}
+
 int ensure_brick(tgh_problem* p) {
   if (p->brick_ready) return TGH_OK;
   const Geom& g = p->geom;
@@ -477,14 +518,9 @@ int ensure_brick(tgh_problem* p) {
     launch_frame(p, p->BW[0], 0.0);
     launch_frame(p, p->BW[1], 0.0);
   }
-  if (p->cs.moving && p->cs.interior == INT_DAMPED && !p->cs.general) {
-    // the layer box: the layer's bounding box about c(t) with a point of margin
-    // each side, whatever t (slots wrap modulo B)
-    p->layer_B = 2 * (int)std::ceil(p->cs.r1 * k.inv_h) + 4;
-    const size_t lb = sizeof(double) * LAYER_FIELDS * (size_t)p->layer_B * p->layer_B * p->layer_B;
-    hipError_t e = hipMalloc(&p->Blayer, lb);
-    if (e != hipSuccess)
-      return fail(TGH_EDEVICE, std::string("allocating the layer box: ") + hipGetErrorString(e));
+  {
+    const int rc = ensure_layer_box(p);
+    if (rc) return rc;
   }
   if (p->cs.gauge == GAUGE_SAMPLED) {
     HIP_TRY(hipMalloc(&p->BH, bytes));
@@ -519,14 +555,7 @@ void rk4_brick_launches(tgh_problem* p, double t, double dt) {
     RhsArgs a = rhs_args(p, Wc, t_stage[s], nullptr);
     a.clock = p->clock_use;
     a.toff = t_off[s];
-    if (p->Blayer) {
-      const long B3 = (long)p->layer_B * p->layer_B * p->layer_B;
-      LayerArgs la{p->brick, p->cs, t_stage[s], p->clock_use, t_off[s], p->Blayer, p->layer_B};
-      hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256), 3), dim3(256), 0, p->stream, la);
-      a.lbox = p->Blayer;
-      a.lB3 = B3;
-      a.lB = p->layer_B;
-    }
+    if (p->Blayer) launch_layer_box(p, t_stage[s], t_off[s], a);
     a.Hsrc = p->BH;
     a.us = p->Bus;
     a.acc = p->Bacc;
@@ -627,6 +656,7 @@ void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
     RhsArgs a = rhs_args(p, Wc, t_stage[s], p->mat.rho ? &p->mat : nullptr);
     a.clock = p->clock_use;
     a.toff = t_off[s];
+    if (p->Blayer) launch_layer_box(p, t_stage[s], t_off[s], a);
     a.us = u;
     a.acc = p->acc;
     a.Wn = Wn;
@@ -1483,6 +1513,8 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     BrickCopyArgs ca{p->geom, p->brick, u, p->BW[0], p->Bus, 0};
     hipLaunchKernelGGL(k_brick_in, grid_brick(p), dim3(256), 0, p->stream, ca);
   } else {
+    int rc = ensure_layer_box(p);
+    if (rc) return rc;
     // the first stage input: u_n scattered into W[0]
     launch_scatter(p, u, p->W[0], 1);
   }
@@ -2014,9 +2046,8 @@ void batch_step_launches(tgh_batch* b, double t, double dt) {
     for (tgh_problem* p : b->probs) {
       if (!p->frame_static) launch_frame(p, p->BW[s % 2], t_stage[s], t_off[s]);
       if (b->hole == HOLE_BOXED) {
-        const long B3 = (long)p->layer_B * p->layer_B * p->layer_B;
-        LayerArgs la{p->brick, p->cs, t_stage[s], p->clock_use, t_off[s], p->Blayer, p->layer_B};
-        hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256), 3), dim3(256), 0, p->stream, la);
+        RhsArgs unused = rhs_args(p, p->BW[s % 2], t_stage[s], nullptr);
+        launch_layer_box(p, t_stage[s], t_off[s], unused);
       }
     }
     const BatchEntry* tab = b->d_tab + s * n;

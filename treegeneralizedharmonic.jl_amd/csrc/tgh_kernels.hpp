@@ -679,6 +679,8 @@ struct RhsArgs {
   const double* lbox;
   long lB3;
   int lB;
+  int lLmax;  // the box's grid: the finest level's (a block point of level L is
+              // at index (B N + i) 2^(lLmax - L) on it)
 };
 // The layer box's slot of brick point (x, y, z).
 TGH_HD long layer_slot(int x, int y, int z, int B) {
@@ -1279,6 +1281,11 @@ __device__ __forceinline__ bool block_point_at(const RhsArgs& a, const double* _
   pr.m_Sij = a.m_Sij ? a.m_Sij + (long)b * 6 * psz + p : nullptr;
   pr.ms = psz;
   pr.lslot = 0;
+  if (a.lbox) {
+    const int* Bc = g.bcoord + 3 * b;
+    const int sh = a.lLmax - block_level(g, b);
+    pr.lslot = layer_slot((Bc[0] * g.N + i) << sh, (Bc[1] * g.N + j) << sh, (Bc[2] * g.N + k) << sh, a.lB);
+  }
   pidx = (long)b * psz + p;
   return true;
 }

@@ -77,6 +77,15 @@ void launch_rhs_g(tgh_problem* p, const RhsArgs& a) {
     }
     return;
   }
+  if constexpr (EPI == EPI_RK4) {
+    if (a.lbox) {  // a moving :damped hole with its layer box
+      if (p->cs.gauge == GAUGE_KS_MOVING)
+        launch_rhs_q<Q, GAUGE_KS_MOVING, HOLE_BOXED, EPI>(p, a);
+      else
+        launch_rhs_q<Q, GAUGE_NONE_MOVING, HOLE_BOXED, EPI>(p, a);
+      return;
+    }
+  }
   switch (p->cs.gauge) {
     case GAUGE_KS:
       if (hole) launch_rhs_q<Q, GAUGE_KS, true, EPI>(p, a);
