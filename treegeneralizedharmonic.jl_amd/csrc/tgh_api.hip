@@ -440,11 +440,18 @@ bool static_background(const CaseP& c) {
   return !c.moving && (kind == BG_MINKOWSKI || kind == BG_KERR_SCHILD || kind == BG_SHIFTED || kind == BG_HARMONIC);
 }
 
-// A moving :damped hole's layer box (k_brick_layer), on the finest level's grid:
-// the layer's bounding box about c(t) with a point of margin each side,
-// whatever t (slots wrap modulo B). Nothing for other cases.
+// The layer box (k_brick_layer) of a moving :damped hole or of a harmonic hole
+// at rest, on the finest level's grid: the layer's bounding box about c(t) with
+// a point of margin each side, whatever t (slots wrap modulo B). Nothing for
+// other cases.
+Brick layer_grid(const tgh_problem* p);
 int ensure_layer_box(tgh_problem* p) {
-  if (p->Blayer || !(p->cs.moving && p->cs.interior == INT_DAMPED && !p->cs.general)) return TGH_OK;
+  if (p->Blayer || p->cs.general || p->cs.interior == INT_NONE) return TGH_OK;
+  const bool moving = p->cs.moving && p->cs.interior == INT_DAMPED;
+  // a harmonic hole at rest (no gauge source): its layer's target is a dual
+  // pass of the harmonic chart, the same at every t, so the box is filled once
+  const bool harmonic = !p->cs.moving && p->cs.gauge == GAUGE_NONE;
+  if (!moving && !harmonic) return TGH_OK;
   int lmax = 0;
   for (int l : p->level_host) lmax = std::max(lmax, l);
   p->layer_lmax = lmax;
@@ -454,6 +461,13 @@ int ensure_layer_box(tgh_problem* p) {
   if (e != hipSuccess) {
     p->Blayer = nullptr;
     return fail(TGH_EDEVICE, std::string("allocating the layer box: ") + hipGetErrorString(e));
+  }
+  p->layer_static = harmonic;
+  if (harmonic) {
+    const long B3 = (long)p->layer_B * p->layer_B * p->layer_B;
+    LayerArgs la{layer_grid(p), p->cs, 0.0, nullptr, 0.0, p->Blayer, p->layer_B};
+    hipLaunchKernelGGL(k_brick_layer<false>, dim3((unsigned)((B3 + 255) / 256), 1), dim3(256), 0, p->stream, la);
+    HIP_TRY(hipGetLastError());
   }
   return TGH_OK;
 }
@@ -473,8 +487,10 @@ Brick layer_grid(const tgh_problem* p) {
 }
 void launch_layer_box(tgh_problem* p, double t, double toff, RhsArgs& a) {
   const long B3 = (long)p->layer_B * p->layer_B * p->layer_B;
-  LayerArgs la{layer_grid(p), p->cs, t, p->clock_use, toff, p->Blayer, p->layer_B};
-  hipLaunchKernelGGL(k_brick_layer, dim3((unsigned)((B3 + 255) / 256), 3), dim3(256), 0, p->stream, la);
+  if (!p->layer_static) {
+    LayerArgs la{layer_grid(p), p->cs, t, p->clock_use, toff, p->Blayer, p->layer_B};
+    hipLaunchKernelGGL(k_brick_layer<true>, dim3((unsigned)((B3 + 255) / 256), 3), dim3(256), 0, p->stream, la);
+  }
   a.lbox = p->Blayer;
   a.lB3 = B3;
   a.lB = p->layer_B;

@@ -18,8 +18,9 @@ enum { INT_NONE = 0, INT_DAMPED = 1, INT_FROZEN = 2, INT_PASTED = 3, INT_FITTED 
 enum { GAUGE_NONE = 0, GAUGE_SAMPLED = 1, GAUGE_KS = 2, GAUGE_KS_MOVING = 3, GAUGE_NONE_MOVING = 4 };
 // The kernels' interior kinds (template HOLE): 0 none, 1 the sphere, 2 the
 // general interior (tracked geometry, :fitted), 3 the sphere of a moving
-// :damped hole on the brick, whose core and layer targets come from the
-// per-stage box of k_brick_layer instead of the dual passes in the head.
+// :damped hole or of a harmonic hole at rest, whose core and layer targets
+// come from the box of k_brick_layer (per stage, or once at rest) instead of
+// the dual passes in the head.
 enum { HOLE_BOXED = 3 };
 constexpr bool gauge_has_source(int gauge) { return gauge == GAUGE_SAMPLED || gauge == GAUGE_KS || gauge == GAUGE_KS_MOVING; }
 enum { EPI_STORE = 0, EPI_RK4 = 1 };
@@ -874,7 +875,7 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
   {
     double ue[20], re[20];
     if (layer) {
-      if constexpr (HOLE == HOLE_BOXED) {
+      if constexpr (HOLE == HOLE_BOXED && MOVE) {
         // a moving :damped hole's box (k_brick_layer): background_state_rate's
         // values, its d_t Pi formed here from Pi at t +- delta as it forms it
 #pragma clang fp contract(off)
@@ -887,6 +888,11 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
           re[v] = lb[(2 * NC + v) * a.lB3];
           re[NC + v] = (lb[(3 * NC + v) * a.lB3] - lb[(4 * NC + v) * a.lB3]) / (tp - tm);
         }
+      } else if constexpr (HOLE == HOLE_BOXED) {
+        // a harmonic hole at rest: its target from the box, filled once
+        const double* lb = a.lbox + pr.lslot;
+#pragma unroll
+        for (int v = 0; v < 20; ++v) ue[v] = lb[v * a.lB3];
       } else if (follows) {
         background_state_rate(c.bg, a.t, pr.x, c.delta, ue, ue + NC, re, re + NC);
       } else {
@@ -948,10 +954,15 @@ __device__ __forceinline__ void head_source(const RhsArgs& a, const PointRef& pr
   }
   double ue[20], re[20];
   if (layer) {
-    if (follows)
+    if constexpr (HOLE == HOLE_BOXED) {
+      const double* lb = a.lbox + pr.lslot;
+#pragma unroll
+      for (int v = 0; v < 20; ++v) ue[v] = lb[v * a.lB3];
+    } else if (follows) {
       background_state_rate(c.bg, a.t, pr.x, c.delta, ue, ue + NC, re, re + NC);
-    else
+    } else {
       case_state<bg_set_for(GAUGE)>(c, a.t, pr.x, ue);
+    }
   }
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
@@ -2698,6 +2709,7 @@ struct LayerArgs {
   double* __restrict__ box;
   int B;
 };
+template <bool MOVING>
 __global__ __launch_bounds__(256) void k_brick_layer(LayerArgs a) {
   a.t = stage_time(a.t, a.clock, a.toff);
   const Brick& k = a.k;
@@ -2720,6 +2732,15 @@ __global__ __launch_bounds__(256) void k_brick_layer(LayerArgs a) {
   const bool core = frozen_core<1>(c, a.t, x, r);
   if (!core && !(r < c.r1)) return;  // the head reads core and layer points only
   double* dst = a.box + layer_slot(p[0], p[1], p[2], B);
+  if constexpr (!MOVING) {
+    // a harmonic hole at rest: the layer's target, once (the core's rates are zero)
+    if (core) return;
+    double ue[20];
+    case_state<bg_set_for(GAUGE_NONE)>(c, a.t, x, ue);
+#pragma unroll
+    for (int v = 0; v < 20; ++v) dst[v * B3] = ue[v];
+    return;
+  }
   double hh[NC], PP[NC], dd[NC];
   if (core) {
     double xq[3];

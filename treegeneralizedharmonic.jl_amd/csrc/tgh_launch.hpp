@@ -78,11 +78,13 @@ void launch_rhs_g(tgh_problem* p, const RhsArgs& a) {
     return;
   }
   if constexpr (EPI == EPI_RK4) {
-    if (a.lbox) {  // a moving :damped hole with its layer box
+    if (a.lbox) {  // a moving :damped hole or a harmonic hole at rest, with its layer box
       if (p->cs.gauge == GAUGE_KS_MOVING)
         launch_rhs_q<Q, GAUGE_KS_MOVING, HOLE_BOXED, EPI>(p, a);
-      else
+      else if (p->cs.gauge == GAUGE_NONE_MOVING)
         launch_rhs_q<Q, GAUGE_NONE_MOVING, HOLE_BOXED, EPI>(p, a);
+      else
+        launch_rhs_q<Q, GAUGE_NONE, HOLE_BOXED, EPI>(p, a);
       return;
     }
   }
@@ -144,11 +146,13 @@ void launch_brick_rhs_q(tgh_problem* p, const RhsArgs& a) {
 template <int Q>
 void launch_brick_rhs_g(tgh_problem* p, const RhsArgs& a) {
   const bool hole = p->cs.interior != INT_NONE;
-  if (a.lbox) {  // a moving :damped hole with its layer box
+  if (a.lbox) {  // a moving :damped hole or a harmonic hole at rest, with its layer box
     if (p->cs.gauge == GAUGE_KS_MOVING)
       launch_brick_rhs_q<Q, GAUGE_KS_MOVING, HOLE_BOXED>(p, a);
-    else
+    else if (p->cs.gauge == GAUGE_NONE_MOVING)
       launch_brick_rhs_q<Q, GAUGE_NONE_MOVING, HOLE_BOXED>(p, a);
+    else
+      launch_brick_rhs_q<Q, GAUGE_NONE, HOLE_BOXED>(p, a);
     return;
   }
   switch (p->cs.gauge) {
@@ -189,11 +193,13 @@ void launch_batch_q(hipStream_t st, const BatchEntry* tab, int n, int head_total
 }
 template <int Q>
 void launch_batch(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total, int gauge, int hole) {
-  if (hole == HOLE_BOXED) {  // moving :damped holes with their layer boxes
+  if (hole == HOLE_BOXED) {  // holes with their layer boxes
     if (gauge == GAUGE_KS_MOVING)
       launch_batch_q<Q, GAUGE_KS_MOVING, HOLE_BOXED>(st, tab, n, head_total, pi_total);
-    else
+    else if (gauge == GAUGE_NONE_MOVING)
       launch_batch_q<Q, GAUGE_NONE_MOVING, HOLE_BOXED>(st, tab, n, head_total, pi_total);
+    else
+      launch_batch_q<Q, GAUGE_NONE, HOLE_BOXED>(st, tab, n, head_total, pi_total);
     return;
   }
   switch (gauge) {

@@ -71,6 +71,7 @@ struct tgh_problem {
   double* Blayer = nullptr;           // a moving :damped hole's layer box (LAYER_FIELDS x layer_B^3)
   int layer_B = 0;
   int layer_lmax = 0;                 // the finest level, whose grid the box is on
+  bool layer_static = false;          // filled once (a hole at rest)
   bool brick_ready = false;
   bool frame_static = false;          // the frame never changes: filled once
   hipStream_t cap_stream = nullptr;   // capture stream when the problem runs on the null stream
