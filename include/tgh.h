@@ -79,8 +79,11 @@ enum { /* tgh_rk4 flags */
                          layout (every point once, a frame of ghosts around
                          the whole brick) */
   TGH_RK4_SPLIT = 8,  /* the right-hand side as two launches: the head (with
-                         d_t h) and the ten Pi components; without FUSED or
-                         SPLIT the layout's faster form is used */
+                         d_t h) and the ten Pi components (on the block layout
+                         each block run as a brick, LDS-tiled, where the case
+                         has no bounds, :pasted core, matter, sampled source
+                         or general interior); without FUSED or SPLIT the
+                         layout's faster form is used */
   TGH_RK4_PER_BLOCK = 16 /* the plain per-block form: the per-block work arrays
                          and the fused kernel launched once per block and
                          stage (implies BLOCKS and FUSED; the baseline the

@@ -150,7 +150,10 @@ def test_forest_rk4_graph_replays_match_launches(tgh):
         assert torch.equal(a, b), f"graph replay {rep} differs"
 
 
-def test_hole_forest_rk4_matches_oracle(tgh):
+@pytest.mark.parametrize("flags", [0, 8], ids=["fused", "bricks"])
+def test_hole_forest_rk4_matches_oracle(tgh, flags):
+    """flags 8 (TGH_RK4_SPLIT): each block run as a brick by the brick's head
+    and LDS-tiled Pi kernel."""
     case, f, p = hole_forest_case(tgh)
     torch, dev, orc = make_pair(tgh, case, f, p)
     u = orc.fill_exact(0.0)
@@ -159,24 +162,24 @@ def test_hole_forest_rk4_matches_oracle(tgh):
     dt = 0.25 * f.minimum_spacing() / lam
     for n in range(2):
         orc.rk4_step(u, n * dt, dt, NTHREADS)
-    tgh._check(tgh._load().tgh_rk4(dev.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, 0))
+    tgh._check(tgh._load().tgh_rk4(dev.ptr, tgh.C.c_void_p(ud.data_ptr()), 0.0, dt, 2, flags))
     g = ud.cpu().numpy()
     diff = float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max())
-    print(f"KS hole on a two-level hole_forest, 2 RK4 steps: max rel diff = {diff:.3e}")
+    print(f"KS hole on a two-level hole_forest, 2 RK4 steps (flags {flags}): max rel diff = {diff:.3e}")
     assert diff <= 1e-12
     l2g, lig = tgh.error_norms(dev, ud, 2 * dt)
     l2o, lio = orc.error_norms(u, 2 * dt)
     assert abs(l2g - l2o) <= 1e-9 * l2o and abs(lig - lio) <= 1e-9 * lio
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_moving_hole_forest_rk4_matches_oracle(tgh, graph):
+@pytest.mark.parametrize("flags", [0, 1, 8 | 1], ids=["launches", "graph", "bricks-graph"])
+def test_moving_hole_forest_rk4_matches_oracle(tgh, flags):
     """The boosted hole (v = 0.3 along x, the moving :damped core) on a two-
     level hole_forest, the block layout: the core's and the layer's targets
     come from the layer box on the finest level's grid (points of both
     levels read it), the Dirichlet hook and the coarse-fine faces move with
-    t; two RK4 steps from t = 0.1 against the oracle, as launches and as a
-    replayed graph."""
+    t; two RK4 steps from t = 0.1 against the oracle, as launches, as a
+    replayed graph and with each block run as a brick (TGH_RK4_SPLIT)."""
     case = tgh.kerr_schild_case(M=1.0, a=0.5, v=0.3, axis=1, halfwidth=4.0, r_0=0.7, r_1=1.7,
                                 gauge_source="algebraic")
     f = tgh.hole_forest(None, case, N=8, roots=4, radii=(2.0,))
@@ -188,10 +191,10 @@ def test_moving_hole_forest_rk4_matches_oracle(tgh, graph):
     dt = 0.25 * f.minimum_spacing() / orc.max_speed(u, t0)
     for n in range(2):
         orc.rk4_step(u, t0 + n * dt, dt, NTHREADS)
-    tgh._check(tgh._load().tgh_rk4(dev.ptr, tgh.C.c_void_p(ud.data_ptr()), t0, dt, 2, 1 if graph else 0))
+    tgh._check(tgh._load().tgh_rk4(dev.ptr, tgh.C.c_void_p(ud.data_ptr()), t0, dt, 2, flags))
     g = ud.cpu().numpy()
     diff = float((np.abs(g - u) / np.maximum(np.abs(u), 1.0)).max())
-    print(f"moving hole on a two-level hole_forest, 2 RK4 steps (graph {graph}): max rel diff = {diff:.3e}")
+    print(f"moving hole on a two-level hole_forest, 2 RK4 steps (flags {flags}): max rel diff = {diff:.3e}")
     assert diff <= 1e-12
 
 

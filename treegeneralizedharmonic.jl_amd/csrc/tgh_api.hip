@@ -655,6 +655,38 @@ void launch_paste(tgh_problem* p, int mode, double* A, double* B, double t, doub
   hipLaunchKernelGGL(p->cs.moving ? k_paste<true> : k_paste<false>, grid, dim3(256), 0, p->stream, a);
 }
 
+// One RK4 step on the block layout run as bricks (form 5): u_n in Wus, the
+// accumulator in Wacc, the coefficients in Wcoef (all in the work layout), the
+// stage input in W[0] holding u_n with its ghosts unfilled.
+void rk4_block_lds_launches(tgh_problem* p, double t, double dt) {
+  const double c_acc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+  const double c_next[4] = {dt / 2, dt / 2, dt, 0.0};
+  const double t_off[4] = {0.0, dt / 2, dt / 2, dt};
+  const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
+  for (int s = 0; s < 4; ++s) {
+    double* Wc = p->W[s % 2];
+    double* Wn = p->W[(s + 1) % 2];
+    const bool prof = p->profiling;
+    int e0 = prof ? mark(p) : -1;
+    launch_fill(p, Wc, t_stage[s], t_off[s]);
+    if (prof) p->ev_fill.push_back({e0, mark(p)});
+    RhsArgs a = rhs_args(p, Wc, t_stage[s], nullptr);
+    a.clock = p->clock_use;
+    a.toff = t_off[s];
+    if (p->Blayer) launch_layer_box(p, t_stage[s], t_off[s], a);
+    a.cacc = c_acc[s];
+    a.cnext = c_next[s];
+    a.stage = s + 1;
+    const BlockBricks bb{Wc, p->Wcoef, p->Wus, p->Wacc, Wn, p->layer_lmax};
+    switch (p->q) {
+      case 2: launch_rhs_blocks_lds<2>(p, a, bb); break;
+      case 4: launch_rhs_blocks_lds<4>(p, a, bb); break;
+      case 6: launch_rhs_blocks_lds<6>(p, a, bb); break;
+      default: launch_rhs_blocks_lds<8>(p, a, bb); break;
+    }
+  }
+}
+
 // One RK4 step on device arrays: u (state, in place), stage work arrays
 // W[0] (holding u_n with ghosts unfilled) and W[1].
 void rk4_step_launches(tgh_problem* p, double* u, double t, double dt) {
@@ -1112,6 +1144,9 @@ int tgh_problem_destroy(tgh_problem* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
   (void)hipFree(p->d_clock);
+  (void)hipFree(p->Wus);
+  (void)hipFree(p->Wacc);
+  (void)hipFree(p->Wcoef);
   (void)hipFree(p->Blayer);
   (void)hipFree(p->d_bcoord);
   (void)hipFree(p->d_lookup);
@@ -1514,9 +1549,16 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     return fail(TGH_EINVAL, "TGH_RK4_FUSED and TGH_RK4_SPLIT exclude each other");
   // forms: 0 block layout, head + Pi; 1 block layout, fused; 2 brick layout,
   // fused; 3 brick layout, head + LDS-tiled Pi; 4 block layout, fused, one
-  // launch per block. Without FUSED or SPLIT the layout's faster form on MI355X.
-  const bool split = (flags & TGH_RK4_SPLIT) != 0 || (!(flags & TGH_RK4_FUSED) && !blocks && kBrickSplitDefault);
-  const int form = (flags & TGH_RK4_PER_BLOCK) ? 4 : blocks ? (split ? 0 : 1) : (split ? 3 : 2);
+  // launch per block; 5 block layout, each block run as a brick by the
+  // brick's head and LDS-tiled Pi kernel (not with bounds, :pasted, matter,
+  // the sampled source or the general interior: form 0 there). Without FUSED
+  // or SPLIT the layout's faster form on MI355X (measured: the brick's split
+  // form; on the block layout form 5 where it runs, else the fused kernel).
+  const bool lds_ok = !p->has_bounds && p->cs.interior != INT_PASTED && !p->mat.rho &&
+                      p->cs.gauge != GAUGE_SAMPLED && !p->cs.general;
+  const bool split = (flags & TGH_RK4_SPLIT) != 0 ||
+                     (!(flags & TGH_RK4_FUSED) && (blocks ? lds_ok : kBrickSplitDefault));
+  const int form = (flags & TGH_RK4_PER_BLOCK) ? 4 : blocks ? (split ? (lds_ok ? 5 : 0) : 1) : (split ? 3 : 2);
   const bool in_brick = form == 2 || form == 3;
   if (in_brick) {
     int rc = ensure_brick(p);
@@ -1533,6 +1575,16 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
     if (rc) return rc;
     // the first stage input: u_n scattered into W[0]
     launch_scatter(p, u, p->W[0], 1);
+    if (form == 5) {
+      const size_t wbytes = sizeof(double) * (size_t)p->work_len;
+      if (!p->Wus) {
+        HIP_TRY(hipMalloc(&p->Wus, wbytes));
+        HIP_TRY(hipMalloc(&p->Wacc, wbytes));
+        HIP_TRY(hipMalloc(&p->Wcoef, wbytes / 20 * CF_N));
+        HIP_TRY(hipMemsetAsync(p->Wcoef, 0, wbytes / 20 * CF_N, p->stream));
+      }
+      launch_scatter(p, u, p->Wus, 1);  // and u_n, in the work layout too
+    }
   }
   if (form != p->graph_form && p->graph_exec) {  // the captured graph holds another form
     HIP_TRY(hipGraphExecDestroy(p->graph_exec));
@@ -1543,6 +1595,8 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
   auto step = [&](double t) {
     if (in_brick)
       rk4_brick_launches(p, t, dt);
+    else if (form == 5)
+      rk4_block_lds_launches(p, t, dt);
     else
       rk4_step_launches(p, u, t, dt);
   };
@@ -1603,6 +1657,7 @@ int tgh_rk4(tgh_problem* p, double* u, double t0, double dt, int32_t nsteps, int
       HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
     }
   }
+  if (rc == TGH_OK && form == 5) launch_scatter(p, u, p->Wus, 0);  // u_{n+1} back into the state
   if (rc == TGH_OK && in_brick) {
     // u_{n+1} back into the state: the copies, then the primaries
     for (int pass = 0; pass < 2; ++pass) {

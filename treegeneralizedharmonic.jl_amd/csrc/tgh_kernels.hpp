@@ -2257,6 +2257,12 @@ struct Brick {
   int G, XO;     // ghost width; x offset of x = 0 in a row (XO >= G)
   long py, pz, fs;
   double lo[3], h, inv_h;
+  // point (x, y, z) is at lo + (x + gx0) h: 0 for the brick; a block's first
+  // index (B N) when a block's work array is run as a brick, so that positions
+  // round as the block layout's do; lsh: its layer-box slot's shift to the
+  // finest level's grid
+  int gx0[3];
+  int lsh;
 };
 TGH_HD long brick_index(const Brick& k, int x, int y, int z) {
   return (long)(x + k.XO) + k.py * (long)(y + k.G) + k.pz * (long)(z + k.G);
@@ -2326,9 +2332,9 @@ __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const dou
   pr.sv = k.fs;
   pr.s2 = (int)k.py;
   pr.s3 = (int)k.pz;
-  pr.x[0] = k.lo[0] + (double)x * k.h;
-  pr.x[1] = k.lo[1] + (double)y * k.h;
-  pr.x[2] = k.lo[2] + (double)z * k.h;
+  pr.x[0] = k.lo[0] + (double)(x + k.gx0[0]) * k.h;
+  pr.x[1] = k.lo[1] + (double)(y + k.gx0[1]) * k.h;
+  pr.x[2] = k.lo[2] + (double)(z + k.gx0[2]) * k.h;
   pr.si0 = off;
   pr.ssv = k.fs;
   pr.wi0 = off;
@@ -2338,7 +2344,8 @@ __global__ __launch_bounds__(256) void k_brick_rhs(RhsArgs a, Brick k, const dou
   pr.m_S = nullptr;
   pr.m_Sij = nullptr;
   pr.ms = 0;
-  pr.lslot = a.lbox ? layer_slot(x, y, z, a.lB) : 0;
+  pr.lslot = a.lbox ? layer_slot((x + k.gx0[0]) << k.lsh, (y + k.gx0[1]) << k.lsh, (z + k.gx0[2]) << k.lsh, a.lB)
+                    : 0;
   rhs_point<Q, GAUGE, HOLE, EPI_RK4>(a, pr, k.inv_h, TGH_OUT_ARGS);
 }
 
@@ -2367,9 +2374,9 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
   pr.sv = k.fs;
   pr.s2 = (int)k.py;
   pr.s3 = (int)k.pz;
-  pr.x[0] = k.lo[0] + (double)x * k.h;
-  pr.x[1] = k.lo[1] + (double)y * k.h;
-  pr.x[2] = k.lo[2] + (double)z * k.h;
+  pr.x[0] = k.lo[0] + (double)(x + k.gx0[0]) * k.h;
+  pr.x[1] = k.lo[1] + (double)(y + k.gx0[1]) * k.h;
+  pr.x[2] = k.lo[2] + (double)(z + k.gx0[2]) * k.h;
   pr.si0 = off;
   pr.ssv = k.fs;
   pr.wi0 = off;
@@ -2379,7 +2386,8 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
   pr.m_S = nullptr;
   pr.m_Sij = nullptr;
   pr.ms = 0;
-  pr.lslot = a.lbox ? layer_slot(x, y, z, a.lB) : 0;
+  pr.lslot = a.lbox ? layer_slot((x + k.gx0[0]) << k.lsh, (y + k.gx0[1]) << k.lsh, (z + k.gx0[2]) << k.lsh, a.lB)
+                    : 0;
   const GlobalSrc src{pr.w0, pr.sv, pr.s2, pr.s3};
   double r;
   if (frozen_core<HOLE>(a.c, a.t, pr.x, r)) {
@@ -2437,11 +2445,15 @@ __device__ __forceinline__ void brick_head_at(const RhsArgs& a, const Brick& k, 
 // same arithmetic in the same order: only the staging differs, so the
 // results are the same bits.
 constexpr int PI_TILES = 6;
+// The tile of a block run as a brick (17 x 5 x 3 threads, one point each): a
+// block's 17 points along x in one row, 4 x 6 tiles a block, 80 % of the
+// threads busy, where the brick's 8 x 8 x 4 tile would keep 43 %.
+constexpr int PI_TILE_BLOCK = 6;
 template <int Q, int TC = 0> struct PiTile {
-  static constexpr int X = TC == 0 || TC >= 4 ? TGH_PI_TILE_X : TC == 1 ? 8 : TC == 2 ? 10 : 16;
-  static constexpr int Y = TC == 0 || TC >= 4 ? TGH_PI_TILE_Y : TC == 1 ? 12 : TC == 2 ? 10 : 8;
-  static constexpr int Z = Q == 8 ? 2 : TC == 0 || TC >= 4 ? TGH_PI_TILE_Z : TC == 3 ? 2 : 4;
-  static constexpr int PZ = Q == 8 || !(TC == 0 || TC >= 4) ? 1 : TGH_PI_POINTS_Z;
+  static constexpr int X = TC == PI_TILE_BLOCK ? 17 : TC == 0 || TC >= 4 ? TGH_PI_TILE_X : TC == 1 ? 8 : TC == 2 ? 10 : 16;
+  static constexpr int Y = TC == PI_TILE_BLOCK ? 5 : TC == 0 || TC >= 4 ? TGH_PI_TILE_Y : TC == 1 ? 12 : TC == 2 ? 10 : 8;
+  static constexpr int Z = TC == PI_TILE_BLOCK ? 3 : Q == 8 ? 2 : TC == 0 || TC >= 4 ? TGH_PI_TILE_Z : TC == 3 ? 2 : 4;
+  static constexpr int PZ = TC == PI_TILE_BLOCK || Q == 8 || !(TC == 0 || TC >= 4) ? 1 : TGH_PI_POINTS_Z;
   static constexpr int pitch(int bx) {
     return TC == 4 ? (bx <= 25 ? 25 : bx) : TC == 5 ? (bx <= 26 ? 26 : bx) : (bx <= 24 ? 24 : bx);
   }
@@ -2638,6 +2650,74 @@ __global__ __launch_bounds__(PiTile<Q>::X * PiTile<Q>::Y * PiTile<Q>::Z) void k_
   tile_at(e.k, PiTile<Q>::X, PiTile<Q>::Y, PiTile<Q>::Z * PiTile<Q>::PZ, L - e.pi0, X0, Y0, Z0);
   brick_pi_tile<Q>(e.a, e.k, (const double*)e.W, (const double*)e.coef, nullptr, (double*)e.us, (double*)e.acc,
                    (double*)e.Wn, X0, Y0, Z0);
+}
+
+// The block layout's split form with the brick's code: block b's work array
+// is a brick of P^3 points (strides S, S^2, S^3 per variable, G ghosts) at
+// index offset B N, run by the brick head (one point a thread) and the
+// LDS-tiled Pi kernel (PI_TILE_BLOCK tiles). During a step u_n, the
+// accumulator and the Pi coefficients sit in the work layout beside W, so
+// that one index addresses a point in all of them, as on the brick.
+struct BlockBricks {
+  const double* __restrict__ W;
+  double* __restrict__ coef;  // CF_N fields of S^3 points a block
+  double* __restrict__ us;
+  double* __restrict__ acc;
+  double* __restrict__ Wn;
+  int lLmax;  // the finest level (the layer box's grid)
+};
+__device__ __forceinline__ Brick block_brick(const Geom& g, int b, int lLmax) {
+  Brick k;
+  const int* B = g.bcoord + 3 * b;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    k.n[d] = g.P;
+    k.tot[d] = g.N;
+    k.periodic[d] = 0;
+    k.refl[d][0] = k.refl[d][1] = 0;
+    k.lo[d] = g.lo[d];
+    k.gx0[d] = B[d] * g.N;
+  }
+  k.rot = 0;
+  k.G = g.G;
+  k.XO = g.G;
+  k.py = g.S;
+  k.pz = (long)g.S * g.S;
+  k.fs = (long)g.S * g.S * g.S;
+  k.h = block_h(g, b);
+  k.inv_h = block_inv_h(g, b);
+  k.lsh = lLmax - block_level(g, b);
+  return k;
+}
+template <int Q, int GAUGE, int HOLE>
+__global__ __launch_bounds__(256) void k_block_head(RhsArgs a, BlockBricks bb) {
+  a.t = stage_time(a.t, a.clock, a.toff);
+  const Geom& g = a.g;
+  const int P = g.P, P3 = g.P * g.P * g.P;
+  int b, wx;
+  if (!wg_map((P3 + 255) / 256, g.nblocks, b, wx)) return;
+  const int p = wx * 256 + (int)threadIdx.x;
+  if (p >= P3) return;
+  const Brick k = block_brick(g, b, bb.lLmax);
+  const long wb = (long)b * 20 * k.fs;
+  brick_head_at<Q, GAUGE, HOLE>(a, k, bb.W + wb, nullptr, bb.coef + (long)b * CF_N * k.fs, nullptr, bb.us + wb,
+                                bb.acc + wb, bb.Wn + wb, p % P, (p / P) % P, p / (P * P));
+}
+template <int Q>
+__global__ __launch_bounds__((PiTile<Q, PI_TILE_BLOCK>::X * PiTile<Q, PI_TILE_BLOCK>::Y *
+                              PiTile<Q, PI_TILE_BLOCK>::Z)) void k_block_pi(RhsArgs a, BlockBricks bb) {
+  a.t = stage_time(a.t, a.clock, a.toff);
+  using T = PiTile<Q, PI_TILE_BLOCK>;
+  const Geom& g = a.g;
+  const int P = g.P;
+  const int tx = (P + T::X - 1) / T::X, ty = (P + T::Y - 1) / T::Y, tz = (P + T::Z - 1) / T::Z;
+  int b, wx;
+  if (!wg_map(tx * ty * tz, g.nblocks, b, wx)) return;  // uniform over the workgroup
+  const int X0 = (wx % tx) * T::X, Y0 = ((wx / tx) % ty) * T::Y, Z0 = (wx / (tx * ty)) * T::Z;
+  const Brick k = block_brick(g, b, bb.lLmax);
+  const long wb = (long)b * 20 * k.fs;
+  brick_pi_tile<Q, PI_TILE_BLOCK>(a, k, bb.W + wb, bb.coef + (long)b * CF_N * k.fs, nullptr, bb.us + wb,
+                                  bb.acc + wb, bb.Wn + wb, X0, Y0, Z0);
 }
 
 // One RK4 stage of a batch on the block layout: the fused kernel over every

@@ -182,6 +182,56 @@ void launch_rhs_brick(tgh_problem* p, const RhsArgs& a) {
   launch_brick_rhs_g<Q>(p, a);
 }
 
+// The block layout's split form run with the brick's code (k_block_head, then
+// k_block_pi over every block's tiles); timed while profiling as the brick's is.
+template <int Q, int GAUGE, int HOLE>
+void launch_block_lds_q(tgh_problem* p, const RhsArgs& a, const BlockBricks& bb) {
+  using T = PiTile<Q, PI_TILE_BLOCK>;
+  const int P = a.g.P;
+  const long P3 = (long)P * P * P;
+  const long tiles = (long)((P + T::X - 1) / T::X) * ((P + T::Y - 1) / T::Y) * ((P + T::Z - 1) / T::Z);
+  const bool prof = p->profiling;
+  int e0 = prof ? mark(p) : -1;
+  hipLaunchKernelGGL((k_block_head<Q, GAUGE, HOLE>), xcd_grid((P3 + 255) / 256 * a.g.nblocks), dim3(256), 0,
+                     p->stream, a, bb);
+  int e1 = prof ? mark(p) : -1;
+  hipLaunchKernelGGL((k_block_pi<Q>), xcd_grid(tiles * a.g.nblocks), dim3(T::X * T::Y * T::Z), 0, p->stream, a,
+                     bb);
+  if (prof) {
+    p->ev_rhs.push_back({e0, e1});
+    p->ev_pi.push_back({e1, mark(p)});
+  }
+}
+template <int Q>
+void launch_rhs_blocks_lds(tgh_problem* p, const RhsArgs& a, const BlockBricks& bb) {
+  const bool hole = p->cs.interior != INT_NONE;
+  if (a.lbox) {  // a moving :damped hole or a harmonic hole at rest, with its layer box
+    if (p->cs.gauge == GAUGE_KS_MOVING)
+      launch_block_lds_q<Q, GAUGE_KS_MOVING, HOLE_BOXED>(p, a, bb);
+    else if (p->cs.gauge == GAUGE_NONE_MOVING)
+      launch_block_lds_q<Q, GAUGE_NONE_MOVING, HOLE_BOXED>(p, a, bb);
+    else
+      launch_block_lds_q<Q, GAUGE_NONE, HOLE_BOXED>(p, a, bb);
+    return;
+  }
+  switch (p->cs.gauge) {
+    case GAUGE_KS:
+      if (hole) launch_block_lds_q<Q, GAUGE_KS, true>(p, a, bb);
+      else launch_block_lds_q<Q, GAUGE_KS, false>(p, a, bb);
+      break;
+    case GAUGE_KS_MOVING:
+      launch_block_lds_q<Q, GAUGE_KS_MOVING, true>(p, a, bb);
+      break;
+    case GAUGE_NONE_MOVING:
+      launch_block_lds_q<Q, GAUGE_NONE_MOVING, true>(p, a, bb);
+      break;
+    default:  // GAUGE_NONE (the sampled source is read in the state layout: not this form)
+      if (hole) launch_block_lds_q<Q, GAUGE_NONE, true>(p, a, bb);
+      else launch_block_lds_q<Q, GAUGE_NONE, false>(p, a, bb);
+      break;
+  }
+}
+
 // One RK4 stage of a batch of problems: the head and the Pi kernel, each one
 // launch over every problem's runs or tiles (the table's ranges).
 template <int Q, int GAUGE, int HOLE>
@@ -250,5 +300,6 @@ void launch_batch_blocks(hipStream_t st, const BatchEntry* tab, int n, int total
 #define TGH_INSTANTIATE_ORDER(Q)                                                            \
   template void tgh::launch_rhs_blocks<Q>(tgh_problem*, const tgh::RhsArgs&, int);          \
   template void tgh::launch_rhs_brick<Q>(tgh_problem*, const tgh::RhsArgs&);                \
+  template void tgh::launch_rhs_blocks_lds<Q>(tgh_problem*, const tgh::RhsArgs&, const tgh::BlockBricks&); \
   template void tgh::launch_batch_blocks<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int); \
   template void tgh::launch_batch<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int, int);

@@ -72,6 +72,11 @@ struct tgh_problem {
   int layer_B = 0;
   int layer_lmax = 0;                 // the finest level, whose grid the box is on
   bool layer_static = false;          // filled once (a hole at rest)
+  // the block layout's LDS form: u_n, the accumulator and the Pi coefficients
+  // in the work layout during a step (allocated at its first use)
+  double* Wus = nullptr;
+  double* Wacc = nullptr;
+  double* Wcoef = nullptr;
   bool brick_ready = false;
   bool frame_static = false;          // the frame never changes: filled once
   hipStream_t cap_stream = nullptr;   // capture stream when the problem runs on the null stream
@@ -116,6 +121,12 @@ int mark(tgh_problem* p);  // a profiling event on the problem's stream
 // tgh_launch.hpp, instantiated in tgh_q<Q>.hip.
 template <int Q> void launch_rhs_blocks(tgh_problem* p, const RhsArgs& a, int epi);
 template <int Q> void launch_rhs_brick(tgh_problem* p, const RhsArgs& a);
+// the block layout's split form with the brick's head and LDS-tiled Pi kernel
+template <int Q> void launch_rhs_blocks_lds(tgh_problem* p, const RhsArgs& a, const BlockBricks& bb);
+extern template void launch_rhs_blocks_lds<2>(tgh_problem*, const RhsArgs&, const BlockBricks&);
+extern template void launch_rhs_blocks_lds<4>(tgh_problem*, const RhsArgs&, const BlockBricks&);
+extern template void launch_rhs_blocks_lds<6>(tgh_problem*, const RhsArgs&, const BlockBricks&);
+extern template void launch_rhs_blocks_lds<8>(tgh_problem*, const RhsArgs&, const BlockBricks&);
 extern template void launch_rhs_blocks<2>(tgh_problem*, const RhsArgs&, int);
 extern template void launch_rhs_blocks<4>(tgh_problem*, const RhsArgs&, int);
 extern template void launch_rhs_blocks<6>(tgh_problem*, const RhsArgs&, int);
