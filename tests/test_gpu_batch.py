@@ -146,9 +146,10 @@ def _forest_problem(tgh, a, halfwidth, roots, radii):
 @pytest.mark.parametrize("graph", [False, True])
 def test_block_batch_of_forests_and_a_uniform_hole_steps_each_as_alone(tgh, graph):
     """Forests with levels among the problems: the batch runs on the block
-    layout, one fused launch over every block of every problem per stage (all
-    levels of all holes), and each problem's result is bit for bit its own
-    fused block-layout step."""
+    layout, its blocks run as bricks -- one head launch and one Pi launch over
+    every block of every problem per stage (all levels of all holes) -- and
+    each problem's result is bit for bit its own block-layout step in the same
+    form."""
     torch = torch_cuda()
     probs = [_forest_problem(tgh, 0.5, 2.0, 4, (1.2,)), _forest_problem(tgh, 0.7, 1.5, 3, (0.9,)),
              ks_problem(tgh, 0.3, 3)]
@@ -157,7 +158,7 @@ def test_block_batch_of_forests_and_a_uniform_hole_steps_each_as_alone(tgh, grap
     ref = [u.clone() for u in us]
     dt = min(tgh.gh_dt(p, u, cfl=0.25) for p, u in zip(probs, us))
     L = tgh._load()
-    flags = (4 | 2) | (1 if graph else 0)  # the block layout, fused, as the batch runs it
+    flags = (4 | 8) | (1 if graph else 0)  # the block layout, blocks run as bricks, as the batch runs it
     for p, r in zip(probs, ref):
         tgh._check(L.tgh_rk4(p.ptr, C.c_void_p(r.data_ptr()), 0.0, dt, 3, flags))
     batch = tgh.GHBatch(probs)

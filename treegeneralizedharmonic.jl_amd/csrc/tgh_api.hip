@@ -1957,6 +1957,7 @@ struct tgh_batch {
   // per stage; the RK4 epilogue writes the states in place, so the tables and
   // the graph hold the states' pointers.
   bool blocks = false;
+  bool lds = false;                  // the block layout with its blocks run as bricks
   int hole_kind = 0;                 // 0 none, 1 the sphere, 2 the general interior
   std::vector<double*> tab_u, graph_u;
   BatchEntry* d_tab = nullptr;  // 4 stage tables of n entries
@@ -2046,7 +2047,7 @@ int batch_tables_blocks(tgh_batch* b, double* const* u, double t, double dt) {
   const double c_acc[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
   const double c_next[4] = {dt / 2, dt / 2, dt, 0.0};
   const double t_stage[4] = {t, t + dt / 2, t + dt / 2, t + dt};
-  int w0 = 0;
+  int w0 = 0, pw0 = 0;
   for (int i = 0; i < n; ++i) {
     tgh_problem* p = b->probs[i];
     const int wgpb = (p->geom.P * p->geom.P * p->geom.P + 255) / 256;
@@ -2054,25 +2055,33 @@ int batch_tables_blocks(tgh_batch* b, double* const* u, double t, double dt) {
       BatchEntry& e = b->h_tab[s * n + i];
       std::memset(&e, 0, sizeof(e));
       e.a = rhs_args(p, p->W[s % 2], t_stage[s], nullptr);
-      e.a.us = u[i];
-      e.a.acc = p->acc;
+      e.a.us = b->lds ? p->Wus : u[i];
+      e.a.acc = b->lds ? p->Wacc : p->acc;
       e.a.Wn = p->W[(s + 1) % 2];
       e.a.du = nullptr;
       e.a.cacc = c_acc[s];
       e.a.cnext = c_next[s];
       e.a.stage = s + 1;
+      if (b->lds) e.a.coef = p->Wcoef;
       e.W = (decltype(e.W))e.a.W;
       e.Hsrc = (decltype(e.Hsrc))e.a.Hsrc;
       e.us = (decltype(e.us))e.a.us;
       e.acc = (decltype(e.acc))e.a.acc;
       e.Wn = (decltype(e.Wn))e.a.Wn;
+      e.coef = (decltype(e.coef))e.a.coef;
       e.head0 = w0;
-      e.pi0 = 0;
+      e.pi0 = pw0;
+      e.lLmax = p->layer_lmax;
     }
     w0 += wgpb * p->geom.nblocks;
+    if (b->lds) {
+      const int P = p->geom.P;
+      using T = PiTile<4, PI_TILE_BLOCK>;  // the block tile's shape is the same at every order
+      pw0 += ((P + T::X - 1) / T::X) * ((P + T::Y - 1) / T::Y) * ((P + T::Z - 1) / T::Z) * p->geom.nblocks;
+    }
   }
   b->head_total = w0;
-  b->pi_total = 0;
+  b->pi_total = pw0;
   HIP_TRY(hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(BatchEntry) * 4 * n, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->tab_key[0] = dt;
@@ -2088,11 +2097,21 @@ void batch_step_launches_blocks(tgh_batch* b, double t, double dt) {
   for (int s = 0; s < 4; ++s) {
     for (tgh_problem* p : b->probs) launch_fill(p, p->W[s % 2], t_stage[s], t_off[s]);
     const BatchEntry* tab = b->d_tab + s * n;
-    switch (b->q) {
-      case 2: launch_batch_blocks<2>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
-      case 4: launch_batch_blocks<4>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
-      case 6: launch_batch_blocks<6>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
-      default: launch_batch_blocks<8>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
+    if (b->lds) {
+      const int ht = b->head_total, pt = b->pi_total, hk = b->hole_kind ? 1 : 0;
+      switch (b->q) {
+        case 2: launch_batch_blocks_lds<2>(b->stream, tab, n, ht, pt, b->gauge, hk); break;
+        case 4: launch_batch_blocks_lds<4>(b->stream, tab, n, ht, pt, b->gauge, hk); break;
+        case 6: launch_batch_blocks_lds<6>(b->stream, tab, n, ht, pt, b->gauge, hk); break;
+        default: launch_batch_blocks_lds<8>(b->stream, tab, n, ht, pt, b->gauge, hk); break;
+      }
+    } else {
+      switch (b->q) {
+        case 2: launch_batch_blocks<2>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
+        case 4: launch_batch_blocks<4>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
+        case 6: launch_batch_blocks<6>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
+        default: launch_batch_blocks<8>(b->stream, tab, n, b->head_total, b->gauge, b->hole_kind); break;
+      }
     }
     for (int i = 0; i < n; ++i) {
       tgh_problem* p = b->probs[i];
@@ -2187,6 +2206,22 @@ int tgh_batch_create(tgh_problem* const* probs, int32_t n, tgh_batch** out) {
     tgh_batch_destroy(b);
     return rc;
   };
+  if (blocks && b->hole_kind != 2) {
+    // the blocks run as bricks where every problem allows it (as tgh_rk4's form 5)
+    bool ok = true;
+    for (tgh_problem* p : b->probs)
+      ok = ok && !p->has_bounds && p->cs.interior != INT_PASTED && !p->mat.rho && p->cs.gauge != GAUGE_SAMPLED &&
+           !p->cs.general;
+    b->lds = ok;
+    for (tgh_problem* p : b->probs) {
+      if (!b->lds || p->Wus) continue;
+      const size_t wbytes = sizeof(double) * (size_t)p->work_len;
+      if (hipMalloc(&p->Wus, wbytes) != hipSuccess || hipMalloc(&p->Wacc, wbytes) != hipSuccess ||
+          hipMalloc(&p->Wcoef, wbytes / 20 * CF_N) != hipSuccess ||
+          hipMemset(p->Wcoef, 0, wbytes / 20 * CF_N) != hipSuccess)
+        return bail(fail(TGH_EDEVICE, "allocating a batch's work-layout arrays"));
+    }
+  }
   for (tgh_problem* p : b->probs) {
     if (blocks) continue;
     int rc = ensure_brick(p);
@@ -2243,6 +2278,7 @@ int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t 
         break;
       }
       launch_scatter(p, u[i], p->W[0], 1);  // u_n into the first stage's work array
+      if (b->lds) launch_scatter(p, u[i], p->Wus, 1);  // and the work-layout u_n
       continue;
     }
     BrickCopyArgs ca{p->geom, p->brick, u[i], p->BW[0], p->Bus, 0};
@@ -2296,6 +2332,7 @@ int tgh_batch_rk4(tgh_batch* b, double* const* u, double t0, double dt, int32_t 
     }
   }
   if (rc == TGH_OK) {
+    for (int i = 0; i < n && b->lds; ++i) launch_scatter(b->probs[i], u[i], b->probs[i]->Wus, 0);
     for (int i = 0; i < n && !b->blocks; ++i) {  // (the block layout wrote u in place)
       tgh_problem* p = b->probs[i];
       for (int pass = 0; pass < 2; ++pass) {

@@ -2623,6 +2623,7 @@ struct BatchEntry {
   TGH_GLOBAL double* acc;
   TGH_GLOBAL double* Wn;  // the next stage's input
   int head0, pi0;  // first logical workgroup in the head and the Pi launch
+  int lLmax;       // the block layout run as bricks: the problem's finest level
 };
 __device__ __forceinline__ int batch_find(const BatchEntry* __restrict__ tab, int n, int L, bool pi) {
   int p = 0;
@@ -2718,6 +2719,45 @@ __global__ __launch_bounds__((PiTile<Q, PI_TILE_BLOCK>::X * PiTile<Q, PI_TILE_BL
   const long wb = (long)b * 20 * k.fs;
   brick_pi_tile<Q, PI_TILE_BLOCK>(a, k, bb.W + wb, bb.coef + (long)b * CF_N * k.fs, nullptr, bb.us + wb,
                                   bb.acc + wb, bb.Wn + wb, X0, Y0, Z0);
+}
+
+// A batch on the block layout with its blocks run as bricks: one head and one
+// Pi launch over every block of every problem, each entry's blocks a range of
+// logical workgroups from head0 (head) and pi0 (Pi tiles); W, Wn, us, acc and
+// coef the problem's work-layout arrays.
+template <int Q, int GAUGE, int HOLE>
+__global__ __launch_bounds__(256) void k_block_head_batch(const BatchEntry* __restrict__ tab, int n, int total) {
+  const int L = wg_logical(total);
+  if (L >= total) return;
+  const BatchEntry& e = tab[batch_find(tab, n, L, false)];
+  const Geom& g = e.a.g;
+  const int P = g.P, P3 = g.P * g.P * g.P, wgpb = (P3 + 255) / 256;
+  const int loc = L - e.head0, b = loc / wgpb;
+  const int p = (loc - b * wgpb) * 256 + (int)threadIdx.x;
+  if (p >= P3) return;
+  const Brick k = block_brick(g, b, e.lLmax);
+  const long wb = (long)b * 20 * k.fs;
+  brick_head_at<Q, GAUGE, HOLE>(e.a, k, (const double*)e.W + wb, nullptr, (double*)e.coef + (long)b * CF_N * k.fs,
+                                nullptr, (double*)e.us + wb, (double*)e.acc + wb, (double*)e.Wn + wb, p % P,
+                                (p / P) % P, p / (P * P));
+}
+template <int Q>
+__global__ __launch_bounds__((PiTile<Q, PI_TILE_BLOCK>::X * PiTile<Q, PI_TILE_BLOCK>::Y *
+                              PiTile<Q, PI_TILE_BLOCK>::Z)) void k_block_pi_batch(const BatchEntry* __restrict__ tab,
+                                                                                   int n, int total) {
+  using T = PiTile<Q, PI_TILE_BLOCK>;
+  const int L = wg_logical(total);
+  if (L >= total) return;  // uniform over the workgroup
+  const BatchEntry& e = tab[batch_find(tab, n, L, true)];
+  const Geom& g = e.a.g;
+  const int P = g.P;
+  const int tx = (P + T::X - 1) / T::X, ty = (P + T::Y - 1) / T::Y, tz = (P + T::Z - 1) / T::Z;
+  const int loc = L - e.pi0, b = loc / (tx * ty * tz), wx = loc - b * (tx * ty * tz);
+  const int X0 = (wx % tx) * T::X, Y0 = ((wx / tx) % ty) * T::Y, Z0 = (wx / (tx * ty)) * T::Z;
+  const Brick k = block_brick(g, b, e.lLmax);
+  const long wb = (long)b * 20 * k.fs;
+  brick_pi_tile<Q, PI_TILE_BLOCK>(e.a, k, (const double*)e.W + wb, (double*)e.coef + (long)b * CF_N * k.fs, nullptr,
+                                  (double*)e.us + wb, (double*)e.acc + wb, (double*)e.Wn + wb, X0, Y0, Z0);
 }
 
 // One RK4 stage of a batch on the block layout: the fused kernel over every

@@ -286,6 +286,25 @@ void launch_batch_blocks_g(hipStream_t st, const BatchEntry* tab, int n, int tot
   else
     hipLaunchKernelGGL((k_rhs_batch<Q, GAUGE, 0>), grid, dim3(256), 0, st, tab, n, total);
 }
+// The same with the blocks run as bricks (static gauge-source kinds; hole 0 or 1).
+template <int Q, int GAUGE, int HOLE>
+void launch_batch_blocks_lds_q(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total) {
+  using T = PiTile<Q, PI_TILE_BLOCK>;
+  hipLaunchKernelGGL((k_block_head_batch<Q, GAUGE, HOLE>), xcd_grid(head_total), dim3(256), 0, st, tab, n,
+                     head_total);
+  hipLaunchKernelGGL((k_block_pi_batch<Q>), xcd_grid(pi_total), dim3(T::X * T::Y * T::Z), 0, st, tab, n, pi_total);
+}
+template <int Q>
+void launch_batch_blocks_lds(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total, int gauge,
+                             int hole) {
+  if (gauge == GAUGE_KS) {
+    if (hole) launch_batch_blocks_lds_q<Q, GAUGE_KS, 1>(st, tab, n, head_total, pi_total);
+    else launch_batch_blocks_lds_q<Q, GAUGE_KS, 0>(st, tab, n, head_total, pi_total);
+  } else {
+    if (hole) launch_batch_blocks_lds_q<Q, GAUGE_NONE, 1>(st, tab, n, head_total, pi_total);
+    else launch_batch_blocks_lds_q<Q, GAUGE_NONE, 0>(st, tab, n, head_total, pi_total);
+  }
+}
 template <int Q>
 void launch_batch_blocks(hipStream_t st, const BatchEntry* tab, int n, int total, int gauge, int hole) {
   switch (gauge) {
@@ -302,4 +321,5 @@ void launch_batch_blocks(hipStream_t st, const BatchEntry* tab, int n, int total
   template void tgh::launch_rhs_brick<Q>(tgh_problem*, const tgh::RhsArgs&);                \
   template void tgh::launch_rhs_blocks_lds<Q>(tgh_problem*, const tgh::RhsArgs&, const tgh::BlockBricks&); \
   template void tgh::launch_batch_blocks<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int); \
+  template void tgh::launch_batch_blocks_lds<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int, int); \
   template void tgh::launch_batch<Q>(hipStream_t, const tgh::BatchEntry*, int, int, int, int, int);

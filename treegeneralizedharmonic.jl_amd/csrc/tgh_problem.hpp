@@ -147,6 +147,13 @@ extern template void launch_batch_blocks<2>(hipStream_t, const BatchEntry*, int,
 extern template void launch_batch_blocks<4>(hipStream_t, const BatchEntry*, int, int, int, int);
 extern template void launch_batch_blocks<6>(hipStream_t, const BatchEntry*, int, int, int, int);
 extern template void launch_batch_blocks<8>(hipStream_t, const BatchEntry*, int, int, int, int);
+template <int Q>
+void launch_batch_blocks_lds(hipStream_t st, const BatchEntry* tab, int n, int head_total, int pi_total, int gauge,
+                             int hole);
+extern template void launch_batch_blocks_lds<2>(hipStream_t, const BatchEntry*, int, int, int, int, int);
+extern template void launch_batch_blocks_lds<4>(hipStream_t, const BatchEntry*, int, int, int, int, int);
+extern template void launch_batch_blocks_lds<6>(hipStream_t, const BatchEntry*, int, int, int, int, int);
+extern template void launch_batch_blocks_lds<8>(hipStream_t, const BatchEntry*, int, int, int, int, int);
 extern template void launch_batch<2>(hipStream_t, const BatchEntry*, int, int, int, int, int);
 extern template void launch_batch<4>(hipStream_t, const BatchEntry*, int, int, int, int, int);
 extern template void launch_batch<6>(hipStream_t, const BatchEntry*, int, int, int, int, int);
