@@ -380,9 +380,11 @@ int tgh_set_tile(tgh_problem* p, int32_t tile);
  * by its own gh_solve (src/stepping.jl:208-239); a problem in a batch computes
  * the bits tgh_rk4 computes for it alone in the brick layout's split form.
  * With a forest with levels or a tracked or :fitted interior among them the
- * batch runs on the block layout instead (one fused launch over every block
- * of every problem per stage; each problem computes the bits of tgh_rk4 with
- * TGH_RK4_BLOCKS | TGH_RK4_FUSED), and moving holes are refused there
+ * batch runs on the block layout instead (one head and one Pi launch over
+ * every block of every problem per stage, each problem computing the bits of
+ * tgh_rk4 with TGH_RK4_BLOCKS | TGH_RK4_SPLIT; with a range projection or a
+ * general interior among them one fused launch, the bits of TGH_RK4_BLOCKS |
+ * TGH_RK4_FUSED), and moving holes are refused there
  * (TGH_EUNSUPPORTED). The problems share the order q, the gauge-source kind
  * and the kind of interior (one kernel specialisation), and none is :pasted;
  * each keeps its own arrays (brick arrays allocated here if missing). The
