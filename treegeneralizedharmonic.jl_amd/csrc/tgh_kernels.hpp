@@ -2113,11 +2113,14 @@ __global__ __launch_bounds__(256) void k_adm_constraint(GCArgs a) {
         }
         // d_m d_n g_ab
         auto DD = [&](int m, int n, int a2, int b2) { return dd[pidx(m, n)][pidx(a2, b2)]; };
+        // R_ab is symmetric: its ten components a <= b, mirrored
         double R[4][4];
         for (int a2 = 0; a2 < 4; ++a2)
-          for (int b2 = 0; b2 < 4; ++b2) {
+          for (int b2 = a2; b2 < 4; ++b2) {
             double A = 0.0, B = 0.0, Cc = 0.0, Dd = 0.0;
+#pragma unroll
             for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
               for (int d = 0; d < 4; ++d) {
                 const double gcd = cf.gu[pidx(cc, d)];
                 A += dgu[cc][pidx(cc, d)] * Gl[d][pidx(a2, b2)] +
@@ -2126,8 +2129,10 @@ __global__ __launch_bounds__(256) void k_adm_constraint(GCArgs a) {
                      gcd * (DD(b2, cc, d, a2) + DD(b2, a2, d, cc) - DD(b2, d, cc, a2)) / 2;
                 Dd += G4[cc][pidx(b2, d)] * G4[d][pidx(cc, a2)];
               }
+#pragma unroll
             for (int d = 0; d < 4; ++d) Cc += Gtr[d] * G4[d][pidx(a2, b2)];
             R[a2][b2] = A - B + Cc - Dd;
+            R[b2][a2] = R[a2][b2];
           }
         double Rs = 0.0;
         for (int a2 = 0; a2 < 4; ++a2)
