@@ -842,10 +842,15 @@ class Forest:
     def _unbalanced(self):
         leafset = self._leaves or set()
         bad = set()
+        seen = set()  # (level, block): siblings share most of their neighbours
         for lf in leafset:
             if lf[0] < 2:
                 continue
             for c in self._neighbours(lf):
+                key = (lf[0],) + c
+                if key in seen:
+                    continue
+                seen.add(key)
                 o = self._containing(leafset, lf[0], c)
                 if o is not None and o[0] < lf[0] - 1:
                     bad.add(o)
